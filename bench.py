@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""B2B + Jacobi-PCG solve time on one MI355X, against the CPU oracle.
+
+One step = one B2B assembly of the x and y systems plus their lock-step
+Jacobi-PCG solve until the true FP64 relative residual ||b - Ax|| / ||b|| is
+at most 1e-6 on both axes, on a seeded synthetic netlist resident in HBM:
+1,000,000 movable cells, 1,100,000 nets (degree 2: 60 %, 3: 20 %, 4-10: 15 %,
+11-40: 4.9 %, 41-200: 0.1 %), 2,000 perimeter pads.  The start point is the
+reference's: seedUniform followed by one alpha = 0 B2B round (untimed setup);
+every step restores it before building, so each step does the same work.
+
+Prints one JSON line (see the keys below).  --gpus N with N > 1 exits: this
+build measures one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "B2B+PCG solve ms to 1e-6 rel residual @1M cells: mixed+learned vs plain FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(nnz: int, n: int, precision: str) -> int:
+    """Algorithmic bytes of one SpMV over one axis: col (4) + val per entry;
+    rowPtr (4) + diag + x + y per row."""
+    if precision == "fp64":
+        return 12 * nnz + 28 * n
+    return 8 * nnz + 16 * n
+
+
+def cpu_baseline(d, x1, y1, tol: float) -> dict:
+    """The oracle (CPU restatement of the reference) on the same start point:
+    one B2B build + solve to tol, timed on this host's cores."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as ol
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    ol.set_threads(threads)
+    o = ol.Oracle.of(d)
+    t0 = time.perf_counter()
+    o.build_b2b(x1, y1, floors=d.flat.floors)
+    t1 = time.perf_counter()
+    r = o.solve(x1, y1, tol=tol, max_iter=1_000_000)
+    t2 = time.perf_counter()
+    return dict(value=(t2 - t0) * 1e3, unit="ms", cores=threads, kind="port",
+                sample=(f"one full step on the same {d.n}-cell system from the same start: "
+                        f"B2B build {1e3 * (t1 - t0):.0f} ms + FP64 Jacobi-PCG to {tol:g} "
+                        f"(recurrence rule) {1e3 * (t2 - t1):.0f} ms, {r['iterations']} iterations"),
+                iterations=r["iterations"])
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cells1m")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the x/y systems and batched designs would shard with no exchange; see the "
+              "design notes)", file=sys.stderr)
+        return 2
+
+    from ktplace_amd import design as dz
+    from ktplace_amd import simpl
+
+    t = time.perf_counter()
+    d = dz.workload(args.workload, seed=args.seed)
+    log(f"[bench] generated {args.workload}: n={d.n} nets={d.flat.num_nets} pins={d.flat.num_pins} "
+        f"({time.perf_counter() - t:.1f} s)")
+    ctx = simpl.Context(0).upload(d)
+    prec = simpl.KT_PREC_FP64 if args.precision == "fp64" else simpl.KT_PREC_MIXED
+
+    # start point: seedUniform + one alpha = 0 B2B round (untimed)
+    x0, y0 = dz.seed_uniform(d, 20240607)
+    ctx.build_b2b(x0, y0)
+    x1, y1 = x0.copy(), y0.copy()
+    r0 = ctx.solve_both(x1, y1, args.tol, 1_000_000)
+    log(f"[bench] start point: round-0 solve {r0['iterations']} iterations")
+    ctx.set_positions(x1, y1)
+    ctx.save_positions()
+
+    def step(profile: bool) -> dict:
+        ctx.restore_positions()
+        ctx.build_b2b_resident(0.0)
+        return ctx.solve_resident(args.tol, 1_000_000, precision=prec,
+                                  stop_rule=simpl.KT_STOP_TRUE, profile=profile)
+
+    for w in range(args.warmup):
+        r = step(False)
+        log(f"[bench] warmup {w}: {r['iterations']} iterations, true resid {r['true_resid']}")
+    ctx.sync()
+    ctx.reset_kernel_times()
+    nnz = [ctx.system_size(a)[1] for a in (0, 1)]
+    results = []
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        results.append(step(True))
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    ms = 1e3 * elapsed / max(args.steps, 1)
+    for i, r in enumerate(results):
+        log(f"[bench] step {i}: {r['iterations']} iterations, true resid {r['true_resid']}")
+
+    kid = simpl.KERNEL_SPMV if args.precision == "fp64" else simpl.KERNEL_SPMV32
+    launches, spmv_ms = ctx.kernel_time(kid)
+    total_bytes = sum(r["steps"][a] * spmv_bytes(nnz[a], d.n, args.precision)
+                      for r in results for a in (0, 1))
+    avg_ms = spmv_ms / max(launches, 1)
+    bytes_per_launch = total_bytes / max(launches, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="cgSpmv (both axes per launch)", launches=launches,
+                    avg_launch_us=round(avg_ms * 1e3, 2),
+                    bytes_per_launch=int(bytes_per_launch))
+
+    base = None
+    if not args.no_cpu_baseline:
+        log("[bench] timing the CPU oracle on the same start point ...")
+        base = cpu_baseline(d, x1, y1, args.tol)
+        log(f"[bench] cpu baseline {base['value']:.0f} ms on {base['cores']} threads")
+
+    last = results[-1]
+    out = {
+        "metric": METRIC,
+        "value": round(ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{args.workload}: one B2B assembly + lock-step Jacobi-PCG (x and y) to "
+                         f"{args.tol:g} true relative residual; {d.n} cells, {d.flat.num_nets} "
+                         f"nets, {d.flat.num_pins} pins, seed {args.seed}"),
+            "path": args.precision,
+            "warm_start": "previous iterate",
+            "iterations": last["iterations"],
+            "true_resid": list(last["true_resid"]),
+            "restarts": last["refinements"],
+            "nnz_per_axis": nnz,
+        },
+        "roofline": roofline,
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,152 @@
+/*
+ * ktplace_amd.h -- C ABI of the MI355X SimPL inner loop.
+ *
+ * Replaces, for one design at a time, the reference's solver interface:
+ *   LinearSystems::buildB2B / buildStar          simpl_lss.h:68-84, simpl_lss.cc:11-201
+ *   SparseSystem (CSR off-diagonals + diag/rhs)  simpl_lss.h:38-58
+ *   solveBothAxes (lock-step Jacobi-PCG)         simpl_lss.h:86-88, simpl_lss.cc:203-290
+ *   getHpwl / getOccupancy / getOverflow         simpl_placer.cc:256-297
+ *   LookAheadLegalizer::legalize                 simpl_lal.h:18-29, simpl_lal.cc:43-124
+ *
+ * Conventions: every entry point returns KT_OK (0) or a negative KT_E* status;
+ * kt_last_error() gives the message for the calling thread.  No C++ exception
+ * crosses the ABI.  A context belongs to one device and one host thread at a
+ * time (like the reference's LinearSystems, it is not re-entrant).  Entry
+ * points taking host pointers are synchronous with respect to those buffers.
+ * The *_resident variants work on device buffers owned by the context and are
+ * asynchronous on the context's stream; kt_sync waits for them.
+ */
+#ifndef KTPLACE_AMD_H
+#define KTPLACE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KT_OK 0
+#define KT_EINVAL -1    /* bad argument or size */
+#define KT_EHIP -2      /* HIP runtime error */
+#define KT_ESTATE -3    /* call out of order (e.g. no netlist uploaded) */
+#define KT_ENOMEM -4
+#define KT_NO_VAR 0xffffffffu /* simpl_lss.h:13 noVar */
+
+#define KT_AXIS_X 0
+#define KT_AXIS_Y 1
+
+/* SimPL's flat solver netlist (simpl_lss.h:27-36, built by simpl_dm.cc:45-85),
+ * plus the per-movable-cell sizes and obstacles of simpl_dm.h:97-112. */
+typedef struct kt_netlist {
+    uint32_t num_vertices;
+    uint32_t num_movable;
+    uint32_t num_nets;
+    uint64_t num_pins;             /* after per-net dedup */
+    uint64_t num_cell_pins;        /* pinsOfCell length */
+    uint32_t num_obstacles;
+    const uint64_t *net_start;     /* [N+1] */
+    const uint32_t *pin_cell;      /* [P] vertex id (NetPins::cells) */
+    const double *pin_off_x;       /* [P] offset from the cell's lower-left */
+    const double *pin_off_y;       /* [P] */
+    const double *net_weight;      /* [N] */
+    const uint32_t *var_of_vertex; /* [V] KT_NO_VAR for fixed */
+    const uint32_t *mov_vertex;    /* [n] */
+    const double *fixed_x;         /* [V] */
+    const double *fixed_y;         /* [V] */
+    const uint64_t *pin_start;     /* [n+1] */
+    const uint32_t *cell_pin_net;  /* [Pm] PinRef::net */
+    const uint32_t *cell_pin_idx;  /* [Pm] PinRef::pin */
+    const double *width;           /* [n] */
+    const double *height;          /* [n] */
+    const double *area;            /* [n] */
+    const double *obstacles;       /* [4*M] xl, yl, xh, yh */
+} kt_netlist;
+
+typedef struct kt_ctx kt_ctx;
+
+/* ---- context ----------------------------------------------------------- */
+int kt_ctx_create(kt_ctx **out, int device);
+int kt_ctx_destroy(kt_ctx *ctx);
+int kt_sync(kt_ctx *ctx);
+const char *kt_last_error(void);
+/* Device memory held by the context, in bytes. */
+int kt_ctx_device_bytes(kt_ctx *ctx, uint64_t *bytes);
+
+/* Copies the netlist to the device once (Netlist is const on the path,
+ * simpl_dm.h:97-112).  Replaces any previous netlist. */
+int kt_upload_netlist(kt_ctx *ctx, const kt_netlist *nl);
+
+/* ---- positions (device-resident x, y of the movable cells) ------------- */
+/* count = rows of the current systems (n, or n + star variables). */
+int kt_set_positions(kt_ctx *ctx, const double *x, const double *y, uint64_t count);
+int kt_get_positions(kt_ctx *ctx, double *x, double *y, uint64_t count);
+int kt_set_anchors(kt_ctx *ctx, const double *ax, const double *ay);
+/* Saves / restores the positions inside the context (device-to-device). */
+int kt_save_positions(kt_ctx *ctx);
+int kt_restore_positions(kt_ctx *ctx);
+
+/* ---- B2B assembly: LinearSystems::buildB2B (simpl_lss.cc:11-114) -------- */
+/* Host-pointer form: x, y [n]; anchor_x/y may be NULL when alpha <= 0.  The
+ * CSR keeps the reference's row order, entry order and duplicates. */
+int kt_build_b2b(kt_ctx *ctx, const double *x, const double *y, double alpha,
+                 const double *anchor_x, const double *anchor_y, double floor_x,
+                 double floor_y);
+/* Resident form: uses the context's positions and anchors. */
+int kt_build_b2b_resident(kt_ctx *ctx, double alpha, double floor_x, double floor_y);
+
+/* ---- SparseSystem access (simpl_lss.h:38-58) ---------------------------- */
+int kt_system_size(kt_ctx *ctx, int axis, uint64_t *rows, uint64_t *nnz);
+int kt_get_system(kt_ctx *ctx, int axis, uint64_t *row_ptr, uint32_t *col, double *val,
+                  double *diag, double *rhs);
+int kt_set_system(kt_ctx *ctx, int axis, uint64_t rows, const uint64_t *row_ptr,
+                  const uint32_t *col, const double *val, const double *diag, const double *rhs);
+/* out = A * in for one axis (SparseSystem::multiply, simpl_lss.h:49-57). */
+int kt_multiply(kt_ctx *ctx, int axis, const double *in, double *out);
+
+/* ---- solveBothAxes (simpl_lss.cc:203-290) -------------------------------- */
+#define KT_PREC_FP64 0   /* reference algorithm in FP64 */
+#define KT_PREC_MIXED 1  /* FP32 PCG with FP64 residual refinement */
+
+#define KT_STOP_RECURRENCE 0 /* reference rule: ||r_k|| / ||b|| <= tol */
+#define KT_STOP_TRUE 1       /* true FP64 ||b - A x|| / ||b|| <= tol */
+
+typedef struct kt_solve_opts {
+    double tol;
+    uint64_t max_iter;
+    int32_t precision;   /* KT_PREC_* */
+    int32_t stop_rule;   /* KT_STOP_* */
+    int32_t check_every; /* iterations per host convergence check (0 = default) */
+    int32_t profile;     /* record per-kernel HIP event timings */
+} kt_solve_opts;
+
+typedef struct kt_solve_result {
+    double resid[2];      /* final recurrence residual per axis (Residuals) */
+    double true_resid[2]; /* final true FP64 relative residual per axis */
+    uint64_t iterations;  /* lock-step iterations (the reference's loop count) */
+    uint64_t steps[2];    /* iterations each axis stepped */
+    uint64_t refinements; /* FP64 refinement sweeps (mixed precision) */
+} kt_solve_result;
+
+/* Called after the solve for every iterate, in order, with the residuals the
+ * reference's IterateObserver would see (simpl_lss.h:65-66). */
+typedef void (*kt_observer)(void *user, uint64_t iterate, double resid_x, double resid_y,
+                            int last);
+
+int kt_solve_both(kt_ctx *ctx, double *x, double *y, uint64_t count, const kt_solve_opts *opts,
+                  kt_observer observe, void *user, kt_solve_result *out);
+int kt_solve_both_resident(kt_ctx *ctx, const kt_solve_opts *opts, kt_solve_result *out);
+
+/* ---- per-kernel timings (HIP events on the context's stream) -------------- */
+#define KT_KERNEL_SPMV 0      /* FP64 CG SpMV + p.Ap */
+#define KT_KERNEL_UPDATE 1    /* FP64 x/r update + r.r, r.z */
+#define KT_KERNEL_PUPDATE 2   /* FP64 p = z + beta p */
+#define KT_KERNEL_SPMV32 3    /* FP32 CG SpMV + p.Ap */
+#define KT_KERNEL_B2B_FILL 4  /* B2B fill pass */
+#define KT_KERNEL_COUNT 8
+int kt_kernel_times(kt_ctx *ctx, int kernel, uint64_t *launches, double *total_ms);
+int kt_reset_kernel_times(kt_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,326 @@
+// Bound2Bound Laplacian assembly for x and y on gfx950.
+//
+// Follows LinearSystems::buildB2B / buildB2BAxis (simpl_lss.cc:11-114) and
+// produces the same CSR bit for bit: the reference's row order, entry order
+// within a row (pinsOfCell order; bound pin -> every other pin in index order,
+// inner pin -> lo then hi), duplicates kept, fixed neighbours dropped from the
+// CSR but kept in diag/rhs, and the same IEEE-754 double expressions in the
+// same order (the library is compiled with -ffp-contract=off, so no multiply
+// is fused into an add).
+//
+// Kernels (both axes per launch, blockIdx.y = axis where it applies):
+//   vertexPositions  px/py = fixed positions with movable cells overwritten (:14-19)
+//   netExtremes      first argmin / first argmax of pos + offset per net  (:35-45)
+//   rowCounts        movable neighbours per row                              (:75-82)
+//   scan             rowPtr = inclusive scan of the counts                  (:83)
+//   rowFill          col, val = -w, diag, rhs, anchor term                  (:89-113)
+#include "kt_internal.h"
+
+namespace kt {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kScanItems = 8;
+constexpr uint64_t kScanTile = kBlock * kScanItems;
+
+__global__ void __launch_bounds__(kBlock)
+vertexPositions(const double *__restrict__ x, const double *__restrict__ y,
+                const uint32_t *__restrict__ varOfVertex, const double *__restrict__ fixedX,
+                const double *__restrict__ fixedY, double *__restrict__ vx, double *__restrict__ vy,
+                uint32_t V) {
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= V) return;
+    const uint32_t var = varOfVertex[v];
+    vx[v] = var != kNoVar ? x[var] : fixedX[v];
+    vy[v] = var != kNoVar ? y[var] : fixedY[v];
+}
+
+// One thread per net; the running best value is the reference's pinAt(lo),
+// recomputed there from the same operands, so the comparisons are identical.
+__global__ void __launch_bounds__(kBlock)
+netExtremes(const uint32_t *__restrict__ netStart, const uint32_t *__restrict__ pinCell,
+            const double *__restrict__ offX, const double *__restrict__ offY,
+            const double *__restrict__ vx, const double *__restrict__ vy, uint4 *__restrict__ ext,
+            uint32_t N) {
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= N) return;
+    const uint32_t b = netStart[j];
+    const uint32_t p = netStart[j + 1] - b;
+    uint32_t lox = 0, hix = 0, loy = 0, hiy = 0;
+    if (p > 0) {
+        const uint32_t c0 = pinCell[b];
+        double bestLoX = vx[c0] + offX[b], bestHiX = bestLoX;
+        double bestLoY = vy[c0] + offY[b], bestHiY = bestLoY;
+        for (uint32_t q = 1; q < p; ++q) {
+            const uint32_t c = pinCell[b + q];
+            const double px = vx[c] + offX[b + q];
+            const double py = vy[c] + offY[b + q];
+            if (px < bestLoX) { lox = q; bestLoX = px; }
+            if (px > bestHiX) { hix = q; bestHiX = px; }
+            if (py < bestLoY) { loy = q; bestLoY = py; }
+            if (py > bestHiY) { hiy = q; bestHiY = py; }
+        }
+    }
+    ext[j] = make_uint4(lox, hix, loy, hiy);
+}
+
+struct RowArgs {
+    const uint32_t *pinStart;
+    const uint32_t *refNet;
+    const uint32_t *refPin;
+    const uint32_t *netStart;
+    const uint32_t *pinCell;
+    const uint32_t *varOfVertex;
+    const uint32_t *netMov;
+    const uint4 *ext;
+    const double *weight;
+    const double *off[2];
+    const double *vpos[2];
+    const double *pos[2];
+    const double *anchor[2];
+    uint32_t *rowPtr[2];
+    uint32_t *col[2];
+    double *val[2];
+    double *diag[2];
+    double *rhs[2];
+    double floor[2];
+    double alpha;
+    uint32_t n;
+};
+
+__device__ __forceinline__ uint32_t extLo(const uint4 &e, int axis) { return axis ? e.z : e.x; }
+__device__ __forceinline__ uint32_t extHi(const uint4 &e, int axis) { return axis ? e.w : e.y; }
+
+// std::max(a, b) is (a < b) ? b : a; written out so NaN behaves the same.
+__device__ __forceinline__ double refMax(double a, double b) { return a < b ? b : a; }
+
+__global__ void __launch_bounds__(kBlock) rowCounts(RowArgs a) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const int axis = blockIdx.y;
+    if (i >= a.n) return;
+    uint32_t count = 0;
+    for (uint32_t e = a.pinStart[i]; e < a.pinStart[i + 1]; ++e) {
+        const uint32_t net = a.refNet[e];
+        const uint32_t q = a.refPin[e];
+        const uint4 x = a.ext[net];
+        const uint32_t lo = extLo(x, axis), hi = extHi(x, axis);
+        if (lo == hi) continue;
+        if (q == lo || q == hi) {
+            count += a.netMov[net] - 1;  // every other pin; q itself is movable
+        } else {
+            const uint32_t b = a.netStart[net];
+            count += (a.varOfVertex[a.pinCell[b + lo]] != kNoVar ? 1u : 0u) +
+                     (a.varOfVertex[a.pinCell[b + hi]] != kNoVar ? 1u : 0u);
+        }
+    }
+    a.rowPtr[axis][i + 1] = count;
+    if (i == 0) a.rowPtr[axis][0] = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) rowFill(RowArgs a) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const int axis = blockIdx.y;
+    if (i >= a.n) return;
+    const double *__restrict__ pos = a.vpos[axis];
+    const double *__restrict__ off = a.off[axis];
+    const double lengthFloor = a.floor[axis];
+    uint32_t *__restrict__ col = a.col[axis];
+    double *__restrict__ val = a.val[axis];
+    uint32_t slot = a.rowPtr[axis][i];
+    double diag = 0.0, rhs = 0.0;
+    for (uint32_t e = a.pinStart[i]; e < a.pinStart[i + 1]; ++e) {
+        const uint32_t net = a.refNet[e];
+        const uint32_t q = a.refPin[e];
+        const uint4 x = a.ext[net];
+        const uint32_t lo = extLo(x, axis), hi = extHi(x, axis);
+        if (lo == hi) continue;
+        const uint32_t b = a.netStart[net];
+        const uint32_t p = a.netStart[net + 1] - b;
+        const double base = a.weight[net] / static_cast<double>(p - 1);
+        const double ownPos = pos[a.pinCell[b + q]];
+        const double own = ownPos + off[b + q];
+        const double ownOffset = own - ownPos;
+        auto edge = [&](uint32_t other) {
+            const uint32_t cell = a.pinCell[b + other];
+            const double cellPos = pos[cell];
+            const double there = cellPos + off[b + other];
+            const double w = base / refMax(fabs(own - there), lengthFloor);
+            const uint32_t var = a.varOfVertex[cell];
+            diag += w;
+            if (var != kNoVar) {
+                col[slot] = var;
+                val[slot] = -w;
+                ++slot;
+                rhs += w * ((there - cellPos) - ownOffset);
+            } else {
+                rhs += w * (there - ownOffset);
+            }
+        };
+        if (q == lo || q == hi) {
+            for (uint32_t other = 0; other < p; ++other) {
+                if (other != q) edge(other);
+            }
+        } else {
+            edge(lo);
+            edge(hi);
+        }
+    }
+    if (a.alpha > 0.0) {
+        const double anchor = a.anchor[axis][i];
+        const double w = a.alpha / refMax(fabs(a.pos[axis][i] - anchor), lengthFloor);
+        diag += w;
+        rhs += w * anchor;
+    }
+    a.diag[axis][i] = refMax(diag, 1e-12);
+    a.rhs[axis][i] = rhs;
+}
+
+// Inclusive in-place scan of one tile of kScanTile elements; tile totals go to sums.
+__global__ void __launch_bounds__(kBlock)
+scanTile(uint32_t *__restrict__ data, uint64_t m, uint32_t *__restrict__ sums) {
+    __shared__ uint32_t waveSums[kBlock / 64];
+    const uint64_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t idx = base + k;
+        s += idx < m ? data[idx] : 0u;
+        v[k] = s;
+    }
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    uint32_t x = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) waveSums[wave] = x;
+    __syncthreads();
+    uint32_t waveOffset = 0;
+    for (int w = 0; w < wave; ++w) waveOffset += waveSums[w];
+    const uint32_t excl = waveOffset + x - s;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t idx = base + k;
+        if (idx < m) data[idx] = v[k] + excl;
+    }
+    if (threadIdx.x == kBlock - 1) sums[blockIdx.x] = waveOffset + x;
+}
+
+__global__ void __launch_bounds__(kBlock)
+scanAdd(uint32_t *__restrict__ data, uint64_t m, const uint32_t *__restrict__ sums) {
+    if (blockIdx.x == 0) return;
+    const uint32_t add = sums[blockIdx.x - 1];
+    const uint64_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t idx = base + k;
+        if (idx < m) data[idx] += add;
+    }
+}
+
+int scanLevel(kt_ctx *ctx, uint32_t *data, uint64_t m, uint64_t tmpOffset) {
+    if (m == 0) return KT_OK;
+    const uint64_t blocks = (m + kScanTile - 1) / kScanTile;
+    KT_CHECK(tmpOffset + blocks <= ctx->scanTmp.cap, KT_EINVAL, "scan: scratch too small");
+    uint32_t *sums = ctx->scanTmp.p + tmpOffset;
+    scanTile<<<dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, ctx->stream>>>(data, m, sums);
+    KT_HIP(hipGetLastError());
+    if (blocks > 1) {
+        KT_TRY(scanLevel(ctx, sums, blocks, tmpOffset + blocks));
+        scanAdd<<<dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, ctx->stream>>>(data, m, sums);
+        KT_HIP(hipGetLastError());
+    }
+    return KT_OK;
+}
+
+}  // namespace
+
+uint64_t scanScratch(uint64_t m) {
+    uint64_t total = 0;
+    while (m > 1) {
+        m = (m + kScanTile - 1) / kScanTile;
+        total += m;
+    }
+    return total + 1;
+}
+
+int scanRowPtr(kt_ctx *ctx, uint32_t *rowPtr, uint64_t rows) {
+    KT_TRY(ctx->scanTmp.reserve(scanScratch(rows)));
+    return scanLevel(ctx, rowPtr + 1, rows, 0);
+}
+
+int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY) {
+    KT_CHECK(ctx->haveNetlist, KT_ESTATE, "buildB2B: no netlist uploaded");
+    KT_CHECK(ctx->posCount == ctx->n, KT_ESTATE, "buildB2B: positions must have one entry per movable cell");
+    KT_CHECK(!(alpha > 0.0) || ctx->haveAnchors, KT_ESTATE, "buildB2B: alpha > 0 needs anchors");
+    const uint32_t n = ctx->n;
+    for (int axis = 0; axis < 2; ++axis) KT_TRY(ensureSystem(ctx, axis, n, ctx->nnzBound));
+    KT_TRY(ctx->vx.reserve(ctx->V));
+    KT_TRY(ctx->vy.reserve(ctx->V));
+    KT_TRY(ctx->ext.reserve(4ull * ctx->N));
+    if (ctx->V > 0) {
+        vertexPositions<<<dim3((ctx->V + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream>>>(
+            ctx->posX.p, ctx->posY.p, ctx->varOfVertex.p, ctx->fixedX.p, ctx->fixedY.p, ctx->vx.p,
+            ctx->vy.p, ctx->V);
+        KT_HIP(hipGetLastError());
+    }
+    uint4 *ext = reinterpret_cast<uint4 *>(ctx->ext.p);
+    if (ctx->N > 0) {
+        netExtremes<<<dim3((ctx->N + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream>>>(
+            ctx->netStart.p, ctx->pinCell.p, ctx->offX.p, ctx->offY.p, ctx->vx.p, ctx->vy.p, ext,
+            ctx->N);
+        KT_HIP(hipGetLastError());
+    }
+    RowArgs a{};
+    a.pinStart = ctx->pinStart.p;
+    a.refNet = ctx->refNet.p;
+    a.refPin = ctx->refPin.p;
+    a.netStart = ctx->netStart.p;
+    a.pinCell = ctx->pinCell.p;
+    a.varOfVertex = ctx->varOfVertex.p;
+    a.netMov = ctx->netMov.p;
+    a.ext = ext;
+    a.weight = ctx->weight.p;
+    a.off[0] = ctx->offX.p;
+    a.off[1] = ctx->offY.p;
+    a.vpos[0] = ctx->vx.p;
+    a.vpos[1] = ctx->vy.p;
+    a.pos[0] = ctx->posX.p;
+    a.pos[1] = ctx->posY.p;
+    a.anchor[0] = ctx->anchorX.p;
+    a.anchor[1] = ctx->anchorY.p;
+    for (int axis = 0; axis < 2; ++axis) {
+        Sys &s = ctx->sys[axis];
+        a.rowPtr[axis] = s.rowPtr.p;
+        a.col[axis] = s.col.p;
+        a.val[axis] = s.val.p;
+        a.diag[axis] = s.diag.p;
+        a.rhs[axis] = s.rhs.p;
+        s.nnzHost = -1;
+        s.have32 = false;
+    }
+    a.floor[0] = floorX;
+    a.floor[1] = floorY;
+    a.alpha = alpha;
+    a.n = n;
+    if (n == 0) {
+        for (int axis = 0; axis < 2; ++axis)
+            KT_HIP(hipMemsetAsync(ctx->sys[axis].rowPtr.p, 0, sizeof(uint32_t), ctx->stream));
+        return KT_OK;
+    }
+    const dim3 grid((n + kBlock - 1) / kBlock, 2);
+    rowCounts<<<grid, dim3(kBlock), 0, ctx->stream>>>(a);
+    KT_HIP(hipGetLastError());
+    for (int axis = 0; axis < 2; ++axis) KT_TRY(scanRowPtr(ctx, ctx->sys[axis].rowPtr.p, n));
+    hipEvent_t t0 = nullptr;
+    KT_TRY(timedBegin(ctx, KT_KERNEL_B2B_FILL, &t0));
+    rowFill<<<grid, dim3(kBlock), 0, ctx->stream>>>(a);
+    KT_HIP(hipGetLastError());
+    KT_TRY(timedEnd(ctx, KT_KERNEL_B2B_FILL, t0));
+    return KT_OK;
+}
+
+}  // namespace kt

@@ -1,0 +1,555 @@
+// C ABI entry points (include/ktplace_amd.h): context, netlist upload,
+// positions, SparseSystem access, solve wrapper and kernel timers.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "kt_internal.h"
+
+namespace kt {
+
+thread_local std::string lastError;
+
+void setError(const std::string &msg) { lastError = msg; }
+
+int hipFail(hipError_t e, const char *what, const char *file, int line) {
+    lastError = std::string(what) + ": " + hipGetErrorString(e) + " (" + file + ":" + std::to_string(line) + ")";
+    return e == hipErrorOutOfMemory ? KT_ENOMEM : KT_EHIP;
+}
+
+int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY);
+int solveFp64(kt_ctx *ctx, const kt_solve_opts &o, kt_solve_result &res);
+int solveMixed(kt_ctx *ctx, const kt_solve_opts &o, kt_solve_result &res);
+int solveHistory(kt_ctx *ctx, const kt_solve_result &res, std::vector<double> &hist, uint64_t &cap);
+int multiply(kt_ctx *ctx, int axis, const double *dIn, double *dOut);
+
+int ensureSystem(kt_ctx *ctx, int axis, uint64_t rows, uint64_t nnzCap) {
+    Sys &s = ctx->sys[axis];
+    KT_CHECK(rows < 0xffffffffull && nnzCap < 0xffffffffull, KT_EINVAL, "system too large for 32-bit indices");
+    KT_TRY(s.rowPtr.reserve(rows + 1));
+    KT_TRY(s.col.reserve(std::max<uint64_t>(nnzCap, 1)));
+    KT_TRY(s.val.reserve(std::max<uint64_t>(nnzCap, 1)));
+    KT_TRY(s.diag.reserve(std::max<uint64_t>(rows, 1)));
+    KT_TRY(s.rhs.reserve(std::max<uint64_t>(rows, 1)));
+    s.rows = rows;
+    s.nnzCap = nnzCap;
+    return KT_OK;
+}
+
+int systemNnz(kt_ctx *ctx, int axis, uint64_t *nnz) {
+    Sys &s = ctx->sys[axis];
+    if (s.nnzHost < 0) {
+        uint32_t v = 0;
+        if (s.rowPtr.p) {
+            KT_HIP(hipMemcpyAsync(&v, s.rowPtr.p + s.rows, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
+            KT_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        s.nnzHost = v;
+    }
+    *nnz = static_cast<uint64_t>(s.nnzHost);
+    return KT_OK;
+}
+
+int timedBegin(kt_ctx *ctx, int kernel, hipEvent_t *start) {
+    *start = nullptr;
+    if (!ctx->profile) return KT_OK;
+    KT_HIP(hipEventCreate(start));
+    KT_HIP(hipEventRecord(*start, ctx->stream));
+    return KT_OK;
+}
+
+int timedEnd(kt_ctx *ctx, int kernel, hipEvent_t start) {
+    if (!ctx->profile || start == nullptr) return KT_OK;
+    hipEvent_t stop;
+    KT_HIP(hipEventCreate(&stop));
+    KT_HIP(hipEventRecord(stop, ctx->stream));
+    ctx->timers[kernel].pending.push_back(start);
+    ctx->timers[kernel].pending.push_back(stop);
+    return KT_OK;
+}
+
+int collectTimers(kt_ctx *ctx) {
+    KT_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto &t : ctx->timers) {
+        for (size_t i = 0; i + 1 < t.pending.size(); i += 2) {
+            float ms = 0.0f;
+            KT_HIP(hipEventElapsedTime(&ms, t.pending[i], t.pending[i + 1]));
+            t.ms += ms;
+            ++t.launches;
+            (void)hipEventDestroy(t.pending[i]);
+            (void)hipEventDestroy(t.pending[i + 1]);
+        }
+        t.pending.clear();
+    }
+    return KT_OK;
+}
+
+namespace {
+
+template <typename T>
+int upload(kt_ctx *ctx, DevBuf<T> &dst, const T *src, size_t count) {
+    KT_TRY(dst.reserve(std::max<size_t>(count, 1)));
+    if (count) KT_HIP(hipMemcpyAsync(dst.p, src, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    return KT_OK;
+}
+
+int narrow(const uint64_t *src, size_t count, std::vector<uint32_t> &out, const char *what) {
+    out.resize(count);
+    for (size_t i = 0; i < count; ++i) {
+        if (src[i] > 0xffffffffull) {
+            setError(std::string(what) + " exceeds 32 bits");
+            return KT_EINVAL;
+        }
+        out[i] = static_cast<uint32_t>(src[i]);
+    }
+    return KT_OK;
+}
+
+int uploadNetlist(kt_ctx *ctx, const kt_netlist &nl) {
+    const uint32_t V = nl.num_vertices, n = nl.num_movable, N = nl.num_nets;
+    const uint64_t P = nl.num_pins, Pm = nl.num_cell_pins;
+    KT_CHECK(P < 0xffffffffull && Pm < 0xffffffffull, KT_EINVAL, "upload: too many pins for 32-bit indices");
+    KT_CHECK(nl.net_start && nl.var_of_vertex && nl.pin_start, KT_EINVAL, "upload: null netlist array");
+    KT_CHECK(nl.net_start[N] == P, KT_EINVAL, "upload: net_start[N] != num_pins");
+    KT_CHECK(nl.pin_start[n] == Pm, KT_EINVAL, "upload: pin_start[n] != num_cell_pins");
+    // validate indices on the host so no kernel can read out of bounds
+    for (uint32_t j = 0; j < N; ++j)
+        KT_CHECK(nl.net_start[j] <= nl.net_start[j + 1], KT_EINVAL, "upload: net_start not monotone");
+    for (uint32_t i = 0; i < n; ++i)
+        KT_CHECK(nl.pin_start[i] <= nl.pin_start[i + 1], KT_EINVAL, "upload: pin_start not monotone");
+    for (uint64_t k = 0; k < P; ++k) KT_CHECK(nl.pin_cell[k] < V, KT_EINVAL, "upload: pin cell out of range");
+    for (uint32_t v = 0; v < V; ++v)
+        KT_CHECK(nl.var_of_vertex[v] == kNoVar || nl.var_of_vertex[v] < n, KT_EINVAL, "upload: var out of range");
+    for (uint32_t i = 0; i < n; ++i) {
+        KT_CHECK(nl.mov_vertex[i] < V && nl.var_of_vertex[nl.mov_vertex[i]] == i, KT_EINVAL,
+                 "upload: mov_vertex / var_of_vertex mismatch");
+    }
+    std::vector<uint32_t> netMov(N, 0);
+    uint64_t bound = 0;
+    for (uint32_t j = 0; j < N; ++j) {
+        const uint64_t p = nl.net_start[j + 1] - nl.net_start[j];
+        for (uint64_t k = nl.net_start[j]; k < nl.net_start[j + 1]; ++k)
+            netMov[j] += nl.var_of_vertex[nl.pin_cell[k]] != kNoVar ? 1u : 0u;
+        if (p >= 2) bound += 4 * p - 6;  // 2 bound pins x (p-1) + (p-2) inner pins x 2
+    }
+    // every pinsOfCell entry of row i must be a pin of cell i: the B2B count and
+    // fill passes rely on it to stay inside the row's CSR slots
+    for (uint32_t i = 0; i < n; ++i) {
+        for (uint64_t e = nl.pin_start[i]; e < nl.pin_start[i + 1]; ++e) {
+            const uint32_t j = nl.cell_pin_net[e];
+            KT_CHECK(j < N, KT_EINVAL, "upload: pinsOfCell net out of range");
+            KT_CHECK(nl.cell_pin_idx[e] < nl.net_start[j + 1] - nl.net_start[j], KT_EINVAL,
+                     "upload: pinsOfCell pin out of range");
+            KT_CHECK(nl.pin_cell[nl.net_start[j] + nl.cell_pin_idx[e]] == nl.mov_vertex[i], KT_EINVAL,
+                     "upload: pinsOfCell entry is not a pin of its row's cell");
+        }
+    }
+    KT_CHECK(bound < 0xffffffffull, KT_EINVAL, "upload: B2B entries exceed 32-bit indices");
+    std::vector<uint32_t> netStart32, pinStart32;
+    KT_TRY(narrow(nl.net_start, N + 1ull, netStart32, "net_start"));
+    KT_TRY(narrow(nl.pin_start, n + 1ull, pinStart32, "pin_start"));
+    KT_TRY(upload(ctx, ctx->netStart, netStart32.data(), netStart32.size()));
+    KT_TRY(upload(ctx, ctx->pinStart, pinStart32.data(), pinStart32.size()));
+    KT_TRY(upload(ctx, ctx->pinCell, nl.pin_cell, P));
+    KT_TRY(upload(ctx, ctx->offX, nl.pin_off_x, P));
+    KT_TRY(upload(ctx, ctx->offY, nl.pin_off_y, P));
+    KT_TRY(upload(ctx, ctx->weight, nl.net_weight, N));
+    KT_TRY(upload(ctx, ctx->varOfVertex, nl.var_of_vertex, V));
+    KT_TRY(upload(ctx, ctx->movVertex, nl.mov_vertex, n));
+    KT_TRY(upload(ctx, ctx->fixedX, nl.fixed_x, V));
+    KT_TRY(upload(ctx, ctx->fixedY, nl.fixed_y, V));
+    KT_TRY(upload(ctx, ctx->refNet, nl.cell_pin_net, Pm));
+    KT_TRY(upload(ctx, ctx->refPin, nl.cell_pin_idx, Pm));
+    KT_TRY(upload(ctx, ctx->netMov, netMov.data(), N));
+    if (nl.width) KT_TRY(upload(ctx, ctx->width, nl.width, n));
+    if (nl.height) KT_TRY(upload(ctx, ctx->height, nl.height, n));
+    if (nl.area) KT_TRY(upload(ctx, ctx->area, nl.area, n));
+    KT_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->V = V;
+    ctx->n = n;
+    ctx->N = N;
+    ctx->P = P;
+    ctx->Pm = Pm;
+    ctx->nnzBound = bound;
+    ctx->haveNetlist = true;
+    ctx->haveAnchors = false;
+    ctx->posCount = 0;
+    return KT_OK;
+}
+
+int setPositions(kt_ctx *ctx, const double *x, const double *y, uint64_t count) {
+    KT_CHECK(x && y, KT_EINVAL, "positions: null pointer");
+    KT_CHECK(count < 0xffffffffull, KT_EINVAL, "positions: too many");
+    KT_TRY(ctx->posX.reserve(std::max<uint64_t>(count, 1)));
+    KT_TRY(ctx->posY.reserve(std::max<uint64_t>(count, 1)));
+    if (count) {
+        KT_HIP(hipMemcpyAsync(ctx->posX.p, x, count * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        KT_HIP(hipMemcpyAsync(ctx->posY.p, y, count * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
+    ctx->posCount = count;
+    return KT_OK;
+}
+
+int getPositions(kt_ctx *ctx, double *x, double *y, uint64_t count) {
+    KT_CHECK(x && y, KT_EINVAL, "positions: null pointer");
+    KT_CHECK(count == ctx->posCount, KT_EINVAL, "positions: count differs from the stored positions");
+    if (count) {
+        KT_HIP(hipMemcpyAsync(x, ctx->posX.p, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KT_HIP(hipMemcpyAsync(y, ctx->posY.p, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    KT_HIP(hipStreamSynchronize(ctx->stream));
+    return KT_OK;
+}
+
+int setAnchors(kt_ctx *ctx, const double *ax, const double *ay) {
+    KT_CHECK(ctx->haveNetlist, KT_ESTATE, "anchors: no netlist uploaded");
+    KT_CHECK(ax && ay, KT_EINVAL, "anchors: null pointer");
+    KT_TRY(upload(ctx, ctx->anchorX, ax, ctx->n));
+    KT_TRY(upload(ctx, ctx->anchorY, ay, ctx->n));
+    ctx->haveAnchors = true;
+    return KT_OK;
+}
+
+int solve(kt_ctx *ctx, const kt_solve_opts *opts, kt_solve_result *out) {
+    KT_CHECK(opts && out, KT_EINVAL, "solve: null argument");
+    KT_CHECK(opts->precision == KT_PREC_FP64 || opts->precision == KT_PREC_MIXED, KT_EINVAL,
+             "solve: unknown precision");
+    ctx->profile = opts->profile != 0;
+    kt_solve_result res{};
+    int rc;
+    if (opts->precision == KT_PREC_FP64) {
+        rc = solveFp64(ctx, *opts, res);
+    } else {
+        rc = solveMixed(ctx, *opts, res);
+    }
+    if (rc != KT_OK) return rc;
+    if (ctx->profile) KT_TRY(collectTimers(ctx));
+    *out = res;
+    return KT_OK;
+}
+
+}  // namespace
+}  // namespace kt
+
+using namespace kt;
+
+#define KT_GUARD(ctx)                                                      \
+    do {                                                                   \
+        if ((ctx) == nullptr) {                                            \
+            setError("null context");                                      \
+            return KT_EINVAL;                                              \
+        }                                                                  \
+        hipError_t kt_se_ = hipSetDevice((ctx)->device);                   \
+        if (kt_se_ != hipSuccess) return hipFail(kt_se_, "hipSetDevice", __FILE__, __LINE__); \
+    } while (0)
+
+#define KT_CATCH(body)                                        \
+    try {                                                     \
+        body                                                  \
+    } catch (const std::bad_alloc &) {                        \
+        setError("host out of memory");                       \
+        return KT_ENOMEM;                                     \
+    } catch (const std::exception &e) {                       \
+        setError(e.what());                                   \
+        return KT_EINVAL;                                     \
+    }
+
+extern "C" {
+
+const char *kt_last_error(void) { return lastError.c_str(); }
+
+int kt_ctx_create(kt_ctx **out, int device) {
+    KT_CHECK(out != nullptr, KT_EINVAL, "kt_ctx_create: null out");
+    *out = nullptr;
+    int count = 0;
+    KT_HIP(hipGetDeviceCount(&count));
+    KT_CHECK(device >= 0 && device < count, KT_EINVAL, "kt_ctx_create: no such device");
+    KT_HIP(hipSetDevice(device));
+    KT_CATCH({
+        auto *ctx = new kt_ctx();
+        ctx->device = device;
+        hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&ctx->ctlHost), 2 * sizeof(AxisCtl));
+        if (e != hipSuccess) {
+            if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+            delete ctx;
+            return hipFail(e, "kt_ctx_create", __FILE__, __LINE__);
+        }
+        *out = ctx;
+        return KT_OK;
+    })
+}
+
+int kt_ctx_destroy(kt_ctx *ctx) {
+    if (ctx == nullptr) return KT_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &t : ctx->timers)
+        for (hipEvent_t ev : t.pending) (void)hipEventDestroy(ev);
+    ctx->netStart.release(); ctx->pinCell.release(); ctx->varOfVertex.release(); ctx->movVertex.release();
+    ctx->pinStart.release(); ctx->refNet.release(); ctx->refPin.release(); ctx->netMov.release();
+    ctx->offX.release(); ctx->offY.release(); ctx->weight.release(); ctx->fixedX.release();
+    ctx->fixedY.release(); ctx->width.release(); ctx->height.release(); ctx->area.release();
+    ctx->posX.release(); ctx->posY.release(); ctx->saveX.release(); ctx->saveY.release();
+    ctx->anchorX.release(); ctx->anchorY.release(); ctx->vx.release(); ctx->vy.release();
+    ctx->ext.release(); ctx->scanTmp.release();
+    for (auto &s : ctx->sys) {
+        s.rowPtr.release(); s.col.release(); s.val.release(); s.diag.release(); s.rhs.release();
+        s.val32.release(); s.diag32.release();
+    }
+    for (int a = 0; a < 2; ++a) {
+        ctx->r[a].release(); ctx->p[a].release(); ctx->ap[a].release(); ctx->xs[a].release();
+        ctx->r32[a].release(); ctx->p32[a].release(); ctx->ap32[a].release(); ctx->x32[a].release();
+    }
+    ctx->partials.release(); ctx->ctl.release(); ctx->history.release();
+    if (ctx->ctlHost) (void)hipHostFree(ctx->ctlHost);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return KT_OK;
+}
+
+int kt_sync(kt_ctx *ctx) {
+    KT_GUARD(ctx);
+    KT_HIP(hipStreamSynchronize(ctx->stream));
+    return KT_OK;
+}
+
+int kt_ctx_device_bytes(kt_ctx *ctx, uint64_t *bytes) {
+    KT_GUARD(ctx);
+    KT_CHECK(bytes, KT_EINVAL, "null out");
+    uint64_t b = 0;
+    b += ctx->netStart.bytes() + ctx->pinCell.bytes() + ctx->varOfVertex.bytes() + ctx->movVertex.bytes();
+    b += ctx->pinStart.bytes() + ctx->refNet.bytes() + ctx->refPin.bytes() + ctx->netMov.bytes();
+    b += ctx->offX.bytes() + ctx->offY.bytes() + ctx->weight.bytes() + ctx->fixedX.bytes() + ctx->fixedY.bytes();
+    b += ctx->width.bytes() + ctx->height.bytes() + ctx->area.bytes();
+    b += ctx->posX.bytes() + ctx->posY.bytes() + ctx->saveX.bytes() + ctx->saveY.bytes();
+    b += ctx->anchorX.bytes() + ctx->anchorY.bytes() + ctx->vx.bytes() + ctx->vy.bytes() + ctx->ext.bytes();
+    b += ctx->scanTmp.bytes();
+    for (auto &s : ctx->sys)
+        b += s.rowPtr.bytes() + s.col.bytes() + s.val.bytes() + s.diag.bytes() + s.rhs.bytes() + s.val32.bytes() +
+             s.diag32.bytes();
+    for (int a = 0; a < 2; ++a)
+        b += ctx->r[a].bytes() + ctx->p[a].bytes() + ctx->ap[a].bytes() + ctx->xs[a].bytes() + ctx->r32[a].bytes() +
+             ctx->p32[a].bytes() + ctx->ap32[a].bytes() + ctx->x32[a].bytes();
+    b += ctx->partials.bytes() + ctx->ctl.bytes() + ctx->history.bytes();
+    *bytes = b;
+    return KT_OK;
+}
+
+int kt_upload_netlist(kt_ctx *ctx, const kt_netlist *nl) {
+    KT_GUARD(ctx);
+    KT_CHECK(nl, KT_EINVAL, "kt_upload_netlist: null netlist");
+    KT_CATCH(return uploadNetlist(ctx, *nl);)
+}
+
+int kt_set_positions(kt_ctx *ctx, const double *x, const double *y, uint64_t count) {
+    KT_GUARD(ctx);
+    return setPositions(ctx, x, y, count);
+}
+
+int kt_get_positions(kt_ctx *ctx, double *x, double *y, uint64_t count) {
+    KT_GUARD(ctx);
+    return getPositions(ctx, x, y, count);
+}
+
+int kt_set_anchors(kt_ctx *ctx, const double *ax, const double *ay) {
+    KT_GUARD(ctx);
+    return setAnchors(ctx, ax, ay);
+}
+
+int kt_save_positions(kt_ctx *ctx) {
+    KT_GUARD(ctx);
+    const uint64_t c = ctx->posCount;
+    KT_TRY(ctx->saveX.reserve(std::max<uint64_t>(c, 1)));
+    KT_TRY(ctx->saveY.reserve(std::max<uint64_t>(c, 1)));
+    if (c) {
+        KT_HIP(hipMemcpyAsync(ctx->saveX.p, ctx->posX.p, c * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        KT_HIP(hipMemcpyAsync(ctx->saveY.p, ctx->posY.p, c * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return KT_OK;
+}
+
+int kt_restore_positions(kt_ctx *ctx) {
+    KT_GUARD(ctx);
+    const uint64_t c = ctx->posCount;
+    KT_CHECK(ctx->saveX.cap >= c, KT_ESTATE, "kt_restore_positions: nothing saved");
+    if (c) {
+        KT_HIP(hipMemcpyAsync(ctx->posX.p, ctx->saveX.p, c * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        KT_HIP(hipMemcpyAsync(ctx->posY.p, ctx->saveY.p, c * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return KT_OK;
+}
+
+int kt_build_b2b(kt_ctx *ctx, const double *x, const double *y, double alpha, const double *anchor_x,
+                 const double *anchor_y, double floor_x, double floor_y) {
+    KT_GUARD(ctx);
+    KT_CHECK(ctx->haveNetlist, KT_ESTATE, "kt_build_b2b: no netlist uploaded");
+    KT_TRY(setPositions(ctx, x, y, ctx->n));
+    if (alpha > 0.0) {
+        KT_CHECK(anchor_x && anchor_y, KT_EINVAL, "kt_build_b2b: alpha > 0 needs anchors");
+        KT_TRY(setAnchors(ctx, anchor_x, anchor_y));
+    }
+    KT_TRY(buildB2B(ctx, alpha, floor_x, floor_y));
+    KT_HIP(hipStreamSynchronize(ctx->stream));
+    return KT_OK;
+}
+
+int kt_build_b2b_resident(kt_ctx *ctx, double alpha, double floor_x, double floor_y) {
+    KT_GUARD(ctx);
+    return buildB2B(ctx, alpha, floor_x, floor_y);
+}
+
+int kt_system_size(kt_ctx *ctx, int axis, uint64_t *rows, uint64_t *nnz) {
+    KT_GUARD(ctx);
+    KT_CHECK(axis == 0 || axis == 1, KT_EINVAL, "axis must be 0 or 1");
+    KT_CHECK(rows && nnz, KT_EINVAL, "null out");
+    *rows = ctx->sys[axis].rows;
+    return systemNnz(ctx, axis, nnz);
+}
+
+int kt_get_system(kt_ctx *ctx, int axis, uint64_t *row_ptr, uint32_t *col, double *val, double *diag,
+                  double *rhs) {
+    KT_GUARD(ctx);
+    KT_CHECK(axis == 0 || axis == 1, KT_EINVAL, "axis must be 0 or 1");
+    Sys &s = ctx->sys[axis];
+    uint64_t nnz;
+    KT_TRY(systemNnz(ctx, axis, &nnz));
+    KT_CATCH({
+        std::vector<uint32_t> rp(s.rows + 1);
+        if (s.rows + 1 > 0 && s.rowPtr.p)
+            KT_HIP(hipMemcpyAsync(rp.data(), s.rowPtr.p, rp.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+        if (nnz) {
+            if (col) KT_HIP(hipMemcpyAsync(col, s.col.p, nnz * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+            if (val) KT_HIP(hipMemcpyAsync(val, s.val.p, nnz * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        }
+        if (s.rows) {
+            if (diag) KT_HIP(hipMemcpyAsync(diag, s.diag.p, s.rows * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            if (rhs) KT_HIP(hipMemcpyAsync(rhs, s.rhs.p, s.rows * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        }
+        KT_HIP(hipStreamSynchronize(ctx->stream));
+        if (row_ptr)
+            for (size_t i = 0; i < rp.size(); ++i) row_ptr[i] = rp[i];
+        return KT_OK;
+    })
+}
+
+int kt_set_system(kt_ctx *ctx, int axis, uint64_t rows, const uint64_t *row_ptr, const uint32_t *col,
+                  const double *val, const double *diag, const double *rhs) {
+    KT_GUARD(ctx);
+    KT_CHECK(axis == 0 || axis == 1, KT_EINVAL, "axis must be 0 or 1");
+    KT_CHECK(row_ptr && diag && rhs, KT_EINVAL, "kt_set_system: null array");
+    const uint64_t nnz = row_ptr[rows];
+    KT_CHECK(nnz == 0 || (col && val), KT_EINVAL, "kt_set_system: null col/val");
+    KT_CHECK(row_ptr[0] == 0, KT_EINVAL, "kt_set_system: row_ptr[0] != 0");
+    for (uint64_t i = 0; i < rows; ++i)
+        KT_CHECK(row_ptr[i] <= row_ptr[i + 1], KT_EINVAL, "kt_set_system: row_ptr not monotone");
+    for (uint64_t e = 0; e < nnz; ++e) KT_CHECK(col[e] < rows, KT_EINVAL, "kt_set_system: column out of range");
+    KT_TRY(ensureSystem(ctx, axis, rows, nnz));
+    Sys &s = ctx->sys[axis];
+    KT_CATCH({
+        std::vector<uint32_t> rp;
+        KT_TRY(narrow(row_ptr, rows + 1, rp, "row_ptr"));
+        KT_HIP(hipMemcpyAsync(s.rowPtr.p, rp.data(), rp.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        if (nnz) {
+            KT_HIP(hipMemcpyAsync(s.col.p, col, nnz * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+            KT_HIP(hipMemcpyAsync(s.val.p, val, nnz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        }
+        if (rows) {
+            KT_HIP(hipMemcpyAsync(s.diag.p, diag, rows * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            KT_HIP(hipMemcpyAsync(s.rhs.p, rhs, rows * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        }
+        KT_HIP(hipStreamSynchronize(ctx->stream));
+        s.nnzHost = static_cast<int64_t>(nnz);
+        s.have32 = false;
+        return KT_OK;
+    })
+}
+
+int kt_multiply(kt_ctx *ctx, int axis, const double *in, double *out) {
+    KT_GUARD(ctx);
+    KT_CHECK(axis == 0 || axis == 1, KT_EINVAL, "axis must be 0 or 1");
+    KT_CHECK(in && out, KT_EINVAL, "kt_multiply: null pointer");
+    const uint64_t rows = ctx->sys[axis].rows;
+    uint64_t nnz;
+    KT_TRY(systemNnz(ctx, axis, &nnz));
+    double *dIn = nullptr, *dOut = nullptr;
+    KT_HIP(hipMalloc(reinterpret_cast<void **>(&dIn), std::max<uint64_t>(rows, 1) * sizeof(double)));
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&dOut), std::max<uint64_t>(rows, 1) * sizeof(double));
+    if (e != hipSuccess) {
+        (void)hipFree(dIn);
+        return hipFail(e, "hipMalloc", __FILE__, __LINE__);
+    }
+    int rc = KT_OK;
+    e = hipMemcpyAsync(dIn, in, rows * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) rc = multiply(ctx, axis, dIn, dOut);
+    if (e == hipSuccess && rc == KT_OK)
+        e = hipMemcpyAsync(out, dOut, rows * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dIn);
+    (void)hipFree(dOut);
+    if (e != hipSuccess) return hipFail(e, "kt_multiply", __FILE__, __LINE__);
+    return rc;
+}
+
+int kt_solve_both_resident(kt_ctx *ctx, const kt_solve_opts *opts, kt_solve_result *out) {
+    KT_GUARD(ctx);
+    KT_CATCH(return solve(ctx, opts, out);)
+}
+
+int kt_solve_both(kt_ctx *ctx, double *x, double *y, uint64_t count, const kt_solve_opts *opts,
+                  kt_observer observe, void *user, kt_solve_result *out) {
+    KT_GUARD(ctx);
+    KT_CATCH({
+        KT_TRY(setPositions(ctx, x, y, count));
+        kt_solve_result res{};
+        KT_TRY(solve(ctx, opts, &res));
+        KT_TRY(getPositions(ctx, x, y, count));
+        if (observe && res.iterations > 0) {
+            std::vector<double> hist;
+            uint64_t cap = 0;
+            KT_TRY(solveHistory(ctx, res, hist, cap));
+            const AxisCtl *c = ctx->ctlHost;
+            for (uint64_t it = 0; it < res.iterations; ++it) {
+                double r[2];
+                for (int a = 0; a < 2; ++a) {
+                    const uint64_t st = res.steps[a];
+                    r[a] = (st == 0 || it >= st || it >= cap) ? res.resid[a] : hist[a * cap + it];
+                    if (st > 0 && it >= st && st - 1 < cap) r[a] = hist[a * cap + st - 1];
+                }
+                bool last = false;
+                if (it + 1 == res.iterations) {
+                    const bool liveX = c[0].active && res.resid[0] > opts->tol;
+                    const bool liveY = c[1].active && res.resid[1] > opts->tol;
+                    last = !liveX && !liveY;
+                }
+                observe(user, it + 1, r[0], r[1], last ? 1 : 0);
+            }
+        }
+        *out = res;
+        return KT_OK;
+    })
+}
+
+int kt_kernel_times(kt_ctx *ctx, int kernel, uint64_t *launches, double *total_ms) {
+    KT_GUARD(ctx);
+    KT_CHECK(kernel >= 0 && kernel < KT_KERNEL_COUNT, KT_EINVAL, "unknown kernel id");
+    KT_TRY(collectTimers(ctx));
+    if (launches) *launches = ctx->timers[kernel].launches;
+    if (total_ms) *total_ms = ctx->timers[kernel].ms;
+    return KT_OK;
+}
+
+int kt_reset_kernel_times(kt_ctx *ctx) {
+    KT_GUARD(ctx);
+    KT_TRY(collectTimers(ctx));
+    for (auto &t : ctx->timers) {
+        t.launches = 0;
+        t.ms = 0.0;
+    }
+    return KT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,163 @@
+// Internal definitions shared by the HIP translation units of libktplace_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ktplace_amd.h"
+
+namespace kt {
+
+// ------------------------------------------------------------------ errors
+void setError(const std::string &msg);
+int hipFail(hipError_t e, const char *what, const char *file, int line);
+
+#define KT_HIP(call)                                                        \
+    do {                                                                    \
+        hipError_t kt_e_ = (call);                                          \
+        if (kt_e_ != hipSuccess) return ::kt::hipFail(kt_e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define KT_CHECK(cond, code, msg)           \
+    do {                                    \
+        if (!(cond)) {                      \
+            ::kt::setError(msg);            \
+            return (code);                  \
+        }                                   \
+    } while (0)
+
+#define KT_TRY(expr)                        \
+    do {                                    \
+        int kt_rc_ = (expr);                \
+        if (kt_rc_ != KT_OK) return kt_rc_; \
+    } while (0)
+
+// ------------------------------------------------------------------ buffers
+// Grow-only device buffer.  Reallocation drops the old contents.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+
+    int reserve(size_t count) {
+        if (count <= cap) return KT_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t bytes = (count == 0 ? 1 : count) * sizeof(T);
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), bytes);
+        if (e != hipSuccess) return hipFail(e, "hipMalloc", __FILE__, __LINE__);
+        cap = count;
+        return KT_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    size_t bytes() const { return cap * sizeof(T); }
+};
+
+constexpr uint32_t kNoVar = KT_NO_VAR;
+
+// Per-axis CG control block in device memory.  Written by the last block of
+// each reduction kernel; read by the following kernels.
+struct AxisCtl {
+    double bNorm;     // ||b||
+    double rho;       // r.z
+    double resid;     // recurrence residual ||r|| / ||b||
+    double alpha;
+    double beta;
+    double pap;
+    double rr;
+    double rz;
+    int32_t active;   // the reference's Axis::active
+    int32_t live;     // step this iteration: active && resid > tol && steps < maxIter
+    int32_t doUpdate; // p.Ap > 0 this iteration
+    int32_t doP;      // r.z > 0 this iteration
+    uint32_t steps;   // iterations stepped
+    uint32_t counter; // last-block-done counter (reset by the last block)
+    double tol;
+    uint32_t maxIter;
+    int32_t frozen;   // excluded from a restart (converged or broken down)
+    double trueResid; // ||b - A x|| / ||b|| from the last true-residual pass
+};
+
+// One axis' CSR as the CG kernels see it.
+struct SysView {
+    const uint32_t *rowPtr;
+    const uint32_t *col;
+    const double *val;
+    const double *diag;
+    const double *rhs;
+    const float *val32;
+    const float *diag32;
+    uint32_t rows;
+    uint32_t blocks;  // blocks this axis occupies in a two-axis launch
+};
+
+struct Sys {
+    uint64_t rows = 0;
+    uint64_t nnzCap = 0;  // entries allocated
+    int64_t nnzHost = -1; // nnz when known on the host (-1: read rowPtr[rows])
+    DevBuf<uint32_t> rowPtr, col;
+    DevBuf<double> val, diag, rhs;
+    DevBuf<float> val32, diag32;
+    bool have32 = false;
+};
+
+struct KernelTimer {
+    uint64_t launches = 0;
+    double ms = 0.0;
+    std::vector<hipEvent_t> pending;  // start/stop pairs not yet read back
+};
+
+}  // namespace kt
+
+struct kt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // netlist (device)
+    bool haveNetlist = false;
+    uint32_t n = 0, V = 0, N = 0;
+    uint64_t P = 0, Pm = 0, nnzBound = 0;
+    kt::DevBuf<uint32_t> netStart, pinCell, varOfVertex, movVertex, pinStart, refNet, refPin, netMov;
+    kt::DevBuf<double> offX, offY, weight, fixedX, fixedY, width, height, area;
+
+    // positions (rows of the current systems) and anchors
+    uint64_t posCount = 0;
+    kt::DevBuf<double> posX, posY, saveX, saveY, anchorX, anchorY;
+    bool haveAnchors = false;
+    kt::DevBuf<double> vx, vy;      // vertex-indexed positions (px, py)
+    kt::DevBuf<uint32_t> ext;       // [N][4]: lo_x, hi_x, lo_y, hi_y
+    kt::DevBuf<uint32_t> scanTmp;   // block sums for the row-pointer scan
+
+    kt::Sys sys[2];
+
+    // CG state
+    kt::DevBuf<double> r[2], p[2], ap[2], xs[2];
+    kt::DevBuf<float> r32[2], p32[2], ap32[2], x32[2];
+    kt::DevBuf<double> partials;    // [2][maxBlocks][3]
+    kt::DevBuf<kt::AxisCtl> ctl;    // [2]
+    kt::AxisCtl *ctlHost = nullptr; // pinned mirror
+    kt::DevBuf<double> history;     // [2][maxIter]
+    uint64_t historyCap = 0;
+
+    // timing
+    bool profile = false;
+    kt::KernelTimer timers[KT_KERNEL_COUNT];
+};
+
+namespace kt {
+int ensureSystem(kt_ctx *ctx, int axis, uint64_t rows, uint64_t nnzCap);
+int scanRowPtr(kt_ctx *ctx, uint32_t *rowPtr, uint64_t rows);
+int systemNnz(kt_ctx *ctx, int axis, uint64_t *nnz);
+int timedBegin(kt_ctx *ctx, int kernel, hipEvent_t *start);
+int timedEnd(kt_ctx *ctx, int kernel, hipEvent_t start);
+int collectTimers(kt_ctx *ctx);
+}  // namespace kt

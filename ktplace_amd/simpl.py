@@ -1,0 +1,300 @@
+"""Host-side mirror of the reference's SimPL solver interface over the C ABI.
+
+Names and argument meaning follow simpl_lss.h:
+
+  SparseSystem                 simpl_lss.h:38-58
+  LinearSystems.buildB2B       simpl_lss.h:75-76  (simpl_lss.cc:11-114)
+  solveBothAxes                simpl_lss.h:86-88  (simpl_lss.cc:203-290)
+
+Everything runs in libktplace_amd.so on the GPU; this module only marshals
+numpy arrays through ctypes.  There is no CPU fallback: if the library or a
+GPU is missing, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import P, U32, U64, F64, I32, lib
+from .design import Design, Flat
+
+KT_PREC_FP64 = 0
+KT_PREC_MIXED = 1
+KT_STOP_RECURRENCE = 0
+KT_STOP_TRUE = 1
+
+KERNEL_SPMV, KERNEL_UPDATE, KERNEL_PUPDATE, KERNEL_SPMV32, KERNEL_B2B_FILL = 0, 1, 2, 3, 4
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iter", C.c_uint64), ("precision", C.c_int32),
+                ("stop_rule", C.c_int32), ("check_every", C.c_int32), ("profile", C.c_int32)]
+
+
+class SolveResult(C.Structure):
+    _fields_ = [("resid", C.c_double * 2), ("true_resid", C.c_double * 2),
+                ("iterations", C.c_uint64), ("steps", C.c_uint64 * 2),
+                ("refinements", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return dict(resid=(self.resid[0], self.resid[1]),
+                    true_resid=(self.true_resid[0], self.true_resid[1]),
+                    iterations=int(self.iterations), steps=(int(self.steps[0]), int(self.steps[1])),
+                    refinements=int(self.refinements))
+
+
+OBSERVER = C.CFUNCTYPE(None, C.c_void_p, C.c_uint64, C.c_double, C.c_double, C.c_int)
+
+_SIGS = {
+    "kt_ctx_create": (I32, [C.POINTER(C.c_void_p), I32]),
+    "kt_ctx_destroy": (I32, [P]),
+    "kt_sync": (I32, [P]),
+    "kt_last_error": (C.c_char_p, []),
+    "kt_ctx_device_bytes": (I32, [P, C.POINTER(C.c_uint64)]),
+    "kt_upload_netlist": (I32, [P, P]),
+    "kt_set_positions": (I32, [P, P, P, U64]),
+    "kt_get_positions": (I32, [P, P, P, U64]),
+    "kt_set_anchors": (I32, [P, P, P]),
+    "kt_save_positions": (I32, [P]),
+    "kt_restore_positions": (I32, [P]),
+    "kt_build_b2b": (I32, [P, P, P, F64, P, P, F64, F64]),
+    "kt_build_b2b_resident": (I32, [P, F64, F64, F64]),
+    "kt_system_size": (I32, [P, I32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "kt_get_system": (I32, [P, I32, P, P, P, P, P]),
+    "kt_set_system": (I32, [P, I32, U64, P, P, P, P, P]),
+    "kt_multiply": (I32, [P, I32, P, P]),
+    "kt_solve_both": (I32, [P, P, P, U64, C.POINTER(SolveOpts), OBSERVER, P,
+                            C.POINTER(SolveResult)]),
+    "kt_solve_both_resident": (I32, [P, C.POINTER(SolveOpts), C.POINTER(SolveResult)]),
+    "kt_kernel_times": (I32, [P, I32, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+    "kt_reset_kernel_times": (I32, [P]),
+}
+
+EXPORTS = tuple(_SIGS)
+
+for _name, (_res, _args) in _SIGS.items():
+    _fn = getattr(lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+
+class KtError(RuntimeError):
+    pass
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise KtError(f"status {rc}: {lib.kt_last_error().decode()}")
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def netlist_struct(flat: Flat) -> _lib.KtgFlat:
+    """kt_netlist view of a Flat (arrays are kept alive by the returned struct)."""
+    s = _lib.KtgFlat()
+    s.num_vertices = flat.num_vertices
+    s.num_movable = flat.num_movable
+    s.num_nets = flat.num_nets
+    s.num_pins = flat.num_pins
+    s.num_cell_pins = len(flat.cell_pin_net)
+    s.num_obstacles = len(flat.obstacles)
+    keep = {}
+    for name in ("net_start", "pin_cell", "pin_off_x", "pin_off_y", "net_weight", "var_of_vertex",
+                 "mov_vertex", "fixed_x", "fixed_y", "pin_start", "cell_pin_net", "cell_pin_idx",
+                 "width", "height", "area"):
+        arr = np.ascontiguousarray(getattr(flat, name))
+        keep[name] = arr
+        setattr(s, name, _lib.ptr(arr))
+    obs = np.ascontiguousarray(flat.obstacles.reshape(-1), np.float64)
+    keep["obstacles"] = obs
+    s.obstacles = _lib.ptr(obs) if len(obs) else None
+    s._keep = keep
+    return s
+
+
+@dataclass
+class SparseSystem:
+    """One axis' system (simpl_lss.h:38-58): CSR off-diagonals + diag + rhs."""
+    row_ptr: np.ndarray  # u64 [n+1]
+    col: np.ndarray      # u32 [nnz]
+    val: np.ndarray      # f64 [nnz]
+    diag: np.ndarray     # f64 [n]
+    rhs: np.ndarray      # f64 [n]
+
+    def size(self) -> int:
+        return len(self.diag)
+
+
+class Context:
+    """One design on one GPU (a kt_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib.kt_ctx_create(C.byref(h), device))
+        self.h = h
+        self.n = 0
+        self.floors = (1.0, 1.0)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.kt_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # netlist -----------------------------------------------------------------
+    def upload(self, design_or_flat) -> "Context":
+        flat = design_or_flat.flat if isinstance(design_or_flat, Design) else design_or_flat
+        s = netlist_struct(flat)
+        _check(lib.kt_upload_netlist(self.h, C.byref(s)))
+        self.n = flat.num_movable
+        self.floors = flat.floors
+        return self
+
+    # positions ---------------------------------------------------------------
+    def set_positions(self, x, y):
+        x, y = _f64(x), _f64(y)
+        _check(lib.kt_set_positions(self.h, _lib.ptr(x), _lib.ptr(y), len(x)))
+
+    def get_positions(self, count: Optional[int] = None):
+        count = self.n if count is None else count
+        x = np.zeros(count)
+        y = np.zeros(count)
+        _check(lib.kt_get_positions(self.h, _lib.ptr(x), _lib.ptr(y), count))
+        return x, y
+
+    def set_anchors(self, ax, ay):
+        ax, ay = _f64(ax), _f64(ay)
+        _check(lib.kt_set_anchors(self.h, _lib.ptr(ax), _lib.ptr(ay)))
+
+    def save_positions(self):
+        _check(lib.kt_save_positions(self.h))
+
+    def restore_positions(self):
+        _check(lib.kt_restore_positions(self.h))
+
+    def sync(self):
+        _check(lib.kt_sync(self.h))
+
+    def device_bytes(self) -> int:
+        b = C.c_uint64()
+        _check(lib.kt_ctx_device_bytes(self.h, C.byref(b)))
+        return b.value
+
+    # LinearSystems -----------------------------------------------------------
+    def build_b2b(self, x, y, alpha: float = 0.0, anchor_x=None, anchor_y=None, floors=None):
+        """LinearSystems::buildB2B (simpl_lss.cc:11-114) into the device systems."""
+        fx, fy = floors if floors is not None else self.floors
+        x, y = _f64(x), _f64(y)
+        ax = None if anchor_x is None else _f64(anchor_x)
+        ay = None if anchor_y is None else _f64(anchor_y)
+        _check(lib.kt_build_b2b(self.h, _lib.ptr(x), _lib.ptr(y), float(alpha), _lib.ptr(ax),
+                                _lib.ptr(ay), float(fx), float(fy)))
+
+    def build_b2b_resident(self, alpha: float = 0.0, floors=None):
+        fx, fy = floors if floors is not None else self.floors
+        _check(lib.kt_build_b2b_resident(self.h, float(alpha), float(fx), float(fy)))
+
+    def system_size(self, axis: int):
+        rows = C.c_uint64()
+        nnz = C.c_uint64()
+        _check(lib.kt_system_size(self.h, axis, C.byref(rows), C.byref(nnz)))
+        return rows.value, nnz.value
+
+    def get_system(self, axis: int) -> SparseSystem:
+        rows, nnz = self.system_size(axis)
+        s = SparseSystem(np.zeros(rows + 1, np.uint64), np.zeros(nnz, np.uint32), np.zeros(nnz),
+                         np.zeros(rows), np.zeros(rows))
+        _check(lib.kt_get_system(self.h, axis, _lib.ptr(s.row_ptr), _lib.ptr(s.col),
+                                 _lib.ptr(s.val), _lib.ptr(s.diag), _lib.ptr(s.rhs)))
+        return s
+
+    def set_system(self, axis: int, s):
+        rp = np.ascontiguousarray(s.row_ptr, np.uint64)
+        col = np.ascontiguousarray(s.col, np.uint32)
+        val = _f64(s.val)
+        diag = _f64(s.diag)
+        rhs = _f64(s.rhs)
+        _check(lib.kt_set_system(self.h, axis, len(diag), _lib.ptr(rp), _lib.ptr(col),
+                                 _lib.ptr(val), _lib.ptr(diag), _lib.ptr(rhs)))
+
+    def multiply(self, axis: int, v) -> np.ndarray:
+        v = _f64(v)
+        out = np.zeros_like(v)
+        _check(lib.kt_multiply(self.h, axis, _lib.ptr(v), _lib.ptr(out)))
+        return out
+
+    # solveBothAxes -----------------------------------------------------------
+    def solve_both(self, x, y, tol: float, max_iter: int, precision: int = KT_PREC_FP64,
+                   stop_rule: int = KT_STOP_RECURRENCE, observe: Optional[Callable] = None,
+                   check_every: int = 0, profile: bool = False):
+        """solveBothAxes: x, y are updated in place (numpy float64 arrays)."""
+        if x.dtype != np.float64 or y.dtype != np.float64 or not x.flags["C_CONTIGUOUS"]:
+            raise TypeError("x and y must be contiguous float64 arrays (updated in place)")
+        opts = SolveOpts(tol=tol, max_iter=max_iter, precision=precision, stop_rule=stop_rule,
+                         check_every=check_every, profile=int(profile))
+        res = SolveResult()
+        cb = OBSERVER(lambda _u, it, rx, ry, last: observe(int(it), (rx, ry), bool(last))) \
+            if observe else OBSERVER()
+        _check(lib.kt_solve_both(self.h, _lib.ptr(x), _lib.ptr(y), len(x), C.byref(opts), cb,
+                                 None, C.byref(res)))
+        return res.as_dict()
+
+    def solve_resident(self, tol: float, max_iter: int, precision: int = KT_PREC_FP64,
+                       stop_rule: int = KT_STOP_RECURRENCE, check_every: int = 0,
+                       profile: bool = False) -> dict:
+        opts = SolveOpts(tol=tol, max_iter=max_iter, precision=precision, stop_rule=stop_rule,
+                         check_every=check_every, profile=int(profile))
+        res = SolveResult()
+        _check(lib.kt_solve_both_resident(self.h, C.byref(opts), C.byref(res)))
+        return res.as_dict()
+
+    def kernel_time(self, kernel: int):
+        n = C.c_uint64()
+        ms = C.c_double()
+        _check(lib.kt_kernel_times(self.h, kernel, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def reset_kernel_times(self):
+        _check(lib.kt_reset_kernel_times(self.h))
+
+
+class LinearSystems:
+    """The reference's LinearSystems (simpl_lss.h:68-84) bound to a GPU context."""
+
+    def __init__(self, ctx: Context, floor_x: float, floor_y: float):
+        self.ctx = ctx
+        self.floorX = floor_x
+        self.floorY = floor_y
+
+    def buildB2B(self, x, y, alpha, anchorX, anchorY):
+        self.ctx.build_b2b(x, y, alpha, anchorX if len(anchorX) else None,
+                           anchorY if len(anchorY) else None, floors=(self.floorX, self.floorY))
+
+    @property
+    def sysX(self) -> SparseSystem:
+        return self.ctx.get_system(0)
+
+    @property
+    def sysY(self) -> SparseSystem:
+        return self.ctx.get_system(1)
+
+
+def solveBothAxes(ctx: Context, x: np.ndarray, y: np.ndarray, tolerance: float,
+                  maxIterations: int, observe: Optional[Callable] = None):
+    """solveBothAxes on the context's current systems; returns (resid_x, resid_y)."""
+    r = ctx.solve_both(x, y, tolerance, maxIterations, observe=observe)
+    return r["resid"]

@@ -1,0 +1,251 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: the oracle is the checker, never the thing measured or
+shipped.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
+load it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from ktplace_amd.design import Design, Raw
+
+ORACLE_PATH = Path(__file__).resolve().parent.parent / "oracle" / "liboracle.so"
+
+
+class PlaceOpts(C.Structure):
+    _fields_ = [
+        ("bins_per_axis", C.c_uint64), ("cg_tol", C.c_double), ("init_max_iters", C.c_uint64),
+        ("initial_net_model", C.c_int32), ("pad0", C.c_int32), ("max_iters", C.c_uint64),
+        ("seed", C.c_uint64), ("density_limit", C.c_double), ("alpha_base", C.c_double),
+        ("refine_iters", C.c_uint64), ("cg_max_iter", C.c_uint64),
+    ]
+
+
+class PlaceStats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "hpwl_seed", "hpwl_init", "hpwl_lower", "hpwl_final", "gap", "overflow_lower",
+        "overflow_final", "build_seconds", "solve_seconds", "spread_seconds")] + [
+        (n, C.c_uint64) for n in ("init_iters", "global_iters", "cg_iterations")]
+
+
+def _load():
+    if not ORACLE_PATH.exists():
+        raise ImportError(f"{ORACLE_PATH} missing: run `make -C oracle`")
+    lib = C.CDLL(str(ORACLE_PATH))
+    P, U64, F64, I = C.c_void_p, C.c_uint64, C.c_double, C.c_int
+    sig = {
+        "ko_set_threads": (None, [I]), "ko_get_threads": (I, []),
+        "ko_design_create": (P, [P]), "ko_design_free": (None, [P]),
+        "ko_design_sizes": (None, [P, P]),
+        "ko_design_flat": (None, [P] * 12),
+        "ko_build_b2b": (None, [P, P, P, F64, P, P, F64, F64]),
+        "ko_build_star": (U64, [P]),
+        "ko_system_sizes": (None, [P, I, P, P]),
+        "ko_system_get": (None, [P, I, P, P, P, P, P]),
+        "ko_system_set": (None, [P, I, U64, P, P, P, P, P]),
+        "ko_multiply": (None, [P, I, P, P]),
+        "ko_dot": (F64, [P, P, U64]),
+        "ko_solve_both": (U64, [P, P, P, F64, U64, P, P]),
+        "ko_hpwl": (F64, [P, P, P]),
+        "ko_seed_uniform": (None, [P, U64, P, P]),
+        "ko_build_grid": (U64, [P, U64, F64]),
+        "ko_grid_avail": (None, [P, P, P]),
+        "ko_occupancy": (None, [P, P, P, P]),
+        "ko_overflow": (F64, [P, P]),
+        "ko_profile": (U64, [P, P, I, P, P, U64]),
+        "ko_lal": (None, [P, P, P]),
+        "ko_place": (None, [P, P, P, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class System:
+    """One axis of a SparseSystem (simpl_lss.h:38-58) as numpy arrays."""
+
+    def __init__(self, row_ptr, col, val, diag, rhs):
+        self.row_ptr, self.col, self.val, self.diag, self.rhs = row_ptr, col, val, diag, rhs
+
+    @property
+    def n(self):
+        return len(self.diag)
+
+    @property
+    def nnz(self):
+        return len(self.col)
+
+    def dense(self) -> np.ndarray:
+        a = np.diag(self.diag.copy())
+        for i in range(self.n):
+            for e in range(int(self.row_ptr[i]), int(self.row_ptr[i + 1])):
+                a[i, self.col[e]] += self.val[e]
+        return a
+
+    def scipy(self):
+        import scipy.sparse as sp
+        n = self.n
+        rows = np.repeat(np.arange(n), np.diff(self.row_ptr.astype(np.int64)))
+        off = sp.csr_matrix((self.val, (rows, self.col.astype(np.int64))), shape=(n, n))
+        return (off + sp.diags(self.diag)).tocsr()
+
+
+class Oracle:
+    """One design in the oracle (a Design handle plus its two SparseSystems)."""
+
+    def __init__(self, raw: Raw):
+        c = raw.to_c()
+        self._raw = raw
+        self.h = lib.ko_design_create(C.addressof(c))
+        if not self.h:
+            raise RuntimeError("ko_design_create failed")
+        sizes = np.zeros(6, np.uint64)
+        lib.ko_design_sizes(self.h, _p(sizes))
+        self.V, self.n, self.N, self.P, self.Pm, self.M = (int(s) for s in sizes)
+
+    @classmethod
+    def of(cls, d: Design) -> "Oracle":
+        return cls(d.raw)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.ko_design_free(self.h)
+            self.h = None
+
+    def flat(self) -> dict:
+        out = dict(
+            net_start=np.zeros(self.N + 1, np.uint64), pin_cell=np.zeros(self.P, np.uint32),
+            pin_off_x=np.zeros(self.P), pin_off_y=np.zeros(self.P), net_weight=np.zeros(self.N),
+            var_of_vertex=np.zeros(self.V, np.uint32), mov_vertex=np.zeros(self.n, np.uint32),
+            pin_start=np.zeros(self.n + 1, np.uint64), cell_pin_net=np.zeros(self.Pm, np.uint32),
+            cell_pin_idx=np.zeros(self.Pm, np.uint32), floors=np.zeros(2))
+        lib.ko_design_flat(self.h, *[_p(out[k]) for k in (
+            "net_start", "pin_cell", "pin_off_x", "pin_off_y", "net_weight", "var_of_vertex",
+            "mov_vertex", "pin_start", "cell_pin_net", "cell_pin_idx", "floors")])
+        return out
+
+    def build_b2b(self, x, y, alpha=0.0, ax=None, ay=None, floors=None):
+        fx, fy = floors if floors is not None else self.flat()["floors"]
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        ax = None if ax is None else np.ascontiguousarray(ax, np.float64)
+        ay = None if ay is None else np.ascontiguousarray(ay, np.float64)
+        lib.ko_build_b2b(self.h, _p(x), _p(y), alpha, _p(ax), _p(ay), float(fx), float(fy))
+
+    def build_star(self) -> int:
+        return int(lib.ko_build_star(self.h))
+
+    def system(self, axis: int) -> System:
+        rows = C.c_uint64()
+        nnz = C.c_uint64()
+        lib.ko_system_sizes(self.h, axis, C.byref(rows), C.byref(nnz))
+        n, m = rows.value, nnz.value
+        s = System(np.zeros(n + 1, np.uint64), np.zeros(m, np.uint32), np.zeros(m), np.zeros(n),
+                   np.zeros(n))
+        lib.ko_system_get(self.h, axis, _p(s.row_ptr), _p(s.col), _p(s.val), _p(s.diag), _p(s.rhs))
+        return s
+
+    def set_system(self, axis: int, s: System):
+        rp = np.ascontiguousarray(s.row_ptr, np.uint64)
+        col = np.ascontiguousarray(s.col, np.uint32)
+        val = np.ascontiguousarray(s.val, np.float64)
+        diag = np.ascontiguousarray(s.diag, np.float64)
+        rhs = np.ascontiguousarray(s.rhs, np.float64)
+        lib.ko_system_set(self.h, axis, len(diag), _p(rp), _p(col), _p(val), _p(diag), _p(rhs))
+
+    def multiply(self, axis: int, v):
+        v = np.ascontiguousarray(v, np.float64)
+        out = np.zeros_like(v)
+        lib.ko_multiply(self.h, axis, _p(v), _p(out))
+        return out
+
+    def solve(self, x, y, tol=1e-6, max_iter=100000, history=False):
+        x = np.array(x, np.float64)
+        y = np.array(y, np.float64)
+        resid = np.zeros(2)
+        hist = np.zeros(2 * max_iter) if history else None
+        it = int(lib.ko_solve_both(self.h, _p(x), _p(y), tol, max_iter, _p(resid), _p(hist)))
+        out = dict(x=x, y=y, resid=resid, iterations=it)
+        if history:
+            out["history"] = hist[:2 * it].reshape(it, 2)
+        return out
+
+    def hpwl(self, x, y) -> float:
+        return float(lib.ko_hpwl(self.h, _p(np.ascontiguousarray(x, np.float64)),
+                                 _p(np.ascontiguousarray(y, np.float64))))
+
+    def seed_uniform(self, seed: int):
+        x = np.zeros(self.n)
+        y = np.zeros(self.n)
+        lib.ko_seed_uniform(self.h, seed, _p(x), _p(y))
+        return x, y
+
+    def build_grid(self, bins_per_axis=0, density=1.0) -> int:
+        self.bins = int(lib.ko_build_grid(self.h, bins_per_axis, density))
+        return self.bins
+
+    def grid_avail(self):
+        a = np.zeros(self.bins * self.bins)
+        tot = C.c_double()
+        lib.ko_grid_avail(self.h, _p(a), C.byref(tot))
+        return a, tot.value
+
+    def occupancy(self, x, y):
+        occ = np.zeros(self.bins * self.bins)
+        lib.ko_occupancy(self.h, _p(np.ascontiguousarray(x, np.float64)),
+                         _p(np.ascontiguousarray(y, np.float64)), _p(occ))
+        return occ
+
+    def overflow(self, occ) -> float:
+        return float(lib.ko_overflow(self.h, _p(np.ascontiguousarray(occ, np.float64))))
+
+    def profile(self, box, vertical: bool, cap=4096):
+        b = np.array(box, np.float64)
+        e = np.zeros(cap)
+        p = np.zeros(cap)
+        k = int(lib.ko_profile(self.h, _p(b), int(vertical), _p(e), _p(p), cap))
+        return e[:k], p[:k]
+
+    def lal(self, x, y):
+        x = np.array(x, np.float64)
+        y = np.array(y, np.float64)
+        lib.ko_lal(self.h, _p(x), _p(y))
+        return x, y
+
+    def place(self, **kw):
+        o = PlaceOpts(bins_per_axis=0, cg_tol=1e-4, init_max_iters=15, initial_net_model=0,
+                      max_iters=100, seed=20240607, density_limit=1.0, alpha_base=0.02,
+                      refine_iters=10, cg_max_iter=60)
+        for k, v in kw.items():
+            setattr(o, k, v)
+        x = np.zeros(self.n)
+        y = np.zeros(self.n)
+        st = PlaceStats()
+        trace = np.zeros(3 * o.max_iters)
+        lib.ko_place(self.h, C.addressof(o), _p(x), _p(y), C.addressof(st), _p(trace))
+        stats = {f: getattr(st, f) for f, _ in PlaceStats._fields_}
+        g = int(stats["global_iters"])
+        return x, y, stats, trace[:3 * g].reshape(-1, 3)
+
+
+def set_threads(k: int):
+    lib.ko_set_threads(k)
+
+
+def dot(a, b) -> float:
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return float(lib.ko_dot(_p(a), _p(b), len(a)))

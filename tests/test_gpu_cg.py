@@ -1,0 +1,159 @@
+"""Jacobi-PCG on the GPU (FP64) against the oracle and scipy.
+
+Tolerances: the GPU reduces dot products in a different (fixed) order than
+the reference's 4096-element chunks, so iterates agree to rounding, not bit
+for bit.  Solutions are compared with a relative 2-norm error bound of 1e-6
+against the oracle solved to 1e-12.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from ktplace_amd import design as dz
+from test_oracle import line_design, spd_system
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def set_both(ctx, o, sx, sy):
+    for axis, s in ((0, sx), (1, sy)):
+        ctx.set_system(axis, s)
+        o.set_system(axis, s)
+
+
+def test_cg_random_spd_vs_scipy(gpu_ctx_factory):
+    from scipy.sparse.linalg import spsolve
+    rng = np.random.default_rng(21)
+    sx, sy = spd_system(rng, 3000, 12000), spd_system(rng, 3000, 9000)
+    ctx = gpu_ctx_factory()
+    ctx.set_system(0, sx)
+    ctx.set_system(1, sy)
+    x = np.zeros(sx.n)
+    y = np.zeros(sy.n)
+    r = ctx.solve_both(x, y, 1e-12, 100000)
+    for sol, s in ((x, sx), (y, sy)):
+        exact = spsolve(s.scipy().tocsc(), s.rhs)
+        assert rel(sol, exact) < 1e-9
+    assert max(r["true_resid"]) < 1e-11
+    ctx.close()
+
+
+@pytest.fixture(scope="module")
+def b2b12k():
+    d = dz.workload("cells12k")
+    o = ol.Oracle.of(d)
+    x0, y0 = o.seed_uniform(20240607)
+    o.build_b2b(x0, y0, floors=d.flat.floors)
+    return d, o, x0, y0
+
+
+def test_cg_tracks_oracle_iterates(b2b12k, gpu_ctx_factory):
+    """Same start, same tolerance: same iteration count (+-1) and solutions
+    within 1e-6; the per-iteration residuals (observer) agree closely."""
+    d, o, x0, y0 = b2b12k
+    ctx = gpu_ctx_factory(d)
+    ctx.build_b2b(x0, y0)
+    x, y = x0.copy(), y0.copy()
+    seen = []
+    r = ctx.solve_both(x, y, 1e-6, 100000, observe=lambda it, res, last: seen.append((it, res, last)))
+    ref = o.solve(x0, y0, tol=1e-6, max_iter=100000, history=True)
+    print("gpu iters", r["iterations"], "oracle iters", ref["iterations"], flush=True)
+    assert abs(r["iterations"] - ref["iterations"]) <= 1
+    assert rel(x, ref["x"]) < 1e-6 and rel(y, ref["y"]) < 1e-6
+    assert [s[0] for s in seen] == list(range(1, r["iterations"] + 1))
+    assert seen[-1][2] and not any(s[2] for s in seen[:-1])
+    hist = np.array([s[1] for s in seen])
+    k = min(len(hist), len(ref["history"])) - 2
+    assert np.allclose(hist[:k], ref["history"][:k], rtol=1e-6)
+    assert max(r["resid"]) <= 1e-6
+
+
+def test_cg_solution_within_1e6_of_tight_oracle(b2b12k, gpu_ctx_factory):
+    d, o, x0, y0 = b2b12k
+    ctx = gpu_ctx_factory(d)
+    ctx.build_b2b(x0, y0)
+    x, y = x0.copy(), y0.copy()
+    r = ctx.solve_both(x, y, 1e-9, 100000)
+    ref = o.solve(x0, y0, tol=1e-12, max_iter=100000)
+    assert rel(x, ref["x"]) < 1e-6 and rel(y, ref["y"]) < 1e-6
+    assert max(r["true_resid"]) < 1e-8
+
+
+def test_cg_rules_zero_rhs_maxiter_breakdown(gpu_ctx_factory):
+    rng = np.random.default_rng(5)
+    sx = spd_system(rng)
+    zero = ol.System(sx.row_ptr, sx.col, sx.val, sx.diag, np.zeros(sx.n))
+    neg = ol.System(sx.row_ptr, sx.col, -sx.val, -sx.diag, sx.rhs)
+    o = ol.Oracle(line_design([1.0, 2.0]))
+    ctx = gpu_ctx_factory()
+    x0 = np.zeros(sx.n)
+    # zero rhs: axis inactive, residual 0
+    set_both(ctx, o, sx, zero)
+    x, y = x0.copy(), x0.copy()
+    r = ctx.solve_both(x, y, 1e-8, 5000)
+    ref = o.solve(x0, x0, tol=1e-8, max_iter=5000)
+    assert r["resid"][1] == 0.0 and r["steps"][1] == 0
+    assert abs(r["iterations"] - ref["iterations"]) <= 1
+    assert np.array_equal(y, x0)
+    # max_iter = 0
+    x, y = x0.copy(), x0.copy()
+    r = ctx.solve_both(x, y, 1e-8, 0)
+    assert r["iterations"] == 0 and np.array_equal(x, x0)
+    # indefinite: p.Ap <= 0 stops the y axis after one step, x continues
+    set_both(ctx, o, sx, neg)
+    x, y = x0.copy(), x0.copy()
+    seen = []
+    r = ctx.solve_both(x, y, 1e-8, 50, observe=lambda it, res, last: seen.append(res))
+    ref = o.solve(x0, x0, tol=1e-8, max_iter=50, history=True)
+    assert r["steps"][1] == 1 and r["iterations"] == ref["iterations"]
+    assert np.array_equal(y, x0)
+    assert all(s[1] == seen[0][1] for s in seen)
+    ctx.close()
+
+
+def test_cg_max_iter_cap_and_warm_start(b2b12k, gpu_ctx_factory):
+    d, o, x0, y0 = b2b12k
+    ctx = gpu_ctx_factory(d)
+    ctx.build_b2b(x0, y0)
+    x, y = x0.copy(), y0.copy()
+    r = ctx.solve_both(x, y, 1e-6, 60)  # the reference's hidden cap
+    ref = o.solve(x0, y0, tol=1e-6, max_iter=60)
+    assert r["iterations"] == ref["iterations"] == 60
+    assert rel(x, ref["x"]) < 1e-9
+    # warm start: continuing from the capped iterate converges in fewer steps
+    r2 = ctx.solve_both(x, y, 1e-6, 100000)
+    ref2 = o.solve(ref["x"], ref["y"], tol=1e-6, max_iter=100000)
+    assert abs(r2["iterations"] - ref2["iterations"]) <= 1
+    ctx.close()
+
+
+def test_cg_isolated_cell_floor(gpu_ctx_factory):
+    """A row with no entries and diag 1e-12 (isolated cell) solves like the oracle."""
+    rng = np.random.default_rng(8)
+    s = spd_system(rng, 200, 500)
+    rp = s.row_ptr.astype(np.int64)
+    # empty row 7: drop its entries and the mirrored ones
+    keep = np.ones(len(s.col), bool)
+    rows = np.repeat(np.arange(s.n), np.diff(rp))
+    keep &= ~((rows == 7) | (s.col == 7))
+    rows, col, val = rows[keep], s.col[keep], s.val[keep]
+    diag = s.diag.copy()
+    diag[7] = 1e-12
+    rhs = s.rhs.copy()
+    rhs[7] = 0.0
+    new_rp = np.zeros(s.n + 1, np.uint64)
+    new_rp[1:] = np.cumsum(np.bincount(rows, minlength=s.n))
+    s2 = ol.System(new_rp, col, val, diag, rhs)
+    o = ol.Oracle(line_design([1.0, 2.0]))
+    ctx = gpu_ctx_factory()
+    set_both(ctx, o, s2, s2)
+    x0 = np.zeros(s.n)
+    x, y = x0.copy(), x0.copy()
+    r = ctx.solve_both(x, y, 1e-10, 10000)
+    ref = o.solve(x0, x0, tol=1e-10, max_iter=10000)
+    assert rel(x, ref["x"]) < 1e-8 and x[7] == 0.0
+    ctx.close()

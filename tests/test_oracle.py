@@ -1,0 +1,507 @@
+"""Pins the CPU oracle against known answers (CPU only).
+
+The reference holds no numeric golden vectors for this path, so each oracle
+function is checked against something independent of its own code: the
+worked examples of docs/foundations/05-net-models.md, a brute-force dense
+assembly of the B2B quadratic, closed-form counts, scipy direct solves, a
+textbook PCG written here in numpy, and invariants.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from ktplace_amd import design as dz
+
+
+def line_design(xs, fixed=None, offsets=None, die=(0.0, 0.0, 100.0, 10.0)):
+    """One net over cells at x positions xs (height 1, width 1), all on one row."""
+    fixed = fixed or [False] * len(xs)
+    offsets = offsets or [0.0] * len(xs)
+    cells = [(x if f else 0.0, 0.0, 1.0, 1.0, f) for x, f in zip(xs, fixed)]
+    net = [(i, offsets[i], 0.0) for i in range(len(xs))]
+    return dz.make_raw(cells, [net], die=die, rows=[(0.0, 1.0, 0.0, 100.0)])
+
+
+def offdiag_pairs(s: ol.System):
+    rows = np.repeat(np.arange(s.n), np.diff(s.row_ptr.astype(np.int64)))
+    return rows, s.col.astype(np.int64), -s.val
+
+
+# ---------------------------------------------------------------- B2B: worked examples
+@pytest.mark.parametrize("xs,edges,total_len,cost", [
+    ([0.0, 2.0, 5.0, 10.0], 5, 30.0, 10.0),  # 05-net-models.md:70-73
+    ([1.0, 4.0, 9.0], 3, 16.0, 8.0),          # 05-net-models.md:113-118
+])
+def test_b2b_docs_examples(xs, edges, total_len, cost):
+    raw = line_design(xs)
+    o = ol.Oracle(raw)
+    x = np.array(xs)
+    y = np.zeros(len(xs))
+    o.build_b2b(x, y, floors=(1e-3, 1e-3))
+    s = o.system(0)
+    i, j, w = offdiag_pairs(s)
+    uniq = {(min(a, b), max(a, b)): ww for a, b, ww in zip(i, j, w)}
+    assert len(uniq) == edges
+    lengths = {k: abs(xs[k[0]] - xs[k[1]]) for k in uniq}
+    assert sum(lengths.values()) == pytest.approx(total_len)
+    # each edge has weight 1/((p-1) l), so sum of w l^2 = HPWL_x
+    assert sum(uniq[k] * lengths[k] ** 2 for k in uniq) == pytest.approx(cost, rel=1e-12)
+    # every unique edge appears in both endpoint rows with the same weight
+    assert len(i) == 2 * edges
+    assert np.allclose(s.diag, np.bincount(i, weights=w, minlength=len(xs)), rtol=1e-15)
+
+
+# ---------------------------------------------------------------- B2B: brute force
+def brute_b2b(raw, x, y, floors, alpha=0.0, ax=None, ay=None):
+    """Dense B2B system from the edge definition (not the row walk): for each
+    net, L/R = first argmin/argmax pin; edges L-R and inner-L, inner-R; weight
+    1/((p-1) max(l, floor)); Hessian and gradient of sum w (p_i - p_j)^2 / 2."""
+    flat = dz.collect(raw).flat
+    n = flat.num_movable
+    var = flat.var_of_vertex.astype(np.int64)
+    out = []
+    for axis, pos, fl, anchor in ((0, x, floors[0], ax), (1, y, floors[1], ay)):
+        A = np.zeros((n, n))
+        b = np.zeros(n)
+        px = (flat.fixed_x if axis == 0 else flat.fixed_y).copy()
+        px[flat.mov_vertex] = pos
+        off = flat.pin_off_x if axis == 0 else flat.pin_off_y
+        for k in range(flat.num_nets):
+            s, e = int(flat.net_start[k]), int(flat.net_start[k + 1])
+            p = e - s
+            if p < 2:
+                continue
+            cells = flat.pin_cell[s:e]
+            o = off[s:e]
+            pins = px[cells] + o
+            lo = int(np.argmin(pins))
+            hi = int(np.argmax(pins))
+            if lo == hi:
+                continue
+            edges = {(min(lo, hi), max(lo, hi))}
+            for q in range(p):
+                if q not in (lo, hi):
+                    edges.add((min(q, lo), max(q, lo)))
+                    edges.add((min(q, hi), max(q, hi)))
+            for a, c in edges:
+                w = flat.net_weight[k] / (p - 1) / max(abs(pins[a] - pins[c]), fl)
+                va, vc = var[cells[a]], var[cells[c]]
+                for (vi, oi, vj, oj, cj) in ((va, o[a], vc, o[c], cells[c]),
+                                             (vc, o[c], va, o[a], cells[a])):
+                    if vi == dz._lib.NO_VAR:
+                        continue
+                    A[vi, vi] += w
+                    if vj != dz._lib.NO_VAR:
+                        A[vi, vj] -= w
+                        b[vi] += w * (oj - oi)
+                    else:
+                        b[vi] += w * (px[cj] + oj - oi)
+        if alpha > 0:
+            wa = alpha / np.maximum(np.abs(pos - anchor), fl)
+            A[np.arange(n), np.arange(n)] += wa
+            b += wa * anchor
+        out.append((A, b))
+    return out
+
+
+def random_raw(rng, n_mov=12, n_fixed=3, n_nets=10, max_p=6):
+    cells = []
+    for v in range(n_mov + n_fixed):
+        if v < n_fixed:
+            cells.append((rng.uniform(0, 50), rng.uniform(0, 20), 0.0, 0.0, True))
+        else:
+            cells.append((0.0, 0.0, float(rng.integers(1, 5)), 1.0, False))
+    order = rng.permutation(len(cells))
+    cells = [cells[k] for k in order]
+    nets = []
+    for _ in range(n_nets):
+        p = int(rng.integers(2, max_p + 1))
+        members = rng.integers(0, len(cells), size=p)  # repeats allowed: dedup exercised
+        nets.append([(int(v), rng.uniform(0, cells[v][2] or 1), rng.uniform(0, 1)) for v in members])
+    return dz.make_raw(cells, nets, die=(0, 0, 50, 20), rows=[(float(r), 1.0, 0.0, 50.0) for r in range(20)])
+
+
+@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("alpha", [0.0, 0.05])
+def test_b2b_matches_bruteforce_dense(seed, alpha):
+    rng = np.random.default_rng(seed)
+    raw = random_raw(rng)
+    o = ol.Oracle(raw)
+    x = rng.uniform(0, 50, o.n)
+    y = rng.uniform(0, 20, o.n)
+    ax = rng.uniform(0, 50, o.n) if alpha else None
+    ay = rng.uniform(0, 20, o.n) if alpha else None
+    floors = (0.7, 0.3)
+    o.build_b2b(x, y, alpha, ax, ay, floors=floors)
+    for axis, (A, b) in enumerate(brute_b2b(raw, x, y, floors, alpha, ax, ay)):
+        s = o.system(axis)
+        dense = s.dense()
+        diag = np.maximum(np.diag(A), 1e-12)
+        A2 = A.copy()
+        A2[np.arange(len(diag)), np.arange(len(diag))] = diag
+        assert np.allclose(dense, A2, rtol=1e-12, atol=1e-14)
+        assert np.allclose(s.rhs, b, rtol=1e-9, atol=1e-9 * max(1.0, np.abs(b).max()))
+
+
+def test_b2b_mutation_is_caught():
+    """A plausible mistake (inner pins joined only to lo) fails the brute force."""
+    rng = np.random.default_rng(3)
+    raw = random_raw(rng, n_fixed=0, max_p=5)
+    o = ol.Oracle(raw)
+    x = rng.uniform(0, 50, o.n)
+    o.build_b2b(x, x, floors=(0.5, 0.5))
+    A, _ = brute_b2b(raw, x, x, (0.5, 0.5))[0]
+    s = o.system(0)
+    i, j, w = offdiag_pairs(s)
+    # drop every second entry of rows with more than two entries: no longer matches
+    keep = np.ones(len(i), bool)
+    rp = s.row_ptr.astype(np.int64)
+    for r in range(s.n):
+        if rp[r + 1] - rp[r] > 2:
+            keep[rp[r] + 1] = False
+    mutated = np.zeros_like(A)
+    np.add.at(mutated, (i[keep], j[keep]), -w[keep])
+    off = A - np.diag(np.diag(A))
+    assert not np.allclose(mutated, off)
+
+
+# ---------------------------------------------------------------- B2B: closed forms on generated designs
+@pytest.fixture(scope="module")
+def small():
+    d = dz.generate(2000, 2300, 40, seed=7)
+    return d, ol.Oracle.of(d)
+
+
+def numpy_extremes(flat, pos, axis):
+    off = flat.pin_off_x if axis == 0 else flat.pin_off_y
+    full = (flat.fixed_x if axis == 0 else flat.fixed_y).copy()
+    full[flat.mov_vertex] = pos
+    pins = full[flat.pin_cell] + off
+    starts = flat.net_start.astype(np.int64)
+    lo = np.zeros(flat.num_nets, np.int64)
+    hi = np.zeros(flat.num_nets, np.int64)
+    for k in range(flat.num_nets):
+        seg = pins[starts[k]:starts[k + 1]]
+        if len(seg):
+            lo[k] = int(np.argmin(seg))  # first index of the minimum
+            hi[k] = int(np.argmax(seg))
+    return lo, hi
+
+
+def test_b2b_row_counts_closed_form(small):
+    d, o = small
+    f = d.flat
+    rng = np.random.default_rng(1)
+    x = rng.uniform(0, d.die[2], d.n)
+    y = rng.uniform(0, d.die[3], d.n)
+    o.build_b2b(x, y, floors=f.floors)
+    movable = f.var_of_vertex[f.pin_cell] != dz._lib.NO_VAR
+    starts = f.net_start.astype(np.int64)
+    net_mov = np.add.reduceat(movable.astype(np.int64), starts[:-1]) if len(movable) else None
+    net_mov[np.diff(starts) == 0] = 0
+    for axis, pos in ((0, x), (1, y)):
+        lo, hi = numpy_extremes(f, pos, axis)
+        counts = np.zeros(d.n, np.int64)
+        for e in range(len(f.cell_pin_net)):
+            k = f.cell_pin_net[e]
+            q = f.cell_pin_idx[e]
+            row = np.searchsorted(f.pin_start.astype(np.int64), e, side="right") - 1
+            if lo[k] == hi[k]:
+                continue
+            if q in (lo[k], hi[k]):
+                counts[row] += net_mov[k] - 1
+            else:
+                counts[row] += int(movable[starts[k] + lo[k]]) + int(movable[starts[k] + hi[k]])
+        s = o.system(axis)
+        assert np.array_equal(np.diff(s.row_ptr.astype(np.int64)), counts)
+
+
+def test_b2b_symmetry_and_diagonal(small):
+    d, o = small
+    f = d.flat
+    rng = np.random.default_rng(2)
+    x = rng.uniform(0, d.die[2], d.n)
+    y = rng.uniform(0, d.die[3], d.n)
+    o.build_b2b(x, y, floors=f.floors)
+    for axis in (0, 1):
+        s = o.system(axis)
+        i, j, w = offdiag_pairs(s)
+        fwd = sorted(zip(i.tolist(), j.tolist(), w.tolist()))
+        bwd = sorted(zip(j.tolist(), i.tolist(), w.tolist()))
+        assert fwd == bwd
+        assert (w > 0).all()
+        # diag >= sum of off-diagonal weights (fixed neighbours add more)
+        rowsum = np.bincount(i, weights=w, minlength=s.n)
+        assert (s.diag >= rowsum * (1 - 1e-12)).all()
+
+
+def test_b2b_hpwl_identity():
+    """Sum over unique edges of w l^2 equals sum_nets weight * HPWL_axis when no
+    edge is shorter than the floor (05-net-models.md:50-61)."""
+    rng = np.random.default_rng(5)
+    cells = [(0.0, 0.0, 1.0, 1.0, False) for _ in range(30)]
+    nets = []
+    for _ in range(12):
+        members = rng.choice(30, size=int(rng.integers(2, 7)), replace=False)
+        nets.append([(int(v), 0.0, 0.0) for v in members])
+    raw = dz.make_raw(cells, nets, die=(0, 0, 1000, 1000), rows=[(0.0, 1.0, 0.0, 1000.0)])
+    o = ol.Oracle(raw)
+    x = rng.permutation(30).astype(float) * 7.0 + 1.0  # distinct, spacing >= 7
+    y = rng.permutation(30).astype(float) * 5.0 + 1.0
+    o.build_b2b(x, y, floors=(1e-6, 1e-6))
+    for axis, pos in ((0, x), (1, y)):
+        s = o.system(axis)
+        i, j, w = offdiag_pairs(s)
+        total = np.sum(w * (pos[i] - pos[j]) ** 2) / 2.0
+        hp = sum(pos[[m[0] for m in net]].max() - pos[[m[0] for m in net]].min() for net in nets)
+        assert total == pytest.approx(hp, rel=1e-12)
+
+
+# ---------------------------------------------------------------- netlist flattening
+def test_collect_dedup_bruteforce():
+    rng = np.random.default_rng(11)
+    raw = random_raw(rng, n_mov=20, n_fixed=4, n_nets=15, max_p=8)
+    o = ol.Oracle(raw).flat()
+    prod = dz.collect(raw).flat
+    # brute force: per net, pins stable-sorted by vertex, first pin per vertex kept
+    starts = raw.net_start.astype(np.int64)
+    cells, offx = [], []
+    for k in range(len(raw.net_weight)):
+        seen = {}
+        for p in range(starts[k], starts[k + 1]):
+            v = int(raw.pin_vertex[p])
+            if v not in seen:
+                seen[v] = raw.pin_off_x[p]
+        for v in sorted(seen):
+            cells.append(v)
+            offx.append(seen[v])
+    assert np.array_equal(o["pin_cell"], np.array(cells, np.uint32))
+    assert np.array_equal(o["pin_off_x"], np.array(offx))
+    for key in ("net_start", "pin_cell", "pin_off_x", "pin_off_y", "var_of_vertex", "mov_vertex",
+                "pin_start", "cell_pin_net", "cell_pin_idx"):
+        assert np.array_equal(o[key], getattr(prod, key)), key
+
+
+def test_collect_product_matches_oracle_on_generated(small):
+    d, o = small
+    f = o.flat()
+    for key in ("net_start", "pin_cell", "pin_off_x", "pin_off_y", "net_weight", "var_of_vertex",
+                "mov_vertex", "pin_start", "cell_pin_net", "cell_pin_idx"):
+        assert np.array_equal(f[key], getattr(d.flat, key)), key
+    assert tuple(f["floors"]) == d.flat.floors
+
+
+def test_generator_is_deterministic_and_connected():
+    a = dz.generate(3000, 3300, 60, seed=42)
+    b = dz.generate(3000, 3300, 60, seed=42)
+    c = dz.generate(3000, 3300, 60, seed=43)
+    assert np.array_equal(a.raw.pin_vertex, b.raw.pin_vertex)
+    assert np.array_equal(a.raw.pin_off_x, b.raw.pin_off_x)
+    assert not np.array_equal(a.raw.pin_vertex, c.raw.pin_vertex)
+    assert (np.diff(a.flat.pin_start.astype(np.int64)) >= 1).all()  # every cell on a net
+    deg = np.diff(a.raw.net_start.astype(np.int64))
+    assert deg.min() >= 2 and deg.max() <= 41 + 8  # appended isolated cells may add a few pins
+    assert a.flat.num_vertices == 3060 and a.n == 3000
+
+
+# ---------------------------------------------------------------- dot, multiply
+def test_dot_is_chunked_and_thread_independent():
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal(20000)
+    b = rng.standard_normal(20000)
+    partial = []
+    for c in range(0, len(a), 4096):
+        s = 0.0
+        for i in range(c, min(len(a), c + 4096)):
+            s += a[i] * b[i]
+        partial.append(s)
+    expect = 0.0
+    for s in partial:
+        expect += s
+    ol.set_threads(1)
+    one = ol.dot(a, b)
+    ol.set_threads(8)
+    eight = ol.dot(a, b)
+    assert one == expect and eight == expect
+
+
+def test_multiply_matches_dense(small):
+    d, o = small
+    rng = np.random.default_rng(3)
+    o.build_b2b(rng.uniform(0, 100, d.n), rng.uniform(0, 100, d.n), floors=d.flat.floors)
+    s = o.system(0)
+    v = rng.standard_normal(d.n)
+    assert np.allclose(o.multiply(0, v), s.scipy() @ v, rtol=1e-12, atol=1e-12)
+
+
+# ---------------------------------------------------------------- CG
+def spd_system(rng, n=300, extra=900):
+    """Random diagonally dominant Laplacian plus a small grounding: SPD."""
+    i = rng.integers(0, n, extra)
+    j = rng.integers(0, n, extra)
+    keep = i != j
+    i, j = i[keep], j[keep]
+    w = rng.uniform(0.1, 2.0, len(i))
+    rows = np.concatenate([i, j])
+    cols = np.concatenate([j, i])
+    vals = -np.concatenate([w, w])
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    diag = np.bincount(rows, weights=-vals, minlength=n) + rng.uniform(0.01, 0.2, n)
+    rp = np.zeros(n + 1, np.uint64)
+    rp[1:] = np.cumsum(np.bincount(rows, minlength=n))
+    return ol.System(rp, cols.astype(np.uint32), vals, diag, rng.standard_normal(n) * 10)
+
+
+def textbook_pcg(A, b, x, iters):
+    d = A.diagonal()
+    r = b - A @ x
+    z = r / d
+    p = z.copy()
+    rho = r @ z
+    bn = np.linalg.norm(b)
+    hist = []
+    for _ in range(iters):
+        ap = A @ p
+        a = rho / (p @ ap)
+        x = x + a * p
+        r = r - a * ap
+        hist.append(np.linalg.norm(r) / bn)
+        z = r / d
+        rn = r @ z
+        p = z + (rn / rho) * p
+        rho = rn
+    return x, np.array(hist)
+
+
+def test_cg_matches_scipy_direct_and_textbook():
+    from scipy.sparse.linalg import spsolve
+    rng = np.random.default_rng(4)
+    sx, sy = spd_system(rng), spd_system(rng)
+    raw = line_design([1.0, 2.0])
+    o = ol.Oracle(raw)
+    o.set_system(0, sx)
+    o.set_system(1, sy)
+    x0 = rng.standard_normal(sx.n)
+    y0 = rng.standard_normal(sy.n)
+    res = o.solve(x0, y0, tol=1e-13, max_iter=10000, history=True)
+    for sol, s in ((res["x"], sx), (res["y"], sy)):
+        exact = spsolve(s.scipy().tocsc(), s.rhs)
+        assert np.linalg.norm(sol - exact) / np.linalg.norm(exact) < 1e-10
+    _, hist = textbook_pcg(sx.scipy(), sx.rhs, x0, 15)
+    assert np.allclose(res["history"][:15, 0], hist, rtol=1e-8)
+
+
+def test_cg_lockstep_and_rules():
+    rng = np.random.default_rng(5)
+    sx = spd_system(rng)
+    raw = line_design([1.0, 2.0])
+    o = ol.Oracle(raw)
+    zero = ol.System(sx.row_ptr, sx.col, sx.val, sx.diag, np.zeros(sx.n))
+    o.set_system(0, sx)
+    o.set_system(1, zero)
+    x0 = np.zeros(sx.n)
+    res = o.solve(x0, x0, tol=1e-8, max_iter=5000, history=True)
+    assert res["resid"][1] == 0.0  # ||b|| = 0: axis inactive, residual stays 0
+    assert (res["history"][:, 1] == 0.0).all()
+    assert res["resid"][0] <= 1e-8
+    # maxIterations = 0: nothing moves
+    r0 = o.solve(x0, x0, tol=1e-8, max_iter=0)
+    assert r0["iterations"] == 0 and np.array_equal(r0["x"], x0)
+    # indefinite (negated) system: p.Ap <= 0 stops the axis on the first step
+    neg = ol.System(sx.row_ptr, sx.col, -sx.val, -sx.diag, sx.rhs)
+    o.set_system(1, neg)
+    rn = o.solve(x0, x0, tol=1e-8, max_iter=50, history=True)
+    assert rn["iterations"] > 1
+    assert np.all(rn["history"][:, 1] == rn["history"][0, 1])
+
+
+# ---------------------------------------------------------------- HPWL, grid, occupancy
+def test_hpwl_closed_form(small):
+    d, o = small
+    f = d.flat
+    rng = np.random.default_rng(6)
+    x = rng.uniform(0, d.die[2], d.n)
+    y = rng.uniform(0, d.die[3], d.n)
+    fx = f.fixed_x.copy()
+    fy = f.fixed_y.copy()
+    fx[f.mov_vertex] = x
+    fy[f.mov_vertex] = y
+    px = fx[f.pin_cell] + f.pin_off_x
+    py = fy[f.pin_cell] + f.pin_off_y
+    st = f.net_start.astype(np.int64)
+    deg = np.diff(st)
+    nz = deg > 0
+    hx = np.maximum.reduceat(px, st[:-1][nz]) - np.minimum.reduceat(px, st[:-1][nz])
+    hy = np.maximum.reduceat(py, st[:-1][nz]) - np.minimum.reduceat(py, st[:-1][nz])
+    expect = np.sum((hx + hy)[deg[nz] >= 2])
+    assert o.hpwl(x, y) == pytest.approx(expect, rel=1e-12)
+
+
+def test_grid_occupancy_overflow(small):
+    d, o = small
+    bins = o.build_grid(0, 1.0)
+    assert bins == 16  # sqrt(2000/51) < 16 -> clamped
+    avail, total = o.grid_avail()
+    W, H = d.die[2], d.die[3]
+    assert total == pytest.approx(W * H, rel=1e-12)  # rows cover the die, no obstacles
+    rng = np.random.default_rng(7)
+    x = rng.uniform(0, W - 8, d.n)
+    y = rng.uniform(0, H - 1, d.n)
+    occ = o.occupancy(x, y)
+    assert occ.sum() == pytest.approx(d.flat.area.sum(), rel=1e-12)
+    cx = x + 0.5 * d.flat.width
+    cy = y + 0.5 * d.flat.height
+    ix = np.clip((cx / (W / bins)), 0, bins - 1e-9).astype(int)
+    iy = np.clip((cy / (H / bins)), 0, bins - 1e-9).astype(int)
+    expect = np.bincount(iy * bins + ix, weights=d.flat.area, minlength=bins * bins)
+    assert np.allclose(occ, expect, rtol=1e-12)
+    ovf = np.maximum(occ - avail, 0).sum() / total
+    assert o.overflow(occ) == pytest.approx(ovf, rel=1e-12)
+
+
+def test_free_space_profile_hand_computed():
+    # die 20 x 4, rows y=0..3 spanning x 0..20, one macro [5,10] x [1,3]
+    cells = [(0.0, 0.0, 1.0, 1.0, False), (5.0, 1.0, 5.0, 2.0, True)]
+    raw = dz.make_raw(cells, [[(0, 0.0, 0.0), (1, 0.0, 0.0)]], die=(0, 0, 20, 4),
+                      rows=[(float(r), 1.0, 0.0, 20.0) for r in range(4)])
+    o = ol.Oracle(raw)
+    o.build_grid(4, 1.0)
+    _, total = o.grid_avail()
+    assert total == pytest.approx(80 - 10)
+    e, p = o.profile((0, 0, 20, 4), vertical=True)
+    # free height: 4 on [0,5), 2 on [5,10), 4 on [10,20)
+    assert list(e) == [0, 5, 10, 20]
+    assert list(p) == [0, 20, 30, 70]
+    e, p = o.profile((0, 0, 20, 4), vertical=False)
+    assert list(e) == [0, 1, 2, 3, 4]
+    assert list(p) == [0, 20, 35, 50, 70]
+
+
+# ---------------------------------------------------------------- LAL and the loop (invariants)
+def test_lal_invariants():
+    d = dz.generate(3000, 3300, 60, seed=9)
+    o = ol.Oracle.of(d)
+    o.build_grid(0, 1.0)
+    rng = np.random.default_rng(8)
+    W, H = d.die[2], d.die[3]
+    x = rng.normal(W / 2, W / 10, d.n)  # clumped
+    y = rng.normal(H / 2, H / 10, d.n)
+    before = o.overflow(o.occupancy(x, y))
+    ux, uy = o.lal(x, y)
+    ux2, uy2 = o.lal(x, y)
+    assert np.array_equal(ux, ux2) and np.array_equal(uy, uy2)
+    assert (ux >= 0).all() and (ux + d.flat.width <= W + 1e-9).all()
+    assert (uy >= 0).all() and (uy + d.flat.height <= H + 1e-9).all()
+    after = o.overflow(o.occupancy(ux, uy))
+    assert after < 0.5 * before
+
+
+def test_place_small_design_runs():
+    d = dz.generate(1500, 1650, 40, seed=3)
+    o = ol.Oracle.of(d)
+    x, y, st, trace = o.place(max_iters=20, refine_iters=2)
+    assert st["hpwl_lower"] <= st["hpwl_final"] * (1 + 1e-12)
+    assert st["overflow_final"] < 0.5
+    assert st["hpwl_final"] == pytest.approx(o.hpwl(x, y), rel=1e-12)
+    assert len(trace) == st["global_iters"]
