@@ -69,10 +69,16 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cells1m")
-    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--paths", default="fp64,mixed",
+                    help="comma list of solver paths to time: fp64 (plain), mixed")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run the plain copy kernel on this many MiB (counter calibration)")
+    ap.add_argument("--lanes", type=int, default=0, help="skip the autotuner: lanes per row")
+    ap.add_argument("--tile", type=int, default=0, help="skip the autotuner: rows per block")
+    ap.add_argument("--xcd-split", type=int, default=1, help="with --lanes/--tile: 1 axis per XCD half")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -88,7 +94,6 @@ def main() -> int:
     log(f"[bench] generated {args.workload}: n={d.n} nets={d.flat.num_nets} pins={d.flat.num_pins} "
         f"({time.perf_counter() - t:.1f} s)")
     ctx = simpl.Context(0).upload(d)
-    prec = simpl.KT_PREC_FP64 if args.precision == "fp64" else simpl.KT_PREC_MIXED
 
     # start point: seedUniform + one alpha = 0 B2B round (untimed)
     x0, y0 = dz.seed_uniform(d, 20240607)
@@ -98,42 +103,71 @@ def main() -> int:
     log(f"[bench] start point: round-0 solve {r0['iterations']} iterations")
     ctx.set_positions(x1, y1)
     ctx.save_positions()
+    ctx.build_b2b_resident(0.0)
+    nnz = [ctx.system_size(a)[1] for a in (0, 1)]
 
-    def step(profile: bool) -> dict:
+    from ktplace_amd.autotune import TuneResult, autotune
+    paths = [p for p in args.paths.split(",") if p]
+    per = {}
+    for path in paths:
+        prec = simpl.KT_PREC_FP64 if path == "fp64" else simpl.KT_PREC_MIXED
         ctx.restore_positions()
         ctx.build_b2b_resident(0.0)
-        return ctx.solve_resident(args.tol, 1_000_000, precision=prec,
-                                  stop_rule=simpl.KT_STOP_TRUE, profile=profile)
+        if args.lanes and args.tile:
+            ctx.set_tuning(lanes=args.lanes, tile=args.tile, xcd_split=args.xcd_split)
+            tuned = TuneResult(args.lanes, args.tile, float("nan"), [], args.xcd_split)
+        else:
+            tuned = autotune(ctx, prec)
+        log(f"[bench] {path}: autotune lanes={tuned.lanes} tile={tuned.tile} xcd_split={tuned.xcd_split} "
+            f"rows={tuned.rows} "
+            f"({tuned.ms_per_iteration:.3f} ms/iteration)")
 
-    for w in range(args.warmup):
-        r = step(False)
-        log(f"[bench] warmup {w}: {r['iterations']} iterations, true resid {r['true_resid']}")
-    ctx.sync()
-    ctx.reset_kernel_times()
-    nnz = [ctx.system_size(a)[1] for a in (0, 1)]
-    results = []
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        results.append(step(True))
-    ctx.sync()
-    elapsed = time.perf_counter() - t0
-    ms = 1e3 * elapsed / max(args.steps, 1)
-    for i, r in enumerate(results):
-        log(f"[bench] step {i}: {r['iterations']} iterations, true resid {r['true_resid']}")
+        def step(profile: bool, prec=prec) -> dict:
+            ctx.restore_positions()
+            ctx.build_b2b_resident(0.0)
+            return ctx.solve_resident(args.tol, 1_000_000, precision=prec,
+                                      stop_rule=simpl.KT_STOP_TRUE, profile=profile)
 
-    kid = simpl.KERNEL_SPMV if args.precision == "fp64" else simpl.KERNEL_SPMV32
-    launches, spmv_ms = ctx.kernel_time(kid)
-    total_bytes = sum(r["steps"][a] * spmv_bytes(nnz[a], d.n, args.precision)
-                      for r in results for a in (0, 1))
-    avg_ms = spmv_ms / max(launches, 1)
-    bytes_per_launch = total_bytes / max(launches, 1)
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="cgSpmv (both axes per launch)", launches=launches,
-                    avg_launch_us=round(avg_ms * 1e3, 2),
-                    bytes_per_launch=int(bytes_per_launch))
+        for w in range(args.warmup):
+            r = step(False)
+            log(f"[bench] {path} warmup {w}: {r['iterations']} iterations, true resid {r['true_resid']}")
+        # timed region: graph-replayed iterations
+        ctx.sync()
+        t0 = time.perf_counter()
+        results = [step(False) for _ in range(args.steps)]
+        ctx.sync()
+        ms = 1e3 * (time.perf_counter() - t0) / max(args.steps, 1)
+        # the same steps launched one kernel at a time with HIP events around
+        # each kernel (per-kernel durations for the roofline)
+        ctx.reset_kernel_times()
+        ctx.sync()
+        t0 = time.perf_counter()
+        presults = [step(True) for _ in range(args.steps)]
+        ctx.sync()
+        ms_prof = 1e3 * (time.perf_counter() - t0) / max(args.steps, 1)
+        kid = simpl.KERNEL_SPMV if path == "fp64" else simpl.KERNEL_SPMV32
+        launches, spmv_ms = ctx.kernel_time(kid)
+        total_bytes = sum(r["steps"][a] * spmv_bytes(nnz[a], d.n, path)
+                          for r in presults for a in (0, 1))
+        avg_ms = spmv_ms / max(launches, 1)
+        bytes_per_launch = total_bytes / max(launches, 1)
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+        last = results[-1]
+        for i, r in enumerate(results):
+            log(f"[bench] {path} step {i}: {r['iterations']} iterations, true resid {r['true_resid']}")
+        log(f"[bench] {path}: {ms:.2f} ms/step (graph replay), {ms_prof:.2f} ms/step with events; "
+            f"SpMV {avg_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s algorithmic")
+        per[path] = dict(
+            ms_per_step=round(ms, 3), ms_per_step_with_kernel_events=round(ms_prof, 3),
+            iterations=last["iterations"], true_resid=list(last["true_resid"]),
+            refreshes_or_restarts=last["refinements"],
+            tuning={"lanes_per_row": tuned.lanes, "tile_rows": tuned.tile,
+                    "xcd_split": tuned.xcd_split, "rows_in_flight": tuned.rows},
+            roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                          frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                          kernel=f"cgSpmv<{'double' if path == 'fp64' else 'float'}> (x and y per launch)",
+                          launches=launches, avg_launch_us=round(avg_ms * 1e3, 2),
+                          bytes_per_launch=int(bytes_per_launch)))
 
     base = None
     if not args.no_cpu_baseline:
@@ -141,32 +175,41 @@ def main() -> int:
         base = cpu_baseline(d, x1, y1, args.tol)
         log(f"[bench] cpu baseline {base['value']:.0f} ms on {base['cores']} threads")
 
-    last = results[-1]
+    copy = None
+    if args.copy_probe:
+        nb = args.copy_probe << 20
+        cms = ctx.probe_copy(nb, 1)
+        copy = dict(bytes=nb, ms=round(cms, 4), read_write_gbs=round(2 * nb / (cms * 1e-3) / 1e9, 1))
+        log(f"[bench] copy probe {args.copy_probe} MiB: {cms:.3f} ms")
+
+    main_path = "mixed" if "mixed" in per else paths[0]
+    ms = per[main_path]["ms_per_step"]
     out = {
         "metric": METRIC,
-        "value": round(ms, 3),
+        "value": ms,
         "unit": "ms",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 3),
+        "ms_per_step": ms,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f64" if main_path == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
             "workload": (f"{args.workload}: one B2B assembly + lock-step Jacobi-PCG (x and y) to "
                          f"{args.tol:g} true relative residual; {d.n} cells, {d.flat.num_nets} "
                          f"nets, {d.flat.num_pins} pins, seed {args.seed}"),
-            "path": args.precision,
+            "path": main_path,
             "warm_start": "previous iterate",
-            "iterations": last["iterations"],
-            "true_resid": list(last["true_resid"]),
-            "restarts": last["refinements"],
+            "paths": per,
+            "speedup_vs_plain_fp64": (round(per["fp64"]["ms_per_step"] / ms, 3)
+                                      if "fp64" in per else None),
             "nnz_per_axis": nnz,
+            "copy_probe": copy,
         },
-        "roofline": roofline,
+        "roofline": per[main_path]["roofline"],
         "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)

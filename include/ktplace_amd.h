@@ -136,6 +136,19 @@ int kt_solve_both(kt_ctx *ctx, double *x, double *y, uint64_t count, const kt_so
                   kt_observer observe, void *user, kt_solve_result *out);
 int kt_solve_both_resident(kt_ctx *ctx, const kt_solve_opts *opts, kt_solve_result *out);
 
+/* ---- launch-shape tuning (0 keeps the default) --------------------------- */
+/* lanes: threads per CSR row in the SpMV (1, 2, 4, 8 or 16); tile: rows per
+ * block (a multiple of 1024); use_graphs: replay iteration pairs from a
+ * captured hipGraph; refresh_factor: mixed precision refreshes the FP64
+ * residual once the FP32 residual has fallen by this factor (0 < f < 1). */
+int kt_set_tuning(kt_ctx *ctx, int lanes, uint32_t tile, int use_graphs, double refresh_factor);
+/* As kt_set_tuning, plus xcd_split: 1 puts the x system on XCDs 0-3 and the y
+ * system on XCDs 4-7 (each XCD's L2 then holds one axis' vectors), 0 deals
+ * blocks of both axes to all XCDs, -1 keeps the default (1); rows_in_flight:
+ * CSR rows each lane group keeps in flight in the SpMV (2, 4 or 8; 0 = 2). */
+int kt_set_tuning2(kt_ctx *ctx, int lanes, uint32_t tile, int use_graphs, double refresh_factor,
+                   int xcd_split, int rows_in_flight);
+
 /* ---- per-kernel timings (HIP events on the context's stream) -------------- */
 #define KT_KERNEL_SPMV 0      /* FP64 CG SpMV + p.Ap */
 #define KT_KERNEL_UPDATE 1    /* FP64 x/r update + r.r, r.z */
@@ -145,6 +158,12 @@ int kt_solve_both_resident(kt_ctx *ctx, const kt_solve_opts *opts, kt_solve_resu
 #define KT_KERNEL_COUNT 8
 int kt_kernel_times(kt_ctx *ctx, int kernel, uint64_t *launches, double *total_ms);
 int kt_reset_kernel_times(kt_ctx *ctx);
+
+/* Calibration helper (no counterpart in the reference): copies `bytes` bytes
+ * between two scratch device buffers `reps` times with a plain streaming copy
+ * kernel (kt_copy_probe) and returns the average milliseconds per copy.  Used
+ * to scale hardware byte counters (FETCH_SIZE / WRITE_SIZE) to real bytes. */
+int kt_probe_copy(kt_ctx *ctx, uint64_t bytes, int reps, double *ms_per_copy);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,76 @@
+"""Launch-shape autotuner for the two-axis Jacobi-PCG kernels.
+
+Chooses the SpMV lanes per row (threads cooperating on one CSR row: 1, 2, 4,
+8 or 16) and the tile (rows per 1024-thread block) by timing a fixed number
+of CG iterations of the context's current systems on the GPU, from the same
+saved start each time.  The search costs a few hundred iterations, i.e. a
+fraction of one solve; results are cached per (rows, nnz, precision)
+signature for the life of the process.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+
+from . import simpl
+
+LANES = (4, 8, 16)
+TILES = (2048, 4096)
+ROWS = (2, 4, 8)
+XCD = (1, 0)
+
+_cache: dict = {}
+
+
+@dataclass
+class TuneResult:
+    lanes: int
+    tile: int
+    ms_per_iteration: float
+    table: list
+    xcd_split: int = 1
+    rows: int = 2
+
+
+def _signature(ctx: "simpl.Context", precision: int):
+    rx, nx = ctx.system_size(0)
+    ry, ny = ctx.system_size(1)
+    return (rx, nx, ry, ny, precision)
+
+
+def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iterations: int = 48,
+             lanes=LANES, tiles=TILES, use_graphs: bool = True, force: bool = False) -> TuneResult:
+    """Time `iterations` CG iterations per candidate and apply the fastest.
+
+    The context's positions are saved before and restored after the search,
+    so the caller's state is unchanged.
+    """
+    key = _signature(ctx, precision)
+    if not force and key in _cache:
+        best = _cache[key]
+        ctx.set_tuning(lanes=best.lanes, tile=best.tile, use_graphs=use_graphs,
+                       xcd_split=best.xcd_split, rows=best.rows)
+        return best
+    rows = key[0]
+    ctx.save_positions()
+    table = []
+    candidates = [(g, t, x, rr) for t in tiles for g in lanes for x in XCD for rr in ROWS
+                  if not (t > 4 * max(rows, 1024) and t != tiles[0])]
+    for g, tile, xs, rr in candidates:
+        ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr)
+        ctx.restore_positions()
+        ctx.solve_resident(0.0, 2, precision=precision)  # graph capture, first touch
+        ctx.restore_positions()
+        ctx.sync()
+        t0 = time.perf_counter()
+        r = ctx.solve_resident(0.0, iterations, precision=precision, check_every=iterations)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        its = max(r["iterations"], 1)
+        table.append((g, tile, xs, rr, 1e3 * dt / its))
+    ctx.restore_positions()
+    g, tile, xs, rr, ms = min(table, key=lambda t: t[4])
+    best = TuneResult(lanes=g, tile=tile, ms_per_iteration=ms, table=table, xcd_split=xs, rows=rr)
+    _cache[key] = best
+    ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr)
+    return best

@@ -1,6 +1,7 @@
 // C ABI entry points (include/ktplace_amd.h): context, netlist upload,
 // positions, SparseSystem access, solve wrapper and kernel timers.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -233,6 +234,16 @@ int solve(kt_ctx *ctx, const kt_solve_opts *opts, kt_solve_result *out) {
 }  // namespace
 }  // namespace kt
 
+namespace kt {
+namespace {
+__global__ void __launch_bounds__(256) kt_copy_probe(const uint4 *__restrict__ in, uint4 *__restrict__ out,
+                                                      uint64_t n) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * 256)
+        out[i] = in[i];
+}
+}  // namespace
+}  // namespace kt
+
 using namespace kt;
 
 #define KT_GUARD(ctx)                                                      \
@@ -270,6 +281,10 @@ int kt_ctx_create(kt_ctx **out, int device) {
     KT_CATCH({
         auto *ctx = new kt_ctx();
         ctx->device = device;
+        // KTPLACE_NO_GRAPHS=1 launches every CG kernel directly instead of
+        // replaying captured hipGraphs (for tools that trace graph launches badly)
+        const char *noGraphs = std::getenv("KTPLACE_NO_GRAPHS");
+        if (noGraphs && noGraphs[0] == '1') ctx->tune.useGraphs = false;
         hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&ctx->ctlHost), 2 * sizeof(AxisCtl));
         if (e != hipSuccess) {
@@ -304,6 +319,10 @@ int kt_ctx_destroy(kt_ctx *ctx) {
         ctx->r32[a].release(); ctx->p32[a].release(); ctx->ap32[a].release(); ctx->x32[a].release();
     }
     ctx->partials.release(); ctx->ctl.release(); ctx->history.release();
+    for (auto &g : ctx->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
     if (ctx->ctlHost) (void)hipHostFree(ctx->ctlHost);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -521,8 +540,8 @@ int kt_solve_both(kt_ctx *ctx, double *x, double *y, uint64_t count, const kt_so
                 }
                 bool last = false;
                 if (it + 1 == res.iterations) {
-                    const bool liveX = c[0].active && res.resid[0] > opts->tol;
-                    const bool liveY = c[1].active && res.resid[1] > opts->tol;
+                    const bool liveX = c[0].s[0].active && res.resid[0] > opts->tol;
+                    const bool liveY = c[1].s[0].active && res.resid[1] > opts->tol;
                     last = !liveX && !liveY;
                 }
                 observe(user, it + 1, r[0], r[1], last ? 1 : 0);
@@ -531,6 +550,63 @@ int kt_solve_both(kt_ctx *ctx, double *x, double *y, uint64_t count, const kt_so
         *out = res;
         return KT_OK;
     })
+}
+
+int kt_probe_copy(kt_ctx *ctx, uint64_t bytes, int reps, double *ms_per_copy) {
+    KT_GUARD(ctx);
+    KT_CHECK(bytes >= 16 && bytes % 16 == 0 && reps > 0 && ms_per_copy, KT_EINVAL,
+             "kt_probe_copy: bytes must be a positive multiple of 16");
+    void *a = nullptr, *b = nullptr;
+    KT_HIP(hipMalloc(&a, bytes));
+    hipError_t e = hipMalloc(&b, bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        return hipFail(e, "hipMalloc", __FILE__, __LINE__);
+    }
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (e == hipSuccess) e = hipMemsetAsync(a, 1, bytes, ctx->stream);
+    if (e == hipSuccess) e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    const uint64_t n = bytes / 16;
+    if (e == hipSuccess) e = hipEventRecord(t0, ctx->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        kt_copy_probe<<<dim3(4096), dim3(256), 0, ctx->stream>>>(static_cast<const uint4 *>(a), static_cast<uint4 *>(b), n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(t1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(t1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return hipFail(e, "kt_probe_copy", __FILE__, __LINE__);
+    *ms_per_copy = ms / reps;
+    return KT_OK;
+}
+
+int kt_set_tuning(kt_ctx *ctx, int lanes, uint32_t tile, int use_graphs, double refresh_factor) {
+    return kt_set_tuning2(ctx, lanes, tile, use_graphs, refresh_factor, -1, 0);
+}
+
+int kt_set_tuning2(kt_ctx *ctx, int lanes, uint32_t tile, int use_graphs, double refresh_factor, int xcd_split,
+                   int rows_in_flight) {
+    KT_GUARD(ctx);
+    KT_CHECK(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16, KT_EINVAL,
+             "kt_set_tuning: lanes must be 0, 1, 2, 4, 8 or 16");
+    KT_CHECK(tile % 1024 == 0, KT_EINVAL, "kt_set_tuning: tile must be a multiple of 1024");
+    KT_CHECK(rows_in_flight == 0 || rows_in_flight == 2 || rows_in_flight == 4 || rows_in_flight == 8, KT_EINVAL,
+             "kt_set_tuning: rows in flight must be 0, 2, 4 or 8");
+    ctx->tune.rows = rows_in_flight;
+    KT_CHECK(refresh_factor >= 0.0 && refresh_factor < 1.0, KT_EINVAL, "kt_set_tuning: refresh factor in [0, 1)");
+    ctx->tune.lanes = lanes;
+    ctx->tune.tile = tile;
+    const char *noGraphs = std::getenv("KTPLACE_NO_GRAPHS");
+    ctx->tune.useGraphs = use_graphs != 0 && !(noGraphs && noGraphs[0] == '1');
+    ctx->tune.refreshFactor = refresh_factor;
+    ctx->tune.xcdSplit = xcd_split < 0 ? -1 : (xcd_split ? 1 : 0);
+    return KT_OK;
 }
 
 int kt_kernel_times(kt_ctx *ctx, int kernel, uint64_t *launches, double *total_ms) {

@@ -7,14 +7,25 @@
 // if rho' <= 0), beta = rho' / rho, p = z + beta p.  An axis steps while
 // active && resid > tol; the loop count is the reference's lock-step count.
 //
-// Every launch covers both axes: blocks [0, blocks_x) work on x, the rest on y.
-// Each block owns a tile of kTile rows.  Dot products are reduced in a fixed
-// order (per-row products summed over the tile by a fixed tree, tile partials
-// summed by the last block to finish, in tile order), so results are identical
-// run to run.  z is never stored: z_i = r_i / d_i is recomputed where needed.
-// The per-axis scalars (alpha, beta, rho, residuals, flags) stay on the device
-// in an AxisCtl block, so the host only polls the live flags every few
-// iterations.
+// Layout and launch shape
+//   * Every launch covers both axes: blocks [0, blocks_x) work on x, the rest
+//     on y.  A block owns a tile of `tile` consecutive rows (4096 by default,
+//     1024 threads), so a 1M-row axis is 245 blocks and the whole launch about
+//     two blocks per CU.
+//   * Dot products: each block reduces its rows with a fixed tree and writes
+//     one partial per tile.  The next kernel's blocks each re-reduce all tile
+//     partials of their axis in tile order (a few hundred values), so every
+//     block derives the same alpha / beta / flags without a grid barrier, an
+//     atomic or a device-scope fence.
+//   * Per-axis scalars (rho, resid, step count, live/active flags) are double
+//     buffered by iteration parity: iteration k reads state[k & 1] and only
+//     block 0 of its last kernel writes state[(k + 1) & 1], so no block ever
+//     reads a value written by its own kernel.
+//   * z = r / d is never stored; it is recomputed where needed.
+//   * Vectors and matrix values are FP64 (plain solve) or FP32 (inner solve of
+//     the mixed-precision path); every dot product accumulates in FP64.
+//   * Iterations are launched in pairs (parity 0, 1) captured in hipGraphs; the
+//     host polls the state every few iterations.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -24,31 +35,30 @@
 namespace kt {
 namespace {
 
-constexpr int kBlock = 256;
-constexpr uint32_t kTile = 256;  // rows per block
+constexpr int kBlock = 1024;
 
-struct CgArgs {
-    SysView sys[2];
-    double *x[2];
-    double *r[2];
-    double *p[2];
-    double *ap[2];
-    double *partials;   // [axis][tile][3]
-    uint32_t partStride;
-    AxisCtl *ctl;
-    double *history;    // [axis][histCap]
-    uint32_t histCap;
-};
-
-__device__ __forceinline__ void decode(const CgArgs &a, int &axis, uint32_t &tile) {
+// Block -> (axis, tile).  Workgroups are dealt to the 8 XCDs round-robin by
+// id, so with xcdSplit the blocks that land on XCDs 0-3 take the x system and
+// those on XCDs 4-7 the y system: each XCD's L2 then caches the gathered
+// vector of one axis only.  Returns false for a block with no tile.
+__device__ __forceinline__ bool decode(const CgArgs &a, int &axis, uint32_t &tile) {
+    if (a.xcdSplit) {
+        const uint32_t b = blockIdx.x, x = b & 7u;
+        axis = static_cast<int>(x >> 2);
+        tile = (b >> 3) * 4u + (x & 3u);
+        return tile < a.sys[axis].blocks;
+    }
     axis = blockIdx.x >= a.sys[0].blocks ? 1 : 0;
     tile = blockIdx.x - (axis ? a.sys[0].blocks : 0);
+    return true;
 }
 
-// Fixed-order block sum of K values per thread; result valid in thread 0.
+// Fixed-order block sum of K values per thread; the result is returned to
+// every thread of the block.
 template <int K>
 __device__ __forceinline__ void blockSum(double (&v)[K]) {
     __shared__ double waveSums[kBlock / 64][K];
+    __shared__ double total[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
 #pragma unroll
@@ -60,274 +70,564 @@ __device__ __forceinline__ void blockSum(double (&v)[K]) {
         for (int k = 0; k < K; ++k) waveSums[wave][k] = v[k];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            double s = waveSums[0][k];
-            for (int w = 1; w < kBlock / 64; ++w) s += waveSums[w][k];
-            v[k] = s;
-        }
+    if (threadIdx.x < K) {
+        double s = waveSums[0][threadIdx.x];
+        for (int w = 1; w < kBlock / 64; ++w) s += waveSums[w][threadIdx.x];
+        total[threadIdx.x] = s;
     }
     __syncthreads();
-}
-
-// Writes this block's partials, then returns true in every thread of the last
-// block of the axis to arrive (which then sees every partial).
-template <int K>
-__device__ bool arriveLast(const CgArgs &a, int axis, uint32_t tile, const double (&v)[K]) {
-    __shared__ int amLast;
-    if (threadIdx.x == 0) {
-        double *dst = a.partials + (static_cast<size_t>(axis) * a.partStride + tile) * 3;
 #pragma unroll
-        for (int k = 0; k < K; ++k) dst[k] = v[k];
-        __threadfence();
-        const uint32_t prev = atomicAdd(&a.ctl[axis].counter, 1u);
-        amLast = prev == a.sys[axis].blocks - 1;
-    }
-    __syncthreads();
-    if (amLast) __threadfence();
-    return amLast != 0;
+    for (int k = 0; k < K; ++k) v[k] = total[k];
 }
 
-// Sum of all tile partials of an axis in tile order (fixed tree); valid in thread 0.
+// Every block of an axis computes the same sum of the axis' tile partials.
 template <int K>
-__device__ void reducePartials(const CgArgs &a, int axis, double (&out)[K]) {
-    const uint32_t blocks = a.sys[axis].blocks;
-    const double *src = a.partials + static_cast<size_t>(axis) * a.partStride * 3;
+__device__ __forceinline__ void sumPartials(const double *part, uint32_t blocks, uint32_t stride, int axis,
+                                            double (&out)[K]) {
+    const double *src = part + static_cast<size_t>(axis) * stride * K;
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = 0.0;
     for (uint32_t t = threadIdx.x; t < blocks; t += kBlock) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) out[k] += __builtin_nontemporal_load(src + t * 3 + k);
+        for (int k = 0; k < K; ++k) out[k] += src[t * K + k];
     }
     blockSum<K>(out);
 }
 
-// Row i of A times v, G lanes per row; the sum is complete in lane 0 of the group.
-template <int G>
-__device__ __forceinline__ double rowDot(const SysView &s, const double *__restrict__ v, uint32_t i,
-                                         bool valid, uint32_t lane) {
-    double sum = 0.0;
-    if (valid) {
-        const uint32_t b = s.rowPtr[i], e = s.rowPtr[i + 1];
-        for (uint32_t k = b + lane; k < e; k += G) sum += s.val[k] * v[s.col[k]];
-    }
+template <int K>
+__device__ __forceinline__ void writePartial(double *part, uint32_t stride, int axis, uint32_t tile,
+                                             const double (&v)[K]) {
+    if (threadIdx.x == 0) {
+        double *dst = part + (static_cast<size_t>(axis) * stride + tile) * K;
 #pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, G);
-    return sum;
+        for (int k = 0; k < K; ++k) dst[k] = v[k];
+    }
 }
 
-// r = b - A x, p = z = r / d; partials b.b, r.z, r.r.
-template <int G>
+template <typename T>
+struct Mat;
+template <>
+struct Mat<double> {
+    static __device__ __forceinline__ const double *val(const SysView &s) { return s.val; }
+    static __device__ __forceinline__ const double *diag(const SysView &s) { return s.diag; }
+};
+template <>
+struct Mat<float> {
+    static __device__ __forceinline__ const float *val(const SysView &s) { return s.val32; }
+    static __device__ __forceinline__ const float *diag(const SysView &s) { return s.diag32; }
+};
+
+// Off-diagonal parts of R rows times v, G lanes per row, FP64 accumulation.
+// All loads of a round are issued before any is consumed: the rowPtr pairs
+// of the R rows, then U = 16 / G column indices and values per lane and row,
+// then the U gathers of v, so a typical row (up to 16 entries) costs three
+// dependent memory latencies instead of one per entry.  Longer rows loop over
+// further rounds.  The summation order (lane-strided rounds, then a fixed
+// butterfly over the G lanes) is fixed, so results are deterministic.
+constexpr uint32_t kNoCol = 0xffffffffu;
+
+template <typename T, int G, int R>
+__device__ __forceinline__ void rowsDot(const uint32_t *__restrict__ rowPtr, const uint32_t *__restrict__ col,
+                                        const T *__restrict__ val, const T *__restrict__ v,
+                                        const uint32_t (&row)[R], const bool (&valid)[R], uint32_t lane,
+                                        double (&out)[R]) {
+    constexpr int U = G >= 16 ? 1 : 16 / G;
+    uint32_t b[R], e[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        b[r] = valid[r] ? rowPtr[row[r]] : 0u;
+        e[r] = valid[r] ? rowPtr[row[r] + 1] : 0u;
+        out[r] = 0.0;
+    }
+    uint32_t c[R][U];
+    T w[R][U];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t k = b[r] + lane + u * G;
+            const bool ok = k < e[r];
+            c[r][u] = ok ? col[k] : kNoCol;
+            w[r][u] = ok ? val[k] : T(0);
+        }
+    }
+    T g[R][U];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) g[r][u] = c[r][u] != kNoCol ? v[c[r][u]] : T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) out[r] += static_cast<double>(w[r][u]) * static_cast<double>(g[r][u]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        for (uint32_t base = b[r] + G * U; base < e[r]; base += G * U) {
+            uint32_t cc[U];
+            T ww[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t k = base + lane + u * G;
+                const bool ok = k < e[r];
+                cc[u] = ok ? col[k] : kNoCol;
+                ww[u] = ok ? val[k] : T(0);
+            }
+            T gg[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) gg[u] = cc[u] != kNoCol ? v[cc[u]] : T(0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) out[r] += static_cast<double>(ww[u]) * static_cast<double>(gg[u]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) out[r] += __shfl_xor(out[r], off, G);
+    }
+}
+
+// Rows handled together by one lane group per pass.
+constexpr int kRowsPerGroup = 2;
+
+// ---------------------------------------------------------------- start
+// r = b - A x (FP64), p = z = r / d in the working precision T; partials
+// b.b, r.z, r.r.  For T = float the FP64 residual is rounded once.
+template <typename T, int G>
 __global__ void __launch_bounds__(kBlock) cgStart(CgArgs a) {
     int axis;
     uint32_t tile;
-    decode(a, axis, tile);
+    if (!decode(a, axis, tile)) return;
     if (a.ctl[axis].frozen) return;
     const SysView s = a.sys[axis];
-    const double *__restrict__ x = a.x[axis];
-    double *__restrict__ r = a.r[axis];
-    double *__restrict__ p = a.p[axis];
+    const double *__restrict__ x = a.x64[axis];
+    T *__restrict__ r = static_cast<T *>(a.r[axis]);
+    T *__restrict__ p = static_cast<T *>(a.p[axis]);
+    T *__restrict__ e = static_cast<T *>(a.e[axis]);
+    const T *__restrict__ dT = Mat<T>::diag(s);
     const uint32_t lane = threadIdx.x % G, grp = threadIdx.x / G;
-    const uint32_t begin = tile * kTile, end = min(begin + kTile, s.rows);
+    const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
     double acc[3] = {0.0, 0.0, 0.0};
-    for (uint32_t r0 = begin; r0 < end; r0 += kBlock / G) {
-        const uint32_t i = r0 + grp;
-        const bool valid = i < end;
-        const double off = rowDot<G>(s, x, i, valid, lane);
-        if (valid && lane == 0) {
-            const double axv = s.diag[i] * x[i] + off;
-            const double ri = s.rhs[i] - axv;
-            const double zi = ri / s.diag[i];
-            r[i] = ri;
-            p[i] = zi;
-            acc[0] += s.rhs[i] * s.rhs[i];
-            acc[1] += ri * zi;
-            acc[2] += ri * ri;
+    constexpr uint32_t kPass = kRowsPerGroup * (kBlock / G);
+    for (uint32_t r0 = begin; r0 < end; r0 += kPass) {
+        uint32_t row[kRowsPerGroup];
+        bool valid[kRowsPerGroup];
+        double off[kRowsPerGroup];
+#pragma unroll
+        for (int q = 0; q < kRowsPerGroup; ++q) {
+            row[q] = r0 + grp + q * (kBlock / G);
+            valid[q] = row[q] < end;
+        }
+        rowsDot<double, G, kRowsPerGroup>(s.rowPtr, s.col, s.val, x, row, valid, lane, off);
+#pragma unroll
+        for (int q = 0; q < kRowsPerGroup; ++q) {
+            const uint32_t i = row[q];
+            if (valid[q] && lane == 0) {
+                const double ri = s.rhs[i] - (s.diag[i] * x[i] + off[q]);
+                const T rt = static_cast<T>(ri);
+                const T zt = rt / dT[i];
+                r[i] = rt;
+                p[i] = zt;
+                if (e) e[i] = T(0);
+                acc[0] += s.rhs[i] * s.rhs[i];
+                acc[1] += static_cast<double>(rt) * static_cast<double>(zt);
+                acc[2] += ri * ri;
+            }
         }
     }
     blockSum<3>(acc);
-    if (!arriveLast<3>(a, axis, tile, acc)) return;
+    writePartial<3>(a.partS, a.stride, axis, tile, acc);
+}
+
+// One block per axis: turns the start partials into state[0].
+__global__ void __launch_bounds__(kBlock) cgStartFinish(CgArgs a) {
+    const int axis = blockIdx.x;
+    AxisCtl &c = a.ctl[axis];
+    if (c.frozen) return;
     double tot[3];
-    reducePartials<3>(a, axis, tot);
+    sumPartials<3>(a.partS, a.sys[axis].blocks, a.stride, axis, tot);
     if (threadIdx.x == 0) {
-        AxisCtl &c = a.ctl[axis];
-        c.counter = 0;
+        CgState &st = c.s[0];
         c.bNorm = sqrt(tot[0]);
         if (!(c.bNorm > 0.0)) {
-            c.active = 0;
-            c.live = 0;
+            st.active = 0;
+            st.live = 0;
         } else {
-            c.rho = tot[1];
-            c.resid = sqrt(tot[2]) / c.bNorm;
-            c.active = 1;
-            c.live = (c.resid > c.tol && c.steps < c.maxIter) ? 1 : 0;
+            st.rho = tot[1];
+            st.resid = sqrt(tot[2]) / c.bNorm;
+            st.active = 1;
+            st.live = (st.resid > c.tol && st.steps < c.maxIter) ? 1 : 0;
         }
+        c.trueResid = st.resid;
+        c.refResid = st.resid;
+        c.needUpdate = 0;
     }
 }
 
-// Ap = A p and p.Ap; the last block sets alpha or stops the axis.
-template <int G>
-__global__ void __launch_bounds__(kBlock) cgSpmv(CgArgs a) {
+// ---------------------------------------------------------------- one iteration
+// Ap = A p and the tile partial of p.Ap.
+template <typename T, int G, int R>
+__global__ void __launch_bounds__(kBlock) cgSpmv(CgArgs a, int parity) {
     int axis;
     uint32_t tile;
-    decode(a, axis, tile);
-    AxisCtl *ctl = a.ctl + axis;
-    if (!ctl->live) return;
+    if (!decode(a, axis, tile)) return;
+    if (!a.ctl[axis].s[parity].live) return;
     const SysView s = a.sys[axis];
-    const double *__restrict__ p = a.p[axis];
-    double *__restrict__ ap = a.ap[axis];
+    const T *__restrict__ p = static_cast<const T *>(a.p[axis]);
+    T *__restrict__ ap = static_cast<T *>(a.ap[axis]);
+    const T *__restrict__ val = Mat<T>::val(s);
+    const T *__restrict__ diag = Mat<T>::diag(s);
     const uint32_t lane = threadIdx.x % G, grp = threadIdx.x / G;
-    const uint32_t begin = tile * kTile, end = min(begin + kTile, s.rows);
+    const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
     double acc[1] = {0.0};
-    for (uint32_t r0 = begin; r0 < end; r0 += kBlock / G) {
-        const uint32_t i = r0 + grp;
-        const bool valid = i < end;
-        const double off = rowDot<G>(s, p, i, valid, lane);
-        if (valid && lane == 0) {
-            const double pi = p[i];
-            const double v = s.diag[i] * pi + off;
-            ap[i] = v;
-            acc[0] += pi * v;
+    constexpr uint32_t kPass = R * (kBlock / G);
+    for (uint32_t r0 = begin; r0 < end; r0 += kPass) {
+        uint32_t row[R];
+        bool valid[R];
+        double off[R];
+        T pi[R], di[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            row[q] = r0 + grp + q * (kBlock / G);
+            valid[q] = row[q] < end;
+            pi[q] = valid[q] ? p[row[q]] : T(0);
+            di[q] = valid[q] ? diag[row[q]] : T(0);
+        }
+        rowsDot<T, G, R>(s.rowPtr, s.col, val, p, row, valid, lane, off);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            if (valid[q] && lane == 0) {
+                const double pd = static_cast<double>(pi[q]);
+                const T v = static_cast<T>(static_cast<double>(di[q]) * pd + off[q]);
+                ap[row[q]] = v;
+                acc[0] += pd * static_cast<double>(v);
+            }
         }
     }
     blockSum<1>(acc);
-    if (!arriveLast<1>(a, axis, tile, acc)) return;
-    double tot[1];
-    reducePartials<1>(a, axis, tot);
-    if (threadIdx.x == 0) {
-        AxisCtl &c = *ctl;
-        c.counter = 0;
-        c.pap = tot[0];
-        ++c.steps;
-        if (!(tot[0] > 0.0)) {
-            c.active = 0;
-            c.live = 0;
-            if (c.steps <= a.histCap) a.history[static_cast<size_t>(axis) * a.histCap + c.steps - 1] = c.resid;
-        } else {
-            c.alpha = c.rho / tot[0];
-        }
-    }
+    writePartial<1>(a.partA, a.stride, axis, tile, acc);
 }
 
-// x += alpha p; r -= alpha Ap; partials r.r and r.z (z = r / d).
-__global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a) {
+// alpha = rho / p.Ap; x += alpha p; r -= alpha Ap; tile partials of r.r, r.z.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a, int parity) {
     int axis;
     uint32_t tile;
-    decode(a, axis, tile);
-    AxisCtl *ctl = a.ctl + axis;
-    if (!ctl->live) return;
+    if (!decode(a, axis, tile)) return;
+    const CgState st = a.ctl[axis].s[parity];
+    if (!st.live) return;
     const SysView s = a.sys[axis];
-    const double alpha = ctl->alpha;
-    double *__restrict__ x = a.x[axis];
-    double *__restrict__ r = a.r[axis];
-    const double *__restrict__ p = a.p[axis];
-    const double *__restrict__ ap = a.ap[axis];
-    const uint32_t i = tile * kTile + threadIdx.x;
+    double pap[1];
+    sumPartials<1>(a.partA, s.blocks, a.stride, axis, pap);
+    if (!(pap[0] > 0.0)) {
+        if (tile == 0 && threadIdx.x == 0) a.ctl[axis].brk[parity] = 1;
+        return;
+    }
+    if (tile == 0 && threadIdx.x == 0) a.ctl[axis].brk[parity] = 0;
+    const double alpha = st.rho / pap[0];
+    T *__restrict__ x = static_cast<T *>(a.e[axis] ? a.e[axis] : static_cast<void *>(a.x64[axis]));
+    T *__restrict__ r = static_cast<T *>(a.r[axis]);
+    const T *__restrict__ p = static_cast<const T *>(a.p[axis]);
+    const T *__restrict__ ap = static_cast<const T *>(a.ap[axis]);
+    const T *__restrict__ d = Mat<T>::diag(s);
+    const T alphaT = static_cast<T>(alpha);
+    const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
     double acc[2] = {0.0, 0.0};
-    if (i < s.rows) {
-        x[i] += alpha * p[i];
-        const double ri = r[i] - alpha * ap[i];
+    for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
+        x[i] += alphaT * p[i];
+        const T ri = r[i] - alphaT * ap[i];
         r[i] = ri;
-        acc[0] = ri * ri;
-        acc[1] = ri * (ri / s.diag[i]);
+        const double rd = static_cast<double>(ri);
+        acc[0] += rd * rd;
+        acc[1] += rd * static_cast<double>(ri / d[i]);
     }
     blockSum<2>(acc);
-    if (!arriveLast<2>(a, axis, tile, acc)) return;
-    double tot[2];
-    reducePartials<2>(a, axis, tot);
-    if (threadIdx.x == 0) {
-        AxisCtl &c = *ctl;
-        c.counter = 0;
-        c.rr = tot[0];
-        c.resid = sqrt(tot[0]) / c.bNorm;
-        if (c.steps <= a.histCap) a.history[static_cast<size_t>(axis) * a.histCap + c.steps - 1] = c.resid;
-        const double rhoNext = tot[1];
-        c.rz = rhoNext;
-        if (!(rhoNext > 0.0)) {
-            c.active = 0;
-            c.live = 0;
-        } else {
-            c.beta = rhoNext / c.rho;
-            c.rho = rhoNext;
-            // live for the next iteration; applied by the p-update's last block
-            c.doP = (c.resid > c.tol && c.steps < c.maxIter) ? 1 : 0;
-        }
-    }
+    writePartial<2>(a.partB, a.stride, axis, tile, acc);
 }
 
-// p = z + beta p; the last block publishes the next iteration's live flag.
-__global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a) {
+// beta = r.z' / r.z; p = z + beta p; block 0 publishes state[parity ^ 1].
+template <typename T>
+__global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
     int axis;
     uint32_t tile;
-    decode(a, axis, tile);
-    AxisCtl *ctl = a.ctl + axis;
-    if (!ctl->live) return;
-    const SysView s = a.sys[axis];
-    const double beta = ctl->beta;
-    double *__restrict__ p = a.p[axis];
-    const double *__restrict__ r = a.r[axis];
-    const uint32_t i = tile * kTile + threadIdx.x;
-    if (i < s.rows) p[i] = r[i] / s.diag[i] + beta * p[i];
-    __shared__ int amLast;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        amLast = atomicAdd(&ctl->counter, 1u) == s.blocks - 1;
+    if (!decode(a, axis, tile)) return;
+    AxisCtl &c = a.ctl[axis];
+    const CgState st = c.s[parity];
+    const bool lead = tile == 0 && threadIdx.x == 0;
+    if (!st.live) {
+        if (lead) c.s[parity ^ 1] = st;  // carry a stopped axis' state forward
+        return;
     }
-    __syncthreads();
-    if (amLast && threadIdx.x == 0) {
-        __threadfence();
-        ctl->counter = 0;
-        ctl->live = ctl->doP;
+    const SysView s = a.sys[axis];
+    const size_t hist = static_cast<size_t>(axis) * a.histCap + st.steps;
+    if (c.brk[parity]) {  // p.Ap <= 0: the reference stops the axis (simpl_lss.cc:331-334)
+        if (lead) {
+            CgState nx = st;
+            nx.active = 0;
+            nx.live = 0;
+            nx.steps = st.steps + 1;
+            c.s[parity ^ 1] = nx;
+            if (st.steps < a.histCap) a.history[hist] = st.resid;
+        }
+        return;
+    }
+    double tot[2];
+    sumPartials<2>(a.partB, s.blocks, a.stride, axis, tot);
+    const double resid = sqrt(tot[0]) / c.bNorm;
+    const double rhoNext = tot[1];
+    const bool ok = rhoNext > 0.0;
+    if (ok) {
+        const T beta = static_cast<T>(rhoNext / st.rho);
+        T *__restrict__ p = static_cast<T *>(a.p[axis]);
+        const T *__restrict__ r = static_cast<const T *>(a.r[axis]);
+        const T *__restrict__ d = Mat<T>::diag(s);
+        const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
+        for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) p[i] = r[i] / d[i] + beta * p[i];
+    }
+    if (lead) {
+        CgState nx;
+        nx.steps = st.steps + 1;
+        nx.resid = resid;
+        nx.rho = ok ? rhoNext : st.rho;
+        nx.active = ok ? 1 : 0;
+        nx.live = (ok && resid > c.tol && nx.steps < c.maxIter) ? 1 : 0;
+        nx.pad = 0;
+        c.s[parity ^ 1] = nx;
+        if (st.steps < a.histCap) a.history[hist] = resid;
+        // mixed precision: ask for an FP64 refresh once the inner residual fell
+        // by the refresh factor, or reached the tolerance
+        if (a.e[axis] && (resid <= c.refreshFactor * c.refResid || resid <= c.tol)) c.needUpdate = 1;
     }
 }
 
-// True residual ||b - A x|| / ||b|| into ctl->rr (no vector written).
+// ---------------------------------------------------------------- FP64 refresh (mixed precision)
+// x64 += e (FP32 correction), e = 0, for axes that asked for a refresh.
+__global__ void __launch_bounds__(kBlock) cgFlush(CgArgs a) {
+    int axis;
+    uint32_t tile;
+    if (!decode(a, axis, tile)) return;
+    if (!a.ctl[axis].needUpdate) return;
+    const SysView s = a.sys[axis];
+    double *__restrict__ x = a.x64[axis];
+    float *__restrict__ e = static_cast<float *>(a.e[axis]);
+    const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
+    for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
+        x[i] += static_cast<double>(e[i]);
+        e[i] = 0.0f;
+    }
+}
+
+// r64 = b - A x64 (FP64), r = (float) r64; partials r64.r64 and r.z.
+template <int G>
+__global__ void __launch_bounds__(kBlock) cgRefresh(CgArgs a) {
+    int axis;
+    uint32_t tile;
+    if (!decode(a, axis, tile)) return;
+    if (!a.ctl[axis].needUpdate) return;
+    const SysView s = a.sys[axis];
+    const double *__restrict__ x = a.x64[axis];
+    float *__restrict__ r = static_cast<float *>(a.r[axis]);
+    const uint32_t lane = threadIdx.x % G, grp = threadIdx.x / G;
+    const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
+    double acc[2] = {0.0, 0.0};
+    constexpr uint32_t kPass = kRowsPerGroup * (kBlock / G);
+    for (uint32_t r0 = begin; r0 < end; r0 += kPass) {
+        uint32_t row[kRowsPerGroup];
+        bool valid[kRowsPerGroup];
+        double off[kRowsPerGroup];
+#pragma unroll
+        for (int q = 0; q < kRowsPerGroup; ++q) {
+            row[q] = r0 + grp + q * (kBlock / G);
+            valid[q] = row[q] < end;
+        }
+        rowsDot<double, G, kRowsPerGroup>(s.rowPtr, s.col, s.val, x, row, valid, lane, off);
+#pragma unroll
+        for (int q = 0; q < kRowsPerGroup; ++q) {
+            const uint32_t i = row[q];
+            if (valid[q] && lane == 0) {
+                const double ri = s.rhs[i] - (s.diag[i] * x[i] + off[q]);
+                const float rf = static_cast<float>(ri);
+                r[i] = rf;
+                acc[0] += ri * ri;
+                acc[1] += static_cast<double>(rf) * static_cast<double>(rf / s.diag32[i]);
+            }
+        }
+    }
+    blockSum<2>(acc);
+    writePartial<2>(a.partB, a.stride, axis, tile, acc);
+}
+
+// One block per axis: the true residual becomes the state; rho restarts from
+// the refreshed r (the search direction p is kept: a reliable update).
+__global__ void __launch_bounds__(kBlock) cgRefreshFinish(CgArgs a) {
+    const int axis = blockIdx.x;
+    AxisCtl &c = a.ctl[axis];
+    if (!c.needUpdate) return;
+    double tot[2];
+    sumPartials<2>(a.partB, a.sys[axis].blocks, a.stride, axis, tot);
+    if (threadIdx.x == 0) {
+        CgState &st = c.s[0];
+        const double resid = c.bNorm > 0.0 ? sqrt(tot[0]) / c.bNorm : 0.0;
+        st.resid = resid;
+        c.trueResid = resid;
+        c.refResid = resid;
+        ++c.refreshes;
+        if (st.active && tot[1] > 0.0) st.rho = tot[1];
+        st.live = (st.active && resid > c.tol && st.steps < c.maxIter) ? 1 : 0;
+        c.needUpdate = 0;
+    }
+}
+
+// ---------------------------------------------------------------- true residual
 template <int G>
 __global__ void __launch_bounds__(kBlock) cgTrueResidual(CgArgs a) {
     int axis;
     uint32_t tile;
-    decode(a, axis, tile);
+    if (!decode(a, axis, tile)) return;
     const SysView s = a.sys[axis];
-    const double *__restrict__ x = a.x[axis];
+    const double *__restrict__ x = a.x64[axis];
     const uint32_t lane = threadIdx.x % G, grp = threadIdx.x / G;
-    const uint32_t begin = tile * kTile, end = min(begin + kTile, s.rows);
+    const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
     double acc[2] = {0.0, 0.0};
-    for (uint32_t r0 = begin; r0 < end; r0 += kBlock / G) {
-        const uint32_t i = r0 + grp;
-        const bool valid = i < end;
-        const double off = rowDot<G>(s, x, i, valid, lane);
-        if (valid && lane == 0) {
-            const double ri = s.rhs[i] - (s.diag[i] * x[i] + off);
-            acc[0] += ri * ri;
-            acc[1] += s.rhs[i] * s.rhs[i];
+    constexpr uint32_t kPass = kRowsPerGroup * (kBlock / G);
+    for (uint32_t r0 = begin; r0 < end; r0 += kPass) {
+        uint32_t row[kRowsPerGroup];
+        bool valid[kRowsPerGroup];
+        double off[kRowsPerGroup];
+#pragma unroll
+        for (int q = 0; q < kRowsPerGroup; ++q) {
+            row[q] = r0 + grp + q * (kBlock / G);
+            valid[q] = row[q] < end;
+        }
+        rowsDot<double, G, kRowsPerGroup>(s.rowPtr, s.col, s.val, x, row, valid, lane, off);
+#pragma unroll
+        for (int q = 0; q < kRowsPerGroup; ++q) {
+            const uint32_t i = row[q];
+            if (valid[q] && lane == 0) {
+                const double ri = s.rhs[i] - (s.diag[i] * x[i] + off[q]);
+                acc[0] += ri * ri;
+                acc[1] += s.rhs[i] * s.rhs[i];
+            }
         }
     }
     blockSum<2>(acc);
-    if (!arriveLast<2>(a, axis, tile, acc)) return;
+    writePartial<2>(a.partB, a.stride, axis, tile, acc);
+}
+
+__global__ void __launch_bounds__(kBlock) cgTrueResidualFinish(CgArgs a) {
+    const int axis = blockIdx.x;
     double tot[2];
-    reducePartials<2>(a, axis, tot);
+    sumPartials<2>(a.partB, a.sys[axis].blocks, a.stride, axis, tot);
     if (threadIdx.x == 0) {
-        AxisCtl &c = a.ctl[axis];
-        c.counter = 0;
         const double bn = sqrt(tot[1]);
-        c.trueResid = bn > 0.0 ? sqrt(tot[0]) / bn : 0.0;
+        a.ctl[axis].trueResid = bn > 0.0 ? sqrt(tot[0]) / bn : 0.0;
     }
+}
+
+// FP32 copies of the matrix values and diagonal (mixed precision).
+__global__ void __launch_bounds__(kBlock)
+toFloat(const double *__restrict__ in, float *__restrict__ out, uint64_t n) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * kBlock)
+        out[i] = static_cast<float>(in[i]);
 }
 
 // Plain product out = A in (SparseSystem::multiply order: diag first, then
 // the row's entries in order), one thread per row.
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(256)
 multiplyKernel(SysView s, const double *__restrict__ in, double *__restrict__ out) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= s.rows) return;
     double sum = s.diag[i] * in[i];
     for (uint32_t e = s.rowPtr[i]; e < s.rowPtr[i + 1]; ++e) sum += s.val[e] * in[s.col[e]];
     out[i] = sum;
 }
 
+uint32_t gridBlocks(const CgArgs &a) {
+    if (a.xcdSplit) return 8u * ((std::max(a.sys[0].blocks, a.sys[1].blocks) + 3u) / 4u);
+    return a.sys[0].blocks + a.sys[1].blocks;
+}
+
+#define KT_DISPATCH_G(G, KERNEL, GRID, STREAM, ...)                                        \
+    switch (G) {                                                                           \
+        case 1: KERNEL<1><<<GRID, dim3(kBlock), 0, STREAM>>>(__VA_ARGS__); break;          \
+        case 2: KERNEL<2><<<GRID, dim3(kBlock), 0, STREAM>>>(__VA_ARGS__); break;          \
+        case 4: KERNEL<4><<<GRID, dim3(kBlock), 0, STREAM>>>(__VA_ARGS__); break;          \
+        case 8: KERNEL<8><<<GRID, dim3(kBlock), 0, STREAM>>>(__VA_ARGS__); break;          \
+        default: KERNEL<16><<<GRID, dim3(kBlock), 0, STREAM>>>(__VA_ARGS__); break;        \
+    }
+
+template <typename T>
+int launchIteration(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, bool timed) {
+    const dim3 grid(gridBlocks(a));
+    hipEvent_t t0 = nullptr;
+    const int kSpmv = sizeof(T) == 8 ? KT_KERNEL_SPMV : KT_KERNEL_SPMV32;
+    if (timed) KT_TRY(timedBegin(ctx, kSpmv, &t0));
+    const int R = ctx->tune.rows > 0 ? ctx->tune.rows : 2;
+#define KT_SPMV_R(GG)                                                                         \
+    if (R >= 8) cgSpmv<T, GG, 8><<<grid, dim3(kBlock), 0, st>>>(a, parity);                  \
+    else if (R >= 4) cgSpmv<T, GG, 4><<<grid, dim3(kBlock), 0, st>>>(a, parity);             \
+    else cgSpmv<T, GG, 2><<<grid, dim3(kBlock), 0, st>>>(a, parity);
+    switch (G) {
+        case 1: KT_SPMV_R(1) break;
+        case 2: KT_SPMV_R(2) break;
+        case 4: KT_SPMV_R(4) break;
+        case 8: KT_SPMV_R(8) break;
+        default: KT_SPMV_R(16) break;
+    }
+#undef KT_SPMV_R
+    KT_HIP(hipGetLastError());
+    if (timed) KT_TRY(timedEnd(ctx, kSpmv, t0));
+    if (timed) KT_TRY(timedBegin(ctx, KT_KERNEL_UPDATE, &t0));
+    cgUpdate<T><<<grid, dim3(kBlock), 0, st>>>(a, parity);
+    KT_HIP(hipGetLastError());
+    if (timed) KT_TRY(timedEnd(ctx, KT_KERNEL_UPDATE, t0));
+    if (timed) KT_TRY(timedBegin(ctx, KT_KERNEL_PUPDATE, &t0));
+    cgPUpdate<T><<<grid, dim3(kBlock), 0, st>>>(a, parity);
+    KT_HIP(hipGetLastError());
+    if (timed) KT_TRY(timedEnd(ctx, KT_KERNEL_PUPDATE, t0));
+    return KT_OK;
+}
+
+template <typename T>
+int launchStart(kt_ctx *ctx, const CgArgs &a, int G) {
+    const dim3 grid(gridBlocks(a));
+    switch (G) {
+        case 1: cgStart<T, 1><<<grid, dim3(kBlock), 0, ctx->stream>>>(a); break;
+        case 2: cgStart<T, 2><<<grid, dim3(kBlock), 0, ctx->stream>>>(a); break;
+        case 4: cgStart<T, 4><<<grid, dim3(kBlock), 0, ctx->stream>>>(a); break;
+        case 8: cgStart<T, 8><<<grid, dim3(kBlock), 0, ctx->stream>>>(a); break;
+        default: cgStart<T, 16><<<grid, dim3(kBlock), 0, ctx->stream>>>(a); break;
+    }
+    KT_HIP(hipGetLastError());
+    cgStartFinish<<<dim3(2), dim3(kBlock), 0, ctx->stream>>>(a);
+    KT_HIP(hipGetLastError());
+    return KT_OK;
+}
+
+int launchTrueResidual(kt_ctx *ctx, const CgArgs &a, int G) {
+    const dim3 grid(gridBlocks(a));
+    KT_DISPATCH_G(G, cgTrueResidual, grid, ctx->stream, a);
+    KT_HIP(hipGetLastError());
+    cgTrueResidualFinish<<<dim3(2), dim3(kBlock), 0, ctx->stream>>>(a);
+    KT_HIP(hipGetLastError());
+    return KT_OK;
+}
+
+int launchRefresh(kt_ctx *ctx, const CgArgs &a, int G) {
+    const dim3 grid(gridBlocks(a));
+    cgFlush<<<grid, dim3(kBlock), 0, ctx->stream>>>(a);
+    KT_HIP(hipGetLastError());
+    KT_DISPATCH_G(G, cgRefresh, grid, ctx->stream, a);
+    KT_HIP(hipGetLastError());
+    cgRefreshFinish<<<dim3(2), dim3(kBlock), 0, ctx->stream>>>(a);
+    KT_HIP(hipGetLastError());
+    return KT_OK;
+}
+
 int lanesFor(kt_ctx *ctx) {
+    if (ctx->tune.lanes > 0) return ctx->tune.lanes;
     uint64_t rows = 0, nnz = 0;
     for (int a = 0; a < 2; ++a) {
         rows += ctx->sys[a].rows;
@@ -341,16 +641,7 @@ int lanesFor(kt_ctx *ctx) {
     return 16;
 }
 
-#define KT_DISPATCH_G(G, KERNEL, GRID, STREAM, ARGS)                                  \
-    switch (G) {                                                                      \
-        case 1: KERNEL<1><<<GRID, dim3(kBlock), 0, STREAM>>>(ARGS); break;            \
-        case 2: KERNEL<2><<<GRID, dim3(kBlock), 0, STREAM>>>(ARGS); break;            \
-        case 4: KERNEL<4><<<GRID, dim3(kBlock), 0, STREAM>>>(ARGS); break;            \
-        case 8: KERNEL<8><<<GRID, dim3(kBlock), 0, STREAM>>>(ARGS); break;            \
-        default: KERNEL<16><<<GRID, dim3(kBlock), 0, STREAM>>>(ARGS); break;          \
-    }
-
-SysView viewOf(kt_ctx *ctx, int axis) {
+SysView viewOf(kt_ctx *ctx, int axis, uint32_t tile) {
     Sys &s = ctx->sys[axis];
     SysView v{};
     v.rowPtr = s.rowPtr.p;
@@ -361,140 +652,276 @@ SysView viewOf(kt_ctx *ctx, int axis) {
     v.val32 = s.val32.p;
     v.diag32 = s.diag32.p;
     v.rows = static_cast<uint32_t>(s.rows);
-    v.blocks = static_cast<uint32_t>((s.rows + kTile - 1) / kTile);
+    v.blocks = static_cast<uint32_t>((s.rows + tile - 1) / tile);
     return v;
+}
+
+// Iteration pairs (parity 0 then 1); without timing they replay a cached
+// hipGraph of GraphCache::kPairs pairs.
+int runPairs(kt_ctx *ctx, const CgArgs &a, int G, bool fp32, uint32_t pairs, bool timed) {
+    const auto direct = [&](uint32_t count) -> int {
+        for (uint32_t k = 0; k < count; ++k)
+            for (int parity = 0; parity < 2; ++parity) {
+                const int rc = fp32 ? launchIteration<float>(ctx, a, G, parity, ctx->stream, timed)
+                                    : launchIteration<double>(ctx, a, G, parity, ctx->stream, timed);
+                if (rc != KT_OK) return rc;
+            }
+        return KT_OK;
+    };
+    if (timed || !ctx->tune.useGraphs) return direct(pairs);
+    GraphCache &gc = ctx->graphs[fp32 ? 1 : 0];
+    if (!gc.exec || std::memcmp(&gc.args, &a, sizeof(CgArgs)) != 0 || gc.lanes != G || gc.rows != ctx->tune.rows) {
+        if (gc.exec) (void)hipGraphExecDestroy(gc.exec);
+        if (gc.graph) (void)hipGraphDestroy(gc.graph);
+        gc.exec = nullptr;
+        gc.graph = nullptr;
+        KT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int rc = KT_OK;
+        for (int rep = 0; rep < GraphCache::kPairs && rc == KT_OK; ++rep)
+            for (int parity = 0; parity < 2 && rc == KT_OK; ++parity)
+                rc = fp32 ? launchIteration<float>(ctx, a, G, parity, ctx->stream, false)
+                          : launchIteration<double>(ctx, a, G, parity, ctx->stream, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc != KT_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hipFail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        gc.graph = g;
+        KT_HIP(hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0));
+        gc.args = a;
+        gc.lanes = G;
+        gc.rows = ctx->tune.rows;
+    }
+    uint32_t k = 0;
+    for (; k + GraphCache::kPairs <= pairs; k += GraphCache::kPairs) KT_HIP(hipGraphLaunch(gc.exec, ctx->stream));
+    return direct(pairs - k);
+}
+
+int readCtl(kt_ctx *ctx) {
+    KT_HIP(hipMemcpyAsync(ctx->ctlHost, ctx->ctl.p, 2 * sizeof(AxisCtl), hipMemcpyDeviceToHost, ctx->stream));
+    KT_HIP(hipStreamSynchronize(ctx->stream));
+    return KT_OK;
+}
+
+int writeCtl(kt_ctx *ctx) {
+    KT_HIP(hipMemcpyAsync(ctx->ctl.p, ctx->ctlHost, 2 * sizeof(AxisCtl), hipMemcpyHostToDevice, ctx->stream));
+    return KT_OK;
+}
+
+// Iterations to launch before the next poll: what the observed convergence
+// rate says is left (rounded up to a pair), so few no-op iterations run.
+uint32_t nextChunk(const AxisCtl *c, const double prevResid[2], const uint32_t prevSteps[2], double target,
+                   uint32_t every, bool first) {
+    uint32_t chunk = 2, cap = 2;
+    for (int a = 0; a < 2; ++a) {
+        const CgState &st = c[a].s[0];
+        if (st.live) cap = std::max(cap, c[a].maxIter - st.steps);
+    }
+    if (first) return (std::min<uint32_t>(8, cap) + 1) & ~1u;
+    for (int a = 0; a < 2; ++a) {
+        const CgState &st = c[a].s[0];
+        if (!st.live) continue;
+        uint32_t need = every;
+        if (st.steps > prevSteps[a] && prevResid[a] > st.resid && st.resid > 0.0) {
+            const double rate = std::log(st.resid / prevResid[a]) / (st.steps - prevSteps[a]);
+            const double left = std::log(std::max(target, c[a].tol) / st.resid) / rate;
+            need = static_cast<uint32_t>(std::min<double>(std::max(std::ceil(left) + 1.0, 2.0), every));
+        }
+        chunk = std::max(chunk, need);
+    }
+    return (std::min(chunk, cap) + 1) & ~1u;
+}
+
+int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
+    const uint64_t rows = ctx->sys[0].rows;
+    KT_CHECK(ctx->sys[1].rows == rows, KT_ESTATE, "solve: x and y systems differ in size");
+    KT_CHECK(ctx->posCount == rows, KT_ESTATE, "solve: positions do not match the systems");
+    uint64_t nnz[2];
+    for (int ax = 0; ax < 2; ++ax) KT_TRY(systemNnz(ctx, ax, &nnz[ax]));
+    G = lanesFor(ctx);
+    const uint32_t tile = ctx->tune.tile > 0 ? ctx->tune.tile : 4096;
+    KT_CHECK(tile % kBlock == 0, KT_EINVAL, "tile must be a multiple of 1024 rows");
+    a = CgArgs{};
+    a.tile = tile;
+    a.xcdSplit = ctx->tune.xcdSplit == 0 ? 0u : 1u;
+    uint32_t maxBlocks = 1;
+    for (int ax = 0; ax < 2; ++ax) {
+        if (fp32) {
+            Sys &s = ctx->sys[ax];
+            if (!s.have32) {
+                KT_TRY(s.val32.reserve(std::max<uint64_t>(nnz[ax], 1)));
+                KT_TRY(s.diag32.reserve(std::max<uint64_t>(rows, 1)));
+                if (nnz[ax]) {
+                    toFloat<<<dim3(1024), dim3(kBlock), 0, ctx->stream>>>(s.val.p, s.val32.p, nnz[ax]);
+                    KT_HIP(hipGetLastError());
+                }
+                if (rows) {
+                    toFloat<<<dim3(256), dim3(kBlock), 0, ctx->stream>>>(s.diag.p, s.diag32.p, rows);
+                    KT_HIP(hipGetLastError());
+                }
+                s.have32 = true;
+            }
+            KT_TRY(ctx->r32[ax].reserve(std::max<uint64_t>(rows, 1)));
+            KT_TRY(ctx->p32[ax].reserve(std::max<uint64_t>(rows, 1)));
+            KT_TRY(ctx->ap32[ax].reserve(std::max<uint64_t>(rows, 1)));
+            KT_TRY(ctx->x32[ax].reserve(std::max<uint64_t>(rows, 1)));
+            a.r[ax] = ctx->r32[ax].p;
+            a.p[ax] = ctx->p32[ax].p;
+            a.ap[ax] = ctx->ap32[ax].p;
+            a.e[ax] = ctx->x32[ax].p;
+        } else {
+            KT_TRY(ctx->r[ax].reserve(std::max<uint64_t>(rows, 1)));
+            KT_TRY(ctx->p[ax].reserve(std::max<uint64_t>(rows, 1)));
+            KT_TRY(ctx->ap[ax].reserve(std::max<uint64_t>(rows, 1)));
+            a.r[ax] = ctx->r[ax].p;
+            a.p[ax] = ctx->p[ax].p;
+            a.ap[ax] = ctx->ap[ax].p;
+            a.e[ax] = nullptr;
+        }
+        a.sys[ax] = viewOf(ctx, ax, tile);
+        maxBlocks = std::max(maxBlocks, a.sys[ax].blocks);
+    }
+    a.x64[0] = ctx->posX.p;
+    a.x64[1] = ctx->posY.p;
+    a.stride = maxBlocks;
+    KT_TRY(ctx->partials.reserve(2ull * maxBlocks * 6));
+    a.partA = ctx->partials.p;
+    a.partB = ctx->partials.p + 2ull * maxBlocks * 1;
+    a.partS = ctx->partials.p + 2ull * maxBlocks * 3;
+    KT_TRY(ctx->ctl.reserve(2));
+    a.ctl = ctx->ctl.p;
+    const uint64_t histCap = std::min<uint64_t>(std::max<uint64_t>(o.max_iter, 1), 1u << 20);
+    KT_TRY(ctx->history.reserve(2 * histCap));
+    a.history = ctx->history.p;
+    a.histCap = static_cast<uint32_t>(histCap);
+    for (int ax = 0; ax < 2; ++ax) {
+        AxisCtl &h = ctx->ctlHost[ax];
+        std::memset(&h, 0, sizeof(h));
+        h.tol = o.tol;
+        h.maxIter = static_cast<uint32_t>(std::min<uint64_t>(o.max_iter, 0xffffffffu));
+        h.refreshFactor = ctx->tune.refreshFactor > 0.0 ? ctx->tune.refreshFactor : 0.05;
+    }
+    KT_TRY(writeCtl(ctx));
+    return KT_OK;
+}
+
+void fillResult(kt_ctx *ctx, kt_solve_result &res) {
+    for (int ax = 0; ax < 2; ++ax) {
+        const AxisCtl &c = ctx->ctlHost[ax];
+        res.resid[ax] = c.s[0].resid;
+        res.true_resid[ax] = c.trueResid;
+        res.steps[ax] = c.s[0].steps;
+    }
+    res.iterations = std::max(res.steps[0], res.steps[1]);
 }
 
 }  // namespace
 
 int multiply(kt_ctx *ctx, int axis, const double *dIn, double *dOut) {
-    const SysView v = viewOf(ctx, axis);
+    const SysView v = viewOf(ctx, axis, 4096);
     if (v.rows == 0) return KT_OK;
-    multiplyKernel<<<dim3((v.rows + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream>>>(v, dIn, dOut);
+    multiplyKernel<<<dim3((v.rows + 255) / 256), dim3(256), 0, ctx->stream>>>(v, dIn, dOut);
     KT_HIP(hipGetLastError());
     return KT_OK;
 }
 
 int solveFp64(kt_ctx *ctx, const kt_solve_opts &o, kt_solve_result &res) {
-    const uint64_t rows = ctx->sys[0].rows;
-    KT_CHECK(ctx->sys[1].rows == rows, KT_ESTATE, "solve: x and y systems differ in size");
-    KT_CHECK(ctx->posCount == rows, KT_ESTATE, "solve: positions do not match the systems");
-    for (int a = 0; a < 2; ++a) {
-        uint64_t nnz;
-        KT_TRY(systemNnz(ctx, a, &nnz));
-    }
-    const int G = lanesFor(ctx);
-    CgArgs args{};
-    uint32_t maxBlocks = 0;
-    for (int a = 0; a < 2; ++a) {
-        args.sys[a] = viewOf(ctx, a);
-        maxBlocks = std::max(maxBlocks, args.sys[a].blocks);
-        KT_TRY(ctx->r[a].reserve(rows));
-        KT_TRY(ctx->p[a].reserve(rows));
-        KT_TRY(ctx->ap[a].reserve(rows));
-    }
-    args.x[0] = ctx->posX.p;
-    args.x[1] = ctx->posY.p;
-    for (int a = 0; a < 2; ++a) {
-        args.r[a] = ctx->r[a].p;
-        args.p[a] = ctx->p[a].p;
-        args.ap[a] = ctx->ap[a].p;
-    }
-    KT_TRY(ctx->partials.reserve(2ull * std::max<uint32_t>(maxBlocks, 1) * 3));
-    args.partials = ctx->partials.p;
-    args.partStride = std::max<uint32_t>(maxBlocks, 1);
-    KT_TRY(ctx->ctl.reserve(2));
-    args.ctl = ctx->ctl.p;
-    const uint64_t histCap = std::min<uint64_t>(std::max<uint64_t>(o.max_iter, 1), 1u << 20);
-    KT_TRY(ctx->history.reserve(2 * histCap));
-    args.history = ctx->history.p;
-    args.histCap = static_cast<uint32_t>(histCap);
-
-    for (int a = 0; a < 2; ++a) {
-        AxisCtl &h = ctx->ctlHost[a];
-        std::memset(&h, 0, sizeof(h));
-        h.tol = o.tol;
-        h.maxIter = static_cast<uint32_t>(std::min<uint64_t>(o.max_iter, 0xffffffffu));
-    }
-    KT_HIP(hipMemcpyAsync(ctx->ctl.p, ctx->ctlHost, 2 * sizeof(AxisCtl), hipMemcpyHostToDevice, ctx->stream));
-    const uint32_t totalBlocks = args.sys[0].blocks + args.sys[1].blocks;
+    CgArgs a;
+    int G;
+    KT_TRY(prepare(ctx, o, false, a, G));
     res = kt_solve_result{};
-    if (totalBlocks == 0) return KT_OK;
-    const dim3 grid(totalBlocks);
-    KT_DISPATCH_G(G, cgStart, grid, ctx->stream, args);
-    KT_HIP(hipGetLastError());
-
+    if (a.sys[0].blocks + a.sys[1].blocks == 0) return KT_OK;
+    const bool timed = ctx->profile;
     const uint32_t every = o.check_every > 0 ? static_cast<uint32_t>(o.check_every) : 64;
+    KT_TRY(launchStart<double>(ctx, a, G));
+    uint64_t restarts = 0;
     for (int round = 0;; ++round) {
-        if (round > 0) {
-            KT_DISPATCH_G(G, cgStart, grid, ctx->stream, args);
-            KT_HIP(hipGetLastError());
-        }
+        if (round > 0) KT_TRY(launchStart<double>(ctx, a, G));
         double prevResid[2] = {0.0, 0.0};
         uint32_t prevSteps[2] = {0, 0};
         bool first = true;
         for (;;) {
-            KT_HIP(hipMemcpyAsync(ctx->ctlHost, ctx->ctl.p, 2 * sizeof(AxisCtl), hipMemcpyDeviceToHost, ctx->stream));
-            KT_HIP(hipStreamSynchronize(ctx->stream));
+            KT_TRY(readCtl(ctx));
             const AxisCtl *c = ctx->ctlHost;
-            if (!c[0].live && !c[1].live) break;
-            // Launch about as many iterations as the observed convergence rate
-            // says are left, so few no-op iterations run past convergence.
-            uint32_t chunk = first ? 8 : 1;
-            for (int a = 0; a < 2; ++a) {
-                if (!c[a].live) continue;
-                uint32_t need = every;
-                if (!first && c[a].steps > prevSteps[a] && prevResid[a] > c[a].resid && c[a].resid > 0.0) {
-                    const double rate = std::log(c[a].resid / prevResid[a]) / (c[a].steps - prevSteps[a]);
-                    const double left = std::log(o.tol / c[a].resid) / rate;
-                    need = static_cast<uint32_t>(std::min<double>(std::max(std::ceil(left) + 1.0, 1.0), every));
-                }
-                if (!first) chunk = std::max(chunk, need);
-                prevResid[a] = c[a].resid;
-                prevSteps[a] = c[a].steps;
+            if (!c[0].s[0].live && !c[1].s[0].live) break;
+            const uint32_t chunk = nextChunk(c, prevResid, prevSteps, o.tol, every, first);
+            for (int ax = 0; ax < 2; ++ax) {
+                prevResid[ax] = c[ax].s[0].resid;
+                prevSteps[ax] = c[ax].s[0].steps;
             }
             first = false;
-            for (uint32_t k = 0; k < chunk; ++k) {
-                hipEvent_t t0 = nullptr;
-                KT_TRY(timedBegin(ctx, KT_KERNEL_SPMV, &t0));
-                KT_DISPATCH_G(G, cgSpmv, grid, ctx->stream, args);
-                KT_HIP(hipGetLastError());
-                KT_TRY(timedEnd(ctx, KT_KERNEL_SPMV, t0));
-                KT_TRY(timedBegin(ctx, KT_KERNEL_UPDATE, &t0));
-                cgUpdate<<<grid, dim3(kBlock), 0, ctx->stream>>>(args);
-                KT_HIP(hipGetLastError());
-                KT_TRY(timedEnd(ctx, KT_KERNEL_UPDATE, t0));
-                KT_TRY(timedBegin(ctx, KT_KERNEL_PUPDATE, &t0));
-                cgPUpdate<<<grid, dim3(kBlock), 0, ctx->stream>>>(args);
-                KT_HIP(hipGetLastError());
-                KT_TRY(timedEnd(ctx, KT_KERNEL_PUPDATE, t0));
-            }
+            KT_TRY(runPairs(ctx, a, G, false, chunk / 2, timed));
         }
-        // true residual per axis
-        KT_DISPATCH_G(G, cgTrueResidual, grid, ctx->stream, args);
-        KT_HIP(hipGetLastError());
-        KT_HIP(hipMemcpyAsync(ctx->ctlHost, ctx->ctl.p, 2 * sizeof(AxisCtl), hipMemcpyDeviceToHost, ctx->stream));
-        KT_HIP(hipStreamSynchronize(ctx->stream));
+        KT_TRY(launchTrueResidual(ctx, a, G));
+        KT_TRY(readCtl(ctx));
         if (o.stop_rule != KT_STOP_TRUE || round >= 16) break;
         // True-residual rule: restart the axes whose true residual is still
         // above tol from their current iterate (r = b - A x recomputed).
         bool again = false;
-        for (int a = 0; a < 2; ++a) {
-            AxisCtl &c = ctx->ctlHost[a];
-            const bool more = c.active && c.trueResid > o.tol && c.steps < c.maxIter;
+        for (int ax = 0; ax < 2; ++ax) {
+            AxisCtl &c = ctx->ctlHost[ax];
+            const bool more = c.s[0].active && c.trueResid > o.tol && c.s[0].steps < c.maxIter;
             c.frozen = more ? 0 : 1;
             again = again || more;
         }
         if (!again) break;
-        ++res.refinements;
-        KT_HIP(hipMemcpyAsync(ctx->ctl.p, ctx->ctlHost, 2 * sizeof(AxisCtl), hipMemcpyHostToDevice, ctx->stream));
+        ++restarts;
+        KT_TRY(writeCtl(ctx));
     }
-    for (int a = 0; a < 2; ++a) {
-        res.resid[a] = ctx->ctlHost[a].resid;
-        res.true_resid[a] = ctx->ctlHost[a].trueResid;
-        res.steps[a] = ctx->ctlHost[a].steps;
+    fillResult(ctx, res);
+    res.refinements = restarts;
+    return KT_OK;
+}
+
+// FP32 PCG on FP32 copies of the matrix, with the iterate kept in FP64 and
+// the residual refreshed in FP64 (reliable updates): whenever the FP32
+// recurrence residual has fallen by the refresh factor since the last
+// refresh, or reached tol, the FP32 correction is added into x (FP64), the
+// residual is recomputed as b - A x in FP64 and rounded once to FP32, and the
+// FP64 residual norm becomes the axis' residual.  An axis stops only on the
+// true FP64 residual.  Every dot product (alpha, beta, residuals) accumulates
+// in FP64.
+int solveMixed(kt_ctx *ctx, const kt_solve_opts &o, kt_solve_result &res) {
+    CgArgs a;
+    int G;
+    KT_TRY(prepare(ctx, o, true, a, G));
+    res = kt_solve_result{};
+    if (a.sys[0].blocks + a.sys[1].blocks == 0) return KT_OK;
+    const bool timed = ctx->profile;
+    const uint32_t every = o.check_every > 0 ? static_cast<uint32_t>(o.check_every) : 64;
+    KT_TRY(launchStart<float>(ctx, a, G));
+    double prevResid[2] = {0.0, 0.0};
+    uint32_t prevSteps[2] = {0, 0};
+    bool first = true;
+    for (int guard = 0; guard < (1 << 24); ++guard) {
+        KT_TRY(readCtl(ctx));
+        const AxisCtl *c = ctx->ctlHost;
+        if (c[0].needUpdate || c[1].needUpdate) {
+            KT_TRY(launchRefresh(ctx, a, G));
+            first = true;
+            continue;
+        }
+        if (!c[0].s[0].live && !c[1].s[0].live) break;
+        double target = o.tol;
+        for (int ax = 0; ax < 2; ++ax)
+            if (c[ax].s[0].live) target = std::max(target, c[ax].refreshFactor * c[ax].refResid);
+        const uint32_t chunk = nextChunk(c, prevResid, prevSteps, target, every, first);
+        for (int ax = 0; ax < 2; ++ax) {
+            prevResid[ax] = c[ax].s[0].resid;
+            prevSteps[ax] = c[ax].s[0].steps;
+        }
+        first = false;
+        KT_TRY(runPairs(ctx, a, G, true, chunk / 2, timed));
     }
-    res.iterations = std::max(res.steps[0], res.steps[1]);
+    // fold the remaining correction into x and report the true residual
+    for (int ax = 0; ax < 2; ++ax) ctx->ctlHost[ax].needUpdate = 1;
+    KT_TRY(writeCtl(ctx));
+    KT_TRY(launchRefresh(ctx, a, G));
+    KT_TRY(readCtl(ctx));
+    fillResult(ctx, res);
+    res.refinements = ctx->ctlHost[0].refreshes + ctx->ctlHost[1].refreshes;
     return KT_OK;
 }
 

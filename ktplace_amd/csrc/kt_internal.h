@@ -64,27 +64,30 @@ struct DevBuf {
 
 constexpr uint32_t kNoVar = KT_NO_VAR;
 
-// Per-axis CG control block in device memory.  Written by the last block of
-// each reduction kernel; read by the following kernels.
+// CG state of one axis at the start of an iteration.  Double buffered by
+// iteration parity (AxisCtl::s[k & 1]).
+struct CgState {
+    double rho;     // r.z
+    double resid;   // ||r|| / ||b|| (recurrence; FP64 after a mixed-precision refresh)
+    uint32_t steps; // iterations this axis has stepped
+    int32_t live;   // steps this iteration: active && resid > tol && steps < maxIter
+    int32_t active; // the reference's Axis::active
+    int32_t pad;
+};
+
+// Per-axis control block in device memory.
 struct AxisCtl {
-    double bNorm;     // ||b||
-    double rho;       // r.z
-    double resid;     // recurrence residual ||r|| / ||b||
-    double alpha;
-    double beta;
-    double pap;
-    double rr;
-    double rz;
-    int32_t active;   // the reference's Axis::active
-    int32_t live;     // step this iteration: active && resid > tol && steps < maxIter
-    int32_t doUpdate; // p.Ap > 0 this iteration
-    int32_t doP;      // r.z > 0 this iteration
-    uint32_t steps;   // iterations stepped
-    uint32_t counter; // last-block-done counter (reset by the last block)
+    CgState s[2];
+    double bNorm;         // ||b||
     double tol;
+    double trueResid;     // ||b - A x|| / ||b|| from the last FP64 pass
+    double refResid;      // residual at the last mixed-precision refresh
+    double refreshFactor; // refresh once the FP32 residual falls by this factor
     uint32_t maxIter;
-    int32_t frozen;   // excluded from a restart (converged or broken down)
-    double trueResid; // ||b - A x|| / ||b|| from the last true-residual pass
+    int32_t brk[2];       // p.Ap <= 0 in the iteration of that parity
+    int32_t frozen;       // excluded from a restart (converged or broken down)
+    int32_t needUpdate;   // mixed precision: FP64 refresh requested
+    uint32_t refreshes;
 };
 
 // One axis' CSR as the CG kernels see it.
@@ -108,6 +111,45 @@ struct Sys {
     DevBuf<double> val, diag, rhs;
     DevBuf<float> val32, diag32;
     bool have32 = false;
+};
+
+// Arguments of every CG kernel (both axes).
+struct CgArgs {
+    SysView sys[2];
+    double *x64[2];     // FP64 iterate (the context's positions)
+    void *r[2];         // residual, working precision
+    void *p[2];         // search direction
+    void *ap[2];        // A p
+    void *e[2];         // FP32 correction to x64 (mixed precision), else null
+    double *partA;      // [axis][stride]    p.Ap per tile
+    double *partB;      // [axis][stride][2] r.r, r.z per tile
+    double *partS;      // [axis][stride][3] start partials
+    AxisCtl *ctl;       // [2]
+    double *history;    // [axis][histCap] residual after each step
+    uint32_t histCap;
+    uint32_t stride;    // partial slots per axis
+    uint32_t tile;      // rows per block
+    uint32_t xcdSplit;  // 1: blocks on XCDs 0-3 take axis x, 4-7 axis y
+};
+
+// Launch-shape knobs (set by the autotuner or the caller; 0 = default).
+struct Tune {
+    int lanes = 0;            // lanes per row in the SpMV (1..16)
+    int rows = 0;             // rows per lane group in flight in the SpMV (2, 4, 8)
+    uint32_t tile = 0;        // rows per block (multiple of 1024)
+    bool useGraphs = true;
+    int xcdSplit = -1;        // -1 default (on), 0 off, 1 on
+    double refreshFactor = 0; // mixed precision refresh factor (default 0.05)
+};
+
+// A captured hipGraph of kPairs iteration pairs for one argument set.
+struct GraphCache {
+    static constexpr int kPairs = 8;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    CgArgs args{};
+    int lanes = 0;
+    int rows = 0;
 };
 
 struct KernelTimer {
@@ -147,6 +189,9 @@ struct kt_ctx {
     kt::AxisCtl *ctlHost = nullptr; // pinned mirror
     kt::DevBuf<double> history;     // [2][maxIter]
     uint64_t historyCap = 0;
+
+    kt::Tune tune;
+    kt::GraphCache graphs[2];  // FP64, FP32 iteration graphs
 
     // timing
     bool profile = false;

@@ -70,8 +70,11 @@ _SIGS = {
     "kt_solve_both": (I32, [P, P, P, U64, C.POINTER(SolveOpts), OBSERVER, P,
                             C.POINTER(SolveResult)]),
     "kt_solve_both_resident": (I32, [P, C.POINTER(SolveOpts), C.POINTER(SolveResult)]),
+    "kt_set_tuning": (I32, [P, I32, U32, I32, F64]),
+    "kt_set_tuning2": (I32, [P, I32, U32, I32, F64, I32, I32]),
     "kt_kernel_times": (I32, [P, I32, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "kt_reset_kernel_times": (I32, [P]),
+    "kt_probe_copy": (I32, [P, U64, I32, C.POINTER(C.c_double)]),
 }
 
 EXPORTS = tuple(_SIGS)
@@ -262,11 +265,22 @@ class Context:
         _check(lib.kt_solve_both_resident(self.h, C.byref(opts), C.byref(res)))
         return res.as_dict()
 
+    def set_tuning(self, lanes: int = 0, tile: int = 0, use_graphs: bool = True,
+                   refresh_factor: float = 0.0, xcd_split: int = -1, rows: int = 0):
+        _check(lib.kt_set_tuning2(self.h, lanes, tile, int(use_graphs), refresh_factor, xcd_split,
+                                  rows))
+
     def kernel_time(self, kernel: int):
         n = C.c_uint64()
         ms = C.c_double()
         _check(lib.kt_kernel_times(self.h, kernel, C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    def probe_copy(self, nbytes: int, reps: int = 3) -> float:
+        """Milliseconds per plain device copy of nbytes (calibration helper)."""
+        ms = C.c_double()
+        _check(lib.kt_probe_copy(self.h, nbytes, reps, C.byref(ms)))
+        return ms.value
 
     def reset_kernel_times(self):
         _check(lib.kt_reset_kernel_times(self.h))

@@ -157,3 +157,70 @@ def test_cg_isolated_cell_floor(gpu_ctx_factory):
     ref = o.solve(x0, x0, tol=1e-10, max_iter=10000)
     assert rel(x, ref["x"]) < 1e-8 and x[7] == 0.0
     ctx.close()
+
+
+# ---------------------------------------------------------------- mixed precision
+def test_mixed_precision_reaches_true_tolerance(b2b12k, gpu_ctx_factory):
+    """FP32 PCG with FP64 refreshes: the true FP64 residual reaches tol and the
+    solution agrees with the tightly solved FP64 oracle."""
+    from ktplace_amd.simpl import KT_PREC_MIXED
+    d, o, x0, y0 = b2b12k
+    ctx = gpu_ctx_factory(d)
+    ctx.build_b2b(x0, y0)
+    ref = o.solve(x0, y0, tol=1e-12, max_iter=100000)
+    for tol in (1e-6, 1e-9):
+        x, y = x0.copy(), y0.copy()
+        r = ctx.solve_both(x, y, tol, 100000, precision=KT_PREC_MIXED)
+        print("mixed", tol, r, flush=True)
+        assert max(r["true_resid"]) <= tol
+        assert r["refinements"] >= 2
+        if tol <= 1e-9:
+            assert rel(x, ref["x"]) < 1e-6 and rel(y, ref["y"]) < 1e-6
+        # independent check of the reported residual with the oracle's SpMV
+        for axis, sol in ((0, x), (1, y)):
+            s = o.system(axis)
+            true = np.linalg.norm(s.rhs - o.multiply(axis, sol)) / np.linalg.norm(s.rhs)
+            assert true <= tol * 1.0001
+    ctx.close()
+
+
+@pytest.mark.parametrize("lanes,tile,graphs", [(1, 1024, False), (2, 2048, True), (8, 4096, True),
+                                               (16, 8192, False)])
+def test_launch_shapes_agree(b2b12k, gpu_ctx_factory, lanes, tile, graphs):
+    """Every tuning point solves the same system to the same answer."""
+    d, o, x0, y0 = b2b12k
+    ctx = gpu_ctx_factory(d)
+    ctx.set_tuning(lanes=lanes, tile=tile, use_graphs=graphs)
+    ctx.build_b2b(x0, y0)
+    x, y = x0.copy(), y0.copy()
+    r = ctx.solve_both(x, y, 1e-9, 100000)
+    ref = o.solve(x0, y0, tol=1e-12, max_iter=100000)
+    assert rel(x, ref["x"]) < 1e-6 and rel(y, ref["y"]) < 1e-6
+    assert max(r["true_resid"]) < 1e-8
+    ctx.close()
+
+
+def test_solve_is_deterministic(b2b12k, gpu_ctx_factory):
+    d, o, x0, y0 = b2b12k
+    ctx = gpu_ctx_factory(d)
+    ctx.build_b2b(x0, y0)
+    outs = []
+    for _ in range(2):
+        x, y = x0.copy(), y0.copy()
+        ctx.solve_both(x, y, 1e-7, 100000)
+        outs.append((x, y))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    ctx.close()
+
+
+@pytest.mark.parametrize("xcd", [0, 1])
+def test_xcd_split_agrees(b2b12k, gpu_ctx_factory, xcd):
+    d, o, x0, y0 = b2b12k
+    ctx = gpu_ctx_factory(d)
+    ctx.set_tuning(lanes=4, tile=2048, xcd_split=xcd)
+    ctx.build_b2b(x0, y0)
+    x, y = x0.copy(), y0.copy()
+    ctx.solve_both(x, y, 1e-9, 100000)
+    ref = o.solve(x0, y0, tol=1e-12, max_iter=100000)
+    assert rel(x, ref["x"]) < 1e-6 and rel(y, ref["y"]) < 1e-6
+    ctx.close()
