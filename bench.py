@@ -28,6 +28,9 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "B2B+PCG solve ms to 1e-6 rel residual @1M cells: mixed+learned vs plain FP64"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per SpMV launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
+# this workload, scaled by the copy-probe calibration (profiles/cgSpmv_cells1m.md)
+MEASURED_TRAFFIC = {"fp64": int((972.2 + 13.7) * 1e6), "mixed": int((455.4 + 14.4) * 1e6)}
 
 
 def log(msg: str) -> None:
@@ -164,7 +167,9 @@ def main() -> int:
             tuning={"lanes_per_row": tuned.lanes, "tile_rows": tuned.tile,
                     "xcd_split": tuned.xcd_split, "rows_in_flight": tuned.rows},
             roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                          frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                          frac=round(achieved / HBM_PEAK_GBS, 4),
+                          traffic=(MEASURED_TRAFFIC.get(path) if args.workload == "cells1m" else None),
+                          traffic_source="rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, cells1m, copy-probe scaled",
                           kernel=f"cgSpmv<{'double' if path == 'fp64' else 'float'}> (x and y per launch)",
                           launches=launches, avg_launch_us=round(avg_ms * 1e3, 2),
                           bytes_per_launch=int(bytes_per_launch)))
