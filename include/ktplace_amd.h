@@ -106,6 +106,8 @@ int kt_multiply(kt_ctx *ctx, int axis, const double *in, double *out);
 /* ---- solveBothAxes (simpl_lss.cc:203-290) -------------------------------- */
 #define KT_PREC_FP64 0   /* reference algorithm in FP64 */
 #define KT_PREC_MIXED 1  /* FP32 PCG with FP64 residual refinement */
+#define KT_PREC_FP64_EXACT 2 /* FP64 in the reference's exact operation order
+                                (bit-identical to the CPU solve; slow) */
 
 #define KT_STOP_RECURRENCE 0 /* reference rule: ||r_k|| / ||b|| <= tol */
 #define KT_STOP_TRUE 1       /* true FP64 ||b - A x|| / ||b|| <= tol */
@@ -135,6 +137,32 @@ typedef void (*kt_observer)(void *user, uint64_t iterate, double resid_x, double
 int kt_solve_both(kt_ctx *ctx, double *x, double *y, uint64_t count, const kt_solve_opts *opts,
                   kt_observer observe, void *user, kt_solve_result *out);
 int kt_solve_both_resident(kt_ctx *ctx, const kt_solve_opts *opts, kt_solve_result *out);
+
+/* ---- placement rows, bin grid and bookkeeping (simpl_placer.cc:181-297) -- */
+/* Rows (one subrow each: [x0, x1) at site pitch `pitch`) and the placement
+ * die box {xl, yl, xh, yh}.  Obstacles come with the netlist (kt_netlist). */
+int kt_set_rows(kt_ctx *ctx, uint32_t num_rows, const double *row_y, const double *row_height,
+                const double *row_x0, const double *row_x1, const double *row_pitch, const double die[4]);
+/* BinGrid of bins_per_axis^2 bins (0: the reference's default rule
+ * clamp(sqrt(n / 51), 16, 256), simpl_placer.cc:181-188) with the free area
+ * of every bin (:190-224); density_limit is clamped to [0.05, 1]. */
+int kt_build_grid(kt_ctx *ctx, uint64_t bins_per_axis, double density_limit, uint64_t *bins_out);
+int kt_grid_avail(kt_ctx *ctx, double *avail, double *total_avail);
+/* Of the context's positions: occupancy per bin (getOccupancy, :282-289),
+ * scaled overflow (getOverflow, :291-297) and HPWL (getHpwl, :256-275; same
+ * per-net values and the same chunked summation order as the reference). */
+int kt_occupancy(kt_ctx *ctx, double *occ);
+int kt_overflow(kt_ctx *ctx, double *overflow);
+int kt_hpwl(kt_ctx *ctx, double *hpwl);
+
+/* ---- LookAheadLegalizer::legalize (simpl_lal.cc:43-124) ------------------ */
+/* max_level / min_cells_to_split / stripe_area_fraction: simpl_options.h:24-26
+ * (20, 4, 0.1 by default; 0 keeps the current value). */
+int kt_set_lal_options(kt_ctx *ctx, uint64_t max_level, uint64_t min_cells_to_split,
+                       double stripe_area_fraction);
+/* Host-pointer form: x, y [n] in/out.  Resident form: the context's positions. */
+int kt_lal(kt_ctx *ctx, double *x, double *y);
+int kt_lal_resident(kt_ctx *ctx);
 
 /* ---- launch-shape tuning (0 keeps the default) --------------------------- */
 /* lanes: threads per CSR row in the SpMV (1, 2, 4, 8 or 16); tile: rows per

@@ -176,31 +176,32 @@ __global__ void __launch_bounds__(kBlock) rowFill(RowArgs a) {
 }
 
 // Inclusive in-place scan of one tile of kScanTile elements; tile totals go to sums.
+template <typename T>
 __global__ void __launch_bounds__(kBlock)
-scanTile(uint32_t *__restrict__ data, uint64_t m, uint32_t *__restrict__ sums) {
-    __shared__ uint32_t waveSums[kBlock / 64];
+scanTile(T *__restrict__ data, uint64_t m, T *__restrict__ sums) {
+    __shared__ T waveSums[kBlock / 64];
     const uint64_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    uint32_t v[kScanItems];
-    uint32_t s = 0;
+    T v[kScanItems];
+    T s = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const uint64_t idx = base + k;
-        s += idx < m ? data[idx] : 0u;
+        s += idx < m ? data[idx] : T(0);
         v[k] = s;
     }
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    uint32_t x = s;
+    T x = s;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off, 64);
+        const T y = __shfl_up(x, off, 64);
         if (lane >= off) x += y;
     }
     if (lane == 63) waveSums[wave] = x;
     __syncthreads();
-    uint32_t waveOffset = 0;
+    T waveOffset = 0;
     for (int w = 0; w < wave; ++w) waveOffset += waveSums[w];
-    const uint32_t excl = waveOffset + x - s;
+    const T excl = waveOffset + x - s;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const uint64_t idx = base + k;
@@ -209,10 +210,11 @@ scanTile(uint32_t *__restrict__ data, uint64_t m, uint32_t *__restrict__ sums) {
     if (threadIdx.x == kBlock - 1) sums[blockIdx.x] = waveOffset + x;
 }
 
+template <typename T>
 __global__ void __launch_bounds__(kBlock)
-scanAdd(uint32_t *__restrict__ data, uint64_t m, const uint32_t *__restrict__ sums) {
+scanAdd(T *__restrict__ data, uint64_t m, const T *__restrict__ sums) {
     if (blockIdx.x == 0) return;
-    const uint32_t add = sums[blockIdx.x - 1];
+    const T add = sums[blockIdx.x - 1];
     const uint64_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
@@ -221,16 +223,17 @@ scanAdd(uint32_t *__restrict__ data, uint64_t m, const uint32_t *__restrict__ su
     }
 }
 
-int scanLevel(kt_ctx *ctx, uint32_t *data, uint64_t m, uint64_t tmpOffset) {
+template <typename T>
+int scanLevel(kt_ctx *ctx, T *data, uint64_t m, T *tmp, uint64_t tmpCap) {
     if (m == 0) return KT_OK;
     const uint64_t blocks = (m + kScanTile - 1) / kScanTile;
-    KT_CHECK(tmpOffset + blocks <= ctx->scanTmp.cap, KT_EINVAL, "scan: scratch too small");
-    uint32_t *sums = ctx->scanTmp.p + tmpOffset;
-    scanTile<<<dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, ctx->stream>>>(data, m, sums);
+    KT_CHECK(blocks <= tmpCap, KT_EINVAL, "scan: scratch too small");
+    T *sums = tmp;
+    scanTile<T><<<dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, ctx->stream>>>(data, m, sums);
     KT_HIP(hipGetLastError());
     if (blocks > 1) {
-        KT_TRY(scanLevel(ctx, sums, blocks, tmpOffset + blocks));
-        scanAdd<<<dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, ctx->stream>>>(data, m, sums);
+        KT_TRY(scanLevel<T>(ctx, sums, blocks, tmp + blocks, tmpCap - blocks));
+        scanAdd<T><<<dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, ctx->stream>>>(data, m, sums);
         KT_HIP(hipGetLastError());
     }
     return KT_OK;
@@ -247,9 +250,32 @@ uint64_t scanScratch(uint64_t m) {
     return total + 1;
 }
 
+// Inclusive in-place scans of m values.
+int scanInclusive(kt_ctx *ctx, uint32_t *data, uint64_t m) {
+    const uint64_t cap = scanScratch(m);
+    KT_TRY(ctx->scanTmp.reserve(cap));
+    return scanLevel<uint32_t>(ctx, data, m, ctx->scanTmp.p, cap);
+}
+
+int scanInclusive64(kt_ctx *ctx, uint64_t *data, uint64_t m) {
+    const uint64_t cap = scanScratch(m);
+    KT_TRY(ctx->scanTmp64.reserve(cap));
+    return scanLevel<uint64_t>(ctx, data, m, ctx->scanTmp64.p, cap);
+}
+
 int scanRowPtr(kt_ctx *ctx, uint32_t *rowPtr, uint64_t rows) {
-    KT_TRY(ctx->scanTmp.reserve(scanScratch(rows)));
-    return scanLevel(ctx, rowPtr + 1, rows, 0);
+    return scanInclusive(ctx, rowPtr + 1, rows);
+}
+
+// px/py: fixed positions with the movable cells' positions written over
+int vertexPositionsLaunch(kt_ctx *ctx) {
+    KT_TRY(ctx->vx.reserve(std::max<uint32_t>(ctx->V, 1)));
+    KT_TRY(ctx->vy.reserve(std::max<uint32_t>(ctx->V, 1)));
+    if (ctx->V == 0) return KT_OK;
+    vertexPositions<<<dim3((ctx->V + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream>>>(
+        ctx->posX.p, ctx->posY.p, ctx->varOfVertex.p, ctx->fixedX.p, ctx->fixedY.p, ctx->vx.p, ctx->vy.p, ctx->V);
+    KT_HIP(hipGetLastError());
+    return KT_OK;
 }
 
 int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY) {
@@ -258,15 +284,8 @@ int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY) {
     KT_CHECK(!(alpha > 0.0) || ctx->haveAnchors, KT_ESTATE, "buildB2B: alpha > 0 needs anchors");
     const uint32_t n = ctx->n;
     for (int axis = 0; axis < 2; ++axis) KT_TRY(ensureSystem(ctx, axis, n, ctx->nnzBound));
-    KT_TRY(ctx->vx.reserve(ctx->V));
-    KT_TRY(ctx->vy.reserve(ctx->V));
-    KT_TRY(ctx->ext.reserve(4ull * ctx->N));
-    if (ctx->V > 0) {
-        vertexPositions<<<dim3((ctx->V + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream>>>(
-            ctx->posX.p, ctx->posY.p, ctx->varOfVertex.p, ctx->fixedX.p, ctx->fixedY.p, ctx->vx.p,
-            ctx->vy.p, ctx->V);
-        KT_HIP(hipGetLastError());
-    }
+    KT_TRY(ctx->ext.reserve(std::max<uint64_t>(4ull * ctx->N, 4)));
+    KT_TRY(vertexPositionsLaunch(ctx));
     uint4 *ext = reinterpret_cast<uint4 *>(ctx->ext.p);
     if (ctx->N > 0) {
         netExtremes<<<dim3((ctx->N + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream>>>(

@@ -1,6 +1,7 @@
 // C ABI entry points (include/ktplace_amd.h): context, netlist upload,
 // positions, SparseSystem access, solve wrapper and kernel timers.
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -23,8 +24,15 @@ int hipFail(hipError_t e, const char *what, const char *file, int line) {
 int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY);
 int solveFp64(kt_ctx *ctx, const kt_solve_opts &o, kt_solve_result &res);
 int solveMixed(kt_ctx *ctx, const kt_solve_opts &o, kt_solve_result &res);
+int solveExact(kt_ctx *ctx, const kt_solve_opts &o, kt_solve_result &res);
 int solveHistory(kt_ctx *ctx, const kt_solve_result &res, std::vector<double> &hist, uint64_t &cap);
 int multiply(kt_ctx *ctx, int axis, const double *dIn, double *dOut);
+int lalSetDesign(kt_ctx *ctx, const std::vector<lal::RowSpec> &rows, const lal::Box &die);
+int lalBuildGrid(kt_ctx *ctx, uint64_t perAxis, double densityLimit);
+int lalOccupancy(kt_ctx *ctx, std::vector<double> *occHost);
+int lalOverflow(kt_ctx *ctx, double *out);
+int lalHpwl(kt_ctx *ctx, double *out);
+int lalLegalize(kt_ctx *ctx);
 
 int ensureSystem(kt_ctx *ctx, int axis, uint64_t rows, uint64_t nnzCap) {
     Sys &s = ctx->sys[axis];
@@ -167,6 +175,11 @@ int uploadNetlist(kt_ctx *ctx, const kt_netlist &nl) {
     if (nl.width) KT_TRY(upload(ctx, ctx->width, nl.width, n));
     if (nl.height) KT_TRY(upload(ctx, ctx->height, nl.height, n));
     if (nl.area) KT_TRY(upload(ctx, ctx->area, nl.area, n));
+    ctx->obstacles.clear();
+    for (uint32_t o = 0; o < nl.num_obstacles; ++o) {
+        const double *b = nl.obstacles + 4ull * o;
+        ctx->obstacles.push_back({b[0], b[1], b[2], b[3]});
+    }
     KT_HIP(hipStreamSynchronize(ctx->stream));
     ctx->V = V;
     ctx->n = n;
@@ -215,15 +228,22 @@ int setAnchors(kt_ctx *ctx, const double *ax, const double *ay) {
 
 int solve(kt_ctx *ctx, const kt_solve_opts *opts, kt_solve_result *out) {
     KT_CHECK(opts && out, KT_EINVAL, "solve: null argument");
-    KT_CHECK(opts->precision == KT_PREC_FP64 || opts->precision == KT_PREC_MIXED, KT_EINVAL,
-             "solve: unknown precision");
+    KT_CHECK(opts->precision == KT_PREC_FP64 || opts->precision == KT_PREC_MIXED ||
+                 opts->precision == KT_PREC_FP64_EXACT,
+             KT_EINVAL, "solve: unknown precision");
     ctx->profile = opts->profile != 0;
     kt_solve_result res{};
     int rc;
     if (opts->precision == KT_PREC_FP64) {
         rc = solveFp64(ctx, *opts, res);
-    } else {
+    } else if (opts->precision == KT_PREC_MIXED) {
         rc = solveMixed(ctx, *opts, res);
+    } else {
+        for (int a = 0; a < 2; ++a) {
+            uint64_t nnz;
+            KT_TRY(systemNnz(ctx, a, &nnz));
+        }
+        rc = solveExact(ctx, *opts, res);
     }
     if (rc != KT_OK) return rc;
     if (ctx->profile) KT_TRY(collectTimers(ctx));
@@ -256,9 +276,9 @@ using namespace kt;
         if (kt_se_ != hipSuccess) return hipFail(kt_se_, "hipSetDevice", __FILE__, __LINE__); \
     } while (0)
 
-#define KT_CATCH(body)                                        \
+#define KT_CATCH(...)                                         \
     try {                                                     \
-        body                                                  \
+        __VA_ARGS__                                           \
     } catch (const std::bad_alloc &) {                        \
         setError("host out of memory");                       \
         return KT_ENOMEM;                                     \
@@ -287,6 +307,7 @@ int kt_ctx_create(kt_ctx **out, int device) {
         if (noGraphs && noGraphs[0] == '1') ctx->tune.useGraphs = false;
         hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&ctx->ctlHost), 2 * sizeof(AxisCtl));
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&ctx->pinned), 64 * sizeof(double));
         if (e != hipSuccess) {
             if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
             delete ctx;
@@ -324,6 +345,7 @@ int kt_ctx_destroy(kt_ctx *ctx) {
         if (g.graph) (void)hipGraphDestroy(g.graph);
     }
     if (ctx->ctlHost) (void)hipHostFree(ctx->ctlHost);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return KT_OK;
@@ -529,7 +551,17 @@ int kt_solve_both(kt_ctx *ctx, double *x, double *y, uint64_t count, const kt_so
         if (observe && res.iterations > 0) {
             std::vector<double> hist;
             uint64_t cap = 0;
-            KT_TRY(solveHistory(ctx, res, hist, cap));
+            if (opts->precision == KT_PREC_FP64_EXACT) {
+                // host history is [iteration][axis]; lay it out [axis][iteration]
+                cap = res.iterations;
+                hist.assign(2 * cap, 0.0);
+                for (uint64_t it = 0; it < cap; ++it) {
+                    hist[it] = ctx->exactHistory[2 * it];
+                    hist[cap + it] = ctx->exactHistory[2 * it + 1];
+                }
+            } else {
+                KT_TRY(solveHistory(ctx, res, hist, cap));
+            }
             const AxisCtl *c = ctx->ctlHost;
             for (uint64_t it = 0; it < res.iterations; ++it) {
                 double r[2];
@@ -549,6 +581,86 @@ int kt_solve_both(kt_ctx *ctx, double *x, double *y, uint64_t count, const kt_so
         }
         *out = res;
         return KT_OK;
+    })
+}
+
+int kt_set_rows(kt_ctx *ctx, uint32_t num_rows, const double *row_y, const double *row_height, const double *row_x0,
+                const double *row_x1, const double *row_pitch, const double die[4]) {
+    KT_GUARD(ctx);
+    KT_CHECK(die && (num_rows == 0 || (row_y && row_height && row_x0 && row_x1 && row_pitch)), KT_EINVAL,
+             "kt_set_rows: null array");
+    KT_CATCH({
+        std::vector<lal::RowSpec> rows(num_rows);
+        for (uint32_t r = 0; r < num_rows; ++r) rows[r] = {row_y[r], row_height[r], row_x0[r], row_x1[r], row_pitch[r]};
+        return lalSetDesign(ctx, rows, lal::Box{die[0], die[1], die[2], die[3]});
+    })
+}
+
+int kt_build_grid(kt_ctx *ctx, uint64_t bins_per_axis, double density_limit, uint64_t *bins_out) {
+    KT_GUARD(ctx);
+    uint64_t per = bins_per_axis;
+    if (per == 0)
+        per = static_cast<uint64_t>(std::clamp(std::sqrt(static_cast<double>(std::max<uint32_t>(ctx->n, 1)) / 51.0),
+                                               16.0, 256.0));
+    KT_CATCH(KT_TRY(lalBuildGrid(ctx, per, density_limit));)
+    if (bins_out) *bins_out = per;
+    return KT_OK;
+}
+
+int kt_grid_avail(kt_ctx *ctx, double *avail, double *total_avail) {
+    KT_GUARD(ctx);
+    KT_CHECK(ctx->grid.size() > 0, KT_ESTATE, "kt_grid_avail: no grid");
+    if (avail) std::copy(ctx->grid.avail.begin(), ctx->grid.avail.end(), avail);
+    if (total_avail) *total_avail = ctx->grid.totalAvail;
+    return KT_OK;
+}
+
+int kt_occupancy(kt_ctx *ctx, double *occ) {
+    KT_GUARD(ctx);
+    KT_CHECK(occ, KT_EINVAL, "kt_occupancy: null out");
+    KT_CHECK(ctx->posCount == ctx->n, KT_ESTATE, "kt_occupancy: positions must have one entry per movable cell");
+    KT_CATCH({
+        std::vector<double> o;
+        KT_TRY(lalOccupancy(ctx, &o));
+        std::copy(o.begin(), o.end(), occ);
+        return KT_OK;
+    })
+}
+
+int kt_overflow(kt_ctx *ctx, double *overflow) {
+    KT_GUARD(ctx);
+    KT_CHECK(overflow, KT_EINVAL, "kt_overflow: null out");
+    KT_CHECK(ctx->posCount == ctx->n, KT_ESTATE, "kt_overflow: positions must have one entry per movable cell");
+    KT_CATCH(return lalOverflow(ctx, overflow);)
+}
+
+int kt_hpwl(kt_ctx *ctx, double *hpwl) {
+    KT_GUARD(ctx);
+    KT_CHECK(hpwl, KT_EINVAL, "kt_hpwl: null out");
+    KT_CATCH(return lalHpwl(ctx, hpwl);)
+}
+
+int kt_set_lal_options(kt_ctx *ctx, uint64_t max_level, uint64_t min_cells_to_split, double stripe_area_fraction) {
+    KT_GUARD(ctx);
+    KT_CHECK(stripe_area_fraction >= 0.0, KT_EINVAL, "kt_set_lal_options: negative stripe fraction");
+    if (max_level) ctx->lalOpts.maxLevel = max_level;
+    if (min_cells_to_split) ctx->lalOpts.minCellsToSplit = min_cells_to_split;
+    if (stripe_area_fraction > 0.0) ctx->lalOpts.stripeAreaFraction = stripe_area_fraction;
+    return KT_OK;
+}
+
+int kt_lal_resident(kt_ctx *ctx) {
+    KT_GUARD(ctx);
+    KT_CATCH(return lalLegalize(ctx);)
+}
+
+int kt_lal(kt_ctx *ctx, double *x, double *y) {
+    KT_GUARD(ctx);
+    KT_CHECK(x && y, KT_EINVAL, "kt_lal: null positions");
+    KT_CATCH({
+        KT_TRY(setPositions(ctx, x, y, ctx->n));
+        KT_TRY(lalLegalize(ctx));
+        return getPositions(ctx, x, y, ctx->n);
     })
 }
 

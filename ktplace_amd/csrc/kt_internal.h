@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "ktplace_amd.h"
+#include "lal_geom.h"
 
 namespace kt {
 
@@ -152,6 +153,20 @@ struct GraphCache {
     int rows = 0;
 };
 
+// Per-block statistics of one LAL level (cells in the block's sorted order).
+struct LalSegStat {
+    uint32_t count;
+    uint32_t m;          // cells in the low half
+    uint64_t cellArea;   // fixed point, 2^-16 units
+    uint64_t lowArea;
+};
+
+struct LalOptions {
+    size_t maxLevel = 20;          // simpl_options.h:24
+    size_t minCellsToSplit = 4;    // :25
+    double stripeAreaFraction = 0.1;  // :26
+};
+
 struct KernelTimer {
     uint64_t launches = 0;
     double ms = 0.0;
@@ -177,7 +192,8 @@ struct kt_ctx {
     bool haveAnchors = false;
     kt::DevBuf<double> vx, vy;      // vertex-indexed positions (px, py)
     kt::DevBuf<uint32_t> ext;       // [N][4]: lo_x, hi_x, lo_y, hi_y
-    kt::DevBuf<uint32_t> scanTmp;   // block sums for the row-pointer scan
+    kt::DevBuf<uint32_t> scanTmp;   // block sums of the u32 scans
+    kt::DevBuf<uint64_t> scanTmp64; // block sums of the u64 scans
 
     kt::Sys sys[2];
 
@@ -187,8 +203,29 @@ struct kt_ctx {
     kt::DevBuf<double> partials;    // [2][maxBlocks][3]
     kt::DevBuf<kt::AxisCtl> ctl;    // [2]
     kt::AxisCtl *ctlHost = nullptr; // pinned mirror
+    double *pinned = nullptr;       // 64 pinned doubles for small host <-> device scalars
     kt::DevBuf<double> history;     // [2][maxIter]
     uint64_t historyCap = 0;
+
+    // look-ahead legalization: geometry (host), grid, per-cell scratch (device)
+    bool haveDesign = false;
+    std::vector<kt::lal::RowSpec> lalRows;
+    kt::lal::Box lalDie;
+    std::vector<kt::lal::Box> obstacles;
+    kt::lal::Grid grid;
+    kt::lal::FreeSpace freeSpace;
+    double densityLimit = 1.0;
+    kt::LalOptions lalOpts;
+    uint64_t lalBatches = 0;
+    kt::DevBuf<double> gridAvail, occ, lalScratch, lalSpan, lalCap, lalCut;
+    kt::DevBuf<uint64_t> occFix, lalKeys, lalKeysTmp, lalKey2, lalPrefix, lalStripeLo, lalStripeHi;
+    kt::DevBuf<uint32_t> binOf, binRegion, lalSeg, lalFlag, lalIds, lalVals, lalValsTmp, lalHalf, lalStripeOf,
+        lalBegin, lalEnd, lalStripeStart, lalStripeEnd, lalRemap, sortCounts, sortRaw;
+    kt::DevBuf<kt::LalSegStat> lalStats;
+    kt::DevBuf<uint8_t> lalSplit;
+
+    kt::DevBuf<double> exact;            // reference-order solver vectors
+    std::vector<double> exactHistory;    // its residuals per iteration
 
     kt::Tune tune;
     kt::GraphCache graphs[2];  // FP64, FP32 iteration graphs

@@ -1,0 +1,776 @@
+// Look-ahead legalization, HPWL, occupancy and overflow on gfx950.
+//
+// Follows LookAheadLegalizer::legalize (simpl_lal.cc:43-372) and the
+// bookkeeping of simplPlacer::Impl (simpl_placer.cc:256-297).
+//
+// Split of the work
+//   GPU  every per-cell operation: bin of each cell and the occupancy grid
+//        (fixed-point atomics, exact and order-free whenever cell areas are
+//        multiples of 2^-16), region membership, and for every level of the
+//        recursive bisection of every region of a batch at once: the sort of
+//        the cells of each block by (centre, id), the split at half the area,
+//        the second sort by distance from the cut, the greedy stripe filling,
+//        the per-stripe extent and the affine map; clamp to the die; HPWL
+//        with the reference's chunked summation order (bit-exact); overflow.
+//   Host the per-block geometry (free-area profile, cutline, stripes:
+//        lal_geom.cc), the flood fill of overfilled bins and the growth of
+//        clusters (summed-area tables), and the batch schedule.
+//
+// Level-synchronous recursion: all blocks of one level of all regions of a
+// batch are processed together.  Cells are kept as lists of ids ordered by
+// (block, key, id) with two stable radix sorts (key, then block) started from
+// id order, which reproduces the reference's total order (simpl_parallel.h:
+// 26-33).  Area prefix sums use 64-bit fixed point, so the split index and the
+// stripe boundaries are exact for areas that are multiples of 2^-16.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "kt_internal.h"
+#include "lal_geom.h"
+
+namespace kt {
+
+int radixSortPairs(kt_ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t *keysTmp, uint32_t *valsTmp, uint64_t n,
+                   int bits, DevBuf<uint32_t> &counts, DevBuf<uint32_t> &raw);
+int scanInclusive(kt_ctx *ctx, uint32_t *data, uint64_t m);
+int scanInclusive64(kt_ctx *ctx, uint64_t *data, uint64_t m);
+
+namespace {
+
+constexpr int kB = 256;
+constexpr double kFixScale = 65536.0;  // areas in units of 2^-16
+
+inline dim3 gridFor(uint64_t n) { return dim3(static_cast<uint32_t>(std::max<uint64_t>((n + kB - 1) / kB, 1))); }
+
+__device__ __forceinline__ uint64_t orderKey(double v) {
+    if (v == 0.0) v = 0.0;  // -0.0 and +0.0 compare equal
+    const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double fromOrderKey(uint64_t k) {
+    const uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double(static_cast<long long>(b));
+}
+
+struct GridDev {
+    double ox, oy, dx, dy;
+    uint32_t nx, ny;
+};
+
+// simpl_grid.h:120-126 clampIndex
+// (a NaN coordinate maps to bin 0 instead of an undefined conversion)
+__device__ __forceinline__ uint32_t clampIndex(double f, uint32_t n) {
+    const double hi = static_cast<double>(n) - 1e-9;
+    const double c = f >= 0.0 ? (hi < f ? hi : f) : 0.0;
+    return static_cast<uint32_t>(c);
+}
+
+__device__ __forceinline__ uint32_t binOfPoint(const GridDev &g, double x, double y) {
+    return clampIndex((y - g.oy) / g.dy, g.ny) * g.nx + clampIndex((x - g.ox) / g.dx, g.nx);
+}
+
+// bin of each cell centre (simpl_placer.cc:277-289) and fixed-point occupancy
+__global__ void __launch_bounds__(kB)
+cellBins(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ w,
+         const double *__restrict__ h, const double *__restrict__ area, GridDev g, uint32_t n,
+         uint32_t *__restrict__ binOf, unsigned long long *__restrict__ occFix) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = binOfPoint(g, x[i] + 0.5 * w[i], y[i] + 0.5 * h[i]);
+    binOf[i] = k;
+    atomicAdd(occFix + k, static_cast<unsigned long long>(llrint(area[i] * kFixScale)));
+}
+
+__global__ void __launch_bounds__(kB)
+fixToDouble(const unsigned long long *__restrict__ fix, double *__restrict__ out, uint32_t m) {
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k < m) out[k] = static_cast<double>(fix[k]) / kFixScale;
+}
+
+// overflow: sum of max(occ - g avail, 0), fixed tree per block, then in order
+__global__ void __launch_bounds__(kB)
+overflowPartials(const double *__restrict__ occ, const double *__restrict__ avail, double g, uint32_t m,
+                 double *__restrict__ part) {
+    __shared__ double s[kB];
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    const double ex = k < m ? occ[k] - g * avail[k] : 0.0;
+    s[threadIdx.x] = ex < 0.0 ? 0.0 : ex;
+    __syncthreads();
+    for (int off = kB / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ void sumInOrder(const double *__restrict__ part, uint32_t m, double *__restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double t = 0.0;
+    for (uint32_t k = 0; k < m; ++k) t += part[k];
+    *out = t;
+}
+
+// ------------------------------------------------------------------ HPWL
+// simpl_placer.cc:256-275: BBox of pos + off over the net's pins, 0 for nets
+// with fewer than two pins.
+__global__ void __launch_bounds__(kB)
+netHpwl(const uint32_t *__restrict__ netStart, const uint32_t *__restrict__ pinCell, const double *__restrict__ offX,
+        const double *__restrict__ offY, const double *__restrict__ vx, const double *__restrict__ vy, uint32_t N,
+        double *__restrict__ out) {
+    const uint32_t j = blockIdx.x * kB + threadIdx.x;
+    if (j >= N) return;
+    const uint32_t b = netStart[j], e = netStart[j + 1];
+    if (e - b < 2) {
+        out[j] = 0.0;
+        return;
+    }
+    double lx = INFINITY, ly = INFINITY, ux = -INFINITY, uy = -INFINITY;
+    for (uint32_t q = b; q < e; ++q) {
+        const uint32_t c = pinCell[q];
+        const double px = vx[c] + offX[q], py = vy[c] + offY[q];
+        // BBox::extend: std::min / std::max (kt_geometry.cc:141-146)
+        lx = px < lx ? px : lx;
+        ly = py < ly ? py : ly;
+        ux = ux < px ? px : ux;
+        uy = uy < py ? py : uy;
+    }
+    out[j] = (lx > ux || ly > uy) ? 0.0 : (ux - lx) + (uy - ly);
+}
+
+// getDeterministicSum (simpl_parallel.h:35-51): one thread per 4096-chunk.
+__global__ void __launch_bounds__(kB)
+chunkSums(const double *__restrict__ v, uint32_t n, double *__restrict__ part) {
+    const uint32_t c = blockIdx.x * kB + threadIdx.x;
+    const uint32_t chunks = (n + 4095) / 4096;
+    if (c >= chunks) return;
+    const uint32_t end = min(n, (c + 1) * 4096u);
+    double s = 0.0;
+    for (uint32_t i = c * 4096u; i < end; ++i) s += v[i];
+    part[c] = s;
+}
+
+// ------------------------------------------------------------------ LAL per-cell kernels
+__global__ void __launch_bounds__(kB)
+regionFlags(const uint32_t *__restrict__ binOf, const uint32_t *__restrict__ binRegion, uint32_t n,
+            uint32_t *__restrict__ seg, uint32_t *__restrict__ flag) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t r = binRegion[binOf[i]];
+    seg[i] = r;
+    flag[i] = r != kNoVar ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(kB)
+activeFlags(const uint32_t *__restrict__ seg, uint32_t n, uint32_t *__restrict__ flag) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i < n) flag[i] = seg[i] != kNoVar ? 1u : 0u;
+}
+
+// ids of flagged cells in id order (flag scanned inclusively into pos)
+__global__ void __launch_bounds__(kB)
+compactIds(const uint32_t *__restrict__ seg, const uint32_t *__restrict__ incl, uint32_t n,
+           uint32_t *__restrict__ ids) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n || seg[i] == kNoVar) return;
+    ids[incl[i] - 1] = i;
+}
+
+struct CentreArgs {
+    const double *pos;   // positions along the level's axis
+    const double *size;  // widths (x) or heights (y)
+};
+
+__device__ __forceinline__ double centreOf(const CentreArgs &c, uint32_t i) { return c.pos[i] + 0.5 * c.size[i]; }
+
+__global__ void __launch_bounds__(kB)
+centreKeys(const uint32_t *__restrict__ ids, uint32_t A, CentreArgs c, uint64_t *__restrict__ keys) {
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k < A) keys[k] = orderKey(centreOf(c, ids[k]));
+}
+
+__global__ void __launch_bounds__(kB)
+segmentKeys(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ seg, uint32_t A,
+            uint64_t *__restrict__ keys) {
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k < A) keys[k] = seg[ids[k]];
+}
+
+// segment ranges of a (segment, ...) sorted id list; begin/end pre-filled with 0
+__global__ void __launch_bounds__(kB)
+segmentBounds(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ seg, uint32_t A,
+              uint32_t *__restrict__ begin, uint32_t *__restrict__ end) {
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k >= A) return;
+    const uint32_t s = seg[ids[k]];
+    if (k == 0 || seg[ids[k - 1]] != s) begin[s] = k;
+    if (k + 1 == A || seg[ids[k + 1]] != s) end[s] = k + 1;
+}
+
+__global__ void __launch_bounds__(kB)
+areaFix(const uint32_t *__restrict__ ids, const double *__restrict__ area, uint32_t A, uint64_t *__restrict__ out) {
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k < A) out[k] = static_cast<uint64_t>(llrint(area[ids[k]] * kFixScale));
+}
+
+using SegStat = LalSegStat;
+
+// Split of each block at half its cell area (simpl_lal.cc:225-236), from the
+// inclusive fixed-point prefix P of the sorted areas.
+__global__ void __launch_bounds__(kB)
+splitBlocks(const uint32_t *__restrict__ begin, const uint32_t *__restrict__ end, const uint64_t *__restrict__ P,
+            uint32_t B, SegStat *__restrict__ out) {
+    const uint32_t s = blockIdx.x * kB + threadIdx.x;
+    if (s >= B) return;
+    const uint32_t b = begin[s], e = end[s];
+    SegStat st{};
+    st.count = e - b;
+    if (st.count > 0) {
+        const uint64_t before = b > 0 ? P[b - 1] : 0ull;
+        st.cellArea = P[e - 1] - before;
+        const double half = 0.5 * (static_cast<double>(st.cellArea) / kFixScale);
+        // smallest m >= 1 with sum of the first m areas >= half, at most count - 1
+        uint32_t lo = 1, hi = st.count > 1 ? st.count - 1 : 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) / 2;
+            const double low = static_cast<double>(P[b + mid - 1] - before) / kFixScale;
+            if (low < half) lo = mid + 1; else hi = mid;
+        }
+        st.m = st.count > 1 ? lo : 0;
+        st.lowArea = st.m > 0 ? P[b + st.m - 1] - before : 0ull;
+    }
+    out[s] = st;
+}
+
+struct HalfArgs {
+    const uint32_t *begin;     // per block, sorted by (block, centre, id)
+    const SegStat *stat;
+    const uint8_t *split;      // block splits this level
+    const double *cut;         // per block cutline
+    CentreArgs c;
+};
+
+// Half of each cell (low: first m cells of its block), key = -fromCut(centre)
+// (simpl_lal.cc:284-287); cells of blocks that do not split leave the level.
+__global__ void __launch_bounds__(kB)
+halfKeys(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ seg, uint32_t A, HalfArgs h,
+         uint32_t *__restrict__ half, uint64_t *__restrict__ key2) {
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k >= A) return;
+    const uint32_t i = ids[k];
+    const uint32_t s = seg[i];
+    if (!h.split[s]) {
+        half[i] = kNoVar;
+        return;
+    }
+    const bool low = (k - h.begin[s]) < h.stat[s].m;
+    const double c = centreOf(h.c, i);
+    const double cut = h.cut[s];
+    const double fromCut = low ? cut - c : c - cut;
+    half[i] = 2 * s + (low ? 0u : 1u);
+    key2[i] = orderKey(-fromCut);
+}
+
+__global__ void __launch_bounds__(kB)
+gatherKeys(const uint32_t *__restrict__ ids, const uint64_t *__restrict__ byCell, uint32_t A,
+           uint64_t *__restrict__ keys) {
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k < A) keys[k] = byCell[ids[k]];
+}
+
+struct StripeArgs {
+    const uint32_t *stripeStart;  // [halves + 1] stripes of each half, fill order
+    const double *span;           // [stripes][2] start, end
+    const double *cap;            // [stripes]
+    uint32_t *stripeEnd;          // [stripes] exclusive end index into the sorted list
+    unsigned long long *lo;       // [stripes] min centre (order key)
+    unsigned long long *hi;       // [stripes] max centre (order key)
+};
+
+// Greedy stripe filling (simpl_lal.cc:288-295): cells in order of distance
+// from the cut, a stripe takes cells while its used area is below capacity.
+// With the inclusive fixed-point prefix P over the sorted list, stripe s ends
+// at the first cell index k with (P[k-1] - P[start_s - 1]) >= cap_s; the last
+// stripe takes the rest.
+__global__ void __launch_bounds__(kB)
+stripeRanges(const uint32_t *__restrict__ begin, const uint32_t *__restrict__ end, const uint64_t *__restrict__ P,
+             uint32_t halves, StripeArgs sa) {
+    const uint32_t hh = blockIdx.x * kB + threadIdx.x;
+    if (hh >= halves) return;
+    const uint32_t s0 = sa.stripeStart[hh], s1 = sa.stripeStart[hh + 1];
+    if (s0 == s1) return;
+    const uint32_t b = begin[hh], e = end[hh];
+    uint32_t start = b;
+    for (uint32_t s = s0; s < s1; ++s) {
+        uint32_t stop = e;
+        if (s + 1 < s1) {
+            const uint64_t base = start > 0 ? P[start - 1] : 0ull;
+            const double cap = sa.cap[s];
+            // first k in [start, e) with used(k) = area of cells [start, k) >= cap
+            uint32_t lo = start, hiK = e;
+            while (lo < hiK) {
+                const uint32_t mid = (lo + hiK) / 2;
+                const uint64_t before = mid > 0 ? P[mid - 1] : 0ull;
+                const double used = static_cast<double>(before - base) / kFixScale;
+                if (used >= cap) hiK = mid; else lo = mid + 1;
+            }
+            stop = lo;
+        }
+        sa.stripeEnd[s] = stop;
+        start = stop;
+    }
+}
+
+__global__ void __launch_bounds__(kB)
+stripeOfCell(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ half, uint32_t A, StripeArgs sa,
+             CentreArgs c, uint32_t *__restrict__ stripeOf) {
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k >= A) return;
+    const uint32_t i = ids[k];
+    const uint32_t hh = half[i];
+    uint32_t s = sa.stripeStart[hh];
+    const uint32_t s1 = sa.stripeStart[hh + 1];
+    while (s + 1 < s1 && sa.stripeEnd[s] <= k) ++s;
+    stripeOf[k] = s;
+    const unsigned long long key = orderKey(centreOf(c, i));
+    atomicMin(sa.lo + s, key);
+    atomicMax(sa.hi + s, key);
+}
+
+// Affine map inside each stripe (simpl_lal.cc:297-314).
+__global__ void __launch_bounds__(kB)
+scaleCells(const uint32_t *__restrict__ ids, uint32_t A, const uint32_t *__restrict__ stripeOf, StripeArgs sa,
+           CentreArgs c, double *__restrict__ posOut) {
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k >= A) return;
+    const uint32_t i = ids[k];
+    const uint32_t s = stripeOf[k];
+    const double lo = fromOrderKey(sa.lo[s]), hi = fromOrderKey(sa.hi[s]);
+    const double ci = centreOf(c, i);
+    const double t = hi > lo ? (ci - lo) / (hi - lo) : 0.5;
+    const double size = c.size[i];
+    const double start = sa.span[2 * s], stop = sa.span[2 * s + 1];
+    const double cc = start + t * (stop - start);
+    const double upper = start < stop - size ? stop - size : start;
+    const double v = cc - 0.5 * size;
+    posOut[i] = v < start ? start : (upper < v ? upper : v);
+}
+
+// simpl_lal.cc:365-372
+__global__ void __launch_bounds__(kB)
+clampToDie(double *__restrict__ x, double *__restrict__ y, const double *__restrict__ w,
+           const double *__restrict__ h, double x0, double y0, double x1, double y1, uint32_t n) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    const double ux = x0 < x1 - w[i] ? x1 - w[i] : x0;
+    const double uy = y0 < y1 - h[i] ? y1 - h[i] : y0;
+    x[i] = x[i] < x0 ? x0 : (ux < x[i] ? ux : x[i]);
+    y[i] = y[i] < y0 ? y0 : (uy < y[i] ? uy : y[i]);
+}
+
+__global__ void __launch_bounds__(kB)
+remapSegments(const uint32_t *__restrict__ half, const uint32_t *__restrict__ remap, uint32_t n,
+              uint32_t *__restrict__ seg) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t h = half[i];
+    seg[i] = h == kNoVar ? kNoVar : remap[h];
+}
+
+__global__ void __launch_bounds__(kB) fillU64(unsigned long long *p, unsigned long long v, uint64_t n) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(kB) + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void __launch_bounds__(kB) fillU32(uint32_t *p, uint32_t v, uint64_t n) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(kB) + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <typename F>
+void parallelFor(size_t n, F f) {
+    const size_t threads = std::min<size_t>(std::max(1u, std::thread::hardware_concurrency()), 16);
+    if (n < 64 || threads <= 1) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    std::atomic<size_t> next{0};
+    for (size_t t = 0; t < threads; ++t)
+        pool.emplace_back([&] {
+            for (size_t i; (i = next.fetch_add(16)) < n;)
+                for (size_t k = i; k < std::min(n, i + 16); ++k) f(k);
+        });
+    for (auto &th : pool) th.join();
+}
+
+int bitsFor(uint64_t v) {
+    int b = 0;
+    while (b < 64 && (v >> b) != 0) ++b;
+    return std::max(b, 1);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host entry points
+int lalSetDesign(kt_ctx *ctx, const std::vector<lal::RowSpec> &rows, const lal::Box &die) {
+    ctx->lalRows = rows;
+    ctx->lalDie = die;
+    ctx->haveDesign = true;
+    return KT_OK;
+}
+
+int lalBuildGrid(kt_ctx *ctx, uint64_t perAxis, double densityLimit) {
+    KT_CHECK(ctx->haveNetlist && ctx->haveDesign, KT_ESTATE, "build_grid: netlist and rows needed");
+    KT_CHECK(perAxis > 0 && perAxis <= 4096, KT_EINVAL, "build_grid: bins per axis out of range");
+    lal::buildGrid(ctx->grid, ctx->freeSpace, ctx->lalRows, ctx->obstacles, ctx->lalDie, perAxis);
+    ctx->densityLimit = std::clamp(densityLimit, 0.05, 1.0);
+    KT_TRY(ctx->gridAvail.reserve(ctx->grid.size()));
+    KT_HIP(hipMemcpyAsync(ctx->gridAvail.p, ctx->grid.avail.data(), ctx->grid.size() * sizeof(double),
+                          hipMemcpyHostToDevice, ctx->stream));
+    KT_HIP(hipStreamSynchronize(ctx->stream));
+    return KT_OK;
+}
+
+namespace {
+
+GridDev gridDev(const lal::Grid &g) {
+    return GridDev{g.ox, g.oy, g.dx, g.dy, static_cast<uint32_t>(g.nx), static_cast<uint32_t>(g.ny)};
+}
+
+}  // namespace
+
+// binOf and occupancy of the context's positions (device), occupancy copied
+// to host `occ` when non-null.
+int lalOccupancy(kt_ctx *ctx, std::vector<double> *occHost) {
+    KT_CHECK(ctx->grid.size() > 0 && ctx->gridAvail.cap >= ctx->grid.size(), KT_ESTATE, "occupancy: no grid");
+    const uint32_t n = ctx->n;
+    const uint64_t m = ctx->grid.size();
+    KT_TRY(ctx->binOf.reserve(std::max<uint32_t>(n, 1)));
+    KT_TRY(ctx->occFix.reserve(m));
+    KT_TRY(ctx->occ.reserve(m));
+    KT_HIP(hipMemsetAsync(ctx->occFix.p, 0, m * sizeof(uint64_t), ctx->stream));
+    if (n) {
+        cellBins<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->posX.p, ctx->posY.p, ctx->width.p, ctx->height.p,
+                                                     ctx->area.p, gridDev(ctx->grid), n, ctx->binOf.p,
+                                                     reinterpret_cast<unsigned long long *>(ctx->occFix.p));
+        KT_HIP(hipGetLastError());
+    }
+    fixToDouble<<<gridFor(m), kB, 0, ctx->stream>>>(reinterpret_cast<unsigned long long *>(ctx->occFix.p),
+                                                     ctx->occ.p, static_cast<uint32_t>(m));
+    KT_HIP(hipGetLastError());
+    if (occHost) {
+        occHost->resize(m);
+        KT_HIP(hipMemcpyAsync(occHost->data(), ctx->occ.p, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KT_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return KT_OK;
+}
+
+int lalOverflow(kt_ctx *ctx, double *out) {
+    KT_TRY(lalOccupancy(ctx, nullptr));
+    const uint64_t m = ctx->grid.size();
+    const uint32_t blocks = static_cast<uint32_t>((m + kB - 1) / kB);
+    KT_TRY(ctx->lalScratch.reserve(blocks + 1));
+    overflowPartials<<<blocks, kB, 0, ctx->stream>>>(ctx->occ.p, ctx->gridAvail.p, ctx->densityLimit,
+                                                     static_cast<uint32_t>(m), ctx->lalScratch.p);
+    KT_HIP(hipGetLastError());
+    sumInOrder<<<1, 64, 0, ctx->stream>>>(ctx->lalScratch.p, blocks, ctx->lalScratch.p + blocks);
+    KT_HIP(hipGetLastError());
+    double excess = 0.0;
+    KT_HIP(hipMemcpyAsync(&excess, ctx->lalScratch.p + blocks, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KT_HIP(hipStreamSynchronize(ctx->stream));
+    *out = ctx->grid.totalAvail > 0.0 ? excess / ctx->grid.totalAvail : 0.0;
+    return KT_OK;
+}
+
+int vertexPositionsLaunch(kt_ctx *ctx);
+
+int lalHpwl(kt_ctx *ctx, double *out) {
+    KT_CHECK(ctx->haveNetlist, KT_ESTATE, "hpwl: no netlist");
+    KT_CHECK(ctx->posCount == ctx->n, KT_ESTATE, "hpwl: positions must have one entry per movable cell");
+    KT_TRY(vertexPositionsLaunch(ctx));
+    const uint32_t N = ctx->N;
+    const uint32_t chunks = (N + 4095) / 4096;
+    KT_TRY(ctx->lalScratch.reserve(static_cast<uint64_t>(N) + chunks + 1));
+    double *perNet = ctx->lalScratch.p;
+    double *part = perNet + N;
+    if (N) {
+        netHpwl<<<gridFor(N), kB, 0, ctx->stream>>>(ctx->netStart.p, ctx->pinCell.p, ctx->offX.p, ctx->offY.p,
+                                                    ctx->vx.p, ctx->vy.p, N, perNet);
+        KT_HIP(hipGetLastError());
+        chunkSums<<<gridFor(chunks), kB, 0, ctx->stream>>>(perNet, N, part);
+        KT_HIP(hipGetLastError());
+    }
+    sumInOrder<<<1, 64, 0, ctx->stream>>>(part, chunks, part + chunks);
+    KT_HIP(hipGetLastError());
+    double h = 0.0;
+    KT_HIP(hipMemcpyAsync(&h, part + chunks, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KT_HIP(hipStreamSynchronize(ctx->stream));
+    *out = h;
+    return KT_OK;
+}
+
+// One batch of non-touching regions (simpl_lal.cc:122-151, 178-316).
+int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std::vector<double> &occHost) {
+    const uint32_t n = ctx->n;
+    const lal::Grid &g = ctx->grid;
+    if (regions.empty() || n == 0) return KT_OK;
+    // bin -> region index
+    std::vector<uint32_t> binRegion(g.size(), kNoVar);
+    struct Block {
+        lal::Box box;
+        double regionRoom;
+    };
+    std::vector<Block> blocks(regions.size());
+    for (size_t r = 0; r < regions.size(); ++r) {
+        const lal::BinRect &b = regions[r];
+        for (size_t iy = b.y0; iy <= b.y1; ++iy)
+            for (size_t ix = b.x0; ix <= b.x1; ++ix) binRegion[g.at(ix, iy)] = static_cast<uint32_t>(r);
+        blocks[r] = {g.box(b), g.sum(g.avail, b)};
+    }
+    KT_TRY(ctx->binRegion.reserve(g.size()));
+    KT_HIP(hipMemcpyAsync(ctx->binRegion.p, binRegion.data(), g.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          ctx->stream));
+    // per-cell scratch
+    for (auto *b : {&ctx->lalSeg, &ctx->lalFlag, &ctx->lalIds, &ctx->lalVals, &ctx->lalValsTmp, &ctx->lalHalf,
+                    &ctx->lalStripeOf})
+        KT_TRY(b->reserve(n));
+    for (auto *b : {&ctx->lalKeys, &ctx->lalKeysTmp, &ctx->lalKey2, &ctx->lalPrefix}) KT_TRY(b->reserve(n));
+    regionFlags<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->binOf.p, ctx->binRegion.p, n, ctx->lalSeg.p, ctx->lalFlag.p);
+    KT_HIP(hipGetLastError());
+
+    bool vertical = true;
+    uint32_t A = 0;
+    for (size_t level = 1;; ++level) {
+        // active cells in id order
+        activeFlags<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalSeg.p, n, ctx->lalFlag.p);
+        KT_HIP(hipGetLastError());
+        KT_TRY(scanInclusive(ctx, ctx->lalFlag.p, n));
+        KT_HIP(hipMemcpyAsync(&A, ctx->lalFlag.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        KT_HIP(hipStreamSynchronize(ctx->stream));
+        if (A == 0) break;
+        compactIds<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalSeg.p, ctx->lalFlag.p, n, ctx->lalIds.p);
+        KT_HIP(hipGetLastError());
+        const uint32_t B = static_cast<uint32_t>(blocks.size());
+        const CentreArgs ca{vertical ? ctx->posX.p : ctx->posY.p, vertical ? ctx->width.p : ctx->height.p};
+        // sort by (block, centre, id): stable sorts by centre, then by block
+        centreKeys<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, A, ca, ctx->lalKeys.p);
+        KT_HIP(hipGetLastError());
+        KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A, 64,
+                              ctx->sortCounts, ctx->sortRaw));
+        segmentKeys<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalSeg.p, A, ctx->lalKeys.p);
+        KT_HIP(hipGetLastError());
+        KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A,
+                              bitsFor(B), ctx->sortCounts, ctx->sortRaw));
+        // block ranges, area prefix, split
+        KT_TRY(ctx->lalBegin.reserve(2ull * B + 2));
+        KT_TRY(ctx->lalEnd.reserve(2ull * B + 2));
+        fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalBegin.p, 0u, 2ull * B + 2);
+        fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalEnd.p, 0u, 2ull * B + 2);
+        segmentBounds<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalSeg.p, A, ctx->lalBegin.p,
+                                                          ctx->lalEnd.p);
+        areaFix<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->area.p, A, ctx->lalPrefix.p);
+        KT_TRY(scanInclusive64(ctx, ctx->lalPrefix.p, A));
+        KT_TRY(ctx->lalStats.reserve(B));
+        splitBlocks<<<gridFor(B), kB, 0, ctx->stream>>>(ctx->lalBegin.p, ctx->lalEnd.p, ctx->lalPrefix.p, B,
+                                                        ctx->lalStats.p);
+        KT_HIP(hipGetLastError());
+        std::vector<SegStat> stats(B);
+        KT_HIP(hipMemcpyAsync(stats.data(), ctx->lalStats.p, B * sizeof(SegStat), hipMemcpyDeviceToHost, ctx->stream));
+        KT_HIP(hipStreamSynchronize(ctx->stream));
+
+        // host geometry for every splitting block (simpl_lal.cc:213-263)
+        std::vector<uint8_t> split(B, 0);
+        std::vector<double> cut(B, 0.0);
+        std::vector<std::vector<lal::Stripe>> halfStripes(2ull * B);
+        std::vector<Block> next(2ull * B);
+        const lal::FreeSpace &fs = ctx->freeSpace;
+        const double g2 = ctx->densityLimit;
+        const double frac = ctx->lalOpts.stripeAreaFraction;
+        parallelFor(B, [&](size_t s) {
+            const SegStat &st = stats[s];
+            if (level >= ctx->lalOpts.maxLevel || st.count <= ctx->lalOpts.minCellsToSplit) return;
+            split[s] = 1;
+            const Block &bl = blocks[s];
+            const double cellArea = static_cast<double>(st.cellArea) / kFixScale;
+            const double lowArea = static_cast<double>(st.lowArea) / kFixScale;
+            const double lo = vertical ? bl.box.x0 : bl.box.y0;
+            const double hi = vertical ? bl.box.x1 : bl.box.y1;
+            const lal::Profile room = fs.profile(bl.box, vertical);
+            const double share = cellArea > 0.0 ? lowArea / cellArea : 0.5;
+            const double cutline = room.total() > 0.0 ? room.coordinateAt(share * room.total()) : lo + share * (hi - lo);
+            cut[s] = cutline;
+            lal::Box lowBox = bl.box, highBox = bl.box;
+            (vertical ? lowBox.x1 : lowBox.y1) = cutline;
+            (vertical ? highBox.x0 : highBox.y0) = cutline;
+            const double highArea = static_cast<double>(st.cellArea - st.lowArea) / kFixScale;
+            if (st.m > 0)
+                halfStripes[2 * s] = lal::stripes(fs, ctx->obstacles, lowBox, vertical, lowArea, bl.regionRoom, g2,
+                                                  frac, cutline, true);
+            if (st.count > st.m)
+                halfStripes[2 * s + 1] = lal::stripes(fs, ctx->obstacles, highBox, vertical, highArea, bl.regionRoom,
+                                                      g2, frac, cutline, false);
+            // a zero-length half box yields no stripe; give its cells one
+            // stripe at the box edge so they stay inside it
+            if (st.m > 0 && halfStripes[2 * s].empty())
+                halfStripes[2 * s].push_back({vertical ? lowBox.x0 : lowBox.y0, vertical ? lowBox.x1 : lowBox.y1, 0.0});
+            if (st.count > st.m && halfStripes[2 * s + 1].empty())
+                halfStripes[2 * s + 1].push_back(
+                    {vertical ? highBox.x0 : highBox.y0, vertical ? highBox.x1 : highBox.y1, 0.0});
+            next[2 * s] = {lowBox, bl.regionRoom};
+            next[2 * s + 1] = {highBox, bl.regionRoom};
+        });
+        uint32_t splits = 0;
+        for (uint8_t f : split) splits += f;
+        if (splits == 0) break;
+        // stripe tables
+        const uint32_t H = 2 * B;
+        std::vector<uint32_t> stripeStart(H + 1, 0);
+        for (uint32_t h = 0; h < H; ++h) stripeStart[h + 1] = stripeStart[h] + halfStripes[h].size();
+        const uint32_t S = stripeStart[H];
+        std::vector<double> span(2ull * S + 2), cap(S + 1);
+        for (uint32_t h = 0; h < H; ++h)
+            for (size_t q = 0; q < halfStripes[h].size(); ++q) {
+                const uint32_t s = stripeStart[h] + static_cast<uint32_t>(q);
+                span[2 * s] = halfStripes[h][q].start;
+                span[2 * s + 1] = halfStripes[h][q].end;
+                cap[s] = halfStripes[h][q].capacity;
+            }
+        KT_TRY(ctx->lalSplit.reserve(B));
+        KT_TRY(ctx->lalCut.reserve(B));
+        KT_TRY(ctx->lalStripeStart.reserve(H + 1));
+        KT_TRY(ctx->lalSpan.reserve(2ull * S + 2));
+        KT_TRY(ctx->lalCap.reserve(S + 1));
+        KT_TRY(ctx->lalStripeEnd.reserve(S + 1));
+        KT_TRY(ctx->lalStripeLo.reserve(S + 1));
+        KT_TRY(ctx->lalStripeHi.reserve(S + 1));
+        KT_HIP(hipMemcpyAsync(ctx->lalSplit.p, split.data(), B, hipMemcpyHostToDevice, ctx->stream));
+        KT_HIP(hipMemcpyAsync(ctx->lalCut.p, cut.data(), B * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        KT_HIP(hipMemcpyAsync(ctx->lalStripeStart.p, stripeStart.data(), (H + 1) * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, ctx->stream));
+        KT_HIP(hipMemcpyAsync(ctx->lalSpan.p, span.data(), span.size() * sizeof(double), hipMemcpyHostToDevice,
+                              ctx->stream));
+        KT_HIP(hipMemcpyAsync(ctx->lalCap.p, cap.data(), cap.size() * sizeof(double), hipMemcpyHostToDevice,
+                              ctx->stream));
+        // halves and the second order: by (half, -fromCut(centre), id)
+        HalfArgs ha{ctx->lalBegin.p, ctx->lalStats.p, ctx->lalSplit.p, ctx->lalCut.p, ca};
+        fillU32<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalHalf.p, kNoVar, n);
+        halfKeys<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalSeg.p, A, ha, ctx->lalHalf.p,
+                                                     ctx->lalKey2.p);
+        KT_HIP(hipGetLastError());
+        // cells of non-splitting blocks leave; the rest are ordered by half
+        activeFlags<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalHalf.p, n, ctx->lalFlag.p);
+        KT_HIP(hipGetLastError());
+        KT_TRY(scanInclusive(ctx, ctx->lalFlag.p, n));
+        uint32_t A2 = 0;
+        KT_HIP(hipMemcpyAsync(&A2, ctx->lalFlag.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        KT_HIP(hipStreamSynchronize(ctx->stream));
+        if (A2 == 0) break;
+        compactIds<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalHalf.p, ctx->lalFlag.p, n, ctx->lalIds.p);
+        gatherKeys<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalKey2.p, A2, ctx->lalKeys.p);
+        KT_HIP(hipGetLastError());
+        KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A2, 64,
+                              ctx->sortCounts, ctx->sortRaw));
+        segmentKeys<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalHalf.p, A2, ctx->lalKeys.p);
+        KT_HIP(hipGetLastError());
+        KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A2,
+                              bitsFor(H), ctx->sortCounts, ctx->sortRaw));
+        fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalBegin.p, 0u, 2ull * B + 2);
+        fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalEnd.p, 0u, 2ull * B + 2);
+        segmentBounds<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalHalf.p, A2, ctx->lalBegin.p,
+                                                           ctx->lalEnd.p);
+        areaFix<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->area.p, A2, ctx->lalPrefix.p);
+        KT_TRY(scanInclusive64(ctx, ctx->lalPrefix.p, A2));
+        StripeArgs sa{ctx->lalStripeStart.p, ctx->lalSpan.p, ctx->lalCap.p, ctx->lalStripeEnd.p,
+                      reinterpret_cast<unsigned long long *>(ctx->lalStripeLo.p),
+                      reinterpret_cast<unsigned long long *>(ctx->lalStripeHi.p)};
+        fillU64<<<gridFor(S + 1), kB, 0, ctx->stream>>>(sa.lo, ~0ull, S + 1);
+        fillU64<<<gridFor(S + 1), kB, 0, ctx->stream>>>(sa.hi, 0ull, S + 1);
+        stripeRanges<<<gridFor(H), kB, 0, ctx->stream>>>(ctx->lalBegin.p, ctx->lalEnd.p, ctx->lalPrefix.p, H, sa);
+        KT_HIP(hipGetLastError());
+        stripeOfCell<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalHalf.p, A2, sa, ca,
+                                                          ctx->lalStripeOf.p);
+        KT_HIP(hipGetLastError());
+        scaleCells<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, A2, ctx->lalStripeOf.p, sa, ca,
+                                                        const_cast<double *>(ca.pos));
+        KT_HIP(hipGetLastError());
+        // next level: the halves that hold cells, renumbered densely
+        std::vector<uint32_t> remap(H, kNoVar);
+        std::vector<Block> kept;
+        for (uint32_t s = 0; s < B; ++s) {
+            if (!split[s]) continue;
+            if (stats[s].m > 0) {
+                remap[2 * s] = static_cast<uint32_t>(kept.size());
+                kept.push_back(next[2 * s]);
+            }
+            if (stats[s].count > stats[s].m) {
+                remap[2 * s + 1] = static_cast<uint32_t>(kept.size());
+                kept.push_back(next[2 * s + 1]);
+            }
+        }
+        KT_TRY(ctx->lalRemap.reserve(H));
+        KT_HIP(hipMemcpyAsync(ctx->lalRemap.p, remap.data(), H * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        remapSegments<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalHalf.p, ctx->lalRemap.p, n, ctx->lalSeg.p);
+        KT_HIP(hipGetLastError());
+        blocks.swap(kept);
+        vertical = !vertical;
+    }
+    return KT_OK;
+}
+
+// LookAheadLegalizer::legalize (simpl_lal.cc:43-124) on the context's positions.
+int lalLegalize(kt_ctx *ctx) {
+    KT_CHECK(ctx->haveNetlist && ctx->grid.size() > 0, KT_ESTATE, "lal: build the grid first");
+    KT_CHECK(ctx->posCount == ctx->n, KT_ESTATE, "lal: positions must have one entry per movable cell");
+    const lal::Grid &g = ctx->grid;
+    std::vector<double> occ;
+    KT_TRY(lalOccupancy(ctx, &occ));
+    lal::AreaTables tables;
+    tables.build(g, occ);
+    const auto reachOf = [&](const lal::BinRect &r) {
+        return lal::BinRect{r.x0 > 0 ? r.x0 - 1 : 0, std::min(r.x1 + 1, g.nx - 1), r.y0 > 0 ? r.y0 - 1 : 0,
+                            std::min(r.y1 + 1, g.ny - 1)};
+    };
+    const auto touches = [](const lal::BinRect &a, const lal::BinRect &b) {
+        return a.x0 <= b.x1 && b.x0 <= a.x1 && a.y0 <= b.y1 && b.y0 <= a.y1;
+    };
+    std::vector<lal::BinRect> batch, reach;
+    uint64_t batches = 0;
+    const auto runBatch = [&]() -> int {
+        if (batch.empty()) return KT_OK;
+        KT_TRY(lalRunBatch(ctx, batch, occ));
+        ++batches;
+        batch.clear();
+        reach.clear();
+        KT_TRY(lalOccupancy(ctx, &occ));
+        tables.build(g, occ);
+        return KT_OK;
+    };
+    for (const lal::BinRect &cluster : lal::overfilledClusters(g, occ, ctx->densityLimit)) {
+        lal::BinRect bins = lal::grow(g, tables, occ, cluster, ctx->densityLimit);
+        bool clash = false;
+        for (const lal::BinRect &r : reach) clash = clash || touches(r, reachOf(bins));
+        if (clash) {
+            KT_TRY(runBatch());
+            bins = lal::grow(g, tables, occ, cluster, ctx->densityLimit);
+        }
+        batch.push_back(bins);
+        reach.push_back(reachOf(bins));
+    }
+    KT_TRY(runBatch());
+    const uint32_t n = ctx->n;
+    if (n) {
+        clampToDie<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->posX.p, ctx->posY.p, ctx->width.p, ctx->height.p,
+                                                       ctx->lalDie.x0, ctx->lalDie.y0, ctx->lalDie.x1, ctx->lalDie.y1, n);
+        KT_HIP(hipGetLastError());
+    }
+    ctx->lalBatches = batches;
+    return KT_OK;
+}
+
+}  // namespace kt

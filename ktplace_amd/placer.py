@@ -1,0 +1,150 @@
+"""SimPL global placement driven from the host, every step on the GPU.
+
+Mirrors simplPlacer::Impl::run (simpl_placer.cc:358-580): seedUniform, initial
+rounds (B2B, or the star model when the context supports it), then outer
+iterations of look-ahead legalization (upper bound), HPWL bookkeeping, the
+convergence test, B2B assembly with anchors and the Jacobi-PCG solve (lower
+bound), and finally the refinement on a finer grid.  Options and their
+defaults follow SimplOptions (simpl_options.h:11-58).  Fences, frames and
+reports are not part of this path.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import simpl
+from .design import Design, seed_uniform
+
+
+@dataclass
+class SimplOptions:
+    binsPerAxis: int = 0
+    cgTol: float = 1e-4
+    initMaxIters: int = 15
+    initialNetModel: str = "b2b"   # the reference defaults to "star"
+    maxIters: int = 100
+    seed: int = 20240607
+    densityLimit: float = 1.0
+    alphaBase: float = 0.02
+    maxLevel: int = 20
+    minCellsToSplit: int = 4
+    stripeAreaFraction: float = 0.1
+    gapReferenceIter: int = 10
+    gapRelaxedFrac: float = 0.25
+    gapTightFrac: float = 0.05
+    stallIters: int = 6
+    refineIters: int = 10
+    refineGridFactor: int = 3
+    cgMaxIter: int = 60
+    precision: int = simpl.KT_PREC_FP64
+
+
+@dataclass
+class PlaceResult:
+    x: np.ndarray
+    y: np.ndarray
+    stats: dict
+    trace: list = field(default_factory=list)
+
+
+class Placer:
+    """simplPlacer over one GPU context (the design is uploaded once)."""
+
+    def __init__(self, design: Design, opts: SimplOptions | None = None, ctx: simpl.Context | None = None):
+        self.d = design
+        self.o = opts or SimplOptions()
+        self.ctx = ctx or simpl.Context(0)
+        self.ctx.upload(design)
+        self.ctx.set_lal_options(self.o.maxLevel, self.o.minCellsToSplit, self.o.stripeAreaFraction)
+        self.stats = dict(init_iters=0, global_iters=0, cg_iterations=0, build_seconds=0.0,
+                          solve_seconds=0.0, spread_seconds=0.0)
+
+    # bookkeeping (simpl_placer.cc:256-297) -----------------------------------
+    def hpwl(self, x, y) -> float:
+        self.ctx.set_positions(x, y)
+        return self.ctx.hpwl()
+
+    def overflow(self, x, y) -> float:
+        self.ctx.set_positions(x, y)
+        return self.ctx.overflow()
+
+    def _solve(self, x, y):
+        r = self.ctx.solve_both(x, y, self.o.cgTol, self.o.cgMaxIter, precision=self.o.precision,
+                                stop_rule=(simpl.KT_STOP_TRUE if self.o.precision == simpl.KT_PREC_MIXED
+                                           else simpl.KT_STOP_RECURRENCE))
+        self.stats["cg_iterations"] += r["iterations"]
+        return r
+
+    def _initial_rounds(self, lx, ly, hpwl_init):
+        # B2B branch of initialRounds (simpl_placer.cc:406-448)
+        for _ in range(self.o.initMaxIters):
+            nx, ny = lx.copy(), ly.copy()
+            self.ctx.build_b2b(nx, ny)
+            self._solve(nx, ny)
+            h = self.hpwl(nx, ny)
+            self.stats["init_iters"] += 1
+            if not (h < hpwl_init):
+                break
+            lx, ly, hpwl_init = nx, ny, h
+        return lx, ly, hpwl_init
+
+    def run(self) -> PlaceResult:
+        o, ctx = self.o, self.ctx
+        d = self.d
+        ctx.build_grid(o.binsPerAxis, o.densityLimit)
+        lx, ly = seed_uniform(d, o.seed)
+        self.stats["hpwl_seed"] = self.hpwl(lx, ly)
+        if o.initialNetModel not in ("b2b",):
+            raise NotImplementedError("only the b2b initial net model runs on the GPU path")
+        lx, ly, self.stats["hpwl_init"] = self._initial_rounds(lx, ly, self.stats["hpwl_seed"])
+        trace = []
+        gap_ref = 0.0
+        best_upper = math.inf
+        stalled = 0
+        ux = uy = None
+        for it in range(o.maxIters):
+            self.stats["global_iters"] = it + 1
+            t0 = time.perf_counter()
+            ux, uy = ctx.lal(lx, ly)
+            self.stats["spread_seconds"] += time.perf_counter() - t0
+            upper = self.hpwl(ux, uy)
+            lower = self.hpwl(lx, ly)
+            gap = upper - lower
+            trace.append((lower, upper, self.overflow(ux, uy)))
+            iteration = it + 1
+            if iteration == o.gapReferenceIter:
+                gap_ref = gap
+            stalled = 0 if upper < best_upper else stalled + 1
+            best_upper = min(best_upper, upper)
+            converged = (iteration > o.gapReferenceIter and gap_ref > 0.0 and
+                         (gap < o.gapTightFrac * gap_ref or
+                          (gap <= o.gapRelaxedFrac * gap_ref and stalled >= o.stallIters)))
+            if converged:
+                break
+            alpha = o.alphaBase * (1.0 + it)
+            t0 = time.perf_counter()
+            ctx.build_b2b(lx, ly, alpha, ux, uy)
+            t1 = time.perf_counter()
+            self._solve(lx, ly)
+            self.stats["build_seconds"] += t1 - t0
+            self.stats["solve_seconds"] += time.perf_counter() - t1
+        if ux is not None and o.refineIters > 0:
+            done = self.stats["global_iters"]
+            ctx.build_grid(ctx.bins * o.refineGridFactor, o.densityLimit)
+            for r in range(o.refineIters):
+                alpha = o.alphaBase * (1.0 + done + r)
+                ctx.build_b2b(lx, ly, alpha, ux, uy)
+                self._solve(lx, ly)
+                ux, uy = ctx.lal(lx, ly)
+        if ux is None:
+            ux, uy = lx.copy(), ly.copy()
+        self.stats["hpwl_final"] = self.hpwl(ux, uy)
+        self.stats["hpwl_lower"] = self.hpwl(lx, ly)
+        self.stats["gap"] = self.stats["hpwl_final"] - self.stats["hpwl_lower"]
+        self.stats["overflow_lower"] = self.overflow(lx, ly)
+        self.stats["overflow_final"] = self.overflow(ux, uy)
+        return PlaceResult(ux, uy, self.stats, trace)

@@ -24,6 +24,7 @@ from .design import Design, Flat
 
 KT_PREC_FP64 = 0
 KT_PREC_MIXED = 1
+KT_PREC_FP64_EXACT = 2
 KT_STOP_RECURRENCE = 0
 KT_STOP_TRUE = 1
 
@@ -75,6 +76,15 @@ _SIGS = {
     "kt_kernel_times": (I32, [P, I32, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "kt_reset_kernel_times": (I32, [P]),
     "kt_probe_copy": (I32, [P, U64, I32, C.POINTER(C.c_double)]),
+    "kt_set_rows": (I32, [P, U32, P, P, P, P, P, P]),
+    "kt_build_grid": (I32, [P, U64, F64, C.POINTER(C.c_uint64)]),
+    "kt_grid_avail": (I32, [P, P, C.POINTER(C.c_double)]),
+    "kt_occupancy": (I32, [P, P]),
+    "kt_overflow": (I32, [P, C.POINTER(C.c_double)]),
+    "kt_hpwl": (I32, [P, C.POINTER(C.c_double)]),
+    "kt_set_lal_options": (I32, [P, U64, U64, F64]),
+    "kt_lal": (I32, [P, P, P]),
+    "kt_lal_resident": (I32, [P]),
 }
 
 EXPORTS = tuple(_SIGS)
@@ -165,7 +175,58 @@ class Context:
         _check(lib.kt_upload_netlist(self.h, C.byref(s)))
         self.n = flat.num_movable
         self.floors = flat.floors
+        if isinstance(design_or_flat, Design):
+            self.set_rows(design_or_flat.raw)
         return self
+
+    # rows, grid, bookkeeping, LAL -------------------------------------------
+    def set_rows(self, raw):
+        arrs = [np.ascontiguousarray(getattr(raw, k), np.float64)
+                for k in ("row_y", "row_height", "row_x0", "row_x1", "row_pitch")]
+        die = np.ascontiguousarray(raw.die, np.float64)
+        _check(lib.kt_set_rows(self.h, len(arrs[0]), *[_lib.ptr(a) for a in arrs], _lib.ptr(die)))
+        self._rows_keep = (arrs, die)
+
+    def build_grid(self, bins_per_axis: int = 0, density: float = 1.0) -> int:
+        b = C.c_uint64()
+        _check(lib.kt_build_grid(self.h, bins_per_axis, density, C.byref(b)))
+        self.bins = b.value
+        return b.value
+
+    def grid_avail(self):
+        a = np.zeros(self.bins * self.bins)
+        t = C.c_double()
+        _check(lib.kt_grid_avail(self.h, _lib.ptr(a), C.byref(t)))
+        return a, t.value
+
+    def occupancy(self):
+        occ = np.zeros(self.bins * self.bins)
+        _check(lib.kt_occupancy(self.h, _lib.ptr(occ)))
+        return occ
+
+    def overflow(self) -> float:
+        v = C.c_double()
+        _check(lib.kt_overflow(self.h, C.byref(v)))
+        return v.value
+
+    def hpwl(self) -> float:
+        v = C.c_double()
+        _check(lib.kt_hpwl(self.h, C.byref(v)))
+        return v.value
+
+    def set_lal_options(self, max_level: int = 0, min_cells_to_split: int = 0,
+                        stripe_area_fraction: float = 0.0):
+        _check(lib.kt_set_lal_options(self.h, max_level, min_cells_to_split, stripe_area_fraction))
+
+    def lal(self, x, y):
+        """LookAheadLegalizer::legalize on copies of x, y (returned)."""
+        x = np.array(x, np.float64)
+        y = np.array(y, np.float64)
+        _check(lib.kt_lal(self.h, _lib.ptr(x), _lib.ptr(y)))
+        return x, y
+
+    def lal_resident(self):
+        _check(lib.kt_lal_resident(self.h))
 
     # positions ---------------------------------------------------------------
     def set_positions(self, x, y):

@@ -224,3 +224,20 @@ def test_xcd_split_agrees(b2b12k, gpu_ctx_factory, xcd):
     ref = o.solve(x0, y0, tol=1e-12, max_iter=100000)
     assert rel(x, ref["x"]) < 1e-6 and rel(y, ref["y"]) < 1e-6
     ctx.close()
+
+
+def test_exact_mode_is_bit_identical_to_oracle(b2b12k, gpu_ctx_factory):
+    """KT_PREC_FP64_EXACT reproduces the reference's operation order."""
+    from ktplace_amd.simpl import KT_PREC_FP64_EXACT
+    d, o, x0, y0 = b2b12k
+    ctx = gpu_ctx_factory(d)
+    ctx.build_b2b(x0, y0)
+    x, y = x0.copy(), y0.copy()
+    seen = []
+    r = ctx.solve_both(x, y, 1e-6, 60, precision=KT_PREC_FP64_EXACT,
+                       observe=lambda it, res, last: seen.append(res))
+    ref = o.solve(x0, y0, tol=1e-6, max_iter=60, history=True)
+    assert r["iterations"] == ref["iterations"]
+    assert np.array_equal(x, ref["x"]) and np.array_equal(y, ref["y"])
+    assert np.array_equal(np.array(seen), ref["history"])
+    ctx.close()
