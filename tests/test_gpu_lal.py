@@ -1,10 +1,10 @@
 """Look-ahead legalization, HPWL, occupancy and overflow on the GPU vs the oracle.
 
 Bar: grid free area, occupancy and HPWL are bit-exact (same arithmetic; areas
-are whole sites); LAL positions agree with the oracle's (the same algorithm;
-growth uses summed-area tables, so rare near-ties may resolve differently) and
-the legalized HPWL is within 0.5 %; the full SimPL loop reaches an HPWL within
-0.5 % of the oracle's.
+are whole sites); LAL positions agree with the oracle's (the same algorithm,
+with near-ties of cluster growth re-summed in the reference's order) and the
+legalized HPWL is within 0.5 %; with the reference-order solver the full SimPL
+loop reproduces the oracle's placement exactly.
 """
 import numpy as np
 import pytest
