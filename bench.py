@@ -72,8 +72,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cells1m")
-    ap.add_argument("--paths", default="fp64,mixed",
-                    help="comma list of solver paths to time: fp64 (plain), mixed")
+    ap.add_argument("--paths", default="fp64,mixed,mixed+learned",
+                    help="comma list of solver paths to time: fp64 (plain), mixed, mixed+learned, "
+                         "fp64+learned")
+    ap.add_argument("--warm-params", default="",
+                    help="warm-start network parameters (.npy; default: the package's trained ones)")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -108,28 +111,39 @@ def main() -> int:
     ctx.save_positions()
     ctx.build_b2b_resident(0.0)
     nnz = [ctx.system_size(a)[1] for a in (0, 1)]
+    bins = ctx.build_grid(0, 1.0)  # density grid for the warm start's bin feature
+    paths = [p for p in args.paths.split(",") if p]
+    if any(p.endswith("+learned") for p in paths):
+        from ktplace_amd.learned import load_params
+        ctx.set_warm_start(load_params(args.warm_params or None))
+        log(f"[bench] warm-start network loaded; {bins}x{bins} bins")
 
     from ktplace_amd.autotune import TuneResult, autotune
-    paths = [p for p in args.paths.split(",") if p]
     per = {}
+    tuned_by_prec = {}
     for path in paths:
-        prec = simpl.KT_PREC_FP64 if path == "fp64" else simpl.KT_PREC_MIXED
+        base_path = path.split("+")[0]
+        prec = simpl.KT_PREC_FP64 if base_path == "fp64" else simpl.KT_PREC_MIXED
+        warm = simpl.KT_WARM_LEARNED if path.endswith("+learned") else simpl.KT_WARM_PREVIOUS
         ctx.restore_positions()
         ctx.build_b2b_resident(0.0)
         if args.lanes and args.tile:
-            ctx.set_tuning(lanes=args.lanes, tile=args.tile, xcd_split=args.xcd_split)
             tuned = TuneResult(args.lanes, args.tile, float("nan"), [], args.xcd_split)
+        elif prec in tuned_by_prec:
+            tuned = tuned_by_prec[prec]
         else:
             tuned = autotune(ctx, prec)
+            tuned_by_prec[prec] = tuned
+        ctx.set_tuning(lanes=tuned.lanes, tile=tuned.tile, xcd_split=tuned.xcd_split, rows=tuned.rows)
         log(f"[bench] {path}: autotune lanes={tuned.lanes} tile={tuned.tile} xcd_split={tuned.xcd_split} "
             f"rows={tuned.rows} "
             f"({tuned.ms_per_iteration:.3f} ms/iteration)")
 
-        def step(profile: bool, prec=prec) -> dict:
+        def step(profile: bool, prec=prec, warm=warm) -> dict:
             ctx.restore_positions()
             ctx.build_b2b_resident(0.0)
             return ctx.solve_resident(args.tol, 1_000_000, precision=prec,
-                                      stop_rule=simpl.KT_STOP_TRUE, profile=profile)
+                                      stop_rule=simpl.KT_STOP_TRUE, profile=profile, warm_start=warm)
 
         for w in range(args.warmup):
             r = step(False)
@@ -148,11 +162,13 @@ def main() -> int:
         presults = [step(True) for _ in range(args.steps)]
         ctx.sync()
         ms_prof = 1e3 * (time.perf_counter() - t0) / max(args.steps, 1)
-        kid = simpl.KERNEL_SPMV if path == "fp64" else simpl.KERNEL_SPMV32
+        kid = simpl.KERNEL_SPMV if base_path == "fp64" else simpl.KERNEL_SPMV32
         launches, spmv_ms = ctx.kernel_time(kid)
-        total_bytes = sum(r["steps"][a] * spmv_bytes(nnz[a], d.n, path)
+        total_bytes = sum(r["steps"][a] * spmv_bytes(nnz[a], d.n, base_path)
                           for r in presults for a in (0, 1))
         avg_ms = spmv_ms / max(launches, 1)
+        warm_n, warm_ms = ctx.kernel_time(simpl.KERNEL_WARM)
+        mlp_n, mlp_ms = ctx.kernel_time(simpl.KERNEL_MLP)
         bytes_per_launch = total_bytes / max(launches, 1)
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
         last = results[-1]
@@ -162,15 +178,20 @@ def main() -> int:
             f"SpMV {avg_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s algorithmic")
         per[path] = dict(
             ms_per_step=round(ms, 3), ms_per_step_with_kernel_events=round(ms_prof, 3),
-            iterations=last["iterations"], true_resid=list(last["true_resid"]),
+            iterations=last["iterations"], steps_per_axis=list(last["steps"]),
+            true_resid=list(last["true_resid"]),
             refreshes_or_restarts=last["refinements"],
+            warm_start=("learned (line-search gamma %.3f, %.3f)" % tuple(last["warm_gamma"])
+                        if warm == simpl.KT_WARM_LEARNED else "previous iterate"),
+            warm_start_ms=(round(warm_ms / warm_n, 3) if warm_n else None),
+            mlp_inference_ms=(round(mlp_ms / mlp_n, 3) if mlp_n else None),
             tuning={"lanes_per_row": tuned.lanes, "tile_rows": tuned.tile,
                     "xcd_split": tuned.xcd_split, "rows_in_flight": tuned.rows},
             roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                           frac=round(achieved / HBM_PEAK_GBS, 4),
-                          traffic=(MEASURED_TRAFFIC.get(path) if args.workload == "cells1m" else None),
+                          traffic=(MEASURED_TRAFFIC.get(base_path) if args.workload == "cells1m" else None),
                           traffic_source="rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, cells1m, copy-probe scaled",
-                          kernel=f"cgSpmv<{'double' if path == 'fp64' else 'float'}> (x and y per launch)",
+                          kernel=f"cgSpmv<{'double' if base_path == 'fp64' else 'float'}> (x and y per launch)",
                           launches=launches, avg_launch_us=round(avg_ms * 1e3, 2),
                           bytes_per_launch=int(bytes_per_launch)))
 
@@ -187,7 +208,7 @@ def main() -> int:
         copy = dict(bytes=nb, ms=round(cms, 4), read_write_gbs=round(2 * nb / (cms * 1e-3) / 1e9, 1))
         log(f"[bench] copy probe {args.copy_probe} MiB: {cms:.3f} ms")
 
-    main_path = "mixed" if "mixed" in per else paths[0]
+    main_path = next((p for p in ("mixed+learned", "mixed") if p in per), paths[0])
     ms = per[main_path]["ms_per_step"]
     out = {
         "metric": METRIC,
@@ -200,14 +221,14 @@ def main() -> int:
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if main_path == "fp64" else "f32+f64",
+        "dtype": "f64" if main_path.startswith("fp64") else "f32+f64",
         "data": "synthetic",
         "config": {
             "workload": (f"{args.workload}: one B2B assembly + lock-step Jacobi-PCG (x and y) to "
                          f"{args.tol:g} true relative residual; {d.n} cells, {d.flat.num_nets} "
                          f"nets, {d.flat.num_pins} pins, seed {args.seed}"),
             "path": main_path,
-            "warm_start": "previous iterate",
+            "warm_start": per[main_path]["warm_start"],
             "paths": per,
             "speedup_vs_plain_fp64": (round(per["fp64"]["ms_per_step"] / ms, 3)
                                       if "fp64" in per else None),

@@ -112,6 +112,10 @@ int kt_multiply(kt_ctx *ctx, int axis, const double *in, double *out);
 #define KT_STOP_RECURRENCE 0 /* reference rule: ||r_k|| / ||b|| <= tol */
 #define KT_STOP_TRUE 1       /* true FP64 ||b - A x|| / ||b|| <= tol */
 
+#define KT_WARM_PREVIOUS 0 /* start from the previous iterate (the reference) */
+#define KT_WARM_LEARNED 1  /* start from previous + gamma * MLP-predicted displacement,
+                              gamma from an exact line search (0 falls back) */
+
 typedef struct kt_solve_opts {
     double tol;
     uint64_t max_iter;
@@ -119,6 +123,8 @@ typedef struct kt_solve_opts {
     int32_t stop_rule;   /* KT_STOP_* */
     int32_t check_every; /* iterations per host convergence check (0 = default) */
     int32_t profile;     /* record per-kernel HIP event timings */
+    int32_t warm_start;  /* KT_WARM_*: where CG starts */
+    int32_t reserved;    /* 0 */
 } kt_solve_opts;
 
 typedef struct kt_solve_result {
@@ -127,6 +133,8 @@ typedef struct kt_solve_result {
     uint64_t iterations;  /* lock-step iterations (the reference's loop count) */
     uint64_t steps[2];    /* iterations each axis stepped */
     uint64_t refinements; /* FP64 refinement sweeps (mixed precision) */
+    double warm_gamma[2]; /* KT_WARM_LEARNED: step taken along the predicted
+                             displacement per axis (0: previous iterate kept) */
 } kt_solve_result;
 
 /* Called after the solve for every iterate, in order, with the residuals the
@@ -183,9 +191,55 @@ int kt_set_tuning2(kt_ctx *ctx, int lanes, uint32_t tile, int use_graphs, double
 #define KT_KERNEL_PUPDATE 2   /* FP64 p = z + beta p */
 #define KT_KERNEL_SPMV32 3    /* FP32 CG SpMV + p.Ap */
 #define KT_KERNEL_B2B_FILL 4  /* B2B fill pass */
+#define KT_KERNEL_WARM 5      /* learned warm start, whole (features .. line search) */
+#define KT_KERNEL_MLP 6       /* learned warm start, network inference */
 #define KT_KERNEL_COUNT 8
 int kt_kernel_times(kt_ctx *ctx, int kernel, uint64_t *launches, double *total_ms);
 int kt_reset_kernel_times(kt_ctx *ctx);
+
+/* ---- learned warm start (no counterpart in the reference) ---------------- */
+/* A 3-layer MLP (KT_MLP_INPUTS -> 128 -> 128 -> 2, ReLU) with BF16 weights and
+ * FP32 accumulation predicts each cell's displacement from its previous
+ * iterate to the solution of the new systems, in units of the placement's
+ * spread.  Parameters: KT_MLP_PARAMS FP32 values, W1 [128][32], b1 [128],
+ * W2 [128][128], b2 [128], W3 [2][128], b3 [2] (row-major, output unit first).
+ * Features: KT_MLP_FEATURES per cell (the remaining inputs are zero). */
+#define KT_MLP_INPUTS 32
+#define KT_MLP_FEATURES 17
+#define KT_MLP_HIDDEN 128
+#define KT_MLP_OUTPUTS 2
+#define KT_MLP_PARAMS 20994
+
+/* Network used by KT_WARM_LEARNED solves on this context (NULL clears it). */
+int kt_set_warm_start(kt_ctx *ctx, const float *params, uint64_t count);
+/* Features [n][KT_MLP_FEATURES] of the context's positions and current
+ * systems (build them first), and the spread scale[2] = {s_x, s_y} that
+ * turns network outputs into displacements.  features may be NULL. */
+int kt_warm_features(kt_ctx *ctx, float *features, double *scale);
+/* The predicted displacement dx, dy [n] (before the line search). */
+int kt_warm_predict(kt_ctx *ctx, double *dx, double *dy);
+
+/* Training and evaluation of the network on the GPU (BF16 matrix-core
+ * forward and backward passes, FP32 master weights, Adam). */
+typedef struct kt_mlp kt_mlp;
+int kt_mlp_create(int device, kt_mlp **out);
+int kt_mlp_destroy(kt_mlp *m);
+int kt_mlp_set_params(kt_mlp *m, const float *params, uint64_t count);
+int kt_mlp_get_params(kt_mlp *m, float *params, uint64_t count);
+/* Training set: features [count][KT_MLP_FEATURES], targets [count][2] and
+ * per-sample loss weights [count] (NULL: all 1). */
+int kt_mlp_set_data(kt_mlp *m, const float *features, const float *targets, const float *weights,
+                    uint64_t count);
+/* out [count][2] = network(features [count][KT_MLP_FEATURES]). */
+int kt_mlp_forward(kt_mlp *m, const float *features, uint64_t count, float *out);
+/* Gradient of the loss (1/count) sum_s w_s |network(f_s) - t_s|^2 over the
+ * listed training samples with respect to every parameter
+ * (grad [KT_MLP_PARAMS]), and the loss itself. */
+int kt_mlp_gradient(kt_mlp *m, const uint32_t *samples, uint64_t count, float *grad, double *loss);
+/* `steps` Adam steps (lr, beta 0.9 / 0.999) on minibatches of `batch` samples
+ * (a multiple of 32) drawn from the training set by a hash of (seed, step);
+ * loss [steps] receives each batch's mean loss (may be NULL). */
+int kt_mlp_train(kt_mlp *m, uint64_t steps, uint32_t batch, double lr, uint64_t seed, float *loss);
 
 /* Calibration helper (no counterpart in the reference): copies `bytes` bytes
  * between two scratch device buffers `reps` times with a plain streaming copy

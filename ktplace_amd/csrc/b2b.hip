@@ -324,6 +324,7 @@ int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY) {
     a.floor[0] = floorX;
     a.floor[1] = floorY;
     a.alpha = alpha;
+    ctx->lastAlpha = alpha;
     a.n = n;
     if (n == 0) {
         for (int axis = 0; axis < 2; ++axis)

@@ -231,8 +231,12 @@ int solve(kt_ctx *ctx, const kt_solve_opts *opts, kt_solve_result *out) {
     KT_CHECK(opts->precision == KT_PREC_FP64 || opts->precision == KT_PREC_MIXED ||
                  opts->precision == KT_PREC_FP64_EXACT,
              KT_EINVAL, "solve: unknown precision");
+    KT_CHECK(opts->warm_start == KT_WARM_PREVIOUS || opts->warm_start == KT_WARM_LEARNED, KT_EINVAL,
+             "solve: unknown warm start");
     ctx->profile = opts->profile != 0;
     kt_solve_result res{};
+    const bool learned = opts->warm_start == KT_WARM_LEARNED;
+    if (learned) KT_TRY(warmStart(ctx));
     int rc;
     if (opts->precision == KT_PREC_FP64) {
         rc = solveFp64(ctx, *opts, res);
@@ -247,6 +251,11 @@ int solve(kt_ctx *ctx, const kt_solve_opts *opts, kt_solve_result *out) {
     }
     if (rc != KT_OK) return rc;
     if (ctx->profile) KT_TRY(collectTimers(ctx));
+    if (learned) {
+        KT_HIP(hipStreamSynchronize(ctx->stream));
+        res.warm_gamma[0] = ctx->pinned[62];
+        res.warm_gamma[1] = ctx->pinned[63];
+    }
     *out = res;
     return KT_OK;
 }
@@ -662,6 +671,22 @@ int kt_lal(kt_ctx *ctx, double *x, double *y) {
         KT_TRY(lalLegalize(ctx));
         return getPositions(ctx, x, y, ctx->n);
     })
+}
+
+int kt_set_warm_start(kt_ctx *ctx, const float *params, uint64_t count) {
+    KT_GUARD(ctx);
+    KT_CATCH(return warmSetParams(ctx, params, count);)
+}
+
+int kt_warm_features(kt_ctx *ctx, float *features, double *scale) {
+    KT_GUARD(ctx);
+    KT_CATCH(return warmFeaturesHost(ctx, features, scale);)
+}
+
+int kt_warm_predict(kt_ctx *ctx, double *dx, double *dy) {
+    KT_GUARD(ctx);
+    KT_CHECK(dx && dy, KT_EINVAL, "kt_warm_predict: null array");
+    KT_CATCH(return warmPredictHost(ctx, dx, dy);)
 }
 
 int kt_probe_copy(kt_ctx *ctx, uint64_t bytes, int reps, double *ms_per_copy) {

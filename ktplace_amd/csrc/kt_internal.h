@@ -38,11 +38,17 @@ int hipFail(hipError_t e, const char *what, const char *file, int line);
     } while (0)
 
 // ------------------------------------------------------------------ buffers
-// Grow-only device buffer.  Reallocation drops the old contents.
+// Grow-only device buffer.  Reallocation drops the old contents.  Owns its
+// allocation (freed on destruction, on the device current at that time).
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t cap = 0;
+
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
 
     int reserve(size_t count) {
         if (count <= cap) return KT_OK;
@@ -224,6 +230,13 @@ struct kt_ctx {
     kt::DevBuf<kt::LalSegStat> lalStats;
     kt::DevBuf<uint8_t> lalSplit;
 
+    // learned warm start: BF16 weights / FP32 biases, features, prediction
+    bool haveWarm = false;
+    double lastAlpha = 0.0;              // alpha of the last B2B build
+    kt::DevBuf<uint16_t> warmW, warmFeat;
+    kt::DevBuf<float> warmB, warmOut, warmShare;
+    kt::DevBuf<double> warmPart, warmScalars;
+
     kt::DevBuf<double> exact;            // reference-order solver vectors
     std::vector<double> exactHistory;    // its residuals per iteration
 
@@ -242,4 +255,8 @@ int systemNnz(kt_ctx *ctx, int axis, uint64_t *nnz);
 int timedBegin(kt_ctx *ctx, int kernel, hipEvent_t *start);
 int timedEnd(kt_ctx *ctx, int kernel, hipEvent_t start);
 int collectTimers(kt_ctx *ctx);
+int warmSetParams(kt_ctx *ctx, const float *params, uint64_t count);
+int warmFeaturesHost(kt_ctx *ctx, float *features, double *scale);
+int warmPredictHost(kt_ctx *ctx, double *dx, double *dy);
+int warmStart(kt_ctx *ctx);
 }  // namespace kt

@@ -74,16 +74,40 @@ __device__ __forceinline__ uint32_t binOfPoint(const GridDev &g, double x, doubl
     return clampIndex((y - g.oy) / g.dy, g.ny) * g.nx + clampIndex((x - g.ox) / g.dx, g.nx);
 }
 
-// bin of each cell centre (simpl_placer.cc:277-289) and fixed-point occupancy
+__device__ __forceinline__ unsigned long long waveSumU64(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// bin of each cell centre (simpl_placer.cc:277-289) and fixed-point occupancy.
+// The lanes of a wave first combine their areas per bin (one pass per
+// distinct bin among the wave's cells) and one lane adds each sum: a clumped
+// placement puts most cells into a few bins, where one atomic per cell would
+// serialise.  Integer sums, so the result does not depend on the grouping.
 __global__ void __launch_bounds__(kB)
 cellBins(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ w,
          const double *__restrict__ h, const double *__restrict__ area, GridDev g, uint32_t n,
          uint32_t *__restrict__ binOf, unsigned long long *__restrict__ occFix) {
     const uint32_t i = blockIdx.x * kB + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t k = binOfPoint(g, x[i] + 0.5 * w[i], y[i] + 0.5 * h[i]);
-    binOf[i] = k;
-    atomicAdd(occFix + k, static_cast<unsigned long long>(llrint(area[i] * kFixScale)));
+    const bool valid = i < n;
+    uint32_t k = 0;
+    unsigned long long v = 0;
+    if (valid) {
+        k = binOfPoint(g, x[i] + 0.5 * w[i], y[i] + 0.5 * h[i]);
+        binOf[i] = k;
+        v = static_cast<unsigned long long>(llrint(area[i] * kFixScale));
+    }
+    bool todo = valid;
+    unsigned long long pending = __ballot(todo);
+    while (pending) {
+        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+        const uint32_t lk = __shfl(k, leader);
+        const bool mine = todo && k == lk;
+        const unsigned long long s = waveSumU64(mine ? v : 0ull);
+        if (static_cast<int>(threadIdx.x & 63) == leader) atomicAdd(occFix + lk, s);
+        todo = todo && !mine;
+        pending = __ballot(todo);
+    }
 }
 
 __global__ void __launch_bounds__(kB)

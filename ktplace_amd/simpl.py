@@ -27,25 +27,32 @@ KT_PREC_MIXED = 1
 KT_PREC_FP64_EXACT = 2
 KT_STOP_RECURRENCE = 0
 KT_STOP_TRUE = 1
+KT_WARM_PREVIOUS = 0
+KT_WARM_LEARNED = 1
+KT_MLP_FEATURES = 17
+KT_MLP_PARAMS = 20994
 
 KERNEL_SPMV, KERNEL_UPDATE, KERNEL_PUPDATE, KERNEL_SPMV32, KERNEL_B2B_FILL = 0, 1, 2, 3, 4
+KERNEL_WARM, KERNEL_MLP = 5, 6
 
 
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_uint64), ("precision", C.c_int32),
-                ("stop_rule", C.c_int32), ("check_every", C.c_int32), ("profile", C.c_int32)]
+                ("stop_rule", C.c_int32), ("check_every", C.c_int32), ("profile", C.c_int32),
+                ("warm_start", C.c_int32), ("reserved", C.c_int32)]
 
 
 class SolveResult(C.Structure):
     _fields_ = [("resid", C.c_double * 2), ("true_resid", C.c_double * 2),
                 ("iterations", C.c_uint64), ("steps", C.c_uint64 * 2),
-                ("refinements", C.c_uint64)]
+                ("refinements", C.c_uint64), ("warm_gamma", C.c_double * 2)]
 
     def as_dict(self) -> dict:
         return dict(resid=(self.resid[0], self.resid[1]),
                     true_resid=(self.true_resid[0], self.true_resid[1]),
                     iterations=int(self.iterations), steps=(int(self.steps[0]), int(self.steps[1])),
-                    refinements=int(self.refinements))
+                    refinements=int(self.refinements),
+                    warm_gamma=(self.warm_gamma[0], self.warm_gamma[1]))
 
 
 OBSERVER = C.CFUNCTYPE(None, C.c_void_p, C.c_uint64, C.c_double, C.c_double, C.c_int)
@@ -85,6 +92,17 @@ _SIGS = {
     "kt_set_lal_options": (I32, [P, U64, U64, F64]),
     "kt_lal": (I32, [P, P, P]),
     "kt_lal_resident": (I32, [P]),
+    "kt_set_warm_start": (I32, [P, P, U64]),
+    "kt_warm_features": (I32, [P, P, P]),
+    "kt_warm_predict": (I32, [P, P, P]),
+    "kt_mlp_create": (I32, [I32, C.POINTER(C.c_void_p)]),
+    "kt_mlp_destroy": (I32, [P]),
+    "kt_mlp_set_params": (I32, [P, P, U64]),
+    "kt_mlp_get_params": (I32, [P, P, U64]),
+    "kt_mlp_set_data": (I32, [P, P, P, P, U64]),
+    "kt_mlp_forward": (I32, [P, P, U64, P]),
+    "kt_mlp_gradient": (I32, [P, P, U64, P, C.POINTER(C.c_double)]),
+    "kt_mlp_train": (I32, [P, U64, U32, F64, U64, P]),
 }
 
 EXPORTS = tuple(_SIGS)
@@ -304,12 +322,12 @@ class Context:
     # solveBothAxes -----------------------------------------------------------
     def solve_both(self, x, y, tol: float, max_iter: int, precision: int = KT_PREC_FP64,
                    stop_rule: int = KT_STOP_RECURRENCE, observe: Optional[Callable] = None,
-                   check_every: int = 0, profile: bool = False):
+                   check_every: int = 0, profile: bool = False, warm_start: int = KT_WARM_PREVIOUS):
         """solveBothAxes: x, y are updated in place (numpy float64 arrays)."""
         if x.dtype != np.float64 or y.dtype != np.float64 or not x.flags["C_CONTIGUOUS"]:
             raise TypeError("x and y must be contiguous float64 arrays (updated in place)")
         opts = SolveOpts(tol=tol, max_iter=max_iter, precision=precision, stop_rule=stop_rule,
-                         check_every=check_every, profile=int(profile))
+                         check_every=check_every, profile=int(profile), warm_start=warm_start)
         res = SolveResult()
         cb = OBSERVER(lambda _u, it, rx, ry, last: observe(int(it), (rx, ry), bool(last))) \
             if observe else OBSERVER()
@@ -319,9 +337,9 @@ class Context:
 
     def solve_resident(self, tol: float, max_iter: int, precision: int = KT_PREC_FP64,
                        stop_rule: int = KT_STOP_RECURRENCE, check_every: int = 0,
-                       profile: bool = False) -> dict:
+                       profile: bool = False, warm_start: int = KT_WARM_PREVIOUS) -> dict:
         opts = SolveOpts(tol=tol, max_iter=max_iter, precision=precision, stop_rule=stop_rule,
-                         check_every=check_every, profile=int(profile))
+                         check_every=check_every, profile=int(profile), warm_start=warm_start)
         res = SolveResult()
         _check(lib.kt_solve_both_resident(self.h, C.byref(opts), C.byref(res)))
         return res.as_dict()
@@ -345,6 +363,29 @@ class Context:
 
     def reset_kernel_times(self):
         _check(lib.kt_reset_kernel_times(self.h))
+
+    # learned warm start ------------------------------------------------------
+    def set_warm_start(self, params: Optional[np.ndarray]):
+        """Network for KT_WARM_LEARNED solves (None clears it)."""
+        if params is None:
+            _check(lib.kt_set_warm_start(self.h, None, 0))
+            return
+        p = np.ascontiguousarray(params, np.float32)
+        _check(lib.kt_set_warm_start(self.h, _lib.ptr(p), len(p)))
+
+    def warm_features(self, with_features: bool = True):
+        """(features [n][17] float32 or None, (s_x, s_y)) at the current
+        positions and systems."""
+        f = np.zeros((self.n, KT_MLP_FEATURES), np.float32) if with_features else None
+        sc = np.zeros(2)
+        _check(lib.kt_warm_features(self.h, _lib.ptr(f) if f is not None else None, _lib.ptr(sc)))
+        return f, (float(sc[0]), float(sc[1]))
+
+    def warm_predict(self):
+        dx = np.zeros(self.n)
+        dy = np.zeros(self.n)
+        _check(lib.kt_warm_predict(self.h, _lib.ptr(dx), _lib.ptr(dy)))
+        return dx, dy
 
 
 class LinearSystems:
