@@ -75,6 +75,8 @@ def main() -> int:
     ap.add_argument("--paths", default="fp64,mixed,mixed+learned",
                     help="comma list of solver paths to time: fp64 (plain), mixed, mixed+learned, "
                          "fp64+learned")
+    ap.add_argument("--batched-count", type=int, default=256,
+                    help="also time the batched small-designs workload with this many netlists (0: skip)")
     ap.add_argument("--warm-params", default="",
                     help="warm-start network parameters (.npy; default: the package's trained ones)")
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -201,6 +203,12 @@ def main() -> int:
         base = cpu_baseline(d, x1, y1, args.tol)
         log(f"[bench] cpu baseline {base['value']:.0f} ms on {base['cores']} threads")
 
+    batched = None
+    if args.batched_count:
+        from ktplace_amd.batched import run as run_batched
+        log(f"[bench] batched small designs: {args.batched_count} netlists ...")
+        batched = run_batched(args.batched_count, steps=2, tol=args.tol, log=log)
+
     copy = None
     if args.copy_probe:
         nb = args.copy_probe << 20
@@ -234,6 +242,7 @@ def main() -> int:
                                       if "fp64" in per else None),
             "nnz_per_axis": nnz,
             "copy_probe": copy,
+            "batched_designs": batched,
         },
         "roofline": per[main_path]["roofline"],
         "cpu_baseline": base,

@@ -241,6 +241,49 @@ int kt_mlp_gradient(kt_mlp *m, const uint32_t *samples, uint64_t count, float *g
  * loss [steps] receives each batch's mean loss (may be NULL). */
 int kt_mlp_train(kt_mlp *m, uint64_t steps, uint32_t batch, double lr, uint64_t seed, float *loss);
 
+/* ---- batched small designs (no counterpart in the reference) ------------ */
+/* Many designs' x/y systems solved together: one launch per CG kernel covers
+ * every tile of every system, each system keeps its own Jacobi-PCG scalars
+ * and stopping rules (solveBothAxes's), and the iteration pairs replay one
+ * captured hipGraph until no system is live.  FP64, previous-iterate start. */
+typedef struct kt_system_view {
+    uint64_t rows;
+    const uint64_t *row_ptr; /* [rows + 1], row_ptr[0] = 0 */
+    const uint32_t *col;     /* [nnz] off-diagonal columns (local to the system) */
+    const double *val;       /* [nnz] */
+    const double *diag;      /* [rows] */
+    const double *rhs;       /* [rows] */
+} kt_system_view;
+
+typedef struct kt_batch_instance_result {
+    double resid[2];
+    double true_resid[2];
+    uint64_t steps[2];
+    double done_ms; /* wall time from the start of kt_batch_solve until both of
+                       this instance's systems had stopped (host poll granularity) */
+} kt_batch_instance_result;
+
+typedef struct kt_batch kt_batch;
+int kt_batch_create(int device, kt_batch **out);
+int kt_batch_destroy(kt_batch *b);
+int kt_batch_clear(kt_batch *b);
+/* Appends one instance (x and y systems of equal size, start positions). */
+int kt_batch_add(kt_batch *b, const kt_system_view *sys_x, const kt_system_view *sys_y, const double *x0,
+                 const double *y0, uint32_t *index);
+/* Appends the context's current systems and positions as one instance. */
+int kt_batch_add_ctx(kt_batch *b, kt_ctx *ctx, uint32_t *index);
+int kt_batch_size(kt_batch *b, uint32_t *instances, uint64_t *rows, uint64_t *nnz);
+/* lanes per CSR row (4, 8, 16), rows per tile (a multiple of 256), rows in
+ * flight per lane group (2, 4), hipGraph replay on/off; 0 keeps a value. */
+int kt_batch_set_tuning(kt_batch *b, int lanes, uint32_t tile, int rows_in_flight, int use_graphs);
+/* Every instance back to the start positions it was added with. */
+int kt_batch_reset(kt_batch *b);
+/* Solves all instances from their current positions (opts: KT_PREC_FP64,
+ * KT_WARM_PREVIOUS); per_instance [instances] and total may be NULL. */
+int kt_batch_solve(kt_batch *b, const kt_solve_opts *opts, kt_batch_instance_result *per_instance,
+                   kt_solve_result *total);
+int kt_batch_get_positions(kt_batch *b, uint32_t index, double *x, double *y, uint64_t count);
+
 /* Calibration helper (no counterpart in the reference): copies `bytes` bytes
  * between two scratch device buffers `reps` times with a plain streaming copy
  * kernel (kt_copy_probe) and returns the average milliseconds per copy.  Used

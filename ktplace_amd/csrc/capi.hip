@@ -766,3 +766,27 @@ int kt_reset_kernel_times(kt_ctx *ctx) {
 }
 
 }  // extern "C"
+
+namespace kt {
+
+// Host copies of a context's system and positions (for kt_batch_add_ctx).
+int getSystemHost(kt_ctx *ctx, int axis, std::vector<uint64_t> &rowPtr, std::vector<uint32_t> &col,
+                  std::vector<double> &val, std::vector<double> &diag, std::vector<double> &rhs) {
+    uint64_t rows = 0, nnz = 0;
+    KT_TRY(kt_system_size(ctx, axis, &rows, &nnz));
+    rowPtr.resize(rows + 1);
+    col.resize(nnz);
+    val.resize(nnz);
+    diag.resize(rows);
+    rhs.resize(rows);
+    return kt_get_system(ctx, axis, rowPtr.data(), col.data(), val.data(), diag.data(), rhs.data());
+}
+
+int getPositionsHost(kt_ctx *ctx, std::vector<double> &x, std::vector<double> &y) {
+    KT_CHECK(ctx, KT_EINVAL, "null context");
+    x.resize(ctx->posCount);
+    y.resize(ctx->posCount);
+    return kt_get_positions(ctx, x.data(), y.data(), ctx->posCount);
+}
+
+}  // namespace kt

@@ -55,6 +55,16 @@ class SolveResult(C.Structure):
                     warm_gamma=(self.warm_gamma[0], self.warm_gamma[1]))
 
 
+class SystemView(C.Structure):
+    _fields_ = [("rows", C.c_uint64), ("row_ptr", C.c_void_p), ("col", C.c_void_p), ("val", C.c_void_p),
+                ("diag", C.c_void_p), ("rhs", C.c_void_p)]
+
+
+class BatchInstanceResult(C.Structure):
+    _fields_ = [("resid", C.c_double * 2), ("true_resid", C.c_double * 2), ("steps", C.c_uint64 * 2),
+                ("done_ms", C.c_double)]
+
+
 OBSERVER = C.CFUNCTYPE(None, C.c_void_p, C.c_uint64, C.c_double, C.c_double, C.c_int)
 
 _SIGS = {
@@ -103,6 +113,16 @@ _SIGS = {
     "kt_mlp_forward": (I32, [P, P, U64, P]),
     "kt_mlp_gradient": (I32, [P, P, U64, P, C.POINTER(C.c_double)]),
     "kt_mlp_train": (I32, [P, U64, U32, F64, U64, P]),
+    "kt_batch_create": (I32, [I32, C.POINTER(C.c_void_p)]),
+    "kt_batch_destroy": (I32, [P]),
+    "kt_batch_clear": (I32, [P]),
+    "kt_batch_add": (I32, [P, P, P, P, P, C.POINTER(C.c_uint32)]),
+    "kt_batch_add_ctx": (I32, [P, P, C.POINTER(C.c_uint32)]),
+    "kt_batch_size": (I32, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "kt_batch_set_tuning": (I32, [P, I32, U32, I32, I32]),
+    "kt_batch_reset": (I32, [P]),
+    "kt_batch_solve": (I32, [P, C.POINTER(SolveOpts), P, C.POINTER(SolveResult)]),
+    "kt_batch_get_positions": (I32, [P, U32, P, P, U64]),
 }
 
 EXPORTS = tuple(_SIGS)
@@ -386,6 +406,82 @@ class Context:
         dy = np.zeros(self.n)
         _check(lib.kt_warm_predict(self.h, _lib.ptr(dx), _lib.ptr(dy)))
         return dx, dy
+
+
+class Batch:
+    """Many designs' x/y systems solved together on one GPU (a kt_batch)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib.kt_batch_create(device, C.byref(h)))
+        self.h = h
+        self.rows = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.kt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def clear(self):
+        _check(lib.kt_batch_clear(self.h))
+        self.rows = []
+
+    def add(self, sx: SparseSystem, sy: SparseSystem, x0, y0) -> int:
+        keep = []
+
+        def view(s):
+            arrs = [np.ascontiguousarray(s.row_ptr, np.uint64), np.ascontiguousarray(s.col, np.uint32),
+                    _f64(s.val), _f64(s.diag), _f64(s.rhs)]
+            keep.extend(arrs)
+            return SystemView(len(arrs[3]), *[a.ctypes.data for a in arrs])
+
+        vx, vy = view(sx), view(sy)
+        x0, y0 = _f64(x0), _f64(y0)
+        idx = C.c_uint32()
+        _check(lib.kt_batch_add(self.h, C.byref(vx), C.byref(vy), _lib.ptr(x0), _lib.ptr(y0), C.byref(idx)))
+        self.rows.append(len(x0))
+        return idx.value
+
+    def add_ctx(self, ctx: Context) -> int:
+        idx = C.c_uint32()
+        _check(lib.kt_batch_add_ctx(self.h, ctx.h, C.byref(idx)))
+        self.rows.append(ctx.system_size(0)[0])
+        return idx.value
+
+    def size(self):
+        n = C.c_uint32()
+        r = C.c_uint64()
+        z = C.c_uint64()
+        _check(lib.kt_batch_size(self.h, C.byref(n), C.byref(r), C.byref(z)))
+        return n.value, r.value, z.value
+
+    def set_tuning(self, lanes: int = 0, tile: int = 0, rows: int = 0, use_graphs: bool = True):
+        _check(lib.kt_batch_set_tuning(self.h, lanes, tile, rows, int(use_graphs)))
+
+    def reset(self):
+        _check(lib.kt_batch_reset(self.h))
+
+    def solve(self, tol: float, max_iter: int, stop_rule: int = KT_STOP_RECURRENCE, check_every: int = 0):
+        """(per-instance list of dicts, totals dict)."""
+        opts = SolveOpts(tol=tol, max_iter=max_iter, precision=KT_PREC_FP64, stop_rule=stop_rule,
+                         check_every=check_every)
+        per = (BatchInstanceResult * max(len(self.rows), 1))()
+        tot = SolveResult()
+        _check(lib.kt_batch_solve(self.h, C.byref(opts), C.addressof(per), C.byref(tot)))
+        out = [dict(resid=(r.resid[0], r.resid[1]), true_resid=(r.true_resid[0], r.true_resid[1]),
+                    steps=(int(r.steps[0]), int(r.steps[1])), done_ms=r.done_ms)
+               for r in per[:len(self.rows)]]
+        return out, tot.as_dict()
+
+    def get_positions(self, index: int):
+        n = self.rows[index]
+        x = np.zeros(n)
+        y = np.zeros(n)
+        _check(lib.kt_batch_get_positions(self.h, index, _lib.ptr(x), _lib.ptr(y), n))
+        return x, y
 
 
 class LinearSystems:
