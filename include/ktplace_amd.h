@@ -94,6 +94,13 @@ int kt_build_b2b(kt_ctx *ctx, const double *x, const double *y, double alpha,
 /* Resident form: uses the context's positions and anchors. */
 int kt_build_b2b_resident(kt_ctx *ctx, double alpha, double floor_x, double floor_y);
 
+/* LinearSystems::buildStar (simpl_lss.cc:116-201): both star-model systems
+ * (n movable cells + one star per net with >= 3 pins and a movable pin, in net
+ * order); *size = n + stars.  Positions do not enter; solve them with
+ * positions of `size` entries (the reference starts the stars at the die
+ * centre, simpl_placer.cc:409-414). */
+int kt_build_star(kt_ctx *ctx, uint64_t *size);
+
 /* ---- SparseSystem access (simpl_lss.h:38-58) ---------------------------- */
 int kt_system_size(kt_ctx *ctx, int axis, uint64_t *rows, uint64_t *nnz);
 int kt_get_system(kt_ctx *ctx, int axis, uint64_t *row_ptr, uint32_t *col, double *val,

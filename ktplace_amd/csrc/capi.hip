@@ -446,6 +446,11 @@ int kt_build_b2b(kt_ctx *ctx, const double *x, const double *y, double alpha, co
     return KT_OK;
 }
 
+int kt_build_star(kt_ctx *ctx, uint64_t *size) {
+    KT_GUARD(ctx);
+    KT_CATCH(return buildStar(ctx, size);)
+}
+
 int kt_build_b2b_resident(kt_ctx *ctx, double alpha, double floor_x, double floor_y) {
     KT_GUARD(ctx);
     return buildB2B(ctx, alpha, floor_x, floor_y);

@@ -237,6 +237,8 @@ struct kt_ctx {
     kt::DevBuf<float> warmB, warmOut, warmShare;
     kt::DevBuf<double> warmPart, warmScalars;
 
+    kt::DevBuf<uint32_t> starScan, starNet;  // star model: scan of star flags, star -> net
+
     kt::DevBuf<double> exact;            // reference-order solver vectors
     std::vector<double> exactHistory;    // its residuals per iteration
 
@@ -259,4 +261,5 @@ int warmSetParams(kt_ctx *ctx, const float *params, uint64_t count);
 int warmFeaturesHost(kt_ctx *ctx, float *features, double *scale);
 int warmPredictHost(kt_ctx *ctx, double *dx, double *dy);
 int warmStart(kt_ctx *ctx);
+int buildStar(kt_ctx *ctx, uint64_t *size);
 }  // namespace kt

@@ -148,8 +148,8 @@ def collect(design, ctx: simpl.Context, cells_per_solve: int, rng: np.random.Gen
             kinds.append("anchored" if f[0, 14] > 0 else "initial")
             return r
 
-    opts = SimplOptions(cgTol=tol, cgMaxIter=100_000, maxIters=max_global, refineIters=0,
-                        precision=simpl.KT_PREC_FP64)
+    opts = SimplOptions(initialNetModel="b2b", cgTol=tol, cgMaxIter=100_000, maxIters=max_global,
+                        refineIters=0, precision=simpl.KT_PREC_FP64)
     rec = Recorder(design, opts, ctx)
     rec.run()
     return np.concatenate(feats), np.concatenate(targs), np.concatenate(ids), kinds

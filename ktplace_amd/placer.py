@@ -1,7 +1,7 @@
 """SimPL global placement driven from the host, every step on the GPU.
 
 Mirrors simplPlacer::Impl::run (simpl_placer.cc:358-580): seedUniform, initial
-rounds (B2B, or the star model when the context supports it), then outer
+rounds (star model and/or B2B, simpl_placer.cc:391-448), then outer
 iterations of look-ahead legalization (upper bound), HPWL bookkeeping, the
 convergence test, B2B assembly with anchors and the Jacobi-PCG solve (lower
 bound), and finally the refinement on a finer grid.  Options and their
@@ -25,7 +25,7 @@ class SimplOptions:
     binsPerAxis: int = 0
     cgTol: float = 1e-4
     initMaxIters: int = 15
-    initialNetModel: str = "b2b"   # the reference defaults to "star"
+    initialNetModel: str = "star"  # "star", "b2b" or "star_b2b" (star rounds, then B2B rounds)
     maxIters: int = 100
     seed: int = 20240607
     densityLimit: float = 1.0
@@ -79,15 +79,33 @@ class Placer:
         self.stats["cg_iterations"] += r["iterations"]
         return r
 
-    def _initial_rounds(self, lx, ly, hpwl_init):
-        # B2B branch of initialRounds (simpl_placer.cc:406-448)
-        for _ in range(self.o.initMaxIters):
-            nx, ny = lx.copy(), ly.copy()
-            self.ctx.build_b2b(nx, ny)
+    def _initial_rounds(self, lx, ly, hpwl_init, star: bool):
+        """initialRounds (simpl_placer.cc:406-448).  Star: the star systems are
+        built once with the stars at the die centre, and every round continues
+        the solve from the previous round's extended solution; the first star
+        round is always accepted.  B2B: the systems are rebuilt at the current
+        placement every round.  A round is kept only while HPWL decreases."""
+        n = self.d.n
+        ex = ey = None
+        if star:
+            size = self.ctx.build_star()
+            die = self.d.die
+            ex = np.concatenate([lx, np.full(size - n, 0.5 * (die[0] + die[2]))])
+            ey = np.concatenate([ly, np.full(size - n, 0.5 * (die[1] + die[3]))])
+        for rnd in range(self.o.initMaxIters):
+            if star:
+                nx, ny = ex.copy(), ey.copy()
+            else:
+                nx, ny = lx.copy(), ly.copy()
+                self.ctx.build_b2b(nx, ny)
             self._solve(nx, ny)
+            if star:
+                ex, ey = nx, ny
+                nx, ny = nx[:n].copy(), ny[:n].copy()
             h = self.hpwl(nx, ny)
             self.stats["init_iters"] += 1
-            if not (h < hpwl_init):
+            first = star and rnd == 0
+            if not first and not (h < hpwl_init):
                 break
             lx, ly, hpwl_init = nx, ny, h
         return lx, ly, hpwl_init
@@ -98,9 +116,14 @@ class Placer:
         ctx.build_grid(o.binsPerAxis, o.densityLimit)
         lx, ly = seed_uniform(d, o.seed)
         self.stats["hpwl_seed"] = self.hpwl(lx, ly)
-        if o.initialNetModel not in ("b2b",):
-            raise NotImplementedError("only the b2b initial net model runs on the GPU path")
-        lx, ly, self.stats["hpwl_init"] = self._initial_rounds(lx, ly, self.stats["hpwl_seed"])
+        if o.initialNetModel not in ("star", "b2b", "star_b2b"):
+            raise ValueError(f"unknown initial net model {o.initialNetModel!r}")
+        hpwl_init = self.stats["hpwl_seed"]
+        if o.initialNetModel != "b2b":
+            lx, ly, hpwl_init = self._initial_rounds(lx, ly, hpwl_init, star=True)
+        if o.initialNetModel != "star":
+            lx, ly, hpwl_init = self._initial_rounds(lx, ly, hpwl_init, star=False)
+        self.stats["hpwl_init"] = hpwl_init
         trace = []
         gap_ref = 0.0
         best_upper = math.inf

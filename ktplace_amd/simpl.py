@@ -81,6 +81,7 @@ _SIGS = {
     "kt_restore_positions": (I32, [P]),
     "kt_build_b2b": (I32, [P, P, P, F64, P, P, F64, F64]),
     "kt_build_b2b_resident": (I32, [P, F64, F64, F64]),
+    "kt_build_star": (I32, [P, C.POINTER(C.c_uint64)]),
     "kt_system_size": (I32, [P, I32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "kt_get_system": (I32, [P, I32, P, P, P, P, P]),
     "kt_set_system": (I32, [P, I32, U64, P, P, P, P, P]),
@@ -305,6 +306,12 @@ class Context:
         ay = None if anchor_y is None else _f64(anchor_y)
         _check(lib.kt_build_b2b(self.h, _lib.ptr(x), _lib.ptr(y), float(alpha), _lib.ptr(ax),
                                 _lib.ptr(ay), float(fx), float(fy)))
+
+    def build_star(self) -> int:
+        """LinearSystems::buildStar (simpl_lss.cc:116-201); returns n + stars."""
+        size = C.c_uint64()
+        _check(lib.kt_build_star(self.h, C.byref(size)))
+        return size.value
 
     def build_b2b_resident(self, alpha: float = 0.0, floors=None):
         fx, fy = floors if floors is not None else self.floors

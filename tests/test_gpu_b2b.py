@@ -108,3 +108,23 @@ def test_multiply_bit_exact(gpu_ctx_factory):
     for axis in (0, 1):
         assert np.array_equal(ctx.multiply(axis, v), o.multiply(axis, v))
     ctx.close()
+
+
+@pytest.mark.parametrize("name", ["cells12k", "cells210k"])
+def test_star_bit_exact(gpu_ctx_factory, name):
+    """LinearSystems::buildStar on the GPU: same variables, CSR, diag and rhs
+    as the oracle, bit for bit."""
+    d = dz.workload(name)
+    o = ol.Oracle.of(d)
+    ctx = gpu_ctx_factory(d)
+    size = ctx.build_star()
+    assert size == o.build_star()
+    assert size > d.n
+    for axis in (0, 1):
+        g, r = ctx.get_system(axis), o.system(axis)
+        assert np.array_equal(g.row_ptr, r.row_ptr)
+        assert np.array_equal(g.col, r.col)
+        assert np.array_equal(g.val, r.val)
+        assert np.array_equal(g.diag, r.diag)
+        assert np.array_equal(g.rhs, r.rhs)
+    ctx.close()

@@ -99,6 +99,25 @@ def test_full_loop_reference_order_is_identical(gpu_ctx_factory):
     ctx.close()
 
 
+def test_full_loop_star_reference_order_is_identical(gpu_ctx_factory):
+    """The reference's default flow (star initial rounds) with the
+    reference-order solver reproduces the oracle's placement."""
+    from ktplace_amd.placer import Placer, SimplOptions
+    from ktplace_amd.simpl import KT_PREC_FP64_EXACT
+    d = dz.workload("cells12k")
+    o = ol.Oracle.of(d)
+    ctx = gpu_ctx_factory()
+    res = Placer(d, SimplOptions(initialNetModel="star", maxIters=30, precision=KT_PREC_FP64_EXACT), ctx).run()
+    ox, oy, st, tr = o.place(initial_net_model=0, max_iters=30)
+    print("gpu", res.stats["hpwl_init"], res.stats["hpwl_final"], "oracle", st["hpwl_init"], st["hpwl_final"],
+          flush=True)
+    assert res.stats["init_iters"] == st["init_iters"]
+    assert res.stats["hpwl_init"] == st["hpwl_init"]
+    assert res.stats["hpwl_final"] == st["hpwl_final"]
+    assert np.array_equal(res.x, ox) and np.array_equal(res.y, oy)
+    ctx.close()
+
+
 def test_full_loop_fast_solver_hpwl(gpu_ctx_factory):
     """The default (reordered-sum) solver follows a slightly different
     trajectory (60-iteration CG cap, chaotic LAL decisions); its final HPWL is
