@@ -178,6 +178,8 @@ int kt_set_lal_options(kt_ctx *ctx, uint64_t max_level, uint64_t min_cells_to_sp
 /* Host-pointer form: x, y [n] in/out.  Resident form: the context's positions. */
 int kt_lal(kt_ctx *ctx, double *x, double *y);
 int kt_lal_resident(kt_ctx *ctx);
+/* Batches of non-touching regions the last legalization ran (simpl_lal.cc:98-122). */
+int kt_lal_batches(kt_ctx *ctx, uint64_t *batches);
 
 /* ---- launch-shape tuning (0 keeps the default) --------------------------- */
 /* lanes: threads per CSR row in the SpMV (1, 2, 4, 8 or 16); tile: rows per

@@ -355,6 +355,7 @@ int kt_ctx_destroy(kt_ctx *ctx) {
     }
     if (ctx->ctlHost) (void)hipHostFree(ctx->ctlHost);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->lalPinned) (void)hipHostFree(ctx->lalPinned);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return KT_OK;
@@ -676,6 +677,13 @@ int kt_lal(kt_ctx *ctx, double *x, double *y) {
         KT_TRY(lalLegalize(ctx));
         return getPositions(ctx, x, y, ctx->n);
     })
+}
+
+int kt_lal_batches(kt_ctx *ctx, uint64_t *batches) {
+    KT_GUARD(ctx);
+    KT_CHECK(batches, KT_EINVAL, "kt_lal_batches: null output");
+    *batches = ctx->lalBatches;
+    return KT_OK;
 }
 
 int kt_set_warm_start(kt_ctx *ctx, const float *params, uint64_t count) {

@@ -227,6 +227,12 @@ struct kt_ctx {
     kt::DevBuf<uint64_t> occFix, lalKeys, lalKeysTmp, lalKey2, lalPrefix, lalStripeLo, lalStripeHi;
     kt::DevBuf<uint32_t> binOf, binRegion, lalSeg, lalFlag, lalIds, lalVals, lalValsTmp, lalHalf, lalStripeOf,
         lalBegin, lalEnd, lalStripeStart, lalStripeEnd, lalRemap, sortCounts, sortRaw;
+    kt::DevBuf<uint32_t> lalList, lalRects, lalCount;  // batch cells, region rectangles, moved count
+    kt::DevBuf<unsigned long long> lalMoved;           // moved-cell records (2 words each)
+    kt::lal::AreaTables lalTables;                     // free-area table of the grid, occupancy rows
+    uint8_t *lalPinned = nullptr;                      // pinned staging of per-level tables
+    size_t lalPinnedCap = 0;
+    kt::DevBuf<double> lalUp;                          // device copy of the staged tables
     kt::DevBuf<kt::LalSegStat> lalStats;
     kt::DevBuf<uint8_t> lalSplit;
 

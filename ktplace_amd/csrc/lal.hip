@@ -14,13 +14,18 @@
 //        with the reference's chunked summation order (bit-exact); overflow.
 //   Host the per-block geometry (free-area profile, cutline, stripes:
 //        lal_geom.cc), the flood fill of overfilled bins and the growth of
-//        clusters (summed-area tables), and the batch schedule.
+//        clusters, and the batch schedule.  As in the reference, occupancy
+//        is updated after each batch from the cells that changed bin only
+//        (the GPU returns them), and rectangle sums use exact fixed-point row
+//        prefixes patched on the rows that changed, so a batch costs work in
+//        proportion to its cells, not to the grid.
 //
 // Level-synchronous recursion: all blocks of one level of all regions of a
 // batch are processed together.  Cells are kept as lists of ids ordered by
 // (block, key, id) with two stable radix sorts (key, then block) started from
-// id order, which reproduces the reference's total order (simpl_parallel.h:
-// 26-33).  Area prefix sums use 64-bit fixed point, so the split index and the
+// id order, or, for up to 2048 cells, one workgroup's bitonic sort on the
+// same triple; either reproduces the reference's total order
+// (simpl_parallel.h:26-33).  Area prefix sums use 64-bit fixed point, so the split index and the
 // stripe boundaries are exact for areas that are multiples of 2^-16.
 #include <algorithm>
 #include <atomic>
@@ -38,6 +43,8 @@ int radixSortPairs(kt_ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t *keysTm
                    int bits, DevBuf<uint32_t> &counts, DevBuf<uint32_t> &raw);
 int scanInclusive(kt_ctx *ctx, uint32_t *data, uint64_t m);
 int scanInclusive64(kt_ctx *ctx, uint64_t *data, uint64_t m);
+bool smallSortFits(uint64_t n);
+int smallSortBySegment(kt_ctx *ctx, uint32_t *ids, const uint64_t *key, const uint32_t *segById, uint64_t n);
 
 namespace {
 
@@ -204,6 +211,71 @@ compactIds(const uint32_t *__restrict__ seg, const uint32_t *__restrict__ incl, 
     ids[incl[i] - 1] = i;
 }
 
+// The same over the cells of one batch (list, in id order).
+__global__ void __launch_bounds__(kB)
+listFlags(const uint32_t *__restrict__ list, const uint32_t *__restrict__ seg, uint32_t m,
+          uint32_t *__restrict__ flag) {
+    const uint32_t j = blockIdx.x * kB + threadIdx.x;
+    if (j < m) flag[j] = seg[list[j]] != kNoVar ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(kB)
+listCompact(const uint32_t *__restrict__ list, const uint32_t *__restrict__ seg, const uint32_t *__restrict__ incl,
+            uint32_t m, uint32_t *__restrict__ ids) {
+    const uint32_t j = blockIdx.x * kB + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t i = list[j];
+    if (seg[i] != kNoVar) ids[incl[j] - 1] = i;
+}
+
+__global__ void __launch_bounds__(kB)
+listFill(const uint32_t *__restrict__ list, uint32_t m, uint32_t *__restrict__ a, uint32_t v) {
+    const uint32_t j = blockIdx.x * kB + threadIdx.x;
+    if (j < m) a[list[j]] = v;
+}
+
+__global__ void __launch_bounds__(kB)
+listRemap(const uint32_t *__restrict__ list, uint32_t m, const uint32_t *__restrict__ half,
+          const uint32_t *__restrict__ remap, uint32_t *__restrict__ seg) {
+    const uint32_t j = blockIdx.x * kB + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t i = list[j];
+    const uint32_t h = half[i];
+    seg[i] = h == kNoVar ? kNoVar : remap[h];
+}
+
+// bin -> region for the bins of each region of a batch (one block per
+// region), or back to kNoVar
+__global__ void __launch_bounds__(kB)
+paintRegions(const uint4 *__restrict__ rects, uint32_t nx, uint32_t *__restrict__ binRegion, int clear) {
+    const uint4 r = rects[blockIdx.x];  // x0, x1, y0, y1
+    const uint32_t w = r.y - r.x + 1, cells = w * (r.w - r.z + 1);
+    const uint32_t v = clear ? kNoVar : blockIdx.x;
+    for (uint32_t k = threadIdx.x; k < cells; k += kB) binRegion[(r.z + k / w) * nx + r.x + k % w] = v;
+}
+
+// Cells of the batch whose bin changed: new bin stored, (old, new, area)
+// appended for the host's occupancy update.
+struct Moved {
+    uint32_t oldBin, newBin;
+    unsigned long long area;  // 2^-16 units
+};
+
+__global__ void __launch_bounds__(kB)
+movedCells(const uint32_t *__restrict__ list, uint32_t m, const double *__restrict__ x, const double *__restrict__ y,
+           const double *__restrict__ w, const double *__restrict__ h, const double *__restrict__ area, GridDev g,
+           uint32_t *__restrict__ binOf, Moved *__restrict__ out, uint32_t *__restrict__ count) {
+    const uint32_t j = blockIdx.x * kB + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t i = list[j];
+    const uint32_t k = binOfPoint(g, x[i] + 0.5 * w[i], y[i] + 0.5 * h[i]);
+    const uint32_t old = binOf[i];
+    if (k == old) return;
+    binOf[i] = k;
+    const uint32_t slot = atomicAdd(count, 1u);
+    out[slot] = Moved{old, k, static_cast<unsigned long long>(llrint(area[i] * kFixScale))};
+}
+
 struct CentreArgs {
     const double *pos;   // positions along the level's axis
     const double *size;  // widths (x) or heights (y)
@@ -225,31 +297,38 @@ segmentKeys(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ seg, 
 }
 
 // segment ranges of a (segment, ...) sorted id list; begin/end pre-filled with 0
-__global__ void __launch_bounds__(kB)
-segmentBounds(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ seg, uint32_t A,
-              uint32_t *__restrict__ begin, uint32_t *__restrict__ end) {
-    const uint32_t k = blockIdx.x * kB + threadIdx.x;
-    if (k >= A) return;
+__device__ __forceinline__ void segmentBoundsAt(uint32_t k, const uint32_t *__restrict__ ids,
+                                                const uint32_t *__restrict__ seg, uint32_t A,
+                                                uint32_t *__restrict__ begin, uint32_t *__restrict__ end) {
     const uint32_t s = seg[ids[k]];
     if (k == 0 || seg[ids[k - 1]] != s) begin[s] = k;
     if (k + 1 == A || seg[ids[k + 1]] != s) end[s] = k + 1;
 }
 
 __global__ void __launch_bounds__(kB)
+segmentBounds(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ seg, uint32_t A,
+              uint32_t *__restrict__ begin, uint32_t *__restrict__ end) {
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k < A) segmentBoundsAt(k, ids, seg, A, begin, end);
+}
+
+__device__ __forceinline__ uint64_t areaFixOf(const double *__restrict__ area, uint32_t i) {
+    return static_cast<uint64_t>(llrint(area[i] * kFixScale));
+}
+
+__global__ void __launch_bounds__(kB)
 areaFix(const uint32_t *__restrict__ ids, const double *__restrict__ area, uint32_t A, uint64_t *__restrict__ out) {
     const uint32_t k = blockIdx.x * kB + threadIdx.x;
-    if (k < A) out[k] = static_cast<uint64_t>(llrint(area[ids[k]] * kFixScale));
+    if (k < A) out[k] = areaFixOf(area, ids[k]);
 }
 
 using SegStat = LalSegStat;
 
 // Split of each block at half its cell area (simpl_lal.cc:225-236), from the
 // inclusive fixed-point prefix P of the sorted areas.
-__global__ void __launch_bounds__(kB)
-splitBlocks(const uint32_t *__restrict__ begin, const uint32_t *__restrict__ end, const uint64_t *__restrict__ P,
-            uint32_t B, SegStat *__restrict__ out) {
-    const uint32_t s = blockIdx.x * kB + threadIdx.x;
-    if (s >= B) return;
+__device__ __forceinline__ void splitBlockAt(uint32_t s, const uint32_t *__restrict__ begin,
+                                             const uint32_t *__restrict__ end, const uint64_t *__restrict__ P,
+                                             SegStat *__restrict__ out) {
     const uint32_t b = begin[s], e = end[s];
     SegStat st{};
     st.count = e - b;
@@ -270,6 +349,13 @@ splitBlocks(const uint32_t *__restrict__ begin, const uint32_t *__restrict__ end
     out[s] = st;
 }
 
+__global__ void __launch_bounds__(kB)
+splitBlocks(const uint32_t *__restrict__ begin, const uint32_t *__restrict__ end, const uint64_t *__restrict__ P,
+            uint32_t B, SegStat *__restrict__ out) {
+    const uint32_t s = blockIdx.x * kB + threadIdx.x;
+    if (s < B) splitBlockAt(s, begin, end, P, out);
+}
+
 struct HalfArgs {
     const uint32_t *begin;     // per block, sorted by (block, centre, id)
     const SegStat *stat;
@@ -280,11 +366,9 @@ struct HalfArgs {
 
 // Half of each cell (low: first m cells of its block), key = -fromCut(centre)
 // (simpl_lal.cc:284-287); cells of blocks that do not split leave the level.
-__global__ void __launch_bounds__(kB)
-halfKeys(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ seg, uint32_t A, HalfArgs h,
-         uint32_t *__restrict__ half, uint64_t *__restrict__ key2) {
-    const uint32_t k = blockIdx.x * kB + threadIdx.x;
-    if (k >= A) return;
+__device__ __forceinline__ void halfKeyAt(uint32_t k, const uint32_t *__restrict__ ids,
+                                          const uint32_t *__restrict__ seg, const HalfArgs &h,
+                                          uint32_t *__restrict__ half, uint64_t *__restrict__ key2) {
     const uint32_t i = ids[k];
     const uint32_t s = seg[i];
     if (!h.split[s]) {
@@ -297,6 +381,13 @@ halfKeys(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ seg, uin
     const double fromCut = low ? cut - c : c - cut;
     half[i] = 2 * s + (low ? 0u : 1u);
     key2[i] = orderKey(-fromCut);
+}
+
+__global__ void __launch_bounds__(kB)
+halfKeys(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ seg, uint32_t A, HalfArgs h,
+         uint32_t *__restrict__ half, uint64_t *__restrict__ key2) {
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k < A) halfKeyAt(k, ids, seg, h, half, key2);
 }
 
 __global__ void __launch_bounds__(kB)
@@ -320,11 +411,9 @@ struct StripeArgs {
 // With the inclusive fixed-point prefix P over the sorted list, stripe s ends
 // at the first cell index k with (P[k-1] - P[start_s - 1]) >= cap_s; the last
 // stripe takes the rest.
-__global__ void __launch_bounds__(kB)
-stripeRanges(const uint32_t *__restrict__ begin, const uint32_t *__restrict__ end, const uint64_t *__restrict__ P,
-             uint32_t halves, StripeArgs sa) {
-    const uint32_t hh = blockIdx.x * kB + threadIdx.x;
-    if (hh >= halves) return;
+__device__ __forceinline__ void stripeRangeAt(uint32_t hh, const uint32_t *__restrict__ begin,
+                                              const uint32_t *__restrict__ end, const uint64_t *__restrict__ P,
+                                              const StripeArgs &sa) {
     const uint32_t s0 = sa.stripeStart[hh], s1 = sa.stripeStart[hh + 1];
     if (s0 == s1) return;
     const uint32_t b = begin[hh], e = end[hh];
@@ -350,10 +439,15 @@ stripeRanges(const uint32_t *__restrict__ begin, const uint32_t *__restrict__ en
 }
 
 __global__ void __launch_bounds__(kB)
-stripeOfCell(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ half, uint32_t A, StripeArgs sa,
-             CentreArgs c, uint32_t *__restrict__ stripeOf) {
-    const uint32_t k = blockIdx.x * kB + threadIdx.x;
-    if (k >= A) return;
+stripeRanges(const uint32_t *__restrict__ begin, const uint32_t *__restrict__ end, const uint64_t *__restrict__ P,
+             uint32_t halves, StripeArgs sa) {
+    const uint32_t hh = blockIdx.x * kB + threadIdx.x;
+    if (hh < halves) stripeRangeAt(hh, begin, end, P, sa);
+}
+
+__device__ __forceinline__ void stripeOfCellAt(uint32_t k, const uint32_t *__restrict__ ids,
+                                               const uint32_t *__restrict__ half, const StripeArgs &sa,
+                                               const CentreArgs &c, uint32_t *__restrict__ stripeOf) {
     const uint32_t i = ids[k];
     const uint32_t hh = half[i];
     uint32_t s = sa.stripeStart[hh];
@@ -365,15 +459,19 @@ stripeOfCell(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ half
     atomicMax(sa.hi + s, key);
 }
 
-// Affine map inside each stripe (simpl_lal.cc:297-314).
 __global__ void __launch_bounds__(kB)
-scaleCells(const uint32_t *__restrict__ ids, uint32_t A, const uint32_t *__restrict__ stripeOf, StripeArgs sa,
-           CentreArgs c, double *__restrict__ posOut) {
+stripeOfCell(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ half, uint32_t A, StripeArgs sa,
+             CentreArgs c, uint32_t *__restrict__ stripeOf) {
     const uint32_t k = blockIdx.x * kB + threadIdx.x;
-    if (k >= A) return;
+    if (k < A) stripeOfCellAt(k, ids, half, sa, c, stripeOf);
+}
+
+// Affine map inside each stripe (simpl_lal.cc:297-314), stripe extent given.
+__device__ __forceinline__ void scaleCellWith(uint32_t k, const uint32_t *__restrict__ ids, uint32_t s,
+                                              unsigned long long loKey, unsigned long long hiKey, const StripeArgs &sa,
+                                              const CentreArgs &c, double *__restrict__ posOut) {
     const uint32_t i = ids[k];
-    const uint32_t s = stripeOf[k];
-    const double lo = fromOrderKey(sa.lo[s]), hi = fromOrderKey(sa.hi[s]);
+    const double lo = fromOrderKey(loKey), hi = fromOrderKey(hiKey);
     const double ci = centreOf(c, i);
     const double t = hi > lo ? (ci - lo) / (hi - lo) : 0.5;
     const double size = c.size[i];
@@ -382,6 +480,177 @@ scaleCells(const uint32_t *__restrict__ ids, uint32_t A, const uint32_t *__restr
     const double upper = start < stop - size ? stop - size : start;
     const double v = cc - 0.5 * size;
     posOut[i] = v < start ? start : (upper < v ? upper : v);
+}
+
+__global__ void __launch_bounds__(kB)
+scaleCells(const uint32_t *__restrict__ ids, uint32_t A, const uint32_t *__restrict__ stripeOf, StripeArgs sa,
+           CentreArgs c, double *__restrict__ posOut) {
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k >= A) return;
+    const uint32_t s = stripeOf[k];
+    scaleCellWith(k, ids, s, sa.lo[s], sa.hi[s], sa, c, posOut);
+}
+
+// ------------------------------------------------------------ small batches
+// A batch of at most kSmallCells cells runs each half of a level in one
+// workgroup: the same per-element routines as the multi-kernel path, with
+// workgroup barriers between the steps instead of kernel boundaries, and the
+// orders taken by one bitonic sort of (block, key, id) in LDS.  Inactive cells
+// sort last, so the list needs no compaction.
+constexpr uint32_t kSmallCells = 2048;
+constexpr int kSmallThreads = 1024;
+
+__device__ void bitonicTriples(uint32_t *ss, uint64_t *sk, uint32_t *si, uint32_t width) {
+    for (uint32_t k = 2; k <= width; k <<= 1) {
+        for (uint32_t d = k >> 1; d > 0; d >>= 1) {
+            for (uint32_t t = threadIdx.x; t < width; t += blockDim.x) {
+                const uint32_t u = t ^ d;
+                if (u > t) {
+                    const bool greater = ss[t] != ss[u] ? ss[t] > ss[u] : sk[t] != sk[u] ? sk[t] > sk[u] : si[t] > si[u];
+                    if (greater == ((t & k) == 0)) {
+                        const uint32_t s0 = ss[t], i0 = si[t];
+                        const uint64_t k0 = sk[t];
+                        ss[t] = ss[u];
+                        sk[t] = sk[u];
+                        si[t] = si[u];
+                        ss[u] = s0;
+                        sk[u] = k0;
+                        si[u] = i0;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Inclusive prefix sum in place of v[0, n), n <= 2 * blockDim.x (exact:
+// integer sums do not depend on the order).
+__device__ void blockScanU64(uint64_t *v, uint32_t n, uint64_t *waveTotals) {
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint64_t a0 = 2 * t < n ? v[2 * t] : 0ull, a1 = 2 * t + 1 < n ? v[2 * t + 1] : 0ull;
+    const uint64_t local = a0 + a1;
+    uint64_t x = local;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t y = __shfl_up(x, off, 64);
+        if (lane >= static_cast<uint32_t>(off)) x += y;
+    }
+    if (lane == 63) waveTotals[wave] = x;
+    __syncthreads();
+    uint64_t offset = 0;
+    for (uint32_t w = 0; w < wave; ++w) offset += waveTotals[w];
+    const uint64_t excl = offset + x - local;
+    if (2 * t < n) v[2 * t] = excl + a0;
+    if (2 * t + 1 < n) v[2 * t + 1] = excl + a0 + a1;
+    __syncthreads();
+}
+
+struct SmallArgs {
+    const uint32_t *list;   // the batch's cells
+    uint32_t M, width;      // cells, sort width (power of two >= M)
+    uint32_t *seg, *half;   // per cell
+    uint64_t *key2;         // per cell
+    CentreArgs c;
+    const double *area;
+    uint32_t *ids;          // sorted cell ids of the level
+    uint32_t *begin, *end;  // per block / half
+    uint64_t *P;            // fixed-point area prefix
+    SegStat *stats;
+    uint32_t *stripeOf;
+    uint32_t B, A;          // blocks and active cells of the level
+};
+
+// centre order, block ranges, area prefix, split (first half of a level)
+__global__ void __launch_bounds__(kSmallThreads) smallLevelA(SmallArgs a) {
+    __shared__ uint32_t ss[kSmallCells], si[kSmallCells];
+    __shared__ uint64_t sk[kSmallCells];
+    __shared__ uint64_t waveTotals[kSmallThreads / 64];
+    for (uint32_t j = threadIdx.x; j < a.width; j += blockDim.x) {
+        if (j < a.M) {
+            const uint32_t id = a.list[j], sg = a.seg[id];
+            ss[j] = sg;
+            si[j] = id;
+            sk[j] = sg != kNoVar ? orderKey(centreOf(a.c, id)) : ~0ull;
+        } else {
+            ss[j] = kNoVar;
+            si[j] = kNoVar;
+            sk[j] = ~0ull;
+        }
+    }
+    __syncthreads();
+    bitonicTriples(ss, sk, si, a.width);
+    for (uint32_t k = threadIdx.x; k < a.A; k += blockDim.x) a.ids[k] = si[k];
+    for (uint32_t q = threadIdx.x; q < 2 * a.B + 2; q += blockDim.x) {
+        a.begin[q] = 0;
+        a.end[q] = 0;
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < a.A; k += blockDim.x) {
+        segmentBoundsAt(k, a.ids, a.seg, a.A, a.begin, a.end);
+        a.P[k] = areaFixOf(a.area, a.ids[k]);
+    }
+    __syncthreads();
+    blockScanU64(a.P, a.A, waveTotals);
+    for (uint32_t s = threadIdx.x; s < a.B; s += blockDim.x) splitBlockAt(s, a.begin, a.end, a.P, a.stats);
+}
+
+// halves, distance order, stripes, affine map, next level's blocks
+__global__ void __launch_bounds__(kSmallThreads)
+smallLevelB(SmallArgs a, HalfArgs ha, StripeArgs sa, const uint32_t *__restrict__ remap, uint32_t H, uint32_t S,
+            uint32_t A2, double *__restrict__ posOut) {
+    __shared__ uint32_t ss[kSmallCells], si[kSmallCells];
+    __shared__ uint64_t sk[kSmallCells];
+    __shared__ uint64_t waveTotals[kSmallThreads / 64];
+    for (uint32_t j = threadIdx.x; j < a.M; j += blockDim.x) a.half[a.list[j]] = kNoVar;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < a.A; k += blockDim.x) halfKeyAt(k, a.ids, a.seg, ha, a.half, a.key2);
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < a.width; j += blockDim.x) {
+        if (j < a.M) {
+            const uint32_t id = a.list[j], h = a.half[id];
+            ss[j] = h;
+            si[j] = id;
+            sk[j] = h != kNoVar ? a.key2[id] : ~0ull;
+        } else {
+            ss[j] = kNoVar;
+            si[j] = kNoVar;
+            sk[j] = ~0ull;
+        }
+    }
+    __syncthreads();
+    bitonicTriples(ss, sk, si, a.width);
+    for (uint32_t k = threadIdx.x; k < A2; k += blockDim.x) a.ids[k] = si[k];
+    for (uint32_t q = threadIdx.x; q < 2 * a.B + 2; q += blockDim.x) {
+        a.begin[q] = 0;
+        a.end[q] = 0;
+    }
+    for (uint32_t q = threadIdx.x; q < S + 1; q += blockDim.x) {
+        sa.lo[q] = ~0ull;
+        sa.hi[q] = 0ull;
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < A2; k += blockDim.x) {
+        segmentBoundsAt(k, a.ids, a.half, A2, a.begin, a.end);
+        a.P[k] = areaFixOf(a.area, a.ids[k]);
+    }
+    __syncthreads();
+    blockScanU64(a.P, A2, waveTotals);
+    for (uint32_t hh = threadIdx.x; hh < H; hh += blockDim.x) stripeRangeAt(hh, a.begin, a.end, a.P, sa);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < A2; k += blockDim.x) stripeOfCellAt(k, a.ids, a.half, sa, a.c, a.stripeOf);
+    __threadfence();
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < A2; k += blockDim.x) {
+        // the stripe extents were formed by atomics: read them back at device scope
+        const uint32_t st = a.stripeOf[k];
+        const unsigned long long lo = __hip_atomic_load(sa.lo + st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long hi = __hip_atomic_load(sa.hi + st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        scaleCellWith(k, a.ids, st, lo, hi, sa, a.c, posOut);
+    }
+    for (uint32_t j = threadIdx.x; j < a.M; j += blockDim.x) {
+        const uint32_t id = a.list[j], h = a.half[id];
+        a.seg[id] = h == kNoVar ? kNoVar : remap[h];
+    }
 }
 
 // simpl_lal.cc:365-372
@@ -452,6 +721,7 @@ int lalBuildGrid(kt_ctx *ctx, uint64_t perAxis, double densityLimit) {
     KT_CHECK(ctx->haveNetlist && ctx->haveDesign, KT_ESTATE, "build_grid: netlist and rows needed");
     KT_CHECK(perAxis > 0 && perAxis <= 4096, KT_EINVAL, "build_grid: bins per axis out of range");
     lal::buildGrid(ctx->grid, ctx->freeSpace, ctx->lalRows, ctx->obstacles, ctx->lalDie, perAxis);
+    ctx->lalTables.buildAvail(ctx->grid);
     ctx->densityLimit = std::clamp(densityLimit, 0.05, 1.0);
     KT_TRY(ctx->gridAvail.reserve(ctx->grid.size()));
     KT_HIP(hipMemcpyAsync(ctx->gridAvail.p, ctx->grid.avail.data(), ctx->grid.size() * sizeof(double),
@@ -540,70 +810,132 @@ int lalHpwl(kt_ctx *ctx, double *out) {
 }
 
 // One batch of non-touching regions (simpl_lal.cc:122-151, 178-316).
-int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std::vector<double> &occHost) {
+// Pinned host staging and its device twin for the per-level tables.
+int lalStaging(kt_ctx *ctx, size_t bytes) {
+    if (bytes > ctx->lalPinnedCap) {
+        KT_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->lalPinned) (void)hipHostFree(ctx->lalPinned);
+        ctx->lalPinned = nullptr;
+        ctx->lalPinnedCap = 0;
+        const size_t cap = std::max<size_t>(bytes, 1 << 16);
+        KT_HIP(hipHostMalloc(reinterpret_cast<void **>(&ctx->lalPinned), cap));
+        ctx->lalPinnedCap = cap;
+    }
+    KT_TRY(ctx->lalUp.reserve((bytes + 7) / 8));
+    return KT_OK;
+}
+
+// Legalizes one batch of non-touching regions; the cells whose bin changed
+// are returned in `moved` (their device binOf is updated).
+int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, std::vector<Moved> &moved) {
     const uint32_t n = ctx->n;
     const lal::Grid &g = ctx->grid;
+    moved.clear();
     if (regions.empty() || n == 0) return KT_OK;
-    // bin -> region index
-    std::vector<uint32_t> binRegion(g.size(), kNoVar);
     struct Block {
         lal::Box box;
         double regionRoom;
     };
     std::vector<Block> blocks(regions.size());
+    std::vector<uint4> rects(regions.size());
     for (size_t r = 0; r < regions.size(); ++r) {
         const lal::BinRect &b = regions[r];
-        for (size_t iy = b.y0; iy <= b.y1; ++iy)
-            for (size_t ix = b.x0; ix <= b.x1; ++ix) binRegion[g.at(ix, iy)] = static_cast<uint32_t>(r);
+        rects[r] = make_uint4(static_cast<uint32_t>(b.x0), static_cast<uint32_t>(b.x1), static_cast<uint32_t>(b.y0),
+                              static_cast<uint32_t>(b.y1));
         blocks[r] = {g.box(b), g.sum(g.avail, b)};
     }
-    KT_TRY(ctx->binRegion.reserve(g.size()));
-    KT_HIP(hipMemcpyAsync(ctx->binRegion.p, binRegion.data(), g.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                          ctx->stream));
+    const uint32_t R = static_cast<uint32_t>(regions.size());
+    // bin -> region for the batch's bins (every other bin holds kNoVar)
+    KT_TRY(ctx->lalRects.reserve(4ull * R));
+    KT_HIP(hipMemcpyAsync(ctx->lalRects.p, rects.data(), R * sizeof(uint4), hipMemcpyHostToDevice, ctx->stream));
+    const uint4 *dRects = reinterpret_cast<const uint4 *>(ctx->lalRects.p);
+    paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 0);
+    KT_HIP(hipGetLastError());
     // per-cell scratch
     for (auto *b : {&ctx->lalSeg, &ctx->lalFlag, &ctx->lalIds, &ctx->lalVals, &ctx->lalValsTmp, &ctx->lalHalf,
-                    &ctx->lalStripeOf})
+                    &ctx->lalStripeOf, &ctx->lalList})
         KT_TRY(b->reserve(n));
     for (auto *b : {&ctx->lalKeys, &ctx->lalKeysTmp, &ctx->lalKey2, &ctx->lalPrefix}) KT_TRY(b->reserve(n));
+    // the batch's cells, in id order
     regionFlags<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->binOf.p, ctx->binRegion.p, n, ctx->lalSeg.p, ctx->lalFlag.p);
     KT_HIP(hipGetLastError());
+    KT_TRY(scanInclusive(ctx, ctx->lalFlag.p, n));
+    compactIds<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalSeg.p, ctx->lalFlag.p, n, ctx->lalList.p);
+    KT_HIP(hipGetLastError());
+    paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 1);
+    KT_HIP(hipGetLastError());
+    uint32_t M = 0;
+    KT_HIP(hipMemcpyAsync(&M, ctx->lalFlag.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    KT_HIP(hipStreamSynchronize(ctx->stream));
+    if (M == 0) return KT_OK;
+    const uint32_t *list = ctx->lalList.p;
+    // batches of up to kSmallCells cells: one workgroup per half level
+    const bool small = M <= kSmallCells;
+    uint32_t width = 2;
+    while (width < M) width <<= 1;
 
     bool vertical = true;
-    uint32_t A = 0;
+    // cells of the level: all of the batch at level 1, then those of the
+    // blocks that split (known on the host from the block statistics)
+    uint32_t A = M;
     for (size_t level = 1;; ++level) {
-        // active cells in id order
-        activeFlags<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalSeg.p, n, ctx->lalFlag.p);
-        KT_HIP(hipGetLastError());
-        KT_TRY(scanInclusive(ctx, ctx->lalFlag.p, n));
-        KT_HIP(hipMemcpyAsync(&A, ctx->lalFlag.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-        KT_HIP(hipStreamSynchronize(ctx->stream));
         if (A == 0) break;
-        compactIds<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalSeg.p, ctx->lalFlag.p, n, ctx->lalIds.p);
-        KT_HIP(hipGetLastError());
         const uint32_t B = static_cast<uint32_t>(blocks.size());
         const CentreArgs ca{vertical ? ctx->posX.p : ctx->posY.p, vertical ? ctx->width.p : ctx->height.p};
-        // sort by (block, centre, id): stable sorts by centre, then by block
-        centreKeys<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, A, ca, ctx->lalKeys.p);
-        KT_HIP(hipGetLastError());
-        KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A, 64,
-                              ctx->sortCounts, ctx->sortRaw));
-        segmentKeys<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalSeg.p, A, ctx->lalKeys.p);
-        KT_HIP(hipGetLastError());
-        KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A,
-                              bitsFor(B), ctx->sortCounts, ctx->sortRaw));
-        // block ranges, area prefix, split
         KT_TRY(ctx->lalBegin.reserve(2ull * B + 2));
         KT_TRY(ctx->lalEnd.reserve(2ull * B + 2));
-        fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalBegin.p, 0u, 2ull * B + 2);
-        fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalEnd.p, 0u, 2ull * B + 2);
-        segmentBounds<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalSeg.p, A, ctx->lalBegin.p,
-                                                          ctx->lalEnd.p);
-        areaFix<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->area.p, A, ctx->lalPrefix.p);
-        KT_TRY(scanInclusive64(ctx, ctx->lalPrefix.p, A));
         KT_TRY(ctx->lalStats.reserve(B));
-        splitBlocks<<<gridFor(B), kB, 0, ctx->stream>>>(ctx->lalBegin.p, ctx->lalEnd.p, ctx->lalPrefix.p, B,
-                                                        ctx->lalStats.p);
-        KT_HIP(hipGetLastError());
+        SmallArgs sm{};
+        if (small) {
+            sm.list = list;
+            sm.M = M;
+            sm.width = width;
+            sm.seg = ctx->lalSeg.p;
+            sm.half = ctx->lalHalf.p;
+            sm.key2 = ctx->lalKey2.p;
+            sm.c = ca;
+            sm.area = ctx->area.p;
+            sm.ids = ctx->lalIds.p;
+            sm.begin = ctx->lalBegin.p;
+            sm.end = ctx->lalEnd.p;
+            sm.P = ctx->lalPrefix.p;
+            sm.stats = ctx->lalStats.p;
+            sm.stripeOf = ctx->lalStripeOf.p;
+            sm.B = B;
+            sm.A = A;
+            smallLevelA<<<1, kSmallThreads, 0, ctx->stream>>>(sm);
+            KT_HIP(hipGetLastError());
+        } else {
+            // active cells in id order
+            listFlags<<<gridFor(M), kB, 0, ctx->stream>>>(list, ctx->lalSeg.p, M, ctx->lalFlag.p);
+            KT_HIP(hipGetLastError());
+            KT_TRY(scanInclusive(ctx, ctx->lalFlag.p, M));
+            listCompact<<<gridFor(M), kB, 0, ctx->stream>>>(list, ctx->lalSeg.p, ctx->lalFlag.p, M, ctx->lalIds.p);
+            KT_HIP(hipGetLastError());
+            // sort by (block, centre, id): stable sorts by centre, then by block
+            centreKeys<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, A, ca, ctx->lalKeys.p);
+            KT_HIP(hipGetLastError());
+            if (smallSortFits(A)) {
+                KT_TRY(smallSortBySegment(ctx, ctx->lalIds.p, ctx->lalKeys.p, ctx->lalSeg.p, A));
+            } else {
+                KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A,
+                                      64, ctx->sortCounts, ctx->sortRaw));
+                segmentKeys<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalSeg.p, A, ctx->lalKeys.p);
+                KT_HIP(hipGetLastError());
+                KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A,
+                                      bitsFor(B), ctx->sortCounts, ctx->sortRaw));
+            }
+            // block ranges, area prefix, split
+            fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalBegin.p, 0u, 2ull * B + 2);
+            fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalEnd.p, 0u, 2ull * B + 2);
+            segmentBounds<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalSeg.p, A, ctx->lalBegin.p,
+                                                              ctx->lalEnd.p);
+            areaFix<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->area.p, A, ctx->lalPrefix.p);
+            KT_TRY(scanInclusive64(ctx, ctx->lalPrefix.p, A));
+            splitBlocks<<<gridFor(B), kB, 0, ctx->stream>>>(ctx->lalBegin.p, ctx->lalEnd.p, ctx->lalPrefix.p, B,
+                                                            ctx->lalStats.p);
+            KT_HIP(hipGetLastError());
+        }
         std::vector<SegStat> stats(B);
         KT_HIP(hipMemcpyAsync(stats.data(), ctx->lalStats.p, B * sizeof(SegStat), hipMemcpyDeviceToHost, ctx->stream));
         KT_HIP(hipStreamSynchronize(ctx->stream));
@@ -649,8 +981,12 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
             next[2 * s] = {lowBox, bl.regionRoom};
             next[2 * s + 1] = {highBox, bl.regionRoom};
         });
-        uint32_t splits = 0;
-        for (uint8_t f : split) splits += f;
+        uint32_t splits = 0, A2 = 0;
+        for (uint32_t s = 0; s < B; ++s)
+            if (split[s]) {
+                ++splits;
+                A2 += stats[s].count;
+            }
         if (splits == 0) break;
         // stripe tables
         const uint32_t H = 2 * B;
@@ -665,64 +1001,6 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
                 span[2 * s + 1] = halfStripes[h][q].end;
                 cap[s] = halfStripes[h][q].capacity;
             }
-        KT_TRY(ctx->lalSplit.reserve(B));
-        KT_TRY(ctx->lalCut.reserve(B));
-        KT_TRY(ctx->lalStripeStart.reserve(H + 1));
-        KT_TRY(ctx->lalSpan.reserve(2ull * S + 2));
-        KT_TRY(ctx->lalCap.reserve(S + 1));
-        KT_TRY(ctx->lalStripeEnd.reserve(S + 1));
-        KT_TRY(ctx->lalStripeLo.reserve(S + 1));
-        KT_TRY(ctx->lalStripeHi.reserve(S + 1));
-        KT_HIP(hipMemcpyAsync(ctx->lalSplit.p, split.data(), B, hipMemcpyHostToDevice, ctx->stream));
-        KT_HIP(hipMemcpyAsync(ctx->lalCut.p, cut.data(), B * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        KT_HIP(hipMemcpyAsync(ctx->lalStripeStart.p, stripeStart.data(), (H + 1) * sizeof(uint32_t),
-                              hipMemcpyHostToDevice, ctx->stream));
-        KT_HIP(hipMemcpyAsync(ctx->lalSpan.p, span.data(), span.size() * sizeof(double), hipMemcpyHostToDevice,
-                              ctx->stream));
-        KT_HIP(hipMemcpyAsync(ctx->lalCap.p, cap.data(), cap.size() * sizeof(double), hipMemcpyHostToDevice,
-                              ctx->stream));
-        // halves and the second order: by (half, -fromCut(centre), id)
-        HalfArgs ha{ctx->lalBegin.p, ctx->lalStats.p, ctx->lalSplit.p, ctx->lalCut.p, ca};
-        fillU32<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalHalf.p, kNoVar, n);
-        halfKeys<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalSeg.p, A, ha, ctx->lalHalf.p,
-                                                     ctx->lalKey2.p);
-        KT_HIP(hipGetLastError());
-        // cells of non-splitting blocks leave; the rest are ordered by half
-        activeFlags<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalHalf.p, n, ctx->lalFlag.p);
-        KT_HIP(hipGetLastError());
-        KT_TRY(scanInclusive(ctx, ctx->lalFlag.p, n));
-        uint32_t A2 = 0;
-        KT_HIP(hipMemcpyAsync(&A2, ctx->lalFlag.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-        KT_HIP(hipStreamSynchronize(ctx->stream));
-        if (A2 == 0) break;
-        compactIds<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalHalf.p, ctx->lalFlag.p, n, ctx->lalIds.p);
-        gatherKeys<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalKey2.p, A2, ctx->lalKeys.p);
-        KT_HIP(hipGetLastError());
-        KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A2, 64,
-                              ctx->sortCounts, ctx->sortRaw));
-        segmentKeys<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalHalf.p, A2, ctx->lalKeys.p);
-        KT_HIP(hipGetLastError());
-        KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A2,
-                              bitsFor(H), ctx->sortCounts, ctx->sortRaw));
-        fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalBegin.p, 0u, 2ull * B + 2);
-        fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalEnd.p, 0u, 2ull * B + 2);
-        segmentBounds<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalHalf.p, A2, ctx->lalBegin.p,
-                                                           ctx->lalEnd.p);
-        areaFix<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->area.p, A2, ctx->lalPrefix.p);
-        KT_TRY(scanInclusive64(ctx, ctx->lalPrefix.p, A2));
-        StripeArgs sa{ctx->lalStripeStart.p, ctx->lalSpan.p, ctx->lalCap.p, ctx->lalStripeEnd.p,
-                      reinterpret_cast<unsigned long long *>(ctx->lalStripeLo.p),
-                      reinterpret_cast<unsigned long long *>(ctx->lalStripeHi.p)};
-        fillU64<<<gridFor(S + 1), kB, 0, ctx->stream>>>(sa.lo, ~0ull, S + 1);
-        fillU64<<<gridFor(S + 1), kB, 0, ctx->stream>>>(sa.hi, 0ull, S + 1);
-        stripeRanges<<<gridFor(H), kB, 0, ctx->stream>>>(ctx->lalBegin.p, ctx->lalEnd.p, ctx->lalPrefix.p, H, sa);
-        KT_HIP(hipGetLastError());
-        stripeOfCell<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalHalf.p, A2, sa, ca,
-                                                          ctx->lalStripeOf.p);
-        KT_HIP(hipGetLastError());
-        scaleCells<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, A2, ctx->lalStripeOf.p, sa, ca,
-                                                        const_cast<double *>(ca.pos));
-        KT_HIP(hipGetLastError());
         // next level: the halves that hold cells, renumbered densely
         std::vector<uint32_t> remap(H, kNoVar);
         std::vector<Block> kept;
@@ -737,12 +1015,99 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
                 kept.push_back(next[2 * s + 1]);
             }
         }
-        KT_TRY(ctx->lalRemap.reserve(H));
-        KT_HIP(hipMemcpyAsync(ctx->lalRemap.p, remap.data(), H * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
-        remapSegments<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalHalf.p, ctx->lalRemap.p, n, ctx->lalSeg.p);
-        KT_HIP(hipGetLastError());
+        // one upload of the level's host tables through pinned memory
+        const auto up8 = [](size_t bytes) { return (bytes + 7) & ~size_t{7}; };
+        const size_t oCut = 0, oSpan = oCut + up8(B * sizeof(double)), oCap = oSpan + up8(span.size() * sizeof(double)),
+                     oStart = oCap + up8(cap.size() * sizeof(double)),
+                     oRemap = oStart + up8((H + 1) * sizeof(uint32_t)), oSplit = oRemap + up8(H * sizeof(uint32_t)),
+                     total = oSplit + up8(B);
+        KT_TRY(lalStaging(ctx, total));
+        uint8_t *host = ctx->lalPinned;
+        std::memcpy(host + oCut, cut.data(), B * sizeof(double));
+        std::memcpy(host + oSpan, span.data(), span.size() * sizeof(double));
+        std::memcpy(host + oCap, cap.data(), cap.size() * sizeof(double));
+        std::memcpy(host + oStart, stripeStart.data(), (H + 1) * sizeof(uint32_t));
+        std::memcpy(host + oRemap, remap.data(), H * sizeof(uint32_t));
+        std::memcpy(host + oSplit, split.data(), B);
+        uint8_t *dev = reinterpret_cast<uint8_t *>(ctx->lalUp.p);
+        KT_HIP(hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, ctx->stream));
+        const double *dCut = reinterpret_cast<const double *>(dev + oCut);
+        const double *dSpan = reinterpret_cast<const double *>(dev + oSpan);
+        const double *dCap = reinterpret_cast<const double *>(dev + oCap);
+        const uint32_t *dStart = reinterpret_cast<const uint32_t *>(dev + oStart);
+        const uint32_t *dRemap = reinterpret_cast<const uint32_t *>(dev + oRemap);
+        const uint8_t *dSplit = dev + oSplit;
+        KT_TRY(ctx->lalStripeEnd.reserve(S + 1));
+        KT_TRY(ctx->lalStripeLo.reserve(S + 1));
+        KT_TRY(ctx->lalStripeHi.reserve(S + 1));
+        // halves and the second order: by (half, -fromCut(centre), id)
+        HalfArgs ha{ctx->lalBegin.p, ctx->lalStats.p, dSplit, dCut, ca};
+        StripeArgs sa{dStart, dSpan, dCap, ctx->lalStripeEnd.p,
+                      reinterpret_cast<unsigned long long *>(ctx->lalStripeLo.p),
+                      reinterpret_cast<unsigned long long *>(ctx->lalStripeHi.p)};
+        if (small) {
+            smallLevelB<<<1, kSmallThreads, 0, ctx->stream>>>(sm, ha, sa, dRemap, H, S, A2, const_cast<double *>(ca.pos));
+            KT_HIP(hipGetLastError());
+        } else {
+            listFill<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, ctx->lalHalf.p, kNoVar);
+            halfKeys<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalSeg.p, A, ha, ctx->lalHalf.p,
+                                                         ctx->lalKey2.p);
+            KT_HIP(hipGetLastError());
+            // cells of non-splitting blocks leave; the rest are ordered by half
+            listFlags<<<gridFor(M), kB, 0, ctx->stream>>>(list, ctx->lalHalf.p, M, ctx->lalFlag.p);
+            KT_HIP(hipGetLastError());
+            KT_TRY(scanInclusive(ctx, ctx->lalFlag.p, M));
+            listCompact<<<gridFor(M), kB, 0, ctx->stream>>>(list, ctx->lalHalf.p, ctx->lalFlag.p, M, ctx->lalIds.p);
+            gatherKeys<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalKey2.p, A2, ctx->lalKeys.p);
+            KT_HIP(hipGetLastError());
+            if (smallSortFits(A2)) {
+                KT_TRY(smallSortBySegment(ctx, ctx->lalIds.p, ctx->lalKeys.p, ctx->lalHalf.p, A2));
+            } else {
+                KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A2, 64,
+                                      ctx->sortCounts, ctx->sortRaw));
+                segmentKeys<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalHalf.p, A2, ctx->lalKeys.p);
+                KT_HIP(hipGetLastError());
+                KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A2,
+                                      bitsFor(H), ctx->sortCounts, ctx->sortRaw));
+            }
+            fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalBegin.p, 0u, 2ull * B + 2);
+            fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalEnd.p, 0u, 2ull * B + 2);
+            segmentBounds<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalHalf.p, A2, ctx->lalBegin.p,
+                                                               ctx->lalEnd.p);
+            areaFix<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->area.p, A2, ctx->lalPrefix.p);
+            KT_TRY(scanInclusive64(ctx, ctx->lalPrefix.p, A2));
+            fillU64<<<gridFor(S + 1), kB, 0, ctx->stream>>>(sa.lo, ~0ull, S + 1);
+            fillU64<<<gridFor(S + 1), kB, 0, ctx->stream>>>(sa.hi, 0ull, S + 1);
+            stripeRanges<<<gridFor(H), kB, 0, ctx->stream>>>(ctx->lalBegin.p, ctx->lalEnd.p, ctx->lalPrefix.p, H, sa);
+            KT_HIP(hipGetLastError());
+            stripeOfCell<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalHalf.p, A2, sa, ca,
+                                                              ctx->lalStripeOf.p);
+            KT_HIP(hipGetLastError());
+            scaleCells<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, A2, ctx->lalStripeOf.p, sa, ca,
+                                                            const_cast<double *>(ca.pos));
+            KT_HIP(hipGetLastError());
+            listRemap<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, ctx->lalHalf.p, dRemap, ctx->lalSeg.p);
+            KT_HIP(hipGetLastError());
+        }
         blocks.swap(kept);
         vertical = !vertical;
+        A = A2;
+    }
+    // the batch's cells whose bin changed
+    KT_TRY(ctx->lalMoved.reserve(2ull * M));
+    KT_TRY(ctx->lalCount.reserve(1));
+    KT_HIP(hipMemsetAsync(ctx->lalCount.p, 0, sizeof(uint32_t), ctx->stream));
+    Moved *dMoved = reinterpret_cast<Moved *>(ctx->lalMoved.p);
+    movedCells<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, ctx->posX.p, ctx->posY.p, ctx->width.p, ctx->height.p,
+                                                   ctx->area.p, gridDev(g), ctx->binOf.p, dMoved, ctx->lalCount.p);
+    KT_HIP(hipGetLastError());
+    uint32_t moves = 0;
+    KT_HIP(hipMemcpyAsync(&moves, ctx->lalCount.p, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    KT_HIP(hipStreamSynchronize(ctx->stream));
+    moved.resize(moves);
+    if (moves) {
+        KT_HIP(hipMemcpyAsync(moved.data(), dMoved, moves * sizeof(Moved), hipMemcpyDeviceToHost, ctx->stream));
+        KT_HIP(hipStreamSynchronize(ctx->stream));
     }
     return KT_OK;
 }
@@ -754,8 +1119,17 @@ int lalLegalize(kt_ctx *ctx) {
     const lal::Grid &g = ctx->grid;
     std::vector<double> occ;
     KT_TRY(lalOccupancy(ctx, &occ));
-    lal::AreaTables tables;
-    tables.build(g, occ);
+    // occupancy is kept on the host from here on and updated with the cells
+    // that change bin, as the reference does (simpl_lal.cc:86-93)
+    std::vector<uint64_t> occFix(g.size());
+    for (size_t k = 0; k < g.size(); ++k) occFix[k] = static_cast<uint64_t>(std::llround(occ[k] * kFixScale));
+    lal::AreaTables &tables = ctx->lalTables;  // free-area table built with the grid
+    tables.setOcc(occFix);
+    KT_TRY(ctx->binRegion.reserve(g.size()));
+    fillU32<<<gridFor(g.size()), kB, 0, ctx->stream>>>(ctx->binRegion.p, kNoVar, g.size());
+    KT_HIP(hipGetLastError());
+    std::vector<Moved> moved;
+    std::vector<uint8_t> rowDirty(g.ny, 0);
     const auto reachOf = [&](const lal::BinRect &r) {
         return lal::BinRect{r.x0 > 0 ? r.x0 - 1 : 0, std::min(r.x1 + 1, g.nx - 1), r.y0 > 0 ? r.y0 - 1 : 0,
                             std::min(r.y1 + 1, g.ny - 1)};
@@ -767,12 +1141,23 @@ int lalLegalize(kt_ctx *ctx) {
     uint64_t batches = 0;
     const auto runBatch = [&]() -> int {
         if (batch.empty()) return KT_OK;
-        KT_TRY(lalRunBatch(ctx, batch, occ));
+        KT_TRY(lalRunBatch(ctx, batch, moved));
         ++batches;
         batch.clear();
         reach.clear();
-        KT_TRY(lalOccupancy(ctx, &occ));
-        tables.build(g, occ);
+        for (const Moved &mv : moved) {
+            occFix[mv.oldBin] -= mv.area;
+            occFix[mv.newBin] += mv.area;
+            occ[mv.oldBin] = static_cast<double>(occFix[mv.oldBin]) / kFixScale;
+            occ[mv.newBin] = static_cast<double>(occFix[mv.newBin]) / kFixScale;
+            rowDirty[mv.oldBin / g.nx] = 1;
+            rowDirty[mv.newBin / g.nx] = 1;
+        }
+        for (size_t iy = 0; iy < g.ny && !moved.empty(); ++iy)
+            if (rowDirty[iy]) {
+                tables.updateRow(occFix, iy);
+                rowDirty[iy] = 0;
+            }
         return KT_OK;
     };
     for (const lal::BinRect &cluster : lal::overfilledClusters(g, occ, ctx->densityLimit)) {

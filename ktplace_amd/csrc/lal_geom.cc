@@ -165,17 +165,26 @@ void buildGrid(Grid &g, FreeSpace &fs, const std::vector<RowSpec> &rows, const s
 }
 
 // ------------------------------------------------------------------ clusters
-void AreaTables::build(const Grid &g, const std::vector<double> &occBins) {
+void AreaTables::buildAvail(const Grid &g) {
     nx = g.nx;
     ny = g.ny;
-    occ.assign((nx + 1) * (ny + 1), 0.0);
     avail.assign((nx + 1) * (ny + 1), 0.0);
     for (size_t iy = 0; iy < ny; ++iy)
         for (size_t ix = 0; ix < nx; ++ix) {
             const size_t o = (iy + 1) * (nx + 1) + ix + 1;
-            occ[o] = occBins[g.at(ix, iy)] + occ[o - 1] + occ[o - (nx + 1)] - occ[o - (nx + 1) - 1];
             avail[o] = g.avail[g.at(ix, iy)] + avail[o - 1] + avail[o - (nx + 1)] - avail[o - (nx + 1) - 1];
         }
+    occRow.assign(ny * (nx + 1), 0);
+}
+
+void AreaTables::setOcc(const std::vector<uint64_t> &occFix) {
+    for (size_t iy = 0; iy < ny; ++iy) updateRow(occFix, iy);
+}
+
+void AreaTables::updateRow(const std::vector<uint64_t> &occFix, size_t iy) {
+    int64_t *row = occRow.data() + iy * (nx + 1);
+    row[0] = 0;
+    for (size_t ix = 0; ix < nx; ++ix) row[ix + 1] = row[ix] + static_cast<int64_t>(occFix[iy * nx + ix]);
 }
 
 namespace {
@@ -185,7 +194,14 @@ double rect(const std::vector<double> &s, size_t nx, const BinRect &r) {
 }
 }  // namespace
 
-double AreaTables::sumOcc(const BinRect &r) const { return rect(occ, nx, r); }
+double AreaTables::sumOcc(const BinRect &r) const {
+    int64_t t = 0;
+    for (size_t iy = r.y0; iy <= r.y1; ++iy) {
+        const int64_t *row = occRow.data() + iy * (nx + 1);
+        t += row[r.x1 + 1] - row[r.x0];
+    }
+    return static_cast<double>(t) / 65536.0;
+}
 double AreaTables::sumAvail(const BinRect &r) const { return rect(avail, nx, r); }
 
 // simpl_lal.cc:86-93

@@ -65,11 +65,17 @@ struct RowSpec {
 void buildGrid(Grid &g, FreeSpace &fs, const std::vector<RowSpec> &rows, const std::vector<Box> &obstacles,
                const Box &die, size_t perAxis);
 
-// Summed-area tables of occupancy and free area for O(1) rectangle sums.
+// Rectangle sums for cluster growth.  Free area: a summed-area table (the
+// grid's free area does not change).  Occupancy: per-row prefix sums of the
+// fixed-point occupancy (units of 2^-16), exact in any order and patched row
+// by row when cells move.
 struct AreaTables {
     size_t nx = 0, ny = 0;
-    std::vector<double> occ, avail;  // (nx+1) x (ny+1)
-    void build(const Grid &g, const std::vector<double> &occBins);
+    std::vector<double> avail;      // (nx+1) x (ny+1)
+    std::vector<int64_t> occRow;    // ny x (nx+1)
+    void buildAvail(const Grid &g);
+    void setOcc(const std::vector<uint64_t> &occFix);
+    void updateRow(const std::vector<uint64_t> &occFix, size_t iy);
     double sumOcc(const BinRect &r) const;
     double sumAvail(const BinRect &r) const;
     double density(const BinRect &r) const;

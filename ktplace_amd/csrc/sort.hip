@@ -99,6 +99,73 @@ int scanInclusive(kt_ctx *ctx, uint32_t *data, uint64_t m);
 
 // Sorts n pairs by key (stable), over the low `bits` bits of the key.  The
 // result ends in (keys, vals); (keysTmp, valsTmp) are scratch of size n.
+namespace {
+
+constexpr uint32_t kSmallSort = 2048;
+
+// One workgroup sorts up to kSmallSort ids by (seg[id], key, id) with a
+// bitonic network in LDS; key is positional (key[j] belongs to ids[j]).  The
+// order is total (ids are unique), so it equals the order of a stable sort by
+// key followed by a stable sort by segment, starting from ascending ids.
+__global__ void __launch_bounds__(1024) smallSortKernel(uint32_t *__restrict__ ids, const uint64_t *__restrict__ key,
+                                                        const uint32_t *__restrict__ segById, uint32_t count,
+                                                        uint32_t width) {
+    __shared__ uint64_t sk[kSmallSort];
+    __shared__ uint32_t ss[kSmallSort], si[kSmallSort];
+    for (uint32_t j = threadIdx.x; j < width; j += blockDim.x) {
+        if (j < count) {
+            const uint32_t id = ids[j];
+            si[j] = id;
+            sk[j] = key[j];
+            ss[j] = segById[id];
+        } else {  // padding sorts last
+            si[j] = 0xffffffffu;
+            sk[j] = ~0ull;
+            ss[j] = 0xffffffffu;
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= width; k <<= 1) {
+        for (uint32_t d = k >> 1; d > 0; d >>= 1) {
+            for (uint32_t t = threadIdx.x; t < width; t += blockDim.x) {
+                const uint32_t u = t ^ d;
+                if (u > t) {
+                    const bool greater = ss[t] != ss[u] ? ss[t] > ss[u]
+                                         : sk[t] != sk[u] ? sk[t] > sk[u]
+                                                          : si[t] > si[u];
+                    const bool ascending = (t & k) == 0;
+                    if (greater == ascending) {
+                        const uint32_t s0 = ss[t], i0 = si[t];
+                        const uint64_t k0 = sk[t];
+                        ss[t] = ss[u];
+                        sk[t] = sk[u];
+                        si[t] = si[u];
+                        ss[u] = s0;
+                        sk[u] = k0;
+                        si[u] = i0;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t j = threadIdx.x; j < count; j += blockDim.x) ids[j] = si[j];
+}
+
+}  // namespace
+
+bool smallSortFits(uint64_t n) { return n <= kSmallSort; }
+
+int smallSortBySegment(kt_ctx *ctx, uint32_t *ids, const uint64_t *key, const uint32_t *segById, uint64_t n) {
+    KT_CHECK(n <= kSmallSort, KT_EINVAL, "smallSortBySegment: too many elements");
+    if (n <= 1) return KT_OK;
+    uint32_t width = 2;
+    while (width < n) width <<= 1;
+    smallSortKernel<<<dim3(1), dim3(1024), 0, ctx->stream>>>(ids, key, segById, static_cast<uint32_t>(n), width);
+    KT_HIP(hipGetLastError());
+    return KT_OK;
+}
+
 int radixSortPairs(kt_ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t *keysTmp, uint32_t *valsTmp, uint64_t n,
                    int bits, DevBuf<uint32_t> &counts, DevBuf<uint32_t> &raw) {
     if (n <= 1 || bits <= 0) return KT_OK;

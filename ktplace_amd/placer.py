@@ -54,7 +54,9 @@ class PlaceResult:
 class Placer:
     """simplPlacer over one GPU context (the design is uploaded once)."""
 
-    def __init__(self, design: Design, opts: SimplOptions | None = None, ctx: simpl.Context | None = None):
+    def __init__(self, design: Design, opts: SimplOptions | None = None, ctx: simpl.Context | None = None,
+                 progress=None):
+        self.progress = progress  # called with one line per outer iteration
         self.d = design
         self.o = opts or SimplOptions()
         self.ctx = ctx or simpl.Context(0)
@@ -138,6 +140,10 @@ class Placer:
             lower = self.hpwl(lx, ly)
             gap = upper - lower
             trace.append((lower, upper, self.overflow(ux, uy)))
+            if self.progress:
+                self.progress(f"iter {it + 1}: lower {lower:.6e} upper {upper:.6e} "
+                              f"lal {self.stats['spread_seconds']:.2f} s total, "
+                              f"{ctx.lal_batches()} batches")
             iteration = it + 1
             if iteration == o.gapReferenceIter:
                 gap_ref = gap
@@ -162,7 +168,12 @@ class Placer:
                 alpha = o.alphaBase * (1.0 + done + r)
                 ctx.build_b2b(lx, ly, alpha, ux, uy)
                 self._solve(lx, ly)
+                t0 = time.perf_counter()
                 ux, uy = ctx.lal(lx, ly)
+                self.stats["spread_seconds"] += time.perf_counter() - t0
+                if self.progress:
+                    self.progress(f"refine {r + 1}: lal {time.perf_counter() - t0:.2f} s, "
+                                  f"{ctx.lal_batches()} batches")
         if ux is None:
             ux, uy = lx.copy(), ly.copy()
         self.stats["hpwl_final"] = self.hpwl(ux, uy)

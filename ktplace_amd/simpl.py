@@ -103,6 +103,7 @@ _SIGS = {
     "kt_set_lal_options": (I32, [P, U64, U64, F64]),
     "kt_lal": (I32, [P, P, P]),
     "kt_lal_resident": (I32, [P]),
+    "kt_lal_batches": (I32, [P, C.POINTER(C.c_uint64)]),
     "kt_set_warm_start": (I32, [P, P, U64]),
     "kt_warm_features": (I32, [P, P, P]),
     "kt_warm_predict": (I32, [P, P, P]),
@@ -266,6 +267,11 @@ class Context:
 
     def lal_resident(self):
         _check(lib.kt_lal_resident(self.h))
+
+    def lal_batches(self) -> int:
+        b = C.c_uint64()
+        _check(lib.kt_lal_batches(self.h, C.byref(b)))
+        return b.value
 
     # positions ---------------------------------------------------------------
     def set_positions(self, x, y):
