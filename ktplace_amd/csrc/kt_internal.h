@@ -228,6 +228,9 @@ struct kt_ctx {
     kt::DevBuf<uint32_t> binOf, binRegion, lalSeg, lalFlag, lalIds, lalVals, lalValsTmp, lalHalf, lalStripeOf,
         lalBegin, lalEnd, lalStripeStart, lalStripeEnd, lalRemap, sortCounts, sortRaw;
     kt::DevBuf<uint32_t> lalList, lalRects, lalCount;  // batch cells, region rectangles, moved count
+    kt::DevBuf<uint32_t> lalRegionOf, lalSaveBin, lalBinStart;  // in-order rule: per-cell restore state
+    kt::DevBuf<double> lalSaveX, lalSaveY;
+    uint64_t lalReruns = 0;                            // region runs repeated for the in-order rule
     kt::DevBuf<unsigned long long> lalMoved;           // moved-cell records (2 words each)
     kt::lal::AreaTables lalTables;                     // free-area table of the grid, occupancy rows
     uint8_t *lalPinned = nullptr;                      // pinned staging of per-level tables

@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -264,16 +265,99 @@ struct Moved {
 __global__ void __launch_bounds__(kB)
 movedCells(const uint32_t *__restrict__ list, uint32_t m, const double *__restrict__ x, const double *__restrict__ y,
            const double *__restrict__ w, const double *__restrict__ h, const double *__restrict__ area, GridDev g,
-           uint32_t *__restrict__ binOf, Moved *__restrict__ out, uint32_t *__restrict__ count) {
+           const uint32_t *__restrict__ binStart, uint32_t *__restrict__ binOf, Moved *__restrict__ out,
+           uint32_t *__restrict__ count) {
     const uint32_t j = blockIdx.x * kB + threadIdx.x;
     if (j >= m) return;
     const uint32_t i = list[j];
     const uint32_t k = binOfPoint(g, x[i] + 0.5 * w[i], y[i] + 0.5 * h[i]);
-    const uint32_t old = binOf[i];
-    if (k == old) return;
+    const uint32_t old = binStart[i];
     binOf[i] = k;
+    if (k == old) return;
     const uint32_t slot = atomicAdd(count, 1u);
     out[slot] = Moved{old, k, static_cast<unsigned long long>(llrint(area[i] * kFixScale))};
+}
+
+// Per-cell state of a batch for the in-order rule (see lalRunBatch).
+struct BatchState {
+    uint32_t *regionOf;           // region whose run last took the cell
+    double *saveX, *saveY;        // position before that run
+    uint32_t *saveBin;            // bin before that run
+    uint32_t *binStart;           // bin at the start of the batch
+    double *x, *y;
+    const double *w, *h;
+    uint32_t *binOf;
+    const uint32_t *binRegion;    // bin -> region of the batch (kNoVar elsewhere)
+    uint32_t *seg;
+    GridDev g;
+};
+
+__global__ void __launch_bounds__(kB) saveBatch(const uint32_t *__restrict__ list, uint32_t m, BatchState s) {
+    const uint32_t j = blockIdx.x * kB + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t i = list[j];
+    s.regionOf[i] = s.seg[i];
+    s.saveX[i] = s.x[i];
+    s.saveY[i] = s.y[i];
+    s.saveBin[i] = s.binOf[i];
+    s.binStart[i] = s.binOf[i];
+}
+
+__device__ __forceinline__ uint32_t binNow(const BatchState &s, uint32_t i) {
+    return binOfPoint(s.g, s.x[i] + 0.5 * s.w[i], s.y[i] + 0.5 * s.h[i]);
+}
+
+// out[0] = the first region a cell of a region >= first moved into, among the
+// regions after its own (stays kNoVar when there is none)
+__global__ void __launch_bounds__(kB)
+escapeCheck(const uint32_t *__restrict__ list, uint32_t m, uint32_t first, BatchState s, uint32_t *__restrict__ out) {
+    const uint32_t j = blockIdx.x * kB + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t i = list[j];
+    const uint32_t r = s.regionOf[i];
+    if (r == kNoVar || r < first) return;
+    const uint32_t rj = s.binRegion[binNow(s, i)];
+    if (rj != kNoVar && rj > r) atomicMin(out, rj);
+}
+
+// regions >= bad back to their restore points; earlier ones keep their result
+__global__ void __launch_bounds__(kB)
+repairBatch(const uint32_t *__restrict__ list, uint32_t m, uint32_t bad, BatchState s) {
+    const uint32_t j = blockIdx.x * kB + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t i = list[j];
+    const uint32_t r = s.regionOf[i];
+    if (r != kNoVar && r >= bad) {
+        s.x[i] = s.saveX[i];
+        s.y[i] = s.saveY[i];
+        s.binOf[i] = s.saveBin[i];
+    } else {
+        s.binOf[i] = binNow(s, i);
+    }
+}
+
+// cells now in the bins of regions >= bad take part in the run again; cells
+// that arrive from an earlier region get that region as their restore point
+__global__ void __launch_bounds__(kB)
+resegBatch(const uint32_t *__restrict__ list, uint32_t m, uint32_t bad, BatchState s, uint32_t *__restrict__ count) {
+    const uint32_t j = blockIdx.x * kB + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t i = list[j];
+    const uint32_t b = s.binOf[i];
+    const uint32_t rj = s.binRegion[b];
+    if (rj != kNoVar && rj >= bad) {
+        s.seg[i] = rj;
+        atomicAdd(count, 1u);
+        const uint32_t r = s.regionOf[i];
+        if (r == kNoVar || r < bad) {
+            s.regionOf[i] = rj;
+            s.saveX[i] = s.x[i];
+            s.saveY[i] = s.y[i];
+            s.saveBin[i] = b;
+        }
+    } else {
+        s.seg[i] = kNoVar;
+    }
 }
 
 struct CentreArgs {
@@ -862,236 +946,283 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, std::vect
     KT_TRY(scanInclusive(ctx, ctx->lalFlag.p, n));
     compactIds<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalSeg.p, ctx->lalFlag.p, n, ctx->lalList.p);
     KT_HIP(hipGetLastError());
-    paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 1);
-    KT_HIP(hipGetLastError());
+    const auto unpaint = [&]() -> int {
+        paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 1);
+        KT_HIP(hipGetLastError());
+        return KT_OK;
+    };
     uint32_t M = 0;
     KT_HIP(hipMemcpyAsync(&M, ctx->lalFlag.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     KT_HIP(hipStreamSynchronize(ctx->stream));
-    if (M == 0) return KT_OK;
+    if (M == 0) return unpaint();
+    // restore points of the batch's cells and their bins at the start
+    for (auto *b : {&ctx->lalRegionOf, &ctx->lalSaveBin, &ctx->lalBinStart}) KT_TRY(b->reserve(n));
+    for (auto *b : {&ctx->lalSaveX, &ctx->lalSaveY}) KT_TRY(b->reserve(n));
+    BatchState bs{ctx->lalRegionOf.p, ctx->lalSaveX.p, ctx->lalSaveY.p, ctx->lalSaveBin.p, ctx->lalBinStart.p,
+                  ctx->posX.p, ctx->posY.p, ctx->width.p, ctx->height.p, ctx->binOf.p, ctx->binRegion.p,
+                  ctx->lalSeg.p, gridDev(g)};
     const uint32_t *list = ctx->lalList.p;
+    saveBatch<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, bs);
+    KT_HIP(hipGetLastError());
     // batches of up to kSmallCells cells: one workgroup per half level
-    const bool small = M <= kSmallCells;
+    static const bool smallPath = [] {
+        const char *v = std::getenv("KTPLACE_LAL_SMALL");
+        return !(v && v[0] == '0');
+    }();
+    const bool small = smallPath && M <= kSmallCells;
     uint32_t width = 2;
     while (width < M) width <<= 1;
 
-    bool vertical = true;
-    // cells of the level: all of the batch at level 1, then those of the
-    // blocks that split (known on the host from the block statistics)
-    uint32_t A = M;
-    for (size_t level = 1;; ++level) {
-        if (A == 0) break;
-        const uint32_t B = static_cast<uint32_t>(blocks.size());
-        const CentreArgs ca{vertical ? ctx->posX.p : ctx->posY.p, vertical ? ctx->width.p : ctx->height.p};
-        KT_TRY(ctx->lalBegin.reserve(2ull * B + 2));
-        KT_TRY(ctx->lalEnd.reserve(2ull * B + 2));
-        KT_TRY(ctx->lalStats.reserve(B));
-        SmallArgs sm{};
-        if (small) {
-            sm.list = list;
-            sm.M = M;
-            sm.width = width;
-            sm.seg = ctx->lalSeg.p;
-            sm.half = ctx->lalHalf.p;
-            sm.key2 = ctx->lalKey2.p;
-            sm.c = ca;
-            sm.area = ctx->area.p;
-            sm.ids = ctx->lalIds.p;
-            sm.begin = ctx->lalBegin.p;
-            sm.end = ctx->lalEnd.p;
-            sm.P = ctx->lalPrefix.p;
-            sm.stats = ctx->lalStats.p;
-            sm.stripeOf = ctx->lalStripeOf.p;
-            sm.B = B;
-            sm.A = A;
-            smallLevelA<<<1, kSmallThreads, 0, ctx->stream>>>(sm);
-            KT_HIP(hipGetLastError());
-        } else {
-            // active cells in id order
-            listFlags<<<gridFor(M), kB, 0, ctx->stream>>>(list, ctx->lalSeg.p, M, ctx->lalFlag.p);
-            KT_HIP(hipGetLastError());
-            KT_TRY(scanInclusive(ctx, ctx->lalFlag.p, M));
-            listCompact<<<gridFor(M), kB, 0, ctx->stream>>>(list, ctx->lalSeg.p, ctx->lalFlag.p, M, ctx->lalIds.p);
-            KT_HIP(hipGetLastError());
-            // sort by (block, centre, id): stable sorts by centre, then by block
-            centreKeys<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, A, ca, ctx->lalKeys.p);
-            KT_HIP(hipGetLastError());
-            if (smallSortFits(A)) {
-                KT_TRY(smallSortBySegment(ctx, ctx->lalIds.p, ctx->lalKeys.p, ctx->lalSeg.p, A));
-            } else {
-                KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A,
-                                      64, ctx->sortCounts, ctx->sortRaw));
-                segmentKeys<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalSeg.p, A, ctx->lalKeys.p);
+    // The recursion over the levels for the cells whose lalSeg names a region.
+    const std::vector<Block> regionBlocks = blocks;
+    const auto runLevels = [&](uint32_t A) -> int {
+        std::vector<Block> blocks = regionBlocks;
+        bool vertical = true;
+        // cells of the level: those with a region at level 1, then those of
+        // the blocks that split (known on the host from the block statistics)
+        for (size_t level = 1;; ++level) {
+            if (A == 0) break;
+            const uint32_t B = static_cast<uint32_t>(blocks.size());
+            const CentreArgs ca{vertical ? ctx->posX.p : ctx->posY.p, vertical ? ctx->width.p : ctx->height.p};
+            KT_TRY(ctx->lalBegin.reserve(2ull * B + 2));
+            KT_TRY(ctx->lalEnd.reserve(2ull * B + 2));
+            KT_TRY(ctx->lalStats.reserve(B));
+            SmallArgs sm{};
+            if (small) {
+                sm.list = list;
+                sm.M = M;
+                sm.width = width;
+                sm.seg = ctx->lalSeg.p;
+                sm.half = ctx->lalHalf.p;
+                sm.key2 = ctx->lalKey2.p;
+                sm.c = ca;
+                sm.area = ctx->area.p;
+                sm.ids = ctx->lalIds.p;
+                sm.begin = ctx->lalBegin.p;
+                sm.end = ctx->lalEnd.p;
+                sm.P = ctx->lalPrefix.p;
+                sm.stats = ctx->lalStats.p;
+                sm.stripeOf = ctx->lalStripeOf.p;
+                sm.B = B;
+                sm.A = A;
+                smallLevelA<<<1, kSmallThreads, 0, ctx->stream>>>(sm);
                 KT_HIP(hipGetLastError());
-                KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A,
-                                      bitsFor(B), ctx->sortCounts, ctx->sortRaw));
+            } else {
+                // active cells in id order
+                listFlags<<<gridFor(M), kB, 0, ctx->stream>>>(list, ctx->lalSeg.p, M, ctx->lalFlag.p);
+                KT_HIP(hipGetLastError());
+                KT_TRY(scanInclusive(ctx, ctx->lalFlag.p, M));
+                listCompact<<<gridFor(M), kB, 0, ctx->stream>>>(list, ctx->lalSeg.p, ctx->lalFlag.p, M, ctx->lalIds.p);
+                KT_HIP(hipGetLastError());
+                // sort by (block, centre, id): stable sorts by centre, then by block
+                centreKeys<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, A, ca, ctx->lalKeys.p);
+                KT_HIP(hipGetLastError());
+                if (smallSortFits(A)) {
+                    KT_TRY(smallSortBySegment(ctx, ctx->lalIds.p, ctx->lalKeys.p, ctx->lalSeg.p, A));
+                } else {
+                    KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A,
+                                          64, ctx->sortCounts, ctx->sortRaw));
+                    segmentKeys<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalSeg.p, A, ctx->lalKeys.p);
+                    KT_HIP(hipGetLastError());
+                    KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A,
+                                          bitsFor(B), ctx->sortCounts, ctx->sortRaw));
+                }
+                // block ranges, area prefix, split
+                fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalBegin.p, 0u, 2ull * B + 2);
+                fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalEnd.p, 0u, 2ull * B + 2);
+                segmentBounds<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalSeg.p, A, ctx->lalBegin.p,
+                                                                  ctx->lalEnd.p);
+                areaFix<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->area.p, A, ctx->lalPrefix.p);
+                KT_TRY(scanInclusive64(ctx, ctx->lalPrefix.p, A));
+                splitBlocks<<<gridFor(B), kB, 0, ctx->stream>>>(ctx->lalBegin.p, ctx->lalEnd.p, ctx->lalPrefix.p, B,
+                                                                ctx->lalStats.p);
+                KT_HIP(hipGetLastError());
             }
-            // block ranges, area prefix, split
-            fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalBegin.p, 0u, 2ull * B + 2);
-            fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalEnd.p, 0u, 2ull * B + 2);
-            segmentBounds<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalSeg.p, A, ctx->lalBegin.p,
-                                                              ctx->lalEnd.p);
-            areaFix<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->area.p, A, ctx->lalPrefix.p);
-            KT_TRY(scanInclusive64(ctx, ctx->lalPrefix.p, A));
-            splitBlocks<<<gridFor(B), kB, 0, ctx->stream>>>(ctx->lalBegin.p, ctx->lalEnd.p, ctx->lalPrefix.p, B,
-                                                            ctx->lalStats.p);
-            KT_HIP(hipGetLastError());
-        }
-        std::vector<SegStat> stats(B);
-        KT_HIP(hipMemcpyAsync(stats.data(), ctx->lalStats.p, B * sizeof(SegStat), hipMemcpyDeviceToHost, ctx->stream));
-        KT_HIP(hipStreamSynchronize(ctx->stream));
+            std::vector<SegStat> stats(B);
+            KT_HIP(hipMemcpyAsync(stats.data(), ctx->lalStats.p, B * sizeof(SegStat), hipMemcpyDeviceToHost, ctx->stream));
+            KT_HIP(hipStreamSynchronize(ctx->stream));
 
-        // host geometry for every splitting block (simpl_lal.cc:213-263)
-        std::vector<uint8_t> split(B, 0);
-        std::vector<double> cut(B, 0.0);
-        std::vector<std::vector<lal::Stripe>> halfStripes(2ull * B);
-        std::vector<Block> next(2ull * B);
-        const lal::FreeSpace &fs = ctx->freeSpace;
-        const double g2 = ctx->densityLimit;
-        const double frac = ctx->lalOpts.stripeAreaFraction;
-        parallelFor(B, [&](size_t s) {
-            const SegStat &st = stats[s];
-            if (level >= ctx->lalOpts.maxLevel || st.count <= ctx->lalOpts.minCellsToSplit) return;
-            split[s] = 1;
-            const Block &bl = blocks[s];
-            const double cellArea = static_cast<double>(st.cellArea) / kFixScale;
-            const double lowArea = static_cast<double>(st.lowArea) / kFixScale;
-            const double lo = vertical ? bl.box.x0 : bl.box.y0;
-            const double hi = vertical ? bl.box.x1 : bl.box.y1;
-            const lal::Profile room = fs.profile(bl.box, vertical);
-            const double share = cellArea > 0.0 ? lowArea / cellArea : 0.5;
-            const double cutline = room.total() > 0.0 ? room.coordinateAt(share * room.total()) : lo + share * (hi - lo);
-            cut[s] = cutline;
-            lal::Box lowBox = bl.box, highBox = bl.box;
-            (vertical ? lowBox.x1 : lowBox.y1) = cutline;
-            (vertical ? highBox.x0 : highBox.y0) = cutline;
-            const double highArea = static_cast<double>(st.cellArea - st.lowArea) / kFixScale;
-            if (st.m > 0)
-                halfStripes[2 * s] = lal::stripes(fs, ctx->obstacles, lowBox, vertical, lowArea, bl.regionRoom, g2,
-                                                  frac, cutline, true);
-            if (st.count > st.m)
-                halfStripes[2 * s + 1] = lal::stripes(fs, ctx->obstacles, highBox, vertical, highArea, bl.regionRoom,
-                                                      g2, frac, cutline, false);
-            // a zero-length half box yields no stripe; give its cells one
-            // stripe at the box edge so they stay inside it
-            if (st.m > 0 && halfStripes[2 * s].empty())
-                halfStripes[2 * s].push_back({vertical ? lowBox.x0 : lowBox.y0, vertical ? lowBox.x1 : lowBox.y1, 0.0});
-            if (st.count > st.m && halfStripes[2 * s + 1].empty())
-                halfStripes[2 * s + 1].push_back(
-                    {vertical ? highBox.x0 : highBox.y0, vertical ? highBox.x1 : highBox.y1, 0.0});
-            next[2 * s] = {lowBox, bl.regionRoom};
-            next[2 * s + 1] = {highBox, bl.regionRoom};
-        });
-        uint32_t splits = 0, A2 = 0;
-        for (uint32_t s = 0; s < B; ++s)
-            if (split[s]) {
-                ++splits;
-                A2 += stats[s].count;
+            // host geometry for every splitting block (simpl_lal.cc:213-263)
+            std::vector<uint8_t> split(B, 0);
+            std::vector<double> cut(B, 0.0);
+            std::vector<std::vector<lal::Stripe>> halfStripes(2ull * B);
+            std::vector<Block> next(2ull * B);
+            const lal::FreeSpace &fs = ctx->freeSpace;
+            const double g2 = ctx->densityLimit;
+            const double frac = ctx->lalOpts.stripeAreaFraction;
+            parallelFor(B, [&](size_t s) {
+                const SegStat &st = stats[s];
+                if (level >= ctx->lalOpts.maxLevel || st.count <= ctx->lalOpts.minCellsToSplit) return;
+                split[s] = 1;
+                const Block &bl = blocks[s];
+                const double cellArea = static_cast<double>(st.cellArea) / kFixScale;
+                const double lowArea = static_cast<double>(st.lowArea) / kFixScale;
+                const double lo = vertical ? bl.box.x0 : bl.box.y0;
+                const double hi = vertical ? bl.box.x1 : bl.box.y1;
+                const lal::Profile room = fs.profile(bl.box, vertical);
+                const double share = cellArea > 0.0 ? lowArea / cellArea : 0.5;
+                const double cutline = room.total() > 0.0 ? room.coordinateAt(share * room.total()) : lo + share * (hi - lo);
+                cut[s] = cutline;
+                lal::Box lowBox = bl.box, highBox = bl.box;
+                (vertical ? lowBox.x1 : lowBox.y1) = cutline;
+                (vertical ? highBox.x0 : highBox.y0) = cutline;
+                const double highArea = static_cast<double>(st.cellArea - st.lowArea) / kFixScale;
+                if (st.m > 0)
+                    halfStripes[2 * s] = lal::stripes(fs, ctx->obstacles, lowBox, vertical, lowArea, bl.regionRoom, g2,
+                                                      frac, cutline, true);
+                if (st.count > st.m)
+                    halfStripes[2 * s + 1] = lal::stripes(fs, ctx->obstacles, highBox, vertical, highArea, bl.regionRoom,
+                                                          g2, frac, cutline, false);
+                // a zero-length half box yields no stripe; give its cells one
+                // stripe at the box edge so they stay inside it
+                if (st.m > 0 && halfStripes[2 * s].empty())
+                    halfStripes[2 * s].push_back({vertical ? lowBox.x0 : lowBox.y0, vertical ? lowBox.x1 : lowBox.y1, 0.0});
+                if (st.count > st.m && halfStripes[2 * s + 1].empty())
+                    halfStripes[2 * s + 1].push_back(
+                        {vertical ? highBox.x0 : highBox.y0, vertical ? highBox.x1 : highBox.y1, 0.0});
+                next[2 * s] = {lowBox, bl.regionRoom};
+                next[2 * s + 1] = {highBox, bl.regionRoom};
+            });
+            uint32_t splits = 0, A2 = 0;
+            for (uint32_t s = 0; s < B; ++s)
+                if (split[s]) {
+                    ++splits;
+                    A2 += stats[s].count;
+                }
+            if (splits == 0) break;
+            // stripe tables
+            const uint32_t H = 2 * B;
+            std::vector<uint32_t> stripeStart(H + 1, 0);
+            for (uint32_t h = 0; h < H; ++h) stripeStart[h + 1] = stripeStart[h] + halfStripes[h].size();
+            const uint32_t S = stripeStart[H];
+            std::vector<double> span(2ull * S + 2), cap(S + 1);
+            for (uint32_t h = 0; h < H; ++h)
+                for (size_t q = 0; q < halfStripes[h].size(); ++q) {
+                    const uint32_t s = stripeStart[h] + static_cast<uint32_t>(q);
+                    span[2 * s] = halfStripes[h][q].start;
+                    span[2 * s + 1] = halfStripes[h][q].end;
+                    cap[s] = halfStripes[h][q].capacity;
+                }
+            // next level: the halves that hold cells, renumbered densely
+            std::vector<uint32_t> remap(H, kNoVar);
+            std::vector<Block> kept;
+            for (uint32_t s = 0; s < B; ++s) {
+                if (!split[s]) continue;
+                if (stats[s].m > 0) {
+                    remap[2 * s] = static_cast<uint32_t>(kept.size());
+                    kept.push_back(next[2 * s]);
+                }
+                if (stats[s].count > stats[s].m) {
+                    remap[2 * s + 1] = static_cast<uint32_t>(kept.size());
+                    kept.push_back(next[2 * s + 1]);
+                }
             }
-        if (splits == 0) break;
-        // stripe tables
-        const uint32_t H = 2 * B;
-        std::vector<uint32_t> stripeStart(H + 1, 0);
-        for (uint32_t h = 0; h < H; ++h) stripeStart[h + 1] = stripeStart[h] + halfStripes[h].size();
-        const uint32_t S = stripeStart[H];
-        std::vector<double> span(2ull * S + 2), cap(S + 1);
-        for (uint32_t h = 0; h < H; ++h)
-            for (size_t q = 0; q < halfStripes[h].size(); ++q) {
-                const uint32_t s = stripeStart[h] + static_cast<uint32_t>(q);
-                span[2 * s] = halfStripes[h][q].start;
-                span[2 * s + 1] = halfStripes[h][q].end;
-                cap[s] = halfStripes[h][q].capacity;
-            }
-        // next level: the halves that hold cells, renumbered densely
-        std::vector<uint32_t> remap(H, kNoVar);
-        std::vector<Block> kept;
-        for (uint32_t s = 0; s < B; ++s) {
-            if (!split[s]) continue;
-            if (stats[s].m > 0) {
-                remap[2 * s] = static_cast<uint32_t>(kept.size());
-                kept.push_back(next[2 * s]);
-            }
-            if (stats[s].count > stats[s].m) {
-                remap[2 * s + 1] = static_cast<uint32_t>(kept.size());
-                kept.push_back(next[2 * s + 1]);
-            }
-        }
-        // one upload of the level's host tables through pinned memory
-        const auto up8 = [](size_t bytes) { return (bytes + 7) & ~size_t{7}; };
-        const size_t oCut = 0, oSpan = oCut + up8(B * sizeof(double)), oCap = oSpan + up8(span.size() * sizeof(double)),
-                     oStart = oCap + up8(cap.size() * sizeof(double)),
-                     oRemap = oStart + up8((H + 1) * sizeof(uint32_t)), oSplit = oRemap + up8(H * sizeof(uint32_t)),
-                     total = oSplit + up8(B);
-        KT_TRY(lalStaging(ctx, total));
-        uint8_t *host = ctx->lalPinned;
-        std::memcpy(host + oCut, cut.data(), B * sizeof(double));
-        std::memcpy(host + oSpan, span.data(), span.size() * sizeof(double));
-        std::memcpy(host + oCap, cap.data(), cap.size() * sizeof(double));
-        std::memcpy(host + oStart, stripeStart.data(), (H + 1) * sizeof(uint32_t));
-        std::memcpy(host + oRemap, remap.data(), H * sizeof(uint32_t));
-        std::memcpy(host + oSplit, split.data(), B);
-        uint8_t *dev = reinterpret_cast<uint8_t *>(ctx->lalUp.p);
-        KT_HIP(hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, ctx->stream));
-        const double *dCut = reinterpret_cast<const double *>(dev + oCut);
-        const double *dSpan = reinterpret_cast<const double *>(dev + oSpan);
-        const double *dCap = reinterpret_cast<const double *>(dev + oCap);
-        const uint32_t *dStart = reinterpret_cast<const uint32_t *>(dev + oStart);
-        const uint32_t *dRemap = reinterpret_cast<const uint32_t *>(dev + oRemap);
-        const uint8_t *dSplit = dev + oSplit;
-        KT_TRY(ctx->lalStripeEnd.reserve(S + 1));
-        KT_TRY(ctx->lalStripeLo.reserve(S + 1));
-        KT_TRY(ctx->lalStripeHi.reserve(S + 1));
-        // halves and the second order: by (half, -fromCut(centre), id)
-        HalfArgs ha{ctx->lalBegin.p, ctx->lalStats.p, dSplit, dCut, ca};
-        StripeArgs sa{dStart, dSpan, dCap, ctx->lalStripeEnd.p,
-                      reinterpret_cast<unsigned long long *>(ctx->lalStripeLo.p),
-                      reinterpret_cast<unsigned long long *>(ctx->lalStripeHi.p)};
-        if (small) {
-            smallLevelB<<<1, kSmallThreads, 0, ctx->stream>>>(sm, ha, sa, dRemap, H, S, A2, const_cast<double *>(ca.pos));
-            KT_HIP(hipGetLastError());
-        } else {
-            listFill<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, ctx->lalHalf.p, kNoVar);
-            halfKeys<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalSeg.p, A, ha, ctx->lalHalf.p,
-                                                         ctx->lalKey2.p);
-            KT_HIP(hipGetLastError());
-            // cells of non-splitting blocks leave; the rest are ordered by half
-            listFlags<<<gridFor(M), kB, 0, ctx->stream>>>(list, ctx->lalHalf.p, M, ctx->lalFlag.p);
-            KT_HIP(hipGetLastError());
-            KT_TRY(scanInclusive(ctx, ctx->lalFlag.p, M));
-            listCompact<<<gridFor(M), kB, 0, ctx->stream>>>(list, ctx->lalHalf.p, ctx->lalFlag.p, M, ctx->lalIds.p);
-            gatherKeys<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalKey2.p, A2, ctx->lalKeys.p);
-            KT_HIP(hipGetLastError());
-            if (smallSortFits(A2)) {
-                KT_TRY(smallSortBySegment(ctx, ctx->lalIds.p, ctx->lalKeys.p, ctx->lalHalf.p, A2));
-            } else {
-                KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A2, 64,
-                                      ctx->sortCounts, ctx->sortRaw));
-                segmentKeys<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalHalf.p, A2, ctx->lalKeys.p);
+            // one upload of the level's host tables through pinned memory
+            const auto up8 = [](size_t bytes) { return (bytes + 7) & ~size_t{7}; };
+            const size_t oCut = 0, oSpan = oCut + up8(B * sizeof(double)), oCap = oSpan + up8(span.size() * sizeof(double)),
+                         oStart = oCap + up8(cap.size() * sizeof(double)),
+                         oRemap = oStart + up8((H + 1) * sizeof(uint32_t)), oSplit = oRemap + up8(H * sizeof(uint32_t)),
+                         total = oSplit + up8(B);
+            KT_TRY(lalStaging(ctx, total));
+            uint8_t *host = ctx->lalPinned;
+            std::memcpy(host + oCut, cut.data(), B * sizeof(double));
+            std::memcpy(host + oSpan, span.data(), span.size() * sizeof(double));
+            std::memcpy(host + oCap, cap.data(), cap.size() * sizeof(double));
+            std::memcpy(host + oStart, stripeStart.data(), (H + 1) * sizeof(uint32_t));
+            std::memcpy(host + oRemap, remap.data(), H * sizeof(uint32_t));
+            std::memcpy(host + oSplit, split.data(), B);
+            uint8_t *dev = reinterpret_cast<uint8_t *>(ctx->lalUp.p);
+            KT_HIP(hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, ctx->stream));
+            const double *dCut = reinterpret_cast<const double *>(dev + oCut);
+            const double *dSpan = reinterpret_cast<const double *>(dev + oSpan);
+            const double *dCap = reinterpret_cast<const double *>(dev + oCap);
+            const uint32_t *dStart = reinterpret_cast<const uint32_t *>(dev + oStart);
+            const uint32_t *dRemap = reinterpret_cast<const uint32_t *>(dev + oRemap);
+            const uint8_t *dSplit = dev + oSplit;
+            KT_TRY(ctx->lalStripeEnd.reserve(S + 1));
+            KT_TRY(ctx->lalStripeLo.reserve(S + 1));
+            KT_TRY(ctx->lalStripeHi.reserve(S + 1));
+            // halves and the second order: by (half, -fromCut(centre), id)
+            HalfArgs ha{ctx->lalBegin.p, ctx->lalStats.p, dSplit, dCut, ca};
+            StripeArgs sa{dStart, dSpan, dCap, ctx->lalStripeEnd.p,
+                          reinterpret_cast<unsigned long long *>(ctx->lalStripeLo.p),
+                          reinterpret_cast<unsigned long long *>(ctx->lalStripeHi.p)};
+            if (small) {
+                smallLevelB<<<1, kSmallThreads, 0, ctx->stream>>>(sm, ha, sa, dRemap, H, S, A2, const_cast<double *>(ca.pos));
                 KT_HIP(hipGetLastError());
-                KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A2,
-                                      bitsFor(H), ctx->sortCounts, ctx->sortRaw));
+            } else {
+                listFill<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, ctx->lalHalf.p, kNoVar);
+                halfKeys<<<gridFor(A), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalSeg.p, A, ha, ctx->lalHalf.p,
+                                                             ctx->lalKey2.p);
+                KT_HIP(hipGetLastError());
+                // cells of non-splitting blocks leave; the rest are ordered by half
+                listFlags<<<gridFor(M), kB, 0, ctx->stream>>>(list, ctx->lalHalf.p, M, ctx->lalFlag.p);
+                KT_HIP(hipGetLastError());
+                KT_TRY(scanInclusive(ctx, ctx->lalFlag.p, M));
+                listCompact<<<gridFor(M), kB, 0, ctx->stream>>>(list, ctx->lalHalf.p, ctx->lalFlag.p, M, ctx->lalIds.p);
+                gatherKeys<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalKey2.p, A2, ctx->lalKeys.p);
+                KT_HIP(hipGetLastError());
+                if (smallSortFits(A2)) {
+                    KT_TRY(smallSortBySegment(ctx, ctx->lalIds.p, ctx->lalKeys.p, ctx->lalHalf.p, A2));
+                } else {
+                    KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A2, 64,
+                                          ctx->sortCounts, ctx->sortRaw));
+                    segmentKeys<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalHalf.p, A2, ctx->lalKeys.p);
+                    KT_HIP(hipGetLastError());
+                    KT_TRY(radixSortPairs(ctx, ctx->lalKeys.p, ctx->lalIds.p, ctx->lalKeysTmp.p, ctx->lalValsTmp.p, A2,
+                                          bitsFor(H), ctx->sortCounts, ctx->sortRaw));
+                }
+                fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalBegin.p, 0u, 2ull * B + 2);
+                fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalEnd.p, 0u, 2ull * B + 2);
+                segmentBounds<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalHalf.p, A2, ctx->lalBegin.p,
+                                                                   ctx->lalEnd.p);
+                areaFix<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->area.p, A2, ctx->lalPrefix.p);
+                KT_TRY(scanInclusive64(ctx, ctx->lalPrefix.p, A2));
+                fillU64<<<gridFor(S + 1), kB, 0, ctx->stream>>>(sa.lo, ~0ull, S + 1);
+                fillU64<<<gridFor(S + 1), kB, 0, ctx->stream>>>(sa.hi, 0ull, S + 1);
+                stripeRanges<<<gridFor(H), kB, 0, ctx->stream>>>(ctx->lalBegin.p, ctx->lalEnd.p, ctx->lalPrefix.p, H, sa);
+                KT_HIP(hipGetLastError());
+                stripeOfCell<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalHalf.p, A2, sa, ca,
+                                                                  ctx->lalStripeOf.p);
+                KT_HIP(hipGetLastError());
+                scaleCells<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, A2, ctx->lalStripeOf.p, sa, ca,
+                                                                const_cast<double *>(ca.pos));
+                KT_HIP(hipGetLastError());
+                listRemap<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, ctx->lalHalf.p, dRemap, ctx->lalSeg.p);
+                KT_HIP(hipGetLastError());
             }
-            fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalBegin.p, 0u, 2ull * B + 2);
-            fillU32<<<gridFor(2ull * B + 2), kB, 0, ctx->stream>>>(ctx->lalEnd.p, 0u, 2ull * B + 2);
-            segmentBounds<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalHalf.p, A2, ctx->lalBegin.p,
-                                                               ctx->lalEnd.p);
-            areaFix<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->area.p, A2, ctx->lalPrefix.p);
-            KT_TRY(scanInclusive64(ctx, ctx->lalPrefix.p, A2));
-            fillU64<<<gridFor(S + 1), kB, 0, ctx->stream>>>(sa.lo, ~0ull, S + 1);
-            fillU64<<<gridFor(S + 1), kB, 0, ctx->stream>>>(sa.hi, 0ull, S + 1);
-            stripeRanges<<<gridFor(H), kB, 0, ctx->stream>>>(ctx->lalBegin.p, ctx->lalEnd.p, ctx->lalPrefix.p, H, sa);
-            KT_HIP(hipGetLastError());
-            stripeOfCell<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, ctx->lalHalf.p, A2, sa, ca,
-                                                              ctx->lalStripeOf.p);
-            KT_HIP(hipGetLastError());
-            scaleCells<<<gridFor(A2), kB, 0, ctx->stream>>>(ctx->lalIds.p, A2, ctx->lalStripeOf.p, sa, ca,
-                                                            const_cast<double *>(ca.pos));
-            KT_HIP(hipGetLastError());
-            listRemap<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, ctx->lalHalf.p, dRemap, ctx->lalSeg.p);
-            KT_HIP(hipGetLastError());
+            blocks.swap(kept);
+            vertical = !vertical;
+            A = A2;
         }
-        blocks.swap(kept);
-        vertical = !vertical;
-        A = A2;
+        return KT_OK;
+    };
+    // The regions of a batch run together on the cells they held at the start.
+    // The reference runs them one after the other: a cell that a region moves
+    // into the bins of a later region of the batch is legalized there too.  So
+    // when that happens, the regions from the first such one on are restored
+    // and run again with the cells their bins hold now (the earlier regions'
+    // results stand), until no cell lands in a later region.
+    uint32_t active = M;
+    for (uint32_t first = 0;;) {
+        KT_TRY(runLevels(active));
+        KT_TRY(ctx->lalCount.reserve(2));
+        const uint32_t none[2] = {kNoVar, 0u};
+        KT_HIP(hipMemcpyAsync(ctx->lalCount.p, none, sizeof(none), hipMemcpyHostToDevice, ctx->stream));
+        escapeCheck<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, first, bs, ctx->lalCount.p);
+        KT_HIP(hipGetLastError());
+        uint32_t bad = kNoVar;
+        KT_HIP(hipMemcpyAsync(&bad, ctx->lalCount.p, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        KT_HIP(hipStreamSynchronize(ctx->stream));
+        if (bad == kNoVar) break;
+        repairBatch<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, bad, bs);
+        KT_HIP(hipGetLastError());
+        resegBatch<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, bad, bs, ctx->lalCount.p + 1);
+        KT_HIP(hipGetLastError());
+        KT_HIP(hipMemcpyAsync(&active, ctx->lalCount.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        KT_HIP(hipStreamSynchronize(ctx->stream));
+        first = bad;
+        ++ctx->lalReruns;
     }
     // the batch's cells whose bin changed
     KT_TRY(ctx->lalMoved.reserve(2ull * M));
@@ -1099,16 +1230,16 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, std::vect
     KT_HIP(hipMemsetAsync(ctx->lalCount.p, 0, sizeof(uint32_t), ctx->stream));
     Moved *dMoved = reinterpret_cast<Moved *>(ctx->lalMoved.p);
     movedCells<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, ctx->posX.p, ctx->posY.p, ctx->width.p, ctx->height.p,
-                                                   ctx->area.p, gridDev(g), ctx->binOf.p, dMoved, ctx->lalCount.p);
+                                                   ctx->area.p, gridDev(g), ctx->lalBinStart.p, ctx->binOf.p, dMoved,
+                                                   ctx->lalCount.p);
     KT_HIP(hipGetLastError());
     uint32_t moves = 0;
     KT_HIP(hipMemcpyAsync(&moves, ctx->lalCount.p, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     KT_HIP(hipStreamSynchronize(ctx->stream));
     moved.resize(moves);
-    if (moves) {
-        KT_HIP(hipMemcpyAsync(moved.data(), dMoved, moves * sizeof(Moved), hipMemcpyDeviceToHost, ctx->stream));
-        KT_HIP(hipStreamSynchronize(ctx->stream));
-    }
+    KT_TRY(unpaint());
+    if (moves) KT_HIP(hipMemcpyAsync(moved.data(), dMoved, moves * sizeof(Moved), hipMemcpyDeviceToHost, ctx->stream));
+    KT_HIP(hipStreamSynchronize(ctx->stream));
     return KT_OK;
 }
 

@@ -168,11 +168,14 @@ void buildGrid(Grid &g, FreeSpace &fs, const std::vector<RowSpec> &rows, const s
 void AreaTables::buildAvail(const Grid &g) {
     nx = g.nx;
     ny = g.ny;
-    avail.assign((nx + 1) * (ny + 1), 0.0);
+    // free area in units of 2^-40: exact integer sums (modulo 2^64, which
+    // cancels in the rectangle difference); per-bin rounding below 1e-12
+    avail.assign((nx + 1) * (ny + 1), 0);
     for (size_t iy = 0; iy < ny; ++iy)
         for (size_t ix = 0; ix < nx; ++ix) {
             const size_t o = (iy + 1) * (nx + 1) + ix + 1;
-            avail[o] = g.avail[g.at(ix, iy)] + avail[o - 1] + avail[o - (nx + 1)] - avail[o - (nx + 1) - 1];
+            const uint64_t v = static_cast<uint64_t>(std::llround(std::ldexp(g.avail[g.at(ix, iy)], 40)));
+            avail[o] = v + avail[o - 1] + avail[o - (nx + 1)] - avail[o - (nx + 1) - 1];
         }
     occRow.assign(ny * (nx + 1), 0);
 }
@@ -188,7 +191,7 @@ void AreaTables::updateRow(const std::vector<uint64_t> &occFix, size_t iy) {
 }
 
 namespace {
-double rect(const std::vector<double> &s, size_t nx, const BinRect &r) {
+uint64_t rect(const std::vector<uint64_t> &s, size_t nx, const BinRect &r) {
     const size_t w = nx + 1;
     return s[(r.y1 + 1) * w + r.x1 + 1] - s[r.y0 * w + r.x1 + 1] - s[(r.y1 + 1) * w + r.x0] + s[r.y0 * w + r.x0];
 }
@@ -202,7 +205,9 @@ double AreaTables::sumOcc(const BinRect &r) const {
     }
     return static_cast<double>(t) / 65536.0;
 }
-double AreaTables::sumAvail(const BinRect &r) const { return rect(avail, nx, r); }
+double AreaTables::sumAvail(const BinRect &r) const {
+    return std::ldexp(static_cast<double>(static_cast<int64_t>(rect(avail, nx, r))), -40);
+}
 
 // simpl_lal.cc:86-93
 double AreaTables::density(const BinRect &r) const {

@@ -65,13 +65,14 @@ struct RowSpec {
 void buildGrid(Grid &g, FreeSpace &fs, const std::vector<RowSpec> &rows, const std::vector<Box> &obstacles,
                const Box &die, size_t perAxis);
 
-// Rectangle sums for cluster growth.  Free area: a summed-area table (the
-// grid's free area does not change).  Occupancy: per-row prefix sums of the
-// fixed-point occupancy (units of 2^-16), exact in any order and patched row
+// Rectangle sums for cluster growth, both exact integer sums.  Free area: a
+// summed-area table in units of 2^-40 (the grid's free area does not change;
+// a double summed-area table would carry rounding that grows along the
+// diagonal).  Occupancy: per-row prefix sums in units of 2^-16, patched row
 // by row when cells move.
 struct AreaTables {
     size_t nx = 0, ny = 0;
-    std::vector<double> avail;      // (nx+1) x (ny+1)
+    std::vector<uint64_t> avail;    // (nx+1) x (ny+1), 2^-40 units
     std::vector<int64_t> occRow;    // ny x (nx+1)
     void buildAvail(const Grid &g);
     void setOcc(const std::vector<uint64_t> &occFix);

@@ -45,10 +45,12 @@ class SimplOptions:
 
 @dataclass
 class PlaceResult:
-    x: np.ndarray
+    x: np.ndarray           # upper-bound (legalized) placement
     y: np.ndarray
     stats: dict
     trace: list = field(default_factory=list)
+    lower_x: np.ndarray | None = None  # lower-bound (solved) placement
+    lower_y: np.ndarray | None = None
 
 
 class Placer:
@@ -181,4 +183,4 @@ class Placer:
         self.stats["gap"] = self.stats["hpwl_final"] - self.stats["hpwl_lower"]
         self.stats["overflow_lower"] = self.overflow(lx, ly)
         self.stats["overflow_final"] = self.overflow(ux, uy)
-        return PlaceResult(ux, uy, self.stats, trace)
+        return PlaceResult(ux, uy, self.stats, trace, lx, ly)

@@ -27,7 +27,15 @@
 //   FreeSpace / AxisProfile           hand-computed profiles
 //   LookAheadLegalizer   invariants only (in die, overflow reduced, order kept
 //                        inside stripes, deterministic): parity unpinned
-//                        beyond these invariants
+//                        beyond these invariants.  The regions of a batch run
+//                        one after the other, which is the reference with one
+//                        thread: simpl_lal.cc:102-122 runs them in a
+//                        parallel_for whose updates of members/binOf/occ
+//                        reach outside a region when a cell wider than its
+//                        stripe ends up centred in another region's bins,
+//                        so with more threads the reference's result there
+//                        depends on timing.  ko_set_lal_snapshot(1) gives the
+//                        other extreme (all regions collect first)
 //   simplPlacer loop     HPWL bounds (lower <= upper), monotone initial rounds:
 //                        parity unpinned beyond these invariants
 #include "../include/ktplace_gen.h"
@@ -53,6 +61,10 @@ namespace {
 // ---------------------------------------------------------------- parallel
 // simpl_parallel.h:14-57
 constexpr std::size_t grain = 4096;
+// Legalization batches: false runs the regions of a batch one after the other
+// (the reference with one thread); true collects every region's cells before
+// any region moves one (its regions running concurrently).
+bool g_lalSnapshot = false;
 
 template <typename Body>
 void taskRange(std::size_t n, std::size_t g, Body &body) {
@@ -950,17 +962,35 @@ struct Lal {
             return a.x0 <= b.x1 && b.x0 <= a.x1 && a.y0 <= b.y1 && b.y0 <= a.y1;
         };
         std::vector<std::uint32_t> taken(n, 0);
-        const auto legalizeRegion = [&](const Turn &t) {
-            Block top{grid.getBox(t.bins), 1, true, {}, grid.sum(grid.avail, t.bins)};
+        const auto collect = [&](const Turn &t) {
+            std::vector<std::uint32_t> cells;
             for (std::size_t iy = t.bins.y0; iy <= t.bins.y1; ++iy)
                 for (std::size_t ix = t.bins.x0; ix <= t.bins.x1; ++ix) {
                     const std::size_t k = grid.at(ix, iy);
                     for (const std::uint32_t i : members[k])
                         if (binOf[i] == k && taken[i] != t.id) {
                             taken[i] = t.id;
-                            top.cells.push_back(i);
+                            cells.push_back(i);
                         }
                 }
+            return cells;
+        };
+        const auto legalizeCells = [&](const Turn &t, const std::vector<std::uint32_t> &cells) {
+            Block top{grid.getBox(t.bins), 1, true, cells, grid.sum(grid.avail, t.bins)};
+            legalizeBlock(top, x, y);
+            for (const std::uint32_t i : cells) {
+                const std::size_t k = grid.getBinOf({centre(x, i, true), centre(y, i, false)});
+                if (k != binOf[i]) {
+                    occ[binOf[i]] -= d.area[i];
+                    occ[k] += d.area[i];
+                    binOf[i] = static_cast<std::uint32_t>(k);
+                    members[k].push_back(i);
+                }
+            }
+        };
+        const auto legalizeRegion = [&](const Turn &t) {
+            Block top{grid.getBox(t.bins), 1, true, {}, grid.sum(grid.avail, t.bins)};
+            top.cells = collect(t);
             const std::vector<std::uint32_t> cells = top.cells;
             legalizeBlock(top, x, y);
             for (const std::uint32_t i : cells) {
@@ -975,7 +1005,14 @@ struct Lal {
         };
         std::vector<Turn> batch;
         const auto runBatch = [&] {
-            for (const Turn &t : batch) legalizeRegion(t);
+            if (g_lalSnapshot) {
+                // every region's cells collected before any region moves one
+                std::vector<std::vector<std::uint32_t>> cellsOf(batch.size());
+                for (std::size_t b = 0; b < batch.size(); ++b) cellsOf[b] = collect(batch[b]);
+                for (std::size_t b = 0; b < batch.size(); ++b) legalizeCells(batch[b], cellsOf[b]);
+            } else {
+                for (const Turn &t : batch) legalizeRegion(t);
+            }
             batch.clear();
         };
         std::uint32_t turn = 0;
@@ -1391,6 +1428,8 @@ uint64_t ko_profile(const ko_design *h, const double *box, int vertical, double 
     }
     return p.edges.size();
 }
+
+void ko_set_lal_snapshot(int on) { g_lalSnapshot = on != 0; }
 
 void ko_lal(ko_design *h, double *x, double *y) {
     Design &d = *reinterpret_cast<Design *>(h);
