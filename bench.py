@@ -28,9 +28,10 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "B2B+PCG solve ms to 1e-6 rel residual @1M cells: mixed+learned vs plain FP64"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per SpMV launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
-# this workload, scaled by the copy-probe calibration (profiles/cgSpmv_cells1m.md)
-MEASURED_TRAFFIC = {"fp64": int((972.2 + 13.7) * 1e6), "mixed": int((455.4 + 14.4) * 1e6)}
+# HBM bytes per SpMV dispatch on this workload (averaged over the dispatches of a
+# solve), from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by the copy-probe
+# calibration: profiles/cgSpmv_f64_cells1m.md, profiles/cgSpmv_f32_cells1m.md
+MEASURED_TRAFFIC = {"fp64": int((936.1 + 13.4) * 1e6), "mixed": int((470.2 + 15.2) * 1e6)}
 
 
 def log(msg: str) -> None:
