@@ -19,7 +19,9 @@
 //                         the off-diagonal multiset, diagonal = row weight sum,
 //                         Sigma w*l^2 = Sigma_nets weight*HPWL when every edge
 //                         is longer than the floor
-//   buildStarAxis         brute-force dense assembly of the star model on tiny nets
+//   buildStarAxis         a hand-solved two-net case (m = 5.25, star 2.75); A v - b
+//                         equals the finite-difference gradient of the star
+//                         model's energy written from its definition
 //   multiply / getDot     dense numpy product; serial chunked sum
 //   solveBothAxes         scipy direct solve; textbook PCG iterates; breakdown and
 //                         zero-rhs rules; lock-step observer history

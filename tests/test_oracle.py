@@ -505,3 +505,92 @@ def test_place_small_design_runs():
     assert st["overflow_final"] < 0.5
     assert st["hpwl_final"] == pytest.approx(o.hpwl(x, y), rel=1e-12)
     assert len(trace) == st["global_iters"]
+
+
+# ------------------------------------------------------------------ star model
+
+
+def test_star_known_answer():
+    """buildStarAxis (simpl_lss.cc:138-201) on a hand-solved case: net A joins
+    pads at x = 0 and x = 3 and a movable cell m (3 pins: a star s with
+    w = 1 * 3 / 2); net B joins m to a pad at x = 9 (2 pins: w = 1).
+    Minimising 1.5 (s^2 + (s - 3)^2 + (m - s)^2) / 2 + (m - 9)^2 / 2 gives
+    3 s = 3 + m and 2.5 m = 1.5 s + 9, so m = 5.25, s = 2.75."""
+    cells = [(0.0, 0.0, 0.0, 0.0, True), (3.0, 0.0, 0.0, 0.0, True), (9.0, 0.0, 0.0, 0.0, True),
+             (1.0, 1.0, 0.0, 1.0, False)]
+    nets = [[(0, 0.0, 0.0), (1, 0.0, 0.0), (3, 0.0, 0.0)], [(3, 0.0, 0.0), (2, 0.0, 0.0)]]
+    raw = dz.make_raw(cells, nets, die=(0, 0, 10, 2), rows=[(0.0, 1.0, 0.0, 10.0), (1.0, 1.0, 0.0, 10.0)])
+    o = ol.Oracle(raw)
+    size = o.build_star()
+    assert size == 2  # one movable cell + one star
+    sx = o.system(0)
+    assert np.allclose(sx.dense(), [[2.5, -1.5], [-1.5, 4.5]])
+    assert np.allclose(sx.rhs, [9.0, 4.5])
+    r = o.solve(np.zeros(2), np.zeros(2), tol=1e-14, max_iter=100)
+    assert r["x"] == pytest.approx([5.25, 2.75], abs=1e-12)
+
+
+def star_energy_gradient(raw, flat, size, v, axis):
+    """Gradient of the star model's energy by central differences, from the
+    model's definition: a 2-pin net is a spring of weight w between its pins;
+    a net of p >= 3 pins with a movable pin has a star joined to every pin by
+    w * p / (p - 1); E = sum over springs w (u_a - u_b)^2 / 2, u = position +
+    offset (a star has offset 0)."""
+    n = flat.num_movable
+    var = flat.var_of_vertex.astype(np.int64)
+    fixed = flat.fixed_x if axis == 0 else flat.fixed_y
+    offs = flat.pin_off_x if axis == 0 else flat.pin_off_y
+    starts = flat.net_start.astype(np.int64)
+    springs = []
+    star = n
+    for k in range(flat.num_nets):
+        pins = range(starts[k], starts[k + 1])
+        p = len(pins)
+        cellsk = [int(flat.pin_cell[q]) for q in pins]
+        if p == 2:
+            springs.append((cellsk[0], offs[pins[0]], cellsk[1], offs[pins[1]], flat.net_weight[k], False))
+        elif p >= 3 and any(var[c] >= 0 and var[c] != 0xFFFFFFFF for c in cellsk):
+            w = flat.net_weight[k] * p / (p - 1)
+            for c, q in zip(cellsk, pins):
+                springs.append((c, offs[q], star, 0.0, w, True))
+            star += 1
+    assert star == size
+
+    def pos(c, is_star, vec):
+        if is_star:
+            return vec[c]
+        j = var[c]
+        return vec[j] if j != 0xFFFFFFFF else fixed[c]
+
+    def energy(vec):
+        e = 0.0
+        for a, oa, b, ob, w, to_star in springs:
+            ua = pos(a, False, vec) + oa
+            ub = (vec[b] if to_star else pos(b, False, vec)) + ob
+            e += 0.5 * w * (ua - ub) ** 2
+        return e
+
+    g = np.zeros(size)
+    h = 1e-4
+    for i in range(size):
+        e = np.zeros(size)
+        e[i] = h
+        g[i] = (energy(v + e) - energy(v - e)) / (2 * h)
+    return g
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_star_system_is_the_energy_gradient(seed):
+    """A v - b equals the gradient of the star-model energy at random points,
+    for both axes (offsets, fixed pins and 2-pin springs included)."""
+    rng = np.random.default_rng(100 + seed)
+    raw = random_raw(rng, n_mov=8, n_fixed=3, n_nets=9, max_p=5)
+    o = ol.Oracle(raw)
+    size = o.build_star()
+    flat = dz.collect(raw).flat
+    for axis in (0, 1):
+        s = o.system(axis)
+        v = rng.uniform(0, 20, size)
+        g = star_energy_gradient(raw, flat, size, v, axis)
+        got = s.dense() @ v - s.rhs
+        assert np.allclose(got, g, rtol=1e-7, atol=1e-6)
