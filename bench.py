@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per SpMV dispatch on this workload (averaged over the dispatches of a
 # solve), from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by the copy-probe
 # calibration: profiles/cgSpmv_f64_cells1m.md, profiles/cgSpmv_f32_cells1m.md
-MEASURED_TRAFFIC = {"fp64": int((936.1 + 13.4) * 1e6), "mixed": int((470.2 + 15.2) * 1e6)}
+MEASURED_TRAFFIC = {"fp64": int((758.4 + 12.7) * 1e6), "mixed": int((486.1 + 7.3) * 1e6)}
 
 
 def log(msg: str) -> None:
@@ -88,6 +88,8 @@ def main() -> int:
     ap.add_argument("--lanes", type=int, default=0, help="skip the autotuner: lanes per row")
     ap.add_argument("--tile", type=int, default=0, help="skip the autotuner: rows per block")
     ap.add_argument("--xcd-split", type=int, default=1, help="with --lanes/--tile: 1 axis per XCD half")
+    ap.add_argument("--rows", type=int, default=2, help="with --lanes/--tile: rows in flight per lane group")
+    ap.add_argument("--halo", type=int, default=0, help="with --lanes/--tile: SpMV LDS window halo (entries)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -131,16 +133,19 @@ def main() -> int:
         ctx.restore_positions()
         ctx.build_b2b_resident(0.0)
         if args.lanes and args.tile:
-            tuned = TuneResult(args.lanes, args.tile, float("nan"), [], args.xcd_split)
+            tuned = TuneResult(args.lanes, args.tile, float("nan"), [], args.xcd_split, args.rows, args.halo)
         elif prec in tuned_by_prec:
             tuned = tuned_by_prec[prec]
         else:
             tuned = autotune(ctx, prec)
             tuned_by_prec[prec] = tuned
-        ctx.set_tuning(lanes=tuned.lanes, tile=tuned.tile, xcd_split=tuned.xcd_split, rows=tuned.rows)
+        ctx.set_tuning(lanes=tuned.lanes, tile=tuned.tile, xcd_split=tuned.xcd_split, rows=tuned.rows,
+                       halo=tuned.halo)
         log(f"[bench] {path}: autotune lanes={tuned.lanes} tile={tuned.tile} xcd_split={tuned.xcd_split} "
-            f"rows={tuned.rows} "
+            f"rows={tuned.rows} halo={tuned.halo} "
             f"({tuned.ms_per_iteration:.3f} ms/iteration)")
+        for row in sorted(tuned.table, key=lambda t: t[-1])[:6]:
+            log(f"[bench]   candidate lanes/tile/xcd/rows/halo {row[:-1]}: {row[-1]:.4f} ms/iteration")
 
         def step(profile: bool, prec=prec, warm=warm) -> dict:
             ctx.restore_positions()
@@ -189,7 +194,7 @@ def main() -> int:
             warm_start_ms=(round(warm_ms / warm_n, 3) if warm_n else None),
             mlp_inference_ms=(round(mlp_ms / mlp_n, 3) if mlp_n else None),
             tuning={"lanes_per_row": tuned.lanes, "tile_rows": tuned.tile,
-                    "xcd_split": tuned.xcd_split, "rows_in_flight": tuned.rows},
+                    "xcd_split": tuned.xcd_split, "rows_in_flight": tuned.rows, "lds_halo": tuned.halo},
             roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                           frac=round(achieved / HBM_PEAK_GBS, 4),
                           traffic=(MEASURED_TRAFFIC.get(base_path) if args.workload == "cells1m" else None),

@@ -193,6 +193,13 @@ int kt_set_tuning(kt_ctx *ctx, int lanes, uint32_t tile, int use_graphs, double 
  * CSR rows each lane group keeps in flight in the SpMV (2, 4 or 8; 0 = 2). */
 int kt_set_tuning2(kt_ctx *ctx, int lanes, uint32_t tile, int use_graphs, double refresh_factor,
                    int xcd_split, int rows_in_flight);
+/* LDS staging of the CG SpMV (a launch-shape knob with no counterpart in the
+ * reference): each block copies p[tile start - halo, tile end + halo) into
+ * LDS and reads the columns inside that window from there.  halo: a multiple
+ * of 256 entries, 0 = no staging (the default).  The window, (tile + 2 halo)
+ * entries of the solve's working precision, must fit 156 KiB, or the solve
+ * fails with KT_EINVAL.  Results do not depend on it. */
+int kt_set_spmv_halo(kt_ctx *ctx, uint32_t halo);
 
 /* ---- per-kernel timings (HIP events on the context's stream) -------------- */
 #define KT_KERNEL_SPMV 0      /* FP64 CG SpMV + p.Ap */

@@ -1,11 +1,14 @@
 """Launch-shape autotuner for the two-axis Jacobi-PCG kernels.
 
 Chooses the SpMV lanes per row (threads cooperating on one CSR row: 1, 2, 4,
-8 or 16) and the tile (rows per 1024-thread block) by timing a fixed number
-of CG iterations of the context's current systems on the GPU, from the same
-saved start each time.  The search costs a few hundred iterations, i.e. a
-fraction of one solve; results are cached per (rows, nnz, precision)
-signature for the life of the process.
+8 or 16), the tile (rows per 1024-thread block), rows in flight, the XCD
+split and the LDS window (`halo`: entries of p staged on each side of a
+tile) by timing a fixed number of CG iterations of the context's current
+systems on the GPU, from the same saved start each time.  First lanes,
+tile, rows and XCD split without staging; then, with the best rows and XCD
+split, every lanes, tile and halo whose window fits the LDS budget.  The search
+costs a few hundred iterations, i.e. a fraction of one solve; results are
+cached per (rows, nnz, precision) signature for the life of the process.
 """
 from __future__ import annotations
 
@@ -15,9 +18,12 @@ from dataclasses import dataclass
 from . import simpl
 
 LANES = (4, 8, 16)
-TILES = (2048, 4096)
+TILES = (1024, 2048, 4096)
 ROWS = (2, 4, 8)
 XCD = (1, 0)
+HALOS = (2048, 4096, 6144, 8192, 12288, 16384)
+HALO_TILES = (1024, 2048, 4096)
+WINDOW_BYTES = 156 * 1024  # LDS a staged SpMV block may take
 
 _cache: dict = {}
 
@@ -30,6 +36,7 @@ class TuneResult:
     table: list
     xcd_split: int = 1
     rows: int = 2
+    halo: int = 0
 
 
 def _signature(ctx: "simpl.Context", precision: int):
@@ -38,26 +45,31 @@ def _signature(ctx: "simpl.Context", precision: int):
     return (rx, nx, ry, ny, precision)
 
 
+def window_fits(tile: int, halo: int, precision: int) -> bool:
+    width = 4 if precision == simpl.KT_PREC_MIXED else 8
+    return (tile + 2 * halo) * width <= WINDOW_BYTES
+
+
 def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iterations: int = 48,
-             lanes=LANES, tiles=TILES, use_graphs: bool = True, force: bool = False) -> TuneResult:
+             lanes=LANES, tiles=TILES, use_graphs: bool = True, force: bool = False,
+             halos=HALOS) -> TuneResult:
     """Time `iterations` CG iterations per candidate and apply the fastest.
 
     The context's positions are saved before and restored after the search,
-    so the caller's state is unchanged.
+    so the caller's state is unchanged.  `halos=()` leaves staging off.
     """
     key = _signature(ctx, precision)
     if not force and key in _cache:
         best = _cache[key]
         ctx.set_tuning(lanes=best.lanes, tile=best.tile, use_graphs=use_graphs,
-                       xcd_split=best.xcd_split, rows=best.rows)
+                       xcd_split=best.xcd_split, rows=best.rows, halo=best.halo)
         return best
     rows = key[0]
     ctx.save_positions()
     table = []
-    candidates = [(g, t, x, rr) for t in tiles for g in lanes for x in XCD for rr in ROWS
-                  if not (t > 4 * max(rows, 1024) and t != tiles[0])]
-    for g, tile, xs, rr in candidates:
-        ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr)
+
+    def timed(g, tile, xs, rr, halo):
+        ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr, halo=halo)
         ctx.restore_positions()
         ctx.solve_resident(0.0, 2, precision=precision)  # graph capture, first touch
         ctx.restore_positions()
@@ -67,10 +79,20 @@ def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iteratio
         ctx.sync()
         dt = time.perf_counter() - t0
         its = max(r["iterations"], 1)
-        table.append((g, tile, xs, rr, 1e3 * dt / its))
+        table.append((g, tile, xs, rr, halo, 1e3 * dt / its))
+
+    small = lambda t: not (t > 4 * max(rows, 1024) and t != tiles[0])
+    for g, tile, xs, rr in [(g, t, x, rr) for t in tiles for g in lanes for x in XCD for rr in ROWS if small(t)]:
+        timed(g, tile, xs, rr, 0)
+    _, _, xs, rr, _, _ = min(table, key=lambda t: t[-1])
+    for tile in HALO_TILES:
+        for halo in halos:
+            if window_fits(tile, halo, precision) and halo < rows and small(tile):
+                for g in lanes:
+                    timed(g, tile, xs, rr, halo)
     ctx.restore_positions()
-    g, tile, xs, rr, ms = min(table, key=lambda t: t[4])
-    best = TuneResult(lanes=g, tile=tile, ms_per_iteration=ms, table=table, xcd_split=xs, rows=rr)
+    g, tile, xs, rr, halo, ms = min(table, key=lambda t: t[-1])
+    best = TuneResult(lanes=g, tile=tile, ms_per_iteration=ms, table=table, xcd_split=xs, rows=rr, halo=halo)
     _cache[key] = best
-    ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr)
+    ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr, halo=halo)
     return best

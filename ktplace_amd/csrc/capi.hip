@@ -759,6 +759,14 @@ int kt_set_tuning2(kt_ctx *ctx, int lanes, uint32_t tile, int use_graphs, double
     return KT_OK;
 }
 
+int kt_set_spmv_halo(kt_ctx *ctx, uint32_t halo) {
+    KT_GUARD(ctx);
+    KT_CHECK(halo % 256 == 0, KT_EINVAL, "kt_set_spmv_halo: halo must be a multiple of 256");
+    KT_CHECK(halo <= 65536, KT_EINVAL, "kt_set_spmv_halo: halo above 65536 entries");
+    ctx->tune.halo = halo;
+    return KT_OK;
+}
+
 int kt_kernel_times(kt_ctx *ctx, int kernel, uint64_t *launches, double *total_ms) {
     KT_GUARD(ctx);
     KT_CHECK(kernel >= 0 && kernel < KT_KERNEL_COUNT, KT_EINVAL, "unknown kernel id");

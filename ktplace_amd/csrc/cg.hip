@@ -37,6 +37,9 @@ namespace kt {
 namespace {
 
 constexpr int kBlock = 1024;
+// Dynamic LDS a staged SpMV block may take (of the 160 KiB per CU; the
+// block's static reduction buffers need a few hundred bytes more).
+constexpr uint64_t kMaxWindowBytes = 156ull * 1024;
 
 // Block -> (axis, tile).  Workgroups are dealt to the 8 XCDs round-robin by
 // id, so with xcdSplit the blocks that land on XCDs 0-3 take the x system and
@@ -196,8 +199,30 @@ __global__ void __launch_bounds__(kBlock) cgStartFinish(CgArgs a) {
 }
 
 // ---------------------------------------------------------------- one iteration
-// Ap = A p and the tile partial of p.Ap.
-template <typename T, int G, int R>
+// Copies v[lo, lo + span) into the block's LDS window (16-byte loads; lo is a
+// multiple of 256, so both ends of the vector part are 16-byte aligned).
+template <typename T>
+__device__ __forceinline__ void stageWindow(const T *__restrict__ v, uint32_t lo, uint32_t span, T *lds) {
+    constexpr uint32_t kVec = 16 / sizeof(T);
+    const uint32_t vecs = span / kVec;
+    const uint4 *__restrict__ src = reinterpret_cast<const uint4 *>(v + lo);
+    uint4 *dst = reinterpret_cast<uint4 *>(lds);
+    for (uint32_t k = threadIdx.x; k < vecs; k += kBlock) dst[k] = src[k];
+    for (uint32_t k = vecs * kVec + threadIdx.x; k < span; k += kBlock) lds[k] = v[lo + k];
+}
+
+// LDS bytes of a tile's window: the tile and `halo` entries on each side.
+template <typename T>
+constexpr uint64_t windowBytes(uint32_t tile, uint32_t halo) {
+    return (static_cast<uint64_t>(tile) + 2ull * halo) * sizeof(T);
+}
+
+// Ap = A p and the tile partial of p.Ap.  With S, the block first stages
+// p[begin - halo, end + halo) in LDS (dynamic shared memory sized by the
+// launch) and gathers the columns inside that window from there: in a
+// placement netlist most of a row's columns lie near the row, so most
+// gathers stop costing an L2 request.
+template <typename T, int G, int R, bool S>
 __global__ void __launch_bounds__(kBlock) cgSpmv(CgArgs a, int parity) {
     int axis;
     uint32_t tile;
@@ -212,28 +237,42 @@ __global__ void __launch_bounds__(kBlock) cgSpmv(CgArgs a, int parity) {
     const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
     double acc[1] = {0.0};
     constexpr uint32_t kPass = R * (kBlock / G);
-    for (uint32_t r0 = begin; r0 < end; r0 += kPass) {
-        uint32_t row[R];
-        bool valid[R];
-        double off[R];
-        T pi[R], di[R];
+    const auto rows = [&](const auto &src) {
+        for (uint32_t r0 = begin; r0 < end; r0 += kPass) {
+            uint32_t row[R];
+            bool valid[R];
+            double off[R];
+            T pi[R], di[R];
 #pragma unroll
-        for (int q = 0; q < R; ++q) {
-            row[q] = r0 + grp + q * (kBlock / G);
-            valid[q] = row[q] < end;
-            pi[q] = valid[q] ? p[row[q]] : T(0);
-            di[q] = valid[q] ? diag[row[q]] : T(0);
-        }
-        rowsDot<T, G, R>(s.rowPtr, s.col, val, p, row, valid, lane, off);
+            for (int q = 0; q < R; ++q) {
+                row[q] = r0 + grp + q * (kBlock / G);
+                valid[q] = row[q] < end;
+                const uint32_t i = valid[q] ? row[q] : begin;
+                pi[q] = ownAt(src, i);
+                di[q] = diag[i];
+            }
+            rowsDot<T, G, R>(s.rowPtr, s.col, val, src, row, valid, lane, off);
 #pragma unroll
-        for (int q = 0; q < R; ++q) {
-            if (valid[q] && lane == 0) {
-                const double pd = static_cast<double>(pi[q]);
-                const T v = static_cast<T>(static_cast<double>(di[q]) * pd + off[q]);
-                ap[row[q]] = v;
-                acc[0] += pd * static_cast<double>(v);
+            for (int q = 0; q < R; ++q) {
+                if (valid[q] && lane == 0) {
+                    const double pd = static_cast<double>(pi[q]);
+                    const T v = static_cast<T>(static_cast<double>(di[q]) * pd + off[q]);
+                    ap[row[q]] = v;
+                    acc[0] += pd * static_cast<double>(v);
+                }
             }
         }
+    };
+    if constexpr (S) {
+        extern __shared__ uint4 cgWindow[];
+        T *lds = reinterpret_cast<T *>(cgWindow);
+        const uint32_t lo = begin > a.halo ? begin - a.halo : 0u;
+        const uint32_t hi = s.rows - end > a.halo ? end + a.halo : s.rows;
+        stageWindow<T>(p, lo, hi - lo, lds);
+        __syncthreads();
+        rows(Window<T>{p, (LdsPtr<T>)lds, lo, hi - lo});
+    } else {
+        rows(p);
     }
     blockSum<1>(acc);
     writePartial<1>(a.partA, a.stride, axis, tile, acc);
@@ -316,18 +355,24 @@ __global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
         for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) p[i] = r[i] / d[i] + beta * p[i];
     }
     if (lead) {
+        // mixed precision: ask for an FP64 refresh once the inner residual fell
+        // by the refresh factor, or reached the tolerance.  The axis pauses
+        // (live = 0) until the refresh, which sets live again from the true
+        // residual: FP32 steps past the refresh point would push the recurrence
+        // residual, and with it p, far below the true residual, and the first
+        // step after the refresh (alpha = r.z / p.Ap with the refreshed r and
+        // that p) would then be far too long.
+        const bool refresh = a.e[axis] && (resid <= c.refreshFactor * c.refResid || resid <= c.tol);
         CgState nx;
         nx.steps = st.steps + 1;
         nx.resid = resid;
         nx.rho = ok ? rhoNext : st.rho;
         nx.active = ok ? 1 : 0;
-        nx.live = (ok && resid > c.tol && nx.steps < c.maxIter) ? 1 : 0;
+        nx.live = (ok && resid > c.tol && nx.steps < c.maxIter && !refresh) ? 1 : 0;
         nx.pad = 0;
         c.s[parity ^ 1] = nx;
         if (st.steps < a.histCap) a.history[hist] = resid;
-        // mixed precision: ask for an FP64 refresh once the inner residual fell
-        // by the refresh factor, or reached the tolerance
-        if (a.e[axis] && (resid <= c.refreshFactor * c.refResid || resid <= c.tol)) c.needUpdate = 1;
+        if (refresh) c.needUpdate = 1;
     }
 }
 
@@ -488,26 +533,48 @@ uint32_t gridBlocks(const CgArgs &a) {
         default: KERNEL<16><<<GRID, dim3(kBlock), 0, STREAM>>>(__VA_ARGS__); break;        \
     }
 
+// The SpMV instance for (G, R, staging).  With attributesOnly it only raises
+// the instance's dynamic LDS limit to its window (done before any capture).
+template <typename T>
+int launchSpmv(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, bool attributesOnly) {
+    const int R = ctx->tune.rows > 0 ? ctx->tune.rows : 2;
+    const dim3 grid(gridBlocks(a));
+    const bool staged = a.halo > 0;
+    const uint32_t lds = staged ? static_cast<uint32_t>(windowBytes<T>(a.tile, a.halo)) : 0u;
+    const auto go = [&](auto kernel) -> int {
+        if (attributesOnly) {
+            if (staged)
+                KT_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+            return KT_OK;
+        }
+        kernel<<<grid, dim3(kBlock), lds, st>>>(a, parity);
+        KT_HIP(hipGetLastError());
+        return KT_OK;
+    };
+#define KT_SPMV_S(GG, RR) return staged ? go(cgSpmv<T, GG, RR, true>) : go(cgSpmv<T, GG, RR, false>);
+#define KT_SPMV_R(GG)                   \
+    if (R >= 8) { KT_SPMV_S(GG, 8) }    \
+    if (R >= 4) { KT_SPMV_S(GG, 4) }    \
+    KT_SPMV_S(GG, 2)
+    switch (G) {
+        case 1: KT_SPMV_R(1)
+        case 2: KT_SPMV_R(2)
+        case 4: KT_SPMV_R(4)
+        case 8: KT_SPMV_R(8)
+        default: KT_SPMV_R(16)
+    }
+#undef KT_SPMV_R
+#undef KT_SPMV_S
+}
+
 template <typename T>
 int launchIteration(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, bool timed) {
     const dim3 grid(gridBlocks(a));
     hipEvent_t t0 = nullptr;
     const int kSpmv = sizeof(T) == 8 ? KT_KERNEL_SPMV : KT_KERNEL_SPMV32;
     if (timed) KT_TRY(timedBegin(ctx, kSpmv, &t0));
-    const int R = ctx->tune.rows > 0 ? ctx->tune.rows : 2;
-#define KT_SPMV_R(GG)                                                                         \
-    if (R >= 8) cgSpmv<T, GG, 8><<<grid, dim3(kBlock), 0, st>>>(a, parity);                  \
-    else if (R >= 4) cgSpmv<T, GG, 4><<<grid, dim3(kBlock), 0, st>>>(a, parity);             \
-    else cgSpmv<T, GG, 2><<<grid, dim3(kBlock), 0, st>>>(a, parity);
-    switch (G) {
-        case 1: KT_SPMV_R(1) break;
-        case 2: KT_SPMV_R(2) break;
-        case 4: KT_SPMV_R(4) break;
-        case 8: KT_SPMV_R(8) break;
-        default: KT_SPMV_R(16) break;
-    }
-#undef KT_SPMV_R
-    KT_HIP(hipGetLastError());
+    KT_TRY(launchSpmv<T>(ctx, a, G, parity, st, false));
     if (timed) KT_TRY(timedEnd(ctx, kSpmv, t0));
     if (timed) KT_TRY(timedBegin(ctx, KT_KERNEL_UPDATE, &t0));
     cgUpdate<T><<<grid, dim3(kBlock), 0, st>>>(a, parity);
@@ -676,6 +743,12 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
     a = CgArgs{};
     a.tile = tile;
     a.xcdSplit = ctx->tune.xcdSplit == 0 ? 0u : 1u;
+    a.halo = ctx->tune.halo;
+    if (a.halo) {
+        const uint64_t bytes = fp32 ? windowBytes<float>(tile, a.halo) : windowBytes<double>(tile, a.halo);
+        KT_CHECK(bytes <= kMaxWindowBytes, KT_EINVAL,
+                 "solve: the SpMV window (tile + 2 halo entries) exceeds the LDS budget");
+    }
     uint32_t maxBlocks = 1;
     for (int ax = 0; ax < 2; ++ax) {
         if (fp32) {
@@ -734,6 +807,8 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
         h.refreshFactor = ctx->tune.refreshFactor > 0.0 ? ctx->tune.refreshFactor : 0.05;
     }
     KT_TRY(writeCtl(ctx));
+    KT_TRY(fp32 ? launchSpmv<float>(ctx, a, G, 0, ctx->stream, true)
+                : launchSpmv<double>(ctx, a, G, 0, ctx->stream, true));
     return KT_OK;
 }
 

@@ -137,6 +137,8 @@ struct CgArgs {
     uint32_t stride;    // partial slots per axis
     uint32_t tile;      // rows per block
     uint32_t xcdSplit;  // 1: blocks on XCDs 0-3 take axis x, 4-7 axis y
+    uint32_t halo;      // SpMV: entries of p staged in LDS on each side of a tile (0: none)
+    uint32_t pad;
 };
 
 // Launch-shape knobs (set by the autotuner or the caller; 0 = default).
@@ -147,6 +149,7 @@ struct Tune {
     bool useGraphs = true;
     int xcdSplit = -1;        // -1 default (on), 0 off, 1 on
     double refreshFactor = 0; // mixed precision refresh factor (default 0.05)
+    uint32_t halo = 0;        // SpMV LDS window: entries on each side of a tile (0: no staging)
 };
 
 // A captured hipGraph of kPairs iteration pairs for one argument set.

@@ -184,13 +184,13 @@ def test_mixed_precision_reaches_true_tolerance(b2b12k, gpu_ctx_factory):
     ctx.close()
 
 
-@pytest.mark.parametrize("lanes,tile,graphs", [(1, 1024, False), (2, 2048, True), (8, 4096, True),
-                                               (16, 8192, False)])
-def test_launch_shapes_agree(b2b12k, gpu_ctx_factory, lanes, tile, graphs):
+@pytest.mark.parametrize("lanes,tile,graphs,rows", [(1, 1024, False, 2), (2, 2048, True, 4), (8, 4096, True, 8),
+                                                    (16, 8192, False, 2)])
+def test_launch_shapes_agree(b2b12k, gpu_ctx_factory, lanes, tile, graphs, rows):
     """Every tuning point solves the same system to the same answer."""
     d, o, x0, y0 = b2b12k
     ctx = gpu_ctx_factory(d)
-    ctx.set_tuning(lanes=lanes, tile=tile, use_graphs=graphs)
+    ctx.set_tuning(lanes=lanes, tile=tile, use_graphs=graphs, rows=rows)
     ctx.build_b2b(x0, y0)
     x, y = x0.copy(), y0.copy()
     r = ctx.solve_both(x, y, 1e-9, 100000)
@@ -240,4 +240,94 @@ def test_exact_mode_is_bit_identical_to_oracle(b2b12k, gpu_ctx_factory):
     assert r["iterations"] == ref["iterations"]
     assert np.array_equal(x, ref["x"]) and np.array_equal(y, ref["y"])
     assert np.array_equal(np.array(seen), ref["history"])
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision,lanes,tile,halo,xcd", [
+    ("fp64", 8, 2048, 256, 1), ("fp64", 4, 1024, 4096, 0), ("fp64", 16, 1024, 8192, 1),
+    ("mixed", 16, 2048, 7168, 1), ("mixed", 4, 1024, 16384, 0), ("mixed", 8, 4096, 2048, 1)])
+def test_lds_window_is_bit_identical(b2b12k, gpu_ctx_factory, precision, lanes, tile, halo, xcd):
+    """Staging p in LDS changes where the SpMV reads p, not what it reads:
+    iterates, residual history and solutions equal the unstaged solve's bit
+    for bit (windows clipped at both ends of the vector, and one wider than
+    the whole system)."""
+    from ktplace_amd.simpl import KT_PREC_FP64, KT_PREC_MIXED
+    prec = KT_PREC_FP64 if precision == "fp64" else KT_PREC_MIXED
+    d, o, x0, y0 = b2b12k
+    ctx = gpu_ctx_factory(d)
+    ctx.build_b2b(x0, y0)
+    outs = []
+    for h in (0, halo):
+        ctx.set_tuning(lanes=lanes, tile=tile, xcd_split=xcd, halo=h)
+        x, y = x0.copy(), y0.copy()
+        seen = []
+        r = ctx.solve_both(x, y, 1e-8, 100000, precision=prec, observe=lambda it, res, last: seen.append(res))
+        outs.append((x, y, np.array(seen), r["steps"]))
+    (x1, y1, h1, s1), (x2, y2, h2, s2) = outs
+    assert s1 == s2
+    assert np.array_equal(h1, h2)
+    assert np.array_equal(x1, x2) and np.array_equal(y1, y2)
+    ctx.close()
+
+
+def test_lds_window_ragged_tail_and_limit(gpu_ctx_factory):
+    """A system whose row count is not a multiple of the 16-byte staging
+    width, and a window beyond the LDS budget (rejected, not truncated)."""
+    from ktplace_amd.simpl import KT_PREC_MIXED, KtError
+    rng = np.random.default_rng(5)
+    sx, sy = spd_system(rng, 3001, 12000), spd_system(rng, 3001, 9000)
+    ctx = gpu_ctx_factory()
+    ctx.set_system(0, sx)
+    ctx.set_system(1, sy)
+    outs = []
+    for halo in (0, 512):
+        for prec in (0, KT_PREC_MIXED):
+            ctx.set_tuning(lanes=4, tile=1024, halo=halo)
+            x, y = np.zeros(sx.n), np.zeros(sy.n)
+            ctx.solve_both(x, y, 1e-10, 100000, precision=prec)
+            outs.append((x, y))
+    for a, b in ((outs[0], outs[2]), (outs[1], outs[3])):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    ctx.set_tuning(lanes=4, tile=2048, halo=16384)  # (2048 + 32768) * 8 bytes
+    with pytest.raises(KtError):
+        ctx.solve_both(np.zeros(sx.n), np.zeros(sy.n), 1e-10, 100, precision=0)
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tolerance_below_fp32_accuracy_on_ragged_system(gpu_ctx_factory, precision):
+    """A well-conditioned system with an empty row, solved to 1e-10, below
+    what FP32 arithmetic resolves: the mixed path must get there by refreshes (an axis pauses at each refresh point;
+    FP32 steps run past it once drove p far below the refreshed residual and
+    the next step diverged), match scipy, and repeat bit for bit."""
+    from scipy.sparse.linalg import spsolve
+    from ktplace_amd.simpl import KT_PREC_FP64, KT_PREC_MIXED, KT_STOP_TRUE
+    prec = KT_PREC_FP64 if precision == "fp64" else KT_PREC_MIXED
+    rng = np.random.default_rng(11)
+    base = spd_system(rng, 2501, 11000)
+    d = base.scipy().tolil()
+    d[5, :] = 0
+    d[:, 5] = 0
+    d = d.tocsr()
+    d.setdiag(base.diag + np.abs(d).sum(axis=1).A1 + 1.0)
+    off = d.copy()
+    off.setdiag(0)
+    off.eliminate_zeros()
+    off.sort_indices()
+    sx = ol.System(off.indptr.astype(np.uint64), off.indices.astype(np.uint32), off.data.copy(),
+                   d.diagonal().copy(), rng.standard_normal(2501))
+    assert sx.row_ptr[6] == sx.row_ptr[5]
+    ctx = gpu_ctx_factory()
+    ctx.set_system(0, sx)
+    ctx.set_system(1, base)
+    outs = []
+    for _ in range(2):
+        ctx.set_tuning(lanes=2, tile=1024, rows=4)
+        x, y = np.zeros(2501), np.zeros(2501)
+        r = ctx.solve_both(x, y, 1e-10, 100000, precision=prec, stop_rule=KT_STOP_TRUE)
+        outs.append((x, y))
+        assert max(r["true_resid"]) <= 1e-10
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    for sol, s in ((outs[0][0], sx), (outs[0][1], base)):
+        assert rel(sol, spsolve(s.scipy().tocsc(), s.rhs)) < 1e-8
     ctx.close()
