@@ -5,8 +5,9 @@
  *
  *   ktg_generate    builds a raw placement design (cells, pads, macros, rows,
  *                   nets with possibly repeated cells) from one uint64 seed.
- *                   The reference reads Bookshelf/LEF-DEF files instead
- *                   (src/adaptor/); the synthetic design stands in for them.
+ *   ktg_read_bookshelf  reads the same from a Bookshelf design directory, as
+ *                   the reference's Bookshelf adaptor does (src/adaptor/);
+ *                   ktg_write_pl writes a placement back.
  *   ktg_collect     flattens a raw design into SimPL's solver netlist exactly
  *                   as simpl_dm.cc:16-85 (collectCells/collectNets) does: pins
  *                   stable-sorted by cell id and deduplicated keeping the first
@@ -87,6 +88,27 @@ int ktg_flat_view(const ktg_design *d, ktg_flat *view);
  * y = yl + U * max(H - h, 0) with U = (mt19937_64() >> 11) * 2^-53, x then y per cell. */
 int ktg_seed_uniform(uint64_t seed, uint64_t n, const double die[4], const double *width,
                      const double *height, double *x, double *y);
+
+/* A Bookshelf design directory <dir>/<name>.{nodes,nets,pl,scl,wts} (each
+ * optionally .gz; <name> = the directory's name), read as
+ * BookshelfInputAdapter::readFromDirectory (src/adaptor/bookshelfToKTAdaptor.cc:140-225)
+ * does, with the die of getPlacementDieBox (src/datamodel/kt_die.cc:45-103),
+ * then flattened as ktg_collect does.  Terminals and /FIXED cells are fixed;
+ * pin offsets are kept from the cell's lower-left. */
+int ktg_read_bookshelf(const char *dir, ktg_design **out);
+/* The vertex names of a design read from files, each followed by '\n', in
+ * vertex order; valid until ktg_free. */
+int ktg_vertex_names(ktg_design *d, const char **blob, uint64_t *bytes);
+/* A Bookshelf placement file as FlowMgr::Impl::writeBookshelf
+ * (src/kt_flowMgr.cc:508-521) writes it: every vertex in order as
+ * "name<TAB>x<TAB>y<TAB>: N", " /FIXED" after a fixed vertex's N, numbers
+ * with 10 significant digits.  A movable vertex v (var_of_vertex[v] !=
+ * KTG_NO_VAR) is written at (x[var], y[var]), a fixed one at (fixed_x[v],
+ * fixed_y[v]); positions are lower-left corners.  names: num_vertices names,
+ * each followed by '\n' (as ktg_vertex_names gives them). */
+int ktg_write_pl(const char *path, uint32_t num_vertices, const char *names, uint64_t names_bytes,
+                 const uint32_t *var_of_vertex, const double *fixed_x, const double *fixed_y, const double *x,
+                 const double *y);
 
 void ktg_free(ktg_design *d);
 const char *ktg_last_error(void);

@@ -112,6 +112,9 @@ _proto("ktg_raw_view", I32, P, C.POINTER(KtgRaw))
 _proto("ktg_collect", I32, C.POINTER(KtgRaw), C.POINTER(P))
 _proto("ktg_flat_view", I32, P, C.POINTER(KtgFlat))
 _proto("ktg_seed_uniform", I32, U64, U64, P, P, P, P, P)
+_proto("ktg_read_bookshelf", I32, C.c_char_p, C.POINTER(P))
+_proto("ktg_vertex_names", I32, P, C.POINTER(P), C.POINTER(U64))
+_proto("ktg_write_pl", I32, C.c_char_p, U32, C.c_char_p, U64, P, P, P, P, P)
 _proto("ktg_free", None, P)
 _proto("ktg_last_error", C.c_char_p)
 

@@ -138,10 +138,15 @@ void buildGrid(Grid &g, FreeSpace &fs, const std::vector<RowSpec> &rows, const s
     g.avail.assign(g.size(), 0.0);
     std::vector<FreeRow> raw;
     double smallestSite = std::numeric_limits<double>::max();
+    // One RowSpec per subrow; consecutive subrows of one row (same y and
+    // height) become the spans of one free-space row, as a row's subrows do.
     for (const RowSpec &r : rows) {
         if (!(r.height > 0.0) || !(r.pitch > 0.0)) continue;
         smallestSite = std::min(smallestSite, r.pitch * r.height);
-        raw.push_back({r.y, r.y + r.height, {{r.x0, r.x1}}});
+        if (!raw.empty() && raw.back().y0 == r.y && raw.back().y1 == r.y + r.height)
+            raw.back().spans.push_back({r.x0, r.x1});
+        else
+            raw.push_back({r.y, r.y + r.height, {{r.x0, r.x1}}});
     }
     fs.build(std::move(raw), obstacles);
     const auto column = [&](double x) { return clampIndex((x - g.ox) / g.dx, g.nx); };

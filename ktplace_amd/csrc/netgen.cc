@@ -25,7 +25,7 @@
 //   * Pin offsets: width * unit(), height * unit() from the lower-left.
 //   * A movable cell left on no net is appended to the first net of a cell in
 //     its window (8 draws), else to a uniform net.
-#include "ktplace_gen.h"
+#include "design_internal.h"
 
 #include <algorithm>
 #include <cmath>
@@ -37,37 +37,24 @@
 #include <string>
 #include <vector>
 
-struct ktg_design {
-    // raw
-    std::vector<uint8_t> isFixed;
-    std::vector<double> posX, posY, width, height;
-    std::vector<uint64_t> netStart;
-    std::vector<uint32_t> pinVertex;
-    std::vector<double> pinOffX, pinOffY, netWeight;
-    std::vector<double> rowY, rowH, rowX0, rowX1, rowPitch;
-    double die[4] = {0, 0, 0, 0};
-    uint32_t numMovable = 0;
-    // flat
-    bool flat = false;
-    std::vector<uint64_t> fNetStart;
-    std::vector<uint32_t> fPinCell;
-    std::vector<double> fOffX, fOffY, fWeight;
-    std::vector<uint32_t> varOfVertex, movVertex;
-    std::vector<double> fixedX, fixedY;
-    std::vector<uint64_t> pinStart;
-    std::vector<uint32_t> cellPinNet, cellPinIdx;
-    std::vector<double> mWidth, mHeight, mArea;
-    std::vector<double> obstacles;
-};
+namespace ktgi {
 
 namespace {
-
 thread_local std::string lastError;
+}  // namespace
 
 int fail(const std::string &msg) {
     lastError = msg;
     return -1;
 }
+
+const char *lastErrorText() { return lastError.c_str(); }
+
+}  // namespace ktgi
+
+namespace {
+
+using ktgi::fail;
 
 struct Rng {
     std::mt19937_64 g;
@@ -280,6 +267,11 @@ int generate(const ktg_spec &s, ktg_design &d) {
     return 0;
 }
 
+
+}  // namespace
+
+namespace ktgi {
+
 int collect(const ktg_raw &raw, ktg_design &d) {
     const uint32_t V = raw.num_vertices;
     // raw copy (so the handle is self-contained)
@@ -376,7 +368,7 @@ int collect(const ktg_raw &raw, ktg_design &d) {
     return 0;
 }
 
-}  // namespace
+}  // namespace ktgi
 
 extern "C" {
 
@@ -407,7 +399,7 @@ int ktg_generate(const ktg_spec *spec, ktg_design **out) {
         ktg_raw raw;
         ktg_raw_view(d, &raw);
         auto *flat = new ktg_design();
-        rc = collect(raw, *flat);
+        rc = ktgi::collect(raw, *flat);
         delete d;
         if (rc != 0) {
             delete flat;
@@ -452,7 +444,7 @@ int ktg_collect(const ktg_raw *raw, ktg_design **out) {
     if (raw == nullptr || out == nullptr) return fail("ktg_collect: null argument");
     try {
         auto *d = new ktg_design();
-        const int rc = collect(*raw, *d);
+        const int rc = ktgi::collect(*raw, *d);
         if (rc != 0) {
             delete d;
             return rc;
@@ -508,6 +500,6 @@ int ktg_seed_uniform(uint64_t seed, uint64_t n, const double die[4], const doubl
 
 void ktg_free(ktg_design *d) { delete d; }
 
-const char *ktg_last_error(void) { return lastError.c_str(); }
+const char *ktg_last_error(void) { return ktgi::lastErrorText(); }
 
 }  // extern "C"

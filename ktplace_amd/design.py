@@ -8,6 +8,7 @@ re-flattening and for the oracle) and the flat netlist that the C ABI uploads.
 from __future__ import annotations
 
 import ctypes as C
+from pathlib import Path
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -247,6 +248,44 @@ def make_raw(cells: list[tuple[float, float, float, float, bool]],
         row_pitch=np.ones(len(rows), np.float64),
         die=np.array(die, np.float64),
     ) if V else None
+
+
+def read_bookshelf(path) -> Design:
+    """A Bookshelf design directory (<dir>/<name>.nodes/.nets/.pl/.scl/.wts,
+    each optionally gzipped), read and flattened by the library's C++ reader
+    (as the reference's Bookshelf adaptor, src/adaptor/bookshelfToKTAdaptor.cc,
+    and getPlacementDieBox, src/datamodel/kt_die.cc:45-103).  The vertex
+    names are kept in `meta["names"]`."""
+    h = C.c_void_p()
+    _lib.check_ktg(lib.ktg_read_bookshelf(str(path).encode(), C.byref(h)))
+    blob, size = C.c_void_p(), C.c_uint64()
+    try:
+        _lib.check_ktg(lib.ktg_vertex_names(h, C.byref(blob), C.byref(size)))
+        names = C.string_at(blob.value, size.value).decode().split("\n")[:-1] if size.value else []
+    except BaseException:
+        lib.ktg_free(h)
+        raise
+    d = _take(h, Path(str(path).rstrip("/")).name)
+    d.meta = dict(names=names, source=str(path))
+    return d
+
+
+def write_pl(d: Design, path, x, y) -> None:
+    """A Bookshelf placement file of `d` with the movable cells at (x, y)
+    (lower-left corners), as the reference's writer (src/kt_flowMgr.cc:508-521)."""
+    names = d.meta.get("names")
+    if names is None or len(names) != d.flat.num_vertices:
+        raise ValueError("write_pl: the design has no vertex names (read it with read_bookshelf)")
+    blob = "".join(n + "\n" for n in names).encode()
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    if len(x) != d.n or len(y) != d.n:
+        raise ValueError("write_pl: x and y need one entry per movable cell")
+    f = d.flat
+    _lib.check_ktg(lib.ktg_write_pl(str(path).encode(), f.num_vertices, blob, len(blob),
+                                    _lib.ptr(np.ascontiguousarray(f.var_of_vertex)),
+                                    _lib.ptr(np.ascontiguousarray(f.fixed_x)),
+                                    _lib.ptr(np.ascontiguousarray(f.fixed_y)), _lib.ptr(x), _lib.ptr(y)))
 
 
 def seed_uniform(d: Design, seed: int = 20240607):

@@ -184,3 +184,49 @@ class Placer:
         self.stats["overflow_lower"] = self.overflow(lx, ly)
         self.stats["overflow_final"] = self.overflow(ux, uy)
         return PlaceResult(ux, uy, self.stats, trace, lx, ly)
+
+
+def main(argv=None) -> int:
+    """Global placement of a Bookshelf design (or a synthetic workload) on the
+    GPU; prints the statistics as JSON and writes the upper-bound placement as
+    a Bookshelf .pl (src/kt_flowMgr.cc:508-521).
+
+        python -m ktplace_amd.placer path/to/ibm01 --out placed.pl
+        python -m ktplace_amd.placer --workload cells210k --bins 512
+    """
+    import argparse
+    import json
+    import sys
+
+    from . import design as dz
+
+    ap = argparse.ArgumentParser(description="SimPL global placement on the GPU")
+    ap.add_argument("design", nargs="?", help="Bookshelf design directory (<dir>/<name>.nodes ...)")
+    ap.add_argument("--workload", help="synthetic workload name instead of a design directory")
+    ap.add_argument("--out", help="write the placement here as a Bookshelf .pl (design directories only)")
+    ap.add_argument("--bins", type=int, default=0, help="bins per axis (0: the reference's default rule)")
+    ap.add_argument("--init-net", default="star", choices=("star", "b2b", "star_b2b"))
+    ap.add_argument("--seed", type=int, default=20240607)
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "exact"),
+                    help="exact: the reference's operation order in the solver (bit-identical runs)")
+    args = ap.parse_args(argv)
+    if bool(args.design) == bool(args.workload):
+        ap.error("give a design directory or --workload")
+    d = dz.read_bookshelf(args.design) if args.design else dz.workload(args.workload)
+    prec = simpl.KT_PREC_FP64_EXACT if args.precision == "exact" else simpl.KT_PREC_FP64
+    opts = SimplOptions(binsPerAxis=args.bins, initialNetModel=args.init_net, seed=args.seed, precision=prec)
+    t0 = time.perf_counter()
+    res = Placer(d, opts, progress=lambda m: print(m, file=sys.stderr, flush=True)).run()
+    out = dict(design=d.name, cells=d.n, nets=d.flat.num_nets, seconds=round(time.perf_counter() - t0, 3),
+               **{k: (float(v) if isinstance(v, (float, np.floating)) else v) for k, v in res.stats.items()})
+    if args.out:
+        if not args.design:
+            ap.error("--out needs a design directory (a synthetic design has no names)")
+        dz.write_pl(d, args.out, res.x, res.y)
+        out["placement"] = args.out
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -681,10 +681,15 @@ void buildGrid(Design &d, std::size_t perAxis) {
     grid.avail.assign(grid.size(), 0.0);
     std::vector<FreeSpace::Row> rows;
     double smallestSite = std::numeric_limits<double>::max();
+    // one input row per subrow; consecutive subrows of one row (same y and
+    // height) are that row's spans (simpl_placer.cc:199-211)
     for (const auto &row : d.rows) {
         if (!(row.height > 0.0) || !(row.pitch > 0.0)) continue;
         smallestSite = std::min(smallestSite, row.pitch * row.height);
-        rows.push_back({{row.coordinate, row.coordinate + row.height}, {{row.x0, row.x1}}});
+        if (!rows.empty() && rows.back().y.start == row.coordinate && rows.back().y.end == row.coordinate + row.height)
+            rows.back().spans.push_back({row.x0, row.x1});
+        else
+            rows.push_back({{row.coordinate, row.coordinate + row.height}, {{row.x0, row.x1}}});
     }
     d.freeSpace.build(std::move(rows), d.obstacles);
     for (const auto &row : d.freeSpace.rows)
