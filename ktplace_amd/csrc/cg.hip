@@ -84,10 +84,11 @@ __device__ __forceinline__ void blockSum(double (&v)[K]) {
     for (int k = 0; k < K; ++k) v[k] = total[k];
 }
 
-// Every block of an axis computes the same sum of the axis' tile partials.
+// Every block of an axis computes the same sum of the axis' tile partials:
+// loadPartials (each thread's share, in tile order) then blockSum.
 template <int K>
-__device__ __forceinline__ void sumPartials(const double *part, uint32_t blocks, uint32_t stride, int axis,
-                                            double (&out)[K]) {
+__device__ __forceinline__ void loadPartials(const double *part, uint32_t blocks, uint32_t stride, int axis,
+                                             double (&out)[K]) {
     const double *src = part + static_cast<size_t>(axis) * stride * K;
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = 0.0;
@@ -95,8 +96,41 @@ __device__ __forceinline__ void sumPartials(const double *part, uint32_t blocks,
 #pragma unroll
         for (int k = 0; k < K; ++k) out[k] += src[t * K + k];
     }
+}
+
+template <int K>
+__device__ __forceinline__ void sumPartials(const double *part, uint32_t blocks, uint32_t stride, int axis,
+                                            double (&out)[K]) {
+    loadPartials<K>(part, blocks, stride, axis, out);
     blockSum<K>(out);
 }
+
+// loadPartials split around other loads: issue() loads each thread's first
+// partial unconditionally (tile 0 exists) and consumes nothing, so loads
+// issued after it stay in flight while it is waited for; finish() selects it,
+// adds the rare further shares (more than 1024 tiles) and sums the block.
+template <int K>
+struct PartialShare {
+    const double *src;
+    uint32_t blocks;
+    double first[K];
+    __device__ __forceinline__ void issue(const double *part, uint32_t nblocks, uint32_t stride, int axis) {
+        src = part + static_cast<size_t>(axis) * stride * K;
+        blocks = nblocks;
+        const uint32_t t = threadIdx.x < blocks ? threadIdx.x : 0u;
+#pragma unroll
+        for (int k = 0; k < K; ++k) first[k] = src[t * K + k];
+    }
+    __device__ __forceinline__ void finish(double (&out)[K]) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[k] = threadIdx.x < blocks ? first[k] : 0.0;
+        for (uint32_t t = threadIdx.x + kBlock; t < blocks; t += kBlock) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) out[k] += src[t * K + k];
+        }
+        blockSum<K>(out);
+    }
+};
 
 template <int K>
 __device__ __forceinline__ void writePartial(double *part, uint32_t stride, int axis, uint32_t tile,
@@ -279,7 +313,11 @@ __global__ void __launch_bounds__(kBlock) cgSpmv(CgArgs a, int parity) {
 }
 
 // alpha = rho / p.Ap; x += alpha p; r -= alpha Ap; tile partials of r.r, r.z.
-template <typename T>
+// PER = rows per thread when the tile is PER * 1024 rows: the thread's vector
+// entries are loaded before the tile partials are summed (they do not depend
+// on alpha), so their latency hides behind that sum.  PER = 0: any tile, the
+// entries loaded after the sum.
+template <typename T, int PER>
 __global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a, int parity) {
     int axis;
     uint32_t tile;
@@ -287,36 +325,66 @@ __global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a, int parity) {
     const CgState st = a.ctl[axis].s[parity];
     if (!st.live) return;
     const SysView s = a.sys[axis];
+    T *__restrict__ x = static_cast<T *>(a.e[axis] ? a.e[axis] : static_cast<void *>(a.x64[axis]));
+    T *__restrict__ r = static_cast<T *>(a.r[axis]);
+    const T *__restrict__ p = static_cast<const T *>(a.p[axis]);
+    const T *__restrict__ ap = static_cast<const T *>(a.ap[axis]);
+    const T *__restrict__ d = Mat<T>::diag(s);
+    const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
+    // the partials are loaded first: the wait for them (an in-order count of
+    // outstanding loads) then leaves the vector loads below in flight
+    PartialShare<1> share;
+    share.issue(a.partA, s.blocks, a.stride, axis);
+    __builtin_amdgcn_sched_barrier(0);
+    constexpr int kPre = PER > 0 ? PER : 1;
+    T xv[kPre], pv[kPre], rv[kPre], av[kPre], dv[kPre];
+    if constexpr (PER > 0) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t i = begin + threadIdx.x + k * kBlock;
+            const uint32_t j = i < end ? i : begin;  // unconditional loads; a tile has a row
+            xv[k] = x[j];
+            pv[k] = p[j];
+            rv[k] = r[j];
+            av[k] = ap[j];
+            dv[k] = d[j];
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
     double pap[1];
-    sumPartials<1>(a.partA, s.blocks, a.stride, axis, pap);
+    share.finish(pap);
     if (!(pap[0] > 0.0)) {
         if (tile == 0 && threadIdx.x == 0) a.ctl[axis].brk[parity] = 1;
         return;
     }
     if (tile == 0 && threadIdx.x == 0) a.ctl[axis].brk[parity] = 0;
     const double alpha = st.rho / pap[0];
-    T *__restrict__ x = static_cast<T *>(a.e[axis] ? a.e[axis] : static_cast<void *>(a.x64[axis]));
-    T *__restrict__ r = static_cast<T *>(a.r[axis]);
-    const T *__restrict__ p = static_cast<const T *>(a.p[axis]);
-    const T *__restrict__ ap = static_cast<const T *>(a.ap[axis]);
-    const T *__restrict__ d = Mat<T>::diag(s);
     const T alphaT = static_cast<T>(alpha);
-    const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
     double acc[2] = {0.0, 0.0};
-    for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
-        x[i] += alphaT * p[i];
-        const T ri = r[i] - alphaT * ap[i];
-        r[i] = ri;
-        const double rd = static_cast<double>(ri);
+    const auto step = [&](uint32_t i, T xi, T pi, T ri, T ai, T di) {
+        x[i] = xi + alphaT * pi;
+        const T rn = ri - alphaT * ai;
+        r[i] = rn;
+        const double rd = static_cast<double>(rn);
         acc[0] += rd * rd;
-        acc[1] += rd * static_cast<double>(ri / d[i]);
+        acc[1] += rd * static_cast<double>(rn / di);
+    };
+    if constexpr (PER > 0) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t i = begin + threadIdx.x + k * kBlock;
+            if (i < end) step(i, xv[k], pv[k], rv[k], av[k], dv[k]);
+        }
+    } else {
+        for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) step(i, x[i], p[i], r[i], ap[i], d[i]);
     }
     blockSum<2>(acc);
     writePartial<2>(a.partB, a.stride, axis, tile, acc);
 }
 
 // beta = r.z' / r.z; p = z + beta p; block 0 publishes state[parity ^ 1].
-template <typename T>
+// PER as in cgUpdate: r, d and p are loaded before the tile partials are summed.
+template <typename T, int PER>
 __global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
     int axis;
     uint32_t tile;
@@ -341,18 +409,42 @@ __global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
         }
         return;
     }
+    T *__restrict__ p = static_cast<T *>(a.p[axis]);
+    const T *__restrict__ r = static_cast<const T *>(a.r[axis]);
+    const T *__restrict__ d = Mat<T>::diag(s);
+    const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
+    PartialShare<2> share;
+    share.issue(a.partB, s.blocks, a.stride, axis);
+    __builtin_amdgcn_sched_barrier(0);
+    constexpr int kPre = PER > 0 ? PER : 1;
+    T rv[kPre], dv[kPre], pv[kPre];
+    if constexpr (PER > 0) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t i = begin + threadIdx.x + k * kBlock;
+            const uint32_t j = i < end ? i : begin;
+            rv[k] = r[j];
+            dv[k] = d[j];
+            pv[k] = p[j];
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
     double tot[2];
-    sumPartials<2>(a.partB, s.blocks, a.stride, axis, tot);
+    share.finish(tot);
     const double resid = sqrt(tot[0]) / c.bNorm;
     const double rhoNext = tot[1];
     const bool ok = rhoNext > 0.0;
     if (ok) {
         const T beta = static_cast<T>(rhoNext / st.rho);
-        T *__restrict__ p = static_cast<T *>(a.p[axis]);
-        const T *__restrict__ r = static_cast<const T *>(a.r[axis]);
-        const T *__restrict__ d = Mat<T>::diag(s);
-        const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
-        for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) p[i] = r[i] / d[i] + beta * p[i];
+        if constexpr (PER > 0) {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const uint32_t i = begin + threadIdx.x + k * kBlock;
+                if (i < end) p[i] = rv[k] / dv[k] + beta * pv[k];
+            }
+        } else {
+            for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) p[i] = r[i] / d[i] + beta * p[i];
+        }
     }
     if (lead) {
         // mixed precision: ask for an FP64 refresh once the inner residual fell
@@ -577,11 +669,21 @@ int launchIteration(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t
     KT_TRY(launchSpmv<T>(ctx, a, G, parity, st, false));
     if (timed) KT_TRY(timedEnd(ctx, kSpmv, t0));
     if (timed) KT_TRY(timedBegin(ctx, KT_KERNEL_UPDATE, &t0));
-    cgUpdate<T><<<grid, dim3(kBlock), 0, st>>>(a, parity);
+    switch (a.tile) {
+        case 1 * kBlock: cgUpdate<T, 1><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
+        case 2 * kBlock: cgUpdate<T, 2><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
+        case 4 * kBlock: cgUpdate<T, 4><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
+        default: cgUpdate<T, 0><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
+    }
     KT_HIP(hipGetLastError());
     if (timed) KT_TRY(timedEnd(ctx, KT_KERNEL_UPDATE, t0));
     if (timed) KT_TRY(timedBegin(ctx, KT_KERNEL_PUPDATE, &t0));
-    cgPUpdate<T><<<grid, dim3(kBlock), 0, st>>>(a, parity);
+    switch (a.tile) {
+        case 1 * kBlock: cgPUpdate<T, 1><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
+        case 2 * kBlock: cgPUpdate<T, 2><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
+        case 4 * kBlock: cgPUpdate<T, 4><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
+        default: cgPUpdate<T, 0><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
+    }
     KT_HIP(hipGetLastError());
     if (timed) KT_TRY(timedEnd(ctx, KT_KERNEL_PUPDATE, t0));
     return KT_OK;
