@@ -173,13 +173,22 @@ void buildGrid(Grid &g, FreeSpace &fs, const std::vector<RowSpec> &rows, const s
 void AreaTables::buildAvail(const Grid &g) {
     nx = g.nx;
     ny = g.ny;
-    // free area in units of 2^-40: exact integer sums (modulo 2^64, which
-    // cancels in the rectangle difference); per-bin rounding below 1e-12
+    // free area in units of 2^-availShift: exact integer sums (modulo 2^64,
+    // which cancels in the rectangle difference).  A rectangle's sum must
+    // stay below 2^63, so the shift is 40 (per-bin rounding below 1e-12)
+    // unless the whole die's free area times 2^40 reaches 2^62; then it is
+    // the largest shift that keeps the total below 2^62 (a die of 6.5e7 area
+    // units takes 36, per-bin rounding below 1e-11).
+    double total = 0.0;
+    for (const double a : g.avail) total += a;
+    availShift = 40;
+    while (availShift > 0 && std::ldexp(total, availShift) >= 0x1p62) --availShift;
     avail.assign((nx + 1) * (ny + 1), 0);
     for (size_t iy = 0; iy < ny; ++iy)
         for (size_t ix = 0; ix < nx; ++ix) {
             const size_t o = (iy + 1) * (nx + 1) + ix + 1;
-            const uint64_t v = static_cast<uint64_t>(std::llround(std::ldexp(g.avail[g.at(ix, iy)], 40)));
+            const uint64_t v =
+                static_cast<uint64_t>(std::llround(std::ldexp(g.avail[g.at(ix, iy)], availShift)));
             avail[o] = v + avail[o - 1] + avail[o - (nx + 1)] - avail[o - (nx + 1) - 1];
         }
     occRow.assign(ny * (nx + 1), 0);
@@ -211,7 +220,7 @@ double AreaTables::sumOcc(const BinRect &r) const {
     return static_cast<double>(t) / 65536.0;
 }
 double AreaTables::sumAvail(const BinRect &r) const {
-    return std::ldexp(static_cast<double>(static_cast<int64_t>(rect(avail, nx, r))), -40);
+    return std::ldexp(static_cast<double>(static_cast<int64_t>(rect(avail, nx, r))), -availShift);
 }
 
 // simpl_lal.cc:86-93

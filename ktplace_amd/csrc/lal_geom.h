@@ -66,13 +66,15 @@ void buildGrid(Grid &g, FreeSpace &fs, const std::vector<RowSpec> &rows, const s
                const Box &die, size_t perAxis);
 
 // Rectangle sums for cluster growth, both exact integer sums.  Free area: a
-// summed-area table in units of 2^-40 (the grid's free area does not change;
+// summed-area table in units of 2^-40, or coarser on a die whose free area
+// would not fit 2^62 of those (the grid's free area does not change;
 // a double summed-area table would carry rounding that grows along the
 // diagonal).  Occupancy: per-row prefix sums in units of 2^-16, patched row
 // by row when cells move.
 struct AreaTables {
     size_t nx = 0, ny = 0;
-    std::vector<uint64_t> avail;    // (nx+1) x (ny+1), 2^-40 units
+    std::vector<uint64_t> avail;    // (nx+1) x (ny+1), units of 2^-availShift
+    int availShift = 40;            // 40, or less where the die's free area needs it
     std::vector<int64_t> occRow;    // ny x (nx+1)
     void buildAvail(const Grid &g);
     void setOcc(const std::vector<uint64_t> &occFix);

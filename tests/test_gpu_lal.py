@@ -133,3 +133,34 @@ def test_full_loop_fast_solver_hpwl(gpu_ctx_factory):
     assert res.stats["overflow_final"] < 0.5
     ctx.close()
 
+
+
+def scaled(d, k):
+    """d with every coordinate, size, offset and row scaled by k (areas by k^2)."""
+    r = d.raw
+    raw = dz.Raw(is_fixed=r.is_fixed.copy(), pos_x=r.pos_x * k, pos_y=r.pos_y * k, width=r.width * k,
+                 height=r.height * k, net_start=r.net_start.copy(), pin_vertex=r.pin_vertex.copy(),
+                 pin_off_x=r.pin_off_x * k, pin_off_y=r.pin_off_y * k, net_weight=r.net_weight.copy(),
+                 row_y=r.row_y * k, row_height=r.row_height * k, row_x0=r.row_x0 * k, row_x1=r.row_x1 * k,
+                 row_pitch=r.row_pitch * k, die=r.die * k)
+    return dz.collect(raw, f"{d.name}-x{k}")
+
+
+def test_lal_matches_oracle_on_a_large_die(gpu_ctx_factory):
+    """A die whose free area times 2^40 passes 2^63 (the 12.5K-cell design
+    scaled by 64: 7.3e7 area units, like the ISPD 2005 adaptec1's 6.5e7): the
+    exact free-area sums of cluster growth must not overflow, and LAL must
+    still reproduce the oracle cell for cell."""
+    d = scaled(dz.workload("cells12k"), 64.0)
+    o = ol.Oracle.of(d)
+    ctx = gpu_ctx_factory(d)
+    ctx.build_grid(0, 1.0)
+    o.build_grid(0, 1.0)
+    assert ctx.grid_avail()[1] * 2.0 ** 40 > 2.0 ** 63
+    for spread in (0.08, 0.2):
+        x, y = clumped(d, 3, spread)
+        gx, gy = ctx.lal(x, y)
+        rx, ry = o.lal(x, y)
+        print(f"spread {spread}: identical {np.mean((gx == rx) & (gy == ry)):.6f}", flush=True)
+        assert np.array_equal(gx, rx) and np.array_equal(gy, ry)
+    ctx.close()
