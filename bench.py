@@ -90,6 +90,7 @@ def main() -> int:
     ap.add_argument("--xcd-split", type=int, default=1, help="with --lanes/--tile: 1 axis per XCD half")
     ap.add_argument("--rows", type=int, default=2, help="with --lanes/--tile: rows in flight per lane group")
     ap.add_argument("--halo", type=int, default=0, help="with --lanes/--tile: SpMV LDS window halo (entries)")
+    ap.add_argument("--layout", type=int, default=0, help="with --lanes/--tile: 1 = entry-block SpMV")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -133,19 +134,20 @@ def main() -> int:
         ctx.restore_positions()
         ctx.build_b2b_resident(0.0)
         if args.lanes and args.tile:
-            tuned = TuneResult(args.lanes, args.tile, float("nan"), [], args.xcd_split, args.rows, args.halo)
+            tuned = TuneResult(args.lanes, args.tile, float("nan"), [], args.xcd_split, args.rows, args.halo,
+                               args.layout)
         elif prec in tuned_by_prec:
             tuned = tuned_by_prec[prec]
         else:
             tuned = autotune(ctx, prec)
             tuned_by_prec[prec] = tuned
         ctx.set_tuning(lanes=tuned.lanes, tile=tuned.tile, xcd_split=tuned.xcd_split, rows=tuned.rows,
-                       halo=tuned.halo)
+                       halo=tuned.halo, layout=tuned.layout)
         log(f"[bench] {path}: autotune lanes={tuned.lanes} tile={tuned.tile} xcd_split={tuned.xcd_split} "
-            f"rows={tuned.rows} halo={tuned.halo} "
+            f"rows={tuned.rows} halo={tuned.halo} layout={tuned.layout} "
             f"({tuned.ms_per_iteration:.3f} ms/iteration)")
         for row in sorted(tuned.table, key=lambda t: t[-1])[:6]:
-            log(f"[bench]   candidate lanes/tile/xcd/rows/halo {row[:-1]}: {row[-1]:.4f} ms/iteration")
+            log(f"[bench]   candidate lanes/tile/xcd/rows/halo/layout {row[:-1]}: {row[-1]:.4f} ms/iteration")
 
         def step(profile: bool, prec=prec, warm=warm) -> dict:
             ctx.restore_positions()
@@ -194,12 +196,14 @@ def main() -> int:
             warm_start_ms=(round(warm_ms / warm_n, 3) if warm_n else None),
             mlp_inference_ms=(round(mlp_ms / mlp_n, 3) if mlp_n else None),
             tuning={"lanes_per_row": tuned.lanes, "tile_rows": tuned.tile,
-                    "xcd_split": tuned.xcd_split, "rows_in_flight": tuned.rows, "lds_halo": tuned.halo},
+                    "xcd_split": tuned.xcd_split, "rows_in_flight": tuned.rows, "lds_halo": tuned.halo,
+                    "spmv_layout": "entry blocks" if tuned.layout else "row groups"},
             roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                           frac=round(achieved / HBM_PEAK_GBS, 4),
                           traffic=(MEASURED_TRAFFIC.get(base_path) if args.workload == "cells1m" else None),
                           traffic_source="rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, cells1m, copy-probe scaled",
-                          kernel=f"cgSpmv<{'double' if base_path == 'fp64' else 'float'}> (x and y per launch)",
+                          kernel=(f"{'cgSpmvFlat' if tuned.layout else 'cgSpmv'}"
+                                  f"<{'double' if base_path == 'fp64' else 'float'}> (x and y per launch)"),
                           launches=launches, avg_launch_us=round(avg_ms * 1e3, 2),
                           bytes_per_launch=int(bytes_per_launch)))
 

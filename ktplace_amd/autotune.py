@@ -1,9 +1,9 @@
 """Launch-shape autotuner for the two-axis Jacobi-PCG kernels.
 
-Chooses the SpMV lanes per row (threads cooperating on one CSR row: 1, 2, 4,
-8 or 16), the tile (rows per 1024-thread block), rows in flight, the XCD
-split and the LDS window (`halo`: entries of p staged on each side of a
-tile) by timing a fixed number of CG iterations of the context's current
+Chooses the SpMV layout (lane groups per row, or entry blocks), the lanes
+per row (threads cooperating on one CSR row: 1, 2, 4, 8 or 16), the tile
+(rows per 1024-thread block), rows in flight, the XCD split and the LDS
+window (`halo`: entries of p staged on each side of a tile) by timing a fixed number of CG iterations of the context's current
 systems on the GPU, from the same saved start each time.  First lanes,
 tile, rows and XCD split without staging; then, with the best rows and XCD
 split, every lanes, tile and halo whose window fits the LDS budget.  The search
@@ -37,6 +37,7 @@ class TuneResult:
     xcd_split: int = 1
     rows: int = 2
     halo: int = 0
+    layout: int = 0
 
 
 def _signature(ctx: "simpl.Context", precision: int):
@@ -62,14 +63,15 @@ def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iteratio
     if not force and key in _cache:
         best = _cache[key]
         ctx.set_tuning(lanes=best.lanes, tile=best.tile, use_graphs=use_graphs,
-                       xcd_split=best.xcd_split, rows=best.rows, halo=best.halo)
+                       xcd_split=best.xcd_split, rows=best.rows, halo=best.halo, layout=best.layout)
         return best
     rows = key[0]
     ctx.save_positions()
     table = []
 
-    def timed(g, tile, xs, rr, halo):
-        ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr, halo=halo)
+    def timed(g, tile, xs, rr, halo, layout=0):
+        ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr, halo=halo,
+                       layout=layout)
         ctx.restore_positions()
         ctx.solve_resident(0.0, 2, precision=precision)  # graph capture, first touch
         ctx.restore_positions()
@@ -79,20 +81,27 @@ def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iteratio
         ctx.sync()
         dt = time.perf_counter() - t0
         its = max(r["iterations"], 1)
-        table.append((g, tile, xs, rr, halo, 1e3 * dt / its))
+        table.append((g, tile, xs, rr, halo, layout, 1e3 * dt / its))
 
     small = lambda t: not (t > 4 * max(rows, 1024) and t != tiles[0])
     for g, tile, xs, rr in [(g, t, x, rr) for t in tiles for g in lanes for x in XCD for rr in ROWS if small(t)]:
         timed(g, tile, xs, rr, 0)
-    _, _, xs, rr, _, _ = min(table, key=lambda t: t[-1])
+    _, _, xs, rr, _, _, _ = min(table, key=lambda t: t[-1])
     for tile in HALO_TILES:
         for halo in halos:
             if window_fits(tile, halo, precision) and halo < rows and small(tile):
                 for g in lanes:
                     timed(g, tile, xs, rr, halo)
+    # entry-block SpMV (lanes, rows in flight and halo do not apply; the tile
+    # and XCD split shape the vector kernels)
+    for tile in tiles:
+        for x in XCD:
+            if small(tile):
+                timed(4, tile, x, 2, 0, simpl.KT_SPMV_ENTRY_BLOCKS)
     ctx.restore_positions()
-    g, tile, xs, rr, halo, ms = min(table, key=lambda t: t[-1])
-    best = TuneResult(lanes=g, tile=tile, ms_per_iteration=ms, table=table, xcd_split=xs, rows=rr, halo=halo)
+    g, tile, xs, rr, halo, layout, ms = min(table, key=lambda t: t[-1])
+    best = TuneResult(lanes=g, tile=tile, ms_per_iteration=ms, table=table, xcd_split=xs, rows=rr, halo=halo,
+                      layout=layout)
     _cache[key] = best
-    ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr, halo=halo)
+    ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr, halo=halo, layout=layout)
     return best

@@ -767,6 +767,14 @@ int kt_set_spmv_halo(kt_ctx *ctx, uint32_t halo) {
     return KT_OK;
 }
 
+int kt_set_spmv_layout(kt_ctx *ctx, int layout) {
+    KT_GUARD(ctx);
+    KT_CHECK(layout == KT_SPMV_ROW_GROUPS || layout == KT_SPMV_ENTRY_BLOCKS, KT_EINVAL,
+             "kt_set_spmv_layout: unknown layout");
+    ctx->tune.layout = layout;
+    return KT_OK;
+}
+
 int kt_kernel_times(kt_ctx *ctx, int kernel, uint64_t *launches, double *total_ms) {
     KT_GUARD(ctx);
     KT_CHECK(kernel >= 0 && kernel < KT_KERNEL_COUNT, KT_EINVAL, "unknown kernel id");

@@ -312,6 +312,120 @@ __global__ void __launch_bounds__(kBlock) cgSpmv(CgArgs a, int parity) {
     writePartial<1>(a.partA, a.stride, axis, tile, acc);
 }
 
+// ---------------------------------------------------------------- entry-block SpMV
+// Block b of an axis takes the rows [rowStart[b], rowStart[b+1]), whose
+// entries are a run of at most kFlatEntries + the longest row (rowStart cuts
+// the entry sequence every kFlatEntries entries at the next row start).
+//   phase 1: thread t takes entries t, t + 1024, ... of the run, so each load
+//            instruction of a wave reads 64 consecutive columns and values;
+//            it gathers p and leaves the product (in double: exact for FP32
+//            operands) in LDS;
+//   phase 2: thread t takes rows R0 + t, R0 + t + 1024, ...: Ap = d p, then
+//            the row's products added in entry order (SparseSystem::multiply's
+//            order, simpl_lss.h:49-57), and the row's share of p.Ap.
+// A run longer than the LDS holds (a row of more than kFlatSlack entries
+// crossing a cut) is done with the lane-group rows instead.
+constexpr uint32_t kFlatEntries = 8192;
+constexpr uint32_t kFlatSlack = 512;
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) cgSpmvFlat(CgArgs a, int parity) {
+    __shared__ double prod[kFlatEntries + kFlatSlack];
+    const int axis = blockIdx.x >= a.sys[0].sblocks ? 1 : 0;
+    const uint32_t b = blockIdx.x - (axis ? a.sys[0].sblocks : 0);
+    if (!a.ctl[axis].s[parity].live) return;
+    const SysView s = a.sys[axis];
+    const T *__restrict__ p = static_cast<const T *>(a.p[axis]);
+    T *__restrict__ ap = static_cast<T *>(a.ap[axis]);
+    const T *__restrict__ val = Mat<T>::val(s);
+    const T *__restrict__ diag = Mat<T>::diag(s);
+    const uint32_t R0 = s.rowStart[b], R1 = s.rowStart[b + 1];
+    const uint32_t E0 = s.rowPtr[R0], count = s.rowPtr[R1] - E0;
+    double acc[1] = {0.0};
+    if (count > kFlatEntries + kFlatSlack) {
+        constexpr int G = 4, R = 2;
+        const uint32_t lane = threadIdx.x % G, grp = threadIdx.x / G;
+        for (uint32_t r0 = R0; r0 < R1; r0 += R * (kBlock / G)) {
+            uint32_t row[R];
+            bool valid[R];
+            double off[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                row[q] = r0 + grp + q * (kBlock / G);
+                valid[q] = row[q] < R1;
+            }
+            rowsDot<T, G, R>(s.rowPtr, s.col, val, p, row, valid, lane, off);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                if (valid[q] && lane == 0) {
+                    const double pd = static_cast<double>(p[row[q]]);
+                    const T v = static_cast<T>(static_cast<double>(diag[row[q]]) * pd + off[q]);
+                    ap[row[q]] = v;
+                    acc[0] += pd * static_cast<double>(v);
+                }
+            }
+        }
+    } else {
+        constexpr int U = 4;
+        for (uint32_t base = 0; base < count; base += U * kBlock) {
+            uint32_t c[U];
+            T w[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t k = base + u * kBlock + threadIdx.x;
+                const bool ok = k < count;
+                const uint32_t kk = E0 + (ok ? k : 0u);  // count > 0 here: entry E0 exists
+                const uint32_t cl = s.col[kk];
+                const T wl = val[kk];
+                c[u] = ok ? cl : kNoCol;
+                w[u] = ok ? wl : T(0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            T g[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) g[u] = gatherAt(p, c[u]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t k = base + u * kBlock + threadIdx.x;
+                if (k < count) prod[k] = static_cast<double>(w[u]) * static_cast<double>(g[u]);
+            }
+        }
+        __syncthreads();
+        for (uint32_t r = R0 + threadIdx.x; r < R1; r += kBlock) {
+            const uint32_t e0 = s.rowPtr[r] - E0, e1 = s.rowPtr[r + 1] - E0;
+            const double pd = static_cast<double>(p[r]);
+            double sum = static_cast<double>(diag[r]) * pd;
+            for (uint32_t k = e0; k < e1; ++k) sum += prod[k];
+            const T v = static_cast<T>(sum);
+            ap[r] = v;
+            acc[0] += pd * static_cast<double>(v);
+        }
+    }
+    blockSum<1>(acc);
+    writePartial<1>(a.partA, a.stride, axis, b, acc);
+}
+
+// rowStart[b] = first row whose entries start at or after b * kFlatEntries
+// (b = 1 .. blocks - 1); rowStart[0] = 0, rowStart[blocks] = rows.
+__global__ void __launch_bounds__(256) flatBounds(const uint32_t *__restrict__ rowPtr, uint32_t rows,
+                                                  uint32_t blocks, uint32_t *__restrict__ rowStart) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b > blocks) return;
+    if (b == 0 || b == blocks) {
+        rowStart[b] = b == 0 ? 0u : rows;
+        return;
+    }
+    const uint64_t target = static_cast<uint64_t>(b) * kFlatEntries;
+    uint32_t lo = 0, hi = rows;  // first r in [0, rows] with rowPtr[r] >= target
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (rowPtr[mid] < target) lo = mid + 1;
+        else hi = mid;
+    }
+    rowStart[b] = lo;
+}
+
 // alpha = rho / p.Ap; x += alpha p; r -= alpha Ap; tile partials of r.r, r.z.
 // PER = rows per thread when the tile is PER * 1024 rows: the thread's vector
 // entries are loaded before the tile partials are summed (they do not depend
@@ -334,7 +448,7 @@ __global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a, int parity) {
     // the partials are loaded first: the wait for them (an in-order count of
     // outstanding loads) then leaves the vector loads below in flight
     PartialShare<1> share;
-    share.issue(a.partA, s.blocks, a.stride, axis);
+    share.issue(a.partA, s.sblocks, a.stride, axis);
     __builtin_amdgcn_sched_barrier(0);
     constexpr int kPre = PER > 0 ? PER : 1;
     T xv[kPre], pv[kPre], rv[kPre], av[kPre], dv[kPre];
@@ -629,6 +743,12 @@ uint32_t gridBlocks(const CgArgs &a) {
 // the instance's dynamic LDS limit to its window (done before any capture).
 template <typename T>
 int launchSpmv(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, bool attributesOnly) {
+    if (a.layout == 1) {
+        if (attributesOnly) return KT_OK;
+        cgSpmvFlat<T><<<dim3(a.sys[0].sblocks + a.sys[1].sblocks), dim3(kBlock), 0, st>>>(a, parity);
+        KT_HIP(hipGetLastError());
+        return KT_OK;
+    }
     const int R = ctx->tune.rows > 0 ? ctx->tune.rows : 2;
     const dim3 grid(gridBlocks(a));
     const bool staged = a.halo > 0;
@@ -752,6 +872,8 @@ SysView viewOf(kt_ctx *ctx, int axis, uint32_t tile) {
     v.diag32 = s.diag32.p;
     v.rows = static_cast<uint32_t>(s.rows);
     v.blocks = static_cast<uint32_t>((s.rows + tile - 1) / tile);
+    v.rowStart = nullptr;
+    v.sblocks = v.blocks;
     return v;
 }
 
@@ -846,6 +968,8 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
     a.tile = tile;
     a.xcdSplit = ctx->tune.xcdSplit == 0 ? 0u : 1u;
     a.halo = ctx->tune.halo;
+    a.layout = ctx->tune.layout == KT_SPMV_ENTRY_BLOCKS ? 1u : 0u;
+    KT_CHECK(!(a.layout && a.halo), KT_EINVAL, "solve: the entry-block SpMV has no LDS window");
     if (a.halo) {
         const uint64_t bytes = fp32 ? windowBytes<float>(tile, a.halo) : windowBytes<double>(tile, a.halo);
         KT_CHECK(bytes <= kMaxWindowBytes, KT_EINVAL,
@@ -886,7 +1010,17 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
             a.e[ax] = nullptr;
         }
         a.sys[ax] = viewOf(ctx, ax, tile);
-        maxBlocks = std::max(maxBlocks, a.sys[ax].blocks);
+        if (a.layout == 1 && rows > 0) {
+            Sys &sy = ctx->sys[ax];
+            const uint32_t sb = static_cast<uint32_t>(std::max<uint64_t>((nnz[ax] + kFlatEntries - 1) / kFlatEntries, 1));
+            KT_TRY(sy.rowStart.reserve(sb + 1));
+            flatBounds<<<dim3((sb + 1 + 255) / 256), dim3(256), 0, ctx->stream>>>(sy.rowPtr.p, static_cast<uint32_t>(rows),
+                                                                               sb, sy.rowStart.p);
+            KT_HIP(hipGetLastError());
+            a.sys[ax].rowStart = sy.rowStart.p;
+            a.sys[ax].sblocks = sb;
+        }
+        maxBlocks = std::max({maxBlocks, a.sys[ax].blocks, a.sys[ax].sblocks});
     }
     a.x64[0] = ctx->posX.p;
     a.x64[1] = ctx->posY.p;

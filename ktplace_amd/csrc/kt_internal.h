@@ -108,6 +108,9 @@ struct SysView {
     const float *diag32;
     uint32_t rows;
     uint32_t blocks;  // blocks this axis occupies in a two-axis launch
+    // entry-block SpMV layout: block b takes rows [rowStart[b], rowStart[b+1])
+    const uint32_t *rowStart;
+    uint32_t sblocks;  // SpMV blocks (and p.Ap partials) of this axis
 };
 
 struct Sys {
@@ -118,6 +121,7 @@ struct Sys {
     DevBuf<double> val, diag, rhs;
     DevBuf<float> val32, diag32;
     bool have32 = false;
+    DevBuf<uint32_t> rowStart;  // entry-block SpMV: first row of each block (+ rows)
 };
 
 // Arguments of every CG kernel (both axes).
@@ -138,7 +142,7 @@ struct CgArgs {
     uint32_t tile;      // rows per block
     uint32_t xcdSplit;  // 1: blocks on XCDs 0-3 take axis x, 4-7 axis y
     uint32_t halo;      // SpMV: entries of p staged in LDS on each side of a tile (0: none)
-    uint32_t pad;
+    uint32_t layout;    // SpMV: 0 lane groups per row, 1 entry blocks (KT_SPMV_ENTRY_BLOCKS)
 };
 
 // Launch-shape knobs (set by the autotuner or the caller; 0 = default).
@@ -150,6 +154,7 @@ struct Tune {
     int xcdSplit = -1;        // -1 default (on), 0 off, 1 on
     double refreshFactor = 0; // mixed precision refresh factor (default 0.05)
     uint32_t halo = 0;        // SpMV LDS window: entries on each side of a tile (0: no staging)
+    int layout = 0;           // SpMV layout (KT_SPMV_ROW_GROUPS / KT_SPMV_ENTRY_BLOCKS)
 };
 
 // A captured hipGraph of kPairs iteration pairs for one argument set.

@@ -331,3 +331,64 @@ def test_tolerance_below_fp32_accuracy_on_ragged_system(gpu_ctx_factory, precisi
     for sol, s in ((outs[0][0], sx), (outs[0][1], base)):
         assert rel(sol, spsolve(s.scipy().tocsc(), s.rhs)) < 1e-8
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_entry_block_layout_solves_and_repeats(b2b12k, gpu_ctx_factory, precision):
+    """The entry-block SpMV: same answer as the oracle solved tight, iteration
+    count within 1 of the lane-group layout's, bit-identical on repeat."""
+    from ktplace_amd.simpl import KT_PREC_FP64, KT_PREC_MIXED, KT_SPMV_ENTRY_BLOCKS, KT_STOP_TRUE
+    prec = KT_PREC_FP64 if precision == "fp64" else KT_PREC_MIXED
+    d, o, x0, y0 = b2b12k
+    ctx = gpu_ctx_factory(d)
+    ctx.build_b2b(x0, y0)
+    ref = o.solve(x0, y0, tol=1e-12, max_iter=100000)
+    runs = []
+    for layout in (0, KT_SPMV_ENTRY_BLOCKS, KT_SPMV_ENTRY_BLOCKS):
+        ctx.set_tuning(lanes=4, tile=2048, layout=layout)
+        x, y = x0.copy(), y0.copy()
+        r = ctx.solve_both(x, y, 1e-9, 100000, precision=prec, stop_rule=KT_STOP_TRUE)
+        runs.append((x, y, r))
+    for x, y, r in runs[1:]:
+        assert rel(x, ref["x"]) < 1e-6 and rel(y, ref["y"]) < 1e-6
+        assert max(r["true_resid"]) <= 1e-9
+        assert all(abs(a - b) <= 1 for a, b in zip(r["steps"], runs[0][2]["steps"]))
+    assert np.array_equal(runs[1][0], runs[2][0]) and np.array_equal(runs[1][1], runs[2][1])
+    ctx.close()
+
+
+def test_entry_block_layout_long_rows(gpu_ctx_factory):
+    """Rows longer than an entry block's LDS share (a hub row of 12,000 entries
+    and one of 9,000 crossing the block cuts, done by the lane-group path), empty
+    rows and a last block of a few entries: the solution matches scipy."""
+    from scipy.sparse.linalg import spsolve
+    import scipy.sparse as sp
+    from ktplace_amd.simpl import KT_PREC_FP64, KT_PREC_MIXED, KT_SPMV_ENTRY_BLOCKS, KT_STOP_TRUE
+    rng = np.random.default_rng(17)
+    n = 30001
+    i = np.concatenate([np.zeros(12000, int), np.full(9000, 20000), rng.integers(0, n, 40000)])
+    j = np.concatenate([rng.choice(np.arange(1, n), 12000, replace=False),
+                        rng.choice(np.arange(0, 20000), 9000, replace=False), rng.integers(0, n, 40000)])
+    keep = (i != j) & (i != 7) & (j != 7)          # row 7 stays empty
+    i, j = i[keep], j[keep]
+    w = rng.uniform(0.1, 2.0, len(i))
+    off = sp.coo_matrix((-np.concatenate([w, w]), (np.concatenate([i, j]), np.concatenate([j, i]))),
+                        shape=(n, n)).tocsr()
+    off.sum_duplicates()
+    off.sort_indices()
+    diag = -off.sum(axis=1).A1 + rng.uniform(0.05, 0.5, n)
+    s = ol.System(off.indptr.astype(np.uint64), off.indices.astype(np.uint32), off.data.copy(), diag,
+                  rng.standard_normal(n))
+    lengths = np.diff(off.indptr)
+    assert lengths.max() >= 12000 and lengths[7] == 0
+    exact = spsolve(s.scipy().tocsc(), s.rhs)
+    ctx = gpu_ctx_factory()
+    ctx.set_system(0, s)
+    ctx.set_system(1, s)
+    for prec in (KT_PREC_FP64, KT_PREC_MIXED):
+        ctx.set_tuning(lanes=4, tile=1024, layout=KT_SPMV_ENTRY_BLOCKS)
+        x, y = np.zeros(n), np.zeros(n)
+        r = ctx.solve_both(x, y, 1e-10, 100000, precision=prec, stop_rule=KT_STOP_TRUE)
+        assert max(r["true_resid"]) <= 1e-10
+        assert rel(x, exact) < 1e-8 and np.array_equal(x, y)
+    ctx.close()
