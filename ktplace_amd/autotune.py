@@ -23,6 +23,8 @@ ROWS = (2, 4, 8)
 XCD = (1, 0)
 HALOS = (2048, 4096, 6144, 8192, 12288, 16384)
 HALO_TILES = (1024, 2048, 4096)
+FLAT_ENTRIES = (4096, 8192, 16384)  # entry-block SpMV: entries per block
+FLAT_UNROLL = (4, 8)                # entry-block SpMV: entries per thread and round
 WINDOW_BYTES = 156 * 1024  # LDS a staged SpMV block may take
 
 _cache: dict = {}
@@ -38,6 +40,8 @@ class TuneResult:
     rows: int = 2
     halo: int = 0
     layout: int = 0
+    block_entries: int = 0
+    unroll: int = 0
 
 
 def _signature(ctx: "simpl.Context", precision: int):
@@ -63,15 +67,16 @@ def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iteratio
     if not force and key in _cache:
         best = _cache[key]
         ctx.set_tuning(lanes=best.lanes, tile=best.tile, use_graphs=use_graphs,
-                       xcd_split=best.xcd_split, rows=best.rows, halo=best.halo, layout=best.layout)
+                       xcd_split=best.xcd_split, rows=best.rows, halo=best.halo, layout=best.layout,
+                       block_entries=best.block_entries, unroll=best.unroll)
         return best
     rows = key[0]
     ctx.save_positions()
     table = []
 
-    def timed(g, tile, xs, rr, halo, layout=0):
+    def timed(g, tile, xs, rr, halo, layout=0, entries=0, unroll=0):
         ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr, halo=halo,
-                       layout=layout)
+                       layout=layout, block_entries=entries, unroll=unroll)
         ctx.restore_positions()
         ctx.solve_resident(0.0, 2, precision=precision)  # graph capture, first touch
         ctx.restore_positions()
@@ -81,12 +86,12 @@ def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iteratio
         ctx.sync()
         dt = time.perf_counter() - t0
         its = max(r["iterations"], 1)
-        table.append((g, tile, xs, rr, halo, layout, 1e3 * dt / its))
+        table.append((g, tile, xs, rr, halo, layout, entries, unroll, 1e3 * dt / its))
 
     small = lambda t: not (t > 4 * max(rows, 1024) and t != tiles[0])
     for g, tile, xs, rr in [(g, t, x, rr) for t in tiles for g in lanes for x in XCD for rr in ROWS if small(t)]:
         timed(g, tile, xs, rr, 0)
-    _, _, xs, rr, _, _, _ = min(table, key=lambda t: t[-1])
+    _, _, xs, rr, _, _, _, _, _ = min(table, key=lambda t: t[-1])
     for tile in HALO_TILES:
         for halo in halos:
             if window_fits(tile, halo, precision) and halo < rows and small(tile):
@@ -97,11 +102,14 @@ def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iteratio
     for tile in tiles:
         for x in XCD:
             if small(tile):
-                timed(4, tile, x, 2, 0, simpl.KT_SPMV_ENTRY_BLOCKS)
+                for entries in FLAT_ENTRIES:
+                    for unroll in FLAT_UNROLL:
+                        timed(4, tile, x, 2, 0, simpl.KT_SPMV_ENTRY_BLOCKS, entries, unroll)
     ctx.restore_positions()
-    g, tile, xs, rr, halo, layout, ms = min(table, key=lambda t: t[-1])
+    g, tile, xs, rr, halo, layout, entries, unroll, ms = min(table, key=lambda t: t[-1])
     best = TuneResult(lanes=g, tile=tile, ms_per_iteration=ms, table=table, xcd_split=xs, rows=rr, halo=halo,
-                      layout=layout)
+                      layout=layout, block_entries=entries, unroll=unroll)
     _cache[key] = best
-    ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr, halo=halo, layout=layout)
+    ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr, halo=halo, layout=layout,
+                   block_entries=entries, unroll=unroll)
     return best

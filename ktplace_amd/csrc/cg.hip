@@ -325,11 +325,12 @@ __global__ void __launch_bounds__(kBlock) cgSpmv(CgArgs a, int parity) {
 //            order, simpl_lss.h:49-57), and the row's share of p.Ap.
 // A run longer than the LDS holds (a row of more than kFlatSlack entries
 // crossing a cut) is done with the lane-group rows instead.
-constexpr uint32_t kFlatEntries = 8192;
 constexpr uint32_t kFlatSlack = 512;
 
-template <typename T>
+// E: entries per block (the cut spacing); U: entries per thread and round.
+template <typename T, uint32_t E, int U>
 __global__ void __launch_bounds__(kBlock) cgSpmvFlat(CgArgs a, int parity) {
+    constexpr uint32_t kFlatEntries = E;
     __shared__ double prod[kFlatEntries + kFlatSlack];
     const int axis = blockIdx.x >= a.sys[0].sblocks ? 1 : 0;
     const uint32_t b = blockIdx.x - (axis ? a.sys[0].sblocks : 0);
@@ -366,7 +367,6 @@ __global__ void __launch_bounds__(kBlock) cgSpmvFlat(CgArgs a, int parity) {
             }
         }
     } else {
-        constexpr int U = 4;
         for (uint32_t base = 0; base < count; base += U * kBlock) {
             uint32_t c[U];
             T w[U];
@@ -406,17 +406,17 @@ __global__ void __launch_bounds__(kBlock) cgSpmvFlat(CgArgs a, int parity) {
     writePartial<1>(a.partA, a.stride, axis, b, acc);
 }
 
-// rowStart[b] = first row whose entries start at or after b * kFlatEntries
+// rowStart[b] = first row whose entries start at or after b * entries
 // (b = 1 .. blocks - 1); rowStart[0] = 0, rowStart[blocks] = rows.
 __global__ void __launch_bounds__(256) flatBounds(const uint32_t *__restrict__ rowPtr, uint32_t rows,
-                                                  uint32_t blocks, uint32_t *__restrict__ rowStart) {
+                                                  uint32_t blocks, uint32_t entries, uint32_t *__restrict__ rowStart) {
     const uint32_t b = blockIdx.x * 256 + threadIdx.x;
     if (b > blocks) return;
     if (b == 0 || b == blocks) {
         rowStart[b] = b == 0 ? 0u : rows;
         return;
     }
-    const uint64_t target = static_cast<uint64_t>(b) * kFlatEntries;
+    const uint64_t target = static_cast<uint64_t>(b) * entries;
     uint32_t lo = 0, hi = rows;  // first r in [0, rows] with rowPtr[r] >= target
     while (lo < hi) {
         const uint32_t mid = lo + (hi - lo) / 2;
@@ -745,7 +745,22 @@ template <typename T>
 int launchSpmv(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, bool attributesOnly) {
     if (a.layout == 1) {
         if (attributesOnly) return KT_OK;
-        cgSpmvFlat<T><<<dim3(a.sys[0].sblocks + a.sys[1].sblocks), dim3(kBlock), 0, st>>>(a, parity);
+        const dim3 fg(a.sys[0].sblocks + a.sys[1].sblocks);
+        const bool wide = ctx->tune.flatUnroll == 8;
+        switch (a.flatEntries) {
+            case 4096:
+                if (wide) cgSpmvFlat<T, 4096, 8><<<fg, dim3(kBlock), 0, st>>>(a, parity);
+                else cgSpmvFlat<T, 4096, 4><<<fg, dim3(kBlock), 0, st>>>(a, parity);
+                break;
+            case 16384:
+                if (wide) cgSpmvFlat<T, 16384, 8><<<fg, dim3(kBlock), 0, st>>>(a, parity);
+                else cgSpmvFlat<T, 16384, 4><<<fg, dim3(kBlock), 0, st>>>(a, parity);
+                break;
+            default:
+                if (wide) cgSpmvFlat<T, 8192, 8><<<fg, dim3(kBlock), 0, st>>>(a, parity);
+                else cgSpmvFlat<T, 8192, 4><<<fg, dim3(kBlock), 0, st>>>(a, parity);
+                break;
+        }
         KT_HIP(hipGetLastError());
         return KT_OK;
     }
@@ -969,6 +984,8 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
     a.xcdSplit = ctx->tune.xcdSplit == 0 ? 0u : 1u;
     a.halo = ctx->tune.halo;
     a.layout = ctx->tune.layout == KT_SPMV_ENTRY_BLOCKS ? 1u : 0u;
+    a.flatEntries = ctx->tune.flatEntries ? ctx->tune.flatEntries : 8192u;
+    a.flatUnroll = ctx->tune.flatUnroll == 8 ? 8u : 4u;
     KT_CHECK(!(a.layout && a.halo), KT_EINVAL, "solve: the entry-block SpMV has no LDS window");
     if (a.halo) {
         const uint64_t bytes = fp32 ? windowBytes<float>(tile, a.halo) : windowBytes<double>(tile, a.halo);
@@ -1012,10 +1029,11 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
         a.sys[ax] = viewOf(ctx, ax, tile);
         if (a.layout == 1 && rows > 0) {
             Sys &sy = ctx->sys[ax];
-            const uint32_t sb = static_cast<uint32_t>(std::max<uint64_t>((nnz[ax] + kFlatEntries - 1) / kFlatEntries, 1));
+            const uint32_t E = a.flatEntries;
+            const uint32_t sb = static_cast<uint32_t>(std::max<uint64_t>((nnz[ax] + E - 1) / E, 1));
             KT_TRY(sy.rowStart.reserve(sb + 1));
             flatBounds<<<dim3((sb + 1 + 255) / 256), dim3(256), 0, ctx->stream>>>(sy.rowPtr.p, static_cast<uint32_t>(rows),
-                                                                               sb, sy.rowStart.p);
+                                                                               sb, E, sy.rowStart.p);
             KT_HIP(hipGetLastError());
             a.sys[ax].rowStart = sy.rowStart.p;
             a.sys[ax].sblocks = sb;

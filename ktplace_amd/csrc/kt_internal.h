@@ -143,6 +143,9 @@ struct CgArgs {
     uint32_t xcdSplit;  // 1: blocks on XCDs 0-3 take axis x, 4-7 axis y
     uint32_t halo;      // SpMV: entries of p staged in LDS on each side of a tile (0: none)
     uint32_t layout;    // SpMV: 0 lane groups per row, 1 entry blocks (KT_SPMV_ENTRY_BLOCKS)
+    uint32_t flatEntries;  // entry blocks: entries per block (4096, 8192, 16384)
+    uint32_t flatUnroll;   // entry blocks: entries per thread and round (4, 8)
+    uint32_t pad;
 };
 
 // Launch-shape knobs (set by the autotuner or the caller; 0 = default).
@@ -155,6 +158,8 @@ struct Tune {
     double refreshFactor = 0; // mixed precision refresh factor (default 0.05)
     uint32_t halo = 0;        // SpMV LDS window: entries on each side of a tile (0: no staging)
     int layout = 0;           // SpMV layout (KT_SPMV_ROW_GROUPS / KT_SPMV_ENTRY_BLOCKS)
+    uint32_t flatEntries = 0; // entry blocks: entries per block (0: 8192)
+    int flatUnroll = 0;       // entry blocks: entries per thread and round (0: 4)
 };
 
 // A captured hipGraph of kPairs iteration pairs for one argument set.

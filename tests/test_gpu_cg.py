@@ -360,7 +360,8 @@ def test_entry_block_layout_solves_and_repeats(b2b12k, gpu_ctx_factory, precisio
 def test_entry_block_layout_long_rows(gpu_ctx_factory):
     """Rows longer than an entry block's LDS share (a hub row of 12,000 entries
     and one of 9,000 crossing the block cuts, done by the lane-group path), empty
-    rows and a last block of a few entries: the solution matches scipy."""
+    rows and a last block of a few entries, for each block size and unroll:
+    the solution matches scipy."""
     from scipy.sparse.linalg import spsolve
     import scipy.sparse as sp
     from ktplace_amd.simpl import KT_PREC_FP64, KT_PREC_MIXED, KT_SPMV_ENTRY_BLOCKS, KT_STOP_TRUE
@@ -385,8 +386,9 @@ def test_entry_block_layout_long_rows(gpu_ctx_factory):
     ctx = gpu_ctx_factory()
     ctx.set_system(0, s)
     ctx.set_system(1, s)
-    for prec in (KT_PREC_FP64, KT_PREC_MIXED):
-        ctx.set_tuning(lanes=4, tile=1024, layout=KT_SPMV_ENTRY_BLOCKS)
+    for prec, entries, unroll in ((KT_PREC_FP64, 0, 0), (KT_PREC_MIXED, 0, 0), (KT_PREC_FP64, 4096, 8),
+                                  (KT_PREC_MIXED, 16384, 8), (KT_PREC_FP64, 16384, 4)):
+        ctx.set_tuning(lanes=4, tile=1024, layout=KT_SPMV_ENTRY_BLOCKS, block_entries=entries, unroll=unroll)
         x, y = np.zeros(n), np.zeros(n)
         r = ctx.solve_both(x, y, 1e-10, 100000, precision=prec, stop_rule=KT_STOP_TRUE)
         assert max(r["true_resid"]) <= 1e-10
