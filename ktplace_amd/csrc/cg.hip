@@ -28,6 +28,7 @@
 //     host polls the state every few iterations.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "kt_internal.h"
@@ -328,12 +329,34 @@ __global__ void __launch_bounds__(kBlock) cgSpmv(CgArgs a, int parity) {
 constexpr uint32_t kFlatSlack = 512;
 
 // E: entries per block (the cut spacing); U: entries per thread and round.
+// Block -> (axis, block of that axis).  With xcdSplit the blocks that land on
+// XCDs 0-3 (workgroups are dealt to the 8 XCDs round-robin) take the x
+// system and those on XCDs 4-7 the y system, so each XCD's L2 sees the
+// gathers of one axis only.
+__device__ __forceinline__ bool decodeFlat(const CgArgs &a, int &axis, uint32_t &b) {
+    if (a.xcdSplit) {
+        const uint32_t id = blockIdx.x, x = id & 7u;
+        axis = static_cast<int>(x >> 2);
+        b = (id >> 3) * 4u + (x & 3u);
+        return b < a.sys[axis].sblocks;
+    }
+    axis = blockIdx.x >= a.sys[0].sblocks ? 1 : 0;
+    b = blockIdx.x - (axis ? a.sys[0].sblocks : 0);
+    return true;
+}
+
+uint32_t flatGrid(const CgArgs &a) {
+    if (a.xcdSplit) return 8u * ((std::max(a.sys[0].sblocks, a.sys[1].sblocks) + 3u) / 4u);
+    return a.sys[0].sblocks + a.sys[1].sblocks;
+}
+
 template <typename T, uint32_t E, int U>
 __global__ void __launch_bounds__(kBlock) cgSpmvFlat(CgArgs a, int parity) {
     constexpr uint32_t kFlatEntries = E;
     __shared__ double prod[kFlatEntries + kFlatSlack];
-    const int axis = blockIdx.x >= a.sys[0].sblocks ? 1 : 0;
-    const uint32_t b = blockIdx.x - (axis ? a.sys[0].sblocks : 0);
+    int axis;
+    uint32_t b;
+    if (!decodeFlat(a, axis, b)) return;
     if (!a.ctl[axis].s[parity].live) return;
     const SysView s = a.sys[axis];
     const T *__restrict__ p = static_cast<const T *>(a.p[axis]);
@@ -745,7 +768,7 @@ template <typename T>
 int launchSpmv(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, bool attributesOnly) {
     if (a.layout == 1) {
         if (attributesOnly) return KT_OK;
-        const dim3 fg(a.sys[0].sblocks + a.sys[1].sblocks);
+        const dim3 fg(flatGrid(a));
         const bool wide = ctx->tune.flatUnroll == 8;
         switch (a.flatEntries) {
             case 4096:
@@ -986,6 +1009,7 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
     a.layout = ctx->tune.layout == KT_SPMV_ENTRY_BLOCKS ? 1u : 0u;
     a.flatEntries = ctx->tune.flatEntries ? ctx->tune.flatEntries : 8192u;
     a.flatUnroll = ctx->tune.flatUnroll == 8 ? 8u : 4u;
+
     KT_CHECK(!(a.layout && a.halo), KT_EINVAL, "solve: the entry-block SpMV has no LDS window");
     if (a.halo) {
         const uint64_t bytes = fp32 ? windowBytes<float>(tile, a.halo) : windowBytes<double>(tile, a.halo);
