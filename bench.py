@@ -93,6 +93,7 @@ def main() -> int:
     ap.add_argument("--layout", type=int, default=0, help="with --lanes/--tile: 1 = entry-block SpMV")
     ap.add_argument("--block-entries", type=int, default=0, help="with --layout 1: entries per block")
     ap.add_argument("--unroll", type=int, default=0, help="with --layout 1: entries per thread and round")
+    ap.add_argument("--row-lanes", type=int, default=0, help="with --layout 1: lanes per row in the row sums")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -137,7 +138,7 @@ def main() -> int:
         ctx.build_b2b_resident(0.0)
         if args.lanes and args.tile:
             tuned = TuneResult(args.lanes, args.tile, float("nan"), [], args.xcd_split, args.rows, args.halo,
-                               args.layout, args.block_entries, args.unroll)
+                               args.layout, args.block_entries, args.unroll, args.row_lanes)
         elif prec in tuned_by_prec:
             tuned = tuned_by_prec[prec]
         else:
@@ -145,13 +146,13 @@ def main() -> int:
             tuned_by_prec[prec] = tuned
         ctx.set_tuning(lanes=tuned.lanes, tile=tuned.tile, xcd_split=tuned.xcd_split, rows=tuned.rows,
                        halo=tuned.halo, layout=tuned.layout, block_entries=tuned.block_entries,
-                       unroll=tuned.unroll)
+                       unroll=tuned.unroll, row_lanes=tuned.row_lanes)
         log(f"[bench] {path}: autotune lanes={tuned.lanes} tile={tuned.tile} xcd_split={tuned.xcd_split} "
             f"rows={tuned.rows} halo={tuned.halo} layout={tuned.layout} "
-            f"block_entries={tuned.block_entries} unroll={tuned.unroll} "
+            f"block_entries={tuned.block_entries} unroll={tuned.unroll} row_lanes={tuned.row_lanes} "
             f"({tuned.ms_per_iteration:.3f} ms/iteration)")
         for row in sorted(tuned.table, key=lambda t: t[-1])[:6]:
-            log(f"[bench]   candidate lanes/tile/xcd/rows/halo/layout/entries/unroll {row[:-1]}: {row[-1]:.4f} ms/iteration")
+            log(f"[bench]   candidate lanes/tile/xcd/rows/halo/layout/entries/unroll/row_lanes {row[:-1]}: {row[-1]:.4f} ms/iteration")
 
         def step(profile: bool, prec=prec, warm=warm) -> dict:
             ctx.restore_positions()
@@ -203,7 +204,8 @@ def main() -> int:
                     "xcd_split": tuned.xcd_split, "rows_in_flight": tuned.rows, "lds_halo": tuned.halo,
                     "spmv_layout": "entry blocks" if tuned.layout else "row groups",
                     "block_entries": tuned.block_entries or (8192 if tuned.layout else None),
-                    "unroll": tuned.unroll or (4 if tuned.layout else None)},
+                    "unroll": tuned.unroll or (4 if tuned.layout else None),
+                    "row_lanes": tuned.row_lanes or (1 if tuned.layout else None)},
             roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                           frac=round(achieved / HBM_PEAK_GBS, 4),
                           traffic=(MEASURED_TRAFFIC.get(base_path) if args.workload == "cells1m" else None),

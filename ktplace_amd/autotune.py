@@ -23,8 +23,10 @@ ROWS = (2, 4, 8)
 XCD = (1, 0)
 HALOS = (2048, 4096, 6144, 8192, 12288, 16384)
 HALO_TILES = (1024, 2048, 4096)
-FLAT_ENTRIES = (4096, 8192, 16384)  # entry-block SpMV: entries per block
-FLAT_UNROLL = (4, 8)                # entry-block SpMV: entries per thread and round
+FLAT_ENTRIES = (4096, 8192)  # entry-block SpMV: entries per block
+FLAT_UNROLL = (4, 8)         # entry-block SpMV: entries per thread and round
+FLAT_ROW_LANES = (1, 4)      # entry-block SpMV: lanes per row in the row sums
+FLAT_TILES = (2048, 4096)    # tiles (of the vector kernels) tried with entry blocks
 WINDOW_BYTES = 156 * 1024  # LDS a staged SpMV block may take
 
 _cache: dict = {}
@@ -42,6 +44,7 @@ class TuneResult:
     layout: int = 0
     block_entries: int = 0
     unroll: int = 0
+    row_lanes: int = 0
 
 
 def _signature(ctx: "simpl.Context", precision: int):
@@ -68,15 +71,15 @@ def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iteratio
         best = _cache[key]
         ctx.set_tuning(lanes=best.lanes, tile=best.tile, use_graphs=use_graphs,
                        xcd_split=best.xcd_split, rows=best.rows, halo=best.halo, layout=best.layout,
-                       block_entries=best.block_entries, unroll=best.unroll)
+                       block_entries=best.block_entries, unroll=best.unroll, row_lanes=best.row_lanes)
         return best
     rows = key[0]
     ctx.save_positions()
     table = []
 
-    def timed(g, tile, xs, rr, halo, layout=0, entries=0, unroll=0):
+    def timed(g, tile, xs, rr, halo, layout=0, entries=0, unroll=0, row_lanes=0):
         ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr, halo=halo,
-                       layout=layout, block_entries=entries, unroll=unroll)
+                       layout=layout, block_entries=entries, unroll=unroll, row_lanes=row_lanes)
         ctx.restore_positions()
         ctx.solve_resident(0.0, 2, precision=precision)  # graph capture, first touch
         ctx.restore_positions()
@@ -86,12 +89,12 @@ def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iteratio
         ctx.sync()
         dt = time.perf_counter() - t0
         its = max(r["iterations"], 1)
-        table.append((g, tile, xs, rr, halo, layout, entries, unroll, 1e3 * dt / its))
+        table.append((g, tile, xs, rr, halo, layout, entries, unroll, row_lanes, 1e3 * dt / its))
 
     small = lambda t: not (t > 4 * max(rows, 1024) and t != tiles[0])
     for g, tile, xs, rr in [(g, t, x, rr) for t in tiles for g in lanes for x in XCD for rr in ROWS if small(t)]:
         timed(g, tile, xs, rr, 0)
-    _, _, xs, rr, _, _, _, _, _ = min(table, key=lambda t: t[-1])
+    _, _, xs, rr, _, _, _, _, _, _ = min(table, key=lambda t: t[-1])
     for tile in HALO_TILES:
         for halo in halos:
             if window_fits(tile, halo, precision) and halo < rows and small(tile):
@@ -99,12 +102,13 @@ def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iteratio
                     timed(g, tile, xs, rr, halo)
     # entry-block SpMV (lanes, rows in flight and halo do not apply; the tile
     # and XCD split shape the vector kernels)
-    for tile in tiles:
+    for tile in FLAT_TILES:
         for x in XCD:
             if small(tile):
                 for entries in FLAT_ENTRIES:
                     for unroll in FLAT_UNROLL:
-                        timed(4, tile, x, 2, 0, simpl.KT_SPMV_ENTRY_BLOCKS, entries, unroll)
+                        for row_lanes in FLAT_ROW_LANES:
+                            timed(4, tile, x, 2, 0, simpl.KT_SPMV_ENTRY_BLOCKS, entries, unroll, row_lanes)
     # the closest candidates again, over twice the iterations, twice each:
     # single short timings of near-equal shapes are noisy
     finalists = sorted(table, key=lambda t: t[-1])[:4]
@@ -113,19 +117,19 @@ def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iteratio
         best_ms = float("inf")
         for _ in range(2):
             ctx.set_tuning(lanes=cand[0], tile=cand[1], use_graphs=use_graphs, xcd_split=cand[2], rows=cand[3],
-                           halo=cand[4], layout=cand[5], block_entries=cand[6], unroll=cand[7])
+                           halo=cand[4], layout=cand[5], block_entries=cand[6], unroll=cand[7], row_lanes=cand[8])
             ctx.restore_positions()
             ctx.sync()
             t0 = time.perf_counter()
             r = ctx.solve_resident(0.0, 2 * iterations, precision=precision, check_every=2 * iterations)
             ctx.sync()
             best_ms = min(best_ms, 1e3 * (time.perf_counter() - t0) / max(r["iterations"], 1))
-        table.append((*cand[:8], best_ms))
+        table.append((*cand[:9], best_ms))
     ctx.restore_positions()
-    g, tile, xs, rr, halo, layout, entries, unroll, ms = min(table, key=lambda t: t[-1])
+    g, tile, xs, rr, halo, layout, entries, unroll, row_lanes, ms = min(table, key=lambda t: t[-1])
     best = TuneResult(lanes=g, tile=tile, ms_per_iteration=ms, table=table, xcd_split=xs, rows=rr, halo=halo,
-                      layout=layout, block_entries=entries, unroll=unroll)
+                      layout=layout, block_entries=entries, unroll=unroll, row_lanes=row_lanes)
     _cache[key] = best
     ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr, halo=halo, layout=layout,
-                   block_entries=entries, unroll=unroll)
+                   block_entries=entries, unroll=unroll, row_lanes=row_lanes)
     return best

@@ -767,13 +767,16 @@ int kt_set_spmv_halo(kt_ctx *ctx, uint32_t halo) {
     return KT_OK;
 }
 
-int kt_set_spmv_layout(kt_ctx *ctx, int layout, uint32_t block_entries, int unroll) {
+int kt_set_spmv_layout(kt_ctx *ctx, int layout, uint32_t block_entries, int unroll, int row_lanes) {
     KT_GUARD(ctx);
     KT_CHECK(layout == KT_SPMV_ROW_GROUPS || layout == KT_SPMV_ENTRY_BLOCKS, KT_EINVAL,
              "kt_set_spmv_layout: unknown layout");
     KT_CHECK(block_entries == 0 || block_entries == 4096 || block_entries == 8192 || block_entries == 16384,
              KT_EINVAL, "kt_set_spmv_layout: block entries must be 0, 4096, 8192 or 16384");
     KT_CHECK(unroll == 0 || unroll == 4 || unroll == 8, KT_EINVAL, "kt_set_spmv_layout: unroll must be 0, 4 or 8");
+    KT_CHECK(row_lanes == 0 || row_lanes == 1 || row_lanes == 4, KT_EINVAL,
+             "kt_set_spmv_layout: row lanes must be 0, 1 or 4");
+    ctx->tune.flatRowLanes = row_lanes;
     ctx->tune.layout = layout;
     ctx->tune.flatEntries = block_entries;
     ctx->tune.flatUnroll = unroll;

@@ -350,7 +350,7 @@ uint32_t flatGrid(const CgArgs &a) {
     return a.sys[0].sblocks + a.sys[1].sblocks;
 }
 
-template <typename T, uint32_t E, int U>
+template <typename T, uint32_t E, int U, int G2>
 __global__ void __launch_bounds__(kBlock) cgSpmvFlat(CgArgs a, int parity) {
     constexpr uint32_t kFlatEntries = E;
     __shared__ double prod[kFlatEntries + kFlatSlack];
@@ -415,14 +415,37 @@ __global__ void __launch_bounds__(kBlock) cgSpmvFlat(CgArgs a, int parity) {
             }
         }
         __syncthreads();
-        for (uint32_t r = R0 + threadIdx.x; r < R1; r += kBlock) {
-            const uint32_t e0 = s.rowPtr[r] - E0, e1 = s.rowPtr[r + 1] - E0;
-            const double pd = static_cast<double>(p[r]);
-            double sum = static_cast<double>(diag[r]) * pd;
-            for (uint32_t k = e0; k < e1; ++k) sum += prod[k];
-            const T v = static_cast<T>(sum);
-            ap[r] = v;
-            acc[0] += pd * static_cast<double>(v);
+        if constexpr (G2 == 1) {
+            for (uint32_t r = R0 + threadIdx.x; r < R1; r += kBlock) {
+                const uint32_t e0 = s.rowPtr[r] - E0, e1 = s.rowPtr[r + 1] - E0;
+                const double pd = static_cast<double>(p[r]);
+                double sum = static_cast<double>(diag[r]) * pd;
+                for (uint32_t k = e0; k < e1; ++k) sum += prod[k];
+                const T v = static_cast<T>(sum);
+                ap[r] = v;
+                acc[0] += pd * static_cast<double>(v);
+            }
+        } else {
+            // G2 lanes per row: lane l adds products l, l + G2, ... of the row,
+            // then a fixed butterfly; shorter waits on long rows, another order
+            constexpr uint32_t kGroups = kBlock / G2;
+            const uint32_t lane = threadIdx.x % G2, grp = threadIdx.x / G2;
+            for (uint32_t r0 = R0; r0 < R1; r0 += kGroups) {
+                const uint32_t r = r0 + grp;
+                const bool valid = r < R1;
+                const uint32_t rr = valid ? r : R0;
+                const uint32_t e0 = s.rowPtr[rr] - E0, e1 = valid ? s.rowPtr[rr + 1] - E0 : e0;
+                double part = 0.0;
+                for (uint32_t k = e0 + lane; k < e1; k += G2) part += prod[k];
+#pragma unroll
+                for (int off = G2 / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, G2);
+                if (valid && lane == 0) {
+                    const double pd = static_cast<double>(p[r]);
+                    const T v = static_cast<T>(static_cast<double>(diag[r]) * pd + part);
+                    ap[r] = v;
+                    acc[0] += pd * static_cast<double>(v);
+                }
+            }
         }
     }
     blockSum<1>(acc);
@@ -770,20 +793,21 @@ int launchSpmv(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, 
         if (attributesOnly) return KT_OK;
         const dim3 fg(flatGrid(a));
         const bool wide = ctx->tune.flatUnroll == 8;
+        const bool lanes4 = ctx->tune.flatRowLanes == 4;
+#define KT_FLAT(EE)                                                                                       \
+    if (lanes4) {                                                                                         \
+        if (wide) cgSpmvFlat<T, EE, 8, 4><<<fg, dim3(kBlock), 0, st>>>(a, parity);                        \
+        else cgSpmvFlat<T, EE, 4, 4><<<fg, dim3(kBlock), 0, st>>>(a, parity);                             \
+    } else {                                                                                              \
+        if (wide) cgSpmvFlat<T, EE, 8, 1><<<fg, dim3(kBlock), 0, st>>>(a, parity);                        \
+        else cgSpmvFlat<T, EE, 4, 1><<<fg, dim3(kBlock), 0, st>>>(a, parity);                             \
+    }
         switch (a.flatEntries) {
-            case 4096:
-                if (wide) cgSpmvFlat<T, 4096, 8><<<fg, dim3(kBlock), 0, st>>>(a, parity);
-                else cgSpmvFlat<T, 4096, 4><<<fg, dim3(kBlock), 0, st>>>(a, parity);
-                break;
-            case 16384:
-                if (wide) cgSpmvFlat<T, 16384, 8><<<fg, dim3(kBlock), 0, st>>>(a, parity);
-                else cgSpmvFlat<T, 16384, 4><<<fg, dim3(kBlock), 0, st>>>(a, parity);
-                break;
-            default:
-                if (wide) cgSpmvFlat<T, 8192, 8><<<fg, dim3(kBlock), 0, st>>>(a, parity);
-                else cgSpmvFlat<T, 8192, 4><<<fg, dim3(kBlock), 0, st>>>(a, parity);
-                break;
+            case 4096: KT_FLAT(4096) break;
+            case 16384: KT_FLAT(16384) break;
+            default: KT_FLAT(8192) break;
         }
+#undef KT_FLAT
         KT_HIP(hipGetLastError());
         return KT_OK;
     }

@@ -160,6 +160,7 @@ struct Tune {
     int layout = 0;           // SpMV layout (KT_SPMV_ROW_GROUPS / KT_SPMV_ENTRY_BLOCKS)
     uint32_t flatEntries = 0; // entry blocks: entries per block (0: 8192)
     int flatUnroll = 0;       // entry blocks: entries per thread and round (0: 4)
+    int flatRowLanes = 0;     // entry blocks: lanes per row in the row sums (0, 1: entry order; 4)
 };
 
 // A captured hipGraph of kPairs iteration pairs for one argument set.

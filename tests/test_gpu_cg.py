@@ -344,8 +344,8 @@ def test_entry_block_layout_solves_and_repeats(b2b12k, gpu_ctx_factory, precisio
     ctx.build_b2b(x0, y0)
     ref = o.solve(x0, y0, tol=1e-12, max_iter=100000)
     runs = []
-    for layout in (0, KT_SPMV_ENTRY_BLOCKS, KT_SPMV_ENTRY_BLOCKS):
-        ctx.set_tuning(lanes=4, tile=2048, layout=layout)
+    for layout, row_lanes in ((0, 0), (KT_SPMV_ENTRY_BLOCKS, 4), (KT_SPMV_ENTRY_BLOCKS, 4), (KT_SPMV_ENTRY_BLOCKS, 1)):
+        ctx.set_tuning(lanes=4, tile=2048, layout=layout, row_lanes=row_lanes)
         x, y = x0.copy(), y0.copy()
         r = ctx.solve_both(x, y, 1e-9, 100000, precision=prec, stop_rule=KT_STOP_TRUE)
         runs.append((x, y, r))
@@ -386,9 +386,11 @@ def test_entry_block_layout_long_rows(gpu_ctx_factory):
     ctx = gpu_ctx_factory()
     ctx.set_system(0, s)
     ctx.set_system(1, s)
-    for prec, entries, unroll in ((KT_PREC_FP64, 0, 0), (KT_PREC_MIXED, 0, 0), (KT_PREC_FP64, 4096, 8),
-                                  (KT_PREC_MIXED, 16384, 8), (KT_PREC_FP64, 16384, 4)):
-        ctx.set_tuning(lanes=4, tile=1024, layout=KT_SPMV_ENTRY_BLOCKS, block_entries=entries, unroll=unroll)
+    for prec, entries, unroll, row_lanes in ((KT_PREC_FP64, 0, 0, 0), (KT_PREC_MIXED, 0, 0, 4),
+                                             (KT_PREC_FP64, 4096, 8, 4), (KT_PREC_MIXED, 16384, 8, 1),
+                                             (KT_PREC_FP64, 16384, 4, 4)):
+        ctx.set_tuning(lanes=4, tile=1024, layout=KT_SPMV_ENTRY_BLOCKS, block_entries=entries, unroll=unroll,
+                       row_lanes=row_lanes)
         x, y = np.zeros(n), np.zeros(n)
         r = ctx.solve_both(x, y, 1e-10, 100000, precision=prec, stop_rule=KT_STOP_TRUE)
         assert max(r["true_resid"]) <= 1e-10
