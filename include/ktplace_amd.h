@@ -307,6 +307,10 @@ int kt_batch_size(kt_batch *b, uint32_t *instances, uint64_t *rows, uint64_t *nn
 /* lanes per CSR row (4, 8, 16), rows per tile (a multiple of 256), rows in
  * flight per lane group (2, 4), hipGraph replay on/off; 0 keeps a value. */
 int kt_batch_set_tuning(kt_batch *b, int lanes, uint32_t tile, int rows_in_flight, int use_graphs);
+/* SpMV layout of the batch (as kt_set_spmv_layout): KT_SPMV_ROW_GROUPS, or
+ * KT_SPMV_ENTRY_BLOCKS with blocks of about 2048 consecutive entries of one
+ * system and row_lanes (1 or 4; 0 = 1) lanes per row in the row sums. */
+int kt_batch_set_layout(kt_batch *b, int layout, int row_lanes);
 /* Every instance back to the start positions it was added with. */
 int kt_batch_reset(kt_batch *b);
 /* Solves all instances from their current positions (opts: KT_PREC_FP64,

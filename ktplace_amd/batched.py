@@ -12,8 +12,8 @@ B2B systems at that point (simpl_placer.cc:419-424).
            systems; iteration pairs replay one captured hipGraph; launch shape
            (lanes per row, tile, rows in flight) from `autotune`;
   loop     one context per instance, solved one after the other with the
-           two-system solver (its own graph replay), the per-instance launch
-           loop.
+           two-system solver (its own graph replay, either SpMV layout, the
+           faster reported), the per-instance launch loop.
 and reports solves per second and the p50/p99 per-instance solve time.  In the
 batched run an instance's time is when the host poll first saw both of its
 systems stopped.
@@ -63,23 +63,32 @@ def start_context(d, tol: float) -> simpl.Context:
 
 
 def autotune(batch: simpl.Batch, iterations: int = 48):
-    """Fastest (lanes, tile, rows) for `iterations` lock-step iterations of
-    the whole batch."""
+    """Fastest SpMV layout and (lanes, tile, rows) for `iterations` lock-step
+    iterations of the whole batch."""
     table = []
+
+    def timed(g, t, r, layout=0, row_lanes=0):
+        batch.set_tuning(lanes=g, tile=t, rows=r, layout=layout, row_lanes=row_lanes)
+        batch.reset()
+        batch.solve(0.0, iterations)  # warm-up (captures the graph)
+        batch.reset()
+        t0 = time.perf_counter()
+        batch.solve(0.0, iterations)
+        table.append(((time.perf_counter() - t0) * 1e3, g, t, r, layout, row_lanes))
+
     for g in LANES:
         for t in TILES:
             for r in ROWS:
-                batch.set_tuning(lanes=g, tile=t, rows=r)
-                batch.reset()
-                batch.solve(0.0, iterations)  # warm-up (captures the graph)
-                batch.reset()
-                t0 = time.perf_counter()
-                batch.solve(0.0, iterations)
-                table.append(((time.perf_counter() - t0) * 1e3, g, t, r))
+                timed(g, t, r)
+    for t in TILES:  # entry blocks: lanes and rows do not apply; the tile shapes the vector kernels
+        for row_lanes in (1, 4):
+            timed(4, t, 2, simpl.KT_SPMV_ENTRY_BLOCKS, row_lanes)
     table.sort()
-    ms, g, t, r = table[0]
-    batch.set_tuning(lanes=g, tile=t, rows=r)
-    return dict(lanes_per_row=g, tile_rows=t, rows_in_flight=r, ms_per_48_iterations=round(ms, 3))
+    ms, g, t, r, layout, row_lanes = table[0]
+    batch.set_tuning(lanes=g, tile=t, rows=r, layout=layout, row_lanes=row_lanes)
+    return dict(lanes_per_row=g, tile_rows=t, rows_in_flight=r,
+                spmv_layout="entry blocks" if layout else "row groups", row_lanes=row_lanes,
+                ms_per_48_iterations=round(ms, 3))
 
 
 def percentiles(v):
@@ -114,22 +123,30 @@ def run(count: int = 256, steps: int = 3, tol: float = 1e-6, seed: int = 5000, l
     worst = max(max(p["true_resid"]) for p in per)
     log(f"[batched] {b_ms:.2f} ms per batch, p50 {b50:.2f} p99 {b99:.2f} ms, worst true resid {worst:.2e}")
 
-    # loop: each instance alone, one after the other
-    for c in ctxs:  # warm-up (graph capture per context)
-        c.restore_positions()
-        c.solve_resident(tol, 1_000_000, stop_rule=simpl.KT_STOP_TRUE)
-    loop_totals, loop_each = [], []
-    for _ in range(steps):
-        each = []
-        for c in ctxs:
+    # loop: each instance alone, one after the other, with the two-system
+    # solver in either SpMV layout (the faster one is reported)
+    best = None
+    for layout in (simpl.KT_SPMV_ROW_GROUPS, simpl.KT_SPMV_ENTRY_BLOCKS):
+        for c in ctxs:  # warm-up (graph capture per context)
+            c.set_tuning(layout=layout)
             c.restore_positions()
-            c.sync()
-            t1 = time.perf_counter()
             c.solve_resident(tol, 1_000_000, stop_rule=simpl.KT_STOP_TRUE)
-            each.append((time.perf_counter() - t1) * 1e3)
-        loop_totals.append(sum(each))
-        loop_each = each
-    l_ms = float(np.median(loop_totals))
+        loop_totals, loop_each = [], []
+        for _ in range(steps):
+            each = []
+            for c in ctxs:
+                c.restore_positions()
+                c.sync()
+                t1 = time.perf_counter()
+                c.solve_resident(tol, 1_000_000, stop_rule=simpl.KT_STOP_TRUE)
+                each.append((time.perf_counter() - t1) * 1e3)
+            loop_totals.append(sum(each))
+            loop_each = each
+        ms = float(np.median(loop_totals))
+        log(f"[batched] loop ({'entry blocks' if layout else 'row groups'}) {ms:.2f} ms per pass")
+        if best is None or ms < best[0]:
+            best = (ms, loop_each, layout)
+    l_ms, loop_each, loop_layout = best
     l50, l99 = percentiles(loop_each)
     log(f"[batched] loop {l_ms:.2f} ms per pass, p50 {l50:.2f} p99 {l99:.2f} ms")
     iters = [max(p["steps"]) for p in per]
@@ -140,7 +157,7 @@ def run(count: int = 256, steps: int = 3, tol: float = 1e-6, seed: int = 5000, l
         batched=dict(ms=round(b_ms, 3), solves_per_s=round(inst / (b_ms * 1e-3), 1), p50_ms=round(b50, 3),
                      p99_ms=round(b99, 3), worst_true_resid=worst, tuning=tuning),
         loop=dict(ms=round(l_ms, 3), solves_per_s=round(inst / (l_ms * 1e-3), 1), p50_ms=round(l50, 3),
-                  p99_ms=round(l99, 3)),
+                  p99_ms=round(l99, 3), spmv_layout="entry blocks" if loop_layout else "row groups"),
         speedup=round(l_ms / b_ms, 2))
     batch.close()
     for c in ctxs:

@@ -29,6 +29,7 @@ constexpr int kBB = 256;  // threads per block
 
 struct BSysDev {
     uint32_t rowBase, rows, tileBase, tiles;
+    uint32_t flatBase, flatBlocks;  // entry-block SpMV: the system's blocks (and p.Ap partials)
 };
 
 struct BArgs {
@@ -37,9 +38,11 @@ struct BArgs {
     double *x, *r, *p, *ap;
     const BSysDev *sys;
     const uint32_t *tileSys;
-    double *partA, *partB, *partS;  // [tiles][1], [tiles][2], [tiles][3]
+    const uint4 *flat;              // entry-block SpMV: per block (system, first row, end row, 0)
+    double *partA, *partB, *partS;  // [SpMV blocks][1], [tiles][2], [tiles][3]
     AxisCtl *ctl;
     uint32_t tile;
+    uint32_t layout;                // 0 lane groups, 1 entry blocks
 };
 
 struct Tile {
@@ -81,15 +84,21 @@ __device__ __forceinline__ void blockSum(double (&v)[K]) {
     for (int k = 0; k < K; ++k) v[k] = total[k];
 }
 
-// Sum of one system's tile partials in tile order (same value in every block).
+// Sum of one system's partials [base, base + count) in order (same value in
+// every block).
 template <int K>
-__device__ __forceinline__ void sumPartials(const double *part, const BSysDev &d, double (&out)[K]) {
+__device__ __forceinline__ void sumPartials(const double *part, uint32_t base, uint32_t count, double (&out)[K]) {
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = 0.0;
-    for (uint32_t t = threadIdx.x; t < d.tiles; t += kBB)
+    for (uint32_t t = threadIdx.x; t < count; t += kBB)
 #pragma unroll
-        for (int k = 0; k < K; ++k) out[k] += part[static_cast<size_t>(d.tileBase + t) * K + k];
+        for (int k = 0; k < K; ++k) out[k] += part[static_cast<size_t>(base + t) * K + k];
     blockSum<K>(out);
+}
+
+template <int K>
+__device__ __forceinline__ void sumPartials(const double *part, const BSysDev &d, double (&out)[K]) {
+    sumPartials<K>(part, d.tileBase, d.tiles, out);
 }
 
 template <int K>
@@ -190,13 +199,105 @@ __global__ void __launch_bounds__(kBB) bSpmv(BArgs a, int parity) {
     writePartial<1>(a.partA, acc);
 }
 
+// Entry-block SpMV over the batch (the two-system solver's cgSpmvFlat, per
+// system): block b takes rows [R0, R1) of system S, whose entries are a run
+// of about kBFlat consecutive entries; the threads read the run's columns and
+// values coalesced, gather p and keep the products in LDS, then G2 lanes per
+// row add the row's products (G2 = 1: one thread, entry order after the
+// diagonal term).  A run longer than the LDS holds takes lane groups.
+constexpr uint32_t kBFlat = 2048;
+constexpr uint32_t kBFlatSlack = 512;
+
+template <int G2>
+__global__ void __launch_bounds__(kBB) bSpmvFlat(BArgs a, int parity) {
+    __shared__ double prod[kBFlat + kBFlatSlack];
+    const uint4 f = a.flat[blockIdx.x];
+    const uint32_t S = f.x, R0 = f.y, R1 = f.z;
+    if (!a.ctl[S].s[parity].live) return;
+    const uint32_t E0 = a.rowPtr[R0], count = a.rowPtr[R1] - E0;
+    double acc[1] = {0.0};
+    if (count > kBFlat + kBFlatSlack) {
+        constexpr int G = 4, R = 2;
+        const uint32_t lane = threadIdx.x % G, grp = threadIdx.x / G;
+        for (uint32_t r0 = R0; r0 < R1; r0 += R * (kBB / G)) {
+            uint32_t row[R];
+            bool valid[R];
+            double off[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                row[q] = r0 + grp + q * (kBB / G);
+                valid[q] = row[q] < R1;
+            }
+            rowsDot<double, G, R>(a.rowPtr, a.col, a.val, a.p, row, valid, lane, off);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                if (valid[q] && lane == 0) {
+                    const double pd = a.p[row[q]];
+                    const double v = a.diag[row[q]] * pd + off[q];
+                    a.ap[row[q]] = v;
+                    acc[0] += pd * v;
+                }
+            }
+        }
+    } else {
+        constexpr int U = 4;
+        for (uint32_t base = 0; base < count; base += U * kBB) {
+            uint32_t c[U];
+            double w[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t k = base + u * kBB + threadIdx.x;
+                const bool ok = k < count;
+                const uint32_t kk = E0 + (ok ? k : 0u);  // count > 0 here: entry E0 exists
+                const uint32_t cl = a.col[kk];
+                const double wl = a.val[kk];
+                c[u] = ok ? cl : kNoCol;
+                w[u] = ok ? wl : 0.0;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            double g[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) g[u] = gatherAt(a.p, c[u]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t k = base + u * kBB + threadIdx.x;
+                if (k < count) prod[k] = w[u] * g[u];
+            }
+        }
+        __syncthreads();
+        constexpr uint32_t kGroups = kBB / G2;
+        const uint32_t lane = threadIdx.x % G2, grp = threadIdx.x / G2;
+        for (uint32_t r0 = R0; r0 < R1; r0 += kGroups) {
+            const uint32_t r = r0 + grp;
+            const bool valid = r < R1;
+            const uint32_t rr = valid ? r : R0;
+            const uint32_t e0 = a.rowPtr[rr] - E0, e1 = valid ? a.rowPtr[rr + 1] - E0 : e0;
+            const double pd = valid ? a.p[rr] : 0.0;
+            double part = G2 == 1 ? a.diag[rr] * pd : 0.0;
+            for (uint32_t k = e0 + lane; k < e1; k += G2) part += prod[k];
+#pragma unroll
+            for (int off = G2 / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, G2);
+            if (valid && lane == 0) {
+                const double v = G2 == 1 ? part : a.diag[r] * pd + part;
+                a.ap[r] = v;
+                acc[0] += pd * v;
+            }
+        }
+    }
+    blockSum<1>(acc);
+    writePartial<1>(a.partA, acc);
+}
+
 __global__ void __launch_bounds__(kBB) bUpdate(BArgs a, int parity) {
     const Tile t = decode(a);
     AxisCtl &c = a.ctl[t.sys];
     const CgState st = c.s[parity];
     if (!st.live) return;
     double pap[1];
-    sumPartials<1>(a.partA, a.sys[t.sys], pap);
+    const BSysDev d = a.sys[t.sys];
+    if (a.layout) sumPartials<1>(a.partA, d.flatBase, d.flatBlocks, pap);
+    else sumPartials<1>(a.partA, d, pap);
     if (!(pap[0] > 0.0)) {
         if (t.first && threadIdx.x == 0) c.brk[parity] = 1;
         return;
@@ -313,6 +414,10 @@ struct kt_batch {
     uint32_t tileUploaded = 0;
     uint32_t tiles = 0;
     kt::DevBuf<uint32_t> dRowPtr, dCol, dTileSys;
+    kt::DevBuf<uint4> dFlat;
+    uint32_t flatBlocks = 0, partASlots = 0;
+    int layout = 0, rowLanes = 1;
+    uint32_t layoutUploaded = 0xffffffffu;
     kt::DevBuf<double> dVal, dDiag, dRhs, dX, dX0, dR, dP, dAp, dPart;
     kt::DevBuf<kt::BSysDev> dSys;
     kt::DevBuf<kt::AxisCtl> dCtl;
@@ -324,7 +429,7 @@ struct kt_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     kt::BArgs graphArgs{};
-    int graphLanes = 0, graphRows = 0;
+    int graphLanes = 0, graphRows = 0, graphLayout = -1, graphRowLanes = 0;
 };
 
 namespace kt {
@@ -340,7 +445,7 @@ void dropGraph(kt_batch *b) {
 }
 
 int upload(kt_batch *b) {
-    if (!b->dirty && b->tileUploaded == b->tile) return KT_OK;
+    if (!b->dirty && b->tileUploaded == b->tile && b->layoutUploaded == static_cast<uint32_t>(b->layout)) return KT_OK;
     const uint64_t rows = b->diag.size(), nnz = b->col.size();
     KT_TRY(b->dRowPtr.reserve(rows + 1));
     KT_TRY(b->dCol.reserve(std::max<uint64_t>(nnz, 1)));
@@ -350,15 +455,33 @@ int upload(kt_batch *b) {
     // tiles
     std::vector<BSysDev> sd(b->sys.size());
     std::vector<uint32_t> tileSys;
+    std::vector<uint4> flat;  // entry blocks: (system, first row, end row, 0)
     for (size_t s = 0; s < b->sys.size(); ++s) {
         const uint32_t t = (b->sys[s].rows + b->tile - 1) / b->tile;
-        sd[s] = BSysDev{b->sys[s].rowBase, b->sys[s].rows, static_cast<uint32_t>(tileSys.size()), t};
+        const uint32_t r0 = b->sys[s].rowBase, r1 = r0 + b->sys[s].rows;
+        const uint32_t fbase = static_cast<uint32_t>(flat.size());
+        // cut the system's entries every kBFlat at the next row start
+        uint32_t start = r0;
+        uint64_t target = static_cast<uint64_t>(b->rowPtr[r0]) + kBFlat;
+        for (uint32_t r = r0; r < r1; ++r) {
+            if (b->rowPtr[r] >= target && r > start) {
+                flat.push_back(make_uint4(static_cast<uint32_t>(s), start, r, 0u));
+                start = r;
+                while (target <= b->rowPtr[r]) target += kBFlat;
+            }
+        }
+        flat.push_back(make_uint4(static_cast<uint32_t>(s), start, r1, 0u));
+        sd[s] = BSysDev{r0, b->sys[s].rows, static_cast<uint32_t>(tileSys.size()), t, fbase,
+                        static_cast<uint32_t>(flat.size()) - fbase};
         tileSys.insert(tileSys.end(), t, static_cast<uint32_t>(s));
     }
     b->tiles = static_cast<uint32_t>(tileSys.size());
+    b->flatBlocks = static_cast<uint32_t>(flat.size());
+    b->partASlots = std::max(b->tiles, b->flatBlocks);
     KT_TRY(b->dSys.reserve(std::max<size_t>(sd.size(), 1)));
     KT_TRY(b->dTileSys.reserve(std::max<size_t>(tileSys.size(), 1)));
-    KT_TRY(b->dPart.reserve(std::max<size_t>(tileSys.size(), 1) * 6));
+    KT_TRY(b->dFlat.reserve(std::max<size_t>(flat.size(), 1)));
+    KT_TRY(b->dPart.reserve(std::max<size_t>(b->partASlots + 5ull * b->tiles, 1)));
     KT_TRY(b->dCtl.reserve(std::max<size_t>(b->sys.size(), 1)));
     if (b->ctlHostCap < b->sys.size()) {
         if (b->ctlHost) (void)hipHostFree(b->ctlHost);
@@ -380,9 +503,11 @@ int upload(kt_batch *b) {
     KT_TRY(h2d(b->dX.p, b->x0.data(), rows * sizeof(double)));
     KT_TRY(h2d(b->dSys.p, sd.data(), sd.size() * sizeof(BSysDev)));
     KT_TRY(h2d(b->dTileSys.p, tileSys.data(), tileSys.size() * sizeof(uint32_t)));
+    KT_TRY(h2d(b->dFlat.p, flat.data(), flat.size() * sizeof(uint4)));
     KT_HIP(hipStreamSynchronize(b->stream));
     b->dirty = false;
     b->tileUploaded = b->tile;
+    b->layoutUploaded = static_cast<uint32_t>(b->layout);
     dropGraph(b);
     return KT_OK;
 }
@@ -400,11 +525,13 @@ BArgs argsOf(kt_batch *b) {
     a.ap = b->dAp.p;
     a.sys = b->dSys.p;
     a.tileSys = b->dTileSys.p;
+    a.flat = b->dFlat.p;
     a.partA = b->dPart.p;
-    a.partB = b->dPart.p + b->tiles;
-    a.partS = b->dPart.p + 3ull * b->tiles;
+    a.partB = b->dPart.p + b->partASlots;
+    a.partS = b->dPart.p + b->partASlots + 2ull * b->tiles;
     a.ctl = b->dCtl.p;
     a.tile = b->tile;
+    a.layout = b->layout ? 1u : 0u;
     return a;
 }
 
@@ -418,6 +545,16 @@ BArgs argsOf(kt_batch *b) {
 int launchIter(kt_batch *b, const BArgs &a, int parity) {
     const dim3 grid(b->tiles);
     const int R = b->rows >= 4 ? 4 : 2;
+    if (a.layout) {
+        if (b->rowLanes == 4) bSpmvFlat<4><<<dim3(b->flatBlocks), dim3(kBB), 0, b->stream>>>(a, parity);
+        else bSpmvFlat<1><<<dim3(b->flatBlocks), dim3(kBB), 0, b->stream>>>(a, parity);
+        KT_HIP(hipGetLastError());
+        bUpdate<<<grid, dim3(kBB), 0, b->stream>>>(a, parity);
+        KT_HIP(hipGetLastError());
+        bPUpdate<<<grid, dim3(kBB), 0, b->stream>>>(a, parity);
+        KT_HIP(hipGetLastError());
+        return KT_OK;
+    }
 #define KT_B_SPMV(GG)                                                                      \
     if (R == 4) bSpmv<GG, 4><<<grid, dim3(kBB), 0, b->stream>>>(a, parity);                \
     else bSpmv<GG, 2><<<grid, dim3(kBB), 0, b->stream>>>(a, parity);
@@ -443,7 +580,7 @@ int runPairs(kt_batch *b, const BArgs &a, uint32_t pairs) {
     };
     if (!b->useGraphs) return direct(pairs);
     if (!b->exec || std::memcmp(&b->graphArgs, &a, sizeof(BArgs)) != 0 || b->graphLanes != b->lanes ||
-        b->graphRows != b->rows) {
+        b->graphRows != b->rows || b->graphLayout != b->layout || b->graphRowLanes != b->rowLanes) {
         dropGraph(b);
         KT_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         int rc = KT_OK;
@@ -461,6 +598,8 @@ int runPairs(kt_batch *b, const BArgs &a, uint32_t pairs) {
         b->graphArgs = a;
         b->graphLanes = b->lanes;
         b->graphRows = b->rows;
+        b->graphLayout = b->layout;
+        b->graphRowLanes = b->rowLanes;
     }
     uint32_t k = 0;
     for (; k + kPairs <= pairs; k += kPairs) KT_HIP(hipGraphLaunch(b->exec, b->stream));
@@ -719,6 +858,15 @@ int kt_batch_set_tuning(kt_batch *b, int lanes, uint32_t tile, int rows_in_fligh
     if (tile) b->tile = tile;
     if (rows_in_flight) b->rows = rows_in_flight;
     b->useGraphs = use_graphs != 0;
+    return KT_OK;
+}
+
+int kt_batch_set_layout(kt_batch *b, int layout, int row_lanes) {
+    KT_BATCH_GUARD(b);
+    KT_CHECK(layout == KT_SPMV_ROW_GROUPS || layout == KT_SPMV_ENTRY_BLOCKS, KT_EINVAL, "unknown SpMV layout");
+    KT_CHECK(row_lanes == 0 || row_lanes == 1 || row_lanes == 4, KT_EINVAL, "row lanes must be 0, 1 or 4");
+    b->layout = layout;
+    b->rowLanes = row_lanes == 4 ? 4 : 1;
     return KT_OK;
 }
 

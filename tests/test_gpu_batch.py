@@ -5,7 +5,8 @@ Bar: every batched solution's true relative residual, computed on the CPU
 with the oracle's own B2B matrix (scipy), at most the tolerance; iteration
 counts within 2 of the two-system solver's on the same system (block sums
 differ in order, so iterates agree to rounding) and its solution within 1e-6;
-bit-identical results on repeat; the same solutions for every launch shape.
+bit-identical results on repeat; the same solutions for every launch shape
+and SpMV layout.
 (The B2B systems are ill-conditioned, so solutions at a residual of 1e-10
 can differ by far more than 1e-10: residuals are the invariant checked.)
 """
@@ -68,9 +69,11 @@ def test_batch_deterministic_and_shapes_agree(instances):
     for c in ctxs[:6]:
         b.add_ctx(c)
     ref = None
-    for lanes, tile, rows, graphs in ((8, 1024, 2, True), (8, 1024, 2, True), (4, 256, 4, False),
-                                      (16, 2048, 2, True), (16, 512, 4, True)):
-        b.set_tuning(lanes=lanes, tile=tile, rows=rows, use_graphs=graphs)
+    for lanes, tile, rows, graphs, layout, rl in ((8, 1024, 2, True, 0, 0), (8, 1024, 2, True, 0, 0),
+                                                  (4, 256, 4, False, 0, 0), (16, 2048, 2, True, 0, 0),
+                                                  (16, 512, 4, True, 0, 0), (4, 512, 2, True, 1, 1),
+                                                  (4, 256, 2, False, 1, 4), (4, 1024, 2, True, 1, 4)):
+        b.set_tuning(lanes=lanes, tile=tile, rows=rows, use_graphs=graphs, layout=layout, row_lanes=rl)
         b.reset()
         per, _ = b.solve(1e-8, 100000, simpl.KT_STOP_TRUE)
         pos = [b.get_positions(i) for i in range(6)]
@@ -78,7 +81,7 @@ def test_batch_deterministic_and_shapes_agree(instances):
         if ref is None:
             ref = (per, pos)
             continue
-        if (lanes, tile, rows) == (8, 1024, 2):  # same shape: bit-identical
+        if (lanes, tile, rows, layout) == (8, 1024, 2, 0):  # same shape: bit-identical
             assert [p["steps"] for p in per] == [p["steps"] for p in ref[0]]
             for (ax, ay), (bx, by) in zip(pos, ref[1]):
                 assert np.array_equal(ax, bx) and np.array_equal(ay, by)
@@ -88,12 +91,15 @@ def test_batch_deterministic_and_shapes_agree(instances):
     b.close()
 
 
-def test_batch_edge_cases():
+@pytest.mark.parametrize("layout", [0, 1])
+def test_batch_edge_cases(layout):
     """A zero right-hand side (axis inactive), a single-row system, a row with
-    no off-diagonals and the max_iter cap, next to an ordinary instance."""
+    no off-diagonals and the max_iter cap, next to an ordinary instance, for
+    both SpMV layouts."""
     from test_oracle import spd_system
     rng = np.random.default_rng(3)
     b = simpl.Batch(0)
+    b.set_tuning(layout=layout, row_lanes=4 if layout else 0)
     s1, s2 = spd_system(rng, 500, 3000), spd_system(rng, 500, 2000)
     zero = simpl.SparseSystem(s2.row_ptr, s2.col, s2.val, s2.diag, np.zeros(500))
     b.add(s1, zero, np.zeros(500), np.zeros(500))
