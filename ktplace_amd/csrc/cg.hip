@@ -45,9 +45,20 @@ constexpr uint64_t kMaxWindowBytes = 156ull * 1024;
 // Block -> (axis, tile).  Workgroups are dealt to the 8 XCDs round-robin by
 // id, so with xcdSplit the blocks that land on XCDs 0-3 take the x system and
 // those on XCDs 4-7 the y system: each XCD's L2 then caches the gathered
-// vector of one axis only.  Returns false for a block with no tile.
-__device__ __forceinline__ bool decode(const CgArgs &a, int &axis, uint32_t &tile) {
+// vector of one axis only.  Inside an iteration (parity >= 0) where only one
+// axis is live, every block serves that axis, so the split does not idle half
+// the GPU once the other axis has converged (all kernels of the iteration
+// read the same flags).  Returns false for a block with no tile.
+__device__ __forceinline__ bool decode(const CgArgs &a, int &axis, uint32_t &tile, int parity = -1) {
     if (a.xcdSplit) {
+        if (parity >= 0) {
+            const bool lx = a.ctl[0].s[parity].live, ly = a.ctl[1].s[parity].live;
+            if (lx != ly) {
+                axis = lx ? 0 : 1;
+                tile = blockIdx.x;
+                return tile < a.sys[axis].blocks;
+            }
+        }
         const uint32_t b = blockIdx.x, x = b & 7u;
         axis = static_cast<int>(x >> 2);
         tile = (b >> 3) * 4u + (x & 3u);
@@ -261,7 +272,7 @@ template <typename T, int G, int R, bool S>
 __global__ void __launch_bounds__(kBlock) cgSpmv(CgArgs a, int parity) {
     int axis;
     uint32_t tile;
-    if (!decode(a, axis, tile)) return;
+    if (!decode(a, axis, tile, parity)) return;
     if (!a.ctl[axis].s[parity].live) return;
     const SysView s = a.sys[axis];
     const T *__restrict__ p = static_cast<const T *>(a.p[axis]);
@@ -333,8 +344,14 @@ constexpr uint32_t kFlatSlack = 512;
 // XCDs 0-3 (workgroups are dealt to the 8 XCDs round-robin) take the x
 // system and those on XCDs 4-7 the y system, so each XCD's L2 sees the
 // gathers of one axis only.
-__device__ __forceinline__ bool decodeFlat(const CgArgs &a, int &axis, uint32_t &b) {
+__device__ __forceinline__ bool decodeFlat(const CgArgs &a, int &axis, uint32_t &b, int parity) {
     if (a.xcdSplit) {
+        const bool lx = a.ctl[0].s[parity].live, ly = a.ctl[1].s[parity].live;
+        if (lx != ly) {  // one axis live: every block serves it (as decode)
+            axis = lx ? 0 : 1;
+            b = blockIdx.x;
+            return b < a.sys[axis].sblocks;
+        }
         const uint32_t id = blockIdx.x, x = id & 7u;
         axis = static_cast<int>(x >> 2);
         b = (id >> 3) * 4u + (x & 3u);
@@ -356,7 +373,7 @@ __global__ void __launch_bounds__(kBlock) cgSpmvFlat(CgArgs a, int parity) {
     __shared__ double prod[kFlatEntries + kFlatSlack];
     int axis;
     uint32_t b;
-    if (!decodeFlat(a, axis, b)) return;
+    if (!decodeFlat(a, axis, b, parity)) return;
     if (!a.ctl[axis].s[parity].live) return;
     const SysView s = a.sys[axis];
     const T *__restrict__ p = static_cast<const T *>(a.p[axis]);
@@ -481,7 +498,7 @@ template <typename T, int PER>
 __global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a, int parity) {
     int axis;
     uint32_t tile;
-    if (!decode(a, axis, tile)) return;
+    if (!decode(a, axis, tile, parity)) return;
     const CgState st = a.ctl[axis].s[parity];
     if (!st.live) return;
     const SysView s = a.sys[axis];
@@ -548,14 +565,18 @@ template <typename T, int PER>
 __global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
     int axis;
     uint32_t tile;
-    if (!decode(a, axis, tile)) return;
+    // grid block 0 carries every stopped axis' state forward (a stopped axis
+    // may have no block of its own in this launch)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+#pragma unroll
+        for (int ax = 0; ax < 2; ++ax)
+            if (!a.ctl[ax].s[parity].live) a.ctl[ax].s[parity ^ 1] = a.ctl[ax].s[parity];
+    }
+    if (!decode(a, axis, tile, parity)) return;
     AxisCtl &c = a.ctl[axis];
     const CgState st = c.s[parity];
     const bool lead = tile == 0 && threadIdx.x == 0;
-    if (!st.live) {
-        if (lead) c.s[parity ^ 1] = st;  // carry a stopped axis' state forward
-        return;
-    }
+    if (!st.live) return;
     const SysView s = a.sys[axis];
     const size_t hist = static_cast<size_t>(axis) * a.histCap + st.steps;
     if (c.brk[parity]) {  // p.Ap <= 0: the reference stops the axis (simpl_lss.cc:331-334)
