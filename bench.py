@@ -94,6 +94,7 @@ def main() -> int:
     ap.add_argument("--block-entries", type=int, default=0, help="with --layout 1: entries per block")
     ap.add_argument("--unroll", type=int, default=0, help="with --layout 1: entries per thread and round")
     ap.add_argument("--row-lanes", type=int, default=0, help="with --layout 1: lanes per row in the row sums")
+    ap.add_argument("--block-threads", type=int, default=0, help="with --layout 1: threads per block")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -138,21 +139,16 @@ def main() -> int:
         ctx.build_b2b_resident(0.0)
         if args.lanes and args.tile:
             tuned = TuneResult(args.lanes, args.tile, float("nan"), [], args.xcd_split, args.rows, args.halo,
-                               args.layout, args.block_entries, args.unroll, args.row_lanes)
+                               args.layout, args.block_entries, args.unroll, args.row_lanes, args.block_threads)
         elif prec in tuned_by_prec:
             tuned = tuned_by_prec[prec]
         else:
             tuned = autotune(ctx, prec)
             tuned_by_prec[prec] = tuned
-        ctx.set_tuning(lanes=tuned.lanes, tile=tuned.tile, xcd_split=tuned.xcd_split, rows=tuned.rows,
-                       halo=tuned.halo, layout=tuned.layout, block_entries=tuned.block_entries,
-                       unroll=tuned.unroll, row_lanes=tuned.row_lanes)
-        log(f"[bench] {path}: autotune lanes={tuned.lanes} tile={tuned.tile} xcd_split={tuned.xcd_split} "
-            f"rows={tuned.rows} halo={tuned.halo} layout={tuned.layout} "
-            f"block_entries={tuned.block_entries} unroll={tuned.unroll} row_lanes={tuned.row_lanes} "
-            f"({tuned.ms_per_iteration:.3f} ms/iteration)")
-        for row in sorted(tuned.table, key=lambda t: t[-1])[:6]:
-            log(f"[bench]   candidate lanes/tile/xcd/rows/halo/layout/entries/unroll/row_lanes {row[:-1]}: {row[-1]:.4f} ms/iteration")
+        ctx.set_tuning(**tuned.kwargs())
+        log(f"[bench] {path}: autotune {tuned.kwargs()} ({tuned.ms_per_iteration:.3f} ms/iteration)")
+        for cand, ms in sorted(tuned.table, key=lambda t: t[1])[:6]:
+            log(f"[bench]   candidate {cand}: {ms:.4f} ms/iteration")
 
         def step(profile: bool, prec=prec, warm=warm) -> dict:
             ctx.restore_positions()
@@ -203,7 +199,9 @@ def main() -> int:
             tuning={"lanes_per_row": tuned.lanes, "tile_rows": tuned.tile,
                     "xcd_split": tuned.xcd_split, "rows_in_flight": tuned.rows, "lds_halo": tuned.halo,
                     "spmv_layout": "entry blocks" if tuned.layout else "row groups",
-                    "block_entries": tuned.block_entries or (8192 if tuned.layout else None),
+                    "block_threads": tuned.block_threads or (1024 if tuned.layout else None),
+                    "block_entries": tuned.block_entries or (8 * (tuned.block_threads or 1024)
+                                                             if tuned.layout else None),
                     "unroll": tuned.unroll or (4 if tuned.layout else None),
                     "row_lanes": tuned.row_lanes or (1 if tuned.layout else None)},
             roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",

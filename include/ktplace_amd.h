@@ -203,18 +203,22 @@ int kt_set_spmv_halo(kt_ctx *ctx, uint32_t halo);
 /* CG SpMV layout (a launch-shape knob with no counterpart in the reference).
  * KT_SPMV_ROW_GROUPS (default): lane groups of `lanes` threads per row, rows
  * in flight and the optional LDS window as set above.  KT_SPMV_ENTRY_BLOCKS:
- * each block takes the rows of a run of about block_entries consecutive
- * entries (4096, 8192 or 16384; 0 = 8192); its threads read the run's
- * columns and values fully coalesced, unroll (4 or 8; 0 = 4) entries each per
- * round, gather p and keep the products in LDS; with row_lanes 1 (or 0) one
- * thread per row adds them in entry order after the diagonal term (the
+ * each block of block_threads threads (1024, 512 or 256; 0 = 1024) takes the
+ * rows of a run of about block_entries consecutive entries (1024 threads:
+ * 4096, 8192 or 16384; 512 threads: 2048, 4096 or 8192; 256 threads: 2048 or
+ * 4096; 0 = 8 per thread); its threads read the run's columns and values
+ * fully coalesced, unroll (4 or 8, 8 with 1024 threads only; 0 = 4) entries
+ * each per round, gather p and keep the products in LDS; with row_lanes 1 (or
+ * 0) one thread per row adds them in entry order after the diagonal term (the
  * reference's multiply order), with row_lanes 4 four lanes per row add every
  * fourth product and combine by a fixed butterfly.  All are deterministic;
- * they round differently.  The entry-block layout takes no LDS window
- * (KT_EINVAL at the solve). */
+ * they round differently (the block size changes only the order in which
+ * p.Ap is summed).  The entry-block layout takes no LDS window (KT_EINVAL at
+ * the solve). */
 #define KT_SPMV_ROW_GROUPS 0
 #define KT_SPMV_ENTRY_BLOCKS 1
-int kt_set_spmv_layout(kt_ctx *ctx, int layout, uint32_t block_entries, int unroll, int row_lanes);
+int kt_set_spmv_layout(kt_ctx *ctx, int layout, uint32_t block_entries, int unroll, int row_lanes,
+                       int block_threads);
 
 /* ---- per-kernel timings (HIP events on the context's stream) -------------- */
 #define KT_KERNEL_SPMV 0      /* FP64 CG SpMV + p.Ap */

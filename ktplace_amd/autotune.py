@@ -2,11 +2,14 @@
 
 Chooses the SpMV layout (lane groups per row, or entry blocks), the lanes
 per row (threads cooperating on one CSR row: 1, 2, 4, 8 or 16), the tile
-(rows per 1024-thread block), rows in flight, the XCD split and the LDS
-window (`halo`: entries of p staged on each side of a tile) by timing a fixed number of CG iterations of the context's current
+(rows per 1024-thread block), rows in flight, the XCD split, the LDS
+window (`halo`: entries of p staged on each side of a tile) and the
+entry-block shape (threads and entries per block, lanes per row in the row
+sums) by timing a fixed number of CG iterations of the context's current
 systems on the GPU, from the same saved start each time.  First lanes,
 tile, rows and XCD split without staging; then, with the best rows and XCD
-split, every lanes, tile and halo whose window fits the LDS budget.  The search
+split, every lanes, tile and halo whose window fits the LDS budget; then the
+entry-block shapes; then the four fastest again over longer runs.  The search
 costs a few hundred iterations, i.e. a fraction of one solve; results are
 cached per (rows, nnz, precision) signature for the life of the process.
 """
@@ -23,8 +26,8 @@ ROWS = (2, 4, 8)
 XCD = (1, 0)
 HALOS = (2048, 4096, 6144, 8192, 12288, 16384)
 HALO_TILES = (1024, 2048, 4096)
-FLAT_ENTRIES = (4096, 8192)  # entry-block SpMV: entries per block
-FLAT_UNROLL = (4, 8)         # entry-block SpMV: entries per thread and round
+# entry-block SpMV: (threads per block, entries per block) shapes
+FLAT_SHAPES = ((256, 2048), (512, 4096), (1024, 8192))
 FLAT_ROW_LANES = (1, 4)      # entry-block SpMV: lanes per row in the row sums
 FLAT_TILES = (2048, 4096)    # tiles (of the vector kernels) tried with entry blocks
 WINDOW_BYTES = 156 * 1024  # LDS a staged SpMV block may take
@@ -45,6 +48,13 @@ class TuneResult:
     block_entries: int = 0
     unroll: int = 0
     row_lanes: int = 0
+    block_threads: int = 0
+
+    def kwargs(self) -> dict:
+        """The choice as `Context.set_tuning` keyword arguments."""
+        return dict(lanes=self.lanes, tile=self.tile, xcd_split=self.xcd_split, rows=self.rows, halo=self.halo,
+                    layout=self.layout, block_entries=self.block_entries, unroll=self.unroll,
+                    row_lanes=self.row_lanes, block_threads=self.block_threads)
 
 
 def _signature(ctx: "simpl.Context", precision: int):
@@ -65,71 +75,60 @@ def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iteratio
 
     The context's positions are saved before and restored after the search,
     so the caller's state is unchanged.  `halos=()` leaves staging off.
+    `table` holds (set_tuning keyword arguments, ms per iteration) pairs.
     """
     key = _signature(ctx, precision)
     if not force and key in _cache:
         best = _cache[key]
-        ctx.set_tuning(lanes=best.lanes, tile=best.tile, use_graphs=use_graphs,
-                       xcd_split=best.xcd_split, rows=best.rows, halo=best.halo, layout=best.layout,
-                       block_entries=best.block_entries, unroll=best.unroll, row_lanes=best.row_lanes)
+        ctx.set_tuning(use_graphs=use_graphs, **best.kwargs())
         return best
     rows = key[0]
     ctx.save_positions()
     table = []
 
-    def timed(g, tile, xs, rr, halo, layout=0, entries=0, unroll=0, row_lanes=0):
-        ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr, halo=halo,
-                       layout=layout, block_entries=entries, unroll=unroll, row_lanes=row_lanes)
-        ctx.restore_positions()
-        ctx.solve_resident(0.0, 2, precision=precision)  # graph capture, first touch
+    def run(cand: dict, its: int) -> float:
+        ctx.set_tuning(use_graphs=use_graphs, **cand)
         ctx.restore_positions()
         ctx.sync()
         t0 = time.perf_counter()
-        r = ctx.solve_resident(0.0, iterations, precision=precision, check_every=iterations)
+        r = ctx.solve_resident(0.0, its, precision=precision, check_every=its)
         ctx.sync()
-        dt = time.perf_counter() - t0
-        its = max(r["iterations"], 1)
-        table.append((g, tile, xs, rr, halo, layout, entries, unroll, row_lanes, 1e3 * dt / its))
+        return 1e3 * (time.perf_counter() - t0) / max(r["iterations"], 1)
+
+    def timed(**cand):
+        ctx.set_tuning(use_graphs=use_graphs, **cand)
+        ctx.restore_positions()
+        ctx.solve_resident(0.0, 2, precision=precision)  # graph capture, first touch
+        table.append((cand, run(cand, iterations)))
 
     small = lambda t: not (t > 4 * max(rows, 1024) and t != tiles[0])
     for g, tile, xs, rr in [(g, t, x, rr) for t in tiles for g in lanes for x in XCD for rr in ROWS if small(t)]:
-        timed(g, tile, xs, rr, 0)
-    _, _, xs, rr, _, _, _, _, _, _ = min(table, key=lambda t: t[-1])
+        timed(lanes=g, tile=tile, xcd_split=xs, rows=rr)
+    best_rows = min(table, key=lambda t: t[1])[0]
+    xs, rr = best_rows["xcd_split"], best_rows["rows"]
     for tile in HALO_TILES:
         for halo in halos:
             if window_fits(tile, halo, precision) and halo < rows and small(tile):
                 for g in lanes:
-                    timed(g, tile, xs, rr, halo)
+                    timed(lanes=g, tile=tile, xcd_split=xs, rows=rr, halo=halo)
     # entry-block SpMV (lanes, rows in flight and halo do not apply; the tile
     # and XCD split shape the vector kernels)
     for tile in FLAT_TILES:
         for x in XCD:
             if small(tile):
-                for entries in FLAT_ENTRIES:
-                    for unroll in FLAT_UNROLL:
-                        for row_lanes in FLAT_ROW_LANES:
-                            timed(4, tile, x, 2, 0, simpl.KT_SPMV_ENTRY_BLOCKS, entries, unroll, row_lanes)
+                for threads, entries in FLAT_SHAPES:
+                    for row_lanes in FLAT_ROW_LANES:
+                        timed(lanes=4, tile=tile, xcd_split=x, rows=2, layout=simpl.KT_SPMV_ENTRY_BLOCKS,
+                              block_entries=entries, unroll=4, row_lanes=row_lanes, block_threads=threads)
     # the closest candidates again, over twice the iterations, twice each:
     # single short timings of near-equal shapes are noisy
-    finalists = sorted(table, key=lambda t: t[-1])[:4]
-    table = [t for t in table if t not in finalists]
-    for cand in finalists:
-        best_ms = float("inf")
-        for _ in range(2):
-            ctx.set_tuning(lanes=cand[0], tile=cand[1], use_graphs=use_graphs, xcd_split=cand[2], rows=cand[3],
-                           halo=cand[4], layout=cand[5], block_entries=cand[6], unroll=cand[7], row_lanes=cand[8])
-            ctx.restore_positions()
-            ctx.sync()
-            t0 = time.perf_counter()
-            r = ctx.solve_resident(0.0, 2 * iterations, precision=precision, check_every=2 * iterations)
-            ctx.sync()
-            best_ms = min(best_ms, 1e3 * (time.perf_counter() - t0) / max(r["iterations"], 1))
-        table.append((*cand[:9], best_ms))
+    table.sort(key=lambda t: t[1])
+    finalists, table = table[:4], table[4:]
+    for cand, _ in finalists:
+        table.append((cand, min(run(cand, 2 * iterations) for _ in range(2))))
     ctx.restore_positions()
-    g, tile, xs, rr, halo, layout, entries, unroll, row_lanes, ms = min(table, key=lambda t: t[-1])
-    best = TuneResult(lanes=g, tile=tile, ms_per_iteration=ms, table=table, xcd_split=xs, rows=rr, halo=halo,
-                      layout=layout, block_entries=entries, unroll=unroll, row_lanes=row_lanes)
+    cand, ms = min(table, key=lambda t: t[1])
+    best = TuneResult(ms_per_iteration=ms, table=table, **{"xcd_split": 1, "rows": 2, **cand})
     _cache[key] = best
-    ctx.set_tuning(lanes=g, tile=tile, use_graphs=use_graphs, xcd_split=xs, rows=rr, halo=halo, layout=layout,
-                   block_entries=entries, unroll=unroll, row_lanes=row_lanes)
+    ctx.set_tuning(use_graphs=use_graphs, **best.kwargs())
     return best

@@ -124,11 +124,14 @@ def run(count: int = 256, steps: int = 3, tol: float = 1e-6, seed: int = 5000, l
     log(f"[batched] {b_ms:.2f} ms per batch, p50 {b50:.2f} p99 {b99:.2f} ms, worst true resid {worst:.2e}")
 
     # loop: each instance alone, one after the other, with the two-system
-    # solver in either SpMV layout (the faster one is reported)
+    # solver in either SpMV layout, entry blocks of 1024 or 256 threads (the
+    # fastest is reported)
     best = None
-    for layout in (simpl.KT_SPMV_ROW_GROUPS, simpl.KT_SPMV_ENTRY_BLOCKS):
+    for layout, threads in ((simpl.KT_SPMV_ROW_GROUPS, 0), (simpl.KT_SPMV_ENTRY_BLOCKS, 0),
+                            (simpl.KT_SPMV_ENTRY_BLOCKS, 256)):
+        name = "row groups" if not layout else f"entry blocks of {threads or 1024} threads"
         for c in ctxs:  # warm-up (graph capture per context)
-            c.set_tuning(layout=layout)
+            c.set_tuning(layout=layout, block_threads=threads)
             c.restore_positions()
             c.solve_resident(tol, 1_000_000, stop_rule=simpl.KT_STOP_TRUE)
         loop_totals, loop_each = [], []
@@ -143,9 +146,9 @@ def run(count: int = 256, steps: int = 3, tol: float = 1e-6, seed: int = 5000, l
             loop_totals.append(sum(each))
             loop_each = each
         ms = float(np.median(loop_totals))
-        log(f"[batched] loop ({'entry blocks' if layout else 'row groups'}) {ms:.2f} ms per pass")
+        log(f"[batched] loop ({name}) {ms:.2f} ms per pass")
         if best is None or ms < best[0]:
-            best = (ms, loop_each, layout)
+            best = (ms, loop_each, name)
     l_ms, loop_each, loop_layout = best
     l50, l99 = percentiles(loop_each)
     log(f"[batched] loop {l_ms:.2f} ms per pass, p50 {l50:.2f} p99 {l99:.2f} ms")
@@ -157,7 +160,7 @@ def run(count: int = 256, steps: int = 3, tol: float = 1e-6, seed: int = 5000, l
         batched=dict(ms=round(b_ms, 3), solves_per_s=round(inst / (b_ms * 1e-3), 1), p50_ms=round(b50, 3),
                      p99_ms=round(b99, 3), worst_true_resid=worst, tuning=tuning),
         loop=dict(ms=round(l_ms, 3), solves_per_s=round(inst / (l_ms * 1e-3), 1), p50_ms=round(l50, 3),
-                  p99_ms=round(l99, 3), spmv_layout="entry blocks" if loop_layout else "row groups"),
+                  p99_ms=round(l99, 3), spmv_layout=loop_layout),
         speedup=round(l_ms / b_ms, 2))
     batch.close()
     for c in ctxs:

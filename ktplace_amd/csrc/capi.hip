@@ -767,15 +767,26 @@ int kt_set_spmv_halo(kt_ctx *ctx, uint32_t halo) {
     return KT_OK;
 }
 
-int kt_set_spmv_layout(kt_ctx *ctx, int layout, uint32_t block_entries, int unroll, int row_lanes) {
+int kt_set_spmv_layout(kt_ctx *ctx, int layout, uint32_t block_entries, int unroll, int row_lanes,
+                       int block_threads) {
     KT_GUARD(ctx);
     KT_CHECK(layout == KT_SPMV_ROW_GROUPS || layout == KT_SPMV_ENTRY_BLOCKS, KT_EINVAL,
              "kt_set_spmv_layout: unknown layout");
-    KT_CHECK(block_entries == 0 || block_entries == 4096 || block_entries == 8192 || block_entries == 16384,
-             KT_EINVAL, "kt_set_spmv_layout: block entries must be 0, 4096, 8192 or 16384");
+    KT_CHECK(block_threads == 0 || block_threads == 256 || block_threads == 512 || block_threads == 1024,
+             KT_EINVAL, "kt_set_spmv_layout: block threads must be 0, 256, 512 or 1024");
+    const int nt = block_threads == 0 ? 1024 : block_threads;
+    // the entry-block sizes built for each block size
+    const bool entriesOk = block_entries == 0 ||
+                           (nt == 1024 && (block_entries == 4096 || block_entries == 8192 || block_entries == 16384)) ||
+                           (nt == 512 && block_entries == 4096) || (nt == 256 && block_entries == 2048);
+    KT_CHECK(entriesOk, KT_EINVAL,
+             "kt_set_spmv_layout: block entries must be 0 or, for 1024 threads, 4096, 8192 or 16384; "
+             "for 512 threads 4096; for 256 threads 2048");
     KT_CHECK(unroll == 0 || unroll == 4 || unroll == 8, KT_EINVAL, "kt_set_spmv_layout: unroll must be 0, 4 or 8");
+    KT_CHECK(unroll != 8 || nt != 512, KT_EINVAL, "kt_set_spmv_layout: unroll 8 needs 256 or 1024 threads per block");
     KT_CHECK(row_lanes == 0 || row_lanes == 1 || row_lanes == 4, KT_EINVAL,
              "kt_set_spmv_layout: row lanes must be 0, 1 or 4");
+    ctx->tune.flatThreads = block_threads;
     ctx->tune.flatRowLanes = row_lanes;
     ctx->tune.layout = layout;
     ctx->tune.flatEntries = block_entries;

@@ -71,9 +71,9 @@ __device__ __forceinline__ bool decode(const CgArgs &a, int &axis, uint32_t &til
 
 // Fixed-order block sum of K values per thread; the result is returned to
 // every thread of the block.
-template <int K>
+template <int K, int NT = kBlock>
 __device__ __forceinline__ void blockSum(double (&v)[K]) {
-    __shared__ double waveSums[kBlock / 64][K];
+    __shared__ double waveSums[NT / 64][K];
     __shared__ double total[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -88,7 +88,7 @@ __device__ __forceinline__ void blockSum(double (&v)[K]) {
     __syncthreads();
     if (threadIdx.x < K) {
         double s = waveSums[0][threadIdx.x];
-        for (int w = 1; w < kBlock / 64; ++w) s += waveSums[w][threadIdx.x];
+        for (int w = 1; w < NT / 64; ++w) s += waveSums[w][threadIdx.x];
         total[threadIdx.x] = s;
     }
     __syncthreads();
@@ -326,20 +326,21 @@ __global__ void __launch_bounds__(kBlock) cgSpmv(CgArgs a, int parity) {
 
 // ---------------------------------------------------------------- entry-block SpMV
 // Block b of an axis takes the rows [rowStart[b], rowStart[b+1]), whose
-// entries are a run of at most kFlatEntries + the longest row (rowStart cuts
-// the entry sequence every kFlatEntries entries at the next row start).
-//   phase 1: thread t takes entries t, t + 1024, ... of the run, so each load
+// entries are a run of at most E + the longest row (rowStart cuts the entry
+// sequence every E entries at the next row start).  With NT threads:
+//   phase 1: thread t takes entries t, t + NT, ... of the run, so each load
 //            instruction of a wave reads 64 consecutive columns and values;
 //            it gathers p and leaves the product (in double: exact for FP32
 //            operands) in LDS;
-//   phase 2: thread t takes rows R0 + t, R0 + t + 1024, ...: Ap = d p, then
+//   phase 2: thread t takes rows R0 + t, R0 + t + NT, ...: Ap = d p, then
 //            the row's products added in entry order (SparseSystem::multiply's
 //            order, simpl_lss.h:49-57), and the row's share of p.Ap.
-// A run longer than the LDS holds (a row of more than kFlatSlack entries
-// crossing a cut) is done with the lane-group rows instead.
-constexpr uint32_t kFlatSlack = 512;
+// A run longer than the LDS holds (more than E + flatSlack(E) entries: a row
+// that long crossing a cut) is done with the lane-group rows instead.  The
+// slack is an eighth of the block, at most 512 entries, so 2048-entry blocks
+// of 256 threads fit eight to a CU's LDS.
+constexpr uint32_t flatSlack(uint32_t entries) { return entries / 8 < 512 ? entries / 8 : 512; }
 
-// E: entries per block (the cut spacing); U: entries per thread and round.
 // Block -> (axis, block of that axis).  With xcdSplit the blocks that land on
 // XCDs 0-3 (workgroups are dealt to the 8 XCDs round-robin) take the x
 // system and those on XCDs 4-7 the y system, so each XCD's L2 sees the
@@ -362,14 +363,21 @@ __device__ __forceinline__ bool decodeFlat(const CgArgs &a, int &axis, uint32_t 
     return true;
 }
 
+// Threads per entry block (0 picks 1024).
+int flatThreads(int tuned) { return tuned == 256 || tuned == 512 ? tuned : kBlock; }
+
 uint32_t flatGrid(const CgArgs &a) {
     if (a.xcdSplit) return 8u * ((std::max(a.sys[0].sblocks, a.sys[1].sblocks) + 3u) / 4u);
     return a.sys[0].sblocks + a.sys[1].sblocks;
 }
 
-template <typename T, uint32_t E, int U, int G2>
-__global__ void __launch_bounds__(kBlock) cgSpmvFlat(CgArgs a, int parity) {
-    constexpr uint32_t kFlatEntries = E;
+// E: entries per block (the cut spacing); U: entries per thread and round;
+// G2: lanes per row in the row sums; NT: threads per block.  Smaller blocks
+// let more of them share a CU, so one block's gathers overlap another's row
+// sums and reduction.
+template <typename T, uint32_t E, int U, int G2, int NT>
+__global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
+    constexpr uint32_t kFlatEntries = E, kFlatSlack = flatSlack(E);
     __shared__ double prod[kFlatEntries + kFlatSlack];
     int axis;
     uint32_t b;
@@ -386,13 +394,13 @@ __global__ void __launch_bounds__(kBlock) cgSpmvFlat(CgArgs a, int parity) {
     if (count > kFlatEntries + kFlatSlack) {
         constexpr int G = 4, R = 2;
         const uint32_t lane = threadIdx.x % G, grp = threadIdx.x / G;
-        for (uint32_t r0 = R0; r0 < R1; r0 += R * (kBlock / G)) {
+        for (uint32_t r0 = R0; r0 < R1; r0 += R * (NT / G)) {
             uint32_t row[R];
             bool valid[R];
             double off[R];
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                row[q] = r0 + grp + q * (kBlock / G);
+                row[q] = r0 + grp + q * (NT / G);
                 valid[q] = row[q] < R1;
             }
             rowsDot<T, G, R>(s.rowPtr, s.col, val, p, row, valid, lane, off);
@@ -407,12 +415,12 @@ __global__ void __launch_bounds__(kBlock) cgSpmvFlat(CgArgs a, int parity) {
             }
         }
     } else {
-        for (uint32_t base = 0; base < count; base += U * kBlock) {
+        for (uint32_t base = 0; base < count; base += U * NT) {
             uint32_t c[U];
             T w[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint32_t k = base + u * kBlock + threadIdx.x;
+                const uint32_t k = base + u * NT + threadIdx.x;
                 const bool ok = k < count;
                 const uint32_t kk = E0 + (ok ? k : 0u);  // count > 0 here: entry E0 exists
                 const uint32_t cl = s.col[kk];
@@ -427,13 +435,13 @@ __global__ void __launch_bounds__(kBlock) cgSpmvFlat(CgArgs a, int parity) {
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint32_t k = base + u * kBlock + threadIdx.x;
+                const uint32_t k = base + u * NT + threadIdx.x;
                 if (k < count) prod[k] = static_cast<double>(w[u]) * static_cast<double>(g[u]);
             }
         }
         __syncthreads();
         if constexpr (G2 == 1) {
-            for (uint32_t r = R0 + threadIdx.x; r < R1; r += kBlock) {
+            for (uint32_t r = R0 + threadIdx.x; r < R1; r += NT) {
                 const uint32_t e0 = s.rowPtr[r] - E0, e1 = s.rowPtr[r + 1] - E0;
                 const double pd = static_cast<double>(p[r]);
                 double sum = static_cast<double>(diag[r]) * pd;
@@ -445,7 +453,7 @@ __global__ void __launch_bounds__(kBlock) cgSpmvFlat(CgArgs a, int parity) {
         } else {
             // G2 lanes per row: lane l adds products l, l + G2, ... of the row,
             // then a fixed butterfly; shorter waits on long rows, another order
-            constexpr uint32_t kGroups = kBlock / G2;
+            constexpr uint32_t kGroups = NT / G2;
             const uint32_t lane = threadIdx.x % G2, grp = threadIdx.x / G2;
             for (uint32_t r0 = R0; r0 < R1; r0 += kGroups) {
                 const uint32_t r = r0 + grp;
@@ -465,7 +473,7 @@ __global__ void __launch_bounds__(kBlock) cgSpmvFlat(CgArgs a, int parity) {
             }
         }
     }
-    blockSum<1>(acc);
+    blockSum<1, NT>(acc);
     writePartial<1>(a.partA, a.stride, axis, b, acc);
 }
 
@@ -815,19 +823,35 @@ int launchSpmv(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, 
         const dim3 fg(flatGrid(a));
         const bool wide = ctx->tune.flatUnroll == 8;
         const bool lanes4 = ctx->tune.flatRowLanes == 4;
+        const int nt = flatThreads(ctx->tune.flatThreads);
 #define KT_FLAT(EE)                                                                                       \
     if (lanes4) {                                                                                         \
-        if (wide) cgSpmvFlat<T, EE, 8, 4><<<fg, dim3(kBlock), 0, st>>>(a, parity);                        \
-        else cgSpmvFlat<T, EE, 4, 4><<<fg, dim3(kBlock), 0, st>>>(a, parity);                             \
+        if (wide) cgSpmvFlat<T, EE, 8, 4, kBlock><<<fg, dim3(kBlock), 0, st>>>(a, parity);                \
+        else cgSpmvFlat<T, EE, 4, 4, kBlock><<<fg, dim3(kBlock), 0, st>>>(a, parity);                     \
     } else {                                                                                              \
-        if (wide) cgSpmvFlat<T, EE, 8, 1><<<fg, dim3(kBlock), 0, st>>>(a, parity);                        \
-        else cgSpmvFlat<T, EE, 4, 1><<<fg, dim3(kBlock), 0, st>>>(a, parity);                             \
+        if (wide) cgSpmvFlat<T, EE, 8, 1, kBlock><<<fg, dim3(kBlock), 0, st>>>(a, parity);                \
+        else cgSpmvFlat<T, EE, 4, 1, kBlock><<<fg, dim3(kBlock), 0, st>>>(a, parity);                     \
     }
-        switch (a.flatEntries) {
-            case 4096: KT_FLAT(4096) break;
-            case 16384: KT_FLAT(16384) break;
-            default: KT_FLAT(8192) break;
+#define KT_FLAT_SMALL(EE, NN)                                                                             \
+    if (lanes4) cgSpmvFlat<T, EE, 4, 4, NN><<<fg, dim3(NN), 0, st>>>(a, parity);                          \
+    else cgSpmvFlat<T, EE, 4, 1, NN><<<fg, dim3(NN), 0, st>>>(a, parity);
+        if (nt == 256) {
+            if (wide) {
+                if (lanes4) cgSpmvFlat<T, 2048, 8, 4, 256><<<fg, dim3(256), 0, st>>>(a, parity);
+                else cgSpmvFlat<T, 2048, 8, 1, 256><<<fg, dim3(256), 0, st>>>(a, parity);
+            } else {
+                KT_FLAT_SMALL(2048, 256)
+            }
+        } else if (nt == 512) {
+            KT_FLAT_SMALL(4096, 512)
+        } else {
+            switch (a.flatEntries) {
+                case 4096: KT_FLAT(4096) break;
+                case 16384: KT_FLAT(16384) break;
+                default: KT_FLAT(8192) break;
+            }
         }
+#undef KT_FLAT_SMALL
 #undef KT_FLAT
         KT_HIP(hipGetLastError());
         return KT_OK;
@@ -1052,7 +1076,7 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
     a.xcdSplit = ctx->tune.xcdSplit == 0 ? 0u : 1u;
     a.halo = ctx->tune.halo;
     a.layout = ctx->tune.layout == KT_SPMV_ENTRY_BLOCKS ? 1u : 0u;
-    a.flatEntries = ctx->tune.flatEntries ? ctx->tune.flatEntries : 8192u;
+    a.flatEntries = ctx->tune.flatEntries ? ctx->tune.flatEntries : 8u * static_cast<uint32_t>(flatThreads(ctx->tune.flatThreads));
     a.flatUnroll = ctx->tune.flatUnroll == 8 ? 8u : 4u;
 
     KT_CHECK(!(a.layout && a.halo), KT_EINVAL, "solve: the entry-block SpMV has no LDS window");

@@ -161,6 +161,7 @@ struct Tune {
     uint32_t flatEntries = 0; // entry blocks: entries per block (0: 8192)
     int flatUnroll = 0;       // entry blocks: entries per thread and round (0: 4)
     int flatRowLanes = 0;     // entry blocks: lanes per row in the row sums (0, 1: entry order; 4)
+    int flatThreads = 0;      // entry blocks: threads per block (0: 1024; 256, 512)
 };
 
 // A captured hipGraph of kPairs iteration pairs for one argument set.

@@ -344,8 +344,10 @@ def test_entry_block_layout_solves_and_repeats(b2b12k, gpu_ctx_factory, precisio
     ctx.build_b2b(x0, y0)
     ref = o.solve(x0, y0, tol=1e-12, max_iter=100000)
     runs = []
-    for layout, row_lanes in ((0, 0), (KT_SPMV_ENTRY_BLOCKS, 4), (KT_SPMV_ENTRY_BLOCKS, 4), (KT_SPMV_ENTRY_BLOCKS, 1)):
-        ctx.set_tuning(lanes=4, tile=2048, layout=layout, row_lanes=row_lanes)
+    cases = ((0, 0, 0), (KT_SPMV_ENTRY_BLOCKS, 4, 0), (KT_SPMV_ENTRY_BLOCKS, 4, 0), (KT_SPMV_ENTRY_BLOCKS, 1, 0),
+             (KT_SPMV_ENTRY_BLOCKS, 4, 256), (KT_SPMV_ENTRY_BLOCKS, 4, 256), (KT_SPMV_ENTRY_BLOCKS, 1, 512))
+    for layout, row_lanes, threads in cases:
+        ctx.set_tuning(lanes=4, tile=2048, layout=layout, row_lanes=row_lanes, block_threads=threads)
         x, y = x0.copy(), y0.copy()
         r = ctx.solve_both(x, y, 1e-9, 100000, precision=prec, stop_rule=KT_STOP_TRUE)
         runs.append((x, y, r))
@@ -353,15 +355,16 @@ def test_entry_block_layout_solves_and_repeats(b2b12k, gpu_ctx_factory, precisio
         assert rel(x, ref["x"]) < 1e-6 and rel(y, ref["y"]) < 1e-6
         assert max(r["true_resid"]) <= 1e-9
         assert all(abs(a - b) <= 1 for a, b in zip(r["steps"], runs[0][2]["steps"]))
-    assert np.array_equal(runs[1][0], runs[2][0]) and np.array_equal(runs[1][1], runs[2][1])
+    for a, b in ((1, 2), (4, 5)):
+        assert np.array_equal(runs[a][0], runs[b][0]) and np.array_equal(runs[a][1], runs[b][1])
     ctx.close()
 
 
 def test_entry_block_layout_long_rows(gpu_ctx_factory):
     """Rows longer than an entry block's LDS share (a hub row of 12,000 entries
     and one of 9,000 crossing the block cuts, done by the lane-group path), empty
-    rows and a last block of a few entries, for each block size and unroll:
-    the solution matches scipy."""
+    rows and a last block of a few entries, for each block size, thread count
+    and unroll: the solution matches scipy."""
     from scipy.sparse.linalg import spsolve
     import scipy.sparse as sp
     from ktplace_amd.simpl import KT_PREC_FP64, KT_PREC_MIXED, KT_SPMV_ENTRY_BLOCKS, KT_STOP_TRUE
@@ -386,11 +389,12 @@ def test_entry_block_layout_long_rows(gpu_ctx_factory):
     ctx = gpu_ctx_factory()
     ctx.set_system(0, s)
     ctx.set_system(1, s)
-    for prec, entries, unroll, row_lanes in ((KT_PREC_FP64, 0, 0, 0), (KT_PREC_MIXED, 0, 0, 4),
-                                             (KT_PREC_FP64, 4096, 8, 4), (KT_PREC_MIXED, 16384, 8, 1),
-                                             (KT_PREC_FP64, 16384, 4, 4)):
+    for prec, entries, unroll, row_lanes, threads in (
+            (KT_PREC_FP64, 0, 0, 0, 0), (KT_PREC_MIXED, 0, 0, 4, 0), (KT_PREC_FP64, 4096, 8, 4, 0),
+            (KT_PREC_MIXED, 16384, 8, 1, 0), (KT_PREC_FP64, 16384, 4, 4, 0), (KT_PREC_FP64, 2048, 4, 4, 256),
+            (KT_PREC_MIXED, 0, 8, 1, 256), (KT_PREC_MIXED, 4096, 4, 4, 512), (KT_PREC_FP64, 0, 0, 1, 512)):
         ctx.set_tuning(lanes=4, tile=1024, layout=KT_SPMV_ENTRY_BLOCKS, block_entries=entries, unroll=unroll,
-                       row_lanes=row_lanes)
+                       row_lanes=row_lanes, block_threads=threads)
         x, y = np.zeros(n), np.zeros(n)
         r = ctx.solve_both(x, y, 1e-10, 100000, precision=prec, stop_rule=KT_STOP_TRUE)
         assert max(r["true_resid"]) <= 1e-10
