@@ -10,6 +10,8 @@
 //
 // Kernels (both axes per launch, blockIdx.y = axis where it applies):
 //   vertexPositions  px/py = fixed positions with movable cells overwritten (:14-19)
+//   pinStage         per pin: position (vertex + offset), rhs term, variable
+//                    (then one record per net: pin range, weight base, lo/hi)
 //   netExtremes      first argmin / first argmax of pos + offset per net  (:35-45)
 //   rowCounts        movable neighbours per row                              (:75-82)
 //   scan             rowPtr = inclusive scan of the counts                  (:83)
@@ -35,47 +37,85 @@ vertexPositions(const double *__restrict__ x, const double *__restrict__ y,
     vy[v] = var != kNoVar ? y[var] : fixedY[v];
 }
 
+// One thread per pin, both axes.  The pin's position (pinAt: vertex position
+// + offset, simpl_lss.cc:28-33), its variable (kNoVar for a fixed vertex) and
+// the term an edge to it adds to rhs before the owner's offset is taken off:
+// position - vertex position for a movable vertex, the position itself for a
+// fixed one (simpl_lss.cc:96-104).  These are the same double expressions the
+// per-entry code would evaluate, once per pin instead of once per CSR entry
+// that names the pin, so the entry loops below read them in net order
+// instead of gathering vertex data at random.
+// Stored per axis as one (position, rhs term) pair per pin: the row kernels
+// read both at once, at random for a net's lo/hi pins and a row's own pin.
+__global__ void __launch_bounds__(kBlock)
+pinStage(const uint32_t *__restrict__ pinCell, const uint32_t *__restrict__ varOfVertex,
+         const double *__restrict__ offX, const double *__restrict__ offY, const double *__restrict__ vx,
+         const double *__restrict__ vy, double2 *__restrict__ pinX, double2 *__restrict__ pinY,
+         uint32_t *__restrict__ pinVar, uint64_t P) {
+    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (k >= P) return;
+    const uint32_t c = pinCell[k];
+    const uint32_t var = varOfVertex[c];
+    const double cx = vx[c], cy = vy[c];
+    const double px = cx + offX[k], py = cy + offY[k];
+    pinX[k] = make_double2(px, var != kNoVar ? px - cx : px);
+    pinY[k] = make_double2(py, var != kNoVar ? py - cy : py);
+    pinVar[k] = var;
+}
+
+// Everything a row's reference to a net reads about the net, in one 64-byte
+// record (one memory sector): its pin range, movable pins and edge weight
+// base (weight / (pins - 1), simpl_lss.cc:58), and this build's first argmin /
+// argmax per axis with their pins' variables.  The row kernels visit nets in
+// random order, so one sector per visit instead of one per array.
+struct alignas(64) NetRec {
+    uint32_t start, size, mov, pad;
+    uint32_t lox, hix, loy, hiy;
+    uint32_t varLoX, varHiX, varLoY, varHiY;
+    double base;
+    double unused;
+};
+static_assert(sizeof(NetRec) == 64, "one sector per net");
+
 // One thread per net; the running best value is the reference's pinAt(lo),
 // recomputed there from the same operands, so the comparisons are identical.
 __global__ void __launch_bounds__(kBlock)
-netExtremes(const uint32_t *__restrict__ netStart, const uint32_t *__restrict__ pinCell,
-            const double *__restrict__ offX, const double *__restrict__ offY,
-            const double *__restrict__ vx, const double *__restrict__ vy, uint4 *__restrict__ ext,
-            uint32_t N) {
+netExtremes(const uint32_t *__restrict__ netStart, const uint32_t *__restrict__ netMov,
+            const double *__restrict__ weight, const double2 *__restrict__ pinX, const double2 *__restrict__ pinY,
+            const uint32_t *__restrict__ pinVar, NetRec *__restrict__ rec, uint32_t N) {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= N) return;
     const uint32_t b = netStart[j];
     const uint32_t p = netStart[j + 1] - b;
     uint32_t lox = 0, hix = 0, loy = 0, hiy = 0;
     if (p > 0) {
-        const uint32_t c0 = pinCell[b];
-        double bestLoX = vx[c0] + offX[b], bestHiX = bestLoX;
-        double bestLoY = vy[c0] + offY[b], bestHiY = bestLoY;
+        double bestLoX = pinX[b].x, bestHiX = bestLoX;
+        double bestLoY = pinY[b].x, bestHiY = bestLoY;
         for (uint32_t q = 1; q < p; ++q) {
-            const uint32_t c = pinCell[b + q];
-            const double px = vx[c] + offX[b + q];
-            const double py = vy[c] + offY[b + q];
+            const double px = pinX[b + q].x;
+            const double py = pinY[b + q].x;
             if (px < bestLoX) { lox = q; bestLoX = px; }
             if (px > bestHiX) { hix = q; bestHiX = px; }
             if (py < bestLoY) { loy = q; bestLoY = py; }
             if (py > bestHiY) { hiy = q; bestHiY = py; }
         }
     }
-    ext[j] = make_uint4(lox, hix, loy, hiy);
+    uint4 *dst = reinterpret_cast<uint4 *>(rec + j);
+    dst[0] = make_uint4(b, p, netMov[j], 0u);
+    dst[1] = make_uint4(lox, hix, loy, hiy);
+    dst[2] = p > 0 ? make_uint4(pinVar[b + lox], pinVar[b + hix], pinVar[b + loy], pinVar[b + hiy])
+                   : make_uint4(kNoVar, kNoVar, kNoVar, kNoVar);
+    // used only where lo != hi, i.e. p >= 2
+    rec[j].base = weight[j] / static_cast<double>(p - 1);
 }
 
 struct RowArgs {
     const uint32_t *pinStart;
     const uint32_t *refNet;
     const uint32_t *refPin;
-    const uint32_t *netStart;
-    const uint32_t *pinCell;
-    const uint32_t *varOfVertex;
-    const uint32_t *netMov;
-    const uint4 *ext;
-    const double *weight;
-    const double *off[2];
-    const double *vpos[2];
+    const uint32_t *pinVar;
+    const NetRec *rec;
+    const double2 *pin[2];  // per pin: position, rhs term (pinStage)
     const double *pos[2];
     const double *anchor[2];
     uint32_t *rowPtr[2];
@@ -90,6 +130,15 @@ struct RowArgs {
 
 __device__ __forceinline__ uint32_t extLo(const uint4 &e, int axis) { return axis ? e.z : e.x; }
 __device__ __forceinline__ uint32_t extHi(const uint4 &e, int axis) { return axis ? e.w : e.y; }
+__device__ __forceinline__ uint4 recHead(const NetRec *rec, uint32_t net) {
+    return reinterpret_cast<const uint4 *>(rec + net)[0];  // start, size, mov
+}
+__device__ __forceinline__ uint4 recExt(const NetRec *rec, uint32_t net) {
+    return reinterpret_cast<const uint4 *>(rec + net)[1];  // lo_x, hi_x, lo_y, hi_y
+}
+__device__ __forceinline__ uint4 recVars(const NetRec *rec, uint32_t net) {
+    return reinterpret_cast<const uint4 *>(rec + net)[2];  // variables of lo_x, hi_x, lo_y, hi_y
+}
 
 // std::max(a, b) is (a < b) ? b : a; written out so NaN behaves the same.
 __device__ __forceinline__ double refMax(double a, double b) { return a < b ? b : a; }
@@ -102,15 +151,14 @@ __global__ void __launch_bounds__(kBlock) rowCounts(RowArgs a) {
     for (uint32_t e = a.pinStart[i]; e < a.pinStart[i + 1]; ++e) {
         const uint32_t net = a.refNet[e];
         const uint32_t q = a.refPin[e];
-        const uint4 x = a.ext[net];
+        const uint4 head = recHead(a.rec, net), x = recExt(a.rec, net);
         const uint32_t lo = extLo(x, axis), hi = extHi(x, axis);
         if (lo == hi) continue;
         if (q == lo || q == hi) {
-            count += a.netMov[net] - 1;  // every other pin; q itself is movable
+            count += head.z - 1;  // every other movable pin; q itself is movable
         } else {
-            const uint32_t b = a.netStart[net];
-            count += (a.varOfVertex[a.pinCell[b + lo]] != kNoVar ? 1u : 0u) +
-                     (a.varOfVertex[a.pinCell[b + hi]] != kNoVar ? 1u : 0u);
+            const uint4 v = recVars(a.rec, net);
+            count += (extLo(v, axis) != kNoVar ? 1u : 0u) + (extHi(v, axis) != kNoVar ? 1u : 0u);
         }
     }
     a.rowPtr[axis][i + 1] = count;
@@ -121,8 +169,8 @@ __global__ void __launch_bounds__(kBlock) rowFill(RowArgs a) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const int axis = blockIdx.y;
     if (i >= a.n) return;
-    const double *__restrict__ pos = a.vpos[axis];
-    const double *__restrict__ off = a.off[axis];
+    const double2 *__restrict__ pin = a.pin[axis];
+    const uint32_t *__restrict__ pinVar = a.pinVar;
     const double lengthFloor = a.floor[axis];
     uint32_t *__restrict__ col = a.col[axis];
     double *__restrict__ val = a.val[axis];
@@ -131,38 +179,36 @@ __global__ void __launch_bounds__(kBlock) rowFill(RowArgs a) {
     for (uint32_t e = a.pinStart[i]; e < a.pinStart[i + 1]; ++e) {
         const uint32_t net = a.refNet[e];
         const uint32_t q = a.refPin[e];
-        const uint4 x = a.ext[net];
+        const uint4 head = recHead(a.rec, net), x = recExt(a.rec, net);
         const uint32_t lo = extLo(x, axis), hi = extHi(x, axis);
         if (lo == hi) continue;
-        const uint32_t b = a.netStart[net];
-        const uint32_t p = a.netStart[net + 1] - b;
-        const double base = a.weight[net] / static_cast<double>(p - 1);
-        const double ownPos = pos[a.pinCell[b + q]];
-        const double own = ownPos + off[b + q];
-        const double ownOffset = own - ownPos;
-        auto edge = [&](uint32_t other) {
-            const uint32_t cell = a.pinCell[b + other];
-            const double cellPos = pos[cell];
-            const double there = cellPos + off[b + other];
-            const double w = base / refMax(fabs(own - there), lengthFloor);
-            const uint32_t var = a.varOfVertex[cell];
+        const uint32_t b = head.x;
+        const uint32_t p = head.y;
+        const double base = a.rec[net].base;  // weight / (p - 1)
+        // q's vertex is this movable row: .y = own - its vertex position
+        const double2 ownPin = pin[b + q];
+        const double own = ownPin.x;
+        const double ownOffset = ownPin.y;
+        auto edge = [&](uint32_t other, uint32_t var) {
+            const double2 there = pin[b + other];
+            const double w = base / refMax(fabs(own - there.x), lengthFloor);
             diag += w;
             if (var != kNoVar) {
                 col[slot] = var;
                 val[slot] = -w;
                 ++slot;
-                rhs += w * ((there - cellPos) - ownOffset);
-            } else {
-                rhs += w * (there - ownOffset);
             }
+            // movable: (there - cellPos) - ownOffset; fixed: there - ownOffset
+            rhs += w * (there.y - ownOffset);
         };
         if (q == lo || q == hi) {
             for (uint32_t other = 0; other < p; ++other) {
-                if (other != q) edge(other);
+                if (other != q) edge(other, pinVar[b + other]);
             }
         } else {
-            edge(lo);
-            edge(hi);
+            const uint4 v = recVars(a.rec, net);
+            edge(lo, extLo(v, axis));
+            edge(hi, extHi(v, axis));
         }
     }
     if (a.alpha > 0.0) {
@@ -284,29 +330,33 @@ int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY) {
     KT_CHECK(!(alpha > 0.0) || ctx->haveAnchors, KT_ESTATE, "buildB2B: alpha > 0 needs anchors");
     const uint32_t n = ctx->n;
     for (int axis = 0; axis < 2; ++axis) KT_TRY(ensureSystem(ctx, axis, n, ctx->nnzBound));
-    KT_TRY(ctx->ext.reserve(std::max<uint64_t>(4ull * ctx->N, 4)));
+    KT_TRY(ctx->ext.reserve(std::max<uint64_t>(16ull * ctx->N, 16)));  // one NetRec per net
     KT_TRY(vertexPositionsLaunch(ctx));
-    uint4 *ext = reinterpret_cast<uint4 *>(ctx->ext.p);
+    const uint64_t pins = std::max<uint64_t>(ctx->P, 1);
+    for (int axis = 0; axis < 2; ++axis) KT_TRY(ctx->pinAt[axis].reserve(2 * pins));
+    KT_TRY(ctx->pinVar.reserve(pins));
+    double2 *pinX = reinterpret_cast<double2 *>(ctx->pinAt[0].p);
+    double2 *pinY = reinterpret_cast<double2 *>(ctx->pinAt[1].p);
+    if (ctx->P > 0) {
+        pinStage<<<dim3(static_cast<uint32_t>((ctx->P + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream>>>(
+            ctx->pinCell.p, ctx->varOfVertex.p, ctx->offX.p, ctx->offY.p, ctx->vx.p, ctx->vy.p, pinX, pinY,
+            ctx->pinVar.p, ctx->P);
+        KT_HIP(hipGetLastError());
+    }
+    NetRec *rec = reinterpret_cast<NetRec *>(ctx->ext.p);
     if (ctx->N > 0) {
         netExtremes<<<dim3((ctx->N + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream>>>(
-            ctx->netStart.p, ctx->pinCell.p, ctx->offX.p, ctx->offY.p, ctx->vx.p, ctx->vy.p, ext,
-            ctx->N);
+            ctx->netStart.p, ctx->netMov.p, ctx->weight.p, pinX, pinY, ctx->pinVar.p, rec, ctx->N);
         KT_HIP(hipGetLastError());
     }
     RowArgs a{};
     a.pinStart = ctx->pinStart.p;
     a.refNet = ctx->refNet.p;
     a.refPin = ctx->refPin.p;
-    a.netStart = ctx->netStart.p;
-    a.pinCell = ctx->pinCell.p;
-    a.varOfVertex = ctx->varOfVertex.p;
-    a.netMov = ctx->netMov.p;
-    a.ext = ext;
-    a.weight = ctx->weight.p;
-    a.off[0] = ctx->offX.p;
-    a.off[1] = ctx->offY.p;
-    a.vpos[0] = ctx->vx.p;
-    a.vpos[1] = ctx->vy.p;
+    a.pinVar = ctx->pinVar.p;
+    a.rec = rec;
+    a.pin[0] = pinX;
+    a.pin[1] = pinY;
     a.pos[0] = ctx->posX.p;
     a.pos[1] = ctx->posY.p;
     a.anchor[0] = ctx->anchorX.p;

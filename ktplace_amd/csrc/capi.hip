@@ -339,7 +339,8 @@ int kt_ctx_destroy(kt_ctx *ctx) {
     ctx->fixedY.release(); ctx->width.release(); ctx->height.release(); ctx->area.release();
     ctx->posX.release(); ctx->posY.release(); ctx->saveX.release(); ctx->saveY.release();
     ctx->anchorX.release(); ctx->anchorY.release(); ctx->vx.release(); ctx->vy.release();
-    ctx->ext.release(); ctx->scanTmp.release();
+    ctx->ext.release(); ctx->scanTmp.release(); ctx->pinVar.release();
+    for (int a = 0; a < 2; ++a) ctx->pinAt[a].release();
     for (auto &s : ctx->sys) {
         s.rowPtr.release(); s.col.release(); s.val.release(); s.diag.release(); s.rhs.release();
         s.val32.release(); s.diag32.release();
@@ -377,6 +378,8 @@ int kt_ctx_device_bytes(kt_ctx *ctx, uint64_t *bytes) {
     b += ctx->width.bytes() + ctx->height.bytes() + ctx->area.bytes();
     b += ctx->posX.bytes() + ctx->posY.bytes() + ctx->saveX.bytes() + ctx->saveY.bytes();
     b += ctx->anchorX.bytes() + ctx->anchorY.bytes() + ctx->vx.bytes() + ctx->vy.bytes() + ctx->ext.bytes();
+    b += ctx->pinAt[0].bytes() + ctx->pinAt[1].bytes();
+    b += ctx->pinVar.bytes();
     b += ctx->scanTmp.bytes();
     for (auto &s : ctx->sys)
         b += s.rowPtr.bytes() + s.col.bytes() + s.val.bytes() + s.diag.bytes() + s.rhs.bytes() + s.val32.bytes() +

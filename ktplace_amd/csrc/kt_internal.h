@@ -212,7 +212,9 @@ struct kt_ctx {
     kt::DevBuf<double> posX, posY, saveX, saveY, anchorX, anchorY;
     bool haveAnchors = false;
     kt::DevBuf<double> vx, vy;      // vertex-indexed positions (px, py)
-    kt::DevBuf<uint32_t> ext;       // [N][4]: lo_x, hi_x, lo_y, hi_y
+    kt::DevBuf<uint32_t> ext;       // [N][16]: per-net record of the B2B build (pins, weight, lo/hi per axis)
+    kt::DevBuf<double> pinAt[2];    // per pin and axis: (vertex position + offset, rhs term of an edge to it)
+    kt::DevBuf<uint32_t> pinVar;    // per pin: its vertex's variable (kNoVar: fixed)
     kt::DevBuf<uint32_t> scanTmp;   // block sums of the u32 scans
     kt::DevBuf<uint64_t> scanTmp64; // block sums of the u64 scans
 
