@@ -38,10 +38,10 @@ vertexPositions(const double *__restrict__ x, const double *__restrict__ y,
 }
 
 // One thread per pin, both axes.  The pin's position (pinAt: vertex position
-// + offset, simpl_lss.cc:28-33), its variable (kNoVar for a fixed vertex) and
+// + offset, simpl_lss.cc:32-34), its variable (kNoVar for a fixed vertex) and
 // the term an edge to it adds to rhs before the owner's offset is taken off:
 // position - vertex position for a movable vertex, the position itself for a
-// fixed one (simpl_lss.cc:96-104).  These are the same double expressions the
+// fixed one (simpl_lss.cc:94-104).  These are the same double expressions the
 // per-entry code would evaluate, once per pin instead of once per CSR entry
 // that names the pin, so the entry loops below read them in net order
 // instead of gathering vertex data at random.
@@ -65,7 +65,7 @@ pinStage(const uint32_t *__restrict__ pinCell, const uint32_t *__restrict__ varO
 
 // Everything a row's reference to a net reads about the net, in one 64-byte
 // record (one memory sector): its pin range, movable pins and edge weight
-// base (weight / (pins - 1), simpl_lss.cc:58), and this build's first argmin /
+// base (weight / (pins - 1), simpl_lss.cc:55), and this build's first argmin /
 // argmax per axis with their pins' variables.  The row kernels visit nets in
 // random order, so one sector per visit instead of one per array.
 struct alignas(64) NetRec {

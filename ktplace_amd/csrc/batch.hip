@@ -325,7 +325,7 @@ __global__ void __launch_bounds__(kBB) bPUpdate(BArgs a, int parity) {
         if (lead) c.s[parity ^ 1] = st;
         return;
     }
-    if (c.brk[parity]) {  // p.Ap <= 0 stops the system (simpl_lss.cc:331-334)
+    if (c.brk[parity]) {  // p.Ap <= 0 stops the system (simpl_lss.cc:242-244)
         if (lead) {
             CgState nx = st;
             nx.active = 0;

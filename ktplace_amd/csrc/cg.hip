@@ -587,7 +587,7 @@ __global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
     if (!st.live) return;
     const SysView s = a.sys[axis];
     const size_t hist = static_cast<size_t>(axis) * a.histCap + st.steps;
-    if (c.brk[parity]) {  // p.Ap <= 0: the reference stops the axis (simpl_lss.cc:331-334)
+    if (c.brk[parity]) {  // p.Ap <= 0: the reference stops the axis (simpl_lss.cc:242-244)
         if (lead) {
             CgState nx = st;
             nx.active = 0;
@@ -821,9 +821,9 @@ int launchSpmv(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, 
     if (a.layout == 1) {
         if (attributesOnly) return KT_OK;
         const dim3 fg(flatGrid(a));
-        const bool wide = ctx->tune.flatUnroll == 8;
-        const bool lanes4 = ctx->tune.flatRowLanes == 4;
-        const int nt = flatThreads(ctx->tune.flatThreads);
+        const bool wide = a.flatUnroll == 8;
+        const bool lanes4 = a.flatRowLanes == 4;
+        const int nt = static_cast<int>(a.flatThreads);
 #define KT_FLAT(EE)                                                                                       \
     if (lanes4) {                                                                                         \
         if (wide) cgSpmvFlat<T, EE, 8, 4, kBlock><<<fg, dim3(kBlock), 0, st>>>(a, parity);                \
@@ -1078,6 +1078,10 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
     a.layout = ctx->tune.layout == KT_SPMV_ENTRY_BLOCKS ? 1u : 0u;
     a.flatEntries = ctx->tune.flatEntries ? ctx->tune.flatEntries : 8u * static_cast<uint32_t>(flatThreads(ctx->tune.flatThreads));
     a.flatUnroll = ctx->tune.flatUnroll == 8 ? 8u : 4u;
+    // every launch-shape choice lives in the arguments: the graph cache
+    // compares them to decide whether its captured iterations still apply
+    a.flatRowLanes = ctx->tune.flatRowLanes == 4 ? 4u : 1u;
+    a.flatThreads = static_cast<uint32_t>(flatThreads(ctx->tune.flatThreads));
 
     KT_CHECK(!(a.layout && a.halo), KT_EINVAL, "solve: the entry-block SpMV has no LDS window");
     if (a.halo) {

@@ -145,7 +145,8 @@ struct CgArgs {
     uint32_t layout;    // SpMV: 0 lane groups per row, 1 entry blocks (KT_SPMV_ENTRY_BLOCKS)
     uint32_t flatEntries;  // entry blocks: entries per block (4096, 8192, 16384)
     uint32_t flatUnroll;   // entry blocks: entries per thread and round (4, 8)
-    uint32_t pad;
+    uint32_t flatRowLanes; // entry blocks: lanes per row in the row sums (1, 4)
+    uint32_t flatThreads;  // entry blocks: threads per block (256, 512, 1024)
 };
 
 // Launch-shape knobs (set by the autotuner or the caller; 0 = default).

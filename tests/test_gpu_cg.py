@@ -357,6 +357,17 @@ def test_entry_block_layout_solves_and_repeats(b2b12k, gpu_ctx_factory, precisio
         assert all(abs(a - b) <= 1 for a, b in zip(r["steps"], runs[0][2]["steps"]))
     for a, b in ((1, 2), (4, 5)):
         assert np.array_equal(runs[a][0], runs[b][0]) and np.array_equal(runs[a][1], runs[b][1])
+    # a change of row lanes or block threads alone must not replay the graph
+    # captured for the previous shape: graph replay equals direct launches
+    for row_lanes, threads in ((4, 256), (1, 256), (1, 1024), (4, 1024)):
+        out = []
+        for graphs in (True, False):
+            ctx.set_tuning(lanes=4, tile=2048, layout=KT_SPMV_ENTRY_BLOCKS, row_lanes=row_lanes,
+                           block_threads=threads, use_graphs=graphs)
+            x, y = x0.copy(), y0.copy()
+            ctx.solve_both(x, y, 1e-9, 100000, precision=prec, stop_rule=KT_STOP_TRUE)
+            out.append((x, y))
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     ctx.close()
 
 
