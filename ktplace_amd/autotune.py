@@ -29,7 +29,7 @@ HALO_TILES = (1024, 2048, 4096)
 # entry-block SpMV: (threads per block, entries per block) shapes
 FLAT_SHAPES = ((256, 2048), (512, 4096), (1024, 8192))
 FLAT_ROW_LANES = (1, 4)      # entry-block SpMV: lanes per row in the row sums
-FLAT_TILES = (2048, 4096)    # tiles (of the vector kernels) tried with entry blocks
+FLAT_TILES = (2048, 4096, 8192)  # tiles (of the vector kernels) tried with entry blocks
 WINDOW_BYTES = 156 * 1024  # LDS a staged SpMV block may take
 
 _cache: dict = {}

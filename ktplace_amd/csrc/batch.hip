@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(kBB) bSpmv(BArgs a, int parity) {
 // row add the row's products (G2 = 1: one thread, entry order after the
 // diagonal term).  A run longer than the LDS holds takes lane groups.
 constexpr uint32_t kBFlat = 2048;
-constexpr uint32_t kBFlatSlack = 512;
+constexpr uint32_t kBFlatSlack = 256;  // an eighth of the block: eight blocks fit a CU's LDS
 
 template <int G2>
 __global__ void __launch_bounds__(kBB) bSpmvFlat(BArgs a, int parity) {

@@ -388,8 +388,8 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
     T *__restrict__ ap = static_cast<T *>(a.ap[axis]);
     const T *__restrict__ val = Mat<T>::val(s);
     const T *__restrict__ diag = Mat<T>::diag(s);
-    const uint32_t R0 = s.rowStart[b], R1 = s.rowStart[b + 1];
-    const uint32_t E0 = s.rowPtr[R0], count = s.rowPtr[R1] - E0;
+    const uint4 info = s.flatInfo[b];
+    const uint32_t R0 = info.x, R1 = info.y, E0 = info.z, count = info.w - info.z;
     double acc[1] = {0.0};
     if (count > kFlatEntries + kFlatSlack) {
         constexpr int G = 4, R = 2;
@@ -415,6 +415,16 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
             }
         }
     } else {
+        // G2 lanes per row in phase 2 (1: the reference's order).  The first
+        // pass's row data is loaded before phase 1, so it arrives while the
+        // products are gathered (a block of E entries usually has fewer than
+        // NT / G2 rows).  Row 0 stands in for a missing row (rows > 0 here).
+        constexpr uint32_t kGroups = NT / G2;
+        const uint32_t lane = threadIdx.x % G2, grp = threadIdx.x / G2;
+        const uint32_t rq = R0 + grp < R1 ? R0 + grp : 0u;
+        const uint32_t pre0 = s.rowPtr[rq], pre1 = s.rowPtr[rq + 1];
+        const T preP = p[rq], preD = diag[rq];
+        __builtin_amdgcn_sched_barrier(0);
         for (uint32_t base = 0; base < count; base += U * NT) {
             uint32_t c[U];
             T w[U];
@@ -440,33 +450,41 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
             }
         }
         __syncthreads();
-        if constexpr (G2 == 1) {
-            for (uint32_t r = R0 + threadIdx.x; r < R1; r += NT) {
-                const uint32_t e0 = s.rowPtr[r] - E0, e1 = s.rowPtr[r + 1] - E0;
-                const double pd = static_cast<double>(p[r]);
-                double sum = static_cast<double>(diag[r]) * pd;
-                for (uint32_t k = e0; k < e1; ++k) sum += prod[k];
-                const T v = static_cast<T>(sum);
-                ap[r] = v;
-                acc[0] += pd * static_cast<double>(v);
+        bool first = true;
+        for (uint32_t r0 = R0; r0 < R1; r0 += kGroups) {
+            const uint32_t r = r0 + grp;
+            const bool valid = r < R1;
+            uint32_t q0 = pre0, q1 = pre1;
+            T pr = preP, dr = preD;
+            if (!first) {
+                const uint32_t rr = valid ? r : 0u;
+                q0 = s.rowPtr[rr];
+                q1 = s.rowPtr[rr + 1];
+                pr = p[rr];
+                dr = diag[rr];
             }
-        } else {
-            // G2 lanes per row: lane l adds products l, l + G2, ... of the row,
-            // then a fixed butterfly; shorter waits on long rows, another order
-            constexpr uint32_t kGroups = NT / G2;
-            const uint32_t lane = threadIdx.x % G2, grp = threadIdx.x / G2;
-            for (uint32_t r0 = R0; r0 < R1; r0 += kGroups) {
-                const uint32_t r = r0 + grp;
-                const bool valid = r < R1;
-                const uint32_t rr = valid ? r : R0;
-                const uint32_t e0 = s.rowPtr[rr] - E0, e1 = valid ? s.rowPtr[rr + 1] - E0 : e0;
+            first = false;
+            const uint32_t e0 = q0 - E0, e1 = valid ? q1 - E0 : e0;
+            const double pd = static_cast<double>(pr);
+            if constexpr (G2 == 1) {
+                // d p first, then the row's products in entry order
+                // (SparseSystem::multiply, simpl_lss.h:49-57)
+                double sum = static_cast<double>(dr) * pd;
+                for (uint32_t k = e0; k < e1; ++k) sum += prod[k];
+                if (valid) {
+                    const T v = static_cast<T>(sum);
+                    ap[r] = v;
+                    acc[0] += pd * static_cast<double>(v);
+                }
+            } else {
+                // lane l adds products l, l + G2, ... of the row, then a fixed
+                // butterfly; shorter waits on long rows, another order
                 double part = 0.0;
                 for (uint32_t k = e0 + lane; k < e1; k += G2) part += prod[k];
 #pragma unroll
                 for (int off = G2 / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, G2);
                 if (valid && lane == 0) {
-                    const double pd = static_cast<double>(p[r]);
-                    const T v = static_cast<T>(static_cast<double>(diag[r]) * pd + part);
+                    const T v = static_cast<T>(static_cast<double>(dr) * pd + part);
                     ap[r] = v;
                     acc[0] += pd * static_cast<double>(v);
                 }
@@ -477,24 +495,31 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
     writePartial<1>(a.partA, a.stride, axis, b, acc);
 }
 
-// rowStart[b] = first row whose entries start at or after b * entries
-// (b = 1 .. blocks - 1); rowStart[0] = 0, rowStart[blocks] = rows.
-__global__ void __launch_bounds__(256) flatBounds(const uint32_t *__restrict__ rowPtr, uint32_t rows,
-                                                  uint32_t blocks, uint32_t entries, uint32_t *__restrict__ rowStart) {
-    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
-    if (b > blocks) return;
-    if (b == 0 || b == blocks) {
-        rowStart[b] = b == 0 ? 0u : rows;
-        return;
-    }
-    const uint64_t target = static_cast<uint64_t>(b) * entries;
+// Cut c (c = 1 .. blocks - 1) = first row whose entries start at or after
+// c * entries; cut 0 = 0, cut blocks = rows.
+__device__ __forceinline__ uint32_t flatCut(const uint32_t *__restrict__ rowPtr, uint32_t rows, uint32_t blocks,
+                                            uint32_t entries, uint32_t c) {
+    if (c == 0) return 0u;
+    if (c >= blocks) return rows;
+    const uint64_t target = static_cast<uint64_t>(c) * entries;
     uint32_t lo = 0, hi = rows;  // first r in [0, rows] with rowPtr[r] >= target
     while (lo < hi) {
         const uint32_t mid = lo + (hi - lo) / 2;
         if (rowPtr[mid] < target) lo = mid + 1;
         else hi = mid;
     }
-    rowStart[b] = lo;
+    return lo;
+}
+
+// info[b] = (first row, end row, first entry, end entry) of entry block b:
+// the block reads one record instead of a cut and then its row pointer.
+__global__ void __launch_bounds__(256) flatBounds(const uint32_t *__restrict__ rowPtr, uint32_t rows,
+                                                  uint32_t blocks, uint32_t entries, uint4 *__restrict__ info) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= blocks) return;
+    const uint32_t r0 = flatCut(rowPtr, rows, blocks, entries, b);
+    const uint32_t r1 = flatCut(rowPtr, rows, blocks, entries, b + 1);
+    info[b] = make_uint4(r0, r1, rowPtr[r0], rowPtr[r1]);
 }
 
 // alpha = rho / p.Ap; x += alpha p; r -= alpha Ap; tile partials of r.r, r.z.
@@ -900,6 +925,7 @@ int launchIteration(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t
         case 1 * kBlock: cgUpdate<T, 1><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
         case 2 * kBlock: cgUpdate<T, 2><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
         case 4 * kBlock: cgUpdate<T, 4><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
+        case 8 * kBlock: cgUpdate<T, 8><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
         default: cgUpdate<T, 0><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
     }
     KT_HIP(hipGetLastError());
@@ -909,6 +935,7 @@ int launchIteration(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t
         case 1 * kBlock: cgPUpdate<T, 1><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
         case 2 * kBlock: cgPUpdate<T, 2><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
         case 4 * kBlock: cgPUpdate<T, 4><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
+        case 8 * kBlock: cgPUpdate<T, 8><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
         default: cgPUpdate<T, 0><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
     }
     KT_HIP(hipGetLastError());
@@ -979,7 +1006,7 @@ SysView viewOf(kt_ctx *ctx, int axis, uint32_t tile) {
     v.diag32 = s.diag32.p;
     v.rows = static_cast<uint32_t>(s.rows);
     v.blocks = static_cast<uint32_t>((s.rows + tile - 1) / tile);
-    v.rowStart = nullptr;
+    v.flatInfo = nullptr;
     v.sblocks = v.blocks;
     return v;
 }
@@ -1128,11 +1155,12 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
             Sys &sy = ctx->sys[ax];
             const uint32_t E = a.flatEntries;
             const uint32_t sb = static_cast<uint32_t>(std::max<uint64_t>((nnz[ax] + E - 1) / E, 1));
-            KT_TRY(sy.rowStart.reserve(sb + 1));
-            flatBounds<<<dim3((sb + 1 + 255) / 256), dim3(256), 0, ctx->stream>>>(sy.rowPtr.p, static_cast<uint32_t>(rows),
-                                                                               sb, E, sy.rowStart.p);
+            KT_TRY(sy.rowStart.reserve(4ull * sb));
+            uint4 *info = reinterpret_cast<uint4 *>(sy.rowStart.p);
+            flatBounds<<<dim3((sb + 255) / 256), dim3(256), 0, ctx->stream>>>(sy.rowPtr.p, static_cast<uint32_t>(rows),
+                                                                          sb, E, info);
             KT_HIP(hipGetLastError());
-            a.sys[ax].rowStart = sy.rowStart.p;
+            a.sys[ax].flatInfo = info;
             a.sys[ax].sblocks = sb;
         }
         maxBlocks = std::max({maxBlocks, a.sys[ax].blocks, a.sys[ax].sblocks});

@@ -108,8 +108,9 @@ struct SysView {
     const float *diag32;
     uint32_t rows;
     uint32_t blocks;  // blocks this axis occupies in a two-axis launch
-    // entry-block SpMV layout: block b takes rows [rowStart[b], rowStart[b+1])
-    const uint32_t *rowStart;
+    // entry-block SpMV layout: block b takes rows [x, y) of flatInfo[b] =
+    // (first row, end row, first entry, end entry)
+    const uint4 *flatInfo;
     uint32_t sblocks;  // SpMV blocks (and p.Ap partials) of this axis
 };
 
@@ -121,7 +122,7 @@ struct Sys {
     DevBuf<double> val, diag, rhs;
     DevBuf<float> val32, diag32;
     bool have32 = false;
-    DevBuf<uint32_t> rowStart;  // entry-block SpMV: first row of each block (+ rows)
+    DevBuf<uint32_t> rowStart;  // entry-block SpMV: (first row, end row, first entry, end entry) per block
 };
 
 // Arguments of every CG kernel (both axes).
