@@ -121,22 +121,35 @@ __device__ __forceinline__ void sumPartials(const double *part, uint32_t blocks,
 // partial unconditionally (tile 0 exists) and consumes nothing, so loads
 // issued after it stay in flight while it is waited for; finish() selects it,
 // adds the rare further shares (more than 1024 tiles) and sums the block.
-template <int K>
+// S: shares per thread loaded at once (unconditional loads, index 0 standing
+// in for a missing tile); more than S * 1024 tiles are summed in a loop after
+// them.  The sum runs in tile order t, t + 1024, ... either way.
+template <int K, int S = 1>
 struct PartialShare {
     const double *src;
     uint32_t blocks;
-    double first[K];
+    double first[S][K];
     __device__ __forceinline__ void issue(const double *part, uint32_t nblocks, uint32_t stride, int axis) {
         src = part + static_cast<size_t>(axis) * stride * K;
         blocks = nblocks;
-        const uint32_t t = threadIdx.x < blocks ? threadIdx.x : 0u;
 #pragma unroll
-        for (int k = 0; k < K; ++k) first[k] = src[t * K + k];
+        for (int j = 0; j < S; ++j) {
+            const uint32_t u = threadIdx.x + j * kBlock;
+            const uint32_t t = u < blocks ? u : 0u;
+#pragma unroll
+            for (int k = 0; k < K; ++k) first[j][k] = src[t * K + k];
+        }
     }
     __device__ __forceinline__ void finish(double (&out)[K]) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) out[k] = threadIdx.x < blocks ? first[k] : 0.0;
-        for (uint32_t t = threadIdx.x + kBlock; t < blocks; t += kBlock) {
+        for (int k = 0; k < K; ++k) out[k] = threadIdx.x < blocks ? first[0][k] : 0.0;
+#pragma unroll
+        for (int j = 1; j < S; ++j) {
+            const bool ok = threadIdx.x + j * kBlock < blocks;
+#pragma unroll
+            for (int k = 0; k < K; ++k) out[k] += ok ? first[j][k] : 0.0;
+        }
+        for (uint32_t t = threadIdx.x + S * kBlock; t < blocks; t += kBlock) {
 #pragma unroll
             for (int k = 0; k < K; ++k) out[k] += src[t * K + k];
         }
@@ -543,7 +556,7 @@ __global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a, int parity) {
     const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
     // the partials are loaded first: the wait for them (an in-order count of
     // outstanding loads) then leaves the vector loads below in flight
-    PartialShare<1> share;
+    PartialShare<1, 8> share;  // the entry-block SpMV leaves several thousand partials per axis
     share.issue(a.partA, s.sblocks, a.stride, axis);
     __builtin_amdgcn_sched_barrier(0);
     constexpr int kPre = PER > 0 ? PER : 1;
