@@ -39,6 +39,7 @@ struct BArgs {
     const BSysDev *sys;
     const uint32_t *tileSys;
     const uint4 *flat;              // entry-block SpMV: per block (system, first row, end row, 0)
+    const uint2 *flatE;             // entry-block SpMV: per block (first entry, end entry)
     double *partA, *partB, *partS;  // [SpMV blocks][1], [tiles][2], [tiles][3]
     AxisCtl *ctl;
     uint32_t tile;
@@ -212,9 +213,10 @@ template <int G2>
 __global__ void __launch_bounds__(kBB) bSpmvFlat(BArgs a, int parity) {
     __shared__ double prod[kBFlat + kBFlatSlack];
     const uint4 f = a.flat[blockIdx.x];
+    const uint2 fe = a.flatE[blockIdx.x];
     const uint32_t S = f.x, R0 = f.y, R1 = f.z;
     if (!a.ctl[S].s[parity].live) return;
-    const uint32_t E0 = a.rowPtr[R0], count = a.rowPtr[R1] - E0;
+    const uint32_t E0 = fe.x, count = fe.y - fe.x;
     double acc[1] = {0.0};
     if (count > kBFlat + kBFlatSlack) {
         constexpr int G = 4, R = 2;
@@ -240,6 +242,21 @@ __global__ void __launch_bounds__(kBB) bSpmvFlat(BArgs a, int parity) {
             }
         }
     } else {
+        // the first pass of phase 2's row data is loaded before the gathers
+        // (a block of kBFlat entries usually has fewer rows than groups)
+        constexpr uint32_t kGroups = kBB / G2;
+        const uint32_t lane = threadIdx.x % G2, grp = threadIdx.x / G2;
+        // (an instance without rows has one block with R0 == R1 and no row to read)
+        uint32_t pre0 = 0, pre1 = 0;
+        double preP = 0.0, preD = 0.0;
+        if (R0 < R1) {
+            const uint32_t rq = R0 + grp < R1 ? R0 + grp : R0;
+            pre0 = a.rowPtr[rq];
+            pre1 = a.rowPtr[rq + 1];
+            preP = a.p[rq];
+            preD = a.diag[rq];
+        }
+        __builtin_amdgcn_sched_barrier(0);
         constexpr int U = 4;
         for (uint32_t base = 0; base < count; base += U * kBB) {
             uint32_t c[U];
@@ -266,20 +283,28 @@ __global__ void __launch_bounds__(kBB) bSpmvFlat(BArgs a, int parity) {
             }
         }
         __syncthreads();
-        constexpr uint32_t kGroups = kBB / G2;
-        const uint32_t lane = threadIdx.x % G2, grp = threadIdx.x / G2;
+        bool first = true;
         for (uint32_t r0 = R0; r0 < R1; r0 += kGroups) {
             const uint32_t r = r0 + grp;
             const bool valid = r < R1;
-            const uint32_t rr = valid ? r : R0;
-            const uint32_t e0 = a.rowPtr[rr] - E0, e1 = valid ? a.rowPtr[rr + 1] - E0 : e0;
-            const double pd = valid ? a.p[rr] : 0.0;
-            double part = G2 == 1 ? a.diag[rr] * pd : 0.0;
+            uint32_t q0 = pre0, q1 = pre1;
+            double pr = preP, dr = preD;
+            if (!first) {
+                const uint32_t rr = valid ? r : R0;
+                q0 = a.rowPtr[rr];
+                q1 = a.rowPtr[rr + 1];
+                pr = a.p[rr];
+                dr = a.diag[rr];
+            }
+            first = false;
+            const uint32_t e0 = q0 - E0, e1 = valid ? q1 - E0 : e0;
+            const double pd = valid ? pr : 0.0;
+            double part = G2 == 1 ? dr * pd : 0.0;
             for (uint32_t k = e0 + lane; k < e1; k += G2) part += prod[k];
 #pragma unroll
             for (int off = G2 / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, G2);
             if (valid && lane == 0) {
-                const double v = G2 == 1 ? part : a.diag[r] * pd + part;
+                const double v = G2 == 1 ? part : dr * pd + part;
                 a.ap[r] = v;
                 acc[0] += pd * v;
             }
@@ -415,6 +440,7 @@ struct kt_batch {
     uint32_t tiles = 0;
     kt::DevBuf<uint32_t> dRowPtr, dCol, dTileSys;
     kt::DevBuf<uint4> dFlat;
+    kt::DevBuf<uint2> dFlatE;
     uint32_t flatBlocks = 0, partASlots = 0;
     int layout = 0, rowLanes = 1;
     uint32_t layoutUploaded = 0xffffffffu;
@@ -456,6 +482,7 @@ int upload(kt_batch *b) {
     std::vector<BSysDev> sd(b->sys.size());
     std::vector<uint32_t> tileSys;
     std::vector<uint4> flat;  // entry blocks: (system, first row, end row, 0)
+    std::vector<uint2> flatE; // entry blocks: (first entry, end entry)
     for (size_t s = 0; s < b->sys.size(); ++s) {
         const uint32_t t = (b->sys[s].rows + b->tile - 1) / b->tile;
         const uint32_t r0 = b->sys[s].rowBase, r1 = r0 + b->sys[s].rows;
@@ -466,11 +493,13 @@ int upload(kt_batch *b) {
         for (uint32_t r = r0; r < r1; ++r) {
             if (b->rowPtr[r] >= target && r > start) {
                 flat.push_back(make_uint4(static_cast<uint32_t>(s), start, r, 0u));
+                flatE.push_back(make_uint2(b->rowPtr[start], b->rowPtr[r]));
                 start = r;
                 while (target <= b->rowPtr[r]) target += kBFlat;
             }
         }
         flat.push_back(make_uint4(static_cast<uint32_t>(s), start, r1, 0u));
+        flatE.push_back(make_uint2(b->rowPtr[start], b->rowPtr[r1]));
         sd[s] = BSysDev{r0, b->sys[s].rows, static_cast<uint32_t>(tileSys.size()), t, fbase,
                         static_cast<uint32_t>(flat.size()) - fbase};
         tileSys.insert(tileSys.end(), t, static_cast<uint32_t>(s));
@@ -481,6 +510,7 @@ int upload(kt_batch *b) {
     KT_TRY(b->dSys.reserve(std::max<size_t>(sd.size(), 1)));
     KT_TRY(b->dTileSys.reserve(std::max<size_t>(tileSys.size(), 1)));
     KT_TRY(b->dFlat.reserve(std::max<size_t>(flat.size(), 1)));
+    KT_TRY(b->dFlatE.reserve(std::max<size_t>(flatE.size(), 1)));
     KT_TRY(b->dPart.reserve(std::max<size_t>(b->partASlots + 5ull * b->tiles, 1)));
     KT_TRY(b->dCtl.reserve(std::max<size_t>(b->sys.size(), 1)));
     if (b->ctlHostCap < b->sys.size()) {
@@ -504,6 +534,7 @@ int upload(kt_batch *b) {
     KT_TRY(h2d(b->dSys.p, sd.data(), sd.size() * sizeof(BSysDev)));
     KT_TRY(h2d(b->dTileSys.p, tileSys.data(), tileSys.size() * sizeof(uint32_t)));
     KT_TRY(h2d(b->dFlat.p, flat.data(), flat.size() * sizeof(uint4)));
+    KT_TRY(h2d(b->dFlatE.p, flatE.data(), flatE.size() * sizeof(uint2)));
     KT_HIP(hipStreamSynchronize(b->stream));
     b->dirty = false;
     b->tileUploaded = b->tile;
@@ -526,6 +557,7 @@ BArgs argsOf(kt_batch *b) {
     a.sys = b->dSys.p;
     a.tileSys = b->dTileSys.p;
     a.flat = b->dFlat.p;
+    a.flatE = b->dFlatE.p;
     a.partA = b->dPart.p;
     a.partB = b->dPart.p + b->partASlots;
     a.partS = b->dPart.p + b->partASlots + 2ull * b->tiles;
