@@ -219,6 +219,12 @@ int kt_set_spmv_halo(kt_ctx *ctx, uint32_t halo);
 #define KT_SPMV_ENTRY_BLOCKS 1
 int kt_set_spmv_layout(kt_ctx *ctx, int layout, uint32_t block_entries, int unroll, int row_lanes,
                        int block_threads);
+/* Bytes per column index read by the entry-block SpMV: 4 (or 0), or 3 (a
+ * 16-bit low half and an 8-bit high byte, converted once per system).  3
+ * applies to blocks of 256 threads, 2048 entries and 4 per round on systems
+ * of fewer than 2^24 rows; elsewhere the 4-byte columns are read.  The
+ * products, their order and the results are the same either way. */
+int kt_set_spmv_columns(kt_ctx *ctx, int bytes);
 
 /* ---- per-kernel timings (HIP events on the context's stream) -------------- */
 #define KT_KERNEL_SPMV 0      /* FP64 CG SpMV + p.Ap */

@@ -5,7 +5,7 @@ per row (threads cooperating on one CSR row: 1, 2, 4, 8 or 16), the tile
 (rows per 1024-thread block), rows in flight, the XCD split, the LDS
 window (`halo`: entries of p staged on each side of a tile) and the
 entry-block shape (threads and entries per block, lanes per row in the row
-sums) by timing a fixed number of CG iterations of the context's current
+sums, 3- or 4-byte column indices) by timing a fixed number of CG iterations of the context's current
 systems on the GPU, from the same saved start each time.  First lanes,
 tile, rows and XCD split without staging; then, with the best rows and XCD
 split, every lanes, tile and halo whose window fits the LDS budget; then the
@@ -49,12 +49,13 @@ class TuneResult:
     unroll: int = 0
     row_lanes: int = 0
     block_threads: int = 0
+    col_bytes: int = 0
 
     def kwargs(self) -> dict:
         """The choice as `Context.set_tuning` keyword arguments."""
         return dict(lanes=self.lanes, tile=self.tile, xcd_split=self.xcd_split, rows=self.rows, halo=self.halo,
                     layout=self.layout, block_entries=self.block_entries, unroll=self.unroll,
-                    row_lanes=self.row_lanes, block_threads=self.block_threads)
+                    row_lanes=self.row_lanes, block_threads=self.block_threads, col_bytes=self.col_bytes)
 
 
 def _signature(ctx: "simpl.Context", precision: int):
@@ -118,8 +119,10 @@ def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iteratio
             if small(tile):
                 for threads, entries in FLAT_SHAPES:
                     for row_lanes in FLAT_ROW_LANES:
-                        timed(lanes=4, tile=tile, xcd_split=x, rows=2, layout=simpl.KT_SPMV_ENTRY_BLOCKS,
-                              block_entries=entries, unroll=4, row_lanes=row_lanes, block_threads=threads)
+                        for col_bytes in ((4, 3) if threads == 256 and rows < (1 << 24) else (4,)):
+                            timed(lanes=4, tile=tile, xcd_split=x, rows=2, layout=simpl.KT_SPMV_ENTRY_BLOCKS,
+                                  block_entries=entries, unroll=4, row_lanes=row_lanes, block_threads=threads,
+                                  col_bytes=col_bytes)
     # the closest candidates again, over twice the iterations, twice each:
     # single short timings of near-equal shapes are noisy
     table.sort(key=lambda t: t[1])

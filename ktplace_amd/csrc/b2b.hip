@@ -370,6 +370,7 @@ int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY) {
         a.rhs[axis] = s.rhs.p;
         s.nnzHost = -1;
         s.have32 = false;
+        s.have24 = false;
     }
     a.floor[0] = floorX;
     a.floor[1] = floorY;

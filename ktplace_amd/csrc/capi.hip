@@ -523,6 +523,7 @@ int kt_set_system(kt_ctx *ctx, int axis, uint64_t rows, const uint64_t *row_ptr,
         KT_HIP(hipStreamSynchronize(ctx->stream));
         s.nnzHost = static_cast<int64_t>(nnz);
         s.have32 = false;
+        s.have24 = false;
         return KT_OK;
     })
 }
@@ -767,6 +768,13 @@ int kt_set_spmv_halo(kt_ctx *ctx, uint32_t halo) {
     KT_CHECK(halo % 256 == 0, KT_EINVAL, "kt_set_spmv_halo: halo must be a multiple of 256");
     KT_CHECK(halo <= 65536, KT_EINVAL, "kt_set_spmv_halo: halo above 65536 entries");
     ctx->tune.halo = halo;
+    return KT_OK;
+}
+
+int kt_set_spmv_columns(kt_ctx *ctx, int bytes) {
+    KT_GUARD(ctx);
+    KT_CHECK(bytes == 0 || bytes == 3 || bytes == 4, KT_EINVAL, "kt_set_spmv_columns: bytes must be 0, 3 or 4");
+    ctx->tune.colBytes = bytes;
     return KT_OK;
 }
 

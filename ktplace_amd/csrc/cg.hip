@@ -388,7 +388,7 @@ uint32_t flatGrid(const CgArgs &a) {
 // G2: lanes per row in the row sums; NT: threads per block.  Smaller blocks
 // let more of them share a CU, so one block's gathers overlap another's row
 // sums and reduction.
-template <typename T, uint32_t E, int U, int G2, int NT>
+template <typename T, uint32_t E, int U, int G2, int NT, bool C24 = false>
 __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
     constexpr uint32_t kFlatEntries = E, kFlatSlack = flatSlack(E);
     __shared__ double prod[kFlatEntries + kFlatSlack];
@@ -446,7 +446,9 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
                 const uint32_t k = base + u * NT + threadIdx.x;
                 const bool ok = k < count;
                 const uint32_t kk = E0 + (ok ? k : 0u);  // count > 0 here: entry E0 exists
-                const uint32_t cl = s.col[kk];
+                uint32_t cl;
+                if constexpr (C24) cl = static_cast<uint32_t>(s.colLo[kk]) | (static_cast<uint32_t>(s.colHi[kk]) << 16);
+                else cl = s.col[kk];
                 const T wl = val[kk];
                 c[u] = ok ? cl : kNoCol;
                 w[u] = ok ? wl : T(0);
@@ -819,6 +821,18 @@ __global__ void __launch_bounds__(kBlock) cgTrueResidualFinish(CgArgs a) {
     }
 }
 
+// Column indices as a 16-bit low half and an 8-bit high byte (rows < 2^24):
+// 3 bytes per entry read by the entry-block SpMV instead of 4.
+__global__ void __launch_bounds__(kBlock)
+splitColumns(const uint32_t *__restrict__ col, uint16_t *__restrict__ lo, uint8_t *__restrict__ hi, uint64_t n) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        const uint32_t c = col[i];
+        lo[i] = static_cast<uint16_t>(c & 0xffffu);
+        hi[i] = static_cast<uint8_t>(c >> 16);
+    }
+}
+
 // FP32 copies of the matrix values and diagonal (mixed precision).
 __global__ void __launch_bounds__(kBlock)
 toFloat(const double *__restrict__ in, float *__restrict__ out, uint64_t n) {
@@ -874,7 +888,10 @@ int launchSpmv(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, 
     if (lanes4) cgSpmvFlat<T, EE, 4, 4, NN><<<fg, dim3(NN), 0, st>>>(a, parity);                          \
     else cgSpmvFlat<T, EE, 4, 1, NN><<<fg, dim3(NN), 0, st>>>(a, parity);
         if (nt == 256) {
-            if (wide) {
+            if (a.col24) {
+                if (lanes4) cgSpmvFlat<T, 2048, 4, 4, 256, true><<<fg, dim3(256), 0, st>>>(a, parity);
+                else cgSpmvFlat<T, 2048, 4, 1, 256, true><<<fg, dim3(256), 0, st>>>(a, parity);
+            } else if (wide) {
                 if (lanes4) cgSpmvFlat<T, 2048, 8, 4, 256><<<fg, dim3(256), 0, st>>>(a, parity);
                 else cgSpmvFlat<T, 2048, 8, 1, 256><<<fg, dim3(256), 0, st>>>(a, parity);
             } else {
@@ -1020,6 +1037,8 @@ SysView viewOf(kt_ctx *ctx, int axis, uint32_t tile) {
     v.rows = static_cast<uint32_t>(s.rows);
     v.blocks = static_cast<uint32_t>((s.rows + tile - 1) / tile);
     v.flatInfo = nullptr;
+    v.colLo = nullptr;
+    v.colHi = nullptr;
     v.sblocks = v.blocks;
     return v;
 }
@@ -1122,6 +1141,9 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
     // compares them to decide whether its captured iterations still apply
     a.flatRowLanes = ctx->tune.flatRowLanes == 4 ? 4u : 1u;
     a.flatThreads = static_cast<uint32_t>(flatThreads(ctx->tune.flatThreads));
+    // 3-byte columns: entry blocks of 256 threads, rows below 2^24
+    a.col24 = (a.layout == 1 && ctx->tune.colBytes == 3 && a.flatThreads == 256 && a.flatUnroll == 4 &&
+               a.flatEntries == 2048 && rows < (1ull << 24)) ? 1u : 0u;
 
     KT_CHECK(!(a.layout && a.halo), KT_EINVAL, "solve: the entry-block SpMV has no LDS window");
     if (a.halo) {
@@ -1174,6 +1196,20 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
                                                                           sb, E, info);
             KT_HIP(hipGetLastError());
             a.sys[ax].flatInfo = info;
+            if (a.col24) {
+                if (!sy.have24) {
+                    KT_TRY(sy.colLo.reserve(std::max<uint64_t>(nnz[ax], 1)));
+                    KT_TRY(sy.colHi.reserve(std::max<uint64_t>(nnz[ax], 1)));
+                    if (nnz[ax]) {
+                        splitColumns<<<dim3(1024), dim3(kBlock), 0, ctx->stream>>>(sy.col.p, sy.colLo.p, sy.colHi.p,
+                                                                                  nnz[ax]);
+                        KT_HIP(hipGetLastError());
+                    }
+                    sy.have24 = true;
+                }
+                a.sys[ax].colLo = sy.colLo.p;
+                a.sys[ax].colHi = sy.colHi.p;
+            }
             a.sys[ax].sblocks = sb;
         }
         maxBlocks = std::max({maxBlocks, a.sys[ax].blocks, a.sys[ax].sblocks});

@@ -112,6 +112,9 @@ struct SysView {
     // (first row, end row, first entry, end entry)
     const uint4 *flatInfo;
     uint32_t sblocks;  // SpMV blocks (and p.Ap partials) of this axis
+    // entry-block SpMV with 3-byte columns: col = colLo | colHi << 16
+    const uint16_t *colLo;
+    const uint8_t *colHi;
 };
 
 struct Sys {
@@ -122,6 +125,9 @@ struct Sys {
     DevBuf<double> val, diag, rhs;
     DevBuf<float> val32, diag32;
     bool have32 = false;
+    DevBuf<uint16_t> colLo;  // 3-byte columns: low 16 bits
+    DevBuf<uint8_t> colHi;   // 3-byte columns: high 8 bits
+    bool have24 = false;
     DevBuf<uint32_t> rowStart;  // entry-block SpMV: (first row, end row, first entry, end entry) per block
 };
 
@@ -148,6 +154,8 @@ struct CgArgs {
     uint32_t flatUnroll;   // entry blocks: entries per thread and round (4, 8)
     uint32_t flatRowLanes; // entry blocks: lanes per row in the row sums (1, 4)
     uint32_t flatThreads;  // entry blocks: threads per block (256, 512, 1024)
+    uint32_t col24;        // entry blocks: 1 = columns read as 16 + 8 bits
+    uint32_t pad;
 };
 
 // Launch-shape knobs (set by the autotuner or the caller; 0 = default).
@@ -164,6 +172,7 @@ struct Tune {
     int flatUnroll = 0;       // entry blocks: entries per thread and round (0: 4)
     int flatRowLanes = 0;     // entry blocks: lanes per row in the row sums (0, 1: entry order; 4)
     int flatThreads = 0;      // entry blocks: threads per block (0: 1024; 256, 512)
+    int colBytes = 0;         // entry blocks: bytes per column index (0, 4; 3 with 256 threads)
 };
 
 // A captured hipGraph of kPairs iteration pairs for one argument set.

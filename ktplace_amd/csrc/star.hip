@@ -211,6 +211,7 @@ int buildStar(kt_ctx *ctx, uint64_t *sizeOut) {
         a.rhs[axis] = s.rhs.p;
         s.nnzHost = -1;
         s.have32 = false;
+        s.have24 = false;
     }
     a.n = n;
     a.size = static_cast<uint32_t>(size);

@@ -411,3 +411,28 @@ def test_entry_block_layout_long_rows(gpu_ctx_factory):
         assert max(r["true_resid"]) <= 1e-10
         assert rel(x, exact) < 1e-8 and np.array_equal(x, y)
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_three_byte_columns_are_bit_identical(b2b12k, gpu_ctx_factory, precision):
+    """Entry blocks reading column indices as 16 + 8 bits give the same
+    solution bit for bit as reading them as 32 bits (same products, same
+    order), through graph replay and direct launches, after the systems are
+    rebuilt (the 3-byte copy follows the new columns)."""
+    from ktplace_amd.simpl import KT_PREC_FP64, KT_PREC_MIXED, KT_SPMV_ENTRY_BLOCKS, KT_STOP_TRUE
+    prec = KT_PREC_FP64 if precision == "fp64" else KT_PREC_MIXED
+    d, o, x0, y0 = b2b12k
+    ctx = gpu_ctx_factory(d)
+    for positions in ((x0, y0), (x0[::-1].copy(), y0)):
+        ctx.build_b2b(*positions)
+        out = []
+        for col_bytes, graphs in ((4, True), (3, True), (3, False)):
+            ctx.set_tuning(lanes=4, tile=2048, layout=KT_SPMV_ENTRY_BLOCKS, block_threads=256, row_lanes=4,
+                           col_bytes=col_bytes, use_graphs=graphs)
+            x, y = x0.copy(), y0.copy()
+            r = ctx.solve_both(x, y, 1e-9, 100000, precision=prec, stop_rule=KT_STOP_TRUE)
+            assert max(r["true_resid"]) <= 1e-9
+            out.append((x, y))
+        for x, y in out[1:]:
+            assert np.array_equal(x, out[0][0]) and np.array_equal(y, out[0][1])
+    ctx.close()
