@@ -321,6 +321,10 @@ int kt_batch_set_tuning(kt_batch *b, int lanes, uint32_t tile, int rows_in_fligh
  * KT_SPMV_ENTRY_BLOCKS with blocks of about 2048 consecutive entries of one
  * system and row_lanes (1 or 4; 0 = 1) lanes per row in the row sums. */
 int kt_batch_set_layout(kt_batch *b, int layout, int row_lanes);
+/* Bytes per column index read by the batch's entry-block SpMV: 4 (or 0), or
+ * 3 (16 + 8 bits, as kt_set_spmv_columns) for batches of fewer than 2^24
+ * rows.  The results are the same either way. */
+int kt_batch_set_columns(kt_batch *b, int bytes);
 /* Every instance back to the start positions it was added with. */
 int kt_batch_reset(kt_batch *b);
 /* Solves all instances from their current positions (opts: KT_PREC_FP64,

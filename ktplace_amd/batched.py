@@ -67,14 +67,14 @@ def autotune(batch: simpl.Batch, iterations: int = 48):
     iterations of the whole batch."""
     table = []
 
-    def timed(g, t, r, layout=0, row_lanes=0):
-        batch.set_tuning(lanes=g, tile=t, rows=r, layout=layout, row_lanes=row_lanes)
+    def timed(g, t, r, layout=0, row_lanes=0, col_bytes=0):
+        batch.set_tuning(lanes=g, tile=t, rows=r, layout=layout, row_lanes=row_lanes, col_bytes=col_bytes)
         batch.reset()
         batch.solve(0.0, iterations)  # warm-up (captures the graph)
         batch.reset()
         t0 = time.perf_counter()
         batch.solve(0.0, iterations)
-        table.append(((time.perf_counter() - t0) * 1e3, g, t, r, layout, row_lanes))
+        table.append(((time.perf_counter() - t0) * 1e3, g, t, r, layout, row_lanes, col_bytes))
 
     for g in LANES:
         for t in TILES:
@@ -82,13 +82,14 @@ def autotune(batch: simpl.Batch, iterations: int = 48):
                 timed(g, t, r)
     for t in TILES:  # entry blocks: lanes and rows do not apply; the tile shapes the vector kernels
         for row_lanes in (1, 4):
-            timed(4, t, 2, simpl.KT_SPMV_ENTRY_BLOCKS, row_lanes)
+            for col_bytes in (4, 3):
+                timed(4, t, 2, simpl.KT_SPMV_ENTRY_BLOCKS, row_lanes, col_bytes)
     table.sort()
-    ms, g, t, r, layout, row_lanes = table[0]
-    batch.set_tuning(lanes=g, tile=t, rows=r, layout=layout, row_lanes=row_lanes)
+    ms, g, t, r, layout, row_lanes, col_bytes = table[0]
+    batch.set_tuning(lanes=g, tile=t, rows=r, layout=layout, row_lanes=row_lanes, col_bytes=col_bytes)
     return dict(lanes_per_row=g, tile_rows=t, rows_in_flight=r,
                 spmv_layout="entry blocks" if layout else "row groups", row_lanes=row_lanes,
-                ms_per_48_iterations=round(ms, 3))
+                column_bytes=(col_bytes if layout else None), ms_per_48_iterations=round(ms, 3))
 
 
 def percentiles(v):
@@ -124,14 +125,15 @@ def run(count: int = 256, steps: int = 3, tol: float = 1e-6, seed: int = 5000, l
     log(f"[batched] {b_ms:.2f} ms per batch, p50 {b50:.2f} p99 {b99:.2f} ms, worst true resid {worst:.2e}")
 
     # loop: each instance alone, one after the other, with the two-system
-    # solver in either SpMV layout, entry blocks of 1024 or 256 threads (the
-    # fastest is reported)
+    # solver in either SpMV layout, entry blocks of 1024 threads or of 256
+    # threads with 3-byte columns (the fastest is reported)
     best = None
-    for layout, threads in ((simpl.KT_SPMV_ROW_GROUPS, 0), (simpl.KT_SPMV_ENTRY_BLOCKS, 0),
-                            (simpl.KT_SPMV_ENTRY_BLOCKS, 256)):
-        name = "row groups" if not layout else f"entry blocks of {threads or 1024} threads"
+    for layout, threads, col_bytes in ((simpl.KT_SPMV_ROW_GROUPS, 0, 0), (simpl.KT_SPMV_ENTRY_BLOCKS, 0, 0),
+                                       (simpl.KT_SPMV_ENTRY_BLOCKS, 256, 3)):
+        name = ("row groups" if not layout else
+                f"entry blocks of {threads or 1024} threads, {col_bytes or 4}-byte columns")
         for c in ctxs:  # warm-up (graph capture per context)
-            c.set_tuning(layout=layout, block_threads=threads)
+            c.set_tuning(layout=layout, block_threads=threads, col_bytes=col_bytes)
             c.restore_positions()
             c.solve_resident(tol, 1_000_000, stop_rule=simpl.KT_STOP_TRUE)
         loop_totals, loop_each = [], []

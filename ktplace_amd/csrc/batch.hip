@@ -44,6 +44,10 @@ struct BArgs {
     AxisCtl *ctl;
     uint32_t tile;
     uint32_t layout;                // 0 lane groups, 1 entry blocks
+    const uint16_t *colLo;          // entry blocks with 3-byte columns: col = colLo | colHi << 16
+    const uint8_t *colHi;
+    uint32_t col24;
+    uint32_t pad;
 };
 
 struct Tile {
@@ -200,6 +204,17 @@ __global__ void __launch_bounds__(kBB) bSpmv(BArgs a, int parity) {
     writePartial<1>(a.partA, acc);
 }
 
+// Column indices as a 16-bit low half and an 8-bit high byte (rows < 2^24).
+__global__ void __launch_bounds__(kBB)
+bSplitColumns(const uint32_t *__restrict__ col, uint16_t *__restrict__ lo, uint8_t *__restrict__ hi, uint64_t n) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBB) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * kBB) {
+        const uint32_t c = col[i];
+        lo[i] = static_cast<uint16_t>(c & 0xffffu);
+        hi[i] = static_cast<uint8_t>(c >> 16);
+    }
+}
+
 // Entry-block SpMV over the batch (the two-system solver's cgSpmvFlat, per
 // system): block b takes rows [R0, R1) of system S, whose entries are a run
 // of about kBFlat consecutive entries; the threads read the run's columns and
@@ -209,7 +224,7 @@ __global__ void __launch_bounds__(kBB) bSpmv(BArgs a, int parity) {
 constexpr uint32_t kBFlat = 2048;
 constexpr uint32_t kBFlatSlack = 256;  // an eighth of the block: eight blocks fit a CU's LDS
 
-template <int G2>
+template <int G2, bool C24>
 __global__ void __launch_bounds__(kBB) bSpmvFlat(BArgs a, int parity) {
     __shared__ double prod[kBFlat + kBFlatSlack];
     const uint4 f = a.flat[blockIdx.x];
@@ -266,7 +281,9 @@ __global__ void __launch_bounds__(kBB) bSpmvFlat(BArgs a, int parity) {
                 const uint32_t k = base + u * kBB + threadIdx.x;
                 const bool ok = k < count;
                 const uint32_t kk = E0 + (ok ? k : 0u);  // count > 0 here: entry E0 exists
-                const uint32_t cl = a.col[kk];
+                uint32_t cl;
+                if constexpr (C24) cl = static_cast<uint32_t>(a.colLo[kk]) | (static_cast<uint32_t>(a.colHi[kk]) << 16);
+                else cl = a.col[kk];
                 const double wl = a.val[kk];
                 c[u] = ok ? cl : kNoCol;
                 w[u] = ok ? wl : 0.0;
@@ -441,6 +458,9 @@ struct kt_batch {
     kt::DevBuf<uint32_t> dRowPtr, dCol, dTileSys;
     kt::DevBuf<uint4> dFlat;
     kt::DevBuf<uint2> dFlatE;
+    kt::DevBuf<uint16_t> dColLo;  // 3-byte columns (batches of fewer than 2^24 rows)
+    kt::DevBuf<uint8_t> dColHi;
+    int colBytes = 0;
     uint32_t flatBlocks = 0, partASlots = 0;
     int layout = 0, rowLanes = 1;
     uint32_t layoutUploaded = 0xffffffffu;
@@ -535,6 +555,14 @@ int upload(kt_batch *b) {
     KT_TRY(h2d(b->dTileSys.p, tileSys.data(), tileSys.size() * sizeof(uint32_t)));
     KT_TRY(h2d(b->dFlat.p, flat.data(), flat.size() * sizeof(uint4)));
     KT_TRY(h2d(b->dFlatE.p, flatE.data(), flatE.size() * sizeof(uint2)));
+    if (rows < (1ull << 24)) {
+        KT_TRY(b->dColLo.reserve(std::max<uint64_t>(nnz, 1)));
+        KT_TRY(b->dColHi.reserve(std::max<uint64_t>(nnz, 1)));
+        if (nnz) {
+            bSplitColumns<<<dim3(1024), dim3(kBB), 0, b->stream>>>(b->dCol.p, b->dColLo.p, b->dColHi.p, nnz);
+            KT_HIP(hipGetLastError());
+        }
+    }
     KT_HIP(hipStreamSynchronize(b->stream));
     b->dirty = false;
     b->tileUploaded = b->tile;
@@ -564,6 +592,9 @@ BArgs argsOf(kt_batch *b) {
     a.ctl = b->dCtl.p;
     a.tile = b->tile;
     a.layout = b->layout ? 1u : 0u;
+    a.col24 = (a.layout && b->colBytes == 3 && b->diag.size() < (1ull << 24)) ? 1u : 0u;
+    a.colLo = a.col24 ? b->dColLo.p : nullptr;
+    a.colHi = a.col24 ? b->dColHi.p : nullptr;
     return a;
 }
 
@@ -578,8 +609,13 @@ int launchIter(kt_batch *b, const BArgs &a, int parity) {
     const dim3 grid(b->tiles);
     const int R = b->rows >= 4 ? 4 : 2;
     if (a.layout) {
-        if (b->rowLanes == 4) bSpmvFlat<4><<<dim3(b->flatBlocks), dim3(kBB), 0, b->stream>>>(a, parity);
-        else bSpmvFlat<1><<<dim3(b->flatBlocks), dim3(kBB), 0, b->stream>>>(a, parity);
+        if (a.col24) {
+            if (b->rowLanes == 4) bSpmvFlat<4, true><<<dim3(b->flatBlocks), dim3(kBB), 0, b->stream>>>(a, parity);
+            else bSpmvFlat<1, true><<<dim3(b->flatBlocks), dim3(kBB), 0, b->stream>>>(a, parity);
+        } else {
+            if (b->rowLanes == 4) bSpmvFlat<4, false><<<dim3(b->flatBlocks), dim3(kBB), 0, b->stream>>>(a, parity);
+            else bSpmvFlat<1, false><<<dim3(b->flatBlocks), dim3(kBB), 0, b->stream>>>(a, parity);
+        }
         KT_HIP(hipGetLastError());
         bUpdate<<<grid, dim3(kBB), 0, b->stream>>>(a, parity);
         KT_HIP(hipGetLastError());
@@ -899,6 +935,13 @@ int kt_batch_set_layout(kt_batch *b, int layout, int row_lanes) {
     KT_CHECK(row_lanes == 0 || row_lanes == 1 || row_lanes == 4, KT_EINVAL, "row lanes must be 0, 1 or 4");
     b->layout = layout;
     b->rowLanes = row_lanes == 4 ? 4 : 1;
+    return KT_OK;
+}
+
+int kt_batch_set_columns(kt_batch *b, int bytes) {
+    KT_BATCH_GUARD(b);
+    KT_CHECK(bytes == 0 || bytes == 3 || bytes == 4, KT_EINVAL, "kt_batch_set_columns: bytes must be 0, 3 or 4");
+    b->colBytes = bytes;
     return KT_OK;
 }
 
