@@ -360,8 +360,8 @@ constexpr uint32_t flatSlack(uint32_t entries) { return entries / 8 < 512 ? entr
 // gathers of one axis only.
 __device__ __forceinline__ bool decodeFlat(const CgArgs &a, int &axis, uint32_t &b, int parity) {
     if (a.xcdSplit) {
-        const bool lx = a.ctl[0].s[parity].live, ly = a.ctl[1].s[parity].live;
-        if (lx != ly) {  // one axis live: every block serves it (as decode)
+        const bool lx = parity >= 0 && a.ctl[0].s[parity].live, ly = parity >= 0 && a.ctl[1].s[parity].live;
+        if (lx != ly) {  // inside an iteration, one axis live: every block serves it (as decode)
             axis = lx ? 0 : 1;
             b = blockIdx.x;
             return b < a.sys[axis].sblocks;
@@ -388,22 +388,51 @@ uint32_t flatGrid(const CgArgs &a) {
 // G2: lanes per row in the row sums; NT: threads per block.  Smaller blocks
 // let more of them share a CU, so one block's gathers overlap another's row
 // sums and reduction.
-template <typename T, uint32_t E, int U, int G2, int NT, bool C24 = false>
+// MODE 0: Ap for the CG iteration (partial p.Ap).  MODE 1 and 2 (T = double)
+// take the FP64 residual b - A x64 of the same rows instead: 1 refreshes the
+// FP32 residual of the axes that asked for it (partials r64.r64, r.z), 2 is
+// the true residual check of both axes (partials r.r, b.b); parity is -1.
+template <typename T, uint32_t E, int U, int G2, int NT, bool C24 = false, int MODE = 0>
 __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
+    static_assert(MODE == 0 || sizeof(T) == 8, "the residual modes run in FP64");
+    constexpr int K = MODE == 0 ? 1 : 2;
     constexpr uint32_t kFlatEntries = E, kFlatSlack = flatSlack(E);
     __shared__ double prod[kFlatEntries + kFlatSlack];
     int axis;
     uint32_t b;
     if (!decodeFlat(a, axis, b, parity)) return;
-    if (!a.ctl[axis].s[parity].live) return;
+    if constexpr (MODE == 0) {
+        if (!a.ctl[axis].s[parity].live) return;
+    } else if constexpr (MODE == 1) {
+        if (!a.ctl[axis].needUpdate) return;
+    }
     const SysView s = a.sys[axis];
-    const T *__restrict__ p = static_cast<const T *>(a.p[axis]);
+    const T *__restrict__ p = MODE == 0 ? static_cast<const T *>(a.p[axis]) : reinterpret_cast<const T *>(a.x64[axis]);
     T *__restrict__ ap = static_cast<T *>(a.ap[axis]);
     const T *__restrict__ val = Mat<T>::val(s);
     const T *__restrict__ diag = Mat<T>::diag(s);
     const uint4 info = s.flatInfo[b];
     const uint32_t R0 = info.x, R1 = info.y, E0 = info.z, count = info.w - info.z;
-    double acc[1] = {0.0};
+    double acc[K] = {};
+    // row r's product (A v)_r = sum, with pd = v_r
+    const auto finish = [&](uint32_t r, double pd, double sum) {
+        if constexpr (MODE == 0) {
+            const T v = static_cast<T>(sum);
+            ap[r] = v;
+            acc[0] += pd * static_cast<double>(v);
+        } else {
+            const double rhs = s.rhs[r];
+            const double ri = rhs - sum;
+            acc[0] += ri * ri;
+            if constexpr (MODE == 1) {
+                const float rf = static_cast<float>(ri);
+                static_cast<float *>(a.r[axis])[r] = rf;
+                acc[1] += static_cast<double>(rf) * static_cast<double>(rf / s.diag32[r]);
+            } else {
+                acc[1] += rhs * rhs;
+            }
+        }
+    };
     if (count > kFlatEntries + kFlatSlack) {
         constexpr int G = 4, R = 2;
         const uint32_t lane = threadIdx.x % G, grp = threadIdx.x / G;
@@ -421,9 +450,7 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
             for (int q = 0; q < R; ++q) {
                 if (valid[q] && lane == 0) {
                     const double pd = static_cast<double>(p[row[q]]);
-                    const T v = static_cast<T>(static_cast<double>(diag[row[q]]) * pd + off[q]);
-                    ap[row[q]] = v;
-                    acc[0] += pd * static_cast<double>(v);
+                    finish(row[q], pd, static_cast<double>(diag[row[q]]) * pd + off[q]);
                 }
             }
         }
@@ -486,11 +513,7 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
                 // (SparseSystem::multiply, simpl_lss.h:49-57)
                 double sum = static_cast<double>(dr) * pd;
                 for (uint32_t k = e0; k < e1; ++k) sum += prod[k];
-                if (valid) {
-                    const T v = static_cast<T>(sum);
-                    ap[r] = v;
-                    acc[0] += pd * static_cast<double>(v);
-                }
+                if (valid) finish(r, pd, sum);
             } else {
                 // lane l adds products l, l + G2, ... of the row, then a fixed
                 // butterfly; shorter waits on long rows, another order
@@ -498,16 +521,12 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
                 for (uint32_t k = e0 + lane; k < e1; k += G2) part += prod[k];
 #pragma unroll
                 for (int off = G2 / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, G2);
-                if (valid && lane == 0) {
-                    const T v = static_cast<T>(static_cast<double>(dr) * pd + part);
-                    ap[r] = v;
-                    acc[0] += pd * static_cast<double>(v);
-                }
+                if (valid && lane == 0) finish(r, pd, static_cast<double>(dr) * pd + part);
             }
         }
     }
-    blockSum<1, NT>(acc);
-    writePartial<1>(a.partA, a.stride, axis, b, acc);
+    blockSum<K, NT>(acc);
+    writePartial<K>(MODE == 0 ? a.partA : a.partB, a.stride, axis, b, acc);
 }
 
 // Cut c (c = 1 .. blocks - 1) = first row whose entries start at or after
@@ -761,7 +780,7 @@ __global__ void __launch_bounds__(kBlock) cgRefreshFinish(CgArgs a) {
     AxisCtl &c = a.ctl[axis];
     if (!c.needUpdate) return;
     double tot[2];
-    sumPartials<2>(a.partB, a.sys[axis].blocks, a.stride, axis, tot);
+    sumPartials<2>(a.partB, a.flatResidual ? a.sys[axis].sblocks : a.sys[axis].blocks, a.stride, axis, tot);
     if (threadIdx.x == 0) {
         CgState &st = c.s[0];
         const double resid = c.bNorm > 0.0 ? sqrt(tot[0]) / c.bNorm : 0.0;
@@ -814,7 +833,7 @@ __global__ void __launch_bounds__(kBlock) cgTrueResidual(CgArgs a) {
 __global__ void __launch_bounds__(kBlock) cgTrueResidualFinish(CgArgs a) {
     const int axis = blockIdx.x;
     double tot[2];
-    sumPartials<2>(a.partB, a.sys[axis].blocks, a.stride, axis, tot);
+    sumPartials<2>(a.partB, a.flatResidual ? a.sys[axis].sblocks : a.sys[axis].blocks, a.stride, axis, tot);
     if (threadIdx.x == 0) {
         const double bn = sqrt(tot[1]);
         a.ctl[axis].trueResid = bn > 0.0 ? sqrt(tot[0]) / bn : 0.0;
@@ -989,9 +1008,26 @@ int launchStart(kt_ctx *ctx, const CgArgs &a, int G) {
     return KT_OK;
 }
 
+// The FP64 residual b - A x64 with the entry-block kernel (MODE 1: refresh,
+// 2: true residual); used when the solve runs 256-thread blocks of 2048
+// entries (flatResidual), with their row lanes and column bytes.
+template <int MODE>
+void launchFlatResidual(kt_ctx *ctx, const CgArgs &a) {
+    const dim3 fg(flatGrid(a));
+    hipStream_t st = ctx->stream;
+    if (a.col24) {
+        if (a.flatRowLanes == 4) cgSpmvFlat<double, 2048, 4, 4, 256, true, MODE><<<fg, dim3(256), 0, st>>>(a, -1);
+        else cgSpmvFlat<double, 2048, 4, 1, 256, true, MODE><<<fg, dim3(256), 0, st>>>(a, -1);
+    } else {
+        if (a.flatRowLanes == 4) cgSpmvFlat<double, 2048, 4, 4, 256, false, MODE><<<fg, dim3(256), 0, st>>>(a, -1);
+        else cgSpmvFlat<double, 2048, 4, 1, 256, false, MODE><<<fg, dim3(256), 0, st>>>(a, -1);
+    }
+}
+
 int launchTrueResidual(kt_ctx *ctx, const CgArgs &a, int G) {
     const dim3 grid(gridBlocks(a));
-    KT_DISPATCH_G(G, cgTrueResidual, grid, ctx->stream, a);
+    if (a.flatResidual) launchFlatResidual<2>(ctx, a);
+    else KT_DISPATCH_G(G, cgTrueResidual, grid, ctx->stream, a);
     KT_HIP(hipGetLastError());
     cgTrueResidualFinish<<<dim3(2), dim3(kBlock), 0, ctx->stream>>>(a);
     KT_HIP(hipGetLastError());
@@ -1002,7 +1038,8 @@ int launchRefresh(kt_ctx *ctx, const CgArgs &a, int G) {
     const dim3 grid(gridBlocks(a));
     cgFlush<<<grid, dim3(kBlock), 0, ctx->stream>>>(a);
     KT_HIP(hipGetLastError());
-    KT_DISPATCH_G(G, cgRefresh, grid, ctx->stream, a);
+    if (a.flatResidual) launchFlatResidual<1>(ctx, a);
+    else KT_DISPATCH_G(G, cgRefresh, grid, ctx->stream, a);
     KT_HIP(hipGetLastError());
     cgRefreshFinish<<<dim3(2), dim3(kBlock), 0, ctx->stream>>>(a);
     KT_HIP(hipGetLastError());
@@ -1144,6 +1181,9 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
     // 3-byte columns: entry blocks of 256 threads, rows below 2^24
     a.col24 = (a.layout == 1 && ctx->tune.colBytes == 3 && a.flatThreads == 256 && a.flatUnroll == 4 &&
                a.flatEntries == 2048 && rows < (1ull << 24)) ? 1u : 0u;
+    // FP64 residuals by the same entry blocks (the shape instantiated for them)
+    a.flatResidual =
+        (a.layout == 1 && a.flatThreads == 256 && a.flatUnroll == 4 && a.flatEntries == 2048 && rows > 0) ? 1u : 0u;
 
     KT_CHECK(!(a.layout && a.halo), KT_EINVAL, "solve: the entry-block SpMV has no LDS window");
     if (a.halo) {

@@ -155,7 +155,7 @@ struct CgArgs {
     uint32_t flatRowLanes; // entry blocks: lanes per row in the row sums (1, 4)
     uint32_t flatThreads;  // entry blocks: threads per block (256, 512, 1024)
     uint32_t col24;        // entry blocks: 1 = columns read as 16 + 8 bits
-    uint32_t pad;
+    uint32_t flatResidual; // 1: FP64 residuals (refresh, true residual) by entry blocks of 256 threads
 };
 
 // Launch-shape knobs (set by the autotuner or the caller; 0 = default).
