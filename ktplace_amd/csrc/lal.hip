@@ -89,33 +89,65 @@ __device__ __forceinline__ unsigned long long waveSumU64(unsigned long long v) {
 
 // bin of each cell centre (simpl_placer.cc:277-289) and fixed-point occupancy.
 // The lanes of a wave first combine their areas per bin (one pass per
-// distinct bin among the wave's cells) and one lane adds each sum: a clumped
-// placement puts most cells into a few bins, where one atomic per cell would
-// serialise.  Integer sums, so the result does not depend on the grouping.
+// distinct bin among the wave's cells), one lane adds each sum to the
+// block's table in LDS, and the block adds its table to memory at the end: a
+// clumped placement puts most cells into a few bins, where one atomic per
+// cell, or even per wave, would serialise on those bins.  Integer sums, so
+// the result does not depend on the grouping.
 __global__ void __launch_bounds__(kB)
 cellBins(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ w,
          const double *__restrict__ h, const double *__restrict__ area, GridDev g, uint32_t n,
          uint32_t *__restrict__ binOf, unsigned long long *__restrict__ occFix) {
-    const uint32_t i = blockIdx.x * kB + threadIdx.x;
-    const bool valid = i < n;
-    uint32_t k = 0;
-    unsigned long long v = 0;
-    if (valid) {
-        k = binOfPoint(g, x[i] + 0.5 * w[i], y[i] + 0.5 * h[i]);
-        binOf[i] = k;
-        v = static_cast<unsigned long long>(llrint(area[i] * kFixScale));
+    // the block's bins and sums, open addressing; a bin that finds no free
+    // slot within kProbes is added to memory directly
+    constexpr uint32_t kSlots = 256, kEmpty = 0xffffffffu;
+    constexpr int kProbes = 8;
+    __shared__ uint32_t keys[kSlots];
+    __shared__ unsigned long long sums[kSlots];
+    for (uint32_t t = threadIdx.x; t < kSlots; t += kB) {
+        keys[t] = kEmpty;
+        sums[t] = 0ull;
     }
-    bool todo = valid;
-    unsigned long long pending = __ballot(todo);
-    while (pending) {
-        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
-        const uint32_t lk = __shfl(k, leader);
-        const bool mine = todo && k == lk;
-        const unsigned long long s = waveSumU64(mine ? v : 0ull);
-        if (static_cast<int>(threadIdx.x & 63) == leader) atomicAdd(occFix + lk, s);
-        todo = todo && !mine;
-        pending = __ballot(todo);
+    __syncthreads();
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kB;
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kB; base < n; base += stride) {  // block-uniform
+        const bool valid = base + threadIdx.x < n;
+        const uint32_t i = valid ? static_cast<uint32_t>(base + threadIdx.x) : 0u;
+        uint32_t k = 0;
+        unsigned long long v = 0;
+        if (valid) {
+            k = binOfPoint(g, x[i] + 0.5 * w[i], y[i] + 0.5 * h[i]);
+            binOf[i] = k;
+            v = static_cast<unsigned long long>(llrint(area[i] * kFixScale));
+        }
+        bool todo = valid;
+        unsigned long long pending = __ballot(todo);
+        while (pending) {
+            const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+            const uint32_t lk = __shfl(k, leader);
+            const bool mine = todo && k == lk;
+            const unsigned long long s = waveSumU64(mine ? v : 0ull);
+            if (static_cast<int>(threadIdx.x & 63) == leader) {
+                uint32_t slot = (lk * 2654435761u) & (kSlots - 1);
+                bool kept = false;
+                for (int probe = 0; probe < kProbes; ++probe) {
+                    const uint32_t prev = atomicCAS(&keys[slot], kEmpty, lk);
+                    if (prev == kEmpty || prev == lk) {
+                        atomicAdd(&sums[slot], s);
+                        kept = true;
+                        break;
+                    }
+                    slot = (slot + 1) & (kSlots - 1);
+                }
+                if (!kept) atomicAdd(occFix + lk, s);
+            }
+            todo = todo && !mine;
+            pending = __ballot(todo);
+        }
     }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < kSlots; t += kB)
+        if (keys[t] != kEmpty) atomicAdd(occFix + keys[t], sums[t]);
 }
 
 __global__ void __launch_bounds__(kB)
@@ -833,7 +865,9 @@ int lalOccupancy(kt_ctx *ctx, std::vector<double> *occHost) {
     KT_TRY(ctx->occ.reserve(m));
     KT_HIP(hipMemsetAsync(ctx->occFix.p, 0, m * sizeof(uint64_t), ctx->stream));
     if (n) {
-        cellBins<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->posX.p, ctx->posY.p, ctx->width.p, ctx->height.p,
+        // a few cells per thread: each block's bins reach memory once
+        const dim3 grid(std::min<uint32_t>(gridFor(n).x, 2048u));
+        cellBins<<<grid, kB, 0, ctx->stream>>>(ctx->posX.p, ctx->posY.p, ctx->width.p, ctx->height.p,
                                                      ctx->area.p, gridDev(ctx->grid), n, ctx->binOf.p,
                                                      reinterpret_cast<unsigned long long *>(ctx->occFix.p));
         KT_HIP(hipGetLastError());
