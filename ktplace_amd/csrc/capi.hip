@@ -357,6 +357,7 @@ int kt_ctx_destroy(kt_ctx *ctx) {
     if (ctx->ctlHost) (void)hipHostFree(ctx->ctlHost);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->lalPinned) (void)hipHostFree(ctx->lalPinned);
+    if (ctx->lalLand) (void)hipHostFree(ctx->lalLand);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return KT_OK;

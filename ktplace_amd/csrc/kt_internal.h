@@ -263,6 +263,8 @@ struct kt_ctx {
     kt::lal::AreaTables lalTables;                     // free-area table of the grid, occupancy rows
     uint8_t *lalPinned = nullptr;                      // pinned staging of per-level tables
     size_t lalPinnedCap = 0;
+    uint8_t *lalLand = nullptr;                        // pinned landing of a batch's small transfers
+    size_t lalLandCap = 0;
     kt::DevBuf<double> lalUp;                          // device copy of the staged tables
     kt::DevBuf<kt::LalSegStat> lalStats;
     kt::DevBuf<uint8_t> lalSplit;

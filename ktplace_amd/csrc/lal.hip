@@ -943,6 +943,32 @@ int lalStaging(kt_ctx *ctx, size_t bytes) {
     return KT_OK;
 }
 
+// Pinned landing area for a batch's small transfers: a batch makes several
+// round trips to the host (member count, per-level block stats, escape
+// checks, moved cells), and a transfer to or from pageable memory is staged
+// by the runtime at extra cost each time.  Layout: kLandScalars 32-bit slots,
+// then one variable part used in turn by the region rectangles, the block
+// stats of a level and the moved cells (each read or sent completely before
+// the next one is staged).  It grows only after a synchronisation, when
+// nothing in flight uses it.
+constexpr size_t kLandScalars = 16;
+enum LandSlot { kLandM = 0, kLandBad = 1, kLandActive = 2, kLandMoves = 3, kLandNone = 4 };
+
+int lalLandReserve(kt_ctx *ctx, size_t varBytes) {
+    const size_t bytes = kLandScalars * sizeof(uint32_t) + varBytes;
+    if (bytes <= ctx->lalLandCap) return KT_OK;
+    KT_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->lalLand) (void)hipHostFree(ctx->lalLand);
+    ctx->lalLand = nullptr;
+    ctx->lalLandCap = 0;
+    const size_t cap = std::max<size_t>(2 * bytes, 1 << 16);
+    KT_HIP(hipHostMalloc(reinterpret_cast<void **>(&ctx->lalLand), cap));
+    ctx->lalLandCap = cap;
+    return KT_OK;
+}
+inline uint32_t *landSlots(kt_ctx *ctx) { return reinterpret_cast<uint32_t *>(ctx->lalLand); }
+inline uint8_t *landVar(kt_ctx *ctx) { return ctx->lalLand + kLandScalars * sizeof(uint32_t); }
+
 // Legalizes one batch of non-touching regions; the cells whose bin changed
 // are returned in `moved` (their device binOf is updated).
 int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, std::vector<Moved> &moved) {
@@ -965,7 +991,9 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, std::vect
     const uint32_t R = static_cast<uint32_t>(regions.size());
     // bin -> region for the batch's bins (every other bin holds kNoVar)
     KT_TRY(ctx->lalRects.reserve(4ull * R));
-    KT_HIP(hipMemcpyAsync(ctx->lalRects.p, rects.data(), R * sizeof(uint4), hipMemcpyHostToDevice, ctx->stream));
+    KT_TRY(lalLandReserve(ctx, R * sizeof(uint4)));
+    std::memcpy(landVar(ctx), rects.data(), R * sizeof(uint4));
+    KT_HIP(hipMemcpyAsync(ctx->lalRects.p, landVar(ctx), R * sizeof(uint4), hipMemcpyHostToDevice, ctx->stream));
     const uint4 *dRects = reinterpret_cast<const uint4 *>(ctx->lalRects.p);
     paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 0);
     KT_HIP(hipGetLastError());
@@ -985,9 +1013,10 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, std::vect
         KT_HIP(hipGetLastError());
         return KT_OK;
     };
-    uint32_t M = 0;
-    KT_HIP(hipMemcpyAsync(&M, ctx->lalFlag.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    KT_HIP(hipMemcpyAsync(landSlots(ctx) + kLandM, ctx->lalFlag.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
     KT_HIP(hipStreamSynchronize(ctx->stream));
+    const uint32_t M = landSlots(ctx)[kLandM];
     if (M == 0) return unpaint();
     // restore points of the batch's cells and their bins at the start
     for (auto *b : {&ctx->lalRegionOf, &ctx->lalSaveBin, &ctx->lalBinStart}) KT_TRY(b->reserve(n));
@@ -1073,8 +1102,10 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, std::vect
                 KT_HIP(hipGetLastError());
             }
             std::vector<SegStat> stats(B);
-            KT_HIP(hipMemcpyAsync(stats.data(), ctx->lalStats.p, B * sizeof(SegStat), hipMemcpyDeviceToHost, ctx->stream));
+            KT_TRY(lalLandReserve(ctx, B * sizeof(SegStat)));
+            KT_HIP(hipMemcpyAsync(landVar(ctx), ctx->lalStats.p, B * sizeof(SegStat), hipMemcpyDeviceToHost, ctx->stream));
             KT_HIP(hipStreamSynchronize(ctx->stream));
+            std::memcpy(stats.data(), landVar(ctx), B * sizeof(SegStat));
 
             // host geometry for every splitting block (simpl_lal.cc:213-263)
             std::vector<uint8_t> split(B, 0);
@@ -1241,20 +1272,25 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, std::vect
     for (uint32_t first = 0;;) {
         KT_TRY(runLevels(active));
         KT_TRY(ctx->lalCount.reserve(2));
-        const uint32_t none[2] = {kNoVar, 0u};
-        KT_HIP(hipMemcpyAsync(ctx->lalCount.p, none, sizeof(none), hipMemcpyHostToDevice, ctx->stream));
+        uint32_t *slot = landSlots(ctx);
+        slot[kLandNone] = kNoVar;
+        slot[kLandNone + 1] = 0u;
+        KT_HIP(hipMemcpyAsync(ctx->lalCount.p, slot + kLandNone, 2 * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              ctx->stream));
         escapeCheck<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, first, bs, ctx->lalCount.p);
         KT_HIP(hipGetLastError());
-        uint32_t bad = kNoVar;
-        KT_HIP(hipMemcpyAsync(&bad, ctx->lalCount.p, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        KT_HIP(hipMemcpyAsync(slot + kLandBad, ctx->lalCount.p, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
         KT_HIP(hipStreamSynchronize(ctx->stream));
+        const uint32_t bad = slot[kLandBad];
         if (bad == kNoVar) break;
         repairBatch<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, bad, bs);
         KT_HIP(hipGetLastError());
         resegBatch<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, bad, bs, ctx->lalCount.p + 1);
         KT_HIP(hipGetLastError());
-        KT_HIP(hipMemcpyAsync(&active, ctx->lalCount.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        KT_HIP(hipMemcpyAsync(slot + kLandActive, ctx->lalCount.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
         KT_HIP(hipStreamSynchronize(ctx->stream));
+        active = slot[kLandActive];
         first = bad;
         ++ctx->lalReruns;
     }
@@ -1267,13 +1303,16 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, std::vect
                                                    ctx->area.p, gridDev(g), ctx->lalBinStart.p, ctx->binOf.p, dMoved,
                                                    ctx->lalCount.p);
     KT_HIP(hipGetLastError());
-    uint32_t moves = 0;
-    KT_HIP(hipMemcpyAsync(&moves, ctx->lalCount.p, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    KT_HIP(hipMemcpyAsync(landSlots(ctx) + kLandMoves, ctx->lalCount.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
     KT_HIP(hipStreamSynchronize(ctx->stream));
+    const uint32_t moves = landSlots(ctx)[kLandMoves];
     moved.resize(moves);
+    KT_TRY(lalLandReserve(ctx, moves * sizeof(Moved)));
     KT_TRY(unpaint());
-    if (moves) KT_HIP(hipMemcpyAsync(moved.data(), dMoved, moves * sizeof(Moved), hipMemcpyDeviceToHost, ctx->stream));
+    if (moves) KT_HIP(hipMemcpyAsync(landVar(ctx), dMoved, moves * sizeof(Moved), hipMemcpyDeviceToHost, ctx->stream));
     KT_HIP(hipStreamSynchronize(ctx->stream));
+    if (moves) std::memcpy(moved.data(), landVar(ctx), moves * sizeof(Moved));
     return KT_OK;
 }
 
