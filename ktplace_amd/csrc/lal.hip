@@ -293,6 +293,7 @@ struct Moved {
     uint32_t oldBin, newBin;
     unsigned long long area;  // 2^-16 units
 };
+static_assert(sizeof(Moved) == 2 * sizeof(unsigned long long), "a moved record is two words of lalMoved");
 
 __global__ void __launch_bounds__(kB)
 movedCells(const uint32_t *__restrict__ list, uint32_t m, const double *__restrict__ x, const double *__restrict__ y,
@@ -1303,15 +1304,26 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, std::vect
                                                    ctx->area.p, gridDev(g), ctx->lalBinStart.p, ctx->binOf.p, dMoved,
                                                    ctx->lalCount.p);
     KT_HIP(hipGetLastError());
+    // a batch of up to kMovedInline cells brings its whole moved list back
+    // with the count (one round trip); a larger one reads the count first
+    constexpr uint32_t kMovedInline = 4096;
+    const bool inlineMoved = M <= kMovedInline;
+    // lalMoved holds M records (two words each): at most one per batch cell
+    if (inlineMoved) KT_TRY(lalLandReserve(ctx, static_cast<size_t>(M) * sizeof(Moved)));
     KT_HIP(hipMemcpyAsync(landSlots(ctx) + kLandMoves, ctx->lalCount.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
                           ctx->stream));
+    if (inlineMoved)
+        KT_HIP(hipMemcpyAsync(landVar(ctx), dMoved, static_cast<size_t>(M) * sizeof(Moved), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    KT_TRY(unpaint());
     KT_HIP(hipStreamSynchronize(ctx->stream));
     const uint32_t moves = landSlots(ctx)[kLandMoves];
     moved.resize(moves);
-    KT_TRY(lalLandReserve(ctx, moves * sizeof(Moved)));
-    KT_TRY(unpaint());
-    if (moves) KT_HIP(hipMemcpyAsync(landVar(ctx), dMoved, moves * sizeof(Moved), hipMemcpyDeviceToHost, ctx->stream));
-    KT_HIP(hipStreamSynchronize(ctx->stream));
+    if (!inlineMoved && moves) {
+        KT_TRY(lalLandReserve(ctx, moves * sizeof(Moved)));
+        KT_HIP(hipMemcpyAsync(landVar(ctx), dMoved, moves * sizeof(Moved), hipMemcpyDeviceToHost, ctx->stream));
+        KT_HIP(hipStreamSynchronize(ctx->stream));
+    }
     if (moves) std::memcpy(moved.data(), landVar(ctx), moves * sizeof(Moved));
     return KT_OK;
 }
