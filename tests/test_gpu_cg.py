@@ -436,3 +436,34 @@ def test_three_byte_columns_are_bit_identical(b2b12k, gpu_ctx_factory, precision
         for x, y in out[1:]:
             assert np.array_equal(x, out[0][0]) and np.array_equal(y, out[0][1])
     ctx.close()
+
+
+def test_launch_shape_setters_reject_bad_values(gpu_ctx_factory):
+    """The entry-block shape setters accept only the shapes that are built and
+    fail with a message otherwise; a rejected call leaves the context usable."""
+    from ktplace_amd import simpl
+    from ktplace_amd.simpl import KT_SPMV_ENTRY_BLOCKS, KtError, lib
+    ctx = gpu_ctx_factory()
+    bad = (dict(block_threads=128), dict(block_threads=256, block_entries=8192),
+           dict(block_threads=512, block_entries=2048), dict(block_threads=512, unroll=8),
+           dict(block_threads=1024, block_entries=2048), dict(row_lanes=2), dict(unroll=2),
+           dict(block_entries=1000), dict(col_bytes=2), dict(col_bytes=5))
+    for kw in bad:
+        with pytest.raises(KtError):
+            ctx.set_tuning(lanes=4, tile=2048, layout=KT_SPMV_ENTRY_BLOCKS, **kw)
+    for kw in (dict(block_threads=256, block_entries=2048, col_bytes=3), dict(block_threads=512, block_entries=4096),
+               dict(block_threads=1024, block_entries=16384, unroll=8)):
+        ctx.set_tuning(lanes=4, tile=2048, layout=KT_SPMV_ENTRY_BLOCKS, **kw)
+    rng = np.random.default_rng(11)
+    s = spd_system(rng)
+    ctx.set_system(0, s)
+    ctx.set_system(1, s)
+    x, y = np.zeros(s.n), np.zeros(s.n)
+    r = ctx.solve_both(x, y, 1e-10, 100000)
+    assert max(r["resid"]) <= 1e-10
+    b = simpl.Batch(0)
+    for v in (2, 5):
+        assert lib.kt_batch_set_columns(b.h, v) != 0
+    assert lib.kt_batch_set_columns(b.h, 3) == 0
+    b.close()
+    ctx.close()
