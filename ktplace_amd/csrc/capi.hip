@@ -29,7 +29,7 @@ int solveHistory(kt_ctx *ctx, const kt_solve_result &res, std::vector<double> &h
 int multiply(kt_ctx *ctx, int axis, const double *dIn, double *dOut);
 int lalSetDesign(kt_ctx *ctx, const std::vector<lal::RowSpec> &rows, const lal::Box &die);
 int lalBuildGrid(kt_ctx *ctx, uint64_t perAxis, double densityLimit);
-int lalOccupancy(kt_ctx *ctx, std::vector<double> *occHost);
+int lalOccupancy(kt_ctx *ctx, std::vector<double> *occHost, std::vector<uint32_t> *countHost = nullptr);
 int lalOverflow(kt_ctx *ctx, double *out);
 int lalHpwl(kt_ctx *ctx, double *out);
 int lalLegalize(kt_ctx *ctx);
@@ -641,7 +641,7 @@ int kt_occupancy(kt_ctx *ctx, double *occ) {
     KT_CHECK(ctx->posCount == ctx->n, KT_ESTATE, "kt_occupancy: positions must have one entry per movable cell");
     KT_CATCH({
         std::vector<double> o;
-        KT_TRY(lalOccupancy(ctx, &o));
+        KT_TRY(lalOccupancy(ctx, &o, nullptr));
         std::copy(o.begin(), o.end(), occ);
         return KT_OK;
     })

@@ -260,6 +260,7 @@ struct kt_ctx {
     kt::DevBuf<double> lalSaveX, lalSaveY;
     uint64_t lalReruns = 0;                            // region runs repeated for the in-order rule
     kt::DevBuf<unsigned long long> lalMoved;           // moved-cell records (2 words each)
+    kt::DevBuf<uint32_t> binCells;                      // cells per bin (with occFix)
     kt::lal::AreaTables lalTables;                     // free-area table of the grid, occupancy rows
     uint8_t *lalPinned = nullptr;                      // pinned staging of per-level tables
     size_t lalPinnedCap = 0;

@@ -97,16 +97,18 @@ __device__ __forceinline__ unsigned long long waveSumU64(unsigned long long v) {
 __global__ void __launch_bounds__(kB)
 cellBins(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ w,
          const double *__restrict__ h, const double *__restrict__ area, GridDev g, uint32_t n,
-         uint32_t *__restrict__ binOf, unsigned long long *__restrict__ occFix) {
-    // the block's bins and sums, open addressing; a bin that finds no free
-    // slot within kProbes is added to memory directly
+         uint32_t *__restrict__ binOf, unsigned long long *__restrict__ occFix, uint32_t *__restrict__ cellCount) {
+    // the block's bins, area sums and cell counts, open addressing; a bin
+    // that finds no free slot within kProbes is added to memory directly
     constexpr uint32_t kSlots = 256, kEmpty = 0xffffffffu;
     constexpr int kProbes = 8;
     __shared__ uint32_t keys[kSlots];
     __shared__ unsigned long long sums[kSlots];
+    __shared__ uint32_t cells[kSlots];
     for (uint32_t t = threadIdx.x; t < kSlots; t += kB) {
         keys[t] = kEmpty;
         sums[t] = 0ull;
+        cells[t] = 0u;
     }
     __syncthreads();
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kB;
@@ -127,6 +129,7 @@ cellBins(const double *__restrict__ x, const double *__restrict__ y, const doubl
             const uint32_t lk = __shfl(k, leader);
             const bool mine = todo && k == lk;
             const unsigned long long s = waveSumU64(mine ? v : 0ull);
+            const uint32_t c = static_cast<uint32_t>(__popcll(__ballot(mine)));
             if (static_cast<int>(threadIdx.x & 63) == leader) {
                 uint32_t slot = (lk * 2654435761u) & (kSlots - 1);
                 bool kept = false;
@@ -134,12 +137,16 @@ cellBins(const double *__restrict__ x, const double *__restrict__ y, const doubl
                     const uint32_t prev = atomicCAS(&keys[slot], kEmpty, lk);
                     if (prev == kEmpty || prev == lk) {
                         atomicAdd(&sums[slot], s);
+                        atomicAdd(&cells[slot], c);
                         kept = true;
                         break;
                     }
                     slot = (slot + 1) & (kSlots - 1);
                 }
-                if (!kept) atomicAdd(occFix + lk, s);
+                if (!kept) {
+                    atomicAdd(occFix + lk, s);
+                    atomicAdd(cellCount + lk, c);
+                }
             }
             todo = todo && !mine;
             pending = __ballot(todo);
@@ -147,7 +154,10 @@ cellBins(const double *__restrict__ x, const double *__restrict__ y, const doubl
     }
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < kSlots; t += kB)
-        if (keys[t] != kEmpty) atomicAdd(occFix + keys[t], sums[t]);
+        if (keys[t] != kEmpty) {
+            atomicAdd(occFix + keys[t], sums[t]);
+            atomicAdd(cellCount + keys[t], cells[t]);
+        }
 }
 
 __global__ void __launch_bounds__(kB)
@@ -857,20 +867,23 @@ GridDev gridDev(const lal::Grid &g) {
 
 // binOf and occupancy of the context's positions (device), occupancy copied
 // to host `occ` when non-null.
-int lalOccupancy(kt_ctx *ctx, std::vector<double> *occHost) {
+int lalOccupancy(kt_ctx *ctx, std::vector<double> *occHost, std::vector<uint32_t> *countHost) {
     KT_CHECK(ctx->grid.size() > 0 && ctx->gridAvail.cap >= ctx->grid.size(), KT_ESTATE, "occupancy: no grid");
     const uint32_t n = ctx->n;
     const uint64_t m = ctx->grid.size();
     KT_TRY(ctx->binOf.reserve(std::max<uint32_t>(n, 1)));
     KT_TRY(ctx->occFix.reserve(m));
     KT_TRY(ctx->occ.reserve(m));
+    KT_TRY(ctx->binCells.reserve(m));
     KT_HIP(hipMemsetAsync(ctx->occFix.p, 0, m * sizeof(uint64_t), ctx->stream));
+    KT_HIP(hipMemsetAsync(ctx->binCells.p, 0, m * sizeof(uint32_t), ctx->stream));
     if (n) {
         // a few cells per thread: each block's bins reach memory once
         const dim3 grid(std::min<uint32_t>(gridFor(n).x, 2048u));
         cellBins<<<grid, kB, 0, ctx->stream>>>(ctx->posX.p, ctx->posY.p, ctx->width.p, ctx->height.p,
                                                      ctx->area.p, gridDev(ctx->grid), n, ctx->binOf.p,
-                                                     reinterpret_cast<unsigned long long *>(ctx->occFix.p));
+                                                     reinterpret_cast<unsigned long long *>(ctx->occFix.p),
+                                                     ctx->binCells.p);
         KT_HIP(hipGetLastError());
     }
     fixToDouble<<<gridFor(m), kB, 0, ctx->stream>>>(reinterpret_cast<unsigned long long *>(ctx->occFix.p),
@@ -879,13 +892,18 @@ int lalOccupancy(kt_ctx *ctx, std::vector<double> *occHost) {
     if (occHost) {
         occHost->resize(m);
         KT_HIP(hipMemcpyAsync(occHost->data(), ctx->occ.p, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        KT_HIP(hipStreamSynchronize(ctx->stream));
     }
+    if (countHost) {
+        countHost->resize(m);
+        KT_HIP(hipMemcpyAsync(countHost->data(), ctx->binCells.p, m * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    }
+    if (occHost || countHost) KT_HIP(hipStreamSynchronize(ctx->stream));
     return KT_OK;
 }
 
 int lalOverflow(kt_ctx *ctx, double *out) {
-    KT_TRY(lalOccupancy(ctx, nullptr));
+    KT_TRY(lalOccupancy(ctx, nullptr, nullptr));
     const uint64_t m = ctx->grid.size();
     const uint32_t blocks = static_cast<uint32_t>((m + kB - 1) / kB);
     KT_TRY(ctx->lalScratch.reserve(blocks + 1));
@@ -972,7 +990,8 @@ inline uint8_t *landVar(kt_ctx *ctx) { return ctx->lalLand + kLandScalars * size
 
 // Legalizes one batch of non-touching regions; the cells whose bin changed
 // are returned in `moved` (their device binOf is updated).
-int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, std::vector<Moved> &moved) {
+int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std::vector<uint32_t> &binCells,
+                std::vector<Moved> &moved) {
     const uint32_t n = ctx->n;
     const lal::Grid &g = ctx->grid;
     moved.clear();
@@ -1009,15 +1028,25 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, std::vect
     KT_TRY(scanInclusive(ctx, ctx->lalFlag.p, n));
     compactIds<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalSeg.p, ctx->lalFlag.p, n, ctx->lalList.p);
     KT_HIP(hipGetLastError());
+    // the device's count of the batch's cells, kept aside (the levels reuse
+    // lalFlag) for the check against the host's count below
+    KT_TRY(ctx->lalCount.reserve(4));
+    KT_HIP(hipMemcpyAsync(ctx->lalCount.p + 3, ctx->lalFlag.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                          ctx->stream));
     const auto unpaint = [&]() -> int {
         paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 1);
         KT_HIP(hipGetLastError());
         return KT_OK;
     };
-    KT_HIP(hipMemcpyAsync(landSlots(ctx) + kLandM, ctx->lalFlag.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    KT_HIP(hipStreamSynchronize(ctx->stream));
-    const uint32_t M = landSlots(ctx)[kLandM];
+    // the batch's cell count from the host's cells per bin (the regions do
+    // not overlap): no round trip; the device's own count (the flag scan's
+    // last element) is checked against it with the batch's last transfer
+    uint64_t cellsInRegions = 0;
+    for (const lal::BinRect &b : regions)
+        for (size_t iy = b.y0; iy <= b.y1; ++iy)
+            for (size_t ix = b.x0; ix <= b.x1; ++ix) cellsInRegions += binCells[g.at(ix, iy)];
+    KT_CHECK(cellsInRegions <= n, KT_ESTATE, "lal: cells per bin out of step with the device");
+    const uint32_t M = static_cast<uint32_t>(cellsInRegions);
     if (M == 0) return unpaint();
     // restore points of the batch's cells and their bins at the start
     for (auto *b : {&ctx->lalRegionOf, &ctx->lalSaveBin, &ctx->lalBinStart}) KT_TRY(b->reserve(n));
@@ -1269,19 +1298,42 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, std::vect
     // when that happens, the regions from the first such one on are restored
     // and run again with the cells their bins hold now (the earlier regions'
     // results stand), until no cell lands in a later region.
+    // Each pass: the levels, the escape check, and (speculatively) the moved
+    // cells, brought back in one round trip.  A cell that moved into a later
+    // region (bad != kNoVar) discards the speculative moved list: the repair
+    // rewrites every batch cell's bin and the regions from `bad` run again.
+    // lalCount: [0] first bad region, [1] cells of the rerun, [2] moved cells,
+    // [3] the device's count of the batch's cells.
+    constexpr uint32_t kMovedInline = 4096;  // up to this many cells, the moved list comes with the counts
+    const bool inlineMoved = M <= kMovedInline;
+    KT_TRY(ctx->lalMoved.reserve(2ull * M));  // M records of two words
+    Moved *dMoved = reinterpret_cast<Moved *>(ctx->lalMoved.p);
     uint32_t active = M;
     for (uint32_t first = 0;;) {
         KT_TRY(runLevels(active));
-        KT_TRY(ctx->lalCount.reserve(2));
+        if (inlineMoved) KT_TRY(lalLandReserve(ctx, static_cast<size_t>(M) * sizeof(Moved)));
         uint32_t *slot = landSlots(ctx);
         slot[kLandNone] = kNoVar;
         slot[kLandNone + 1] = 0u;
-        KT_HIP(hipMemcpyAsync(ctx->lalCount.p, slot + kLandNone, 2 * sizeof(uint32_t), hipMemcpyHostToDevice,
+        slot[kLandNone + 2] = 0u;
+        KT_HIP(hipMemcpyAsync(ctx->lalCount.p, slot + kLandNone, 3 * sizeof(uint32_t), hipMemcpyHostToDevice,
                               ctx->stream));
         escapeCheck<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, first, bs, ctx->lalCount.p);
         KT_HIP(hipGetLastError());
+        movedCells<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, ctx->posX.p, ctx->posY.p, ctx->width.p,
+                                                       ctx->height.p, ctx->area.p, gridDev(g), ctx->lalBinStart.p,
+                                                       ctx->binOf.p, dMoved, ctx->lalCount.p + 2);
+        KT_HIP(hipGetLastError());
         KT_HIP(hipMemcpyAsync(slot + kLandBad, ctx->lalCount.p, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        KT_HIP(hipMemcpyAsync(slot + kLandMoves, ctx->lalCount.p + 2, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        KT_HIP(hipMemcpyAsync(slot + kLandM, ctx->lalCount.p + 3, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        if (inlineMoved)
+            KT_HIP(hipMemcpyAsync(landVar(ctx), dMoved, static_cast<size_t>(M) * sizeof(Moved), hipMemcpyDeviceToHost,
+                                  ctx->stream));
         KT_HIP(hipStreamSynchronize(ctx->stream));
+        KT_CHECK(slot[kLandM] == M, KT_ESTATE, "lal: the batch's cell count differs from the cells per bin");
         const uint32_t bad = slot[kLandBad];
         if (bad == kNoVar) break;
         repairBatch<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, bad, bs);
@@ -1295,28 +1347,7 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, std::vect
         first = bad;
         ++ctx->lalReruns;
     }
-    // the batch's cells whose bin changed
-    KT_TRY(ctx->lalMoved.reserve(2ull * M));
-    KT_TRY(ctx->lalCount.reserve(1));
-    KT_HIP(hipMemsetAsync(ctx->lalCount.p, 0, sizeof(uint32_t), ctx->stream));
-    Moved *dMoved = reinterpret_cast<Moved *>(ctx->lalMoved.p);
-    movedCells<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, ctx->posX.p, ctx->posY.p, ctx->width.p, ctx->height.p,
-                                                   ctx->area.p, gridDev(g), ctx->lalBinStart.p, ctx->binOf.p, dMoved,
-                                                   ctx->lalCount.p);
-    KT_HIP(hipGetLastError());
-    // a batch of up to kMovedInline cells brings its whole moved list back
-    // with the count (one round trip); a larger one reads the count first
-    constexpr uint32_t kMovedInline = 4096;
-    const bool inlineMoved = M <= kMovedInline;
-    // lalMoved holds M records (two words each): at most one per batch cell
-    if (inlineMoved) KT_TRY(lalLandReserve(ctx, static_cast<size_t>(M) * sizeof(Moved)));
-    KT_HIP(hipMemcpyAsync(landSlots(ctx) + kLandMoves, ctx->lalCount.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    if (inlineMoved)
-        KT_HIP(hipMemcpyAsync(landVar(ctx), dMoved, static_cast<size_t>(M) * sizeof(Moved), hipMemcpyDeviceToHost,
-                              ctx->stream));
-    KT_TRY(unpaint());
-    KT_HIP(hipStreamSynchronize(ctx->stream));
+    // the batch's cells whose bin changed (binOf holds every batch cell's bin)
     const uint32_t moves = landSlots(ctx)[kLandMoves];
     moved.resize(moves);
     if (!inlineMoved && moves) {
@@ -1325,7 +1356,7 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, std::vect
         KT_HIP(hipStreamSynchronize(ctx->stream));
     }
     if (moves) std::memcpy(moved.data(), landVar(ctx), moves * sizeof(Moved));
-    return KT_OK;
+    return unpaint();
 }
 
 // LookAheadLegalizer::legalize (simpl_lal.cc:43-124) on the context's positions.
@@ -1334,7 +1365,8 @@ int lalLegalize(kt_ctx *ctx) {
     KT_CHECK(ctx->posCount == ctx->n, KT_ESTATE, "lal: positions must have one entry per movable cell");
     const lal::Grid &g = ctx->grid;
     std::vector<double> occ;
-    KT_TRY(lalOccupancy(ctx, &occ));
+    std::vector<uint32_t> cells;  // cells per bin, kept in step with the device's binOf
+    KT_TRY(lalOccupancy(ctx, &occ, &cells));
     // occupancy is kept on the host from here on and updated with the cells
     // that change bin, as the reference does (simpl_lal.cc:86-93)
     std::vector<uint64_t> occFix(g.size());
@@ -1357,11 +1389,13 @@ int lalLegalize(kt_ctx *ctx) {
     uint64_t batches = 0;
     const auto runBatch = [&]() -> int {
         if (batch.empty()) return KT_OK;
-        KT_TRY(lalRunBatch(ctx, batch, moved));
+        KT_TRY(lalRunBatch(ctx, batch, cells, moved));
         ++batches;
         batch.clear();
         reach.clear();
         for (const Moved &mv : moved) {
+            --cells[mv.oldBin];
+            ++cells[mv.newBin];
             occFix[mv.oldBin] -= mv.area;
             occFix[mv.newBin] += mv.area;
             occ[mv.oldBin] = static_cast<double>(occFix[mv.oldBin]) / kFixScale;

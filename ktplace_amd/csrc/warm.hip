@@ -33,7 +33,7 @@
 
 namespace kt {
 
-int lalOccupancy(kt_ctx *ctx, std::vector<double> *occHost);
+int lalOccupancy(kt_ctx *ctx, std::vector<double> *occHost, std::vector<uint32_t> *countHost = nullptr);
 int mlpInferLaunch(hipStream_t stream, const uint16_t *w, const float *b, const uint16_t *feat, uint64_t count,
                    float *out);
 int mlpPackLaunch(hipStream_t stream, const float *params, uint16_t *w, float *b);
@@ -302,7 +302,7 @@ int warmFeaturesRun(kt_ctx *ctx, float *feat32) {
     const uint32_t n = ctx->n;
     const uint32_t chunks = (n + kChunk - 1) / kChunk;
     const bool haveGrid = ctx->grid.size() > 0 && ctx->gridAvail.cap >= ctx->grid.size();
-    if (haveGrid) KT_TRY(lalOccupancy(ctx, nullptr));
+    if (haveGrid) KT_TRY(lalOccupancy(ctx, nullptr, nullptr));
     KT_TRY(ctx->warmPart.reserve(static_cast<uint64_t>(chunks) * 6));
     KT_TRY(ctx->warmScalars.reserve(sizeof(WarmScalars) / sizeof(double)));
     KT_TRY(ctx->warmFeat.reserve(static_cast<uint64_t>(n) * KT_MLP_INPUTS));
