@@ -1289,6 +1289,18 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
             blocks.swap(kept);
             vertical = !vertical;
             A = A2;
+            // The next level would split no block when every half holds at
+            // most minCellsToSplit cells (their counts are the split's m and
+            // count - m) or the level cap is reached: it would only sort and
+            // measure, so it and its round trip are skipped.
+            bool more = level + 1 < ctx->lalOpts.maxLevel;
+            if (more) {
+                more = false;
+                for (uint32_t s = 0; s < B && !more; ++s)
+                    more = split[s] && (stats[s].m > ctx->lalOpts.minCellsToSplit ||
+                                        stats[s].count - stats[s].m > ctx->lalOpts.minCellsToSplit);
+            }
+            if (!more) break;
         }
         return KT_OK;
     };
