@@ -239,6 +239,28 @@ regionFlags(const uint32_t *__restrict__ binOf, const uint32_t *__restrict__ bin
     flag[i] = r != kNoVar ? 1u : 0u;
 }
 
+// The batch's cells in any order (the small path sorts them on a total
+// order itself): seg as regionFlags, and each wave appends its members to
+// list with one atomic on count.
+__global__ void __launch_bounds__(kB)
+regionMembers(const uint32_t *__restrict__ binOf, const uint32_t *__restrict__ binRegion, uint32_t n,
+              uint32_t *__restrict__ seg, uint32_t *__restrict__ list, uint32_t *__restrict__ count) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t r = valid ? binRegion[binOf[i]] : kNoVar;
+    if (valid) seg[i] = r;
+    const bool member = r != kNoVar;
+    const unsigned long long mask = __ballot(member);
+    if (mask == 0ull) return;
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader);
+    const uint32_t below = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+    if (member) list[base + below] = i;
+}
+
 __global__ void __launch_bounds__(kB)
 activeFlags(const uint32_t *__restrict__ seg, uint32_t n, uint32_t *__restrict__ flag) {
     const uint32_t i = blockIdx.x * kB + threadIdx.x;
@@ -1022,31 +1044,46 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
                     &ctx->lalStripeOf, &ctx->lalList})
         KT_TRY(b->reserve(n));
     for (auto *b : {&ctx->lalKeys, &ctx->lalKeysTmp, &ctx->lalKey2, &ctx->lalPrefix}) KT_TRY(b->reserve(n));
-    // the batch's cells, in id order
-    regionFlags<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->binOf.p, ctx->binRegion.p, n, ctx->lalSeg.p, ctx->lalFlag.p);
-    KT_HIP(hipGetLastError());
-    KT_TRY(scanInclusive(ctx, ctx->lalFlag.p, n));
-    compactIds<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalSeg.p, ctx->lalFlag.p, n, ctx->lalList.p);
-    KT_HIP(hipGetLastError());
-    // the device's count of the batch's cells, kept aside (the levels reuse
-    // lalFlag) for the check against the host's count below
-    KT_TRY(ctx->lalCount.reserve(4));
-    KT_HIP(hipMemcpyAsync(ctx->lalCount.p + 3, ctx->lalFlag.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToDevice,
-                          ctx->stream));
-    const auto unpaint = [&]() -> int {
-        paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 1);
-        KT_HIP(hipGetLastError());
-        return KT_OK;
-    };
     // the batch's cell count from the host's cells per bin (the regions do
-    // not overlap): no round trip; the device's own count (the flag scan's
-    // last element) is checked against it with the batch's last transfer
+    // not overlap): no round trip; the device's own count is checked against
+    // it with the batch's last transfer
     uint64_t cellsInRegions = 0;
     for (const lal::BinRect &b : regions)
         for (size_t iy = b.y0; iy <= b.y1; ++iy)
             for (size_t ix = b.x0; ix <= b.x1; ++ix) cellsInRegions += binCells[g.at(ix, iy)];
     KT_CHECK(cellsInRegions <= n, KT_ESTATE, "lal: cells per bin out of step with the device");
     const uint32_t M = static_cast<uint32_t>(cellsInRegions);
+    // batches of up to kSmallCells cells: one workgroup per half level
+    static const bool smallPath = [] {
+        const char *v = std::getenv("KTPLACE_LAL_SMALL");
+        return !(v && v[0] == '0');
+    }();
+    const bool small = smallPath && M <= kSmallCells;
+    // the batch's cells: for the small path in any order, appended in one
+    // pass; otherwise in id order (the stable radix sorts start from it).
+    // lalCount[3] receives the device's count of them.
+    KT_TRY(ctx->lalCount.reserve(4));
+    if (small) {
+        KT_HIP(hipMemsetAsync(ctx->lalCount.p + 3, 0, sizeof(uint32_t), ctx->stream));
+        regionMembers<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->binOf.p, ctx->binRegion.p, n, ctx->lalSeg.p,
+                                                          ctx->lalList.p, ctx->lalCount.p + 3);
+        KT_HIP(hipGetLastError());
+    } else {
+        regionFlags<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->binOf.p, ctx->binRegion.p, n, ctx->lalSeg.p,
+                                                        ctx->lalFlag.p);
+        KT_HIP(hipGetLastError());
+        KT_TRY(scanInclusive(ctx, ctx->lalFlag.p, n));
+        compactIds<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalSeg.p, ctx->lalFlag.p, n, ctx->lalList.p);
+        KT_HIP(hipGetLastError());
+        // kept aside: the levels reuse lalFlag
+        KT_HIP(hipMemcpyAsync(ctx->lalCount.p + 3, ctx->lalFlag.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+    }
+    const auto unpaint = [&]() -> int {
+        paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 1);
+        KT_HIP(hipGetLastError());
+        return KT_OK;
+    };
     if (M == 0) return unpaint();
     // restore points of the batch's cells and their bins at the start
     for (auto *b : {&ctx->lalRegionOf, &ctx->lalSaveBin, &ctx->lalBinStart}) KT_TRY(b->reserve(n));
@@ -1057,12 +1094,6 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
     const uint32_t *list = ctx->lalList.p;
     saveBatch<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, bs);
     KT_HIP(hipGetLastError());
-    // batches of up to kSmallCells cells: one workgroup per half level
-    static const bool smallPath = [] {
-        const char *v = std::getenv("KTPLACE_LAL_SMALL");
-        return !(v && v[0] == '0');
-    }();
-    const bool small = smallPath && M <= kSmallCells;
     uint32_t width = 2;
     while (width < M) width <<= 1;
 
