@@ -993,7 +993,8 @@ int lalStaging(kt_ctx *ctx, size_t bytes) {
 // the next one is staged).  It grows only after a synchronisation, when
 // nothing in flight uses it.
 constexpr size_t kLandScalars = 16;
-enum LandSlot { kLandM = 0, kLandBad = 1, kLandActive = 2, kLandMoves = 3, kLandNone = 4 };
+// kLandCount..+3 receive lalCount[0..3] in one transfer
+enum LandSlot { kLandM = 0, kLandBad = 1, kLandActive = 2, kLandMoves = 3, kLandNone = 4, kLandCount = 8 };
 
 int lalLandReserve(kt_ctx *ctx, size_t varBytes) {
     const size_t bytes = kLandScalars * sizeof(uint32_t) + varBytes;
@@ -1367,17 +1368,15 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
                                                        ctx->height.p, ctx->area.p, gridDev(g), ctx->lalBinStart.p,
                                                        ctx->binOf.p, dMoved, ctx->lalCount.p + 2);
         KT_HIP(hipGetLastError());
-        KT_HIP(hipMemcpyAsync(slot + kLandBad, ctx->lalCount.p, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-        KT_HIP(hipMemcpyAsync(slot + kLandMoves, ctx->lalCount.p + 2, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                              ctx->stream));
-        KT_HIP(hipMemcpyAsync(slot + kLandM, ctx->lalCount.p + 3, sizeof(uint32_t), hipMemcpyDeviceToHost,
+        KT_HIP(hipMemcpyAsync(slot + kLandCount, ctx->lalCount.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
                               ctx->stream));
         if (inlineMoved)
             KT_HIP(hipMemcpyAsync(landVar(ctx), dMoved, static_cast<size_t>(M) * sizeof(Moved), hipMemcpyDeviceToHost,
                                   ctx->stream));
         KT_HIP(hipStreamSynchronize(ctx->stream));
-        KT_CHECK(slot[kLandM] == M, KT_ESTATE, "lal: the batch's cell count differs from the cells per bin");
-        const uint32_t bad = slot[kLandBad];
+        KT_CHECK(slot[kLandCount + 3] == M, KT_ESTATE, "lal: the batch's cell count differs from the cells per bin");
+        slot[kLandMoves] = slot[kLandCount + 2];
+        const uint32_t bad = slot[kLandCount];
         if (bad == kNoVar) break;
         repairBatch<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, bad, bs);
         KT_HIP(hipGetLastError());
