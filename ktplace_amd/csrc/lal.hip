@@ -239,28 +239,6 @@ regionFlags(const uint32_t *__restrict__ binOf, const uint32_t *__restrict__ bin
     flag[i] = r != kNoVar ? 1u : 0u;
 }
 
-// The batch's cells in any order (the small path sorts them on a total
-// order itself): seg as regionFlags, and each wave appends its members to
-// list with one atomic on count.
-__global__ void __launch_bounds__(kB)
-regionMembers(const uint32_t *__restrict__ binOf, const uint32_t *__restrict__ binRegion, uint32_t n,
-              uint32_t *__restrict__ seg, uint32_t *__restrict__ list, uint32_t *__restrict__ count) {
-    const uint32_t i = blockIdx.x * kB + threadIdx.x;
-    const bool valid = i < n;
-    const uint32_t r = valid ? binRegion[binOf[i]] : kNoVar;
-    if (valid) seg[i] = r;
-    const bool member = r != kNoVar;
-    const unsigned long long mask = __ballot(member);
-    if (mask == 0ull) return;
-    const int lane = static_cast<int>(threadIdx.x & 63);
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(count, static_cast<uint32_t>(__popcll(mask)));
-    base = __shfl(base, leader);
-    const uint32_t below = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
-    if (member) list[base + below] = i;
-}
-
 __global__ void __launch_bounds__(kB)
 activeFlags(const uint32_t *__restrict__ seg, uint32_t n, uint32_t *__restrict__ flag) {
     const uint32_t i = blockIdx.x * kB + threadIdx.x;
@@ -319,6 +297,20 @@ paintRegions(const uint4 *__restrict__ rects, uint32_t nx, uint32_t *__restrict_
     for (uint32_t k = threadIdx.x; k < cells; k += kB) binRegion[(r.z + k / w) * nx + r.x + k % w] = v;
 }
 
+// The same for a batch of at most kRectsByValue regions, passed in the
+// launch arguments (no copy to the device first).
+constexpr uint32_t kRectsByValue = 8;
+struct RectSet {
+    uint4 r[kRectsByValue];
+};
+__global__ void __launch_bounds__(kB)
+paintRegionsArg(RectSet rects, uint32_t nx, uint32_t *__restrict__ binRegion, int clear) {
+    const uint4 r = rects.r[blockIdx.x];
+    const uint32_t w = r.y - r.x + 1, cells = w * (r.w - r.z + 1);
+    const uint32_t v = clear ? kNoVar : blockIdx.x;
+    for (uint32_t k = threadIdx.x; k < cells; k += kB) binRegion[(r.z + k / w) * nx + r.x + k % w] = v;
+}
+
 // Cells of the batch whose bin changed: new bin stored, (old, new, area)
 // appended for the host's occupancy update.
 struct Moved {
@@ -366,6 +358,36 @@ __global__ void __launch_bounds__(kB) saveBatch(const uint32_t *__restrict__ lis
     s.saveY[i] = s.y[i];
     s.saveBin[i] = s.binOf[i];
     s.binStart[i] = s.binOf[i];
+}
+
+// The batch's cells in any order (the small path sorts them on a total
+// order itself), with their restore points as saveBatch records them: seg
+// as regionFlags, and each wave appends its members to list with one
+// atomic on count.
+__global__ void __launch_bounds__(kB)
+regionMembers(uint32_t n, BatchState s, uint32_t *__restrict__ list, uint32_t *__restrict__ count) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t bin = valid ? s.binOf[i] : 0u;
+    const uint32_t r = valid ? s.binRegion[bin] : kNoVar;
+    if (valid) s.seg[i] = r;
+    const bool member = r != kNoVar;
+    const unsigned long long mask = __ballot(member);
+    if (mask == 0ull) return;
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader);
+    const uint32_t below = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+    if (member) {
+        list[base + below] = i;
+        s.regionOf[i] = r;
+        s.saveX[i] = s.x[i];
+        s.saveY[i] = s.y[i];
+        s.saveBin[i] = bin;
+        s.binStart[i] = bin;
+    }
 }
 
 __device__ __forceinline__ uint32_t binNow(const BatchState &s, uint32_t i) {
@@ -1032,13 +1054,22 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
         blocks[r] = {g.box(b), g.sum(g.avail, b)};
     }
     const uint32_t R = static_cast<uint32_t>(regions.size());
-    // bin -> region for the batch's bins (every other bin holds kNoVar)
-    KT_TRY(ctx->lalRects.reserve(4ull * R));
-    KT_TRY(lalLandReserve(ctx, R * sizeof(uint4)));
-    std::memcpy(landVar(ctx), rects.data(), R * sizeof(uint4));
-    KT_HIP(hipMemcpyAsync(ctx->lalRects.p, landVar(ctx), R * sizeof(uint4), hipMemcpyHostToDevice, ctx->stream));
-    const uint4 *dRects = reinterpret_cast<const uint4 *>(ctx->lalRects.p);
-    paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 0);
+    // bin -> region for the batch's bins (every other bin holds kNoVar); a
+    // few regions travel in the launch arguments, more through a copy
+    const bool rectsByValue = R <= kRectsByValue;
+    RectSet rectSet{};
+    const uint4 *dRects = nullptr;
+    if (rectsByValue) {
+        for (uint32_t r = 0; r < R; ++r) rectSet.r[r] = rects[r];
+        paintRegionsArg<<<R, kB, 0, ctx->stream>>>(rectSet, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 0);
+    } else {
+        KT_TRY(ctx->lalRects.reserve(4ull * R));
+        KT_TRY(lalLandReserve(ctx, R * sizeof(uint4)));
+        std::memcpy(landVar(ctx), rects.data(), R * sizeof(uint4));
+        KT_HIP(hipMemcpyAsync(ctx->lalRects.p, landVar(ctx), R * sizeof(uint4), hipMemcpyHostToDevice, ctx->stream));
+        dRects = reinterpret_cast<const uint4 *>(ctx->lalRects.p);
+        paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 0);
+    }
     KT_HIP(hipGetLastError());
     // per-cell scratch
     for (auto *b : {&ctx->lalSeg, &ctx->lalFlag, &ctx->lalIds, &ctx->lalVals, &ctx->lalValsTmp, &ctx->lalHalf,
@@ -1062,12 +1093,25 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
     const bool small = smallPath && M <= kSmallCells;
     // the batch's cells: for the small path in any order, appended in one
     // pass; otherwise in id order (the stable radix sorts start from it).
-    // lalCount[3] receives the device's count of them.
+    // lalCount (see below) starts as {kNoVar, 0, 0, 0}; [3] receives the
+    // device's count of the batch's cells.
     KT_TRY(ctx->lalCount.reserve(4));
+    KT_TRY(lalLandReserve(ctx, 0));  // the landing area exists before its first use
+    {
+        uint32_t *slot = landSlots(ctx);
+        slot[kLandNone] = kNoVar;
+        slot[kLandNone + 1] = slot[kLandNone + 2] = slot[kLandNone + 3] = 0u;
+        KT_HIP(hipMemcpyAsync(ctx->lalCount.p, slot + kLandNone, 4 * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              ctx->stream));
+    }
+    // restore points of the batch's cells and their bins at the start
+    for (auto *b : {&ctx->lalRegionOf, &ctx->lalSaveBin, &ctx->lalBinStart}) KT_TRY(b->reserve(n));
+    for (auto *b : {&ctx->lalSaveX, &ctx->lalSaveY}) KT_TRY(b->reserve(n));
+    BatchState bs{ctx->lalRegionOf.p, ctx->lalSaveX.p, ctx->lalSaveY.p, ctx->lalSaveBin.p, ctx->lalBinStart.p,
+                  ctx->posX.p, ctx->posY.p, ctx->width.p, ctx->height.p, ctx->binOf.p, ctx->binRegion.p,
+                  ctx->lalSeg.p, gridDev(g)};
     if (small) {
-        KT_HIP(hipMemsetAsync(ctx->lalCount.p + 3, 0, sizeof(uint32_t), ctx->stream));
-        regionMembers<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->binOf.p, ctx->binRegion.p, n, ctx->lalSeg.p,
-                                                          ctx->lalList.p, ctx->lalCount.p + 3);
+        regionMembers<<<gridFor(n), kB, 0, ctx->stream>>>(n, bs, ctx->lalList.p, ctx->lalCount.p + 3);
         KT_HIP(hipGetLastError());
     } else {
         regionFlags<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->binOf.p, ctx->binRegion.p, n, ctx->lalSeg.p,
@@ -1081,20 +1125,19 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
                               ctx->stream));
     }
     const auto unpaint = [&]() -> int {
-        paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 1);
+        if (rectsByValue)
+            paintRegionsArg<<<R, kB, 0, ctx->stream>>>(rectSet, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 1);
+        else
+            paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 1);
         KT_HIP(hipGetLastError());
         return KT_OK;
     };
     if (M == 0) return unpaint();
-    // restore points of the batch's cells and their bins at the start
-    for (auto *b : {&ctx->lalRegionOf, &ctx->lalSaveBin, &ctx->lalBinStart}) KT_TRY(b->reserve(n));
-    for (auto *b : {&ctx->lalSaveX, &ctx->lalSaveY}) KT_TRY(b->reserve(n));
-    BatchState bs{ctx->lalRegionOf.p, ctx->lalSaveX.p, ctx->lalSaveY.p, ctx->lalSaveBin.p, ctx->lalBinStart.p,
-                  ctx->posX.p, ctx->posY.p, ctx->width.p, ctx->height.p, ctx->binOf.p, ctx->binRegion.p,
-                  ctx->lalSeg.p, gridDev(g)};
     const uint32_t *list = ctx->lalList.p;
-    saveBatch<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, bs);
-    KT_HIP(hipGetLastError());
+    if (!small) {  // the small path's member pass recorded them
+        saveBatch<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, bs);
+        KT_HIP(hipGetLastError());
+    }
     uint32_t width = 2;
     while (width < M) width <<= 1;
 
@@ -1357,11 +1400,6 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
         KT_TRY(runLevels(active));
         if (inlineMoved) KT_TRY(lalLandReserve(ctx, static_cast<size_t>(M) * sizeof(Moved)));
         uint32_t *slot = landSlots(ctx);
-        slot[kLandNone] = kNoVar;
-        slot[kLandNone + 1] = 0u;
-        slot[kLandNone + 2] = 0u;
-        KT_HIP(hipMemcpyAsync(ctx->lalCount.p, slot + kLandNone, 3 * sizeof(uint32_t), hipMemcpyHostToDevice,
-                              ctx->stream));
         escapeCheck<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, first, bs, ctx->lalCount.p);
         KT_HIP(hipGetLastError());
         movedCells<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, ctx->posX.p, ctx->posY.p, ctx->width.p,
@@ -1388,6 +1426,11 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
         active = slot[kLandActive];
         first = bad;
         ++ctx->lalReruns;
+        // counters back to {kNoVar, 0, 0} for the rerun ([3] stays)
+        slot[kLandNone] = kNoVar;
+        slot[kLandNone + 1] = slot[kLandNone + 2] = 0u;
+        KT_HIP(hipMemcpyAsync(ctx->lalCount.p, slot + kLandNone, 3 * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              ctx->stream));
     }
     // the batch's cells whose bin changed (binOf holds every batch cell's bin)
     const uint32_t moves = landSlots(ctx)[kLandMoves];
