@@ -255,7 +255,7 @@ struct kt_ctx {
     kt::DevBuf<uint64_t> occFix, lalKeys, lalKeysTmp, lalKey2, lalPrefix, lalStripeLo, lalStripeHi;
     kt::DevBuf<uint32_t> binOf, binRegion, lalSeg, lalFlag, lalIds, lalVals, lalValsTmp, lalHalf, lalStripeOf,
         lalBegin, lalEnd, lalStripeStart, lalStripeEnd, lalRemap, sortCounts, sortRaw;
-    kt::DevBuf<uint32_t> lalList, lalRects, lalCount;  // batch cells, region rectangles, moved count
+    kt::DevBuf<uint32_t> lalList, lalRects;  // batch cells, region rectangles
     kt::DevBuf<uint32_t> lalRegionOf, lalSaveBin, lalBinStart;  // in-order rule: per-cell restore state
     kt::DevBuf<double> lalSaveX, lalSaveY;
     uint64_t lalReruns = 0;                            // region runs repeated for the in-order rule

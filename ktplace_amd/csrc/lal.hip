@@ -319,22 +319,6 @@ struct Moved {
 };
 static_assert(sizeof(Moved) == 2 * sizeof(unsigned long long), "a moved record is two words of lalMoved");
 
-__global__ void __launch_bounds__(kB)
-movedCells(const uint32_t *__restrict__ list, uint32_t m, const double *__restrict__ x, const double *__restrict__ y,
-           const double *__restrict__ w, const double *__restrict__ h, const double *__restrict__ area, GridDev g,
-           const uint32_t *__restrict__ binStart, uint32_t *__restrict__ binOf, Moved *__restrict__ out,
-           uint32_t *__restrict__ count) {
-    const uint32_t j = blockIdx.x * kB + threadIdx.x;
-    if (j >= m) return;
-    const uint32_t i = list[j];
-    const uint32_t k = binOfPoint(g, x[i] + 0.5 * w[i], y[i] + 0.5 * h[i]);
-    const uint32_t old = binStart[i];
-    binOf[i] = k;
-    if (k == old) return;
-    const uint32_t slot = atomicAdd(count, 1u);
-    out[slot] = Moved{old, k, static_cast<unsigned long long>(llrint(area[i] * kFixScale))};
-}
-
 // Per-cell state of a batch for the in-order rule (see lalRunBatch).
 struct BatchState {
     uint32_t *regionOf;           // region whose run last took the cell
@@ -394,17 +378,28 @@ __device__ __forceinline__ uint32_t binNow(const BatchState &s, uint32_t i) {
     return binOfPoint(s.g, s.x[i] + 0.5 * s.w[i], s.y[i] + 0.5 * s.h[i]);
 }
 
-// out[0] = the first region a cell of a region >= first moved into, among the
-// regions after its own (stays kNoVar when there is none)
+// The end of a pass over the batch's cells: each cell's current bin, computed
+// once, is stored in binOf; cnt[0] = the first region, among those after its
+// own, that a cell of a region >= first moved into (stays kNoVar when there is
+// none); a cell whose bin differs from its bin at the batch start appends
+// (old, new, area) to out, counted in cnt[2], for the host's occupancy update.
 __global__ void __launch_bounds__(kB)
-escapeCheck(const uint32_t *__restrict__ list, uint32_t m, uint32_t first, BatchState s, uint32_t *__restrict__ out) {
+passEnd(const uint32_t *__restrict__ list, uint32_t m, uint32_t first, BatchState s,
+        const double *__restrict__ area, uint32_t *__restrict__ cnt, Moved *__restrict__ out) {
     const uint32_t j = blockIdx.x * kB + threadIdx.x;
     if (j >= m) return;
     const uint32_t i = list[j];
+    const uint32_t k = binNow(s, i);
     const uint32_t r = s.regionOf[i];
-    if (r == kNoVar || r < first) return;
-    const uint32_t rj = s.binRegion[binNow(s, i)];
-    if (rj != kNoVar && rj > r) atomicMin(out, rj);
+    if (r != kNoVar && r >= first) {
+        const uint32_t rj = s.binRegion[k];
+        if (rj != kNoVar && rj > r) atomicMin(cnt, rj);
+    }
+    s.binOf[i] = k;
+    const uint32_t old = s.binStart[i];
+    if (k == old) return;
+    const uint32_t slot = atomicAdd(cnt + 2, 1u);
+    out[slot] = Moved{old, k, static_cast<unsigned long long>(llrint(area[i] * kFixScale))};
 }
 
 // regions >= bad back to their restore points; earlier ones keep their result
@@ -1015,8 +1010,7 @@ int lalStaging(kt_ctx *ctx, size_t bytes) {
 // the next one is staged).  It grows only after a synchronisation, when
 // nothing in flight uses it.
 constexpr size_t kLandScalars = 16;
-// kLandCount..+3 receive lalCount[0..3] in one transfer
-enum LandSlot { kLandM = 0, kLandBad = 1, kLandActive = 2, kLandMoves = 3, kLandNone = 4, kLandCount = 8 };
+enum LandSlot { kLandActive = 2, kLandNone = 4 };
 
 int lalLandReserve(kt_ctx *ctx, size_t varBytes) {
     const size_t bytes = kLandScalars * sizeof(uint32_t) + varBytes;
@@ -1093,16 +1087,20 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
     const bool small = smallPath && M <= kSmallCells;
     // the batch's cells: for the small path in any order, appended in one
     // pass; otherwise in id order (the stable radix sorts start from it).
-    // lalCount (see below) starts as {kNoVar, 0, 0, 0}; [3] receives the
-    // device's count of the batch's cells.
-    KT_TRY(ctx->lalCount.reserve(4));
+    // lalMoved holds the batch's four counters (two words) and then up to M
+    // moved records (two words each), so one transfer returns both.  The
+    // counters: [0] first region a cell escaped into (kNoVar: none), [1]
+    // cells of a rerun, [2] moved records, [3] the device's count of the
+    // batch's cells; they start as {kNoVar, 0, 0, 0}.
+    KT_TRY(ctx->lalMoved.reserve(2ull + 2ull * M));
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(ctx->lalMoved.p);
+    Moved *dMoved = reinterpret_cast<Moved *>(ctx->lalMoved.p + 2);
     KT_TRY(lalLandReserve(ctx, 0));  // the landing area exists before its first use
     {
         uint32_t *slot = landSlots(ctx);
         slot[kLandNone] = kNoVar;
         slot[kLandNone + 1] = slot[kLandNone + 2] = slot[kLandNone + 3] = 0u;
-        KT_HIP(hipMemcpyAsync(ctx->lalCount.p, slot + kLandNone, 4 * sizeof(uint32_t), hipMemcpyHostToDevice,
-                              ctx->stream));
+        KT_HIP(hipMemcpyAsync(cnt, slot + kLandNone, 4 * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
     }
     // restore points of the batch's cells and their bins at the start
     for (auto *b : {&ctx->lalRegionOf, &ctx->lalSaveBin, &ctx->lalBinStart}) KT_TRY(b->reserve(n));
@@ -1111,7 +1109,7 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
                   ctx->posX.p, ctx->posY.p, ctx->width.p, ctx->height.p, ctx->binOf.p, ctx->binRegion.p,
                   ctx->lalSeg.p, gridDev(g)};
     if (small) {
-        regionMembers<<<gridFor(n), kB, 0, ctx->stream>>>(n, bs, ctx->lalList.p, ctx->lalCount.p + 3);
+        regionMembers<<<gridFor(n), kB, 0, ctx->stream>>>(n, bs, ctx->lalList.p, cnt + 3);
         KT_HIP(hipGetLastError());
     } else {
         regionFlags<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->binOf.p, ctx->binRegion.p, n, ctx->lalSeg.p,
@@ -1121,7 +1119,7 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
         compactIds<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->lalSeg.p, ctx->lalFlag.p, n, ctx->lalList.p);
         KT_HIP(hipGetLastError());
         // kept aside: the levels reuse lalFlag
-        KT_HIP(hipMemcpyAsync(ctx->lalCount.p + 3, ctx->lalFlag.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToDevice,
+        KT_HIP(hipMemcpyAsync(cnt + 3, ctx->lalFlag.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToDevice,
                               ctx->stream));
     }
     const auto unpaint = [&]() -> int {
@@ -1385,43 +1383,34 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
     // when that happens, the regions from the first such one on are restored
     // and run again with the cells their bins hold now (the earlier regions'
     // results stand), until no cell lands in a later region.
-    // Each pass: the levels, the escape check, and (speculatively) the moved
-    // cells, brought back in one round trip.  A cell that moved into a later
-    // region (bad != kNoVar) discards the speculative moved list: the repair
-    // rewrites every batch cell's bin and the regions from `bad` run again.
-    // lalCount: [0] first bad region, [1] cells of the rerun, [2] moved cells,
-    // [3] the device's count of the batch's cells.
-    constexpr uint32_t kMovedInline = 4096;  // up to this many cells, the moved list comes with the counts
+    // Each pass: the levels, then the escape check and (speculatively) the
+    // moved cells in one kernel, brought back in one round trip.  A cell that
+    // moved into a later region (bad != kNoVar) discards the speculative
+    // moved list: the repair rewrites every batch cell's bin and the regions
+    // from `bad` run again.
+    constexpr uint32_t kMovedInline = 4096;  // up to this many cells, the moved list comes with the counters
     const bool inlineMoved = M <= kMovedInline;
-    KT_TRY(ctx->lalMoved.reserve(2ull * M));  // M records of two words
-    Moved *dMoved = reinterpret_cast<Moved *>(ctx->lalMoved.p);
-    uint32_t active = M;
+    const size_t headBytes = 4 * sizeof(uint32_t);
+    uint32_t active = M, moves = 0;
     for (uint32_t first = 0;;) {
         KT_TRY(runLevels(active));
-        if (inlineMoved) KT_TRY(lalLandReserve(ctx, static_cast<size_t>(M) * sizeof(Moved)));
-        uint32_t *slot = landSlots(ctx);
-        escapeCheck<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, first, bs, ctx->lalCount.p);
+        const size_t back = headBytes + (inlineMoved ? static_cast<size_t>(M) * sizeof(Moved) : 0);
+        KT_TRY(lalLandReserve(ctx, back));
+        passEnd<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, first, bs, ctx->area.p, cnt, dMoved);
         KT_HIP(hipGetLastError());
-        movedCells<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, ctx->posX.p, ctx->posY.p, ctx->width.p,
-                                                       ctx->height.p, ctx->area.p, gridDev(g), ctx->lalBinStart.p,
-                                                       ctx->binOf.p, dMoved, ctx->lalCount.p + 2);
-        KT_HIP(hipGetLastError());
-        KT_HIP(hipMemcpyAsync(slot + kLandCount, ctx->lalCount.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                              ctx->stream));
-        if (inlineMoved)
-            KT_HIP(hipMemcpyAsync(landVar(ctx), dMoved, static_cast<size_t>(M) * sizeof(Moved), hipMemcpyDeviceToHost,
-                                  ctx->stream));
+        KT_HIP(hipMemcpyAsync(landVar(ctx), cnt, back, hipMemcpyDeviceToHost, ctx->stream));
         KT_HIP(hipStreamSynchronize(ctx->stream));
-        KT_CHECK(slot[kLandCount + 3] == M, KT_ESTATE, "lal: the batch's cell count differs from the cells per bin");
-        slot[kLandMoves] = slot[kLandCount + 2];
-        const uint32_t bad = slot[kLandCount];
+        const uint32_t *got = reinterpret_cast<const uint32_t *>(landVar(ctx));
+        KT_CHECK(got[3] == M, KT_ESTATE, "lal: the batch's cell count differs from the cells per bin");
+        const uint32_t bad = got[0];
+        moves = got[2];
         if (bad == kNoVar) break;
         repairBatch<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, bad, bs);
         KT_HIP(hipGetLastError());
-        resegBatch<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, bad, bs, ctx->lalCount.p + 1);
+        resegBatch<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, bad, bs, cnt + 1);
         KT_HIP(hipGetLastError());
-        KT_HIP(hipMemcpyAsync(slot + kLandActive, ctx->lalCount.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                              ctx->stream));
+        uint32_t *slot = landSlots(ctx);
+        KT_HIP(hipMemcpyAsync(slot + kLandActive, cnt + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
         KT_HIP(hipStreamSynchronize(ctx->stream));
         active = slot[kLandActive];
         first = bad;
@@ -1429,18 +1418,17 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
         // counters back to {kNoVar, 0, 0} for the rerun ([3] stays)
         slot[kLandNone] = kNoVar;
         slot[kLandNone + 1] = slot[kLandNone + 2] = 0u;
-        KT_HIP(hipMemcpyAsync(ctx->lalCount.p, slot + kLandNone, 3 * sizeof(uint32_t), hipMemcpyHostToDevice,
-                              ctx->stream));
+        KT_HIP(hipMemcpyAsync(cnt, slot + kLandNone, 3 * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
     }
     // the batch's cells whose bin changed (binOf holds every batch cell's bin)
-    const uint32_t moves = landSlots(ctx)[kLandMoves];
     moved.resize(moves);
     if (!inlineMoved && moves) {
-        KT_TRY(lalLandReserve(ctx, moves * sizeof(Moved)));
-        KT_HIP(hipMemcpyAsync(landVar(ctx), dMoved, moves * sizeof(Moved), hipMemcpyDeviceToHost, ctx->stream));
+        KT_TRY(lalLandReserve(ctx, headBytes + moves * sizeof(Moved)));
+        KT_HIP(hipMemcpyAsync(landVar(ctx) + headBytes, dMoved, moves * sizeof(Moved), hipMemcpyDeviceToHost,
+                              ctx->stream));
         KT_HIP(hipStreamSynchronize(ctx->stream));
     }
-    if (moves) std::memcpy(moved.data(), landVar(ctx), moves * sizeof(Moved));
+    if (moves) std::memcpy(moved.data(), landVar(ctx) + headBytes, moves * sizeof(Moved));
     return unpaint();
 }
 
