@@ -290,7 +290,9 @@ listRemap(const uint32_t *__restrict__ list, uint32_t m, const uint32_t *__restr
 // bin -> region for the bins of each region of a batch (one block per
 // region), or back to kNoVar
 __global__ void __launch_bounds__(kB)
-paintRegions(const uint4 *__restrict__ rects, uint32_t nx, uint32_t *__restrict__ binRegion, int clear) {
+paintRegions(const uint4 *__restrict__ rects, uint32_t nx, uint32_t *__restrict__ binRegion, int clear,
+             uint32_t *__restrict__ counters) {
+    if (counters && blockIdx.x == 0 && threadIdx.x < 4) counters[threadIdx.x] = threadIdx.x == 0 ? kNoVar : 0u;
     const uint4 r = rects[blockIdx.x];  // x0, x1, y0, y1
     const uint32_t w = r.y - r.x + 1, cells = w * (r.w - r.z + 1);
     const uint32_t v = clear ? kNoVar : blockIdx.x;
@@ -304,7 +306,9 @@ struct RectSet {
     uint4 r[kRectsByValue];
 };
 __global__ void __launch_bounds__(kB)
-paintRegionsArg(RectSet rects, uint32_t nx, uint32_t *__restrict__ binRegion, int clear) {
+paintRegionsArg(RectSet rects, uint32_t nx, uint32_t *__restrict__ binRegion, int clear,
+                uint32_t *__restrict__ counters) {
+    if (counters && blockIdx.x == 0 && threadIdx.x < 4) counters[threadIdx.x] = threadIdx.x == 0 ? kNoVar : 0u;
     const uint4 r = rects.r[blockIdx.x];
     const uint32_t w = r.y - r.x + 1, cells = w * (r.w - r.z + 1);
     const uint32_t v = clear ? kNoVar : blockIdx.x;
@@ -1048,23 +1052,6 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
         blocks[r] = {g.box(b), g.sum(g.avail, b)};
     }
     const uint32_t R = static_cast<uint32_t>(regions.size());
-    // bin -> region for the batch's bins (every other bin holds kNoVar); a
-    // few regions travel in the launch arguments, more through a copy
-    const bool rectsByValue = R <= kRectsByValue;
-    RectSet rectSet{};
-    const uint4 *dRects = nullptr;
-    if (rectsByValue) {
-        for (uint32_t r = 0; r < R; ++r) rectSet.r[r] = rects[r];
-        paintRegionsArg<<<R, kB, 0, ctx->stream>>>(rectSet, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 0);
-    } else {
-        KT_TRY(ctx->lalRects.reserve(4ull * R));
-        KT_TRY(lalLandReserve(ctx, R * sizeof(uint4)));
-        std::memcpy(landVar(ctx), rects.data(), R * sizeof(uint4));
-        KT_HIP(hipMemcpyAsync(ctx->lalRects.p, landVar(ctx), R * sizeof(uint4), hipMemcpyHostToDevice, ctx->stream));
-        dRects = reinterpret_cast<const uint4 *>(ctx->lalRects.p);
-        paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 0);
-    }
-    KT_HIP(hipGetLastError());
     // per-cell scratch
     for (auto *b : {&ctx->lalSeg, &ctx->lalFlag, &ctx->lalIds, &ctx->lalVals, &ctx->lalValsTmp, &ctx->lalHalf,
                     &ctx->lalStripeOf, &ctx->lalList})
@@ -1091,17 +1078,29 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
     // moved records (two words each), so one transfer returns both.  The
     // counters: [0] first region a cell escaped into (kNoVar: none), [1]
     // cells of a rerun, [2] moved records, [3] the device's count of the
-    // batch's cells; they start as {kNoVar, 0, 0, 0}.
+    // batch's cells; the paint kernel sets them to {kNoVar, 0, 0, 0}.
     KT_TRY(ctx->lalMoved.reserve(2ull + 2ull * M));
     uint32_t *cnt = reinterpret_cast<uint32_t *>(ctx->lalMoved.p);
     Moved *dMoved = reinterpret_cast<Moved *>(ctx->lalMoved.p + 2);
     KT_TRY(lalLandReserve(ctx, 0));  // the landing area exists before its first use
-    {
-        uint32_t *slot = landSlots(ctx);
-        slot[kLandNone] = kNoVar;
-        slot[kLandNone + 1] = slot[kLandNone + 2] = slot[kLandNone + 3] = 0u;
-        KT_HIP(hipMemcpyAsync(cnt, slot + kLandNone, 4 * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    // bin -> region for the batch's bins (every other bin holds kNoVar); a
+    // few regions travel in the launch arguments, more through a copy.  The
+    // paint's first block also sets the counters.
+    const bool rectsByValue = R <= kRectsByValue;
+    RectSet rectSet{};
+    const uint4 *dRects = nullptr;
+    if (rectsByValue) {
+        for (uint32_t r = 0; r < R; ++r) rectSet.r[r] = rects[r];
+        paintRegionsArg<<<R, kB, 0, ctx->stream>>>(rectSet, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 0, cnt);
+    } else {
+        KT_TRY(ctx->lalRects.reserve(4ull * R));
+        KT_TRY(lalLandReserve(ctx, R * sizeof(uint4)));
+        std::memcpy(landVar(ctx), rects.data(), R * sizeof(uint4));
+        KT_HIP(hipMemcpyAsync(ctx->lalRects.p, landVar(ctx), R * sizeof(uint4), hipMemcpyHostToDevice, ctx->stream));
+        dRects = reinterpret_cast<const uint4 *>(ctx->lalRects.p);
+        paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 0, cnt);
     }
+    KT_HIP(hipGetLastError());
     // restore points of the batch's cells and their bins at the start
     for (auto *b : {&ctx->lalRegionOf, &ctx->lalSaveBin, &ctx->lalBinStart}) KT_TRY(b->reserve(n));
     for (auto *b : {&ctx->lalSaveX, &ctx->lalSaveY}) KT_TRY(b->reserve(n));
@@ -1124,9 +1123,11 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
     }
     const auto unpaint = [&]() -> int {
         if (rectsByValue)
-            paintRegionsArg<<<R, kB, 0, ctx->stream>>>(rectSet, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 1);
+            paintRegionsArg<<<R, kB, 0, ctx->stream>>>(rectSet, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 1,
+                                                       nullptr);
         else
-            paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 1);
+            paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 1,
+                                                    nullptr);
         KT_HIP(hipGetLastError());
         return KT_OK;
     };
