@@ -1061,11 +1061,21 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
     // not overlap): no round trip; the device's own count is checked against
     // it with the batch's last transfer
     uint64_t cellsInRegions = 0;
-    for (const lal::BinRect &b : regions)
+    bool splits = false;
+    for (const lal::BinRect &b : regions) {
+        uint64_t inRegion = 0;
         for (size_t iy = b.y0; iy <= b.y1; ++iy)
-            for (size_t ix = b.x0; ix <= b.x1; ++ix) cellsInRegions += binCells[g.at(ix, iy)];
+            for (size_t ix = b.x0; ix <= b.x1; ++ix) inRegion += binCells[g.at(ix, iy)];
+        cellsInRegions += inRegion;
+        splits = splits || inRegion > ctx->lalOpts.minCellsToSplit;
+    }
     KT_CHECK(cellsInRegions <= n, KT_ESTATE, "lal: cells per bin out of step with the device");
     const uint32_t M = static_cast<uint32_t>(cellsInRegions);
+    // A region of at most minCellsToSplit cells is a block that does not
+    // split (simpl_lal.cc:216-218): its cells keep their positions and bins.
+    // A batch of such regions (or with a level cap of 1) moves no cell, so it
+    // has nothing to do on the device.
+    if (!splits || ctx->lalOpts.maxLevel <= 1) return KT_OK;
     // batches of up to kSmallCells cells: one workgroup per half level
     static const bool smallPath = [] {
         const char *v = std::getenv("KTPLACE_LAL_SMALL");
