@@ -1131,16 +1131,17 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
         KT_HIP(hipMemcpyAsync(cnt + 3, ctx->lalFlag.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToDevice,
                               ctx->stream));
     }
-    const auto unpaint = [&]() -> int {
+    const auto paint = [&](int clear) -> int {
         if (rectsByValue)
-            paintRegionsArg<<<R, kB, 0, ctx->stream>>>(rectSet, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 1,
+            paintRegionsArg<<<R, kB, 0, ctx->stream>>>(rectSet, static_cast<uint32_t>(g.nx), ctx->binRegion.p, clear,
                                                        nullptr);
         else
-            paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, 1,
+            paintRegions<<<R, kB, 0, ctx->stream>>>(dRects, static_cast<uint32_t>(g.nx), ctx->binRegion.p, clear,
                                                     nullptr);
         KT_HIP(hipGetLastError());
         return KT_OK;
     };
+    const auto unpaint = [&]() -> int { return paint(1); };
     if (M == 0) return unpaint();
     const uint32_t *list = ctx->lalList.p;
     if (!small) {  // the small path's member pass recorded them
@@ -1409,6 +1410,9 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
         KT_TRY(lalLandReserve(ctx, back));
         passEnd<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, first, bs, ctx->area.p, cnt, dMoved);
         KT_HIP(hipGetLastError());
+        // the regions' bins are cleared before the transfer, on the bet that
+        // this pass is the last (a rerun paints them again)
+        KT_TRY(unpaint());
         KT_HIP(hipMemcpyAsync(landVar(ctx), cnt, back, hipMemcpyDeviceToHost, ctx->stream));
         KT_HIP(hipStreamSynchronize(ctx->stream));
         const uint32_t *got = reinterpret_cast<const uint32_t *>(landVar(ctx));
@@ -1416,6 +1420,7 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
         const uint32_t bad = got[0];
         moves = got[2];
         if (bad == kNoVar) break;
+        KT_TRY(paint(0));
         repairBatch<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, bad, bs);
         KT_HIP(hipGetLastError());
         resegBatch<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, bad, bs, cnt + 1);
@@ -1440,7 +1445,7 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
         KT_HIP(hipStreamSynchronize(ctx->stream));
     }
     if (moves) std::memcpy(moved.data(), landVar(ctx) + headBytes, moves * sizeof(Moved));
-    return unpaint();
+    return KT_OK;
 }
 
 // LookAheadLegalizer::legalize (simpl_lal.cc:43-124) on the context's positions.
