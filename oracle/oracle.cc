@@ -1438,6 +1438,15 @@ uint64_t ko_profile(const ko_design *h, const double *box, int vertical, double 
 
 void ko_set_lal_snapshot(int on) { g_lalSnapshot = on != 0; }
 
+// LAL options of simpl_options.h:24-26 (0 keeps the current value)
+void ko_set_lal_options(ko_design *h, uint64_t max_level, uint64_t min_cells_to_split,
+                        double stripe_area_fraction) {
+    Design &d = *reinterpret_cast<Design *>(h);
+    if (max_level) d.opts.maxLevel = max_level;
+    if (min_cells_to_split) d.opts.minCellsToSplit = min_cells_to_split;
+    if (stripe_area_fraction > 0.0) d.opts.stripeAreaFraction = stripe_area_fraction;
+}
+
 void ko_lal(ko_design *h, double *x, double *y) {
     Design &d = *reinterpret_cast<Design *>(h);
     const std::size_t n = d.netlist.numMovable;

@@ -58,6 +58,7 @@ def _load():
         "ko_overflow": (F64, [P, P]),
         "ko_profile": (U64, [P, P, I, P, P, U64]),
         "ko_lal": (None, [P, P, P]),
+        "ko_set_lal_options": (None, [P, U64, U64, F64]),
         "ko_place": (None, [P, P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
@@ -218,6 +219,9 @@ class Oracle:
         p = np.zeros(cap)
         k = int(lib.ko_profile(self.h, _p(b), int(vertical), _p(e), _p(p), cap))
         return e[:k], p[:k]
+
+    def set_lal_options(self, max_level=0, min_cells_to_split=0, stripe_area_fraction=0.0):
+        lib.ko_set_lal_options(self.h, max_level, min_cells_to_split, stripe_area_fraction)
 
     def lal(self, x, y):
         x = np.array(x, np.float64)

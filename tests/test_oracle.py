@@ -497,6 +497,29 @@ def test_lal_invariants():
     assert after < 0.5 * before
 
 
+def test_lal_blocks_at_or_below_the_split_threshold_stay():
+    """A block of at most minCellsToSplit cells, or at the level cap, does not
+    split (simpl_lal.cc:216-218), so its cells keep their positions: with a
+    threshold above every region's cell count, or a level cap of 1,
+    legalization moves no cell inside the die.  The GPU legalizer skips such
+    batches without running them, on this rule."""
+    d = dz.generate(3000, 3300, 60, seed=9)
+    rng = np.random.default_rng(8)
+    W, H = d.die[2], d.die[3]
+    x = np.clip(rng.normal(W / 2, W / 10, d.n), 0.0, W - d.flat.width)  # clumped, inside the die
+    y = np.clip(rng.normal(H / 2, H / 10, d.n), 0.0, H - d.flat.height)
+    moved = ol.Oracle.of(d)
+    moved.build_grid(0, 1.0)
+    ux, uy = moved.lal(x, y)
+    assert not (np.array_equal(ux, x) and np.array_equal(uy, y))
+    for opts in (dict(min_cells_to_split=10 ** 9), dict(max_level=1)):
+        o = ol.Oracle.of(d)
+        o.build_grid(0, 1.0)
+        o.set_lal_options(**opts)
+        sx, sy = o.lal(x, y)
+        assert np.array_equal(sx, x) and np.array_equal(sy, y), opts
+
+
 def test_place_small_design_runs():
     d = dz.generate(1500, 1650, 40, seed=3)
     o = ol.Oracle.of(d)
