@@ -1438,6 +1438,27 @@ uint64_t ko_profile(const ko_design *h, const double *box, int vertical, double 
 
 void ko_set_lal_snapshot(int on) { g_lalSnapshot = on != 0; }
 
+// getStripes (simpl_lal.cc:238-262) in fill order (the stable sort of
+// nonlinearScale, :275-282) for tests: (start, end, capacity) triples into
+// out (capacity cap stripes); returns the number of stripes.
+uint64_t ko_stripes(const ko_design *h, const double *box, int vertical, double cell_area, double region_room,
+                    double cutline, int below_cut, double *out, uint64_t cap) {
+    const Design &d = *reinterpret_cast<const Design *>(h);
+    const Lal lal{d, d.grid, d.freeSpace, d.opts, d.densityLimit, d.die};
+    std::vector<Stripe> s =
+        lal.getStripes({{box[0], box[1]}, {box[2], box[3]}}, vertical != 0, cell_area, region_room);
+    const auto fromCut = [&](double c) { return below_cut ? cutline - c : c - cutline; };
+    std::stable_sort(s.begin(), s.end(), [&](const Stripe &a, const Stripe &b) {
+        return fromCut(a.span.getCenter()) > fromCut(b.span.getCenter());
+    });
+    for (std::size_t i = 0; i < s.size() && i < cap; ++i) {
+        out[3 * i] = s[i].span.start;
+        out[3 * i + 1] = s[i].span.end;
+        out[3 * i + 2] = s[i].capacity;
+    }
+    return s.size();
+}
+
 // LAL options of simpl_options.h:24-26 (0 keeps the current value)
 void ko_set_lal_options(ko_design *h, uint64_t max_level, uint64_t min_cells_to_split,
                         double stripe_area_fraction) {

@@ -1,12 +1,14 @@
 // Test harness (CPU): the product's host geometry of look-ahead legalization
 // (ktplace_amd/csrc/lal_geom.cc) on a design read from stdin.  Prints the
-// grid's free area per bin and the free-area profile of each query box, as
-// the bit patterns of the doubles, for tests/test_lal_geom_host.py to compare
-// with the oracle.
+// grid's free area per bin, then for each query the free-area profile of a
+// box or the stripes of a half block, as the bit patterns of the doubles, for
+// tests/test_lal_geom_host.py to compare with the oracle.
 //
 // stdin: R, then R lines "y height x0 x1 pitch"; the die "x0 y0 x1 y1"; K,
-// then K obstacle lines "x0 y0 x1 y1"; bins per axis; Q, then Q lines
-// "x0 y0 x1 y1 vertical".  Numbers are hex floats (%a) so they arrive exact.
+// then K obstacle lines "x0 y0 x1 y1"; bins per axis; Q, then Q queries,
+// "0 x0 y0 x1 y1 vertical" (profile) or "1 x0 y0 x1 y1 vertical cellArea
+// regionRoom densityLimit stripeAreaFraction cutline belowCut" (stripes).
+// Numbers are hex floats (%a) so they arrive exact.
 #include <cinttypes>
 #include <cstdio>
 #include <cstdlib>
@@ -53,12 +55,27 @@ int main() {
     std::printf("\n");
     const size_t q = static_cast<size_t>(num());
     for (size_t k = 0; k < q; ++k) {
+        const bool isStripes = num() != 0.0;
         Box b{};
         b.x0 = num();
         b.y0 = num();
         b.x1 = num();
         b.y1 = num();
         const bool vertical = num() != 0.0;
+        if (isStripes) {
+            const double cellArea = num(), regionRoom = num(), densityLimit = num(), fraction = num();
+            const double cutline = num();
+            const bool belowCut = num() != 0.0;
+            std::printf("stripes");
+            for (const Stripe &s :
+                 stripes(fs, obstacles, b, vertical, cellArea, regionRoom, densityLimit, fraction, cutline, belowCut)) {
+                put(s.start);
+                put(s.end);
+                put(s.capacity);
+            }
+            std::printf("\n");
+            continue;
+        }
         const Profile p = fs.profile(b, vertical);
         std::printf("edges");
         for (const double e : p.edges) put(e);

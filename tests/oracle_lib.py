@@ -59,6 +59,7 @@ def _load():
         "ko_profile": (U64, [P, P, I, P, P, U64]),
         "ko_lal": (None, [P, P, P]),
         "ko_set_lal_options": (None, [P, U64, U64, F64]),
+        "ko_stripes": (U64, [P, P, I, F64, F64, F64, I, P, U64]),
         "ko_place": (None, [P, P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
@@ -219,6 +220,13 @@ class Oracle:
         p = np.zeros(cap)
         k = int(lib.ko_profile(self.h, _p(b), int(vertical), _p(e), _p(p), cap))
         return e[:k], p[:k]
+
+    def stripes(self, box, vertical: bool, cell_area, region_room, cutline, below_cut: bool, cap=4096):
+        b = np.array(box, np.float64)
+        out = np.zeros(3 * cap)
+        k = int(lib.ko_stripes(self.h, _p(b), int(vertical), cell_area, region_room, cutline, int(below_cut),
+                               _p(out), cap))
+        return out[:3 * min(k, cap)].reshape(-1, 3)
 
     def set_lal_options(self, max_level=0, min_cells_to_split=0, stripe_area_fraction=0.0):
         lib.ko_set_lal_options(self.h, max_level, min_cells_to_split, stripe_area_fraction)

@@ -1,7 +1,7 @@
 """The product's host geometry of look-ahead legalization (csrc/lal_geom.cc:
-free space, the grid's free area per bin, free-area profiles) against the
-oracle's restatement of simpl_grid.h:67-231 and simpl_placer.cc:190-224, bit
-for bit, on the CPU.  lal_geom.cc is compiled with g++ into a small harness
+free space, the grid's free area per bin, free-area profiles, stripes in
+fill order) against the oracle's restatement of simpl_grid.h:67-231,
+simpl_placer.cc:190-224 and simpl_lal.cc:238-282, bit for bit, on the CPU.  lal_geom.cc is compiled with g++ into a small harness
 (tests/lal_geom_check.cc); the GPU legalizer's parity tests cover the rest."""
 import shutil
 import struct
@@ -34,7 +34,7 @@ def _bits(a) -> list:
     return [struct.unpack("<Q", struct.pack("<d", float(v)))[0] for v in a]
 
 
-def _run(exe, d, bins, boxes) -> tuple:
+def _run(exe, d, bins, queries) -> tuple:
     raw, flat = d.raw, d.flat
     lines = [str(len(raw.row_y))]
     for r in range(len(raw.row_y)):
@@ -45,17 +45,27 @@ def _run(exe, d, bins, boxes) -> tuple:
     lines.append(str(len(obs)))
     lines += [" ".join(float(v).hex() for v in o) for o in obs]
     lines.append(str(bins))
-    lines.append(str(len(boxes)))
-    lines += [" ".join(float(v).hex() for v in b[:4]) + f" {int(b[4])}" for b in boxes]
-    out = subprocess.run([str(exe)], input="\n".join(lines) + "\n", capture_output=True, text=True,
-                         check=True, timeout=120).stdout.split("\n")
-    avail = [int(t, 16) for t in out[0].split()[1:]]
-    profiles = []
-    for k in range(len(boxes)):
-        e = [int(t, 16) for t in out[1 + 2 * k].split()[1:]]
-        p = [int(t, 16) for t in out[2 + 2 * k].split()[1:]]
-        profiles.append((e, p))
-    return avail, profiles
+    lines.append(str(len(queries)))
+    for q in queries:  # ("P", box, vertical) or ("S", box, vertical, cell, room, g, frac, cut, below)
+        nums = list(q[1]) + [int(q[2])] + [v for v in q[3:]]
+        lines.append(("1 " if q[0] == "S" else "0 ") +
+                     " ".join(v.hex() if isinstance(v, float) else str(v) for v in map(_num, nums)))
+    out = iter(subprocess.run([str(exe)], input="\n".join(lines) + "\n", capture_output=True, text=True,
+                              check=True, timeout=120).stdout.split("\n"))
+    avail = [int(t, 16) for t in next(out).split()[1:]]
+    results = []
+    for q in queries:
+        if q[0] == "S":
+            results.append([int(t, 16) for t in next(out).split()[1:]])
+        else:
+            e = [int(t, 16) for t in next(out).split()[1:]]
+            p = [int(t, 16) for t in next(out).split()[1:]]
+            results.append((e, p))
+    return avail, results
+
+
+def _num(v):
+    return int(v) if isinstance(v, (bool, np.bool_)) else float(v)
 
 
 @pytest.mark.parametrize("seed,bins,macros", [(9, 0, 0), (4, 37, 12), (5, 16, 30)])
@@ -66,17 +76,33 @@ def test_free_space_grid_and_profiles_match_the_oracle(harness, seed, bins, macr
     per = o.build_grid(bins, 1.0)
     rng = np.random.default_rng(seed)
     W, H = d.die[2], d.die[3]
-    boxes = []
+    queries = []
     for k in range(200):
         x0, y0 = rng.uniform(0, W), rng.uniform(0, H)
         x1 = x0 + rng.uniform(0, W - x0) * (0.1 if k % 3 == 0 else 1.0)
         y1 = y0 + rng.uniform(0, H - y0) * (0.1 if k % 3 == 1 else 1.0)
-        boxes.append((x0, y0, x1, y1, k % 2))
-    boxes.append((0.0, 0.0, W, H, 1))
-    boxes.append((0.0, 0.0, W, H, 0))
-    avail, profiles = _run(harness, d, per, boxes)
+        box, vertical = (x0, y0, x1, y1), bool(k % 2)
+        queries.append(("P", box, vertical))
+        # a half block's stripes: cut inside the box, the region's room at or
+        # above the half's (zero: the half's own room), cells filling part of it
+        lo, hi = (x0, x1) if vertical else (y0, y1)
+        cut = lo + rng.uniform(0, 1) * (hi - lo)
+        _, rp = o.profile(box, vertical)
+        room = float(rp[-1]) if len(rp) else 0.0
+        region = 0.0 if k % 5 == 0 else room * (1.0 + rng.uniform(0, 3))
+        queries.append(("S", box, vertical, rng.uniform(0, 1.2) * room, region, 1.0, 0.1, cut, k % 4 < 2))
+    for vertical in (True, False):
+        queries.append(("P", (0.0, 0.0, W, H), vertical))
+    avail, results = _run(harness, d, per, queries)
     ref_avail, _ = o.grid_avail()
     assert avail == _bits(ref_avail)
-    for b, (e, p) in zip(boxes, profiles):
-        re, rp = o.profile(b[:4], bool(b[4]))
-        assert e == _bits(re) and p == _bits(rp), b
+    stripes_seen = 0
+    for q, got in zip(queries, results):
+        if q[0] == "S":
+            ref = o.stripes(q[1], q[2], q[3], q[4], q[7], q[8])
+            assert got == _bits(ref.reshape(-1)), q
+            stripes_seen += len(ref)
+        else:
+            re, rp = o.profile(q[1], q[2])
+            assert got == (_bits(re), _bits(rp)), q
+    assert stripes_seen > 400
