@@ -43,6 +43,30 @@ class SimplOptions:
     precision: int = simpl.KT_PREC_FP64
 
 
+class Convergence:
+    """Convergence::update (simpl_placer.cc:450-461): the gap of iteration
+    gapReferenceIter is the reference; stop once the gap is below
+    gapTightFrac of it, or at most gapRelaxedFrac of it while the upper bound
+    has not improved for stallIters iterations."""
+
+    def __init__(self, o: SimplOptions):
+        self.o = o
+        self.gap_reference = 0.0
+        self.best_upper = math.inf
+        self.stalled_for = 0
+
+    def update(self, iteration: int, gap: float, upper: float) -> bool:
+        o = self.o
+        if iteration == o.gapReferenceIter:
+            self.gap_reference = gap
+        self.stalled_for = 0 if upper < self.best_upper else self.stalled_for + 1
+        self.best_upper = min(self.best_upper, upper)
+        stalled = self.stalled_for >= o.stallIters
+        return (iteration > o.gapReferenceIter and self.gap_reference > 0.0 and
+                (gap < o.gapTightFrac * self.gap_reference or
+                 (gap <= o.gapRelaxedFrac * self.gap_reference and stalled)))
+
+
 @dataclass
 class PlaceResult:
     x: np.ndarray           # upper-bound (legalized) placement
@@ -129,9 +153,7 @@ class Placer:
             lx, ly, hpwl_init = self._initial_rounds(lx, ly, hpwl_init, star=False)
         self.stats["hpwl_init"] = hpwl_init
         trace = []
-        gap_ref = 0.0
-        best_upper = math.inf
-        stalled = 0
+        convergence = Convergence(o)
         ux = uy = None
         for it in range(o.maxIters):
             self.stats["global_iters"] = it + 1
@@ -146,15 +168,7 @@ class Placer:
                 self.progress(f"iter {it + 1}: lower {lower:.6e} upper {upper:.6e} "
                               f"lal {self.stats['spread_seconds']:.2f} s total, "
                               f"{ctx.lal_batches()} batches")
-            iteration = it + 1
-            if iteration == o.gapReferenceIter:
-                gap_ref = gap
-            stalled = 0 if upper < best_upper else stalled + 1
-            best_upper = min(best_upper, upper)
-            converged = (iteration > o.gapReferenceIter and gap_ref > 0.0 and
-                         (gap < o.gapTightFrac * gap_ref or
-                          (gap <= o.gapRelaxedFrac * gap_ref and stalled >= o.stallIters)))
-            if converged:
+            if convergence.update(it + 1, gap, upper):
                 break
             alpha = o.alphaBase * (1.0 + it)
             t0 = time.perf_counter()

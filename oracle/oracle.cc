@@ -67,6 +67,11 @@ constexpr std::size_t grain = 4096;
 // (the reference with one thread); true collects every region's cells before
 // any region moves one (its regions running concurrently).
 bool g_lalSnapshot = false;
+// A planted mistake, for the tests that show the LAL pins catch one (set only
+// by ko_set_mutation from those tests; zero everywhere else):
+//   1  stripes filled nearest to the cut first instead of farthest first
+//   2  the affine map inside a stripe reversed (the lowest cell to the top)
+int g_mutation = 0;
 
 template <typename Body>
 void taskRange(std::size_t n, std::size_t g, Body &body) {
@@ -761,6 +766,64 @@ struct Stripe {
     std::vector<std::uint32_t> cells;
 };
 
+// The second half of nonlinearScale (simpl_lal.cc:275-363), after getStripes:
+// stripes ordered farthest from the cut first (stable), cells ordered farthest
+// first (total order on (key, id)), each cell into the first stripe not yet
+// at capacity, then the affine map of each stripe's cells onto its span.
+// size[i] is the cell's extent along the axis; pos holds lower-left corners.
+void packStripes(std::vector<Stripe> &strips, const std::vector<std::uint32_t> &cells, double cutline,
+                 bool belowCut, std::vector<double> &pos, const std::vector<double> &size,
+                 const std::vector<double> &area) {
+    const auto centre = [&](std::uint32_t i) { return pos[i] + 0.5 * size[i]; };
+    const auto fromCut = [&](double c) { return belowCut ? cutline - c : c - cutline; };
+    std::stable_sort(strips.begin(), strips.end(), [&](const Stripe &a, const Stripe &b) {
+        if (g_mutation == 1) return fromCut(a.span.getCenter()) < fromCut(b.span.getCenter());
+        return fromCut(a.span.getCenter()) > fromCut(b.span.getCenter());
+    });
+    std::vector<std::uint32_t> order = cells;
+    sortCells(order, [&](std::uint32_t i) { return -fromCut(centre(i)); });
+    std::size_t s = 0;
+    for (const std::uint32_t i : order) {
+        while (s + 1 < strips.size() && strips[s].used >= strips[s].capacity) ++s;
+        strips[s].cells.push_back(i);
+        strips[s].used += area[i];
+    }
+    for (const Stripe &strip : strips) {
+        if (strip.cells.empty()) continue;
+        double lo = std::numeric_limits<double>::max();
+        double hi = -lo;
+        for (const std::uint32_t i : strip.cells) {
+            lo = std::min(lo, centre(i));
+            hi = std::max(hi, centre(i));
+        }
+        for (const std::uint32_t i : strip.cells) {
+            double t = hi > lo ? (centre(i) - lo) / (hi - lo) : 0.5;
+            if (g_mutation == 2) t = 1.0 - t;
+            const double c = strip.span.start + t * (strip.span.getLength());
+            pos[i] = std::clamp(c - 0.5 * size[i], strip.span.start, std::max(strip.span.start, strip.span.end - size[i]));
+        }
+    }
+}
+
+// Convergence::update (simpl_placer.cc:450-461): the gap of iteration
+// gapReferenceIter is the reference; stop once the gap is below the tight
+// share of it, or within the relaxed share while the upper bound has not
+// improved for stallIters iterations.
+struct Convergence {
+    double gapReference = 0.0;
+    double bestUpper = std::numeric_limits<double>::max();
+    std::size_t stalledFor = 0;
+    bool update(std::size_t iteration, double gap, double upperHpwl, std::size_t referenceIter, double tightFrac,
+                double relaxedFrac, std::size_t stallIters) {
+        if (iteration == referenceIter) gapReference = gap;
+        stalledFor = upperHpwl < bestUpper ? 0 : stalledFor + 1;
+        bestUpper = std::min(bestUpper, upperHpwl);
+        const bool upperStalled = stalledFor >= stallIters;
+        return iteration > referenceIter && gapReference > 0.0 &&
+               (gap < tightFrac * gapReference || (gap <= relaxedFrac * gapReference && upperStalled));
+    }
+};
+
 struct Lal {
     const Design &d;
     const BinGrid &grid;
@@ -812,7 +875,8 @@ struct Lal {
         return regions;
     }
 
-    BinRect getGrown(BinRect r, const std::vector<double> &occ) const {
+    BinRect getGrown(BinRect r, const std::vector<double> &occ, std::vector<BinRect> *steps = nullptr) const {
+        if (steps) steps->push_back(r);
         while (getDensity(occ, r) > densityLimit) {
             const std::array<BinRect, 4> wider{
                 BinRect{r.x0 > 0 ? r.x0 - 1 : r.x0, r.x1, r.y0, r.y1},
@@ -831,6 +895,7 @@ struct Lal {
             }
             if (best == nullptr) break;
             r = *best;
+            if (steps) steps->push_back(r);
         }
         return r;
     }
@@ -873,33 +938,7 @@ struct Lal {
         const double cellArea = std::accumulate(cells.begin(), cells.end(), 0.0,
                                                 [&](double sum, std::uint32_t i) { return sum + d.area[i]; });
         std::vector<Stripe> strips = getStripes(box, vertical, cellArea, regionRoom);
-        const auto fromCut = [&](double c) { return belowCut ? cutline - c : c - cutline; };
-        std::stable_sort(strips.begin(), strips.end(), [&](const Stripe &a, const Stripe &b) {
-            return fromCut(a.span.getCenter()) > fromCut(b.span.getCenter());
-        });
-        std::vector<std::uint32_t> order = cells;
-        sortCells(order, [&](std::uint32_t i) { return -fromCut(centre(pos, i, vertical)); });
-        std::size_t s = 0;
-        for (const std::uint32_t i : order) {
-            while (s + 1 < strips.size() && strips[s].used >= strips[s].capacity) ++s;
-            strips[s].cells.push_back(i);
-            strips[s].used += d.area[i];
-        }
-        for (const Stripe &strip : strips) {
-            if (strip.cells.empty()) continue;
-            double lo = std::numeric_limits<double>::max();
-            double hi = -lo;
-            for (const std::uint32_t i : strip.cells) {
-                lo = std::min(lo, centre(pos, i, vertical));
-                hi = std::max(hi, centre(pos, i, vertical));
-            }
-            for (const std::uint32_t i : strip.cells) {
-                const double t = hi > lo ? (centre(pos, i, vertical) - lo) / (hi - lo) : 0.5;
-                const double size = vertical ? d.width[i] : d.height[i];
-                const double c = strip.span.start + t * (strip.span.getLength());
-                pos[i] = std::clamp(c - 0.5 * size, strip.span.start, std::max(strip.span.start, strip.span.end - size));
-            }
-        }
+        packStripes(strips, cells, cutline, belowCut, pos, vertical ? d.width : d.height, d.area);
     }
 
     std::optional<std::pair<Block, Block>> split(Block &region, std::vector<double> &x, std::vector<double> &y) const {
@@ -1110,8 +1149,7 @@ struct Placer {
         if (o.initialNetModel != 1) initialRounds(lx, ly, true, accepted);
         if (o.initialNetModel != 0) initialRounds(lx, ly, false, accepted);
         // getGlobalPlacement (simpl_placer.cc:463-539) with Convergence (:450-461)
-        double gapReference = 0.0, bestUpper = std::numeric_limits<double>::max();
-        std::size_t stalledFor = 0;
+        Convergence convergence;
         for (std::size_t it = 0; it < o.maxIters; ++it) {
             st.globalIters = it + 1;
             double t0 = nowSeconds();
@@ -1127,14 +1165,9 @@ struct Placer {
                 trace->push_back(upperHpwl);
                 trace->push_back(getOverflow(d, getOccupancy(d, ux, uy)));
             }
-            const std::size_t iteration = it + 1;
-            if (iteration == o.gapReferenceIter) gapReference = gap;
-            stalledFor = upperHpwl < bestUpper ? 0 : stalledFor + 1;
-            bestUpper = std::min(bestUpper, upperHpwl);
-            const bool converged = iteration > o.gapReferenceIter && gapReference > 0.0 &&
-                                   (gap < o.gapTightFrac * gapReference ||
-                                    (gap <= o.gapRelaxedFrac * gapReference && stalledFor >= o.stallIters));
-            if (converged) break;
+            if (convergence.update(it + 1, gap, upperHpwl, o.gapReferenceIter, o.gapTightFrac, o.gapRelaxedFrac,
+                                   o.stallIters))
+                break;
             const double alpha = o.alphaBase * (1.0 + static_cast<double>(it));
             t0 = nowSeconds();
             buildB2B(d, lx, ly, alpha, ux, uy);
@@ -1457,6 +1490,94 @@ uint64_t ko_stripes(const ko_design *h, const double *box, int vertical, double 
         out[3 * i + 2] = s[i].capacity;
     }
     return s.size();
+}
+
+void ko_set_mutation(int which) { g_mutation = which; }
+
+// packStripes for tests, on caller arrays: k stripes (start, end, capacity) in
+// any order, m cells with lower-left pos (in/out), extent size and area along
+// the axis.  cell_stripe (may be null) receives each cell's stripe, as its
+// index in the input order.
+void ko_pack_stripes(uint64_t k, const double *stripes, uint64_t m, double *pos, const double *size,
+                     const double *area, double cutline, int below_cut, int64_t *cell_stripe) {
+    std::vector<Stripe> strips(k);
+    for (uint64_t s = 0; s < k; ++s) strips[s] = {{stripes[3 * s], stripes[3 * s + 1]}, stripes[3 * s + 2], 0.0, {}};
+    const std::vector<Stripe> input = strips;
+    std::vector<double> vpos(pos, pos + m), vsize(size, size + m), varea(area, area + m);
+    std::vector<std::uint32_t> cells(m);
+    std::iota(cells.begin(), cells.end(), 0u);
+    packStripes(strips, cells, cutline, below_cut != 0, vpos, vsize, varea);
+    std::copy(vpos.begin(), vpos.end(), pos);
+    if (cell_stripe) {
+        for (const Stripe &strip : strips) {
+            int64_t at = -1;
+            for (uint64_t s = 0; s < k; ++s)
+                if (input[s].span.start == strip.span.start && input[s].span.end == strip.span.end) at = int64_t(s);
+            for (const std::uint32_t i : strip.cells) cell_stripe[i] = at;
+        }
+    }
+}
+
+// One split (simpl_lal.cc:213-264) of a block built from the arguments:
+// cells (m of them) are cell ids of the design; x, y in/out.  out receives
+// {split happened, m (cells in the low half), cutline}.
+void ko_split(ko_design *h, const double *box, int vertical, uint64_t level, uint64_t m, const uint32_t *cells,
+              double region_room, double *x, double *y, double *out) {
+    Design &d = *reinterpret_cast<Design *>(h);
+    const Lal lal{d, d.grid, d.freeSpace, d.opts, d.densityLimit, d.die};
+    const std::size_t n = d.netlist.numMovable;
+    std::vector<double> vx(x, x + n), vy(y, y + n);
+    Block b{{{box[0], box[1]}, {box[2], box[3]}}, level, vertical != 0, std::vector<std::uint32_t>(cells, cells + m),
+            region_room};
+    const auto halves = lal.split(b, vx, vy);
+    out[0] = halves ? 1.0 : 0.0;
+    out[1] = halves ? static_cast<double>(halves->first.cells.size()) : 0.0;
+    out[2] = halves ? (vertical ? halves->first.box.upperRight.x : halves->first.box.upperRight.y) : 0.0;
+    std::copy(vx.begin(), vx.end(), x);
+    std::copy(vy.begin(), vy.end(), y);
+}
+
+// getOverfilledClusters (simpl_lal.cc:141-183) on a caller occupancy: (x0, x1,
+// y0, y1) per cluster into out (capacity cap); returns the cluster count.
+uint64_t ko_overfilled_clusters(const ko_design *h, const double *occ, uint64_t *out, uint64_t cap) {
+    const Design &d = *reinterpret_cast<const Design *>(h);
+    const Lal lal{d, d.grid, d.freeSpace, d.opts, d.densityLimit, d.die};
+    const std::vector<BinRect> r = lal.getOverfilledClusters(std::vector<double>(occ, occ + d.grid.size()));
+    for (std::size_t i = 0; i < r.size() && i < cap; ++i) {
+        out[4 * i] = r[i].x0;
+        out[4 * i + 1] = r[i].x1;
+        out[4 * i + 2] = r[i].y0;
+        out[4 * i + 3] = r[i].y1;
+    }
+    return r.size();
+}
+
+// getGrown (simpl_lal.cc:185-211) from rect (x0, x1, y0, y1): every rectangle
+// it passes through, the start first, into out (capacity cap); returns the
+// number of rectangles.
+uint64_t ko_grow(const ko_design *h, const double *occ, const uint64_t *rect, uint64_t *out, uint64_t cap) {
+    const Design &d = *reinterpret_cast<const Design *>(h);
+    const Lal lal{d, d.grid, d.freeSpace, d.opts, d.densityLimit, d.die};
+    std::vector<BinRect> steps;
+    lal.getGrown({rect[0], rect[1], rect[2], rect[3]}, std::vector<double>(occ, occ + d.grid.size()), &steps);
+    for (std::size_t i = 0; i < steps.size() && i < cap; ++i) {
+        out[4 * i] = steps[i].x0;
+        out[4 * i + 1] = steps[i].x1;
+        out[4 * i + 2] = steps[i].y0;
+        out[4 * i + 3] = steps[i].y1;
+    }
+    return steps.size();
+}
+
+// Convergence::update over a sequence of (gap, upper HPWL) for iterations
+// 1..n; returns the first iteration at which it says stop, 0 if none does.
+uint64_t ko_convergence(uint64_t n, const double *gaps, const double *uppers, uint64_t reference_iter,
+                        double tight_frac, double relaxed_frac, uint64_t stall_iters) {
+    Convergence c;
+    for (uint64_t it = 0; it < n; ++it)
+        if (c.update(it + 1, gaps[it], uppers[it], reference_iter, tight_frac, relaxed_frac, stall_iters))
+            return it + 1;
+    return 0;
 }
 
 // LAL options of simpl_options.h:24-26 (0 keeps the current value)

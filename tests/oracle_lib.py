@@ -61,6 +61,12 @@ def _load():
         "ko_set_lal_options": (None, [P, U64, U64, F64]),
         "ko_stripes": (U64, [P, P, I, F64, F64, F64, I, P, U64]),
         "ko_place": (None, [P, P, P, P, P, P]),
+        "ko_set_mutation": (None, [I]),
+        "ko_pack_stripes": (None, [U64, P, U64, P, P, P, F64, I, P]),
+        "ko_split": (None, [P, P, I, U64, U64, P, F64, P, P, P]),
+        "ko_overfilled_clusters": (U64, [P, P, P, U64]),
+        "ko_grow": (U64, [P, P, P, P, U64]),
+        "ko_convergence": (U64, [U64, P, P, U64, F64, F64, U64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -228,6 +234,30 @@ class Oracle:
                                _p(out), cap))
         return out[:3 * min(k, cap)].reshape(-1, 3)
 
+    def split(self, box, vertical: bool, level: int, cells, region_room, x, y):
+        """One split of a block made of `cells`; returns (split, m, cutline, x, y)."""
+        b = np.array(box, np.float64)
+        cells = np.ascontiguousarray(cells, np.uint32)
+        x = np.array(x, np.float64)
+        y = np.array(y, np.float64)
+        out = np.zeros(3)
+        lib.ko_split(self.h, _p(b), int(vertical), level, len(cells), _p(cells), float(region_room),
+                     _p(x), _p(y), _p(out))
+        return bool(out[0]), int(out[1]), float(out[2]), x, y
+
+    def overfilled_clusters(self, occ, cap=1 << 16):
+        occ = np.ascontiguousarray(occ, np.float64)
+        out = np.zeros(4 * cap, np.uint64)
+        k = int(lib.ko_overfilled_clusters(self.h, _p(occ), _p(out), cap))
+        return out[:4 * min(k, cap)].reshape(-1, 4).astype(np.int64)
+
+    def grow(self, occ, rect, cap=1 << 16):
+        occ = np.ascontiguousarray(occ, np.float64)
+        r = np.array(rect, np.uint64)
+        out = np.zeros(4 * cap, np.uint64)
+        k = int(lib.ko_grow(self.h, _p(occ), _p(r), _p(out), cap))
+        return out[:4 * min(k, cap)].reshape(-1, 4).astype(np.int64)
+
     def set_lal_options(self, max_level=0, min_cells_to_split=0, stripe_area_fraction=0.0):
         lib.ko_set_lal_options(self.h, max_level, min_cells_to_split, stripe_area_fraction)
 
@@ -251,6 +281,32 @@ class Oracle:
         stats = {f: getattr(st, f) for f, _ in PlaceStats._fields_}
         g = int(stats["global_iters"])
         return x, y, stats, trace[:3 * g].reshape(-1, 3)
+
+
+def set_mutation(which: int):
+    """Plant a mistake in the oracle's stripe packing (0 removes it)."""
+    lib.ko_set_mutation(which)
+
+
+def pack_stripes(stripes, pos, size, area, cutline, below_cut: bool):
+    """packStripes on plain arrays: stripes (k, 3) rows of (start, end,
+    capacity) in any order; returns (new lower-left positions, stripe index of
+    each cell in the input order)."""
+    st = np.ascontiguousarray(stripes, np.float64).reshape(-1, 3)
+    pos = np.array(pos, np.float64)
+    size = np.ascontiguousarray(size, np.float64)
+    area = np.ascontiguousarray(area, np.float64)
+    which = np.full(len(pos), -1, np.int64)
+    lib.ko_pack_stripes(len(st), _p(st), len(pos), _p(pos), _p(size), _p(area), float(cutline),
+                        int(below_cut), _p(which))
+    return pos, which
+
+
+def convergence(gaps, uppers, reference_iter=10, tight=0.05, relaxed=0.25, stall=6) -> int:
+    """Convergence::update over iterations 1..n; the iteration it stops at, or 0."""
+    g = np.ascontiguousarray(gaps, np.float64)
+    u = np.ascontiguousarray(uppers, np.float64)
+    return int(lib.ko_convergence(len(g), _p(g), _p(u), reference_iter, tight, relaxed, stall))
 
 
 def set_threads(k: int):

@@ -1,0 +1,442 @@
+"""Pins the oracle's look-ahead legalization and loop bookkeeping (CPU only).
+
+The reference holds no numeric vectors for these steps, so each oracle
+function is checked against something that does not share its code:
+
+* packStripes (nonlinearScale's fill and affine map, simpl_lal.cc:266-363):
+  the two worked answers of the reference's docs
+  (docs/global-placement/simpl/02-look-ahead-legalization.md:97-109 and
+  :150-155), a hand-computed two-sided split, and the invariants that the
+  stripes are filled farthest from the cut first and that cells keep their
+  order along the cut axis;
+* split (simpl_lal.cc:213-264): a numpy brute force of the area-median index
+  (with its m == 0 and m + 1 < size guards) and of the cutline as the same
+  share of the block's free area, integrated from the rows and macros
+  directly, not through FreeSpace::getProfile;
+* getOverfilledClusters (simpl_lal.cc:141-183): scipy.ndimage.label with
+  4-connectivity, bounding boxes in the order of each cluster's first bin;
+* getGrown (simpl_lal.cc:185-211): each step takes the least dense of the
+  valid one-bin expansions (the first on ties) and growth stops at the
+  density limit;
+* Convergence::update (simpl_placer.cc:450-461): hand-built (gap, upper)
+  sequences with known stopping iterations, and the product's host mirror
+  agrees with it.
+
+Two planted mistakes (stripes filled nearest first, the affine map reversed)
+are shown to fail these checks.
+"""
+import numpy as np
+import pytest
+from scipy import ndimage
+
+import oracle_lib as ol
+from ktplace_amd import design as dz
+from ktplace_amd import placer as kp
+
+
+def pack_centres(stripes, centres, widths, cutline, below_cut):
+    """packStripes on cells of unit height given by their centres."""
+    c = np.asarray(centres, np.float64)
+    w = np.asarray(widths, np.float64)
+    pos, which = ol.pack_stripes(stripes, c - 0.5 * w, w, w * 1.0, cutline, below_cut)
+    return pos + 0.5 * w, which
+
+
+def eight_cell_example():
+    """02-look-ahead-legalization.md:97-109: B0 = [0, 50], C_B = 50, five
+    stripes of width 10 holding 10 each, eight cells of width 5 at centres
+    22..29; the answer is centres 2.5, 7.5, ..., 37.5."""
+    stripes = [(10.0 * k, 10.0 * k + 10.0, 10.0) for k in range(5)]
+    got, which = pack_centres(stripes, np.arange(22.0, 30.0), np.full(8, 5.0), 50.0, True)
+    return got, which
+
+
+def four_cell_exercise():
+    """02-look-ahead-legalization.md:150-155 (exercise 2): [0, 20], C_B = 20,
+    stripes [0, 10] and [10, 20] of 10 each, cells of width 5 at 14..17; the
+    answer is centres 2.5, 7.5, 12.5, 17.5."""
+    got, _ = pack_centres([(0.0, 10.0, 10.0), (10.0, 20.0, 10.0)], [14.0, 15.0, 16.0, 17.0],
+                          np.full(4, 5.0), 20.0, True)
+    return got
+
+
+def docs_answers_hold() -> bool:
+    got, which = eight_cell_example()
+    ok8 = got.tolist() == [2.5, 7.5, 12.5, 17.5, 22.5, 27.5, 32.5, 37.5]
+    ok8 = ok8 and which.tolist() == [0, 0, 1, 1, 2, 2, 3, 3]
+    ok4 = four_cell_exercise().tolist() == [2.5, 7.5, 12.5, 17.5]
+    return ok8 and ok4
+
+
+def test_docs_worked_answers():
+    got, which = eight_cell_example()
+    assert got.tolist() == [2.5, 7.5, 12.5, 17.5, 22.5, 27.5, 32.5, 37.5]
+    assert which.tolist() == [0, 0, 1, 1, 2, 2, 3, 3]  # stripe [40, 50] stays empty
+    assert four_cell_exercise().tolist() == [2.5, 7.5, 12.5, 17.5]
+    # the same cells above a cut at 0: stripe [10, 20] is farthest and fills first
+    got, _ = pack_centres([(0.0, 10.0, 10.0), (10.0, 20.0, 10.0)], [3.0, 4.0, 5.0, 6.0],
+                          np.full(4, 5.0), 0.0, False)
+    assert got.tolist() == [2.5, 7.5, 12.5, 17.5]
+
+
+def row_design(n_cells, width, die_w, die_h=1.0, macros=()):
+    """Movable cells of one width and unit height on unit rows across the
+    die, plus fixed macros (x0, y0, x1, y1); one net over all movable cells."""
+    cells = [(0.0, 0.0, width, 1.0, False) for _ in range(n_cells)]
+    cells += [(m[0], m[1], m[2] - m[0], m[3] - m[1], True) for m in macros]
+    net = [(i, 0.0, 0.0) for i in range(n_cells)]
+    rows = [(float(r), 1.0, 0.0, float(die_w)) for r in range(int(die_h))]
+    return dz.make_raw(cells, [net], die=(0.0, 0.0, float(die_w), float(die_h)), rows=rows)
+
+
+def test_stripes_of_the_exercise_come_from_get_stripes():
+    """getStripes (simpl_lal.cc:238-262) cuts [0, 20] of one unit row, with a
+    limit of 10 % of a region room of 100, into [0, 10] and [10, 20], each
+    holding min(1, 20 / 20) * 10: the exercise's stripes, farthest from the
+    cut at 20 first."""
+    o = ol.Oracle(row_design(4, 5.0, 20.0))
+    o.build_grid(1, 1.0)
+    s = o.stripes((0.0, 0.0, 20.0, 1.0), True, 20.0, 100.0, 20.0, True)
+    assert s.tolist() == [[0.0, 10.0, 10.0], [10.0, 20.0, 10.0]]
+
+
+def test_two_sided_split_hand_computed():
+    """One split of [0, 40] x [0, 1] holding eight cells of width 5 at
+    centres 16..23: m = 4 (the low half reaches half the area), the cutline
+    is at half the free area (20), each half is cut into two stripes of 10
+    (region room 100) and filled farthest from the cut first, and the cells
+    end at centres 2.5, 7.5, ..., 37.5."""
+    o = ol.Oracle(row_design(8, 5.0, 40.0))
+    o.build_grid(1, 1.0)
+    x = np.arange(16.0, 24.0) - 2.5
+    y = np.zeros(8)
+    did, m, cut, nx, ny = o.split((0.0, 0.0, 40.0, 1.0), True, 1, np.arange(8), 100.0, x, y)
+    assert did and m == 4 and cut == 20.0
+    assert (nx + 2.5).tolist() == [2.5, 7.5, 12.5, 17.5, 22.5, 27.5, 32.5, 37.5]
+    assert ny.tolist() == y.tolist()
+
+
+# ---------------------------------------------------------------- split: brute force
+def free_intervals(raw):
+    """Per row: (y0, y1, free x intervals), the row's span minus the x extent
+    of every fixed macro that overlaps the row's height."""
+    fixed = raw.is_fixed.astype(bool) & (raw.width > 0) & (raw.height > 0)
+    mac = np.stack([raw.pos_x[fixed], raw.pos_y[fixed], raw.pos_x[fixed] + raw.width[fixed],
+                    raw.pos_y[fixed] + raw.height[fixed]], 1)
+    rows = []
+    for y, h, a, b in zip(raw.row_y, raw.row_height, raw.row_x0, raw.row_x1):
+        cuts = sorted((m[0], m[2]) for m in mac if m[3] > y and m[1] < y + h)
+        free, at = [], a
+        for c0, c1 in cuts:
+            if c1 <= at or c0 >= b:
+                continue
+            if c0 > at:
+                free.append((at, c0))
+            at = max(at, c1)
+        if b > at:
+            free.append((at, b))
+        rows.append((y, y + h, free))
+    return rows
+
+
+def free_area_before(rows, box, vertical, c):
+    """Free area of the part of box below coordinate c along the cut axis."""
+    x0, y0, x1, y1 = box
+    total = 0.0
+    for ry0, ry1, free in rows:
+        if vertical:
+            h = min(y1, ry1) - max(y0, ry0)
+            if h <= 0:
+                continue
+            total += h * sum(max(0.0, min(b, c, x1) - max(a, x0)) for a, b in free)
+        else:
+            h = min(y1, ry1, c) - max(y0, ry0)
+            if h <= 0:
+                continue
+            total += h * sum(max(0.0, min(b, x1) - max(a, x0)) for a, b in free)
+    return total
+
+
+def brute_split(area, centres, ids, rows, box, vertical):
+    """The area-median index and the cutline, from their definitions."""
+    order = np.lexsort((ids, centres[ids]))
+    a = area[ids][order]
+    prefix = np.cumsum(a)  # sequential, like the reference's running sum
+    total = prefix[-1]
+    # smallest m >= 1 with the low part holding half the area; at most size - 1
+    reach = np.nonzero(prefix >= 0.5 * total)[0]
+    m = int(reach[0]) + 1 if len(reach) else len(a)
+    m = min(max(m, 1), len(a) - 1)
+    share = prefix[m - 1] / total if total > 0 else 0.5
+    lo, hi = (box[0], box[2]) if vertical else (box[1], box[3])
+    room = free_area_before(rows, box, vertical, hi)
+    target = share * room
+    a_, b_ = lo, hi
+    for _ in range(200):  # leftmost coordinate whose free area reaches the target
+        mid = 0.5 * (a_ + b_)
+        if free_area_before(rows, box, vertical, mid) >= target:
+            b_ = mid
+        else:
+            a_ = mid
+    return m, b_, target, room
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_split_matches_brute_force(seed):
+    d = dz.generate(1500, 1650, 40, seed=seed + 1, num_macros=6, utilization=0.6)
+    o = ol.Oracle.of(d)
+    o.build_grid(0, 1.0)
+    rows = free_intervals(d.raw)
+    rng = np.random.default_rng(seed)
+    W, H = d.die[2], d.die[3]
+    f = d.flat
+    for trial in range(12):
+        bx0, bx1 = np.sort(rng.uniform(0, W, 2))
+        by0, by1 = np.sort(rng.uniform(0, H, 2))
+        box = (bx0, by0, bx1, by1)
+        vertical = bool(trial % 2)
+        k = int(rng.integers(5, 60))
+        ids = rng.choice(d.n, k, replace=False).astype(np.uint32)
+        x = rng.uniform(0, W - 8, d.n)
+        y = rng.uniform(0, H - 1, d.n)
+        if trial % 3 == 0:  # ties in the sort key
+            x[ids[: k // 2]] = x[ids[0]]
+            y[ids[: k // 2]] = y[ids[0]]
+        did, m, cut, _, _ = o.split(box, vertical, 1, ids, 100.0, x, y)
+        assert did
+        centres = (x + 0.5 * f.width) if vertical else (y + 0.5 * f.height)
+        bm, bcut, target, room = brute_split(f.area, centres, ids.astype(np.int64), rows, box, vertical)
+        assert m == bm
+        extent = (bx1 - bx0) if vertical else (by1 - by0)
+        if room > 0:
+            # the cutline holds the low half's share of the free area
+            assert free_area_before(rows, box, vertical, cut) == pytest.approx(target, rel=1e-9, abs=1e-9 * room)
+            assert abs(cut - bcut) <= 1e-7 * extent or \
+                free_area_before(rows, box, vertical, min(cut, bcut)) == pytest.approx(target, rel=1e-9)
+
+
+def test_split_guards():
+    """m is at least 1 even when the first cell holds more than half the
+    area, and at most size - 1; a block at or below minCellsToSplit (4) or
+    at the level cap (20) does not split."""
+    o = ol.Oracle(row_design(6, 1.0, 40.0))
+    o.build_grid(1, 1.0)
+    x = np.array([1.0, 2.0, 3.0, 4.0, 5.0, 6.0])
+    y = np.zeros(6)
+    box = (0.0, 0.0, 40.0, 1.0)
+    assert o.split(box, True, 1, np.arange(6), 100.0, x, y)[:2] == (True, 3)
+    assert o.split(box, True, 1, np.arange(4), 100.0, x, y)[0] is False
+    assert o.split(box, True, 20, np.arange(6), 100.0, x, y)[0] is False
+    big = row_design(6, 1.0, 40.0)
+    big.width[0] = 30.0  # the leftmost cell (centre 1) holds 30 of 35
+    o2 = ol.Oracle(big)
+    o2.build_grid(1, 1.0)
+    xb = x.copy()
+    xb[0] = -14.0
+    assert o2.split(box, True, 1, np.arange(6), 100.0, xb, y)[:2] == (True, 1)
+    big.width[:] = 1.0
+    big.width[5] = 30.0  # the rightmost holds 30 of 35: the low part takes all but it
+    o3 = ol.Oracle(big)
+    o3.build_grid(1, 1.0)
+    assert o3.split(box, True, 1, np.arange(6), 100.0, x, y)[:2] == (True, 5)
+    # zero-area cells: the low half still takes one cell, and the cutline
+    # falls at half the free area (share 0.5)
+    o4 = ol.Oracle(row_design(6, 0.0, 40.0))
+    o4.build_grid(1, 1.0)
+    assert o4.split(box, True, 1, np.arange(6), 100.0, x, y)[:3] == (True, 1, 20.0)
+
+
+# ---------------------------------------------------------------- order and fill order
+def order_kept_on_blocks(seeds=range(4)) -> bool:
+    """Cells of one size keep their order along the cut axis through a split
+    (with macros cutting stripes); checked on random clumped blocks."""
+    macros = [(30.0, 4.0, 38.0, 12.0), (70.0, 0.0, 74.0, 20.0), (5.0, 15.0, 20.0, 18.0)]
+    raw = row_design(300, 2.0, 100.0, 20.0, macros)
+    o = ol.Oracle(raw)
+    o.build_grid(8, 1.0)
+    for seed in seeds:
+        rng = np.random.default_rng(seed)
+        for vertical in (True, False):
+            x = rng.normal(50, 8, 300)
+            y = rng.normal(10, 3, 300)
+            ids = np.arange(300, dtype=np.uint32)
+            _, m, cut, nx, ny = o.split((0.0, 0.0, 100.0, 20.0), vertical, 1, ids, 2000.0 * 0.1 * 10, x, y)
+            old = x if vertical else y
+            new = nx if vertical else ny
+            order = np.lexsort((ids, old))
+            if np.any(np.diff(new[order]) < 0):
+                return False
+            low, high = order[:m], order[m:]
+            lo_ok = (new[low] >= 0).all() and (new[low] + (2.0 if vertical else 1.0) <= cut + 1e-9).all()
+            hi_ok = (new[high] >= cut - 1e-9).all()
+            if not (lo_ok and hi_ok):
+                return False
+    return True
+
+
+def test_order_kept_inside_and_across_stripes():
+    assert order_kept_on_blocks()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_stripes_filled_farthest_first(seed):
+    """Random stripes tiling [0, L] and random cells: the stripes are taken
+    in order of decreasing distance of their centre from the cut, the cells
+    in order of decreasing distance, each stripe until it reaches its
+    capacity (the last one takes the rest)."""
+    rng = np.random.default_rng(seed)
+    k = int(rng.integers(2, 9))
+    edges = np.concatenate([[0.0], np.sort(rng.uniform(0, 100, k - 1)), [100.0]])
+    caps = rng.uniform(2, 30, k)
+    stripes = np.stack([edges[:-1], edges[1:], caps], 1)
+    perm = rng.permutation(k)  # any input order
+    stripes = stripes[perm]
+    below = bool(seed % 2)
+    cut = 100.0 if below else 0.0
+    m = int(rng.integers(3, 40))
+    w = rng.integers(1, 6, m).astype(np.float64)
+    centres = rng.uniform(0, 100, m)
+    _, which = pack_centres(stripes, centres, w, cut, below)
+    dist = (cut - centres) if below else (centres - cut)
+    sdist = np.abs(0.5 * (stripes[:, 0] + stripes[:, 1]) - cut)
+    rank = np.empty(k, np.int64)
+    rank[np.argsort(-sdist, kind="stable")] = np.arange(k)
+    cell_order = np.lexsort((np.arange(m), -dist))
+    r = rank[which[cell_order]]
+    assert np.all(np.diff(r) >= 0)  # farthest stripe first, never back
+    used = np.zeros(k)
+    for i in cell_order:
+        s = which[i]
+        # every stripe taken before s is at capacity; s was below it, unless it is the last
+        for t in np.nonzero(rank < rank[s])[0]:
+            assert used[t] >= stripes[t, 2]
+        assert used[s] < stripes[s, 2] or rank[s] == k - 1
+        used[s] += w[i]
+
+
+@pytest.mark.parametrize("mutation", [1, 2])
+def test_planted_mistakes_are_caught(mutation):
+    """Stripes filled nearest first (1) and a reversed affine map (2) each
+    break the docs' answers and the order invariant."""
+    assert docs_answers_hold() and order_kept_on_blocks()
+    ol.set_mutation(mutation)
+    try:
+        assert not docs_answers_hold()
+        assert not order_kept_on_blocks()
+    finally:
+        ol.set_mutation(0)
+    assert docs_answers_hold()
+
+
+# ---------------------------------------------------------------- clusters and growth
+def grid_design(nx=16, size=64, macros=()):
+    """A size x size die of unit rows, nx bins per axis of whole area, so
+    free area and occupancy sums stay exact integers."""
+    raw = row_design(4, 1.0, float(size), float(size), macros)
+    o = ol.Oracle(raw)
+    assert o.build_grid(nx, 1.0) == nx
+    return o
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_clusters_match_ndimage_label(seed):
+    o = grid_design(macros=[(8.0, 8.0, 16.0, 20.0), (40.0, 36.0, 44.0, 60.0)])
+    avail, _ = o.grid_avail()
+    nx = o.bins
+    rng = np.random.default_rng(seed)
+    occ = np.floor(avail * rng.uniform(0.0, 1.6, nx * nx))
+    occ[rng.random(nx * nx) < 0.05] += 5.0  # over-full bins with no room (macros)
+    got = o.overfilled_clusters(occ)
+    over = (occ > 1.0 * avail).reshape(nx, nx)  # [iy, ix]
+    labels, count = ndimage.label(over, structure=[[0, 1, 0], [1, 1, 1], [0, 1, 0]])
+    want = [(s[1].start, s[1].stop - 1, s[0].start, s[0].stop - 1) for s in ndimage.find_objects(labels)]
+    assert count == len(got) > 3
+    assert [tuple(r) for r in got.tolist()] == want
+
+
+def rect_density(occ2, avail2, r):
+    x0, x1, y0, y1 = r
+    c = occ2[y0:y1 + 1, x0:x1 + 1].sum()
+    a = avail2[y0:y1 + 1, x0:x1 + 1].sum()
+    if a > 0:
+        return c / a
+    return np.inf if c > 0 else 0.0
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_growth_takes_the_least_dense_expansion(seed):
+    o = grid_design(macros=[(8.0, 8.0, 16.0, 20.0), (40.0, 36.0, 44.0, 60.0)])
+    avail, _ = o.grid_avail()
+    nx = o.bins
+    rng = np.random.default_rng(100 + seed)
+    occ = np.floor(avail * rng.uniform(0.0, 1.3, nx * nx))
+    occ[rng.integers(0, nx * nx, 6)] += 60.0
+    occ2, avail2 = occ.reshape(nx, nx), avail.reshape(nx, nx)
+    steps_seen = 0
+    for cluster in o.overfilled_clusters(occ):
+        trace = o.grow(occ, cluster)
+        assert tuple(trace[0]) == tuple(cluster)
+        for a, b in zip(trace[:-1], trace[1:]):
+            assert rect_density(occ2, avail2, a) > 1.0  # it grows only while over the limit
+            x0, x1, y0, y1 = a
+            wider = [(max(x0 - 1, 0), x1, y0, y1), (x0, min(x1 + 1, nx - 1), y0, y1),
+                     (x0, x1, max(y0 - 1, 0), y1), (x0, x1, y0, min(y1 + 1, nx - 1))]
+            valid = [w for w in wider if w != tuple(a)]
+            dens = [rect_density(occ2, avail2, w) for w in valid]
+            best = valid[int(np.argmin(dens))]  # first of the least dense
+            assert tuple(b) == best
+            steps_seen += 1
+        last = tuple(trace[-1])
+        assert rect_density(occ2, avail2, last) <= 1.0 or last == (0, nx - 1, 0, nx - 1)
+    assert steps_seen > 5
+
+
+# ---------------------------------------------------------------- convergence
+def sequence(n, gap_at, upper_at):
+    gaps = [gap_at(i) for i in range(1, n + 1)]
+    uppers = [upper_at(i) for i in range(1, n + 1)]
+    return gaps, uppers
+
+
+CONVERGENCE_CASES = [
+    # gap below 5 % of the iteration-10 gap at 12 (4.9 < 5): stop at 12
+    ("tight", sequence(30, lambda i: 100.0 if i <= 10 else (50.0 if i == 11 else 4.9), lambda i: 1000.0 - i), 12),
+    # exactly 5 % is not below it; upper still improving: no stop
+    ("tight-boundary", sequence(30, lambda i: 100.0 if i <= 10 else 5.0, lambda i: 1000.0 - i), 0),
+    # within 25 %, upper flat from iteration 10: stalled for 6 at 16
+    ("relaxed-stall", sequence(30, lambda i: 100.0 if i <= 10 else 25.0, lambda i: 1000.0 - min(i, 10)), 16),
+    # within 25 % but 25.0001 %: never
+    ("relaxed-above", sequence(30, lambda i: 100.0 if i <= 10 else 25.0001, lambda i: 1000.0 - min(i, 10)), 0),
+    # tiny gaps before the reference iteration do not stop
+    ("before-reference", sequence(12, lambda i: 0.0 if i < 10 else (100.0 if i == 10 else 1.0), lambda i: 1.0), 11),
+    # a zero reference gap never stops
+    ("zero-reference", sequence(30, lambda i: 0.0, lambda i: 1.0), 0),
+    # a new best upper at 14 resets the stall count: stop at 20, not 16
+    ("stall-reset", sequence(30, lambda i: 100.0 if i <= 10 else 20.0,
+                             lambda i: 1000.0 - i if i <= 10 else (980.0 if i == 14 else 990.0)), 20),
+]
+
+
+@pytest.mark.parametrize("name,seq,stop", CONVERGENCE_CASES, ids=[c[0] for c in CONVERGENCE_CASES])
+def test_convergence_known_answers(name, seq, stop):
+    gaps, uppers = seq
+    assert ol.convergence(gaps, uppers) == stop
+
+
+@pytest.mark.parametrize("name,seq,stop", CONVERGENCE_CASES, ids=[c[0] for c in CONVERGENCE_CASES])
+def test_product_convergence_mirror(name, seq, stop):
+    gaps, uppers = seq
+    c = kp.Convergence(kp.SimplOptions())
+    got = next((i for i in range(1, len(gaps) + 1) if c.update(i, gaps[i - 1], uppers[i - 1])), 0)
+    assert got == stop
+
+
+def test_product_convergence_mirror_random():
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        n = int(rng.integers(5, 40))
+        gaps = rng.choice([0.0, 1.0, 5.0, 20.0, 25.0, 30.0, 100.0], n)
+        uppers = rng.choice([900.0, 950.0, 1000.0], n)
+        c = kp.Convergence(kp.SimplOptions())
+        got = next((i for i in range(1, n + 1) if c.update(i, gaps[i - 1], uppers[i - 1])), 0)
+        assert got == ol.convergence(gaps, uppers)
