@@ -180,6 +180,11 @@ int kt_lal(kt_ctx *ctx, double *x, double *y);
 int kt_lal_resident(kt_ctx *ctx);
 /* Batches of non-touching regions the last legalization ran (simpl_lal.cc:98-122). */
 int kt_lal_batches(kt_ctx *ctx, uint64_t *batches);
+/* Region runs the last legalization repeated so that a batch's result is the
+ * one of its regions run in order (a cell that moved into a later region of
+ * the same batch is legalized again there, as the reference does with one
+ * thread, simpl_lal.cc:102-122). */
+int kt_lal_reruns(kt_ctx *ctx, uint64_t *reruns);
 
 /* ---- launch-shape tuning (0 keeps the default) --------------------------- */
 /* lanes: threads per CSR row in the SpMV (1, 2, 4, 8 or 16); tile: rows per

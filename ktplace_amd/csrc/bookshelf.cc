@@ -31,6 +31,7 @@
 #include <zlib.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cctype>
 #include <cerrno>
 #include <cstdio>
@@ -200,9 +201,12 @@ struct Reader {
                 size_t k = 1;
                 while (k < t.size() && t[k] == ":") ++k;
                 double d;
-                if (k >= t.size() || !number(t[k], d) || d < 0.0) continue;
+                // a degree that is not a finite non-negative number is
+                // malformed; one past the file's line count cannot be filled
+                // and is capped there (no out-of-range conversion)
+                if (k >= t.size() || !number(t[k], d) || !std::isfinite(d) || d < 0.0) continue;
                 Open o;
-                o.degree = static_cast<size_t>(d);
+                o.degree = d >= static_cast<double>(lines.size()) ? lines.size() : static_cast<size_t>(d);
                 o.name = k + 1 < t.size() ? t[k + 1] : std::string();
                 open.push_back(std::move(o));
                 continue;

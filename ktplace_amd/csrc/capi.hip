@@ -691,6 +691,13 @@ int kt_lal_batches(kt_ctx *ctx, uint64_t *batches) {
     return KT_OK;
 }
 
+int kt_lal_reruns(kt_ctx *ctx, uint64_t *reruns) {
+    KT_GUARD(ctx);
+    KT_CHECK(reruns, KT_EINVAL, "kt_lal_reruns: null output");
+    *reruns = ctx->lalReruns;
+    return KT_OK;
+}
+
 int kt_set_warm_start(kt_ctx *ctx, const float *params, uint64_t count) {
     KT_GUARD(ctx);
     KT_CATCH(return warmSetParams(ctx, params, count);)

@@ -1452,6 +1452,7 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
 int lalLegalize(kt_ctx *ctx) {
     KT_CHECK(ctx->haveNetlist && ctx->grid.size() > 0, KT_ESTATE, "lal: build the grid first");
     KT_CHECK(ctx->posCount == ctx->n, KT_ESTATE, "lal: positions must have one entry per movable cell");
+    ctx->lalReruns = 0;
     const lal::Grid &g = ctx->grid;
     std::vector<double> occ;
     std::vector<uint32_t> cells;  // cells per bin, kept in step with the device's binOf

@@ -1,4 +1,29 @@
 // Host geometry of look-ahead legalization; see lal_geom.h.
+//
+// Profile::before / coordinateAt, FreeSpace::profile, the free-area deposit
+// of buildGrid and stripes() keep the reference's floating-point expressions
+// and their order, because legalized positions are compared with the
+// reference's bit for bit.  Those parts follow KTPlace (simpl_grid.h,
+// simpl_lal.cc, simpl_placer.cc), which carries this notice:
+//
+//   MIT License
+//   Copyright (c) 2026 Coherent17
+//   Permission is hereby granted, free of charge, to any person obtaining a
+//   copy of this software and associated documentation files (the
+//   "Software"), to deal in the Software without restriction, including
+//   without limitation the rights to use, copy, modify, merge, publish,
+//   distribute, sublicense, and/or sell copies of the Software, and to permit
+//   persons to whom the Software is furnished to do so, subject to the
+//   following conditions: The above copyright notice and this permission
+//   notice shall be included in all copies or substantial portions of the
+//   Software.
+//   THE SOFTWARE IS PROVIDED "AS IS", WITHOUT WARRANTY OF ANY KIND, EXPRESS OR
+//   IMPLIED, INCLUDING BUT NOT LIMITED TO THE WARRANTIES OF MERCHANTABILITY,
+//   FITNESS FOR A PARTICULAR PURPOSE AND NONINFRINGEMENT. IN NO EVENT SHALL THE
+//   AUTHORS OR COPYRIGHT HOLDERS BE LIABLE FOR ANY CLAIM, DAMAGES OR OTHER
+//   LIABILITY, WHETHER IN AN ACTION OF CONTRACT, TORT OR OTHERWISE, ARISING
+//   FROM, OUT OF OR IN CONNECTION WITH THE SOFTWARE OR THE USE OR OTHER
+//   DEALINGS IN THE SOFTWARE.
 #include "lal_geom.h"
 
 #include <algorithm>
@@ -29,28 +54,43 @@ double Profile::coordinateAt(double target) const {
 }
 
 // ------------------------------------------------------------------ FreeSpace
-// simpl_grid.h:140-176: rows sorted by bottom, each row's spans minus the
-// x-intervals of obstacles overlapping the row.
+// The free part of each row (simpl_grid.h:130-178 defines it): the row's spans
+// minus the x extent of every obstacle whose height overlaps the row.  The
+// obstacles of a row are first merged into disjoint blocked intervals
+// (intervals that overlap or touch become one; a zero-width obstacle stays a
+// point that splits a span); each span then keeps its gaps between
+// consecutive blocked intervals, clipped to the span, where they have
+// positive length.  Rows end up ordered by their bottom edge and each row's
+// spans by (start, end).
 void FreeSpace::build(std::vector<FreeRow> raw, const std::vector<Box> &obstacles) {
     rows = std::move(raw);
     std::stable_sort(rows.begin(), rows.end(), [](const FreeRow &a, const FreeRow &b) { return a.y0 < b.y0; });
+    std::vector<Span> blocked;
     for (FreeRow &row : rows) {
-        std::vector<Span> cuts;
+        blocked.clear();
         for (const Box &o : obstacles)
-            if (o.y1 > row.y0 && o.y0 < row.y1) cuts.push_back({o.x0, o.x1});
-        std::stable_sort(cuts.begin(), cuts.end());
-        std::stable_sort(row.spans.begin(), row.spans.end());
-        std::vector<Span> open;
-        for (const Span &span : row.spans) {
-            double a = span.a;
-            for (const Span &cut : cuts) {
-                if (cut.b <= a || cut.a >= span.b) continue;
-                if (cut.a > a) open.push_back({a, cut.a});
-                a = std::max(a, cut.b);
-            }
-            if (span.b > a) open.push_back({a, span.b});
+            if (o.y1 > row.y0 && o.y0 < row.y1) blocked.push_back({o.x0, o.x1});
+        std::sort(blocked.begin(), blocked.end());
+        size_t m = 0;
+        for (const Span &c : blocked) {
+            if (m > 0 && c.a <= blocked[m - 1].b)
+                blocked[m - 1].b = std::max(blocked[m - 1].b, c.b);
+            else
+                blocked[m++] = c;
         }
-        row.spans = std::move(open);
+        blocked.resize(m);
+        std::stable_sort(row.spans.begin(), row.spans.end());
+        std::vector<Span> free;
+        for (const Span &span : row.spans) {
+            double from = span.a;  // start of the current gap
+            for (const Span &c : blocked) {
+                const double to = std::min(span.b, c.a);
+                if (to > from) free.push_back({from, to});
+                from = std::max(from, c.b);
+            }
+            if (span.b > from) free.push_back({from, span.b});
+        }
+        row.spans = std::move(free);
     }
 }
 
@@ -231,35 +271,54 @@ double AreaTables::density(const BinRect &r) const {
     return cells > 0.0 ? std::numeric_limits<double>::infinity() : 0.0;
 }
 
+// Overfilled clusters (simpl_lal.cc:141-183 defines them): the bounding
+// boxes of the 4-connected components of bins with occ > g * avail, in the
+// order of each component's first bin in row-major order.  Labelled in one
+// row-major pass with a union-find over bin indices (each overfilled bin
+// joins its left and lower neighbours), then a second pass that grows each
+// root's box and lists the roots as they are first met.
 std::vector<BinRect> overfilledClusters(const Grid &g, const std::vector<double> &occ, double densityLimit) {
-    std::vector<char> over(g.size(), 0);
-    for (size_t k = 0; k < g.size(); ++k) over[k] = occ[k] > densityLimit * g.avail[k] ? 1 : 0;
-    std::vector<BinRect> regions;
-    std::vector<size_t> frontier;
-    for (size_t seed = 0; seed < g.size(); ++seed) {
-        if (!over[seed]) continue;
-        over[seed] = 0;
-        frontier.assign(1, seed);
-        BinRect r{seed % g.nx, seed % g.nx, seed / g.nx, seed / g.nx};
-        while (!frontier.empty()) {
-            const size_t k = frontier.back();
-            frontier.pop_back();
-            const size_t ix = k % g.nx, iy = k / g.nx;
-            r = {std::min(r.x0, ix), std::max(r.x1, ix), std::min(r.y0, iy), std::max(r.y1, iy)};
-            const std::array<std::pair<long, long>, 4> steps{{{1, 0}, {-1, 0}, {0, 1}, {0, -1}}};
-            for (const auto &[sx, sy] : steps) {
-                const long nx = static_cast<long>(ix) + sx, ny = static_cast<long>(iy) + sy;
-                if (nx < 0 || ny < 0 || nx >= static_cast<long>(g.nx) || ny >= static_cast<long>(g.ny)) continue;
-                const size_t kk = g.at(static_cast<size_t>(nx), static_cast<size_t>(ny));
-                if (over[kk]) {
-                    over[kk] = 0;
-                    frontier.push_back(kk);
-                }
-            }
+    const size_t nb = g.size();
+    constexpr uint32_t kNone = 0xffffffffu;
+    std::vector<uint32_t> parent(nb, kNone);
+    const auto root = [&parent](uint32_t k) {
+        uint32_t r = k;
+        while (parent[r] != r) r = parent[r];
+        while (parent[k] != r) {  // path compression
+            const uint32_t next = parent[k];
+            parent[k] = r;
+            k = next;
         }
-        regions.push_back(r);
+        return r;
+    };
+    const auto join = [&](uint32_t a, uint32_t b) {
+        const uint32_t ra = root(a), rb = root(b);
+        if (ra != rb) parent[std::max(ra, rb)] = std::min(ra, rb);  // the smaller index stays the root
+    };
+    for (size_t iy = 0; iy < g.ny; ++iy)
+        for (size_t ix = 0; ix < g.nx; ++ix) {
+            const size_t k = g.at(ix, iy);
+            if (!(occ[k] > densityLimit * g.avail[k])) continue;
+            parent[k] = static_cast<uint32_t>(k);
+            if (ix > 0 && parent[k - 1] != kNone) join(static_cast<uint32_t>(k - 1), static_cast<uint32_t>(k));
+            if (iy > 0 && parent[k - g.nx] != kNone)
+                join(static_cast<uint32_t>(k - g.nx), static_cast<uint32_t>(k));
+        }
+    std::vector<uint32_t> slot(nb, kNone);  // root -> output index
+    std::vector<BinRect> out;
+    for (size_t k = 0; k < nb; ++k) {
+        if (parent[k] == kNone) continue;
+        const uint32_t r = root(static_cast<uint32_t>(k));
+        const size_t ix = k % g.nx, iy = k / g.nx;
+        if (slot[r] == kNone) {
+            slot[r] = static_cast<uint32_t>(out.size());
+            out.push_back({ix, ix, iy, iy});
+        } else {
+            BinRect &b = out[slot[r]];
+            b = {std::min(b.x0, ix), std::max(b.x1, ix), std::min(b.y0, iy), std::max(b.y1, iy)};
+        }
     }
-    return regions;
+    return out;
 }
 
 namespace {

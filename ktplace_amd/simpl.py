@@ -108,6 +108,7 @@ _SIGS = {
     "kt_lal": (I32, [P, P, P]),
     "kt_lal_resident": (I32, [P]),
     "kt_lal_batches": (I32, [P, C.POINTER(C.c_uint64)]),
+    "kt_lal_reruns": (I32, [P, C.POINTER(C.c_uint64)]),
     "kt_set_warm_start": (I32, [P, P, U64]),
     "kt_warm_features": (I32, [P, P, P]),
     "kt_warm_predict": (I32, [P, P, P]),
@@ -277,6 +278,11 @@ class Context:
     def lal_batches(self) -> int:
         b = C.c_uint64()
         _check(lib.kt_lal_batches(self.h, C.byref(b)))
+        return b.value
+
+    def lal_reruns(self) -> int:
+        b = C.c_uint64()
+        _check(lib.kt_lal_reruns(self.h, C.byref(b)))
         return b.value
 
     # positions ---------------------------------------------------------------

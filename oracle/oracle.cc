@@ -27,9 +27,16 @@
 //                         zero-rhs rules; lock-step observer history
 //   getHpwl / occupancy / overflow   closed forms and area conservation
 //   FreeSpace / AxisProfile           hand-computed profiles
-//   LookAheadLegalizer   invariants only (in die, overflow reduced, order kept
-//                        inside stripes, deterministic): parity unpinned
-//                        beyond these invariants.  The regions of a batch run
+//   LookAheadLegalizer   packStripes: the reference docs' two worked answers
+//                        (02-look-ahead-legalization.md:97-109, :150-155), a
+//                        hand-computed split, farthest-first fill and order
+//                        kept along the cut axis; split: a numpy brute force
+//                        of the area median and of the cutline's free-area
+//                        share; getOverfilledClusters: scipy.ndimage.label;
+//                        getGrown: the least dense expansion at every step;
+//                        planted mistakes (nearest-first fill, reversed map)
+//                        fail these (tests/test_oracle_lal.py); in die,
+//                        overflow reduced, deterministic.  The regions of a batch run
 //                        one after the other, which is the reference with one
 //                        thread: simpl_lal.cc:102-122 runs them in a
 //                        parallel_for whose updates of members/binOf/occ
@@ -38,8 +45,32 @@
 //                        so with more threads the reference's result there
 //                        depends on timing.  ko_set_lal_snapshot(1) gives the
 //                        other extreme (all regions collect first)
-//   simplPlacer loop     HPWL bounds (lower <= upper), monotone initial rounds:
-//                        parity unpinned beyond these invariants
+//   simplPlacer loop     Convergence::update: hand-built (gap, upper) sequences
+//                        with known stopping iterations; HPWL bounds
+//                        (lower <= upper); the loop is the composition of
+//                        the pinned steps above
+//
+// This file restates KTPlace's code closely (that is its purpose), so it
+// carries KTPlace's notice:
+//
+//   MIT License
+//   Copyright (c) 2026 Coherent17
+//   Permission is hereby granted, free of charge, to any person obtaining a
+//   copy of this software and associated documentation files (the
+//   "Software"), to deal in the Software without restriction, including
+//   without limitation the rights to use, copy, modify, merge, publish,
+//   distribute, sublicense, and/or sell copies of the Software, and to permit
+//   persons to whom the Software is furnished to do so, subject to the
+//   following conditions: The above copyright notice and this permission
+//   notice shall be included in all copies or substantial portions of the
+//   Software.
+//   THE SOFTWARE IS PROVIDED "AS IS", WITHOUT WARRANTY OF ANY KIND, EXPRESS OR
+//   IMPLIED, INCLUDING BUT NOT LIMITED TO THE WARRANTIES OF MERCHANTABILITY,
+//   FITNESS FOR A PARTICULAR PURPOSE AND NONINFRINGEMENT. IN NO EVENT SHALL THE
+//   AUTHORS OR COPYRIGHT HOLDERS BE LIABLE FOR ANY CLAIM, DAMAGES OR OTHER
+//   LIABILITY, WHETHER IN AN ACTION OF CONTRACT, TORT OR OTHERWISE, ARISING
+//   FROM, OUT OF OR IN CONNECTION WITH THE SOFTWARE OR THE USE OR OTHER
+//   DEALINGS IN THE SOFTWARE.
 #include "../include/ktplace_gen.h"
 
 #include <omp.h>

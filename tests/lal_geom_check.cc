@@ -7,7 +7,9 @@
 // stdin: R, then R lines "y height x0 x1 pitch"; the die "x0 y0 x1 y1"; K,
 // then K obstacle lines "x0 y0 x1 y1"; bins per axis; Q, then Q queries,
 // "0 x0 y0 x1 y1 vertical" (profile) or "1 x0 y0 x1 y1 vertical cellArea
-// regionRoom densityLimit stripeAreaFraction cutline belowCut" (stripes).
+// regionRoom densityLimit stripeAreaFraction cutline belowCut" (stripes), or
+// "2 densityLimit occ[0] ... occ[bins-1]" (overfilled clusters of that
+// occupancy, printed as x0 x1 y0 y1 per cluster in decimal).
 // Numbers are hex floats (%a) so they arrive exact.
 #include <cinttypes>
 #include <cstdio>
@@ -55,7 +57,18 @@ int main() {
     std::printf("\n");
     const size_t q = static_cast<size_t>(num());
     for (size_t k = 0; k < q; ++k) {
-        const bool isStripes = num() != 0.0;
+        const double kind = num();
+        if (kind == 2.0) {
+            const double limit = num();
+            std::vector<double> occ(g.size());
+            for (double &v : occ) v = num();
+            std::printf("clusters");
+            for (const BinRect &r : overfilledClusters(g, occ, limit))
+                std::printf(" %zu %zu %zu %zu", r.x0, r.x1, r.y0, r.y1);
+            std::printf("\n");
+            continue;
+        }
+        const bool isStripes = kind != 0.0;
         Box b{};
         b.x0 = num();
         b.y0 = num();

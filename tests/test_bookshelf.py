@@ -255,3 +255,30 @@ def test_ibm01_hpwl_of_its_placement_matches_an_independent_parse(ibm01):
     y = np.array([pos[names[v]][1] for v in mov])
     o = ol.Oracle.of(d)
     assert o.hpwl(x, y) == pytest.approx(hpwl, rel=1e-12)
+
+
+def test_malformed_net_degrees_are_skipped(tmp_path):
+    """NetDegree values that are not finite non-negative numbers (nan, inf,
+    -2, 1e300, 2^64) are malformed headers: the reader skips them (their pin
+    lines join the previous net, as any line after a header does) instead of
+    converting them to an integer; a degree beyond the file is capped."""
+    nets = """UCLA nets 1.0
+NumNets : 6
+NumPins : 8
+NetDegree : 2 good
+   a I : 0 0
+   b O : 0 0
+NetDegree : nan n1
+NetDegree : inf n2
+NetDegree : -2 n3
+NetDegree : 1e300 huge
+   a I : 0 0
+   m O : 0 0
+NetDegree : 18446744073709551616 big
+   b I : 0 0
+   m O : 0 0
+"""
+    d = dz.read_bookshelf(write_design(tmp_path, nodes=NODES, nets=nets, pl=PL, scl=SCL, wts=WTS))
+    names = d.meta["names"]
+    got = [sorted(names[v] for v, _, _ in pins_of(d, k)) for k in range(d.flat.num_nets)]
+    assert got == [["a", "b"], ["a", "m"], ["b", "m"]]

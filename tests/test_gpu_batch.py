@@ -126,3 +126,41 @@ def test_batch_edge_cases(layout):
     per, _ = b.solve(1e-12, 5)
     assert per[0]["steps"][0] == 5
     b.close()
+
+
+def test_batch_all_256_instances_of_the_benchmark():
+    """The benchmark's batched workload at its full size: 256 netlists of
+    2,000-40,000 cells (log-uniform), every x and y system solved in one
+    batch to a true relative residual of 1e-6, each solution's residual
+    checked on the CPU with the oracle's own B2B matrix."""
+    from ktplace_amd.batched import make_designs, start_context
+    designs = make_designs(256)
+    ctxs = []
+    b = simpl.Batch(0)
+    try:
+        for k, d in enumerate(designs):
+            ctxs.append(start_context(d, 1e-6))
+            b.add_ctx(ctxs[-1])
+            if k % 64 == 63:
+                print(f"{k + 1} instances ready", flush=True)
+        n, rows, nnz = b.size()
+        assert n == 256 and rows == 2 * sum(d.n for d in designs)
+        per, tot = b.solve(1e-6, 100000, simpl.KT_STOP_TRUE)
+        print(f"batched: {rows} rows, {nnz} entries, {tot}", flush=True)
+        worst = 0.0
+        for i, (d, c) in enumerate(zip(designs, ctxs)):
+            x0, y0 = c.get_positions()  # the start point the instance was added with
+            o = ol.Oracle.of(d)
+            o.build_b2b(x0, y0, floors=d.flat.floors)
+            gx, gy = b.get_positions(i)
+            for ax, v in ((0, gx), (1, gy)):
+                s = o.system(ax)
+                true = np.linalg.norm(s.rhs - s.scipy() @ v) / np.linalg.norm(s.rhs)
+                worst = max(worst, true)
+                assert true <= 1.05e-6, (i, d.n, ax, true)
+            assert max(per[i]["true_resid"]) <= 1e-6
+        print(f"worst true residual over 512 systems (oracle matrices): {worst:.3e}", flush=True)
+    finally:
+        b.close()
+        for c in ctxs:
+            c.close()

@@ -1,0 +1,96 @@
+"""The 1M-cell workloads at their full size, against the FP64 oracle.
+
+Workload `cells1m`: 1,000,000 movable cells, 1,100,000 nets with a degree tail
+to 200 pins (0.1 % of nets), 2,000 pads.  Start point as in the benchmark:
+seedUniform, one alpha = 0 B2B round solved to 1e-6, then the B2B systems at
+that point.  The oracle builds the same systems (bit-identical CSR) and
+solves them in FP64 to a relative residual of 1e-12: that is the reference
+solution every GPU path is measured against.
+
+Bars:
+* every path (plain FP64; FP32 CG with FP64 refinement, from the previous
+  iterate and from the learned start) solved to a true relative residual of
+  1e-9 is within 1e-6 relative 2-norm error of the oracle's solution, per axis;
+* at the benchmark's tolerance 1e-6, the learned start needs fewer lock-step
+  iterations than the previous-iterate start, and both reach the tolerance
+  on the true FP64 residual (the network was trained on 50K-200K-cell
+  netlists; this one is unseen).
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from ktplace_amd import design as dz
+from ktplace_amd import simpl
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def cells1m():
+    d = dz.workload("cells1m")
+    ctx = simpl.Context(0).upload(d)
+    x0, y0 = dz.seed_uniform(d, 20240607)
+    ctx.build_b2b(x0, y0)
+    x1, y1 = x0.copy(), y0.copy()
+    r0 = ctx.solve_both(x1, y1, 1e-6, 1_000_000)
+    print(f"cells1m start point: round-0 solve {r0['iterations']} iterations", flush=True)
+    ctx.set_positions(x1, y1)
+    ctx.save_positions()
+    ctx.build_b2b_resident(0.0)
+    ctx.build_grid(0, 1.0)
+    from ktplace_amd.learned import load_params
+    ctx.set_warm_start(load_params())
+    o = ol.Oracle.of(d)
+    o.build_b2b(x1, y1, floors=d.flat.floors)
+    for axis in (0, 1):
+        g, s = ctx.get_system(axis), o.system(axis)
+        assert np.array_equal(g.col, s.col) and np.array_equal(g.val, s.val)
+        assert np.array_equal(g.diag, s.diag) and np.array_equal(g.rhs, s.rhs)
+    ref = o.solve(x1, y1, tol=1e-12, max_iter=1_000_000)
+    print(f"cells1m oracle FP64 solve to 1e-12: {ref['iterations']} iterations", flush=True)
+    yield d, ctx, ref
+    ctx.close()
+
+
+def solve_from_start(ctx, tol, precision, warm):
+    ctx.restore_positions()
+    ctx.build_b2b_resident(0.0)
+    r = ctx.solve_resident(tol, 1_000_000, precision=precision, stop_rule=simpl.KT_STOP_TRUE,
+                           warm_start=warm)
+    x, y = ctx.get_positions()
+    return r, x, y
+
+
+PATHS = [("fp64", simpl.KT_PREC_FP64, simpl.KT_WARM_PREVIOUS),
+         ("mixed", simpl.KT_PREC_MIXED, simpl.KT_WARM_PREVIOUS),
+         ("mixed+learned", simpl.KT_PREC_MIXED, simpl.KT_WARM_LEARNED)]
+
+
+@pytest.mark.parametrize("name,prec,warm", PATHS, ids=[p[0] for p in PATHS])
+def test_solution_error_against_fp64_oracle(cells1m, name, prec, warm):
+    d, ctx, ref = cells1m
+    r, x, y = solve_from_start(ctx, 1e-9, prec, warm)
+    ex, ey = rel(x, ref["x"]), rel(y, ref["y"])
+    print(f"{name}: {r['iterations']} iterations, true resid {r['true_resid']}, "
+          f"error vs FP64 oracle x {ex:.3e} y {ey:.3e}", flush=True)
+    assert max(r["true_resid"]) <= 1e-9
+    assert ex <= 1e-6 and ey <= 1e-6
+
+
+def test_learned_start_cuts_iterations_at_1e6(cells1m):
+    d, ctx, ref = cells1m
+    rp, xp, yp = solve_from_start(ctx, 1e-6, simpl.KT_PREC_MIXED, simpl.KT_WARM_PREVIOUS)
+    rl, xl, yl = solve_from_start(ctx, 1e-6, simpl.KT_PREC_MIXED, simpl.KT_WARM_LEARNED)
+    print(f"previous-iterate start: {rp['iterations']} iterations (axis steps {rp['steps']}), "
+          f"error x {rel(xp, ref['x']):.3e} y {rel(yp, ref['y']):.3e}", flush=True)
+    print(f"learned start: {rl['iterations']} iterations (axis steps {rl['steps']}), gamma {rl['warm_gamma']}, "
+          f"error x {rel(xl, ref['x']):.3e} y {rel(yl, ref['y']):.3e}", flush=True)
+    assert max(rp["true_resid"]) <= 1e-6 and max(rl["true_resid"]) <= 1e-6
+    assert rl["iterations"] < rp["iterations"]
+    assert sum(rl["steps"]) < sum(rp["steps"])
+    assert min(rl["warm_gamma"]) > 0

@@ -66,7 +66,53 @@ def test_lal_matches_oracle(case, gpu_ctx_factory, spread):
     og = o.overflow(o.occupancy(gx, gy))
     orr = o.overflow(o.occupancy(rx, ry))
     assert og <= orr + 0.01
-    assert same > 0.99
+    assert np.array_equal(gx, rx) and np.array_equal(gy, ry)
+    ctx.close()
+
+
+@pytest.mark.parametrize("opts", [dict(min_cells_to_split=10 ** 9), dict(max_level=1),
+                                  dict(max_level=3), dict(stripe_area_fraction=0.03)])
+def test_lal_options_match_oracle(gpu_ctx_factory, opts):
+    """The LAL options (simpl_options.h:24-26) set through the C ABI: a split
+    threshold above every region (the GPU skips those batches without running
+    them), a level cap of 1 (no block splits) or 3, finer stripes: positions
+    identical to the oracle's with the same options."""
+    d = dz.workload("cells12k")
+    o = ol.Oracle.of(d)
+    ctx = gpu_ctx_factory(d)
+    ctx.build_grid(0, 1.0)
+    o.build_grid(0, 1.0)
+    ctx.set_lal_options(**opts)
+    o.set_lal_options(**opts)
+    x, y = clumped(d, 4, 0.1)
+    gx, gy = ctx.lal(x, y)
+    rx, ry = o.lal(x, y)
+    print(opts, "identical", np.mean((gx == rx) & (gy == ry)), flush=True)
+    assert np.array_equal(gx, rx) and np.array_equal(gy, ry)
+    if "min_cells_to_split" in opts or opts.get("max_level") == 1:
+        inside = np.clip(x, 0, d.die[2] - d.flat.width), np.clip(y, 0, d.die[3] - d.flat.height)
+        assert np.array_equal(gx, inside[0]) and np.array_equal(gy, inside[1])
+    ctx.close()
+
+
+def test_lal_reruns_match_oracle(gpu_ctx_factory):
+    """Bins narrower than cells (a 1536 x 1536 grid on the 12.5K-cell die):
+    cells end up centred in other regions of the same batch, the GPU reruns
+    those regions in order, and the result is the oracle's cell for cell."""
+    d = dz.workload("cells12k")
+    o = ol.Oracle.of(d)
+    ctx = gpu_ctx_factory(d)
+    reruns = 0
+    for bins, seed in ((1536, 5), (768, 6), (1024, 7)):
+        ctx.build_grid(bins, 1.0)
+        o.build_grid(bins, 1.0)
+        x, y = clumped(d, seed, 0.05)
+        gx, gy = ctx.lal(x, y)
+        rx, ry = o.lal(x, y)
+        reruns += ctx.lal_reruns()
+        print(bins, "batches", ctx.lal_batches(), "reruns", ctx.lal_reruns(), flush=True)
+        assert np.array_equal(gx, rx) and np.array_equal(gy, ry)
+    assert reruns > 0
     ctx.close()
 
 

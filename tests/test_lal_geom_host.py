@@ -1,6 +1,6 @@
 """The product's host geometry of look-ahead legalization (csrc/lal_geom.cc:
 free space, the grid's free area per bin, free-area profiles, stripes in
-fill order) against the oracle's restatement of simpl_grid.h:67-231,
+fill order, overfilled clusters) against the oracle's restatement of simpl_grid.h:67-231,
 simpl_placer.cc:190-224 and simpl_lal.cc:238-282, bit for bit, on the CPU.  lal_geom.cc is compiled with g++ into a small harness
 (tests/lal_geom_check.cc); the GPU legalizer's parity tests cover the rest."""
 import shutil
@@ -46,7 +46,10 @@ def _run(exe, d, bins, queries) -> tuple:
     lines += [" ".join(float(v).hex() for v in o) for o in obs]
     lines.append(str(bins))
     lines.append(str(len(queries)))
-    for q in queries:  # ("P", box, vertical) or ("S", box, vertical, cell, room, g, frac, cut, below)
+    for q in queries:  # ("P", box, vertical), ("S", box, vertical, cell, room, g, frac, cut, below)
+        if q[0] == "C":  # ("C", density limit, occupancy per bin)
+            lines.append("2 " + float(q[1]).hex() + " " + " ".join(float(v).hex() for v in q[2]))
+            continue
         nums = list(q[1]) + [int(q[2])] + [v for v in q[3:]]
         lines.append(("1 " if q[0] == "S" else "0 ") +
                      " ".join(v.hex() if isinstance(v, float) else str(v) for v in map(_num, nums)))
@@ -57,6 +60,8 @@ def _run(exe, d, bins, queries) -> tuple:
     for q in queries:
         if q[0] == "S":
             results.append([int(t, 16) for t in next(out).split()[1:]])
+        elif q[0] == "C":
+            results.append([int(t) for t in next(out).split()[1:]])
         else:
             e = [int(t, 16) for t in next(out).split()[1:]]
             p = [int(t, 16) for t in next(out).split()[1:]]
@@ -106,3 +111,64 @@ def test_free_space_grid_and_profiles_match_the_oracle(harness, seed, bins, macr
             re, rp = o.profile(q[1], q[2])
             assert got == (_bits(re), _bits(rp)), q
     assert stripes_seen > 400
+
+
+def test_free_space_with_touching_and_overlapping_macros(harness):
+    """FreeSpace::build on macros that overlap, touch, reach past the die or
+    cover part of a row's height, and on rows split into touching or
+    separated subrows: free area per bin and profiles equal the oracle's bit
+    for bit."""
+    cells = [(1.0, 0.0, 2.0, 1.0, False), (3.0, 0.0, 2.0, 1.0, False)]
+    macros = [(10.0, 2.0, 20.0, 8.0), (15.0, 4.0, 25.0, 12.0),     # overlapping
+              (30.0, 0.0, 34.0, 6.0), (34.0, 3.0, 38.0, 9.0),      # touching
+              (55.0, 15.0, 70.0, 25.0),                            # past the die
+              (40.5, 10.25, 45.5, 13.75),                          # part of four rows' height
+              (2.0, 16.0, 3.0, 17.0), (2.0, 16.0, 3.0, 17.0)]      # the same twice
+    cells += [(m[0], m[1], m[2] - m[0], m[3] - m[1], True) for m in macros]
+    rows = [(float(r), 1.0, 0.0, 60.0) for r in range(20) if r not in (5, 7)]
+    rows += [(5.0, 1.0, 0.0, 30.0), (5.0, 1.0, 30.0, 60.0), (7.0, 1.0, 0.0, 25.0), (7.0, 1.0, 35.0, 60.0)]
+    rows.sort(key=lambda r: (r[0], r[2]))
+    raw = dz.make_raw(cells, [[(0, 0.0, 0.0), (1, 0.0, 0.0)]], die=(0.0, 0.0, 60.0, 20.0), rows=rows)
+    d = dz.collect(raw)
+    o = ol.Oracle(raw)
+    rng = np.random.default_rng(3)
+    for bins in (1, 7, 16, 40):
+        per = o.build_grid(bins, 1.0)
+        queries = [("P", (0.0, 0.0, 60.0, 20.0), True), ("P", (0.0, 0.0, 60.0, 20.0), False)]
+        for k in range(60):
+            x0, x1 = np.sort(rng.uniform(-2, 62, 2))
+            y0, y1 = np.sort(rng.uniform(-1, 21, 2))
+            queries.append(("P", (x0, y0, x1, y1), bool(k % 2)))
+        avail, results = _run(harness, d, per, queries)
+        ref_avail, _ = o.grid_avail()
+        assert avail == _bits(ref_avail), bins
+        for q, got in zip(queries, results):
+            re, rp = o.profile(q[1], q[2])
+            assert got == (_bits(re), _bits(rp)), (bins, q)
+
+
+@pytest.mark.parametrize("seed,bins", [(1, 16), (2, 37), (3, 64)])
+def test_overfilled_clusters_match_label_and_oracle(harness, seed, bins):
+    """overfilledClusters (a union-find labelling) against the oracle's flood
+    fill and scipy.ndimage.label (4-connected), bounding boxes in the order
+    of each cluster's first bin, on random occupancies with many touching
+    and diagonal-only neighbours."""
+    from scipy import ndimage
+    d = dz.generate(3000, 3300, 60, seed=seed, num_macros=10)
+    o = ol.Oracle.of(d)
+    per = o.build_grid(bins, 1.0)
+    avail, _ = o.grid_avail()
+    rng = np.random.default_rng(seed)
+    queries = []
+    for frac in (0.2, 0.45, 0.6):
+        occ = np.where(rng.random(per * per) < frac, avail * 1.5 + 1.0, avail * 0.5)
+        queries.append(("C", 1.0, occ))
+    _, results = _run(harness, d, per, queries)
+    for (_, limit, occ), got in zip(queries, results):
+        ref = o.overfilled_clusters(occ).reshape(-1).tolist()
+        assert got == ref
+        over = (occ > limit * avail).reshape(per, per)
+        labels, count = ndimage.label(over, structure=[[0, 1, 0], [1, 1, 1], [0, 1, 0]])
+        want = [v for sl in ndimage.find_objects(labels)
+                for v in (sl[1].start, sl[1].stop - 1, sl[0].start, sl[0].stop - 1)]
+        assert got == want and count > 5

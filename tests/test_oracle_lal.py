@@ -440,3 +440,50 @@ def test_product_convergence_mirror_random():
         c = kp.Convergence(kp.SimplOptions())
         got = next((i for i in range(1, n + 1) if c.update(i, gaps[i - 1], uppers[i - 1])), 0)
         assert got == ol.convergence(gaps, uppers)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_get_stripes_tile_the_half_block(seed):
+    """getStripes (simpl_lal.cc:238-262) on random boxes of a design with
+    macros: the stripes tile [lo, hi] without gaps, every macro edge inside
+    the box is a stripe boundary, no stripe holds more free area than the
+    limit (stripeAreaFraction of the region's room), and each capacity is
+    min(g, cellArea / room) times the stripe's free area, with the free area
+    integrated from the rows and macros directly."""
+    d = dz.generate(1500, 1650, 40, seed=seed + 11, num_macros=8, utilization=0.6)
+    o = ol.Oracle.of(d)
+    o.build_grid(0, 1.0)
+    rows = free_intervals(d.raw)
+    rng = np.random.default_rng(seed)
+    W, H = d.die[2], d.die[3]
+    macros = np.asarray(d.flat.obstacles).reshape(-1, 4)
+    checked = 0
+    for trial in range(30):
+        bx0, bx1 = np.sort(rng.uniform(0, W, 2))
+        by0, by1 = np.sort(rng.uniform(0, H, 2))
+        box = (bx0, by0, bx1, by1)
+        vertical = bool(trial % 2)
+        lo, hi = (bx0, bx1) if vertical else (by0, by1)
+        room = free_area_before(rows, box, vertical, hi)
+        if room <= 0:
+            continue
+        region = room * rng.uniform(1.0, 4.0)
+        cell_area = room * rng.uniform(0.2, 1.3)
+        s = o.stripes(box, vertical, cell_area, region, hi, True)
+        s = s[np.argsort(s[:, 0])]
+        assert s[0, 0] == lo and s[-1, 1] == hi and np.all(s[1:, 0] == s[:-1, 1])
+        bounds = set(s[:, 0].tolist()) | {hi}
+        for m in macros:
+            if m[2] <= bx0 or m[0] >= bx1 or m[3] <= by0 or m[1] >= by1:
+                continue
+            for e in ((m[0], m[2]) if vertical else (m[1], m[3])):
+                if lo < e < hi:
+                    assert e in bounds
+        fill = min(1.0, cell_area / room)
+        limit = 0.1 * region
+        for a, b, cap in s:
+            here = free_area_before(rows, box, vertical, b) - free_area_before(rows, box, vertical, a)
+            assert here <= limit * (1 + 1e-9) + 1e-9
+            assert cap == pytest.approx(fill * here, rel=1e-9, abs=1e-9 * room)
+        checked += 1
+    assert checked > 20
