@@ -46,25 +46,90 @@ def spmv_bytes(nnz: int, n: int, precision: str) -> int:
     return 8 * nnz + 16 * n
 
 
-def cpu_baseline(d, x1, y1, tol: float) -> dict:
-    """The oracle (CPU restatement of the reference) on the same start point:
-    one B2B build + solve to tol, timed on this host's cores."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(d, x1, y1, tol: float, runs: int = 5) -> dict:
+    """The oracle (CPU restatement of the reference) on the same start point,
+    timed on this host's cores.
+
+    All cores: one warm-up step, then the median of `runs` full steps (B2B
+    build of both axes + FP64 Jacobi-PCG to tol, the reference's recurrence
+    stopping rule, iteration cap lifted).  One core: a bounded sample, the
+    median of 3 runs of the build plus 60 CG iterations (the reference's
+    default cap), scaled to the full step's iteration count."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as ol
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    threads = max(1, min(threads, os.cpu_count() or 1))
-    ol.set_threads(threads)
+    nproc = os.cpu_count() or 1
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    threads = int(os.environ.get("OMP_NUM_THREADS", nproc))
+    threads = max(1, min(threads, nproc))
     o = ol.Oracle.of(d)
-    t0 = time.perf_counter()
+
+    def full_step():
+        t0 = time.perf_counter()
+        o.build_b2b(x1, y1, floors=d.flat.floors)
+        t1 = time.perf_counter()
+        r = o.solve(x1, y1, tol=tol, max_iter=1_000_000)
+        return 1e3 * (t1 - t0), 1e3 * (time.perf_counter() - t1), r["iterations"]
+
+    ol.set_threads(threads)
+    full_step()  # warm-up
+    all_runs = [full_step() for _ in range(runs)]
+    builds, solves, iters = zip(*all_runs)
+    total = [b + s for b, s, _ in all_runs]
+    iterations = iters[-1]
+
+    ol.set_threads(1)
+    sample = 60
+    one = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        o.build_b2b(x1, y1, floors=d.flat.floors)
+        t1 = time.perf_counter()
+        o.solve(x1, y1, tol=0.0, max_iter=sample)
+        t2 = time.perf_counter()
+        one.append((1e3 * (t1 - t0), 1e3 * (t2 - t1) / sample))
+    one_build = float(np.median([b for b, _ in one]))
+    one_iter = float(np.median([i for _, i in one]))
+    ol.set_threads(threads)
+    return dict(
+        value=float(np.median(total)), unit="ms", cores=threads, kind="port",
+        sample=(f"the oracle on the same {d.n}-cell system and start point: one B2B build of both axes + "
+                f"FP64 Jacobi-PCG to {tol:g} (recurrence rule), {iterations} iterations; median of {runs} "
+                f"runs after one warm-up on {threads} threads"),
+        build_ms=round(float(np.median(builds)), 1), solve_ms=round(float(np.median(solves)), 1),
+        runs_ms=[round(v, 1) for v in total], iterations=iterations,
+        one_core=dict(value=round(one_build + one_iter * iterations, 1), unit="ms", build_ms=round(one_build, 1),
+                      ms_per_iteration=round(one_iter, 3),
+                      sample=(f"median of 3 runs of the build + {sample} CG iterations on 1 thread, "
+                              f"scaled to {iterations} iterations")),
+        cpu_model=cpu_model(), nproc=nproc)
+
+
+def oracle_solution(d, x1, y1) -> dict:
+    """The oracle's FP64 solution of the start-point systems to a relative
+    residual of 1e-12: the reference every GPU path's error is taken against
+    (computed outside every timed region)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as ol
+    o = ol.Oracle.of(d)
     o.build_b2b(x1, y1, floors=d.flat.floors)
-    t1 = time.perf_counter()
-    r = o.solve(x1, y1, tol=tol, max_iter=1_000_000)
-    t2 = time.perf_counter()
-    return dict(value=(t2 - t0) * 1e3, unit="ms", cores=threads, kind="port",
-                sample=(f"one full step on the same {d.n}-cell system from the same start: "
-                        f"B2B build {1e3 * (t1 - t0):.0f} ms + FP64 Jacobi-PCG to {tol:g} "
-                        f"(recurrence rule) {1e3 * (t2 - t1):.0f} ms, {r['iterations']} iterations"),
-                iterations=r["iterations"])
+    return o.solve(x1, y1, tol=1e-12, max_iter=1_000_000)
+
+
+def rel_error(a, b) -> float:
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
 def main() -> int:
@@ -83,6 +148,8 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-oracle-error", action="store_true",
+                    help="skip the FP64 oracle solution that every path's error is taken against")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run the plain copy kernel on this many MiB (counter calibration)")
     ap.add_argument("--lanes", type=int, default=0, help="skip the autotuner: lanes per row")
@@ -129,8 +196,16 @@ def main() -> int:
         ctx.set_warm_start(load_params(args.warm_params or None))
         log(f"[bench] warm-start network loaded; {bins}x{bins} bins")
 
+    ref = None
+    if not args.no_oracle_error:
+        t = time.perf_counter()
+        ref = oracle_solution(d, x1, y1)
+        log(f"[bench] FP64 oracle solution to 1e-12: {ref['iterations']} iterations "
+            f"({time.perf_counter() - t:.1f} s)")
+
     from ktplace_amd.autotune import TuneResult, autotune
     per = {}
+    failed = []
     tuned_by_prec = {}
     for path in paths:
         base_path = path.split("+")[0]
@@ -187,12 +262,23 @@ def main() -> int:
         last = results[-1]
         for i, r in enumerate(results):
             log(f"[bench] {path} step {i}: {r['iterations']} iterations, true resid {r['true_resid']}")
+        bad = [r["true_resid"] for r in results + presults if not max(r["true_resid"]) <= args.tol]
+        if bad:
+            failed.append(path)
+            log(f"[bench] ERROR {path}: {len(bad)} timed steps ended above tol {args.tol:g}: {bad[:3]}")
+        error = None
+        if ref is not None:  # outside the timed region: one more step, then its positions
+            step(False)
+            gx, gy = ctx.get_positions()
+            error = [rel_error(gx, ref["x"]), rel_error(gy, ref["y"])]
+            log(f"[bench] {path}: relative error vs FP64 oracle x {error[0]:.3e} y {error[1]:.3e}")
         log(f"[bench] {path}: {ms:.2f} ms/step (graph replay), {ms_prof:.2f} ms/step with events; "
             f"SpMV {avg_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s algorithmic")
         per[path] = dict(
             ms_per_step=round(ms, 3), ms_per_step_with_kernel_events=round(ms_prof, 3),
             iterations=last["iterations"], steps_per_axis=list(last["steps"]),
             true_resid=list(last["true_resid"]),
+            error_vs_fp64_oracle=error,
             refreshes_or_restarts=last["refinements"],
             warm_start=("learned (line-search gamma %.3f, %.3f)" % tuple(last["warm_gamma"])
                         if warm == simpl.KT_WARM_LEARNED else "previous iterate"),
@@ -220,7 +306,8 @@ def main() -> int:
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU oracle on the same start point ...")
         base = cpu_baseline(d, x1, y1, args.tol)
-        log(f"[bench] cpu baseline {base['value']:.0f} ms on {base['cores']} threads")
+        log(f"[bench] cpu baseline {base['value']:.0f} ms on {base['cores']} threads ({base['cpu_model']}); "
+            f"1 core {base['one_core']['value']:.0f} ms")
 
     batched = None
     if args.batched_count:
@@ -259,6 +346,9 @@ def main() -> int:
             "paths": per,
             "speedup_vs_plain_fp64": (round(per["fp64"]["ms_per_step"] / ms, 3)
                                       if "fp64" in per else None),
+            "error_vs_fp64_oracle": per[main_path]["error_vs_fp64_oracle"],
+            "oracle_reference": ("the oracle's FP64 solution to a relative residual of 1e-12 "
+                                 f"({ref['iterations']} iterations)" if ref is not None else None),
             "nnz_per_axis": nnz,
             "copy_probe": copy,
             "batched_designs": batched,
@@ -268,6 +358,9 @@ def main() -> int:
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+    if failed:
+        log(f"[bench] paths that missed the tolerance: {failed}")
+        return 1
     return 0
 
 

@@ -258,6 +258,11 @@ int kt_reset_kernel_times(kt_ctx *ctx);
 
 /* Network used by KT_WARM_LEARNED solves on this context (NULL clears it). */
 int kt_set_warm_start(kt_ctx *ctx, const float *params, uint64_t count);
+/* Movable cells with no net path to a fixed vertex (their component's
+ * translation is in the null space of the systems built without anchors):
+ * the learned start leaves them at the previous iterate.  flags [n] (may be
+ * null) receives 1 for each such cell.  No counterpart in the reference. */
+int kt_floating_cells(kt_ctx *ctx, uint32_t *count, uint8_t *flags);
 /* Features [n][KT_MLP_FEATURES] of the context's positions and current
  * systems (build them first), and the spread scale[2] = {s_x, s_y} that
  * turns network outputs into displacements.  features may be NULL. */

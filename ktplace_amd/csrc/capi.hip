@@ -156,6 +156,35 @@ int uploadNetlist(kt_ctx *ctx, const kt_netlist &nl) {
         }
     }
     KT_CHECK(bound < 0xffffffffull, KT_EINVAL, "upload: B2B entries exceed 32-bit indices");
+    // Cells with no net path to a fixed vertex: union-find over vertices, each
+    // net joining its pins; a component holding a fixed vertex is anchored.
+    std::vector<uint32_t> parent(V);
+    for (uint32_t v = 0; v < V; ++v) parent[v] = v;
+    const auto root = [&parent](uint32_t v) {
+        while (parent[v] != v) {
+            parent[v] = parent[parent[v]];  // path halving
+            v = parent[v];
+        }
+        return v;
+    };
+    for (uint32_t j = 0; j < N; ++j) {
+        const uint64_t s0 = nl.net_start[j], s1 = nl.net_start[j + 1];
+        if (s1 - s0 < 2) continue;
+        const uint32_t first = root(nl.pin_cell[s0]);
+        for (uint64_t k = s0 + 1; k < s1; ++k) {
+            const uint32_t r = root(nl.pin_cell[k]), f = root(first);
+            if (r != f) parent[std::max(r, f)] = std::min(r, f);
+        }
+    }
+    std::vector<uint8_t> anchoredRoot(V, 0);
+    for (uint32_t v = 0; v < V; ++v)
+        if (nl.var_of_vertex[v] == kNoVar) anchoredRoot[root(v)] = 1;
+    std::vector<uint8_t> floating(n);
+    uint32_t floatingCells = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        floating[i] = anchoredRoot[root(nl.mov_vertex[i])] ? 0 : 1;
+        floatingCells += floating[i];
+    }
     std::vector<uint32_t> netStart32, pinStart32;
     KT_TRY(narrow(nl.net_start, N + 1ull, netStart32, "net_start"));
     KT_TRY(narrow(nl.pin_start, n + 1ull, pinStart32, "pin_start"));
@@ -172,6 +201,7 @@ int uploadNetlist(kt_ctx *ctx, const kt_netlist &nl) {
     KT_TRY(upload(ctx, ctx->refNet, nl.cell_pin_net, Pm));
     KT_TRY(upload(ctx, ctx->refPin, nl.cell_pin_idx, Pm));
     KT_TRY(upload(ctx, ctx->netMov, netMov.data(), N));
+    KT_TRY(upload(ctx, ctx->floating, floating.data(), n));
     if (nl.width) KT_TRY(upload(ctx, ctx->width, nl.width, n));
     if (nl.height) KT_TRY(upload(ctx, ctx->height, nl.height, n));
     if (nl.area) KT_TRY(upload(ctx, ctx->area, nl.area, n));
@@ -187,6 +217,7 @@ int uploadNetlist(kt_ctx *ctx, const kt_netlist &nl) {
     ctx->P = P;
     ctx->Pm = Pm;
     ctx->nnzBound = bound;
+    ctx->floatingCells = floatingCells;
     ctx->haveNetlist = true;
     ctx->haveAnchors = false;
     ctx->posCount = 0;
@@ -335,6 +366,7 @@ int kt_ctx_destroy(kt_ctx *ctx) {
         for (hipEvent_t ev : t.pending) (void)hipEventDestroy(ev);
     ctx->netStart.release(); ctx->pinCell.release(); ctx->varOfVertex.release(); ctx->movVertex.release();
     ctx->pinStart.release(); ctx->refNet.release(); ctx->refPin.release(); ctx->netMov.release();
+    ctx->floating.release();
     ctx->offX.release(); ctx->offY.release(); ctx->weight.release(); ctx->fixedX.release();
     ctx->fixedY.release(); ctx->width.release(); ctx->height.release(); ctx->area.release();
     ctx->posX.release(); ctx->posY.release(); ctx->saveX.release(); ctx->saveY.release();
@@ -375,6 +407,7 @@ int kt_ctx_device_bytes(kt_ctx *ctx, uint64_t *bytes) {
     uint64_t b = 0;
     b += ctx->netStart.bytes() + ctx->pinCell.bytes() + ctx->varOfVertex.bytes() + ctx->movVertex.bytes();
     b += ctx->pinStart.bytes() + ctx->refNet.bytes() + ctx->refPin.bytes() + ctx->netMov.bytes();
+    b += ctx->floating.bytes();
     b += ctx->offX.bytes() + ctx->offY.bytes() + ctx->weight.bytes() + ctx->fixedX.bytes() + ctx->fixedY.bytes();
     b += ctx->width.bytes() + ctx->height.bytes() + ctx->area.bytes();
     b += ctx->posX.bytes() + ctx->posY.bytes() + ctx->saveX.bytes() + ctx->saveY.bytes();
@@ -688,6 +721,18 @@ int kt_lal_batches(kt_ctx *ctx, uint64_t *batches) {
     KT_GUARD(ctx);
     KT_CHECK(batches, KT_EINVAL, "kt_lal_batches: null output");
     *batches = ctx->lalBatches;
+    return KT_OK;
+}
+
+int kt_floating_cells(kt_ctx *ctx, uint32_t *count, uint8_t *flags) {
+    KT_GUARD(ctx);
+    KT_CHECK(count, KT_EINVAL, "kt_floating_cells: null count");
+    KT_CHECK(ctx->haveNetlist, KT_ESTATE, "kt_floating_cells: upload a netlist first");
+    *count = ctx->floatingCells;
+    if (flags && ctx->n) {
+        KT_HIP(hipMemcpyAsync(flags, ctx->floating.p, ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+        KT_HIP(hipStreamSynchronize(ctx->stream));
+    }
     return KT_OK;
 }
 

@@ -217,6 +217,11 @@ struct kt_ctx {
     uint64_t P = 0, Pm = 0, nnzBound = 0;
     kt::DevBuf<uint32_t> netStart, pinCell, varOfVertex, movVertex, pinStart, refNet, refPin, netMov;
     kt::DevBuf<double> offX, offY, weight, fixedX, fixedY, width, height, area;
+    // per movable cell: 1 if no net path joins it to a fixed vertex (its
+    // component's translation is in the null space of the systems without
+    // anchors, so no start may move it)
+    kt::DevBuf<uint8_t> floating;
+    uint32_t floatingCells = 0;
 
     // positions (rows of the current systems) and anchors
     uint64_t posCount = 0;

@@ -237,14 +237,19 @@ __global__ void __launch_bounds__(kB) warmFeatures(FeatArgs a) {
         for (int k = 0; k < KT_MLP_FEATURES; ++k) a.feat32[static_cast<uint64_t>(i) * KT_MLP_FEATURES + k] = f[k];
 }
 
-// delta = s * prediction
+// delta = s * prediction; zero for a cell with no net path to a fixed vertex
+// when the systems have no anchors (keepFloating == 0): moving such a cell's
+// component is in the null space of A, so no solve would ever undo it and the
+// solution would leave the reference's (the previous iterate's) there.
 __global__ void __launch_bounds__(kB) warmDelta(const float2 *__restrict__ pred, const WarmScalars *__restrict__ sc,
-                                                uint32_t n, double *__restrict__ dx, double *__restrict__ dy) {
+                                                const uint8_t *__restrict__ floating, int keepFloating, uint32_t n,
+                                                double *__restrict__ dx, double *__restrict__ dy) {
     const uint32_t i = blockIdx.x * kB + threadIdx.x;
     if (i >= n) return;
     const float2 p = pred[i];
-    dx[i] = sc->sx * static_cast<double>(p.x);
-    dy[i] = sc->sy * static_cast<double>(p.y);
+    const bool off = !keepFloating && floating[i];
+    dx[i] = off ? 0.0 : sc->sx * static_cast<double>(p.x);
+    dy[i] = off ? 0.0 : sc->sy * static_cast<double>(p.y);
 }
 
 // gamma per axis from the per-block partials (fixed order: thread t sums
@@ -363,8 +368,9 @@ int warmPredictRun(kt_ctx *ctx) {
     KT_TRY(mlpInferLaunch(ctx->stream, ctx->warmW.p, ctx->warmB.p, ctx->warmFeat.p, n, ctx->warmOut.p));
     if (ctx->profile) KT_TRY(timedEnd(ctx, KT_KERNEL_MLP, t0));
     warmDelta<<<gridFor(n), kB, 0, ctx->stream>>>(reinterpret_cast<const float2 *>(ctx->warmOut.p),
-                                                  reinterpret_cast<const WarmScalars *>(ctx->warmScalars.p), n,
-                                                  ctx->p[0].p, ctx->p[1].p);
+                                                  reinterpret_cast<const WarmScalars *>(ctx->warmScalars.p),
+                                                  ctx->floating.p, ctx->lastAlpha > 0.0 ? 1 : 0, n, ctx->p[0].p,
+                                                  ctx->p[1].p);
     KT_HIP(hipGetLastError());
     return KT_OK;
 }

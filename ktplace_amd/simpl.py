@@ -109,6 +109,7 @@ _SIGS = {
     "kt_lal_resident": (I32, [P]),
     "kt_lal_batches": (I32, [P, C.POINTER(C.c_uint64)]),
     "kt_lal_reruns": (I32, [P, C.POINTER(C.c_uint64)]),
+    "kt_floating_cells": (I32, [P, C.POINTER(C.c_uint32), P]),
     "kt_set_warm_start": (I32, [P, P, U64]),
     "kt_warm_features": (I32, [P, P, P]),
     "kt_warm_predict": (I32, [P, P, P]),
@@ -279,6 +280,13 @@ class Context:
         b = C.c_uint64()
         _check(lib.kt_lal_batches(self.h, C.byref(b)))
         return b.value
+
+    def floating_cells(self):
+        """(count, flags[n]) of the movable cells with no net path to a fixed vertex."""
+        c = C.c_uint32()
+        f = np.zeros(self.n, np.uint8)
+        _check(lib.kt_floating_cells(self.h, C.byref(c), _lib.ptr(f)))
+        return c.value, f
 
     def lal_reruns(self) -> int:
         b = C.c_uint64()

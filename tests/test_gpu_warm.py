@@ -159,3 +159,46 @@ def test_trained_network_cuts_iterations_on_unseen_netlist(gpu_ctx_factory):
         assert ab["learned"]["iterations"] < ab["previous"]["iterations"]
         assert min(ab["learned"]["gamma"]) > 0
     ctx.close()
+
+
+def floating_cells_scipy(d):
+    """Movable cells whose connected component (vertices joined by nets) holds
+    no fixed vertex, from scipy's connected components of the pin graph."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+    f = d.flat
+    V, N = f.num_vertices, f.num_nets
+    net_of_pin = np.repeat(np.arange(N), np.diff(f.net_start.astype(np.int64)))
+    g = sp.csr_matrix((np.ones(len(f.pin_cell)), (f.pin_cell.astype(np.int64), net_of_pin)), shape=(V, N))
+    _, lab = connected_components(sp.bmat([[None, g], [g.T, None]]).tocsr(), directed=False)
+    fixed = f.var_of_vertex == 0xFFFFFFFF
+    anchored = np.zeros(lab.max() + 1, bool)
+    anchored[lab[:V][fixed]] = True
+    return ~anchored[lab[:V][f.mov_vertex.astype(np.int64)]]
+
+
+def test_learned_start_leaves_floating_components(gpu_ctx_factory):
+    """Cells with no net path to a pad or macro form components whose
+    translation is in the null space of A when there are no anchors: the
+    learned start must not move them (no solve would bring them back, and the
+    solution would then differ from the reference's there by the move)."""
+    from ktplace_amd.learned import DEFAULT_PARAMS, load_params
+    if not DEFAULT_PARAMS.exists():
+        pytest.skip("no trained parameters")
+    d = dz.generate(150_000, 165_000, 300, seed=777)
+    want = floating_cells_scipy(d)
+    ctx = gpu_ctx_factory(d)
+    count, flags = ctx.floating_cells()
+    assert count == int(want.sum()) > 0 and np.array_equal(flags.astype(bool), want)
+    ctx.build_grid(0, 1.0)
+    ctx.set_warm_start(load_params())
+    x, y = dz.seed_uniform(d, 20240607)
+    ctx.build_b2b(x, y)
+    ctx.solve_both(x, y, 1e-6, 100000)
+    ctx.build_b2b(x, y)
+    sx, sy = x.copy(), y.copy()
+    r = ctx.solve_both(sx, sy, 1e-6, 0, warm_start=simpl.KT_WARM_LEARNED)  # the start only
+    assert min(r["warm_gamma"]) > 0
+    assert np.array_equal(sx[want], x[want]) and np.array_equal(sy[want], y[want])
+    assert np.mean(sx[~want] != x[~want]) > 0.9
+    ctx.close()
