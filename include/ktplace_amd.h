@@ -169,6 +169,13 @@ int kt_grid_avail(kt_ctx *ctx, double *avail, double *total_avail);
 int kt_occupancy(kt_ctx *ctx, double *occ);
 int kt_overflow(kt_ctx *ctx, double *overflow);
 int kt_hpwl(kt_ctx *ctx, double *hpwl);
+/* kt_hpwl with no allocation: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same result as kt_hpwl,
+ * bit for bit.  Its chunk sums live in a fixed per-GPU device array of 128
+ * KiB, so it takes at most 67,104,768 nets (KT_EINVAL beyond); every call on
+ * one GPU shares that array, so calls that overlap on one GPU race. */
+int kt_hpwl_noalloc(kt_ctx *ctx, double *hpwl);
 
 /* ---- LookAheadLegalizer::legalize (simpl_lal.cc:43-124) ------------------ */
 /* max_level / min_cells_to_split / stripe_area_fraction: simpl_options.h:24-26

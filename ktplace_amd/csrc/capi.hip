@@ -353,6 +353,15 @@ int kt_ctx_create(kt_ctx **out, int device) {
             delete ctx;
             return hipFail(e, "kt_ctx_create", __FILE__, __LINE__);
         }
+        // this device's copy of kt_hpwl_noalloc's fixed scratch
+        const int rc = noAllocInit(device);
+        if (rc != KT_OK) {
+            (void)hipStreamDestroy(ctx->stream);
+            (void)hipHostFree(ctx->ctlHost);
+            (void)hipHostFree(ctx->pinned);
+            delete ctx;
+            return rc;
+        }
         *out = ctx;
         return KT_OK;
     })

@@ -307,4 +307,5 @@ int warmFeaturesHost(kt_ctx *ctx, float *features, double *scale);
 int warmPredictHost(kt_ctx *ctx, double *dx, double *dy);
 int warmStart(kt_ctx *ctx);
 int buildStar(kt_ctx *ctx, uint64_t *size);
+int noAllocInit(int device);
 }  // namespace kt

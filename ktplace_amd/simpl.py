@@ -104,6 +104,7 @@ _SIGS = {
     "kt_occupancy": (I32, [P, P]),
     "kt_overflow": (I32, [P, C.POINTER(C.c_double)]),
     "kt_hpwl": (I32, [P, C.POINTER(C.c_double)]),
+    "kt_hpwl_noalloc": (I32, [P, C.POINTER(C.c_double)]),
     "kt_set_lal_options": (I32, [P, U64, U64, F64]),
     "kt_lal": (I32, [P, P, P]),
     "kt_lal_resident": (I32, [P]),
@@ -260,6 +261,12 @@ class Context:
     def hpwl(self) -> float:
         v = C.c_double()
         _check(lib.kt_hpwl(self.h, C.byref(v)))
+        return v.value
+
+    def hpwl_noalloc(self) -> float:
+        """kt_hpwl_noalloc: the same HPWL with no device allocation."""
+        v = C.c_double()
+        _check(lib.kt_hpwl_noalloc(self.h, C.byref(v)))
         return v.value
 
     def set_lal_options(self, max_level: int = 0, min_cells_to_split: int = 0,
