@@ -177,6 +177,37 @@ int kt_hpwl(kt_ctx *ctx, double *hpwl);
  * one GPU shares that array, so calls that overlap on one GPU race. */
 int kt_hpwl_noalloc(kt_ctx *ctx, double *hpwl);
 
+/* ---- the outer loop's bookkeeping on the device -------------------------- */
+/* getHpwl, getOccupancy / getOverflow (simpl_placer.cc:256-297) and
+ * Convergence::update (:450-461) of simplPlacer::Impl::getGlobalPlacement
+ * (:463-539), with the lower bound in the context's positions and the upper
+ * bound in its anchors (the next B2B build's anchor set), so no placement
+ * crosses PCIe during the loop. */
+typedef struct kt_loop_state {
+    double lower_hpwl;      /* HPWL of the positions (lower bound) */
+    double upper_hpwl;      /* HPWL of the anchors (upper bound) */
+    double gap;             /* upper - lower */
+    double overflow_upper;  /* getOverflow of the anchors */
+    double overflow_lower;  /* getOverflow of the positions (measure-only calls) */
+    double gap_reference;   /* gap of iteration reference_iter */
+    double best_upper;
+    uint64_t iteration;
+    uint64_t stalled_for;
+    int32_t converged;      /* Convergence::update said stop */
+    int32_t reserved;
+} kt_loop_state;
+/* Convergence options (simpl_options.h: gapReferenceIter 10, gapTightFrac 0.05,
+ * gapRelaxedFrac 0.25, stallIters 6); clears the convergence state. */
+int kt_loop_reset(kt_ctx *ctx, uint64_t reference_iter, double tight_frac, double relaxed_frac,
+                  uint64_t stall_iters);
+/* anchors := positions on the device (the upper bound after legalization). */
+int kt_anchors_from_positions(kt_ctx *ctx);
+int kt_get_anchors(kt_ctx *ctx, double *x, double *y);
+/* HPWL of anchors and positions, overflow of the anchors and, for iteration
+ * >= 1, Convergence::update for that iteration; iteration 0 measures only
+ * (and adds the overflow of the positions).  One small device-to-host copy. */
+int kt_loop_update(kt_ctx *ctx, uint64_t iteration, kt_loop_state *out);
+
 /* ---- LookAheadLegalizer::legalize (simpl_lal.cc:43-124) ------------------ */
 /* max_level / min_cells_to_split / stripe_area_fraction: simpl_options.h:24-26
  * (20, 4, 0.1 by default; 0 keeps the current value). */
@@ -201,7 +232,9 @@ int kt_lal_reruns(kt_ctx *ctx, uint64_t *reruns);
 int kt_set_tuning(kt_ctx *ctx, int lanes, uint32_t tile, int use_graphs, double refresh_factor);
 /* As kt_set_tuning, plus xcd_split: 1 puts the x system on XCDs 0-3 and the y
  * system on XCDs 4-7 (each XCD's L2 then holds one axis' vectors), 0 deals
- * blocks of both axes to all XCDs, -1 keeps the default (1); rows_in_flight:
+ * blocks of both axes to all XCDs, 2 gives each XCD one contiguous share of
+ * the entry blocks (the entry-block SpMV; the vector kernels deal blocks as
+ * with 0), -1 keeps the default (1); rows_in_flight:
  * CSR rows each lane group keeps in flight in the SpMV (2, 4 or 8; 0 = 2). */
 int kt_set_tuning2(kt_ctx *ctx, int lanes, uint32_t tile, int use_graphs, double refresh_factor,
                    int xcd_split, int rows_in_flight);

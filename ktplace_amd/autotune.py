@@ -24,6 +24,7 @@ LANES = (4, 8, 16)
 TILES = (1024, 2048, 4096)
 ROWS = (2, 4, 8)
 XCD = (1, 0)
+FLAT_XCD = (1, 0, 2)  # entry-block SpMV: 2 = one contiguous share of the blocks per XCD
 HALOS = (2048, 4096, 6144, 8192, 12288, 16384)
 HALO_TILES = (1024, 2048, 4096)
 # entry-block SpMV: (threads per block, entries per block) shapes
@@ -115,7 +116,7 @@ def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iteratio
     # entry-block SpMV (lanes, rows in flight and halo do not apply; the tile
     # and XCD split shape the vector kernels)
     for tile in FLAT_TILES:
-        for x in XCD:
+        for x in FLAT_XCD:
             if small(tile):
                 for threads, entries in FLAT_SHAPES:
                     for row_lanes in FLAT_ROW_LANES:

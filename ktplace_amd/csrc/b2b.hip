@@ -314,15 +314,17 @@ int scanRowPtr(kt_ctx *ctx, uint32_t *rowPtr, uint64_t rows) {
 }
 
 // px/py: fixed positions with the movable cells' positions written over
-int vertexPositionsLaunch(kt_ctx *ctx) {
+int vertexPositionsOf(kt_ctx *ctx, const double *x, const double *y) {
     KT_TRY(ctx->vx.reserve(std::max<uint32_t>(ctx->V, 1)));
     KT_TRY(ctx->vy.reserve(std::max<uint32_t>(ctx->V, 1)));
     if (ctx->V == 0) return KT_OK;
     vertexPositions<<<dim3((ctx->V + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream>>>(
-        ctx->posX.p, ctx->posY.p, ctx->varOfVertex.p, ctx->fixedX.p, ctx->fixedY.p, ctx->vx.p, ctx->vy.p, ctx->V);
+        x, y, ctx->varOfVertex.p, ctx->fixedX.p, ctx->fixedY.p, ctx->vx.p, ctx->vy.p, ctx->V);
     KT_HIP(hipGetLastError());
     return KT_OK;
 }
+
+int vertexPositionsLaunch(kt_ctx *ctx) { return vertexPositionsOf(ctx, ctx->posX.p, ctx->posY.p); }
 
 int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY) {
     KT_CHECK(ctx->haveNetlist, KT_ESTATE, "buildB2B: no netlist uploaded");

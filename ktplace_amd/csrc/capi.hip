@@ -32,6 +32,9 @@ int lalBuildGrid(kt_ctx *ctx, uint64_t perAxis, double densityLimit);
 int lalOccupancy(kt_ctx *ctx, std::vector<double> *occHost, std::vector<uint32_t> *countHost = nullptr);
 int lalOverflow(kt_ctx *ctx, double *out);
 int lalHpwl(kt_ctx *ctx, double *out);
+int loopReset(kt_ctx *ctx, uint64_t referenceIter, double tight, double relaxed, uint64_t stall);
+int anchorsFromPositions(kt_ctx *ctx);
+int loopUpdate(kt_ctx *ctx, uint64_t iteration, kt_loop_state *out);
 int lalLegalize(kt_ctx *ctx);
 
 int ensureSystem(kt_ctx *ctx, int axis, uint64_t rows, uint64_t nnzCap) {
@@ -375,7 +378,7 @@ int kt_ctx_destroy(kt_ctx *ctx) {
         for (hipEvent_t ev : t.pending) (void)hipEventDestroy(ev);
     ctx->netStart.release(); ctx->pinCell.release(); ctx->varOfVertex.release(); ctx->movVertex.release();
     ctx->pinStart.release(); ctx->refNet.release(); ctx->refPin.release(); ctx->netMov.release();
-    ctx->floating.release();
+    ctx->floating.release(); ctx->hpwlScratch.release(); ctx->loopDev.release();
     ctx->offX.release(); ctx->offY.release(); ctx->weight.release(); ctx->fixedX.release();
     ctx->fixedY.release(); ctx->width.release(); ctx->height.release(); ctx->area.release();
     ctx->posX.release(); ctx->posY.release(); ctx->saveX.release(); ctx->saveY.release();
@@ -397,6 +400,7 @@ int kt_ctx_destroy(kt_ctx *ctx) {
     }
     if (ctx->ctlHost) (void)hipHostFree(ctx->ctlHost);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->loopHost) (void)hipHostFree(ctx->loopHost);
     if (ctx->lalPinned) (void)hipHostFree(ctx->lalPinned);
     if (ctx->lalLand) (void)hipHostFree(ctx->lalLand);
     (void)hipStreamDestroy(ctx->stream);
@@ -416,7 +420,7 @@ int kt_ctx_device_bytes(kt_ctx *ctx, uint64_t *bytes) {
     uint64_t b = 0;
     b += ctx->netStart.bytes() + ctx->pinCell.bytes() + ctx->varOfVertex.bytes() + ctx->movVertex.bytes();
     b += ctx->pinStart.bytes() + ctx->refNet.bytes() + ctx->refPin.bytes() + ctx->netMov.bytes();
-    b += ctx->floating.bytes();
+    b += ctx->floating.bytes() + ctx->hpwlScratch.bytes() + ctx->loopDev.bytes();
     b += ctx->offX.bytes() + ctx->offY.bytes() + ctx->weight.bytes() + ctx->fixedX.bytes() + ctx->fixedY.bytes();
     b += ctx->width.bytes() + ctx->height.bytes() + ctx->area.bytes();
     b += ctx->posX.bytes() + ctx->posY.bytes() + ctx->saveX.bytes() + ctx->saveY.bytes();
@@ -733,6 +737,35 @@ int kt_lal_batches(kt_ctx *ctx, uint64_t *batches) {
     return KT_OK;
 }
 
+int kt_loop_reset(kt_ctx *ctx, uint64_t reference_iter, double tight_frac, double relaxed_frac,
+                  uint64_t stall_iters) {
+    KT_GUARD(ctx);
+    KT_CATCH(return loopReset(ctx, reference_iter, tight_frac, relaxed_frac, stall_iters);)
+}
+
+int kt_anchors_from_positions(kt_ctx *ctx) {
+    KT_GUARD(ctx);
+    KT_CATCH(return anchorsFromPositions(ctx);)
+}
+
+int kt_get_anchors(kt_ctx *ctx, double *x, double *y) {
+    KT_GUARD(ctx);
+    KT_CHECK(x && y, KT_EINVAL, "kt_get_anchors: null output");
+    KT_CHECK(ctx->haveAnchors, KT_ESTATE, "kt_get_anchors: no anchors");
+    if (ctx->n) {
+        KT_HIP(hipMemcpyAsync(x, ctx->anchorX.p, ctx->n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KT_HIP(hipMemcpyAsync(y, ctx->anchorY.p, ctx->n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    KT_HIP(hipStreamSynchronize(ctx->stream));
+    return KT_OK;
+}
+
+int kt_loop_update(kt_ctx *ctx, uint64_t iteration, kt_loop_state *out) {
+    KT_GUARD(ctx);
+    KT_CHECK(out, KT_EINVAL, "kt_loop_update: null output");
+    KT_CATCH(return loopUpdate(ctx, iteration, out);)
+}
+
 int kt_floating_cells(kt_ctx *ctx, uint32_t *count, uint8_t *flags) {
     KT_GUARD(ctx);
     KT_CHECK(count, KT_EINVAL, "kt_floating_cells: null count");
@@ -821,7 +854,8 @@ int kt_set_tuning2(kt_ctx *ctx, int lanes, uint32_t tile, int use_graphs, double
     const char *noGraphs = std::getenv("KTPLACE_NO_GRAPHS");
     ctx->tune.useGraphs = use_graphs != 0 && !(noGraphs && noGraphs[0] == '1');
     ctx->tune.refreshFactor = refresh_factor;
-    ctx->tune.xcdSplit = xcd_split < 0 ? -1 : (xcd_split ? 1 : 0);
+    KT_CHECK(xcd_split <= 2, KT_EINVAL, "kt_set_tuning: xcd_split must be -1, 0, 1 or 2");
+    ctx->tune.xcdSplit = xcd_split < 0 ? -1 : xcd_split;
     return KT_OK;
 }
 

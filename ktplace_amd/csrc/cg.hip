@@ -50,7 +50,7 @@ constexpr uint64_t kMaxWindowBytes = 156ull * 1024;
 // the GPU once the other axis has converged (all kernels of the iteration
 // read the same flags).  Returns false for a block with no tile.
 __device__ __forceinline__ bool decode(const CgArgs &a, int &axis, uint32_t &tile, int parity = -1) {
-    if (a.xcdSplit) {
+    if (a.xcdSplit == 1) {
         if (parity >= 0) {
             const bool lx = a.ctl[0].s[parity].live, ly = a.ctl[1].s[parity].live;
             if (lx != ly) {
@@ -358,8 +358,34 @@ constexpr uint32_t flatSlack(uint32_t entries) { return entries / 8 < 512 ? entr
 // XCDs 0-3 (workgroups are dealt to the 8 XCDs round-robin) take the x
 // system and those on XCDs 4-7 the y system, so each XCD's L2 sees the
 // gathers of one axis only.
+// Position of block `id` (of g blocks, id < g) when each XCD takes one
+// contiguous share of [0, g): workgroups are dealt to the 8 XCDs round-robin
+// (id % 8), so XCD k's i-th block gets index k * (g / 8) + min(k, g % 8) + i.
+// A share of consecutive entry blocks gathers from one stretch of the vector
+// (the netlist's local nets), which then stays in that XCD's L2 instead of
+// being fetched by all eight.
+__device__ __forceinline__ uint32_t xcdShare(uint32_t id, uint32_t g) {
+    const uint32_t k = id & 7u, i = id >> 3, q = g >> 3, rem = g & 7u;
+    return k * q + (k < rem ? k : rem) + i;
+}
+
 __device__ __forceinline__ bool decodeFlat(const CgArgs &a, int &axis, uint32_t &b, int parity) {
-    if (a.xcdSplit) {
+    if (a.xcdSplit == 2) {
+        const uint32_t nx = a.sys[0].sblocks, ny = a.sys[1].sblocks;
+        const bool lx = parity < 0 || a.ctl[0].s[parity].live, ly = parity < 0 || a.ctl[1].s[parity].live;
+        if (lx != ly) {  // one axis live: its blocks spread over every XCD
+            const uint32_t g = lx ? nx : ny;
+            if (blockIdx.x >= g) return false;
+            axis = lx ? 0 : 1;
+            b = xcdShare(blockIdx.x, g);
+            return true;
+        }
+        const uint32_t l = xcdShare(blockIdx.x, nx + ny);  // the grid is nx + ny blocks
+        axis = l >= nx ? 1 : 0;
+        b = l - (axis ? nx : 0);
+        return true;
+    }
+    if (a.xcdSplit == 1) {
         const bool lx = parity >= 0 && a.ctl[0].s[parity].live, ly = parity >= 0 && a.ctl[1].s[parity].live;
         if (lx != ly) {  // inside an iteration, one axis live: every block serves it (as decode)
             axis = lx ? 0 : 1;
@@ -380,7 +406,7 @@ __device__ __forceinline__ bool decodeFlat(const CgArgs &a, int &axis, uint32_t 
 int flatThreads(int tuned) { return tuned == 256 || tuned == 512 ? tuned : kBlock; }
 
 uint32_t flatGrid(const CgArgs &a) {
-    if (a.xcdSplit) return 8u * ((std::max(a.sys[0].sblocks, a.sys[1].sblocks) + 3u) / 4u);
+    if (a.xcdSplit == 1) return 8u * ((std::max(a.sys[0].sblocks, a.sys[1].sblocks) + 3u) / 4u);
     return a.sys[0].sblocks + a.sys[1].sblocks;
 }
 
@@ -872,7 +898,7 @@ multiplyKernel(SysView s, const double *__restrict__ in, double *__restrict__ ou
 }
 
 uint32_t gridBlocks(const CgArgs &a) {
-    if (a.xcdSplit) return 8u * ((std::max(a.sys[0].blocks, a.sys[1].blocks) + 3u) / 4u);
+    if (a.xcdSplit == 1) return 8u * ((std::max(a.sys[0].blocks, a.sys[1].blocks) + 3u) / 4u);
     return a.sys[0].blocks + a.sys[1].blocks;
 }
 
@@ -1169,7 +1195,7 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
     KT_CHECK(tile % kBlock == 0, KT_EINVAL, "tile must be a multiple of 1024 rows");
     a = CgArgs{};
     a.tile = tile;
-    a.xcdSplit = ctx->tune.xcdSplit == 0 ? 0u : 1u;
+    a.xcdSplit = ctx->tune.xcdSplit < 0 ? 1u : static_cast<uint32_t>(ctx->tune.xcdSplit);
     a.halo = ctx->tune.halo;
     a.layout = ctx->tune.layout == KT_SPMV_ENTRY_BLOCKS ? 1u : 0u;
     a.flatEntries = ctx->tune.flatEntries ? ctx->tune.flatEntries : 8u * static_cast<uint32_t>(flatThreads(ctx->tune.flatThreads));

@@ -257,6 +257,10 @@ struct kt_ctx {
     kt::LalOptions lalOpts;
     uint64_t lalBatches = 0;
     kt::DevBuf<double> gridAvail, occ, lalScratch, lalSpan, lalCap, lalCut;
+    kt::DevBuf<double> hpwlScratch;  // per-net half-perimeters and chunk sums of getHpwl
+    // device-resident bookkeeping of the outer loop (HPWL, overflow, Convergence)
+    kt::DevBuf<double> loopDev;
+    double *loopHost = nullptr;      // pinned mirror of loopDev
     kt::DevBuf<uint64_t> occFix, lalKeys, lalKeysTmp, lalKey2, lalPrefix, lalStripeLo, lalStripeHi;
     kt::DevBuf<uint32_t> binOf, binRegion, lalSeg, lalFlag, lalIds, lalVals, lalValsTmp, lalHalf, lalStripeOf,
         lalBegin, lalEnd, lalStripeStart, lalStripeEnd, lalRemap, sortCounts, sortRaw;

@@ -908,9 +908,9 @@ GridDev gridDev(const lal::Grid &g) {
 
 }  // namespace
 
-// binOf and occupancy of the context's positions (device), occupancy copied
-// to host `occ` when non-null.
-int lalOccupancy(kt_ctx *ctx, std::vector<double> *occHost, std::vector<uint32_t> *countHost) {
+// binOf and occupancy of positions (x, y) on the device (enqueued, no
+// synchronisation).
+int lalOccupancyOf(kt_ctx *ctx, const double *x, const double *y) {
     KT_CHECK(ctx->grid.size() > 0 && ctx->gridAvail.cap >= ctx->grid.size(), KT_ESTATE, "occupancy: no grid");
     const uint32_t n = ctx->n;
     const uint64_t m = ctx->grid.size();
@@ -923,15 +923,37 @@ int lalOccupancy(kt_ctx *ctx, std::vector<double> *occHost, std::vector<uint32_t
     if (n) {
         // a few cells per thread: each block's bins reach memory once
         const dim3 grid(std::min<uint32_t>(gridFor(n).x, 2048u));
-        cellBins<<<grid, kB, 0, ctx->stream>>>(ctx->posX.p, ctx->posY.p, ctx->width.p, ctx->height.p,
-                                                     ctx->area.p, gridDev(ctx->grid), n, ctx->binOf.p,
-                                                     reinterpret_cast<unsigned long long *>(ctx->occFix.p),
-                                                     ctx->binCells.p);
+        cellBins<<<grid, kB, 0, ctx->stream>>>(x, y, ctx->width.p, ctx->height.p, ctx->area.p, gridDev(ctx->grid), n,
+                                                ctx->binOf.p, reinterpret_cast<unsigned long long *>(ctx->occFix.p),
+                                                ctx->binCells.p);
         KT_HIP(hipGetLastError());
     }
     fixToDouble<<<gridFor(m), kB, 0, ctx->stream>>>(reinterpret_cast<unsigned long long *>(ctx->occFix.p),
                                                      ctx->occ.p, static_cast<uint32_t>(m));
     KT_HIP(hipGetLastError());
+    return KT_OK;
+}
+
+// Overflow excess sum (occ - g avail)+ of positions (x, y) into *dst on the
+// device (enqueued; getOverflow divides it by the total free area).
+int lalExcessInto(kt_ctx *ctx, const double *x, const double *y, double *dst) {
+    KT_TRY(lalOccupancyOf(ctx, x, y));
+    const uint64_t m = ctx->grid.size();
+    const uint32_t blocks = static_cast<uint32_t>((m + kB - 1) / kB);
+    KT_TRY(ctx->lalScratch.reserve(blocks + 1));
+    overflowPartials<<<blocks, kB, 0, ctx->stream>>>(ctx->occ.p, ctx->gridAvail.p, ctx->densityLimit,
+                                                     static_cast<uint32_t>(m), ctx->lalScratch.p);
+    KT_HIP(hipGetLastError());
+    sumInOrder<<<1, 64, 0, ctx->stream>>>(ctx->lalScratch.p, blocks, dst);
+    KT_HIP(hipGetLastError());
+    return KT_OK;
+}
+
+// binOf and occupancy of the context's positions (device), occupancy copied
+// to host `occ` when non-null.
+int lalOccupancy(kt_ctx *ctx, std::vector<double> *occHost, std::vector<uint32_t> *countHost) {
+    KT_TRY(lalOccupancyOf(ctx, ctx->posX.p, ctx->posY.p));
+    const uint64_t m = ctx->grid.size();
     if (occHost) {
         occHost->resize(m);
         KT_HIP(hipMemcpyAsync(occHost->data(), ctx->occ.p, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -946,15 +968,10 @@ int lalOccupancy(kt_ctx *ctx, std::vector<double> *occHost, std::vector<uint32_t
 }
 
 int lalOverflow(kt_ctx *ctx, double *out) {
-    KT_TRY(lalOccupancy(ctx, nullptr, nullptr));
     const uint64_t m = ctx->grid.size();
     const uint32_t blocks = static_cast<uint32_t>((m + kB - 1) / kB);
     KT_TRY(ctx->lalScratch.reserve(blocks + 1));
-    overflowPartials<<<blocks, kB, 0, ctx->stream>>>(ctx->occ.p, ctx->gridAvail.p, ctx->densityLimit,
-                                                     static_cast<uint32_t>(m), ctx->lalScratch.p);
-    KT_HIP(hipGetLastError());
-    sumInOrder<<<1, 64, 0, ctx->stream>>>(ctx->lalScratch.p, blocks, ctx->lalScratch.p + blocks);
-    KT_HIP(hipGetLastError());
+    KT_TRY(lalExcessInto(ctx, ctx->posX.p, ctx->posY.p, ctx->lalScratch.p + blocks));
     double excess = 0.0;
     KT_HIP(hipMemcpyAsync(&excess, ctx->lalScratch.p + blocks, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     KT_HIP(hipStreamSynchronize(ctx->stream));
@@ -963,15 +980,16 @@ int lalOverflow(kt_ctx *ctx, double *out) {
 }
 
 int vertexPositionsLaunch(kt_ctx *ctx);
+int vertexPositionsOf(kt_ctx *ctx, const double *x, const double *y);
 
-int lalHpwl(kt_ctx *ctx, double *out) {
+// getHpwl of positions (x, y) into *dst on the device (enqueued).
+int lalHpwlInto(kt_ctx *ctx, const double *x, const double *y, double *dst) {
     KT_CHECK(ctx->haveNetlist, KT_ESTATE, "hpwl: no netlist");
-    KT_CHECK(ctx->posCount == ctx->n, KT_ESTATE, "hpwl: positions must have one entry per movable cell");
-    KT_TRY(vertexPositionsLaunch(ctx));
+    KT_TRY(vertexPositionsOf(ctx, x, y));
     const uint32_t N = ctx->N;
     const uint32_t chunks = (N + 4095) / 4096;
-    KT_TRY(ctx->lalScratch.reserve(static_cast<uint64_t>(N) + chunks + 1));
-    double *perNet = ctx->lalScratch.p;
+    KT_TRY(ctx->hpwlScratch.reserve(static_cast<uint64_t>(N) + chunks + 1));
+    double *perNet = ctx->hpwlScratch.p;
     double *part = perNet + N;
     if (N) {
         netHpwl<<<gridFor(N), kB, 0, ctx->stream>>>(ctx->netStart.p, ctx->pinCell.p, ctx->offX.p, ctx->offY.p,
@@ -980,10 +998,17 @@ int lalHpwl(kt_ctx *ctx, double *out) {
         chunkSums<<<gridFor(chunks), kB, 0, ctx->stream>>>(perNet, N, part);
         KT_HIP(hipGetLastError());
     }
-    sumInOrder<<<1, 64, 0, ctx->stream>>>(part, chunks, part + chunks);
+    sumInOrder<<<1, 64, 0, ctx->stream>>>(part, chunks, dst);
     KT_HIP(hipGetLastError());
+    return KT_OK;
+}
+
+int lalHpwl(kt_ctx *ctx, double *out) {
+    KT_CHECK(ctx->posCount == ctx->n, KT_ESTATE, "hpwl: positions must have one entry per movable cell");
+    KT_TRY(ctx->lalScratch.reserve(1));
+    KT_TRY(lalHpwlInto(ctx, ctx->posX.p, ctx->posY.p, ctx->lalScratch.p));
     double h = 0.0;
-    KT_HIP(hipMemcpyAsync(&h, part + chunks, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KT_HIP(hipMemcpyAsync(&h, ctx->lalScratch.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     KT_HIP(hipStreamSynchronize(ctx->stream));
     *out = h;
     return KT_OK;

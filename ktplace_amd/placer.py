@@ -96,16 +96,29 @@ class Placer:
         self.ctx.set_positions(x, y)
         return self.ctx.hpwl()
 
-    def overflow(self, x, y) -> float:
-        self.ctx.set_positions(x, y)
-        return self.ctx.overflow()
-
     def _solve(self, x, y):
         r = self.ctx.solve_both(x, y, self.o.cgTol, self.o.cgMaxIter, precision=self.o.precision,
                                 stop_rule=(simpl.KT_STOP_TRUE if self.o.precision == simpl.KT_PREC_MIXED
                                            else simpl.KT_STOP_RECURRENCE))
         self.stats["cg_iterations"] += r["iterations"]
         return r
+
+    def _solve_resident(self):
+        r = self.ctx.solve_resident(self.o.cgTol, self.o.cgMaxIter, precision=self.o.precision,
+                                    stop_rule=(simpl.KT_STOP_TRUE if self.o.precision == simpl.KT_PREC_MIXED
+                                               else simpl.KT_STOP_RECURRENCE))
+        self.stats["cg_iterations"] += r["iterations"]
+        return r
+
+    def _legalize_into_anchors(self):
+        """upper = LAL(lower) into the anchors; the positions keep the lower bound."""
+        ctx = self.ctx
+        ctx.save_positions()
+        t0 = time.perf_counter()
+        ctx.lal_resident()
+        self.stats["spread_seconds"] += time.perf_counter() - t0
+        ctx.anchors_from_positions()
+        ctx.restore_positions()
 
     def _initial_rounds(self, lx, ly, hpwl_init, star: bool):
         """initialRounds (simpl_placer.cc:406-448).  Star: the star systems are
@@ -152,51 +165,55 @@ class Placer:
         if o.initialNetModel != "star":
             lx, ly, hpwl_init = self._initial_rounds(lx, ly, hpwl_init, star=False)
         self.stats["hpwl_init"] = hpwl_init
+        # getGlobalPlacement (simpl_placer.cc:463-539) and refine (:541-557),
+        # resident on the device: the lower bound stays in the context's
+        # positions, the upper bound in its anchors; one small state record
+        # comes back per outer iteration (HPWL, overflow, Convergence).
         trace = []
-        convergence = Convergence(o)
-        ux = uy = None
+        ctx.set_positions(lx, ly)
+        ctx.loop_reset(o.gapReferenceIter, o.gapTightFrac, o.gapRelaxedFrac, o.stallIters)
+        have_upper = False
         for it in range(o.maxIters):
             self.stats["global_iters"] = it + 1
-            t0 = time.perf_counter()
-            ux, uy = ctx.lal(lx, ly)
-            self.stats["spread_seconds"] += time.perf_counter() - t0
-            upper = self.hpwl(ux, uy)
-            lower = self.hpwl(lx, ly)
-            gap = upper - lower
-            trace.append((lower, upper, self.overflow(ux, uy)))
+            self._legalize_into_anchors()
+            have_upper = True
+            st = ctx.loop_update(it + 1)
+            trace.append((st["lower_hpwl"], st["upper_hpwl"], st["overflow_upper"]))
             if self.progress:
-                self.progress(f"iter {it + 1}: lower {lower:.6e} upper {upper:.6e} "
+                self.progress(f"iter {it + 1}: lower {st['lower_hpwl']:.6e} upper {st['upper_hpwl']:.6e} "
                               f"lal {self.stats['spread_seconds']:.2f} s total, "
                               f"{ctx.lal_batches()} batches")
-            if convergence.update(it + 1, gap, upper):
+            if st["converged"]:
                 break
             alpha = o.alphaBase * (1.0 + it)
             t0 = time.perf_counter()
-            ctx.build_b2b(lx, ly, alpha, ux, uy)
+            ctx.build_b2b_resident(alpha)
             t1 = time.perf_counter()
-            self._solve(lx, ly)
+            self._solve_resident()
             self.stats["build_seconds"] += t1 - t0
             self.stats["solve_seconds"] += time.perf_counter() - t1
-        if ux is not None and o.refineIters > 0:
+        if have_upper and o.refineIters > 0:
             done = self.stats["global_iters"]
             ctx.build_grid(ctx.bins * o.refineGridFactor, o.densityLimit)
             for r in range(o.refineIters):
                 alpha = o.alphaBase * (1.0 + done + r)
-                ctx.build_b2b(lx, ly, alpha, ux, uy)
-                self._solve(lx, ly)
+                ctx.build_b2b_resident(alpha)
+                self._solve_resident()
                 t0 = time.perf_counter()
-                ux, uy = ctx.lal(lx, ly)
-                self.stats["spread_seconds"] += time.perf_counter() - t0
+                self._legalize_into_anchors()
                 if self.progress:
                     self.progress(f"refine {r + 1}: lal {time.perf_counter() - t0:.2f} s, "
                                   f"{ctx.lal_batches()} batches")
-        if ux is None:
-            ux, uy = lx.copy(), ly.copy()
-        self.stats["hpwl_final"] = self.hpwl(ux, uy)
-        self.stats["hpwl_lower"] = self.hpwl(lx, ly)
-        self.stats["gap"] = self.stats["hpwl_final"] - self.stats["hpwl_lower"]
-        self.stats["overflow_lower"] = self.overflow(lx, ly)
-        self.stats["overflow_final"] = self.overflow(ux, uy)
+        if not have_upper:
+            ctx.anchors_from_positions()  # upper = lower
+        st = ctx.loop_update(0)
+        self.stats["hpwl_final"] = st["upper_hpwl"]
+        self.stats["hpwl_lower"] = st["lower_hpwl"]
+        self.stats["gap"] = st["upper_hpwl"] - st["lower_hpwl"]
+        self.stats["overflow_lower"] = st["overflow_lower"]
+        self.stats["overflow_final"] = st["overflow_upper"]
+        ux, uy = ctx.get_anchors()
+        lx, ly = ctx.get_positions()
         return PlaceResult(ux, uy, self.stats, trace, lx, ly)
 
 

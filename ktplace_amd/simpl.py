@@ -56,6 +56,16 @@ class SolveResult(C.Structure):
                     warm_gamma=(self.warm_gamma[0], self.warm_gamma[1]))
 
 
+class LoopState(C.Structure):
+    _fields_ = [("lower_hpwl", C.c_double), ("upper_hpwl", C.c_double), ("gap", C.c_double),
+                ("overflow_upper", C.c_double), ("overflow_lower", C.c_double), ("gap_reference", C.c_double),
+                ("best_upper", C.c_double), ("iteration", C.c_uint64), ("stalled_for", C.c_uint64),
+                ("converged", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
 class SystemView(C.Structure):
     _fields_ = [("rows", C.c_uint64), ("row_ptr", C.c_void_p), ("col", C.c_void_p), ("val", C.c_void_p),
                 ("diag", C.c_void_p), ("rhs", C.c_void_p)]
@@ -111,6 +121,10 @@ _SIGS = {
     "kt_lal_batches": (I32, [P, C.POINTER(C.c_uint64)]),
     "kt_lal_reruns": (I32, [P, C.POINTER(C.c_uint64)]),
     "kt_floating_cells": (I32, [P, C.POINTER(C.c_uint32), P]),
+    "kt_loop_reset": (I32, [P, C.c_uint64, C.c_double, C.c_double, C.c_uint64]),
+    "kt_anchors_from_positions": (I32, [P]),
+    "kt_get_anchors": (I32, [P, P, P]),
+    "kt_loop_update": (I32, [P, C.c_uint64, P]),
     "kt_set_warm_start": (I32, [P, P, U64]),
     "kt_warm_features": (I32, [P, P, P]),
     "kt_warm_predict": (I32, [P, P, P]),
@@ -287,6 +301,25 @@ class Context:
         b = C.c_uint64()
         _check(lib.kt_lal_batches(self.h, C.byref(b)))
         return b.value
+
+    # the outer loop's bookkeeping on the device ---------------------------------
+    def loop_reset(self, reference_iter: int = 10, tight_frac: float = 0.05, relaxed_frac: float = 0.25,
+                   stall_iters: int = 6):
+        _check(lib.kt_loop_reset(self.h, reference_iter, tight_frac, relaxed_frac, stall_iters))
+
+    def anchors_from_positions(self):
+        _check(lib.kt_anchors_from_positions(self.h))
+
+    def get_anchors(self):
+        x = np.zeros(self.n)
+        y = np.zeros(self.n)
+        _check(lib.kt_get_anchors(self.h, _lib.ptr(x), _lib.ptr(y)))
+        return x, y
+
+    def loop_update(self, iteration: int) -> dict:
+        st = LoopState()
+        _check(lib.kt_loop_update(self.h, iteration, C.byref(st)))
+        return st.as_dict()
 
     def floating_cells(self):
         """(count, flags[n]) of the movable cells with no net path to a fixed vertex."""

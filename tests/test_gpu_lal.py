@@ -210,3 +210,29 @@ def test_lal_matches_oracle_on_a_large_die(gpu_ctx_factory):
         print(f"spread {spread}: identical {np.mean((gx == rx) & (gy == ry)):.6f}", flush=True)
         assert np.array_equal(gx, rx) and np.array_equal(gy, ry)
     ctx.close()
+
+
+def test_device_loop_bookkeeping_matches_host_rules(gpu_ctx_factory):
+    """The outer loop runs resident on the device: its trace's HPWLs are the
+    oracle's getHpwl of the placements, its stop iteration is the one the
+    host mirror of Convergence::update (pinned against the oracle on the CPU)
+    gives for the same (gap, upper) sequence, and the final statistics are
+    the oracle's HPWL and overflow of the returned bounds."""
+    from ktplace_amd.placer import Convergence, Placer, SimplOptions
+    d = dz.workload("cells12k")
+    o = ol.Oracle.of(d)
+    opts = SimplOptions(initialNetModel="b2b", maxIters=100, refineIters=2)
+    ctx = gpu_ctx_factory()
+    res = Placer(d, opts, ctx).run()
+    c = Convergence(opts)
+    stop = next((i + 1 for i, (lo, up, _) in enumerate(res.trace) if c.update(i + 1, up - lo, up)), None)
+    print("global iterations", res.stats["global_iters"], "host-rule stop", stop, flush=True)
+    assert stop == res.stats["global_iters"] or (stop is None and res.stats["global_iters"] == opts.maxIters)
+    assert res.stats["hpwl_final"] == o.hpwl(res.x, res.y)
+    assert res.stats["hpwl_lower"] == o.hpwl(res.lower_x, res.lower_y)
+    bins = o.build_grid(0, 1.0)
+    o.build_grid(bins * opts.refineGridFactor, 1.0)  # the bookkeeping of the final grid
+    assert res.stats["overflow_final"] == pytest.approx(o.overflow(o.occupancy(res.x, res.y)), rel=1e-12, abs=1e-15)
+    assert res.stats["overflow_lower"] == pytest.approx(
+        o.overflow(o.occupancy(res.lower_x, res.lower_y)), rel=1e-12, abs=1e-15)
+    ctx.close()
