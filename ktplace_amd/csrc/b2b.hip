@@ -13,9 +13,10 @@
 //   pinStage         per pin: position (vertex + offset), rhs term, variable
 //                    (then one record per net: pin range, weight base, lo/hi)
 //   netExtremes      first argmin / first argmax of pos + offset per net  (:35-45)
-//   rowCounts        movable neighbours per row                              (:75-82)
-//   scan             rowPtr = inclusive scan of the counts                  (:83)
-//   rowFill          col, val = -w, diag, rhs, anchor term                  (:89-113)
+//   refCounts        per pinsOfCell reference: edges and CSR entries     (:47-82)
+//   scans            each reference's first edge and CSR slot; rowPtr     (:83)
+//   edgeFill         w and rhs term per edge, col / val = -w              (:89-104)
+//   rowSums          diag and rhs per row in the row walk's order, anchor (:105-113)
 #include "kt_internal.h"
 
 namespace kt {
@@ -37,30 +38,33 @@ vertexPositions(const double *__restrict__ x, const double *__restrict__ y,
     vy[v] = var != kNoVar ? y[var] : fixedY[v];
 }
 
-// One thread per pin, both axes.  The pin's position (pinAt: vertex position
-// + offset, simpl_lss.cc:32-34), its variable (kNoVar for a fixed vertex) and
-// the term an edge to it adds to rhs before the owner's offset is taken off:
-// position - vertex position for a movable vertex, the position itself for a
-// fixed one (simpl_lss.cc:94-104).  These are the same double expressions the
-// per-entry code would evaluate, once per pin instead of once per CSR entry
-// that names the pin, so the entry loops below read them in net order
-// instead of gathering vertex data at random.
-// Stored per axis as one (position, rhs term) pair per pin: the row kernels
-// read both at once, at random for a net's lo/hi pins and a row's own pin.
+// One thread per pin, both axes.  The pin's position (vertex position +
+// offset, simpl_lss.cc:32-34) and the term an edge to it adds to rhs before
+// the owner's offset is taken off: position - vertex position for a movable
+// vertex, the position itself for a fixed one (simpl_lss.cc:94-104).  These
+// are the same double expressions the per-edge code would evaluate, once per
+// pin instead of once per edge that names the pin.  Both axes share one
+// 32-byte record, and the pin's variable and its movable rank in the net one
+// 8-byte pair, so an edge reads one line of each for both axes.
+struct alignas(32) PinRec {
+    double posX, termX, posY, termY;
+};
+__device__ __forceinline__ double pinPos(const PinRec &r, int axis) { return axis ? r.posY : r.posX; }
+__device__ __forceinline__ double pinTerm(const PinRec &r, int axis) { return axis ? r.termY : r.termX; }
+
 __global__ void __launch_bounds__(kBlock)
 pinStage(const uint32_t *__restrict__ pinCell, const uint32_t *__restrict__ varOfVertex,
          const double *__restrict__ offX, const double *__restrict__ offY, const double *__restrict__ vx,
-         const double *__restrict__ vy, double2 *__restrict__ pinX, double2 *__restrict__ pinY,
-         uint32_t *__restrict__ pinVar, uint64_t P) {
+         const double *__restrict__ vy, const uint32_t *__restrict__ movPrefix, PinRec *__restrict__ pin,
+         uint2 *__restrict__ varMov, uint64_t P) {
     const uint64_t k = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (k >= P) return;
     const uint32_t c = pinCell[k];
     const uint32_t var = varOfVertex[c];
     const double cx = vx[c], cy = vy[c];
     const double px = cx + offX[k], py = cy + offY[k];
-    pinX[k] = make_double2(px, var != kNoVar ? px - cx : px);
-    pinY[k] = make_double2(py, var != kNoVar ? py - cy : py);
-    pinVar[k] = var;
+    pin[k] = PinRec{px, var != kNoVar ? px - cx : px, py, var != kNoVar ? py - cy : py};
+    varMov[k] = make_uint2(var, movPrefix[k]);
 }
 
 // Everything a row's reference to a net reads about the net, in one 64-byte
@@ -81,19 +85,19 @@ static_assert(sizeof(NetRec) == 64, "one sector per net");
 // recomputed there from the same operands, so the comparisons are identical.
 __global__ void __launch_bounds__(kBlock)
 netExtremes(const uint32_t *__restrict__ netStart, const uint32_t *__restrict__ netMov,
-            const double *__restrict__ weight, const double2 *__restrict__ pinX, const double2 *__restrict__ pinY,
-            const uint32_t *__restrict__ pinVar, NetRec *__restrict__ rec, uint32_t N) {
+            const double *__restrict__ weight, const PinRec *__restrict__ pin, const uint2 *__restrict__ varMov,
+            NetRec *__restrict__ rec, uint32_t N) {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= N) return;
     const uint32_t b = netStart[j];
     const uint32_t p = netStart[j + 1] - b;
     uint32_t lox = 0, hix = 0, loy = 0, hiy = 0;
     if (p > 0) {
-        double bestLoX = pinX[b].x, bestHiX = bestLoX;
-        double bestLoY = pinY[b].x, bestHiY = bestLoY;
+        double bestLoX = pin[b].posX, bestHiX = bestLoX;
+        double bestLoY = pin[b].posY, bestHiY = bestLoY;
         for (uint32_t q = 1; q < p; ++q) {
-            const double px = pinX[b + q].x;
-            const double py = pinY[b + q].x;
+            const double px = pin[b + q].posX;
+            const double py = pin[b + q].posY;
             if (px < bestLoX) { lox = q; bestLoX = px; }
             if (px > bestHiX) { hix = q; bestHiX = px; }
             if (py < bestLoY) { loy = q; bestLoY = py; }
@@ -103,21 +107,41 @@ netExtremes(const uint32_t *__restrict__ netStart, const uint32_t *__restrict__ 
     uint4 *dst = reinterpret_cast<uint4 *>(rec + j);
     dst[0] = make_uint4(b, p, netMov[j], 0u);
     dst[1] = make_uint4(lox, hix, loy, hiy);
-    dst[2] = p > 0 ? make_uint4(pinVar[b + lox], pinVar[b + hix], pinVar[b + loy], pinVar[b + hiy])
+    dst[2] = p > 0 ? make_uint4(varMov[b + lox].x, varMov[b + hix].x, varMov[b + loy].x, varMov[b + hiy].x)
                    : make_uint4(kNoVar, kNoVar, kNoVar, kNoVar);
     // used only where lo != hi, i.e. p >= 2
     rec[j].base = weight[j] / static_cast<double>(p - 1);
 }
 
-struct RowArgs {
+// The assembly is organised around the pinsOfCell references (e: row i's
+// reference to pin q of net j, in the reference's row walk order) instead of
+// one thread walking each row:
+//   refCounts  per reference and axis, the edges the row walk emits for it
+//              (bound pin: every other pin; inner pin: lo and hi; a net with
+//              lo == hi: none, simpl_lss.cc:47-73) and how many of them reach
+//              movable vertices (CSR entries); packed x | y << 32
+//   scans      exclusive scans of both counts over the references: each
+//              reference's first edge slot and first CSR slot, so rowPtr[i]
+//              is the CSR scan at the row's first reference
+//   edgeFill   each edge's weight w and rhs term w (there - own offset) into
+//              an edge list in row-walk order, and col / val = -w of the
+//              movable ones straight into their CSR slots, one lane per edge
+//   rowSums    diag and rhs of each row as sequential sums over its edge
+//              list, in the row walk's order, then the anchor term
+// Every double is the reference's expression and every sum keeps its order,
+// so rowPtr, col, val, diag and rhs are the reference's bit for bit.
+struct RefArgs {
     const uint32_t *pinStart;
     const uint32_t *refNet;
     const uint32_t *refPin;
-    const uint32_t *pinVar;
+    const uint2 *varMov;        // per pin: variable, movable pins before it in its net
     const NetRec *rec;
-    const double2 *pin[2];  // per pin: position, rhs term (pinStage)
+    const PinRec *pin;          // per pin: position, rhs term per axis (pinStage)
     const double *pos[2];
     const double *anchor[2];
+    unsigned long long *edgeOff;  // [Pm + 1] edges: counts at e + 1, then the scan (x | y << 32)
+    unsigned long long *entOff;   // [Pm + 1] CSR entries, likewise
+    double2 *edges[2];          // per axis: (w, rhs term) per edge in row-walk order
     uint32_t *rowPtr[2];
     uint32_t *col[2];
     double *val[2];
@@ -126,6 +150,7 @@ struct RowArgs {
     double floor[2];
     double alpha;
     uint32_t n;
+    uint32_t Pm;
 };
 
 __device__ __forceinline__ uint32_t extLo(const uint4 &e, int axis) { return axis ? e.z : e.x; }
@@ -139,81 +164,176 @@ __device__ __forceinline__ uint4 recExt(const NetRec *rec, uint32_t net) {
 __device__ __forceinline__ uint4 recVars(const NetRec *rec, uint32_t net) {
     return reinterpret_cast<const uint4 *>(rec + net)[2];  // variables of lo_x, hi_x, lo_y, hi_y
 }
+__device__ __forceinline__ uint32_t half(unsigned long long v, int axis) {
+    return static_cast<uint32_t>(axis ? v >> 32 : v);
+}
 
 // std::max(a, b) is (a < b) ? b : a; written out so NaN behaves the same.
 __device__ __forceinline__ double refMax(double a, double b) { return a < b ? b : a; }
 
-__global__ void __launch_bounds__(kBlock) rowCounts(RowArgs a) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const int axis = blockIdx.y;
-    if (i >= a.n) return;
-    uint32_t count = 0;
-    for (uint32_t e = a.pinStart[i]; e < a.pinStart[i + 1]; ++e) {
-        const uint32_t net = a.refNet[e];
-        const uint32_t q = a.refPin[e];
-        const uint4 head = recHead(a.rec, net), x = recExt(a.rec, net);
+__global__ void __launch_bounds__(kBlock) refCounts(RefArgs a) {
+    const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+    if (e == 0) {
+        a.edgeOff[0] = 0ull;
+        a.entOff[0] = 0ull;
+    }
+    if (e >= a.Pm) return;
+    const uint32_t net = a.refNet[e];
+    const uint32_t q = a.refPin[e];
+    const uint4 head = recHead(a.rec, net), x = recExt(a.rec, net), v = recVars(a.rec, net);
+    uint32_t edges[2], ents[2];
+#pragma unroll
+    for (int axis = 0; axis < 2; ++axis) {
         const uint32_t lo = extLo(x, axis), hi = extHi(x, axis);
-        if (lo == hi) continue;
-        if (q == lo || q == hi) {
-            count += head.z - 1;  // every other movable pin; q itself is movable
+        if (lo == hi) {
+            edges[axis] = ents[axis] = 0;
+        } else if (q == lo || q == hi) {
+            edges[axis] = head.y - 1;  // every other pin
+            ents[axis] = head.z - 1;   // every other movable pin; q itself is movable
         } else {
-            const uint4 v = recVars(a.rec, net);
-            count += (extLo(v, axis) != kNoVar ? 1u : 0u) + (extHi(v, axis) != kNoVar ? 1u : 0u);
+            edges[axis] = 2;
+            ents[axis] = (extLo(v, axis) != kNoVar ? 1u : 0u) + (extHi(v, axis) != kNoVar ? 1u : 0u);
         }
     }
-    a.rowPtr[axis][i + 1] = count;
-    if (i == 0) a.rowPtr[axis][0] = 0;
+    a.edgeOff[e + 1] = static_cast<unsigned long long>(edges[0]) | (static_cast<unsigned long long>(edges[1]) << 32);
+    a.entOff[e + 1] = static_cast<unsigned long long>(ents[0]) | (static_cast<unsigned long long>(ents[1]) << 32);
 }
 
-__global__ void __launch_bounds__(kBlock) rowFill(RowArgs a) {
+// rowPtr[i] = the CSR scan at row i's first reference (rows [0, n]).
+__global__ void __launch_bounds__(kBlock) refRowPtr(RefArgs a) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i > a.n) return;
+    const unsigned long long s = a.entOff[a.pinStart[i]];
+    a.rowPtr[0][i] = half(s, 0);
+    a.rowPtr[1][i] = half(s, 1);
+}
+
+// Edge-parallel fill: the edges of each axis (in row-walk order) are cut into
+// blocks of kFillEdges; lane k of a block takes edge start + k, finds its
+// reference (the last one whose first edge is at or before it) and writes
+// that edge.  Consecutive lanes write consecutive edge-list slots and, mostly,
+// consecutive CSR slots, and lanes on one reference share its net record and
+// read the net's pins side by side; no lane walks a long run alone.
+constexpr uint32_t kFillEdges = 2048;
+constexpr uint32_t kFillRefs = 4096;  // references a block keeps in LDS for the lookup
+
+// first[axis][b] = the last reference whose first edge is at or before edge
+// b * kFillEdges of that axis.
+__global__ void __launch_bounds__(kBlock) edgeBlockStart(RefArgs a, uint32_t blocks, uint32_t *__restrict__ first) {
+    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
     const int axis = blockIdx.y;
-    if (i >= a.n) return;
-    const double2 *__restrict__ pin = a.pin[axis];
-    const uint32_t *__restrict__ pinVar = a.pinVar;
-    const double lengthFloor = a.floor[axis];
-    uint32_t *__restrict__ col = a.col[axis];
-    double *__restrict__ val = a.val[axis];
-    uint32_t slot = a.rowPtr[axis][i];
-    double diag = 0.0, rhs = 0.0;
-    for (uint32_t e = a.pinStart[i]; e < a.pinStart[i + 1]; ++e) {
+    if (b >= blocks) return;
+    const uint32_t start = b * kFillEdges;
+    uint32_t lo = 0, hi = a.Pm + 1;  // first e in [0, Pm] with edgeOff[e] > start
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (half(a.edgeOff[mid], axis) <= start) lo = mid + 1;
+        else hi = mid;
+    }
+    first[axis * blocks + b] = lo - 1;  // edgeOff[0] = 0 <= start, so lo >= 1
+}
+
+__global__ void __launch_bounds__(kBlock) edgeFill(RefArgs a, uint32_t blocks, const uint32_t *__restrict__ first) {
+    __shared__ uint32_t refOff[kFillRefs];
+    __shared__ uint32_t refCount;
+    const int axis = blockIdx.y;
+    const uint32_t total = half(a.edgeOff[a.Pm], axis);
+    const uint32_t start = blockIdx.x * kFillEdges;
+    if (start >= total) return;  // block-uniform
+    const uint32_t end = min(start + kFillEdges, total);
+    const uint32_t e0 = first[axis * blocks + blockIdx.x];
+    // the block's references e0, e0 + 1, ... up to the one holding edge end - 1
+    if (threadIdx.x == 0) refCount = kFillRefs + 1;
+    __syncthreads();
+    for (uint32_t base = 0; base < kFillRefs; base += kBlock) {
+        const uint32_t r = base + threadIdx.x;
+        const uint32_t e = e0 + r;
+        if (r < kFillRefs && e <= a.Pm) {
+            const uint32_t o = half(a.edgeOff[e], axis);
+            refOff[r] = o;
+            if (o >= end) atomicMin(&refCount, r);  // references from here on start past the block
+        }
+        __syncthreads();
+        if (refCount <= base + kBlock) break;  // block-uniform
+    }
+    const uint32_t R = refCount;  // the block's references are e0 .. e0 + R - 1 when R <= kFillRefs
+    for (uint32_t k = start + threadIdx.x; k < end; k += kBlock) {
+        uint32_t e, t;
+        if (R <= kFillRefs) {  // largest r < R with refOff[r] <= k
+            uint32_t lo = 0, hi = R;
+            while (hi - lo > 1) {
+                const uint32_t mid = lo + (hi - lo) / 2;
+                if (refOff[mid] <= k) lo = mid;
+                else hi = mid;
+            }
+            e = e0 + lo;
+            t = k - refOff[lo];
+        } else {  // more zero-edge references than the table holds: search memory
+            uint32_t lo = e0, hi = a.Pm + 1;
+            while (lo < hi) {
+                const uint32_t mid = lo + (hi - lo) / 2;
+                if (half(a.edgeOff[mid], axis) <= k) lo = mid + 1;
+                else hi = mid;
+            }
+            e = lo - 1;
+            t = k - half(a.edgeOff[e], axis);
+        }
         const uint32_t net = a.refNet[e];
         const uint32_t q = a.refPin[e];
         const uint4 head = recHead(a.rec, net), x = recExt(a.rec, net);
         const uint32_t lo = extLo(x, axis), hi = extHi(x, axis);
-        if (lo == hi) continue;
         const uint32_t b = head.x;
-        const uint32_t p = head.y;
+        const bool bound = q == lo || q == hi;
         const double base = a.rec[net].base;  // weight / (p - 1)
-        // q's vertex is this movable row: .y = own - its vertex position
-        const double2 ownPin = pin[b + q];
-        const double own = ownPin.x;
-        const double ownOffset = ownPin.y;
-        auto edge = [&](uint32_t other, uint32_t var) {
-            const double2 there = pin[b + other];
-            const double w = base / refMax(fabs(own - there.x), lengthFloor);
-            diag += w;
-            if (var != kNoVar) {
-                col[slot] = var;
-                val[slot] = -w;
-                ++slot;
-            }
-            // movable: (there - cellPos) - ownOffset; fixed: there - ownOffset
-            rhs += w * (there.y - ownOffset);
-        };
-        if (q == lo || q == hi) {
-            for (uint32_t other = 0; other < p; ++other) {
-                if (other != q) edge(other, pinVar[b + other]);
-            }
-        } else {
-            const uint4 v = recVars(a.rec, net);
-            edge(lo, extLo(v, axis));
-            edge(hi, extHi(v, axis));
+        const PinRec ownPin = a.pin[b + q];
+        const double own = pinPos(ownPin, axis), ownOffset = pinTerm(ownPin, axis);
+        const uint32_t other = bound ? (t < q ? t : t + 1) : (t == 0 ? lo : hi);
+        const PinRec there = a.pin[b + other];
+        const double w = base / refMax(fabs(own - pinPos(there, axis)), a.floor[axis]);
+        // movable: (there - cellPos) - ownOffset; fixed: there - ownOffset
+        a.edges[axis][k] = make_double2(w, w * (pinTerm(there, axis) - ownOffset));
+        const uint2 vm = a.varMov[b + other];
+        if (vm.x != kNoVar) {
+            const uint32_t loMovable = a.varMov[b + lo].x != kNoVar ? 1u : 0u;
+            const uint32_t mi = bound ? vm.y - (other > q ? 1u : 0u) : (t == 0 ? 0u : loMovable);
+            const uint32_t slot = half(a.entOff[e], axis) + mi;
+            a.col[axis][slot] = vm.x;
+            a.val[axis][slot] = -w;
         }
+    }
+}
+
+// A block sums kSumRows consecutive rows.  Their edges are one contiguous run
+// of the edge list; when it fits kSumEdges, the block first copies it to LDS
+// with coalesced loads, then each thread adds its row's edges from there in
+// order.  A longer run (rows with bound pins of big nets) is summed straight
+// from memory.
+constexpr uint32_t kSumRows = 64, kSumEdges = 1536, kSumThreads = 128;
+
+__global__ void __launch_bounds__(kSumThreads) rowSums(RefArgs a) {
+    __shared__ double2 stage[kSumEdges];
+    const uint32_t r0 = blockIdx.x * kSumRows;
+    const uint32_t i = r0 + threadIdx.x;  // threads past kSumRows only help stage
+    const int axis = blockIdx.y;
+    const uint32_t r1 = min(r0 + kSumRows, a.n);
+    const uint32_t E0 = half(a.edgeOff[a.pinStart[r0]], axis), E1 = half(a.edgeOff[a.pinStart[r1]], axis);
+    const double2 *__restrict__ edges = a.edges[axis];
+    const bool staged = E1 - E0 <= kSumEdges;  // block-uniform
+    if (staged) {
+        for (uint32_t k = threadIdx.x; k < E1 - E0; k += kSumThreads) stage[k] = edges[E0 + k];
+        __syncthreads();
+    }
+    if (threadIdx.x >= kSumRows || i >= a.n) return;
+    const uint32_t e0 = half(a.edgeOff[a.pinStart[i]], axis), e1 = half(a.edgeOff[a.pinStart[i + 1]], axis);
+    double diag = 0.0, rhs = 0.0;
+    for (uint32_t k = e0; k < e1; ++k) {
+        const double2 d = staged ? stage[k - E0] : edges[k];
+        diag += d.x;
+        rhs += d.y;
     }
     if (a.alpha > 0.0) {
         const double anchor = a.anchor[axis][i];
-        const double w = a.alpha / refMax(fabs(a.pos[axis][i] - anchor), lengthFloor);
+        const double w = a.alpha / refMax(fabs(a.pos[axis][i] - anchor), a.floor[axis]);
         diag += w;
         rhs += w * anchor;
     }
@@ -335,36 +455,42 @@ int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY) {
     KT_TRY(ctx->ext.reserve(std::max<uint64_t>(16ull * ctx->N, 16)));  // one NetRec per net
     KT_TRY(vertexPositionsLaunch(ctx));
     const uint64_t pins = std::max<uint64_t>(ctx->P, 1);
-    for (int axis = 0; axis < 2; ++axis) KT_TRY(ctx->pinAt[axis].reserve(2 * pins));
-    KT_TRY(ctx->pinVar.reserve(pins));
-    double2 *pinX = reinterpret_cast<double2 *>(ctx->pinAt[0].p);
-    double2 *pinY = reinterpret_cast<double2 *>(ctx->pinAt[1].p);
+    KT_TRY(ctx->pinAt[0].reserve(4 * pins));  // one PinRec (4 doubles) per pin
+    KT_TRY(ctx->pinVar.reserve(2 * pins));    // (variable, movable rank) per pin
+    PinRec *pin = reinterpret_cast<PinRec *>(ctx->pinAt[0].p);
+    uint2 *varMov = reinterpret_cast<uint2 *>(ctx->pinVar.p);
     if (ctx->P > 0) {
         pinStage<<<dim3(static_cast<uint32_t>((ctx->P + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream>>>(
-            ctx->pinCell.p, ctx->varOfVertex.p, ctx->offX.p, ctx->offY.p, ctx->vx.p, ctx->vy.p, pinX, pinY,
-            ctx->pinVar.p, ctx->P);
+            ctx->pinCell.p, ctx->varOfVertex.p, ctx->offX.p, ctx->offY.p, ctx->vx.p, ctx->vy.p, ctx->movPrefix.p, pin,
+            varMov, ctx->P);
         KT_HIP(hipGetLastError());
     }
     NetRec *rec = reinterpret_cast<NetRec *>(ctx->ext.p);
     if (ctx->N > 0) {
         netExtremes<<<dim3((ctx->N + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream>>>(
-            ctx->netStart.p, ctx->netMov.p, ctx->weight.p, pinX, pinY, ctx->pinVar.p, rec, ctx->N);
+            ctx->netStart.p, ctx->netMov.p, ctx->weight.p, pin, varMov, rec, ctx->N);
         KT_HIP(hipGetLastError());
     }
-    RowArgs a{};
+    RefArgs a{};
     a.pinStart = ctx->pinStart.p;
     a.refNet = ctx->refNet.p;
     a.refPin = ctx->refPin.p;
-    a.pinVar = ctx->pinVar.p;
+    a.varMov = varMov;
     a.rec = rec;
-    a.pin[0] = pinX;
-    a.pin[1] = pinY;
+    a.pin = pin;
     a.pos[0] = ctx->posX.p;
     a.pos[1] = ctx->posY.p;
     a.anchor[0] = ctx->anchorX.p;
     a.anchor[1] = ctx->anchorY.p;
+    const uint64_t Pm = ctx->Pm;
+    KT_TRY(ctx->b2bEdgeOff.reserve(Pm + 1));
+    KT_TRY(ctx->b2bEntOff.reserve(Pm + 1));
+    for (int axis = 0; axis < 2; ++axis) KT_TRY(ctx->b2bEdges[axis].reserve(2 * std::max<uint64_t>(ctx->nnzBound, 1)));
+    a.edgeOff = reinterpret_cast<unsigned long long *>(ctx->b2bEdgeOff.p);
+    a.entOff = reinterpret_cast<unsigned long long *>(ctx->b2bEntOff.p);
     for (int axis = 0; axis < 2; ++axis) {
         Sys &s = ctx->sys[axis];
+        a.edges[axis] = reinterpret_cast<double2 *>(ctx->b2bEdges[axis].p);
         a.rowPtr[axis] = s.rowPtr.p;
         a.col[axis] = s.col.p;
         a.val[axis] = s.val.p;
@@ -379,18 +505,32 @@ int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY) {
     a.alpha = alpha;
     ctx->lastAlpha = alpha;
     a.n = n;
+    a.Pm = static_cast<uint32_t>(Pm);
     if (n == 0) {
         for (int axis = 0; axis < 2; ++axis)
             KT_HIP(hipMemsetAsync(ctx->sys[axis].rowPtr.p, 0, sizeof(uint32_t), ctx->stream));
         return KT_OK;
     }
-    const dim3 grid((n + kBlock - 1) / kBlock, 2);
-    rowCounts<<<grid, dim3(kBlock), 0, ctx->stream>>>(a);
-    KT_HIP(hipGetLastError());
-    for (int axis = 0; axis < 2; ++axis) KT_TRY(scanRowPtr(ctx, ctx->sys[axis].rowPtr.p, n));
     hipEvent_t t0 = nullptr;
     KT_TRY(timedBegin(ctx, KT_KERNEL_B2B_FILL, &t0));
-    rowFill<<<grid, dim3(kBlock), 0, ctx->stream>>>(a);
+    const uint32_t refBlocks = static_cast<uint32_t>(std::max<uint64_t>((Pm + kBlock - 1) / kBlock, 1));
+    refCounts<<<dim3(refBlocks), dim3(kBlock), 0, ctx->stream>>>(a);
+    KT_HIP(hipGetLastError());
+    KT_TRY(scanInclusive64(ctx, reinterpret_cast<uint64_t *>(a.edgeOff), Pm + 1));
+    KT_TRY(scanInclusive64(ctx, reinterpret_cast<uint64_t *>(a.entOff), Pm + 1));
+    refRowPtr<<<dim3((n + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream>>>(a);
+    KT_HIP(hipGetLastError());
+    if (Pm > 0) {
+        // edges per axis are at most nnzBound (every edge of every net)
+        const uint32_t blocks = static_cast<uint32_t>(std::max<uint64_t>((ctx->nnzBound + kFillEdges - 1) / kFillEdges, 1));
+        KT_TRY(ctx->b2bLong.reserve(2ull * blocks));
+        edgeBlockStart<<<dim3((blocks + kBlock - 1) / kBlock, 2), dim3(kBlock), 0, ctx->stream>>>(a, blocks,
+                                                                                                   ctx->b2bLong.p);
+        KT_HIP(hipGetLastError());
+        edgeFill<<<dim3(blocks, 2), dim3(kBlock), 0, ctx->stream>>>(a, blocks, ctx->b2bLong.p);
+        KT_HIP(hipGetLastError());
+    }
+    rowSums<<<dim3((n + kSumRows - 1) / kSumRows, 2), dim3(kSumThreads), 0, ctx->stream>>>(a);
     KT_HIP(hipGetLastError());
     KT_TRY(timedEnd(ctx, KT_KERNEL_B2B_FILL, t0));
     return KT_OK;

@@ -139,11 +139,14 @@ int uploadNetlist(kt_ctx *ctx, const kt_netlist &nl) {
                  "upload: mov_vertex / var_of_vertex mismatch");
     }
     std::vector<uint32_t> netMov(N, 0);
+    std::vector<uint32_t> movPrefix(P);  // per pin: movable pins before it in its net
     uint64_t bound = 0;
     for (uint32_t j = 0; j < N; ++j) {
         const uint64_t p = nl.net_start[j + 1] - nl.net_start[j];
-        for (uint64_t k = nl.net_start[j]; k < nl.net_start[j + 1]; ++k)
+        for (uint64_t k = nl.net_start[j]; k < nl.net_start[j + 1]; ++k) {
+            movPrefix[k] = netMov[j];
             netMov[j] += nl.var_of_vertex[nl.pin_cell[k]] != kNoVar ? 1u : 0u;
+        }
         if (p >= 2) bound += 4 * p - 6;  // 2 bound pins x (p-1) + (p-2) inner pins x 2
     }
     // every pinsOfCell entry of row i must be a pin of cell i: the B2B count and
@@ -204,6 +207,7 @@ int uploadNetlist(kt_ctx *ctx, const kt_netlist &nl) {
     KT_TRY(upload(ctx, ctx->refNet, nl.cell_pin_net, Pm));
     KT_TRY(upload(ctx, ctx->refPin, nl.cell_pin_idx, Pm));
     KT_TRY(upload(ctx, ctx->netMov, netMov.data(), N));
+    KT_TRY(upload(ctx, ctx->movPrefix, movPrefix.data(), P));
     KT_TRY(upload(ctx, ctx->floating, floating.data(), n));
     if (nl.width) KT_TRY(upload(ctx, ctx->width, nl.width, n));
     if (nl.height) KT_TRY(upload(ctx, ctx->height, nl.height, n));
@@ -378,7 +382,9 @@ int kt_ctx_destroy(kt_ctx *ctx) {
         for (hipEvent_t ev : t.pending) (void)hipEventDestroy(ev);
     ctx->netStart.release(); ctx->pinCell.release(); ctx->varOfVertex.release(); ctx->movVertex.release();
     ctx->pinStart.release(); ctx->refNet.release(); ctx->refPin.release(); ctx->netMov.release();
-    ctx->floating.release(); ctx->hpwlScratch.release(); ctx->loopDev.release();
+    ctx->floating.release(); ctx->hpwlScratch.release(); ctx->loopDev.release(); ctx->movPrefix.release();
+    ctx->b2bEdgeOff.release(); ctx->b2bEntOff.release(); ctx->b2bLong.release();
+    for (int a = 0; a < 2; ++a) ctx->b2bEdges[a].release();
     ctx->offX.release(); ctx->offY.release(); ctx->weight.release(); ctx->fixedX.release();
     ctx->fixedY.release(); ctx->width.release(); ctx->height.release(); ctx->area.release();
     ctx->posX.release(); ctx->posY.release(); ctx->saveX.release(); ctx->saveY.release();
@@ -420,7 +426,9 @@ int kt_ctx_device_bytes(kt_ctx *ctx, uint64_t *bytes) {
     uint64_t b = 0;
     b += ctx->netStart.bytes() + ctx->pinCell.bytes() + ctx->varOfVertex.bytes() + ctx->movVertex.bytes();
     b += ctx->pinStart.bytes() + ctx->refNet.bytes() + ctx->refPin.bytes() + ctx->netMov.bytes();
-    b += ctx->floating.bytes() + ctx->hpwlScratch.bytes() + ctx->loopDev.bytes();
+    b += ctx->floating.bytes() + ctx->hpwlScratch.bytes() + ctx->loopDev.bytes() + ctx->movPrefix.bytes();
+    b += ctx->b2bEdgeOff.bytes() + ctx->b2bEntOff.bytes() + ctx->b2bLong.bytes() + ctx->b2bEdges[0].bytes() +
+         ctx->b2bEdges[1].bytes();
     b += ctx->offX.bytes() + ctx->offY.bytes() + ctx->weight.bytes() + ctx->fixedX.bytes() + ctx->fixedY.bytes();
     b += ctx->width.bytes() + ctx->height.bytes() + ctx->area.bytes();
     b += ctx->posX.bytes() + ctx->posY.bytes() + ctx->saveX.bytes() + ctx->saveY.bytes();

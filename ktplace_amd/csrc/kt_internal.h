@@ -231,6 +231,10 @@ struct kt_ctx {
     kt::DevBuf<uint32_t> ext;       // [N][16]: per-net record of the B2B build (pins, weight, lo/hi per axis)
     kt::DevBuf<double> pinAt[2];    // per pin and axis: (vertex position + offset, rhs term of an edge to it)
     kt::DevBuf<uint32_t> pinVar;    // per pin: its vertex's variable (kNoVar: fixed)
+    kt::DevBuf<uint32_t> movPrefix; // per pin: movable pins before it in its net (uploaded)
+    kt::DevBuf<uint64_t> b2bEdgeOff, b2bEntOff;  // B2B: per reference, first edge / CSR slot (x | y << 32)
+    kt::DevBuf<double> b2bEdges[2];              // B2B: (w, rhs term) per edge in row-walk order
+    kt::DevBuf<uint32_t> b2bLong;                // B2B: first reference of each edge block, per axis
     kt::DevBuf<uint32_t> scanTmp;   // block sums of the u32 scans
     kt::DevBuf<uint64_t> scanTmp64; // block sums of the u64 scans
 
