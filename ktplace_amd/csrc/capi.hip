@@ -8,9 +8,21 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "kt_internal.h"
 
 namespace kt {
+
+// A roctx range named after the reference's phase timers (ScopedTimer in
+// simpl_placer.cc:411-520: simpl-init, simpl-build, simpl-solve,
+// simpl-legalize), so a rocprofv3 --marker-trace timeline shows the phases.
+struct PhaseRange {
+    explicit PhaseRange(const char *name) { roctxRangePushA(name); }
+    ~PhaseRange() { roctxRangePop(); }
+    PhaseRange(const PhaseRange &) = delete;
+    PhaseRange &operator=(const PhaseRange &) = delete;
+};
 
 thread_local std::string lastError;
 
@@ -494,6 +506,7 @@ int kt_restore_positions(kt_ctx *ctx) {
 int kt_build_b2b(kt_ctx *ctx, const double *x, const double *y, double alpha, const double *anchor_x,
                  const double *anchor_y, double floor_x, double floor_y) {
     KT_GUARD(ctx);
+    const PhaseRange range("simpl-build");
     KT_CHECK(ctx->haveNetlist, KT_ESTATE, "kt_build_b2b: no netlist uploaded");
     KT_TRY(setPositions(ctx, x, y, ctx->n));
     if (alpha > 0.0) {
@@ -507,11 +520,13 @@ int kt_build_b2b(kt_ctx *ctx, const double *x, const double *y, double alpha, co
 
 int kt_build_star(kt_ctx *ctx, uint64_t *size) {
     KT_GUARD(ctx);
+    const PhaseRange range("simpl-init");
     KT_CATCH(return buildStar(ctx, size);)
 }
 
 int kt_build_b2b_resident(kt_ctx *ctx, double alpha, double floor_x, double floor_y) {
     KT_GUARD(ctx);
+    const PhaseRange range("simpl-build");
     return buildB2B(ctx, alpha, floor_x, floor_y);
 }
 
@@ -611,12 +626,14 @@ int kt_multiply(kt_ctx *ctx, int axis, const double *in, double *out) {
 
 int kt_solve_both_resident(kt_ctx *ctx, const kt_solve_opts *opts, kt_solve_result *out) {
     KT_GUARD(ctx);
+    const PhaseRange range("simpl-solve");
     KT_CATCH(return solve(ctx, opts, out);)
 }
 
 int kt_solve_both(kt_ctx *ctx, double *x, double *y, uint64_t count, const kt_solve_opts *opts,
                   kt_observer observe, void *user, kt_solve_result *out) {
     KT_GUARD(ctx);
+    const PhaseRange range("simpl-solve");
     KT_CATCH({
         KT_TRY(setPositions(ctx, x, y, count));
         kt_solve_result res{};
@@ -725,11 +742,13 @@ int kt_set_lal_options(kt_ctx *ctx, uint64_t max_level, uint64_t min_cells_to_sp
 
 int kt_lal_resident(kt_ctx *ctx) {
     KT_GUARD(ctx);
+    const PhaseRange range("simpl-legalize");
     KT_CATCH(return lalLegalize(ctx);)
 }
 
 int kt_lal(kt_ctx *ctx, double *x, double *y) {
     KT_GUARD(ctx);
+    const PhaseRange range("simpl-legalize");
     KT_CHECK(x && y, KT_EINVAL, "kt_lal: null positions");
     KT_CATCH({
         KT_TRY(setPositions(ctx, x, y, ctx->n));
