@@ -670,6 +670,15 @@ scaleCells(const uint32_t *__restrict__ ids, uint32_t A, const uint32_t *__restr
 constexpr uint32_t kSmallCells = 2048;
 constexpr int kSmallThreads = 1024;
 
+// Threads for a small batch sorted at `width` (a power of two): width / 2,
+// at least one wave.  Every loop of the small kernels strides by blockDim.x,
+// and the block scan covers 2 * blockDim.x >= width elements, so a batch of a
+// few cells runs in one wave, whose barriers cost next to nothing, instead of
+// sixteen.
+inline uint32_t smallThreads(uint32_t width) {
+    return std::min<uint32_t>(kSmallThreads, std::max<uint32_t>(64u, width / 2));
+}
+
 __device__ void bitonicTriples(uint32_t *ss, uint64_t *sk, uint32_t *si, uint32_t width) {
     for (uint32_t k = 2; k <= width; k <<= 1) {
         for (uint32_t d = k >> 1; d > 0; d >>= 1) {
@@ -1208,7 +1217,7 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
                 sm.stripeOf = ctx->lalStripeOf.p;
                 sm.B = B;
                 sm.A = A;
-                smallLevelA<<<1, kSmallThreads, 0, ctx->stream>>>(sm);
+                smallLevelA<<<1, smallThreads(width), 0, ctx->stream>>>(sm);
                 KT_HIP(hipGetLastError());
             } else {
                 // active cells in id order
@@ -1353,7 +1362,8 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
                           reinterpret_cast<unsigned long long *>(ctx->lalStripeLo.p),
                           reinterpret_cast<unsigned long long *>(ctx->lalStripeHi.p)};
             if (small) {
-                smallLevelB<<<1, kSmallThreads, 0, ctx->stream>>>(sm, ha, sa, dRemap, H, S, A2, const_cast<double *>(ca.pos));
+                smallLevelB<<<1, smallThreads(width), 0, ctx->stream>>>(sm, ha, sa, dRemap, H, S, A2,
+                                                                         const_cast<double *>(ca.pos));
                 KT_HIP(hipGetLastError());
             } else {
                 listFill<<<gridFor(M), kB, 0, ctx->stream>>>(list, M, ctx->lalHalf.p, kNoVar);
