@@ -51,14 +51,15 @@ def _run(exe, d, bins, queries) -> tuple:
             lines.append("2 " + float(q[1]).hex() + " " + " ".join(float(v).hex() for v in q[2]))
             continue
         nums = list(q[1]) + [int(q[2])] + [v for v in q[3:]]
-        lines.append(("1 " if q[0] == "S" else "0 ") +
+        kind = {"P": "0 ", "S": "1 ", "p": "3 ", "s": "4 "}[q[0]]
+        lines.append(kind +
                      " ".join(v.hex() if isinstance(v, float) else str(v) for v in map(_num, nums)))
     out = iter(subprocess.run([str(exe)], input="\n".join(lines) + "\n", capture_output=True, text=True,
                               check=True, timeout=120).stdout.split("\n"))
     avail = [int(t, 16) for t in next(out).split()[1:]]
     results = []
     for q in queries:
-        if q[0] == "S":
+        if q[0] in "Ss":
             results.append([int(t, 16) for t in next(out).split()[1:]])
         elif q[0] == "C":
             results.append([int(t) for t in next(out).split()[1:]])
@@ -172,3 +173,48 @@ def test_overfilled_clusters_match_label_and_oracle(harness, seed, bins):
         want = [v for sl in ndimage.find_objects(labels)
                 for v in (sl[1].start, sl[1].stop - 1, sl[0].start, sl[0].stop - 1)]
         assert got == want and count > 5
+
+
+@pytest.mark.parametrize("seed,bins,macros,g", [(6, 16, 20, 1.0), (7, 0, 0, 0.8), (8, 48, 40, 0.9)])
+def test_flat_geometry_of_the_batch_kernel_matches(harness, seed, bins, macros, g):
+    """The fused GPU batch kernel computes profiles and stripes with the
+    sequential steps of csrc/lal_seq.h on flat arrays (steps and cuts sorted
+    as values, the stripe subdivision as a depth-first walk, the fill order
+    by rank).  On the CPU those steps give the same profiles and stripes as
+    lal_geom.cc and the oracle, bit for bit, on random boxes, thin boxes and
+    boxes on row edges."""
+    d = dz.generate(3000, 3300, 60, seed=seed, num_macros=macros)
+    o = ol.Oracle.of(d)
+    per = o.build_grid(bins, g)
+    rng = np.random.default_rng(seed)
+    W, H = d.die[2], d.die[3]
+    rows = np.unique(np.asarray(d.raw.row_y, np.float64))
+    queries = []
+    for k in range(240):
+        x0, y0 = rng.uniform(0, W), rng.uniform(0, H)
+        if k % 4 == 3:  # on row edges
+            y0 = float(rows[rng.integers(len(rows))])
+        x1 = x0 + rng.uniform(0, W - x0) * (0.02 if k % 3 == 0 else 1.0)
+        y1 = min(H, y0 + rng.uniform(0, H - y0) * (0.05 if k % 3 == 1 else 1.0))
+        if k % 11 == 0:
+            x1 = x0  # zero width
+        box, vertical = (x0, y0, x1, y1), bool(k % 2)
+        lo, hi = (x0, x1) if vertical else (y0, y1)
+        cut = lo + rng.uniform(0, 1) * (hi - lo)
+        _, rp = o.profile(box, vertical)
+        room = float(rp[-1]) if len(rp) else 0.0
+        region = 0.0 if k % 5 == 0 else room * (1.0 + rng.uniform(0, 3))
+        frac = (0.1, 0.05, 0.3)[k % 3]
+        s = (box, vertical, rng.uniform(0, 1.2) * room, region, g, frac, cut, k % 4 < 2)
+        queries += [("P", box, vertical), ("p", box, vertical), ("S",) + s, ("s",) + s]
+    _, results = _run(harness, d, per, queries)
+    stripes_seen = 0
+    for k in range(0, len(queries), 4):
+        (qp, gp), (_, fp), (qs, gs), (_, fs) = zip(queries[k:k + 4], results[k:k + 4])
+        assert fp == gp, qp
+        assert fs == gs, qs
+        ref = o.stripes(qs[1], qs[2], qs[3], qs[4], qs[7], qs[8]) if qs[5] == g and qs[6] == 0.1 else None
+        if ref is not None:
+            assert gs == _bits(ref.reshape(-1)), qs
+        stripes_seen += len(fs) // 3
+    assert stripes_seen > 300
