@@ -223,6 +223,16 @@ int kt_lal_batches(kt_ctx *ctx, uint64_t *batches);
  * the same batch is legalized again there, as the reference does with one
  * thread, simpl_lal.cc:102-122). */
 int kt_lal_reruns(kt_ctx *ctx, uint64_t *reruns);
+/* Batches of up to max_cells cells (at most 1024) in up to 64 regions run in
+ * one launch each, every level with its block geometry (free-area profile,
+ * cutline, stripes) on the device; larger batches run level by level with
+ * the geometry on host threads.  Both give the same positions.  Default 0:
+ * every batch level by level, which is the faster of the two on the
+ * 210,000-cell loop (one launch costs more than a level's host round trip
+ * there).  kt_lal_fused_batches: batches of the last legalization that ran
+ * in one launch. */
+int kt_set_lal_fused(kt_ctx *ctx, uint32_t max_cells);
+int kt_lal_fused_batches(kt_ctx *ctx, uint64_t *batches);
 
 /* ---- launch-shape tuning (0 keeps the default) --------------------------- */
 /* lanes: threads per CSR row in the SpMV (1, 2, 4, 8 or 16); tile: rows per

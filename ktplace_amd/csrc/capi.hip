@@ -11,6 +11,7 @@
 #include <rocprofiler-sdk-roctx/roctx.h>
 
 #include "kt_internal.h"
+#include "lal_dev.h"
 
 namespace kt {
 
@@ -809,6 +810,20 @@ int kt_lal_reruns(kt_ctx *ctx, uint64_t *reruns) {
     KT_GUARD(ctx);
     KT_CHECK(reruns, KT_EINVAL, "kt_lal_reruns: null output");
     *reruns = ctx->lalReruns;
+    return KT_OK;
+}
+
+int kt_set_lal_fused(kt_ctx *ctx, uint32_t max_cells) {
+    KT_GUARD(ctx);
+    KT_CHECK(max_cells <= kLalFusedCells, KT_EINVAL, "kt_set_lal_fused: at most 1024 cells");
+    ctx->lalFusedCells = max_cells;
+    return KT_OK;
+}
+
+int kt_lal_fused_batches(kt_ctx *ctx, uint64_t *batches) {
+    KT_GUARD(ctx);
+    KT_CHECK(batches, KT_EINVAL, "kt_lal_fused_batches: null output");
+    *batches = ctx->lalFusedBatches;
     return KT_OK;
 }
 

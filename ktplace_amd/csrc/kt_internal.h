@@ -260,6 +260,8 @@ struct kt_ctx {
     double densityLimit = 1.0;
     kt::LalOptions lalOpts;
     uint64_t lalBatches = 0;
+    uint64_t lalFusedBatches = 0;                      // of those, run in one launch each
+    uint32_t lalFusedCells = 0;                        // batches of at most this many cells run fused (0: none)
     kt::DevBuf<double> gridAvail, occ, lalScratch, lalSpan, lalCap, lalCut;
     kt::DevBuf<double> hpwlScratch;  // per-net half-perimeters and chunk sums of getHpwl
     // device-resident bookkeeping of the outer loop (HPWL, overflow, Convergence)
@@ -282,6 +284,8 @@ struct kt_ctx {
     kt::DevBuf<double> lalUp;                          // device copy of the staged tables
     kt::DevBuf<kt::LalSegStat> lalStats;
     kt::DevBuf<uint8_t> lalSplit;
+    kt::DevBuf<uint8_t> lalFusedScratch, lalFusedRecord;  // fused one-launch batches: tables, packed record
+    kt::DevBuf<unsigned long long> lalFusedOut;          // its status and moved cells
 
     // learned warm start: BF16 weights / FP32 biases, features, prediction
     bool haveWarm = false;

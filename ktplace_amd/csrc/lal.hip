@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "kt_internal.h"
+#include "lal_dev.h"
 #include "lal_geom.h"
 
 namespace kt {
@@ -50,37 +51,8 @@ int smallSortBySegment(kt_ctx *ctx, uint32_t *ids, const uint64_t *key, const ui
 namespace {
 
 constexpr int kB = 256;
-constexpr double kFixScale = 65536.0;  // areas in units of 2^-16
 
 inline dim3 gridFor(uint64_t n) { return dim3(static_cast<uint32_t>(std::max<uint64_t>((n + kB - 1) / kB, 1))); }
-
-__device__ __forceinline__ uint64_t orderKey(double v) {
-    if (v == 0.0) v = 0.0;  // -0.0 and +0.0 compare equal
-    const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
-    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-
-__device__ __forceinline__ double fromOrderKey(uint64_t k) {
-    const uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
-    return __longlong_as_double(static_cast<long long>(b));
-}
-
-struct GridDev {
-    double ox, oy, dx, dy;
-    uint32_t nx, ny;
-};
-
-// simpl_grid.h:120-126 clampIndex
-// (a NaN coordinate maps to bin 0 instead of an undefined conversion)
-__device__ __forceinline__ uint32_t clampIndex(double f, uint32_t n) {
-    const double hi = static_cast<double>(n) - 1e-9;
-    const double c = f >= 0.0 ? (hi < f ? hi : f) : 0.0;
-    return static_cast<uint32_t>(c);
-}
-
-__device__ __forceinline__ uint32_t binOfPoint(const GridDev &g, double x, double y) {
-    return clampIndex((y - g.oy) / g.dy, g.ny) * g.nx + clampIndex((x - g.ox) / g.dx, g.nx);
-}
 
 __device__ __forceinline__ unsigned long long waveSumU64(unsigned long long v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -317,11 +289,7 @@ paintRegionsArg(RectSet rects, uint32_t nx, uint32_t *__restrict__ binRegion, in
 
 // Cells of the batch whose bin changed: new bin stored, (old, new, area)
 // appended for the host's occupancy update.
-struct Moved {
-    uint32_t oldBin, newBin;
-    unsigned long long area;  // 2^-16 units
-};
-static_assert(sizeof(Moved) == 2 * sizeof(unsigned long long), "a moved record is two words of lalMoved");
+using Moved = LalMoved;
 
 // Per-cell state of a batch for the in-order rule (see lalRunBatch).
 struct BatchState {
@@ -403,7 +371,7 @@ passEnd(const uint32_t *__restrict__ list, uint32_t m, uint32_t first, BatchStat
     const uint32_t old = s.binStart[i];
     if (k == old) return;
     const uint32_t slot = atomicAdd(cnt + 2, 1u);
-    out[slot] = Moved{old, k, static_cast<unsigned long long>(llrint(area[i] * kFixScale))};
+    out[slot] = Moved{old, k, i, 0u, static_cast<unsigned long long>(llrint(area[i] * kFixScale))};
 }
 
 // regions >= bad back to their restore points; earlier ones keep their result
@@ -480,10 +448,6 @@ segmentBounds(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ seg
               uint32_t *__restrict__ begin, uint32_t *__restrict__ end) {
     const uint32_t k = blockIdx.x * kB + threadIdx.x;
     if (k < A) segmentBoundsAt(k, ids, seg, A, begin, end);
-}
-
-__device__ __forceinline__ uint64_t areaFixOf(const double *__restrict__ area, uint32_t i) {
-    return static_cast<uint64_t>(llrint(area[i] * kFixScale));
 }
 
 __global__ void __launch_bounds__(kB)
@@ -1064,6 +1028,10 @@ int lalLandReserve(kt_ctx *ctx, size_t varBytes) {
 }
 inline uint32_t *landSlots(kt_ctx *ctx) { return reinterpret_cast<uint32_t *>(ctx->lalLand); }
 inline uint8_t *landVar(kt_ctx *ctx) { return ctx->lalLand + kLandScalars * sizeof(uint32_t); }
+uint8_t *lalLandVar(kt_ctx *ctx) { return landVar(ctx); }
+
+int lalRunBatchFused(kt_ctx *ctx, const std::vector<lal::BinRect> &regions,
+                     const std::vector<std::pair<uint32_t, uint32_t>> &cells, std::vector<LalMoved> &moved, bool *ran);
 
 // Legalizes one batch of non-touching regions; the cells whose bin changed
 // are returned in `moved` (their device binOf is updated).
@@ -1119,11 +1087,11 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
     // the batch's cells: for the small path in any order, appended in one
     // pass; otherwise in id order (the stable radix sorts start from it).
     // lalMoved holds the batch's four counters (two words) and then up to M
-    // moved records (two words each), so one transfer returns both.  The
+    // moved records (three words each), so one transfer returns both.  The
     // counters: [0] first region a cell escaped into (kNoVar: none), [1]
     // cells of a rerun, [2] moved records, [3] the device's count of the
     // batch's cells; the paint kernel sets them to {kNoVar, 0, 0, 0}.
-    KT_TRY(ctx->lalMoved.reserve(2ull + 2ull * M));
+    KT_TRY(ctx->lalMoved.reserve(2ull + 3ull * M));
     uint32_t *cnt = reinterpret_cast<uint32_t *>(ctx->lalMoved.p);
     Moved *dMoved = reinterpret_cast<Moved *>(ctx->lalMoved.p + 2);
     KT_TRY(lalLandReserve(ctx, 0));  // the landing area exists before its first use
@@ -1489,9 +1457,40 @@ int lalLegalize(kt_ctx *ctx) {
     KT_CHECK(ctx->posCount == ctx->n, KT_ESTATE, "lal: positions must have one entry per movable cell");
     ctx->lalReruns = 0;
     const lal::Grid &g = ctx->grid;
+    const uint32_t n = ctx->n;
     std::vector<double> occ;
     std::vector<uint32_t> cells;  // cells per bin, kept in step with the device's binOf
     KT_TRY(lalOccupancy(ctx, &occ, &cells));
+    // the members of each bin, kept on the host like the occupancy (from the
+    // cells that change bin), as doubly linked lists: a batch of the fused
+    // kernel takes its cells from them
+    const uint32_t fusedCells = std::min(ctx->lalFusedCells, kLalFusedCells);
+    const bool members = fusedCells > 0;  // only the fused path reads them
+    std::vector<uint32_t> binOfHost, head, next, prev;
+    if (members) {
+        binOfHost.resize(n);
+        head.assign(g.size(), kNoVar);
+        next.assign(n, kNoVar);
+        prev.assign(n, kNoVar);
+    }
+    if (members && n) {
+        KT_HIP(hipMemcpyAsync(binOfHost.data(), ctx->binOf.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        KT_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    const auto link = [&](uint32_t i, uint32_t k) {
+        next[i] = head[k];
+        prev[i] = kNoVar;
+        if (head[k] != kNoVar) prev[head[k]] = i;
+        head[k] = i;
+    };
+    const auto unlink = [&](uint32_t i, uint32_t k) {
+        if (prev[i] != kNoVar) next[prev[i]] = next[i]; else head[k] = next[i];
+        if (next[i] != kNoVar) prev[next[i]] = prev[i];
+    };
+    for (uint32_t i = members ? n : 0; i-- > 0;) {
+        KT_CHECK(binOfHost[i] < g.size(), KT_ESTATE, "lal: a cell's bin is outside the grid");
+        link(i, binOfHost[i]);
+    }
     // occupancy is kept on the host from here on and updated with the cells
     // that change bin, as the reference does (simpl_lal.cc:86-93)
     std::vector<uint64_t> occFix(g.size());
@@ -1502,6 +1501,7 @@ int lalLegalize(kt_ctx *ctx) {
     fillU32<<<gridFor(g.size()), kB, 0, ctx->stream>>>(ctx->binRegion.p, kNoVar, g.size());
     KT_HIP(hipGetLastError());
     std::vector<Moved> moved;
+    std::vector<std::pair<uint32_t, uint32_t>> batchCells;
     std::vector<uint8_t> rowDirty(g.ny, 0);
     const auto reachOf = [&](const lal::BinRect &r) {
         return lal::BinRect{r.x0 > 0 ? r.x0 - 1 : 0, std::min(r.x1 + 1, g.nx - 1), r.y0 > 0 ? r.y0 - 1 : 0,
@@ -1511,14 +1511,42 @@ int lalLegalize(kt_ctx *ctx) {
         return a.x0 <= b.x1 && b.x0 <= a.x1 && a.y0 <= b.y1 && b.y0 <= a.y1;
     };
     std::vector<lal::BinRect> batch, reach;
-    uint64_t batches = 0;
+    uint64_t batches = 0, fused = 0;
     const auto runBatch = [&]() -> int {
         if (batch.empty()) return KT_OK;
-        KT_TRY(lalRunBatch(ctx, batch, cells, moved));
+        bool ran = false;
+        if (fusedCells > 0 && batch.size() <= kLalFusedRegions && ctx->lalOpts.maxLevel <= 64) {
+            // the batch's cells from the bins' members, when few enough
+            uint64_t count = 0;
+            bool splits = false;
+            for (const lal::BinRect &b : batch) {
+                uint64_t inRegion = 0;
+                for (size_t iy = b.y0; iy <= b.y1; ++iy)
+                    for (size_t ix = b.x0; ix <= b.x1; ++ix) inRegion += cells[g.at(ix, iy)];
+                count += inRegion;
+                splits = splits || inRegion > ctx->lalOpts.minCellsToSplit;
+            }
+            if (count <= fusedCells && splits && ctx->lalOpts.maxLevel > 1) {
+                batchCells.clear();
+                for (uint32_t r = 0; r < batch.size(); ++r) {
+                    const lal::BinRect &b = batch[r];
+                    for (size_t iy = b.y0; iy <= b.y1; ++iy)
+                        for (size_t ix = b.x0; ix <= b.x1; ++ix)
+                            for (uint32_t i = head[g.at(ix, iy)]; i != kNoVar; i = next[i]) batchCells.push_back({i, r});
+                }
+                KT_CHECK(batchCells.size() == count, KT_ESTATE, "lal: bin members out of step with the cell counts");
+                std::sort(batchCells.begin(), batchCells.end());
+                KT_TRY(lalRunBatchFused(ctx, batch, batchCells, moved, &ran));
+                fused += ran ? 1 : 0;
+            }
+        }
+        if (!ran) KT_TRY(lalRunBatch(ctx, batch, cells, moved));
         ++batches;
         batch.clear();
         reach.clear();
         for (const Moved &mv : moved) {
+            KT_CHECK(mv.cell < n && mv.oldBin < g.size() && mv.newBin < g.size(), KT_ESTATE,
+                     "lal: a moved cell's record is out of range");
             --cells[mv.oldBin];
             ++cells[mv.newBin];
             occFix[mv.oldBin] -= mv.area;
@@ -1527,6 +1555,10 @@ int lalLegalize(kt_ctx *ctx) {
             occ[mv.newBin] = static_cast<double>(occFix[mv.newBin]) / kFixScale;
             rowDirty[mv.oldBin / g.nx] = 1;
             rowDirty[mv.newBin / g.nx] = 1;
+            if (members) {
+                unlink(mv.cell, mv.oldBin);
+                link(mv.cell, mv.newBin);
+            }
         }
         for (size_t iy = 0; iy < g.ny && !moved.empty(); ++iy)
             if (rowDirty[iy]) {
@@ -1547,13 +1579,13 @@ int lalLegalize(kt_ctx *ctx) {
         reach.push_back(reachOf(bins));
     }
     KT_TRY(runBatch());
-    const uint32_t n = ctx->n;
     if (n) {
         clampToDie<<<gridFor(n), kB, 0, ctx->stream>>>(ctx->posX.p, ctx->posY.p, ctx->width.p, ctx->height.p,
                                                        ctx->lalDie.x0, ctx->lalDie.y0, ctx->lalDie.x1, ctx->lalDie.y1, n);
         KT_HIP(hipGetLastError());
     }
     ctx->lalBatches = batches;
+    ctx->lalFusedBatches = fused;
     return KT_OK;
 }
 

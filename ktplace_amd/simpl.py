@@ -120,6 +120,8 @@ _SIGS = {
     "kt_lal_resident": (I32, [P]),
     "kt_lal_batches": (I32, [P, C.POINTER(C.c_uint64)]),
     "kt_lal_reruns": (I32, [P, C.POINTER(C.c_uint64)]),
+    "kt_set_lal_fused": (I32, [P, U32]),
+    "kt_lal_fused_batches": (I32, [P, C.POINTER(C.c_uint64)]),
     "kt_floating_cells": (I32, [P, C.POINTER(C.c_uint32), P]),
     "kt_loop_reset": (I32, [P, C.c_uint64, C.c_double, C.c_double, C.c_uint64]),
     "kt_anchors_from_positions": (I32, [P]),
@@ -331,6 +333,15 @@ class Context:
     def lal_reruns(self) -> int:
         b = C.c_uint64()
         _check(lib.kt_lal_reruns(self.h, C.byref(b)))
+        return b.value
+
+    def set_lal_fused(self, max_cells: int):
+        """Batches of at most max_cells cells (<= 1024) run in one launch each; 0: all level by level."""
+        _check(lib.kt_set_lal_fused(self.h, int(max_cells)))
+
+    def lal_fused_batches(self) -> int:
+        b = C.c_uint64()
+        _check(lib.kt_lal_fused_batches(self.h, C.byref(b)))
         return b.value
 
     # positions ---------------------------------------------------------------

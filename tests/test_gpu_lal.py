@@ -116,6 +116,39 @@ def test_lal_reruns_match_oracle(gpu_ctx_factory):
     ctx.close()
 
 
+@pytest.mark.parametrize("bins,seed,spread", [(64, 11, None), (256, 12, None), (512, 14, 0.3), (1536, 13, 0.05)])
+def test_lal_fused_batches_match_level_path_and_oracle(gpu_ctx_factory, bins, seed, spread):
+    """Batches of up to 1024 cells run in one launch each, block geometry on
+    the device (csrc/lal_batch.hip); with that path switched off every batch
+    runs level by level with the geometry on the host.  Both give the
+    oracle's positions cell for cell, on uniformly scattered and clumped
+    placements from 64 to 1536 bins per axis (on the finest grid batches
+    rerun regions)."""
+    d = dz.workload("cells12k")
+    o = ol.Oracle.of(d)
+    ctx = gpu_ctx_factory(d)
+    ctx.build_grid(bins, 1.0)
+    o.build_grid(bins, 1.0)
+    if spread is None:
+        rng = np.random.default_rng(seed)
+        x = rng.uniform(0, 1, d.n) * (d.die[2] - d.flat.width)
+        y = rng.uniform(0, 1, d.n) * (d.die[3] - d.flat.height)
+    else:
+        x, y = clumped(d, seed, spread)
+    rx, ry = o.lal(x, y)
+    got = {}
+    for fused in (True, False):
+        ctx.set_lal_fused(1024 if fused else 0)
+        gx, gy = ctx.lal(x, y)
+        got[fused] = (ctx.lal_batches(), ctx.lal_fused_batches(), ctx.lal_reruns())
+        print(f"bins {bins} fused {fused}: batches, fused, reruns {got[fused]}; identical "
+              f"{np.mean((gx == rx) & (gy == ry)):.6f}", flush=True)
+        assert np.array_equal(gx, rx) and np.array_equal(gy, ry)
+    assert got[True][1] > 0 and got[False][1] == 0
+    assert got[True][0] == got[False][0] and got[True][2] == got[False][2]
+    ctx.close()
+
+
 def test_lal_deterministic(gpu_ctx_factory):
     d = dz.workload("cells12k")
     ctx = gpu_ctx_factory(d)
