@@ -697,7 +697,6 @@ struct SmallArgs {
     const double *area;
     uint32_t *ids;          // sorted cell ids of the level
     uint32_t *begin, *end;  // per block / half
-    uint64_t *P;            // fixed-point area prefix
     SegStat *stats;
     uint32_t *stripeOf;
     uint32_t B, A;          // blocks and active cells of the level
@@ -707,6 +706,7 @@ struct SmallArgs {
 __global__ void __launch_bounds__(kSmallThreads) smallLevelA(SmallArgs a) {
     __shared__ uint32_t ss[kSmallCells], si[kSmallCells];
     __shared__ uint64_t sk[kSmallCells];
+    __shared__ uint64_t sP[kSmallCells];  // the area prefix, searched by the split
     __shared__ uint64_t waveTotals[kSmallThreads / 64];
     for (uint32_t j = threadIdx.x; j < a.width; j += blockDim.x) {
         if (j < a.M) {
@@ -728,13 +728,15 @@ __global__ void __launch_bounds__(kSmallThreads) smallLevelA(SmallArgs a) {
         a.end[q] = 0;
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < a.A; k += blockDim.x) {
-        segmentBoundsAt(k, a.ids, a.seg, a.A, a.begin, a.end);
-        a.P[k] = areaFixOf(a.area, a.ids[k]);
+    for (uint32_t k = threadIdx.x; k < a.A; k += blockDim.x) {  // ss[k], si[k]: block and id of cell k
+        const uint32_t sg = ss[k];
+        if (k == 0 || ss[k - 1] != sg) a.begin[sg] = k;
+        if (k + 1 == a.A || ss[k + 1] != sg) a.end[sg] = k + 1;
+        sP[k] = areaFixOf(a.area, si[k]);
     }
     __syncthreads();
-    blockScanU64(a.P, a.A, waveTotals);
-    for (uint32_t s = threadIdx.x; s < a.B; s += blockDim.x) splitBlockAt(s, a.begin, a.end, a.P, a.stats);
+    blockScanU64(sP, a.A, waveTotals);
+    for (uint32_t s = threadIdx.x; s < a.B; s += blockDim.x) splitBlockAt(s, a.begin, a.end, sP, a.stats);
 }
 
 // halves, distance order, stripes, affine map, next level's blocks
@@ -743,6 +745,7 @@ smallLevelB(SmallArgs a, HalfArgs ha, StripeArgs sa, const uint32_t *__restrict_
             uint32_t A2, double *__restrict__ posOut) {
     __shared__ uint32_t ss[kSmallCells], si[kSmallCells];
     __shared__ uint64_t sk[kSmallCells];
+    __shared__ uint64_t sP[kSmallCells];  // the area prefix, searched by the stripe filling
     __shared__ uint64_t waveTotals[kSmallThreads / 64];
     for (uint32_t j = threadIdx.x; j < a.M; j += blockDim.x) a.half[a.list[j]] = kNoVar;
     __syncthreads();
@@ -772,15 +775,26 @@ smallLevelB(SmallArgs a, HalfArgs ha, StripeArgs sa, const uint32_t *__restrict_
         sa.hi[q] = 0ull;
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < A2; k += blockDim.x) {
-        segmentBoundsAt(k, a.ids, a.half, A2, a.begin, a.end);
-        a.P[k] = areaFixOf(a.area, a.ids[k]);
+    for (uint32_t k = threadIdx.x; k < A2; k += blockDim.x) {  // ss[k], si[k]: half and id of cell k
+        const uint32_t hh = ss[k];
+        if (k == 0 || ss[k - 1] != hh) a.begin[hh] = k;
+        if (k + 1 == A2 || ss[k + 1] != hh) a.end[hh] = k + 1;
+        sP[k] = areaFixOf(a.area, si[k]);
     }
     __syncthreads();
-    blockScanU64(a.P, A2, waveTotals);
-    for (uint32_t hh = threadIdx.x; hh < H; hh += blockDim.x) stripeRangeAt(hh, a.begin, a.end, a.P, sa);
+    blockScanU64(sP, A2, waveTotals);
+    for (uint32_t hh = threadIdx.x; hh < H; hh += blockDim.x) stripeRangeAt(hh, a.begin, a.end, sP, sa);
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < A2; k += blockDim.x) stripeOfCellAt(k, a.ids, a.half, sa, a.c, a.stripeOf);
+    for (uint32_t k = threadIdx.x; k < A2; k += blockDim.x) {
+        const uint32_t hh = ss[k];
+        uint32_t st = sa.stripeStart[hh];
+        const uint32_t s1 = sa.stripeStart[hh + 1];
+        while (st + 1 < s1 && sa.stripeEnd[st] <= k) ++st;
+        a.stripeOf[k] = st;
+        const unsigned long long key = orderKey(centreOf(a.c, si[k]));
+        atomicMin(sa.lo + st, key);
+        atomicMax(sa.hi + st, key);
+    }
     __threadfence();
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < A2; k += blockDim.x) {
@@ -788,7 +802,7 @@ smallLevelB(SmallArgs a, HalfArgs ha, StripeArgs sa, const uint32_t *__restrict_
         const uint32_t st = a.stripeOf[k];
         const unsigned long long lo = __hip_atomic_load(sa.lo + st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned long long hi = __hip_atomic_load(sa.hi + st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        scaleCellWith(k, a.ids, st, lo, hi, sa, a.c, posOut);
+        scaleCellWith(k, si, st, lo, hi, sa, a.c, posOut);
     }
     for (uint32_t j = threadIdx.x; j < a.M; j += blockDim.x) {
         const uint32_t id = a.list[j], h = a.half[id];
@@ -1180,7 +1194,6 @@ int lalRunBatch(kt_ctx *ctx, const std::vector<lal::BinRect> &regions, const std
                 sm.ids = ctx->lalIds.p;
                 sm.begin = ctx->lalBegin.p;
                 sm.end = ctx->lalEnd.p;
-                sm.P = ctx->lalPrefix.p;
                 sm.stats = ctx->lalStats.p;
                 sm.stripeOf = ctx->lalStripeOf.p;
                 sm.B = B;
