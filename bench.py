@@ -163,6 +163,7 @@ def main() -> int:
     ap.add_argument("--row-lanes", type=int, default=0, help="with --layout 1: lanes per row in the row sums")
     ap.add_argument("--block-threads", type=int, default=0, help="with --layout 1: threads per block")
     ap.add_argument("--col-bytes", type=int, default=0, help="with --layout 1: bytes per column index (3 or 4)")
+    ap.add_argument("--wide", type=int, default=0, help="with --layout 1: 1 = four consecutive entries per load")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -216,7 +217,7 @@ def main() -> int:
         if args.lanes and args.tile:
             tuned = TuneResult(args.lanes, args.tile, float("nan"), [], args.xcd_split, args.rows, args.halo,
                                args.layout, args.block_entries, args.unroll, args.row_lanes, args.block_threads,
-                               args.col_bytes)
+                               args.col_bytes, args.wide)
         elif prec in tuned_by_prec:
             tuned = tuned_by_prec[prec]
         else:
@@ -292,7 +293,8 @@ def main() -> int:
                                                              if tuned.layout else None),
                     "unroll": tuned.unroll or (4 if tuned.layout else None),
                     "row_lanes": tuned.row_lanes or (1 if tuned.layout else None),
-                    "column_bytes": (3 if tuned.col_bytes == 3 else 4) if tuned.layout else None},
+                    "column_bytes": (3 if tuned.col_bytes == 3 else 4) if tuned.layout else None,
+                    "wide_loads": bool(tuned.wide) if tuned.layout else None},
             roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                           frac=round(achieved / HBM_PEAK_GBS, 4),
                           traffic=(MEASURED_TRAFFIC.get(base_path) if args.workload == "cells1m" else None),

@@ -280,6 +280,10 @@ int kt_set_spmv_layout(kt_ctx *ctx, int layout, uint32_t block_entries, int unro
  * of fewer than 2^24 rows; elsewhere the 4-byte columns are read.  The
  * products, their order and the results are the same either way. */
 int kt_set_spmv_columns(kt_ctx *ctx, int bytes);
+/* 1: the entry-block SpMV (blocks of 256 threads and 2048 entries, 4 lanes
+ * per row) reads four consecutive entries per thread with one wide load per
+ * array; 0 (default): one entry per load.  Results are the same either way. */
+int kt_set_spmv_wide_loads(kt_ctx *ctx, int on);
 
 /* ---- per-kernel timings (HIP events on the context's stream) -------------- */
 #define KT_KERNEL_SPMV 0      /* FP64 CG SpMV + p.Ap */

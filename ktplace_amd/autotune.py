@@ -5,7 +5,8 @@ per row (threads cooperating on one CSR row: 1, 2, 4, 8 or 16), the tile
 (rows per 1024-thread block), rows in flight, the XCD split, the LDS
 window (`halo`: entries of p staged on each side of a tile) and the
 entry-block shape (threads and entries per block, lanes per row in the row
-sums, 3- or 4-byte column indices) by timing a fixed number of CG iterations of the context's current
+sums, 3- or 4-byte column indices, one entry or four consecutive entries per
+load) by timing a fixed number of CG iterations of the context's current
 systems on the GPU, from the same saved start each time.  First lanes,
 tile, rows and XCD split without staging; then, with the best rows and XCD
 split, every lanes, tile and halo whose window fits the LDS budget; then the
@@ -51,12 +52,14 @@ class TuneResult:
     row_lanes: int = 0
     block_threads: int = 0
     col_bytes: int = 0
+    wide: int = 0
 
     def kwargs(self) -> dict:
         """The choice as `Context.set_tuning` keyword arguments."""
         return dict(lanes=self.lanes, tile=self.tile, xcd_split=self.xcd_split, rows=self.rows, halo=self.halo,
                     layout=self.layout, block_entries=self.block_entries, unroll=self.unroll,
-                    row_lanes=self.row_lanes, block_threads=self.block_threads, col_bytes=self.col_bytes)
+                    row_lanes=self.row_lanes, block_threads=self.block_threads, col_bytes=self.col_bytes,
+                    wide=self.wide)
 
 
 def _signature(ctx: "simpl.Context", precision: int):
@@ -121,9 +124,11 @@ def autotune(ctx: "simpl.Context", precision: int = simpl.KT_PREC_FP64, iteratio
                 for threads, entries in FLAT_SHAPES:
                     for row_lanes in FLAT_ROW_LANES:
                         for col_bytes in ((4, 3) if threads == 256 and rows < (1 << 24) else (4,)):
-                            timed(lanes=4, tile=tile, xcd_split=x, rows=2, layout=simpl.KT_SPMV_ENTRY_BLOCKS,
-                                  block_entries=entries, unroll=4, row_lanes=row_lanes, block_threads=threads,
-                                  col_bytes=col_bytes)
+                            # wide loads are built for 256-thread blocks with 4 lanes per row
+                            for wide in ((0, 1) if threads == 256 and row_lanes == 4 else (0,)):
+                                timed(lanes=4, tile=tile, xcd_split=x, rows=2, layout=simpl.KT_SPMV_ENTRY_BLOCKS,
+                                      block_entries=entries, unroll=4, row_lanes=row_lanes, block_threads=threads,
+                                      col_bytes=col_bytes, wide=wide)
     # the closest candidates again, over twice the iterations, twice each:
     # single short timings of near-equal shapes are noisy
     table.sort(key=lambda t: t[1])

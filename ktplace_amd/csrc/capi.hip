@@ -54,8 +54,9 @@ int ensureSystem(kt_ctx *ctx, int axis, uint64_t rows, uint64_t nnzCap) {
     Sys &s = ctx->sys[axis];
     KT_CHECK(rows < 0xffffffffull && nnzCap < 0xffffffffull, KT_EINVAL, "system too large for 32-bit indices");
     KT_TRY(s.rowPtr.reserve(rows + 1));
-    KT_TRY(s.col.reserve(std::max<uint64_t>(nnzCap, 1)));
-    KT_TRY(s.val.reserve(std::max<uint64_t>(nnzCap, 1)));
+    // four spare entries: the SpMV's wide loads read whole groups of four
+    KT_TRY(s.col.reserve(nnzCap + kQuadPad));
+    KT_TRY(s.val.reserve(nnzCap + kQuadPad));
     KT_TRY(s.diag.reserve(std::max<uint64_t>(rows, 1)));
     KT_TRY(s.rhs.reserve(std::max<uint64_t>(rows, 1)));
     s.rows = rows;
@@ -92,6 +93,18 @@ int timedEnd(kt_ctx *ctx, int kernel, hipEvent_t start) {
     KT_HIP(hipEventRecord(stop, ctx->stream));
     ctx->timers[kernel].pending.push_back(start);
     ctx->timers[kernel].pending.push_back(stop);
+    return KT_OK;
+}
+
+// Events for a launch that records them itself (hipExtLaunchKernelGGL): the
+// pair is queued for collectTimers like timedBegin / timedEnd's.
+int timedKernelEvents(kt_ctx *ctx, int kernel, hipEvent_t *start, hipEvent_t *stop) {
+    *start = *stop = nullptr;
+    if (!ctx->profile) return KT_OK;
+    KT_HIP(hipEventCreate(start));
+    KT_HIP(hipEventCreate(stop));
+    ctx->timers[kernel].pending.push_back(*start);
+    ctx->timers[kernel].pending.push_back(*stop);
     return KT_OK;
 }
 
@@ -913,6 +926,13 @@ int kt_set_spmv_columns(kt_ctx *ctx, int bytes) {
     KT_GUARD(ctx);
     KT_CHECK(bytes == 0 || bytes == 3 || bytes == 4, KT_EINVAL, "kt_set_spmv_columns: bytes must be 0, 3 or 4");
     ctx->tune.colBytes = bytes;
+    return KT_OK;
+}
+
+int kt_set_spmv_wide_loads(kt_ctx *ctx, int on) {
+    KT_GUARD(ctx);
+    KT_CHECK(on == 0 || on == 1, KT_EINVAL, "kt_set_spmv_wide_loads: on must be 0 or 1");
+    ctx->tune.wideLoads = on;
     return KT_OK;
 }
 

@@ -31,6 +31,8 @@
 #include <cstdlib>
 #include <cstring>
 
+#include <hip/hip_ext.h>
+
 #include "kt_internal.h"
 #include "spmv_rows.h"
 
@@ -352,6 +354,41 @@ __global__ void __launch_bounds__(kBlock) cgSpmv(CgArgs a, int parity) {
 // that long crossing a cut) is done with the lane-group rows instead.  The
 // slack is an eighth of the block, at most 512 entries, so 2048-entry blocks
 // of 256 threads fit eight to a CU's LDS.
+// Four consecutive entries k0 .. k0 + 3 (k0 a multiple of four): columns
+// and values by one wide load per array.
+template <typename T, bool C24>
+__device__ __forceinline__ void loadQuad(const SysView &s, const T *__restrict__ val, uint32_t k0, uint32_t (&c)[4],
+                                         T (&w)[4]) {
+    if constexpr (C24) {
+        const uint2 lo = *reinterpret_cast<const uint2 *>(s.colLo + k0);
+        const uint32_t hi = *reinterpret_cast<const uint32_t *>(s.colHi + k0);
+        c[0] = (lo.x & 0xffffu) | ((hi & 0xffu) << 16);
+        c[1] = (lo.x >> 16) | (((hi >> 8) & 0xffu) << 16);
+        c[2] = (lo.y & 0xffffu) | (((hi >> 16) & 0xffu) << 16);
+        c[3] = (lo.y >> 16) | ((hi >> 24) << 16);
+    } else {
+        const uint4 q = *reinterpret_cast<const uint4 *>(s.col + k0);
+        c[0] = q.x;
+        c[1] = q.y;
+        c[2] = q.z;
+        c[3] = q.w;
+    }
+    if constexpr (sizeof(T) == 4) {
+        const float4 v = *reinterpret_cast<const float4 *>(val + k0);
+        w[0] = v.x;
+        w[1] = v.y;
+        w[2] = v.z;
+        w[3] = v.w;
+    } else {
+        const double2 v0 = *reinterpret_cast<const double2 *>(val + k0);
+        const double2 v1 = *reinterpret_cast<const double2 *>(val + k0 + 2);
+        w[0] = v0.x;
+        w[1] = v0.y;
+        w[2] = v1.x;
+        w[3] = v1.y;
+    }
+}
+
 constexpr uint32_t flatSlack(uint32_t entries) { return entries / 8 < 512 ? entries / 8 : 512; }
 
 // Block -> (axis, block of that axis).  With xcdSplit the blocks that land on
@@ -418,7 +455,14 @@ uint32_t flatGrid(const CgArgs &a) {
 // take the FP64 residual b - A x64 of the same rows instead: 1 refreshes the
 // FP32 residual of the axes that asked for it (partials r64.r64, r.z), 2 is
 // the true residual check of both axes (partials r.r, b.b); parity is -1.
-template <typename T, uint32_t E, int U, int G2, int NT, bool C24 = false, int MODE = 0>
+// W4: phase 1 reads four consecutive entries per thread with one wide load
+// per array (16-byte values, 8 + 4 bytes of 3-byte columns), from the block's
+// first entry rounded down to a multiple of four; entries outside the block
+// are loaded and dropped.  A wave then issues three loads per 256 entries
+// instead of twelve per 64, which relieves the vector-memory address path
+// that the gathers share.  The arrays are padded to a multiple of four
+// entries.  Products and their order are unchanged.
+template <typename T, uint32_t E, int U, int G2, int NT, bool C24 = false, int MODE = 0, bool W4 = false>
 __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
     static_assert(MODE == 0 || sizeof(T) == 8, "the residual modes run in FP64");
     constexpr int K = MODE == 0 ? 1 : 2;
@@ -491,6 +535,44 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
         const uint32_t pre0 = s.rowPtr[rq], pre1 = s.rowPtr[rq + 1];
         const T preP = p[rq], preD = diag[rq];
         __builtin_amdgcn_sched_barrier(0);
+        if constexpr (W4) {
+            // U / 4 quads per thread and round; quad q covers entries
+            // A + 4q .. A + 4q + 3, A = E0 rounded down to a multiple of 4
+            constexpr int Q = U / 4 > 0 ? U / 4 : 1;
+            const uint32_t A = E0 & ~3u, lead = E0 - A, quads = (lead + count + 3) / 4;
+            for (uint32_t qb = 0; qb < quads; qb += Q * NT) {
+                uint32_t c[Q][4];
+                T w[Q][4];
+#pragma unroll
+                for (int u = 0; u < Q; ++u) {
+                    const uint32_t q = qb + u * NT + threadIdx.x;
+                    loadQuad<T, C24>(s, val, A + 4u * (q < quads ? q : 0u), c[u], w[u]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                T g[Q][4];
+#pragma unroll
+                for (int u = 0; u < Q; ++u) {
+                    const uint32_t q = qb + u * NT + threadIdx.x;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const uint32_t k = 4u * q + j;  // entry A + k, slot k - lead of the block
+                        const bool ok = q < quads && k >= lead && k < lead + count;
+                        g[u][j] = gatherAt(p, ok ? c[u][j] : kNoCol);
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < Q; ++u) {
+                    const uint32_t q = qb + u * NT + threadIdx.x;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const uint32_t k = 4u * q + j;
+                        if (q < quads && k >= lead && k < lead + count)
+                            prod[k - lead] = static_cast<double>(w[u][j]) * static_cast<double>(g[u][j]);
+                    }
+                }
+            }
+        } else
         for (uint32_t base = 0; base < count; base += U * NT) {
             uint32_t c[U];
             T w[U];
@@ -914,47 +996,59 @@ uint32_t gridBlocks(const CgArgs &a) {
 // The SpMV instance for (G, R, staging).  With attributesOnly it only raises
 // the instance's dynamic LDS limit to its window (done before any capture).
 template <typename T>
-int launchSpmv(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, bool attributesOnly) {
+int launchSpmv(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, bool attributesOnly,
+               hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     if (a.layout == 1) {
         if (attributesOnly) return KT_OK;
         const dim3 fg(flatGrid(a));
-        const bool wide = a.flatUnroll == 8;
+        const bool u8 = a.flatUnroll == 8;
         const bool lanes4 = a.flatRowLanes == 4;
         const int nt = static_cast<int>(a.flatThreads);
-#define KT_FLAT(EE)                                                                                       \
-    if (lanes4) {                                                                                         \
-        if (wide) cgSpmvFlat<T, EE, 8, 4, kBlock><<<fg, dim3(kBlock), 0, st>>>(a, parity);                \
-        else cgSpmvFlat<T, EE, 4, 4, kBlock><<<fg, dim3(kBlock), 0, st>>>(a, parity);                     \
-    } else {                                                                                              \
-        if (wide) cgSpmvFlat<T, EE, 8, 1, kBlock><<<fg, dim3(kBlock), 0, st>>>(a, parity);                \
-        else cgSpmvFlat<T, EE, 4, 1, kBlock><<<fg, dim3(kBlock), 0, st>>>(a, parity);                     \
-    }
-#define KT_FLAT_SMALL(EE, NN)                                                                             \
-    if (lanes4) cgSpmvFlat<T, EE, 4, 4, NN><<<fg, dim3(NN), 0, st>>>(a, parity);                          \
-    else cgSpmvFlat<T, EE, 4, 1, NN><<<fg, dim3(NN), 0, st>>>(a, parity);
+        // with events, the dispatch itself records the kernel's start and end
+        // (no separate event packets around it)
+        const auto run = [&](auto kernel, int threads) -> int {
+            if (e0) hipExtLaunchKernelGGL(kernel, fg, dim3(threads), 0, st, e0, e1, 0, a, parity);
+            else kernel<<<fg, dim3(threads), 0, st>>>(a, parity);
+            KT_HIP(hipGetLastError());
+            return KT_OK;
+        };
         if (nt == 256) {
+            if (a.wide && a.col24) {
+                if (u8) return run(cgSpmvFlat<T, 2048, 8, 4, 256, true, 0, true>, 256);
+                return run(cgSpmvFlat<T, 2048, 4, 4, 256, true, 0, true>, 256);
+            }
+            if (a.wide) {
+                if (u8) return run(cgSpmvFlat<T, 2048, 8, 4, 256, false, 0, true>, 256);
+                return run(cgSpmvFlat<T, 2048, 4, 4, 256, false, 0, true>, 256);
+            }
             if (a.col24) {
-                if (lanes4) cgSpmvFlat<T, 2048, 4, 4, 256, true><<<fg, dim3(256), 0, st>>>(a, parity);
-                else cgSpmvFlat<T, 2048, 4, 1, 256, true><<<fg, dim3(256), 0, st>>>(a, parity);
-            } else if (wide) {
-                if (lanes4) cgSpmvFlat<T, 2048, 8, 4, 256><<<fg, dim3(256), 0, st>>>(a, parity);
-                else cgSpmvFlat<T, 2048, 8, 1, 256><<<fg, dim3(256), 0, st>>>(a, parity);
-            } else {
-                KT_FLAT_SMALL(2048, 256)
+                if (lanes4) return run(cgSpmvFlat<T, 2048, 4, 4, 256, true>, 256);
+                return run(cgSpmvFlat<T, 2048, 4, 1, 256, true>, 256);
             }
-        } else if (nt == 512) {
-            KT_FLAT_SMALL(4096, 512)
-        } else {
-            switch (a.flatEntries) {
-                case 4096: KT_FLAT(4096) break;
-                case 16384: KT_FLAT(16384) break;
-                default: KT_FLAT(8192) break;
+            if (u8) {
+                if (lanes4) return run(cgSpmvFlat<T, 2048, 8, 4, 256>, 256);
+                return run(cgSpmvFlat<T, 2048, 8, 1, 256>, 256);
             }
+            if (lanes4) return run(cgSpmvFlat<T, 2048, 4, 4, 256>, 256);
+            return run(cgSpmvFlat<T, 2048, 4, 1, 256>, 256);
         }
-#undef KT_FLAT_SMALL
+        if (nt == 512) {
+            if (lanes4) return run(cgSpmvFlat<T, 4096, 4, 4, 512>, 512);
+            return run(cgSpmvFlat<T, 4096, 4, 1, 512>, 512);
+        }
+#define KT_FLAT(EE)                                                      \
+    if (lanes4) {                                                        \
+        if (u8) return run(cgSpmvFlat<T, EE, 8, 4, kBlock>, kBlock);     \
+        return run(cgSpmvFlat<T, EE, 4, 4, kBlock>, kBlock);             \
+    }                                                                    \
+    if (u8) return run(cgSpmvFlat<T, EE, 8, 1, kBlock>, kBlock);         \
+    return run(cgSpmvFlat<T, EE, 4, 1, kBlock>, kBlock);
+        switch (a.flatEntries) {
+            case 4096: KT_FLAT(4096)
+            case 16384: KT_FLAT(16384)
+            default: KT_FLAT(8192)
+        }
 #undef KT_FLAT
-        KT_HIP(hipGetLastError());
-        return KT_OK;
     }
     const int R = ctx->tune.rows > 0 ? ctx->tune.rows : 2;
     const dim3 grid(gridBlocks(a));
@@ -992,9 +1086,15 @@ int launchIteration(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t
     const dim3 grid(gridBlocks(a));
     hipEvent_t t0 = nullptr;
     const int kSpmv = sizeof(T) == 8 ? KT_KERNEL_SPMV : KT_KERNEL_SPMV32;
-    if (timed) KT_TRY(timedBegin(ctx, kSpmv, &t0));
-    KT_TRY(launchSpmv<T>(ctx, a, G, parity, st, false));
-    if (timed) KT_TRY(timedEnd(ctx, kSpmv, t0));
+    if (timed && a.layout == 1) {
+        hipEvent_t e0, e1;
+        KT_TRY(timedKernelEvents(ctx, kSpmv, &e0, &e1));
+        KT_TRY(launchSpmv<T>(ctx, a, G, parity, st, false, e0, e1));
+    } else {
+        if (timed) KT_TRY(timedBegin(ctx, kSpmv, &t0));
+        KT_TRY(launchSpmv<T>(ctx, a, G, parity, st, false));
+        if (timed) KT_TRY(timedEnd(ctx, kSpmv, t0));
+    }
     if (timed) KT_TRY(timedBegin(ctx, KT_KERNEL_UPDATE, &t0));
     switch (a.tile) {
         case 1 * kBlock: cgUpdate<T, 1><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
@@ -1207,6 +1307,10 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
     // 3-byte columns: entry blocks of 256 threads, rows below 2^24
     a.col24 = (a.layout == 1 && ctx->tune.colBytes == 3 && a.flatThreads == 256 && a.flatUnroll == 4 &&
                a.flatEntries == 2048 && rows < (1ull << 24)) ? 1u : 0u;
+    // wide loads: entry blocks of 256 threads and 2048 entries, 4 lanes per row
+    a.wide = (a.layout == 1 && ctx->tune.wideLoads && a.flatThreads == 256 && a.flatEntries == 2048 &&
+              a.flatRowLanes == 4) ? 1u : 0u;
+    a.col24 = a.col24 || (a.wide && ctx->tune.colBytes == 3 && rows < (1ull << 24)) ? 1u : 0u;
     // FP64 residuals by the same entry blocks (the shape instantiated for them)
     a.flatResidual =
         (a.layout == 1 && a.flatThreads == 256 && a.flatUnroll == 4 && a.flatEntries == 2048 && rows > 0) ? 1u : 0u;
@@ -1222,7 +1326,7 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
         if (fp32) {
             Sys &s = ctx->sys[ax];
             if (!s.have32) {
-                KT_TRY(s.val32.reserve(std::max<uint64_t>(nnz[ax], 1)));
+                KT_TRY(s.val32.reserve(nnz[ax] + kQuadPad));
                 KT_TRY(s.diag32.reserve(std::max<uint64_t>(rows, 1)));
                 if (nnz[ax]) {
                     toFloat<<<dim3(1024), dim3(kBlock), 0, ctx->stream>>>(s.val.p, s.val32.p, nnz[ax]);
@@ -1264,8 +1368,8 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
             a.sys[ax].flatInfo = info;
             if (a.col24) {
                 if (!sy.have24) {
-                    KT_TRY(sy.colLo.reserve(std::max<uint64_t>(nnz[ax], 1)));
-                    KT_TRY(sy.colHi.reserve(std::max<uint64_t>(nnz[ax], 1)));
+                    KT_TRY(sy.colLo.reserve(nnz[ax] + kQuadPad));
+                    KT_TRY(sy.colHi.reserve(nnz[ax] + kQuadPad));
                     if (nnz[ax]) {
                         splitColumns<<<dim3(1024), dim3(kBlock), 0, ctx->stream>>>(sy.col.p, sy.colLo.p, sy.colHi.p,
                                                                                   nnz[ax]);

@@ -70,6 +70,9 @@ struct DevBuf {
 };
 
 constexpr uint32_t kNoVar = KT_NO_VAR;
+// Spare entries after a system's column and value arrays: the entry-block
+// SpMV's wide loads read whole aligned groups of four entries.
+constexpr uint64_t kQuadPad = 4;
 
 // CG state of one axis at the start of an iteration.  Double buffered by
 // iteration parity (AxisCtl::s[k & 1]).
@@ -156,6 +159,7 @@ struct CgArgs {
     uint32_t flatThreads;  // entry blocks: threads per block (256, 512, 1024)
     uint32_t col24;        // entry blocks: 1 = columns read as 16 + 8 bits
     uint32_t flatResidual; // 1: FP64 residuals (refresh, true residual) by entry blocks of 256 threads
+    uint32_t wide;         // entry blocks: 1 = four consecutive entries per thread by wide loads
 };
 
 // Launch-shape knobs (set by the autotuner or the caller; 0 = default).
@@ -173,6 +177,7 @@ struct Tune {
     int flatRowLanes = 0;     // entry blocks: lanes per row in the row sums (0, 1: entry order; 4)
     int flatThreads = 0;      // entry blocks: threads per block (0: 1024; 256, 512)
     int colBytes = 0;         // entry blocks: bytes per column index (0, 4; 3 with 256 threads)
+    int wideLoads = 0;        // entry blocks: four consecutive entries per thread by wide loads
 };
 
 // A captured hipGraph of kPairs iteration pairs for one argument set.
@@ -313,6 +318,7 @@ int scanRowPtr(kt_ctx *ctx, uint32_t *rowPtr, uint64_t rows);
 int systemNnz(kt_ctx *ctx, int axis, uint64_t *nnz);
 int timedBegin(kt_ctx *ctx, int kernel, hipEvent_t *start);
 int timedEnd(kt_ctx *ctx, int kernel, hipEvent_t start);
+int timedKernelEvents(kt_ctx *ctx, int kernel, hipEvent_t *start, hipEvent_t *stop);
 int collectTimers(kt_ctx *ctx);
 int warmSetParams(kt_ctx *ctx, const float *params, uint64_t count);
 int warmFeaturesHost(kt_ctx *ctx, float *features, double *scale);

@@ -105,6 +105,7 @@ _SIGS = {
     "kt_set_spmv_halo": (I32, [P, U32]),
     "kt_set_spmv_layout": (I32, [P, I32, U32, I32, I32, I32]),
     "kt_set_spmv_columns": (I32, [P, I32]),
+    "kt_set_spmv_wide_loads": (I32, [P, I32]),
     "kt_kernel_times": (I32, [P, I32, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "kt_reset_kernel_times": (I32, [P]),
     "kt_probe_copy": (I32, [P, U64, I32, C.POINTER(C.c_double)]),
@@ -451,13 +452,14 @@ class Context:
     def set_tuning(self, lanes: int = 0, tile: int = 0, use_graphs: bool = True,
                    refresh_factor: float = 0.0, xcd_split: int = -1, rows: int = 0, halo: int = 0,
                    layout: int = 0, block_entries: int = 0, unroll: int = 0, row_lanes: int = 0,
-                   block_threads: int = 0, col_bytes: int = 0):
+                   block_threads: int = 0, col_bytes: int = 0, wide: int = 0):
         _check(lib.kt_set_tuning2(self.h, lanes, tile, int(use_graphs), refresh_factor, xcd_split,
                                   rows))
         _check(lib.kt_set_spmv_halo(self.h, halo))
         _check(lib.kt_set_spmv_layout(self.h, layout, block_entries, unroll, row_lanes,
                                       block_threads))
         _check(lib.kt_set_spmv_columns(self.h, col_bytes))
+        _check(lib.kt_set_spmv_wide_loads(self.h, int(wide)))
 
     def kernel_time(self, kernel: int):
         n = C.c_uint64()

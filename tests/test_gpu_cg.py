@@ -403,9 +403,11 @@ def test_entry_block_layout_long_rows(gpu_ctx_factory):
     for prec, entries, unroll, row_lanes, threads in (
             (KT_PREC_FP64, 0, 0, 0, 0), (KT_PREC_MIXED, 0, 0, 4, 0), (KT_PREC_FP64, 4096, 8, 4, 0),
             (KT_PREC_MIXED, 16384, 8, 1, 0), (KT_PREC_FP64, 16384, 4, 4, 0), (KT_PREC_FP64, 2048, 4, 4, 256),
-            (KT_PREC_MIXED, 0, 8, 1, 256), (KT_PREC_MIXED, 4096, 4, 4, 512), (KT_PREC_FP64, 0, 0, 1, 512)):
-        ctx.set_tuning(lanes=4, tile=1024, layout=KT_SPMV_ENTRY_BLOCKS, block_entries=entries, unroll=unroll,
-                       row_lanes=row_lanes, block_threads=threads)
+            (KT_PREC_MIXED, 0, 8, 1, 256), (KT_PREC_MIXED, 4096, 4, 4, 512), (KT_PREC_FP64, 0, 0, 1, 512),
+            (KT_PREC_FP64, 2048, -4, 4, 256), (KT_PREC_MIXED, 2048, -8, 4, 256)):
+        # unroll < 0: wide loads with |unroll| entries per thread and round
+        ctx.set_tuning(lanes=4, tile=1024, layout=KT_SPMV_ENTRY_BLOCKS, block_entries=entries, unroll=abs(unroll),
+                       row_lanes=row_lanes, block_threads=threads, wide=int(unroll < 0))
         x, y = np.zeros(n), np.zeros(n)
         r = ctx.solve_both(x, y, 1e-10, 100000, precision=prec, stop_rule=KT_STOP_TRUE)
         assert max(r["true_resid"]) <= 1e-10
@@ -429,6 +431,33 @@ def test_three_byte_columns_are_bit_identical(b2b12k, gpu_ctx_factory, precision
         for col_bytes, graphs in ((4, True), (3, True), (3, False)):
             ctx.set_tuning(lanes=4, tile=2048, layout=KT_SPMV_ENTRY_BLOCKS, block_threads=256, row_lanes=4,
                            col_bytes=col_bytes, use_graphs=graphs)
+            x, y = x0.copy(), y0.copy()
+            r = ctx.solve_both(x, y, 1e-9, 100000, precision=prec, stop_rule=KT_STOP_TRUE)
+            assert max(r["true_resid"]) <= 1e-9
+            out.append((x, y))
+        for x, y in out[1:]:
+            assert np.array_equal(x, out[0][0]) and np.array_equal(y, out[0][1])
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_wide_loads_are_bit_identical(b2b12k, gpu_ctx_factory, precision):
+    """Entry blocks reading four consecutive entries per thread by wide loads
+    (from each block's first entry rounded down to a multiple of four, with 3-
+    or 4-byte columns, 4 or 8 entries per round) give the same solution bit for
+    bit as one entry per load, through graph replay and direct launches, after
+    the systems are rebuilt."""
+    from ktplace_amd.simpl import KT_PREC_FP64, KT_PREC_MIXED, KT_SPMV_ENTRY_BLOCKS, KT_STOP_TRUE
+    prec = KT_PREC_FP64 if precision == "fp64" else KT_PREC_MIXED
+    d, o, x0, y0 = b2b12k
+    ctx = gpu_ctx_factory(d)
+    for positions in ((x0, y0), (x0[::-1].copy(), y0)):
+        ctx.build_b2b(*positions)
+        out = []
+        for wide, unroll, col_bytes, graphs in ((0, 4, 4, True), (1, 4, 3, True), (1, 8, 3, True), (1, 4, 4, True),
+                                               (1, 8, 4, False), (1, 4, 3, False)):
+            ctx.set_tuning(lanes=4, tile=2048, layout=KT_SPMV_ENTRY_BLOCKS, block_threads=256, row_lanes=4,
+                           unroll=unroll, col_bytes=col_bytes, use_graphs=graphs, wide=wide)
             x, y = x0.copy(), y0.copy()
             r = ctx.solve_both(x, y, 1e-9, 100000, precision=prec, stop_rule=KT_STOP_TRUE)
             assert max(r["true_resid"]) <= 1e-9
