@@ -467,7 +467,9 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
     static_assert(MODE == 0 || sizeof(T) == 8, "the residual modes run in FP64");
     constexpr int K = MODE == 0 ? 1 : 2;
     constexpr uint32_t kFlatEntries = E, kFlatSlack = flatSlack(E);
-    __shared__ double prod[kFlatEntries + kFlatSlack];
+    // W4 keeps product k of the quads at slot k (the block's entry j at
+    // j + lead), so each quad is two aligned 16-byte LDS stores
+    __shared__ alignas(16) double prod[kFlatEntries + kFlatSlack + (W4 ? 8 : 0)];
     int axis;
     uint32_t b;
     if (!decodeFlat(a, axis, b, parity)) return;
@@ -531,9 +533,29 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
         // NT / G2 rows).  Row 0 stands in for a missing row (rows > 0 here).
         constexpr uint32_t kGroups = NT / G2;
         const uint32_t lane = threadIdx.x % G2, grp = threadIdx.x / G2;
-        const uint32_t rq = R0 + grp < R1 ? R0 + grp : 0u;
-        const uint32_t pre0 = s.rowPtr[rq], pre1 = s.rowPtr[rq + 1];
-        const T preP = p[rq], preD = diag[rq];
+        // W4 also loads the row data of G2 passes (G2 * NT / G2 rows) with one
+        // load per array and wave: lane l of wave w holds row
+        // R0 + chunk + kGroups (l / kWaveGroups) + kWaveGroups w + l % kWaveGroups,
+        // which pass l / kWaveGroups gives to group l % kWaveGroups of the
+        // wave by a shuffle.  Every row keeps its lane group, so the sums and
+        // their order are those of one load per row and pass.
+        constexpr uint32_t kWaveGroups = 64 / G2;
+        const uint32_t lane64 = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+        uint32_t pre0, pre1;
+        T preP, preD;
+        const auto loadRows = [&](uint32_t r) {
+            const uint32_t rq = r < R1 ? r : 0u;
+            pre0 = s.rowPtr[rq];
+            pre1 = s.rowPtr[rq + 1];
+            preP = p[rq];
+            preD = diag[rq];
+        };
+        const auto chunkRow = [&](uint32_t chunk) {
+            return R0 + chunk * (kGroups * G2) + kGroups * (lane64 / kWaveGroups) + kWaveGroups * wave +
+                   lane64 % kWaveGroups;
+        };
+        if constexpr (W4) loadRows(chunkRow(0));
+        else loadRows(R0 + grp);
         __builtin_amdgcn_sched_barrier(0);
         if constexpr (W4) {
             // U / 4 quads per thread and round; quad q covers entries
@@ -555,7 +577,7 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
                     const uint32_t q = qb + u * NT + threadIdx.x;
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        const uint32_t k = 4u * q + j;  // entry A + k, slot k - lead of the block
+                        const uint32_t k = 4u * q + j;  // entry A + k, LDS slot k
                         const bool ok = q < quads && k >= lead && k < lead + count;
                         g[u][j] = gatherAt(p, ok ? c[u][j] : kNoCol);
                     }
@@ -563,12 +585,15 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int u = 0; u < Q; ++u) {
+                    // slots outside the block's entries get the dropped
+                    // entries' products, which no row reads
                     const uint32_t q = qb + u * NT + threadIdx.x;
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const uint32_t k = 4u * q + j;
-                        if (q < quads && k >= lead && k < lead + count)
-                            prod[k - lead] = static_cast<double>(w[u][j]) * static_cast<double>(g[u][j]);
+                    if (q < quads) {
+                        double2 *dst = reinterpret_cast<double2 *>(prod + 4u * q);
+                        dst[0] = make_double2(static_cast<double>(w[u][0]) * static_cast<double>(g[u][0]),
+                                              static_cast<double>(w[u][1]) * static_cast<double>(g[u][1]));
+                        dst[1] = make_double2(static_cast<double>(w[u][2]) * static_cast<double>(g[u][2]),
+                                              static_cast<double>(w[u][3]) * static_cast<double>(g[u][3]));
                     }
                 }
             }
@@ -600,21 +625,12 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
             }
         }
         __syncthreads();
-        bool first = true;
-        for (uint32_t r0 = R0; r0 < R1; r0 += kGroups) {
+        // the rows r0 + grp of one pass, with their row data
+        const uint32_t slot0 = W4 ? (E0 & ~3u) : E0;  // LDS slot of entry e: e - slot0
+        const auto pass = [&](uint32_t r0, uint32_t q0, uint32_t q1, T pr, T dr) {
             const uint32_t r = r0 + grp;
             const bool valid = r < R1;
-            uint32_t q0 = pre0, q1 = pre1;
-            T pr = preP, dr = preD;
-            if (!first) {
-                const uint32_t rr = valid ? r : 0u;
-                q0 = s.rowPtr[rr];
-                q1 = s.rowPtr[rr + 1];
-                pr = p[rr];
-                dr = diag[rr];
-            }
-            first = false;
-            const uint32_t e0 = q0 - E0, e1 = valid ? q1 - E0 : e0;
+            const uint32_t e0 = q0 - slot0, e1 = valid ? q1 - slot0 : e0;
             const double pd = static_cast<double>(pr);
             if constexpr (G2 == 1) {
                 // d p first, then the row's products in entry order
@@ -630,6 +646,24 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
 #pragma unroll
                 for (int off = G2 / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, G2);
                 if (valid && lane == 0) finish(r, pd, static_cast<double>(dr) * pd + part);
+            }
+        };
+        if constexpr (W4) {
+            uint32_t r0 = R0;
+            for (uint32_t chunk = 0; r0 < R1; ++chunk) {
+                if (chunk > 0) loadRows(chunkRow(chunk));
+#pragma unroll
+                for (uint32_t k = 0; k < G2; ++k, r0 += kGroups) {
+                    if (r0 >= R1) break;
+                    const int src = static_cast<int>(k * kWaveGroups + lane64 / G2);
+                    pass(r0, __shfl(pre0, src), __shfl(pre1, src), __shfl(preP, src), __shfl(preD, src));
+                }
+            }
+        } else {
+            if (R0 < R1) pass(R0, pre0, pre1, preP, preD);
+            for (uint32_t r0 = R0 + kGroups; r0 < R1; r0 += kGroups) {
+                loadRows(r0 + grp);
+                pass(r0, pre0, pre1, preP, preD);
             }
         }
     }
