@@ -499,6 +499,7 @@ int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY) {
         s.nnzHost = -1;
         s.have32 = false;
         s.have24 = false;
+        s.havePk = false;
     }
     a.floor[0] = floorX;
     a.floor[1] = floorY;

@@ -608,6 +608,7 @@ int kt_set_system(kt_ctx *ctx, int axis, uint64_t rows, const uint64_t *row_ptr,
         s.nnzHost = static_cast<int64_t>(nnz);
         s.have32 = false;
         s.have24 = false;
+        s.havePk = false;
         return KT_OK;
     })
 }

@@ -355,17 +355,18 @@ __global__ void __launch_bounds__(kBlock) cgSpmv(CgArgs a, int parity) {
 // slack is an eighth of the block, at most 512 entries, so 2048-entry blocks
 // of 256 threads fit eight to a CU's LDS.
 // Four consecutive entries k0 .. k0 + 3 (k0 a multiple of four): columns
-// and values by one wide load per array.
+// and values by one wide load per array (3-byte columns: one 12-byte load of
+// the packed quad).
 template <typename T, bool C24>
 __device__ __forceinline__ void loadQuad(const SysView &s, const T *__restrict__ val, uint32_t k0, uint32_t (&c)[4],
                                          T (&w)[4]) {
     if constexpr (C24) {
-        const uint2 lo = *reinterpret_cast<const uint2 *>(s.colLo + k0);
-        const uint32_t hi = *reinterpret_cast<const uint32_t *>(s.colHi + k0);
-        c[0] = (lo.x & 0xffffu) | ((hi & 0xffu) << 16);
-        c[1] = (lo.x >> 16) | (((hi >> 8) & 0xffu) << 16);
-        c[2] = (lo.y & 0xffffu) | (((hi >> 16) & 0xffu) << 16);
-        c[3] = (lo.y >> 16) | ((hi >> 24) << 16);
+        const uint32_t *w = s.colPk + 3u * (k0 >> 2);
+        const uint32_t w0 = w[0], w1 = w[1], w2 = w[2];
+        c[0] = w0 & 0xffffffu;
+        c[1] = (w0 >> 24) | ((w1 & 0xffffu) << 8);
+        c[2] = (w1 >> 16) | ((w2 & 0xffu) << 16);
+        c[3] = w2 >> 8;
     } else {
         const uint4 q = *reinterpret_cast<const uint4 *>(s.col + k0);
         c[0] = q.x;
@@ -994,6 +995,22 @@ splitColumns(const uint32_t *__restrict__ col, uint16_t *__restrict__ lo, uint8_
     }
 }
 
+// Columns (all below 2^24) packed four to three words: quad q's columns
+// c0 .. c3 become c0 | c1 << 24, c1 >> 8 | c2 << 16, c2 >> 16 | c3 << 8.
+// Entries past n pack as 0.
+__global__ void __launch_bounds__(kBlock)
+packColumns(const uint32_t *__restrict__ col, uint32_t *__restrict__ pk, uint64_t n) {
+    const uint64_t quads = (n + 3) / 4;
+    for (uint64_t q = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; q < quads;
+         q += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        uint32_t c[4];
+        for (int j = 0; j < 4; ++j) c[j] = 4 * q + j < n ? col[4 * q + j] : 0u;
+        pk[3 * q] = c[0] | (c[1] << 24);
+        pk[3 * q + 1] = (c[1] >> 8) | (c[2] << 16);
+        pk[3 * q + 2] = (c[2] >> 16) | (c[3] << 8);
+    }
+}
+
 // FP32 copies of the matrix values and diagonal (mixed precision).
 __global__ void __launch_bounds__(kBlock)
 toFloat(const double *__restrict__ in, float *__restrict__ out, uint64_t n) {
@@ -1175,7 +1192,10 @@ template <int MODE>
 void launchFlatResidual(kt_ctx *ctx, const CgArgs &a) {
     const dim3 fg(flatGrid(a));
     hipStream_t st = ctx->stream;
-    if (a.col24) {
+    if (a.wide) {  // wide loads imply 4 row lanes
+        if (a.col24) cgSpmvFlat<double, 2048, 4, 4, 256, true, MODE, true><<<fg, dim3(256), 0, st>>>(a, -1);
+        else cgSpmvFlat<double, 2048, 4, 4, 256, false, MODE, true><<<fg, dim3(256), 0, st>>>(a, -1);
+    } else if (a.col24) {
         if (a.flatRowLanes == 4) cgSpmvFlat<double, 2048, 4, 4, 256, true, MODE><<<fg, dim3(256), 0, st>>>(a, -1);
         else cgSpmvFlat<double, 2048, 4, 1, 256, true, MODE><<<fg, dim3(256), 0, st>>>(a, -1);
     } else {
@@ -1236,6 +1256,7 @@ SysView viewOf(kt_ctx *ctx, int axis, uint32_t tile) {
     v.flatInfo = nullptr;
     v.colLo = nullptr;
     v.colHi = nullptr;
+    v.colPk = nullptr;
     v.sblocks = v.blocks;
     return v;
 }
@@ -1400,7 +1421,18 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
                                                                           sb, E, info);
             KT_HIP(hipGetLastError());
             a.sys[ax].flatInfo = info;
-            if (a.col24) {
+            if (a.col24 && a.wide) {
+                if (!sy.havePk) {
+                    // one spare quad: a block's loads may reach the quad after nnz
+                    KT_TRY(sy.colPk.reserve(3 * (nnz[ax] / 4 + 2)));
+                    if (nnz[ax]) {
+                        packColumns<<<dim3(1024), dim3(kBlock), 0, ctx->stream>>>(sy.col.p, sy.colPk.p, nnz[ax]);
+                        KT_HIP(hipGetLastError());
+                    }
+                    sy.havePk = true;
+                }
+                a.sys[ax].colPk = sy.colPk.p;
+            } else if (a.col24) {
                 if (!sy.have24) {
                     KT_TRY(sy.colLo.reserve(nnz[ax] + kQuadPad));
                     KT_TRY(sy.colHi.reserve(nnz[ax] + kQuadPad));

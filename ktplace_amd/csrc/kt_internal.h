@@ -118,6 +118,9 @@ struct SysView {
     // entry-block SpMV with 3-byte columns: col = colLo | colHi << 16
     const uint16_t *colLo;
     const uint8_t *colHi;
+    // entry-block SpMV with wide loads and 3-byte columns: the columns of
+    // entries 4q .. 4q + 3 packed little-endian into words 3q .. 3q + 2
+    const uint32_t *colPk;
 };
 
 struct Sys {
@@ -131,6 +134,8 @@ struct Sys {
     DevBuf<uint16_t> colLo;  // 3-byte columns: low 16 bits
     DevBuf<uint8_t> colHi;   // 3-byte columns: high 8 bits
     bool have24 = false;
+    DevBuf<uint32_t> colPk;  // 3-byte columns packed four to three words (wide loads)
+    bool havePk = false;
     DevBuf<uint32_t> rowStart;  // entry-block SpMV: (first row, end row, first entry, end entry) per block
 };
 

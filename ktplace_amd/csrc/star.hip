@@ -212,6 +212,7 @@ int buildStar(kt_ctx *ctx, uint64_t *sizeOut) {
         s.nnzHost = -1;
         s.have32 = false;
         s.have24 = false;
+        s.havePk = false;
     }
     a.n = n;
     a.size = static_cast<uint32_t>(size);
