@@ -426,10 +426,11 @@ int kt_ctx_destroy(kt_ctx *ctx) {
         ctx->r32[a].release(); ctx->p32[a].release(); ctx->ap32[a].release(); ctx->x32[a].release();
     }
     ctx->partials.release(); ctx->ctl.release(); ctx->history.release();
-    for (auto &g : ctx->graphs) {
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
-        if (g.graph) (void)hipGraphDestroy(g.graph);
-    }
+    for (auto &byPrecision : ctx->graphs)
+        for (auto &g : byPrecision) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            if (g.graph) (void)hipGraphDestroy(g.graph);
+        }
     if (ctx->ctlHost) (void)hipHostFree(ctx->ctlHost);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->loopHost) (void)hipHostFree(ctx->loopHost);

@@ -66,6 +66,16 @@ __device__ __forceinline__ bool decode(const CgArgs &a, int &axis, uint32_t &til
         tile = (b >> 3) * 4u + (x & 3u);
         return tile < a.sys[axis].blocks;
     }
+    // inside an iteration with one axis live, the first blocks serve it and
+    // the rest exit, so the empty blocks are dispatched after the working ones
+    if (parity >= 0) {
+        const bool lx = a.ctl[0].s[parity].live, ly = a.ctl[1].s[parity].live;
+        if (lx != ly) {
+            axis = lx ? 0 : 1;
+            tile = blockIdx.x;
+            return tile < a.sys[axis].blocks;
+        }
+    }
     axis = blockIdx.x >= a.sys[0].blocks ? 1 : 0;
     tile = blockIdx.x - (axis ? a.sys[0].blocks : 0);
     return true;
@@ -435,6 +445,14 @@ __device__ __forceinline__ bool decodeFlat(const CgArgs &a, int &axis, uint32_t 
         b = (id >> 3) * 4u + (x & 3u);
         return b < a.sys[axis].sblocks;
     }
+    if (parity >= 0) {  // one axis live: the first blocks serve it (as decode)
+        const bool lx = a.ctl[0].s[parity].live, ly = a.ctl[1].s[parity].live;
+        if (lx != ly) {
+            axis = lx ? 0 : 1;
+            b = blockIdx.x;
+            return b < a.sys[axis].sblocks;
+        }
+    }
     axis = blockIdx.x >= a.sys[0].sblocks ? 1 : 0;
     b = blockIdx.x - (axis ? a.sys[0].sblocks : 0);
     return true;
@@ -443,7 +461,11 @@ __device__ __forceinline__ bool decodeFlat(const CgArgs &a, int &axis, uint32_t 
 // Threads per entry block (0 picks 1024).
 int flatThreads(int tuned) { return tuned == 256 || tuned == 512 ? tuned : kBlock; }
 
-uint32_t flatGrid(const CgArgs &a) {
+// only = 0 or 1: an iteration launched while that axis alone is live gets
+// just its blocks (the kernels map every block to the one live axis), so no
+// workgroups of the stopped axis are dispatched; -1: both axes.
+uint32_t flatGrid(const CgArgs &a, int only = -1) {
+    if (only >= 0) return std::max(a.sys[only].sblocks, 1u);
     if (a.xcdSplit == 1) return 8u * ((std::max(a.sys[0].sblocks, a.sys[1].sblocks) + 3u) / 4u);
     return a.sys[0].sblocks + a.sys[1].sblocks;
 }
@@ -1030,7 +1052,8 @@ multiplyKernel(SysView s, const double *__restrict__ in, double *__restrict__ ou
     out[i] = sum;
 }
 
-uint32_t gridBlocks(const CgArgs &a) {
+uint32_t gridBlocks(const CgArgs &a, int only = -1) {
+    if (only >= 0) return std::max(a.sys[only].blocks, 1u);
     if (a.xcdSplit == 1) return 8u * ((std::max(a.sys[0].blocks, a.sys[1].blocks) + 3u) / 4u);
     return a.sys[0].blocks + a.sys[1].blocks;
 }
@@ -1048,10 +1071,10 @@ uint32_t gridBlocks(const CgArgs &a) {
 // the instance's dynamic LDS limit to its window (done before any capture).
 template <typename T>
 int launchSpmv(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, bool attributesOnly,
-               hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+               hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, int only = -1) {
     if (a.layout == 1) {
         if (attributesOnly) return KT_OK;
-        const dim3 fg(flatGrid(a));
+        const dim3 fg(flatGrid(a, only));
         const bool u8 = a.flatUnroll == 8;
         const bool lanes4 = a.flatRowLanes == 4;
         const int nt = static_cast<int>(a.flatThreads);
@@ -1102,7 +1125,7 @@ int launchSpmv(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, 
 #undef KT_FLAT
     }
     const int R = ctx->tune.rows > 0 ? ctx->tune.rows : 2;
-    const dim3 grid(gridBlocks(a));
+    const dim3 grid(gridBlocks(a, only));
     const bool staged = a.halo > 0;
     const uint32_t lds = staged ? static_cast<uint32_t>(windowBytes<T>(a.tile, a.halo)) : 0u;
     const auto go = [&](auto kernel) -> int {
@@ -1133,17 +1156,17 @@ int launchSpmv(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, 
 }
 
 template <typename T>
-int launchIteration(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, bool timed) {
-    const dim3 grid(gridBlocks(a));
+int launchIteration(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, bool timed, int only) {
+    const dim3 grid(gridBlocks(a, only));
     hipEvent_t t0 = nullptr;
     const int kSpmv = sizeof(T) == 8 ? KT_KERNEL_SPMV : KT_KERNEL_SPMV32;
     if (timed && a.layout == 1) {
         hipEvent_t e0, e1;
         KT_TRY(timedKernelEvents(ctx, kSpmv, &e0, &e1));
-        KT_TRY(launchSpmv<T>(ctx, a, G, parity, st, false, e0, e1));
+        KT_TRY(launchSpmv<T>(ctx, a, G, parity, st, false, e0, e1, only));
     } else {
         if (timed) KT_TRY(timedBegin(ctx, kSpmv, &t0));
-        KT_TRY(launchSpmv<T>(ctx, a, G, parity, st, false));
+        KT_TRY(launchSpmv<T>(ctx, a, G, parity, st, false, nullptr, nullptr, only));
         if (timed) KT_TRY(timedEnd(ctx, kSpmv, t0));
     }
     if (timed) KT_TRY(timedBegin(ctx, KT_KERNEL_UPDATE, &t0));
@@ -1263,18 +1286,20 @@ SysView viewOf(kt_ctx *ctx, int axis, uint32_t tile) {
 
 // Iteration pairs (parity 0 then 1); without timing they replay a cached
 // hipGraph of GraphCache::kPairs pairs.
-int runPairs(kt_ctx *ctx, const CgArgs &a, int G, bool fp32, uint32_t pairs, bool timed) {
+// only: the one live axis (0, 1) when the other has stopped, else -1; each
+// case has its own graph.
+int runPairs(kt_ctx *ctx, const CgArgs &a, int G, bool fp32, uint32_t pairs, bool timed, int only) {
     const auto direct = [&](uint32_t count) -> int {
         for (uint32_t k = 0; k < count; ++k)
             for (int parity = 0; parity < 2; ++parity) {
-                const int rc = fp32 ? launchIteration<float>(ctx, a, G, parity, ctx->stream, timed)
-                                    : launchIteration<double>(ctx, a, G, parity, ctx->stream, timed);
+                const int rc = fp32 ? launchIteration<float>(ctx, a, G, parity, ctx->stream, timed, only)
+                                    : launchIteration<double>(ctx, a, G, parity, ctx->stream, timed, only);
                 if (rc != KT_OK) return rc;
             }
         return KT_OK;
     };
     if (timed || !ctx->tune.useGraphs) return direct(pairs);
-    GraphCache &gc = ctx->graphs[fp32 ? 1 : 0];
+    GraphCache &gc = ctx->graphs[fp32 ? 1 : 0][only + 1];
     if (!gc.exec || std::memcmp(&gc.args, &a, sizeof(CgArgs)) != 0 || gc.lanes != G || gc.rows != ctx->tune.rows) {
         if (gc.exec) (void)hipGraphExecDestroy(gc.exec);
         if (gc.graph) (void)hipGraphDestroy(gc.graph);
@@ -1284,8 +1309,8 @@ int runPairs(kt_ctx *ctx, const CgArgs &a, int G, bool fp32, uint32_t pairs, boo
         int rc = KT_OK;
         for (int rep = 0; rep < GraphCache::kPairs && rc == KT_OK; ++rep)
             for (int parity = 0; parity < 2 && rc == KT_OK; ++parity)
-                rc = fp32 ? launchIteration<float>(ctx, a, G, parity, ctx->stream, false)
-                          : launchIteration<double>(ctx, a, G, parity, ctx->stream, false);
+                rc = fp32 ? launchIteration<float>(ctx, a, G, parity, ctx->stream, false, only)
+                          : launchIteration<double>(ctx, a, G, parity, ctx->stream, false, only);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
         if (rc != KT_OK) {
@@ -1302,6 +1327,15 @@ int runPairs(kt_ctx *ctx, const CgArgs &a, int G, bool fp32, uint32_t pairs, boo
     uint32_t k = 0;
     for (; k + GraphCache::kPairs <= pairs; k += GraphCache::kPairs) KT_HIP(hipGraphLaunch(gc.exec, ctx->stream));
     return direct(pairs - k);
+}
+
+// The one live axis when the other is stopped or paused, else -1.  Between
+// two polls an axis can only stop or pause (a refresh, which may make it live
+// again, runs at a poll), so a chunk launched for one axis never misses the
+// other.
+int liveOnly(const AxisCtl *c) {
+    const bool lx = c[0].s[0].live, ly = c[1].s[0].live;
+    return lx == ly ? -1 : (lx ? 0 : 1);
 }
 
 int readCtl(kt_ctx *ctx) {
@@ -1521,7 +1555,7 @@ int solveFp64(kt_ctx *ctx, const kt_solve_opts &o, kt_solve_result &res) {
                 prevSteps[ax] = c[ax].s[0].steps;
             }
             first = false;
-            KT_TRY(runPairs(ctx, a, G, false, chunk / 2, timed));
+            KT_TRY(runPairs(ctx, a, G, false, chunk / 2, timed, liveOnly(c)));
         }
         KT_TRY(launchTrueResidual(ctx, a, G));
         KT_TRY(readCtl(ctx));
@@ -1582,7 +1616,7 @@ int solveMixed(kt_ctx *ctx, const kt_solve_opts &o, kt_solve_result &res) {
             prevSteps[ax] = c[ax].s[0].steps;
         }
         first = false;
-        KT_TRY(runPairs(ctx, a, G, true, chunk / 2, timed));
+        KT_TRY(runPairs(ctx, a, G, true, chunk / 2, timed, liveOnly(c)));
     }
     // fold the remaining correction into x and report the true residual
     for (int ax = 0; ax < 2; ++ax) ctx->ctlHost[ax].needUpdate = 1;

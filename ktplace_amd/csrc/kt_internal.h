@@ -310,7 +310,7 @@ struct kt_ctx {
     std::vector<double> exactHistory;    // its residuals per iteration
 
     kt::Tune tune;
-    kt::GraphCache graphs[2];  // FP64, FP32 iteration graphs
+    kt::GraphCache graphs[2][3];  // FP64, FP32 iteration graphs; both axes, x only, y only
 
     // timing
     bool profile = false;
