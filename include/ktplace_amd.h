@@ -389,6 +389,10 @@ int kt_batch_set_layout(kt_batch *b, int layout, int row_lanes);
  * 3 (16 + 8 bits, as kt_set_spmv_columns) for batches of fewer than 2^24
  * rows.  The results are the same either way. */
 int kt_batch_set_columns(kt_batch *b, int bytes);
+/* 1: the batch's entry-block SpMV with 4 row lanes reads four consecutive
+ * entries per thread with one wide load per array (as kt_set_spmv_wide_loads);
+ * 0 (default): one entry per load.  The results are the same either way. */
+int kt_batch_set_wide_loads(kt_batch *b, int on);
 /* Every instance back to the start positions it was added with. */
 int kt_batch_reset(kt_batch *b);
 /* Solves all instances from their current positions (opts: KT_PREC_FP64,

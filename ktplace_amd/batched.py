@@ -67,14 +67,15 @@ def autotune(batch: simpl.Batch, iterations: int = 48):
     iterations of the whole batch."""
     table = []
 
-    def timed(g, t, r, layout=0, row_lanes=0, col_bytes=0):
-        batch.set_tuning(lanes=g, tile=t, rows=r, layout=layout, row_lanes=row_lanes, col_bytes=col_bytes)
+    def timed(g, t, r, layout=0, row_lanes=0, col_bytes=0, wide=0):
+        batch.set_tuning(lanes=g, tile=t, rows=r, layout=layout, row_lanes=row_lanes, col_bytes=col_bytes,
+                         wide=wide)
         batch.reset()
         batch.solve(0.0, iterations)  # warm-up (captures the graph)
         batch.reset()
         t0 = time.perf_counter()
         batch.solve(0.0, iterations)
-        table.append(((time.perf_counter() - t0) * 1e3, g, t, r, layout, row_lanes, col_bytes))
+        table.append(((time.perf_counter() - t0) * 1e3, g, t, r, layout, row_lanes, col_bytes, wide))
 
     for g in LANES:
         for t in TILES:
@@ -83,13 +84,15 @@ def autotune(batch: simpl.Batch, iterations: int = 48):
     for t in TILES:  # entry blocks: lanes and rows do not apply; the tile shapes the vector kernels
         for row_lanes in (1, 4):
             for col_bytes in (4, 3):
-                timed(4, t, 2, simpl.KT_SPMV_ENTRY_BLOCKS, row_lanes, col_bytes)
+                for wide in ((0, 1) if row_lanes == 4 else (0,)):  # wide loads need 4 row lanes
+                    timed(4, t, 2, simpl.KT_SPMV_ENTRY_BLOCKS, row_lanes, col_bytes, wide)
     table.sort()
-    ms, g, t, r, layout, row_lanes, col_bytes = table[0]
-    batch.set_tuning(lanes=g, tile=t, rows=r, layout=layout, row_lanes=row_lanes, col_bytes=col_bytes)
+    ms, g, t, r, layout, row_lanes, col_bytes, wide = table[0]
+    batch.set_tuning(lanes=g, tile=t, rows=r, layout=layout, row_lanes=row_lanes, col_bytes=col_bytes, wide=wide)
     return dict(lanes_per_row=g, tile_rows=t, rows_in_flight=r,
                 spmv_layout="entry blocks" if layout else "row groups", row_lanes=row_lanes,
-                column_bytes=(col_bytes if layout else None), ms_per_48_iterations=round(ms, 3))
+                column_bytes=(col_bytes if layout else None), wide_loads=(bool(wide) if layout else None),
+                ms_per_48_iterations=round(ms, 3))
 
 
 def percentiles(v):

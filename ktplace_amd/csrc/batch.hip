@@ -46,8 +46,9 @@ struct BArgs {
     uint32_t layout;                // 0 lane groups, 1 entry blocks
     const uint16_t *colLo;          // entry blocks with 3-byte columns: col = colLo | colHi << 16
     const uint8_t *colHi;
+    const uint32_t *colPk;          // wide loads with 3-byte columns: quad q's columns in words 3q .. 3q + 2
     uint32_t col24;
-    uint32_t pad;
+    uint32_t wide;                  // entry blocks, 4 row lanes: four consecutive entries per load
 };
 
 struct Tile {
@@ -215,22 +216,70 @@ bSplitColumns(const uint32_t *__restrict__ col, uint16_t *__restrict__ lo, uint8
     }
 }
 
+// Columns (all below 2^24) packed four to three words, as the two-system
+// solver's packColumns: quad q's c0 .. c3 become c0 | c1 << 24,
+// c1 >> 8 | c2 << 16, c2 >> 16 | c3 << 8; entries past n pack as 0.
+__global__ void __launch_bounds__(kBB)
+bPackColumns(const uint32_t *__restrict__ col, uint32_t *__restrict__ pk, uint64_t n) {
+    const uint64_t quads = (n + 3) / 4;
+    for (uint64_t q = blockIdx.x * static_cast<uint64_t>(kBB) + threadIdx.x; q < quads;
+         q += static_cast<uint64_t>(gridDim.x) * kBB) {
+        uint32_t c[4];
+        for (int j = 0; j < 4; ++j) c[j] = 4 * q + j < n ? col[4 * q + j] : 0u;
+        pk[3 * q] = c[0] | (c[1] << 24);
+        pk[3 * q + 1] = (c[1] >> 8) | (c[2] << 16);
+        pk[3 * q + 2] = (c[2] >> 16) | (c[3] << 8);
+    }
+}
+
 // Entry-block SpMV over the batch (the two-system solver's cgSpmvFlat, per
 // system): block b takes rows [R0, R1) of system S, whose entries are a run
 // of about kBFlat consecutive entries; the threads read the run's columns and
 // values coalesced, gather p and keep the products in LDS, then G2 lanes per
 // row add the row's products (G2 = 1: one thread, entry order after the
 // diagonal term).  A run longer than the LDS holds takes lane groups.
+// W4 (G2 = 4): four consecutive entries per thread by one wide load per array
+// from the run's first entry rounded down to a multiple of four (the arrays
+// carry four spare entries), the products at their slot from that base, and
+// the row data of four passes by one load per array and wave, handed to the
+// lane groups by shuffles; the sums and their order are unchanged.
 constexpr uint32_t kBFlat = 2048;
 constexpr uint32_t kBFlatSlack = 256;  // an eighth of the block: eight blocks fit a CU's LDS
 
-template <int G2, bool C24>
+template <bool C24>
+__device__ __forceinline__ void bLoadQuad(const BArgs &a, uint32_t k0, uint32_t (&c)[4], double (&w)[4]) {
+    if constexpr (C24) {
+        const uint32_t *pk = a.colPk + 3u * (k0 >> 2);
+        const uint32_t w0 = pk[0], w1 = pk[1], w2 = pk[2];
+        c[0] = w0 & 0xffffffu;
+        c[1] = (w0 >> 24) | ((w1 & 0xffffu) << 8);
+        c[2] = (w1 >> 16) | ((w2 & 0xffu) << 16);
+        c[3] = w2 >> 8;
+    } else {
+        const uint4 q = *reinterpret_cast<const uint4 *>(a.col + k0);
+        c[0] = q.x;
+        c[1] = q.y;
+        c[2] = q.z;
+        c[3] = q.w;
+    }
+    const double2 v0 = *reinterpret_cast<const double2 *>(a.val + k0);
+    const double2 v1 = *reinterpret_cast<const double2 *>(a.val + k0 + 2);
+    w[0] = v0.x;
+    w[1] = v0.y;
+    w[2] = v1.x;
+    w[3] = v1.y;
+}
+
+template <int G2, bool C24, bool W4 = false>
 __global__ void __launch_bounds__(kBB) bSpmvFlat(BArgs a, int parity) {
-    __shared__ double prod[kBFlat + kBFlatSlack];
+    static_assert(!W4 || G2 == 4, "wide loads run with 4 lanes per row");
+    __shared__ alignas(16) double prod[kBFlat + kBFlatSlack + (W4 ? 8 : 0)];
     const uint4 f = a.flat[blockIdx.x];
     const uint2 fe = a.flatE[blockIdx.x];
     const uint32_t S = f.x, R0 = f.y, R1 = f.z;
     if (!a.ctl[S].s[parity].live) return;
+    const double *__restrict__ p = a.p;
+    double *__restrict__ ap = a.ap;
     const uint32_t E0 = fe.x, count = fe.y - fe.x;
     double acc[1] = {0.0};
     if (count > kBFlat + kBFlatSlack) {
@@ -245,13 +294,13 @@ __global__ void __launch_bounds__(kBB) bSpmvFlat(BArgs a, int parity) {
                 row[q] = r0 + grp + q * (kBB / G);
                 valid[q] = row[q] < R1;
             }
-            rowsDot<double, G, R>(a.rowPtr, a.col, a.val, a.p, row, valid, lane, off);
+            rowsDot<double, G, R>(a.rowPtr, a.col, a.val, p, row, valid, lane, off);
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 if (valid[q] && lane == 0) {
-                    const double pd = a.p[row[q]];
+                    const double pd = p[row[q]];
                     const double v = a.diag[row[q]] * pd + off[q];
-                    a.ap[row[q]] = v;
+                    ap[row[q]] = v;
                     acc[0] += pd * v;
                 }
             }
@@ -261,18 +310,49 @@ __global__ void __launch_bounds__(kBB) bSpmvFlat(BArgs a, int parity) {
         // (a block of kBFlat entries usually has fewer rows than groups)
         constexpr uint32_t kGroups = kBB / G2;
         const uint32_t lane = threadIdx.x % G2, grp = threadIdx.x / G2;
+        constexpr uint32_t kWaveGroups = 64 / G2;
+        const uint32_t lane64 = threadIdx.x & 63u, wave = threadIdx.x >> 6;
         // (an instance without rows has one block with R0 == R1 and no row to read)
         uint32_t pre0 = 0, pre1 = 0;
         double preP = 0.0, preD = 0.0;
-        if (R0 < R1) {
-            const uint32_t rq = R0 + grp < R1 ? R0 + grp : R0;
+        const auto loadRows = [&](uint32_t r) {
+            const uint32_t rq = r < R1 ? r : R0;
             pre0 = a.rowPtr[rq];
             pre1 = a.rowPtr[rq + 1];
-            preP = a.p[rq];
+            preP = p[rq];
             preD = a.diag[rq];
+        };
+        // W4: lane l of wave w holds row R0 + chunk + kGroups (l / kWaveGroups)
+        // + kWaveGroups w + l % kWaveGroups (pass l / kWaveGroups)
+        const auto chunkRow = [&](uint32_t chunk) {
+            return R0 + chunk * (kGroups * G2) + kGroups * (lane64 / kWaveGroups) + kWaveGroups * wave +
+                   lane64 % kWaveGroups;
+        };
+        if (R0 < R1) {
+            if constexpr (W4) loadRows(chunkRow(0));
+            else loadRows(R0 + grp);
         }
         __builtin_amdgcn_sched_barrier(0);
         constexpr int U = 4;
+        if constexpr (W4) {
+            const uint32_t A = E0 & ~3u, lead = E0 - A, quads = (lead + count + 3) / 4;
+            for (uint32_t q = threadIdx.x; q < quads; q += kBB) {
+                uint32_t c[4];
+                double w[4];
+                bLoadQuad<C24>(a, A + 4u * q, c, w);
+                __builtin_amdgcn_sched_barrier(0);
+                double g[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t k = 4u * q + j;
+                    g[j] = gatherAt(p, k >= lead && k < lead + count ? c[j] : kNoCol);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                double2 *dst = reinterpret_cast<double2 *>(prod + 4u * q);
+                dst[0] = make_double2(w[0] * g[0], w[1] * g[1]);
+                dst[1] = make_double2(w[2] * g[2], w[3] * g[3]);
+            }
+        } else
         for (uint32_t base = 0; base < count; base += U * kBB) {
             uint32_t c[U];
             double w[U];
@@ -291,7 +371,7 @@ __global__ void __launch_bounds__(kBB) bSpmvFlat(BArgs a, int parity) {
             __builtin_amdgcn_sched_barrier(0);
             double g[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) g[u] = gatherAt(a.p, c[u]);
+            for (int u = 0; u < U; ++u) g[u] = gatherAt(p, c[u]);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -300,21 +380,11 @@ __global__ void __launch_bounds__(kBB) bSpmvFlat(BArgs a, int parity) {
             }
         }
         __syncthreads();
-        bool first = true;
-        for (uint32_t r0 = R0; r0 < R1; r0 += kGroups) {
+        const uint32_t slot0 = W4 ? (E0 & ~3u) : E0;  // LDS slot of entry e: e - slot0
+        const auto pass = [&](uint32_t r0, uint32_t q0, uint32_t q1, double pr, double dr) {
             const uint32_t r = r0 + grp;
             const bool valid = r < R1;
-            uint32_t q0 = pre0, q1 = pre1;
-            double pr = preP, dr = preD;
-            if (!first) {
-                const uint32_t rr = valid ? r : R0;
-                q0 = a.rowPtr[rr];
-                q1 = a.rowPtr[rr + 1];
-                pr = a.p[rr];
-                dr = a.diag[rr];
-            }
-            first = false;
-            const uint32_t e0 = q0 - E0, e1 = valid ? q1 - E0 : e0;
+            const uint32_t e0 = q0 - slot0, e1 = valid ? q1 - slot0 : e0;
             const double pd = valid ? pr : 0.0;
             double part = G2 == 1 ? dr * pd : 0.0;
             for (uint32_t k = e0 + lane; k < e1; k += G2) part += prod[k];
@@ -322,8 +392,26 @@ __global__ void __launch_bounds__(kBB) bSpmvFlat(BArgs a, int parity) {
             for (int off = G2 / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, G2);
             if (valid && lane == 0) {
                 const double v = G2 == 1 ? part : dr * pd + part;
-                a.ap[r] = v;
+                ap[r] = v;
                 acc[0] += pd * v;
+            }
+        };
+        if constexpr (W4) {
+            uint32_t r0 = R0;
+            for (uint32_t chunk = 0; r0 < R1; ++chunk) {
+                if (chunk > 0) loadRows(chunkRow(chunk));
+#pragma unroll
+                for (uint32_t k = 0; k < G2; ++k, r0 += kGroups) {
+                    if (r0 >= R1) break;
+                    const int src = static_cast<int>(k * kWaveGroups + lane64 / G2);
+                    pass(r0, __shfl(pre0, src), __shfl(pre1, src), __shfl(preP, src), __shfl(preD, src));
+                }
+            }
+        } else {
+            if (R0 < R1) pass(R0, pre0, pre1, preP, preD);
+            for (uint32_t r0 = R0 + kGroups; r0 < R1; r0 += kGroups) {
+                loadRows(r0 + grp);
+                pass(r0, pre0, pre1, preP, preD);
             }
         }
     }
@@ -331,11 +419,32 @@ __global__ void __launch_bounds__(kBB) bSpmvFlat(BArgs a, int parity) {
     writePartial<1>(a.partA, acc);
 }
 
+// Rows per thread whose vector entries the update kernels load before the
+// partial sums (they do not depend on alpha or beta), so the sums hide their
+// latency; rows beyond kPre * kBB of a tile are loaded after.
+constexpr int kPre = 4;
+
 __global__ void __launch_bounds__(kBB) bUpdate(BArgs a, int parity) {
     const Tile t = decode(a);
     AxisCtl &c = a.ctl[t.sys];
     const CgState st = c.s[parity];
     if (!st.live) return;
+    double *__restrict__ x = a.x;
+    double *__restrict__ r = a.r;
+    const double *__restrict__ p = a.p;
+    const double *__restrict__ ap = a.ap;
+    const double *__restrict__ dg = a.diag;
+    double xv[kPre], pv[kPre], rv[kPre], av[kPre], dv[kPre];
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+        const uint32_t i = t.begin + threadIdx.x + k * kBB;
+        const uint32_t j = i < t.end ? i : t.begin;  // unconditional loads; a tile has a row
+        xv[k] = x[j];
+        pv[k] = p[j];
+        rv[k] = r[j];
+        av[k] = ap[j];
+        dv[k] = dg[j];
+    }
     double pap[1];
     const BSysDev d = a.sys[t.sys];
     if (a.layout) sumPartials<1>(a.partA, d.flatBase, d.flatBlocks, pap);
@@ -347,13 +456,19 @@ __global__ void __launch_bounds__(kBB) bUpdate(BArgs a, int parity) {
     if (t.first && threadIdx.x == 0) c.brk[parity] = 0;
     const double alpha = st.rho / pap[0];
     double acc[2] = {0.0, 0.0};
-    for (uint32_t i = t.begin + threadIdx.x; i < t.end; i += kBB) {
-        a.x[i] += alpha * a.p[i];
-        const double ri = a.r[i] - alpha * a.ap[i];
-        a.r[i] = ri;
+    const auto step = [&](uint32_t i, double xi, double pi, double ri0, double ai, double di) {
+        x[i] = xi + alpha * pi;
+        const double ri = ri0 - alpha * ai;
+        r[i] = ri;
         acc[0] += ri * ri;
-        acc[1] += ri * (ri / a.diag[i]);
+        acc[1] += ri * (ri / di);
+    };
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+        const uint32_t i = t.begin + threadIdx.x + k * kBB;
+        if (i < t.end) step(i, xv[k], pv[k], rv[k], av[k], dv[k]);
     }
+    for (uint32_t i = t.begin + threadIdx.x + kPre * kBB; i < t.end; i += kBB) step(i, x[i], p[i], r[i], ap[i], dg[i]);
     blockSum<2>(acc);
     writePartial<2>(a.partB, acc);
 }
@@ -377,6 +492,18 @@ __global__ void __launch_bounds__(kBB) bPUpdate(BArgs a, int parity) {
         }
         return;
     }
+    double *__restrict__ p = a.p;
+    const double *__restrict__ r = a.r;
+    const double *__restrict__ dg = a.diag;
+    double rv[kPre], dv[kPre], pv[kPre];
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+        const uint32_t i = t.begin + threadIdx.x + k * kBB;
+        const uint32_t j = i < t.end ? i : t.begin;
+        rv[k] = r[j];
+        dv[k] = dg[j];
+        pv[k] = p[j];
+    }
     double tot[2];
     sumPartials<2>(a.partB, a.sys[t.sys], tot);
     const double resid = sqrt(tot[0]) / c.bNorm;
@@ -384,7 +511,12 @@ __global__ void __launch_bounds__(kBB) bPUpdate(BArgs a, int parity) {
     const bool ok = rhoNext > 0.0;
     if (ok) {
         const double beta = rhoNext / st.rho;
-        for (uint32_t i = t.begin + threadIdx.x; i < t.end; i += kBB) a.p[i] = a.r[i] / a.diag[i] + beta * a.p[i];
+#pragma unroll
+        for (int k = 0; k < kPre; ++k) {
+            const uint32_t i = t.begin + threadIdx.x + k * kBB;
+            if (i < t.end) p[i] = rv[k] / dv[k] + beta * pv[k];
+        }
+        for (uint32_t i = t.begin + threadIdx.x + kPre * kBB; i < t.end; i += kBB) p[i] = r[i] / dg[i] + beta * p[i];
     }
     if (lead) {
         CgState nx;
@@ -460,7 +592,9 @@ struct kt_batch {
     kt::DevBuf<uint2> dFlatE;
     kt::DevBuf<uint16_t> dColLo;  // 3-byte columns (batches of fewer than 2^24 rows)
     kt::DevBuf<uint8_t> dColHi;
+    kt::DevBuf<uint32_t> dColPk;  // 3-byte columns packed four to three words (wide loads)
     int colBytes = 0;
+    int wide = 0;                 // entry blocks with 4 row lanes: four entries per load
     uint32_t flatBlocks = 0, partASlots = 0;
     int layout = 0, rowLanes = 1;
     uint32_t layoutUploaded = 0xffffffffu;
@@ -494,8 +628,9 @@ int upload(kt_batch *b) {
     if (!b->dirty && b->tileUploaded == b->tile && b->layoutUploaded == static_cast<uint32_t>(b->layout)) return KT_OK;
     const uint64_t rows = b->diag.size(), nnz = b->col.size();
     KT_TRY(b->dRowPtr.reserve(rows + 1));
-    KT_TRY(b->dCol.reserve(std::max<uint64_t>(nnz, 1)));
-    KT_TRY(b->dVal.reserve(std::max<uint64_t>(nnz, 1)));
+    // four spare entries: the wide loads read whole aligned groups of four
+    KT_TRY(b->dCol.reserve(nnz + kQuadPad));
+    KT_TRY(b->dVal.reserve(nnz + kQuadPad));
     for (auto *v : {&b->dDiag, &b->dRhs, &b->dX, &b->dX0, &b->dR, &b->dP, &b->dAp})
         KT_TRY(v->reserve(std::max<uint64_t>(rows, 1)));
     // tiles
@@ -562,6 +697,11 @@ int upload(kt_batch *b) {
             bSplitColumns<<<dim3(1024), dim3(kBB), 0, b->stream>>>(b->dCol.p, b->dColLo.p, b->dColHi.p, nnz);
             KT_HIP(hipGetLastError());
         }
+        KT_TRY(b->dColPk.reserve(3 * (nnz / 4 + 2)));  // one spare quad
+        if (nnz) {
+            bPackColumns<<<dim3(1024), dim3(kBB), 0, b->stream>>>(b->dCol.p, b->dColPk.p, nnz);
+            KT_HIP(hipGetLastError());
+        }
     }
     KT_HIP(hipStreamSynchronize(b->stream));
     b->dirty = false;
@@ -595,6 +735,8 @@ BArgs argsOf(kt_batch *b) {
     a.col24 = (a.layout && b->colBytes == 3 && b->diag.size() < (1ull << 24)) ? 1u : 0u;
     a.colLo = a.col24 ? b->dColLo.p : nullptr;
     a.colHi = a.col24 ? b->dColHi.p : nullptr;
+    a.wide = (a.layout && b->wide && b->rowLanes == 4) ? 1u : 0u;
+    a.colPk = a.col24 && a.wide ? b->dColPk.p : nullptr;
     return a;
 }
 
@@ -609,7 +751,10 @@ int launchIter(kt_batch *b, const BArgs &a, int parity) {
     const dim3 grid(b->tiles);
     const int R = b->rows >= 4 ? 4 : 2;
     if (a.layout) {
-        if (a.col24) {
+        if (a.wide) {
+            if (a.col24) bSpmvFlat<4, true, true><<<dim3(b->flatBlocks), dim3(kBB), 0, b->stream>>>(a, parity);
+            else bSpmvFlat<4, false, true><<<dim3(b->flatBlocks), dim3(kBB), 0, b->stream>>>(a, parity);
+        } else if (a.col24) {
             if (b->rowLanes == 4) bSpmvFlat<4, true><<<dim3(b->flatBlocks), dim3(kBB), 0, b->stream>>>(a, parity);
             else bSpmvFlat<1, true><<<dim3(b->flatBlocks), dim3(kBB), 0, b->stream>>>(a, parity);
         } else {
@@ -942,6 +1087,13 @@ int kt_batch_set_columns(kt_batch *b, int bytes) {
     KT_BATCH_GUARD(b);
     KT_CHECK(bytes == 0 || bytes == 3 || bytes == 4, KT_EINVAL, "kt_batch_set_columns: bytes must be 0, 3 or 4");
     b->colBytes = bytes;
+    return KT_OK;
+}
+
+int kt_batch_set_wide_loads(kt_batch *b, int on) {
+    KT_BATCH_GUARD(b);
+    KT_CHECK(on == 0 || on == 1, KT_EINVAL, "kt_batch_set_wide_loads: on must be 0 or 1");
+    b->wide = on;
     return KT_OK;
 }
 

@@ -148,6 +148,7 @@ _SIGS = {
     "kt_batch_set_tuning": (I32, [P, I32, U32, I32, I32]),
     "kt_batch_set_layout": (I32, [P, I32, I32]),
     "kt_batch_set_columns": (I32, [P, I32]),
+    "kt_batch_set_wide_loads": (I32, [P, I32]),
     "kt_batch_reset": (I32, [P]),
     "kt_batch_solve": (I32, [P, C.POINTER(SolveOpts), P, C.POINTER(SolveResult)]),
     "kt_batch_get_positions": (I32, [P, U32, P, P, U64]),
@@ -551,10 +552,11 @@ class Batch:
         return n.value, r.value, z.value
 
     def set_tuning(self, lanes: int = 0, tile: int = 0, rows: int = 0, use_graphs: bool = True,
-                   layout: int = 0, row_lanes: int = 0, col_bytes: int = 0):
+                   layout: int = 0, row_lanes: int = 0, col_bytes: int = 0, wide: int = 0):
         _check(lib.kt_batch_set_tuning(self.h, lanes, tile, rows, int(use_graphs)))
         _check(lib.kt_batch_set_layout(self.h, layout, row_lanes))
         _check(lib.kt_batch_set_columns(self.h, col_bytes))
+        _check(lib.kt_batch_set_wide_loads(self.h, int(wide)))
 
     def reset(self):
         _check(lib.kt_batch_reset(self.h))

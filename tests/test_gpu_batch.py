@@ -70,18 +70,18 @@ def test_batch_deterministic_and_shapes_agree(instances):
         b.add_ctx(c)
     ref = None
     flat = {}
-    for lanes, tile, rows, graphs, layout, rl, cb in ((8, 1024, 2, True, 0, 0, 0), (8, 1024, 2, True, 0, 0, 0),
-                                                      (4, 256, 4, False, 0, 0, 0), (16, 2048, 2, True, 0, 0, 0),
-                                                      (16, 512, 4, True, 0, 0, 0), (4, 512, 2, True, 1, 1, 0),
-                                                      (4, 256, 2, False, 1, 4, 0), (4, 1024, 2, True, 1, 4, 0),
-                                                      (4, 1024, 2, True, 1, 4, 3), (4, 1024, 2, False, 1, 4, 3)):
+    for lanes, tile, rows, graphs, layout, rl, cb, wd in (
+            (8, 1024, 2, True, 0, 0, 0, 0), (8, 1024, 2, True, 0, 0, 0, 0), (4, 256, 4, False, 0, 0, 0, 0),
+            (16, 2048, 2, True, 0, 0, 0, 0), (16, 512, 4, True, 0, 0, 0, 0), (4, 512, 2, True, 1, 1, 0, 0),
+            (4, 256, 2, False, 1, 4, 0, 0), (4, 1024, 2, True, 1, 4, 0, 0), (4, 1024, 2, True, 1, 4, 3, 0),
+            (4, 1024, 2, False, 1, 4, 3, 0), (4, 1024, 2, True, 1, 4, 3, 1), (4, 1024, 2, False, 1, 4, 4, 1)):
         b.set_tuning(lanes=lanes, tile=tile, rows=rows, use_graphs=graphs, layout=layout, row_lanes=rl,
-                     col_bytes=cb)
+                     col_bytes=cb, wide=wd)
         b.reset()
         per, _ = b.solve(1e-8, 100000, simpl.KT_STOP_TRUE)
         pos = [b.get_positions(i) for i in range(6)]
         assert all(max(p["true_resid"]) <= 1e-8 for p in per)
-        if (tile, layout, rl) == (1024, 1, 4):  # 3- and 4-byte columns: bit-identical
+        if (tile, layout, rl) == (1024, 1, 4):  # 3- and 4-byte columns, wide loads: bit-identical
             if flat:
                 for (ax, ay), (bx, by) in zip(pos, flat["pos"]):
                     assert np.array_equal(ax, bx) and np.array_equal(ay, by)
@@ -99,15 +99,15 @@ def test_batch_deterministic_and_shapes_agree(instances):
     b.close()
 
 
-@pytest.mark.parametrize("layout", [0, 1])
-def test_batch_edge_cases(layout):
+@pytest.mark.parametrize("layout,wide", [(0, 0), (1, 0), (1, 1)])
+def test_batch_edge_cases(layout, wide):
     """A zero right-hand side (axis inactive), a single-row system, a row with
     no off-diagonals and the max_iter cap, next to an ordinary instance, for
-    both SpMV layouts."""
+    both SpMV layouts and the entry blocks' wide loads."""
     from test_oracle import spd_system
     rng = np.random.default_rng(3)
     b = simpl.Batch(0)
-    b.set_tuning(layout=layout, row_lanes=4 if layout else 0)
+    b.set_tuning(layout=layout, row_lanes=4 if layout else 0, col_bytes=3 if wide else 0, wide=wide)
     s1, s2 = spd_system(rng, 500, 3000), spd_system(rng, 500, 2000)
     zero = simpl.SparseSystem(s2.row_ptr, s2.col, s2.val, s2.diag, np.zeros(500))
     b.add(s1, zero, np.zeros(500), np.zeros(500))
