@@ -125,19 +125,116 @@ __device__ __forceinline__ void forward(const Net &net, const Frag &x, int c, in
     o[1] = acc[1] + net.b[fB3 + 1];
 }
 
-// Inference: out[s] = MLP(feat[s]) for s < count; one wave per 16 samples,
-// grid-stride.  feat is [count][32] BF16.
-__global__ void __launch_bounds__(256) mlpInfer(Net net, const uint16_t *__restrict__ feat, uint64_t count,
-                                                float2 *__restrict__ out) {
+// Inference: out[s] = MLP(feat[s]) for s < count (feat [count][32] BF16),
+// with the weights in LDS and two 16-sample tiles per wave, grid-stride: each
+// weight fragment is read once from LDS and used by two MFMAs.  Rows are
+// padded (W1 to 40 halves, W2 and W3 to 136) so the lanes of a fragment read
+// spread over the LDS banks.  Same MFMA sequence per tile as forward(), so
+// the outputs are the same bit for bit.
+constexpr int kS1 = kIn + 8, kS2 = kHid + 8;
+constexpr uint32_t lW1 = 0, lW2 = lW1 + kHid * kS1, lW3 = lW2 + kHid * kS2, kLdsHalfs = lW3 + kOutPad * kS2;
+static_assert(kLdsHalfs % 8 == 0, "LDS weights in whole 16-byte words");
+
+__device__ __forceinline__ Frag packPair(const float (&lo)[4], const float (&hi)[4]) {
+    Frag f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        f.h[i] = toBf16(lo[i]);
+        f.h[4 + i] = toBf16(hi[i]);
+    }
+    return f;
+}
+
+// A operand of chunk k of a 128-unit input from a padded LDS matrix.
+__device__ __forceinline__ Frag ldsChunkRow(const uint16_t *w, uint32_t row, int k, int g) {
+    Frag f;
+    const uint16_t *base = w + row * kS2 + 32 * k + 4 * g;
+    f.d[0] = *reinterpret_cast<const uint2 *>(base);
+    f.d[1] = *reinterpret_cast<const uint2 *>(base + 16);
+    return f;
+}
+
+__global__ void __launch_bounds__(256) mlpInferLds(Net net, const uint16_t *__restrict__ feat, uint64_t count,
+                                                   float2 *__restrict__ out) {
+    __shared__ alignas(16) uint16_t sW[kLdsHalfs];
+    __shared__ float sB[kBiasFloats];
+    // stage the weights (16-byte words; the padding columns stay unread)
+    for (uint32_t u = threadIdx.x; u < (kHid * kIn + kHid * kHid + kOutPad * kHid) / 8; u += blockDim.x) {
+        const uint32_t h = 8 * u;  // half index in the unpadded block
+        uint32_t dst;
+        if (h < bW2) dst = lW1 + (h / kIn) * kS1 + h % kIn;
+        else if (h < bW3) dst = lW2 + ((h - bW2) / kHid) * kS2 + (h - bW2) % kHid;
+        else dst = lW3 + ((h - bW3) / kHid) * kS2 + (h - bW3) % kHid;
+        *reinterpret_cast<uint4 *>(sW + dst) = *reinterpret_cast<const uint4 *>(net.w + h);
+    }
+    for (uint32_t u = threadIdx.x; u < kBiasFloats; u += blockDim.x) sB[u] = net.b[u];
+    __syncthreads();
     const int lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
+    const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
     const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x >> 6);
-    for (uint64_t tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile * 16 < count; tile += waves) {
-        const uint64_t s = tile * 16 + c;
-        Frag x;
-        x.q = s < count ? *reinterpret_cast<const uint4 *>(feat + s * kIn + 8 * g) : make_uint4(0, 0, 0, 0);
-        float h1[8][4], h2[8][4], o[2];
-        forward(net, x, c, g, h1, h2, o);
-        if (g == 0 && s < count) out[s] = make_float2(o[0], o[1]);
+    for (uint64_t pair = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); pair * 32 < count; pair += waves) {
+        const uint64_t s0 = pair * 32 + c, s1 = s0 + 16;
+        Frag x0, x1;
+        x0.q = s0 < count ? *reinterpret_cast<const uint4 *>(feat + s0 * kIn + 8 * g) : make_uint4(0, 0, 0, 0);
+        x1.q = s1 < count ? *reinterpret_cast<const uint4 *>(feat + s1 * kIn + 8 * g) : make_uint4(0, 0, 0, 0);
+        // layer 1: tiles 2k and 2k + 1 make chunk k of layer 2's input
+        Frag hb0[4], hb1[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float v0[2][4], v1[2][4];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int t = 2 * k + u;
+                Frag a;
+                a.q = *reinterpret_cast<const uint4 *>(sW + lW1 + (16 * t + c) * kS1 + 8 * g);
+                const floatx4 a0 = mma(a, x0, zero), a1 = mma(a, x1, zero);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float bias = sB[fB1 + 16 * t + 4 * g + i];
+                    v0[u][i] = fmaxf(a0[i] + bias, 0.f);
+                    v1[u][i] = fmaxf(a1[i] + bias, 0.f);
+                }
+            }
+            hb0[k] = packPair(v0[0], v0[1]);
+            hb1[k] = packPair(v1[0], v1[1]);
+        }
+        // layer 2
+        Frag hc0[4], hc1[4];
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) {
+            float v0[2][4], v1[2][4];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int t = 2 * k2 + u;
+                floatx4 a0 = zero, a1 = zero;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const Frag a = ldsChunkRow(sW + lW2, 16 * t + c, k, g);
+                    a0 = mma(a, hb0[k], a0);
+                    a1 = mma(a, hb1[k], a1);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float bias = sB[fB2 + 16 * t + 4 * g + i];
+                    v0[u][i] = fmaxf(a0[i] + bias, 0.f);
+                    v1[u][i] = fmaxf(a1[i] + bias, 0.f);
+                }
+            }
+            hc0[k2] = packPair(v0[0], v0[1]);
+            hc1[k2] = packPair(v1[0], v1[1]);
+        }
+        // layer 3
+        floatx4 o0 = zero, o1 = zero;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const Frag a = ldsChunkRow(sW + lW3, c, k, g);
+            o0 = mma(a, hc0[k], o0);
+            o1 = mma(a, hc1[k], o1);
+        }
+        if (g == 0) {
+            if (s0 < count) out[s0] = make_float2(o0[0] + sB[fB3 + 0], o0[1] + sB[fB3 + 1]);
+            if (s1 < count) out[s1] = make_float2(o1[0] + sB[fB3 + 0], o1[1] + sB[fB3 + 1]);
+        }
     }
 }
 
@@ -391,9 +488,10 @@ inline uint32_t blocksFor(uint64_t n, uint32_t b) { return static_cast<uint32_t>
 int mlpInferLaunch(hipStream_t stream, const uint16_t *w, const float *b, const uint16_t *feat, uint64_t count,
                    float *out) {
     if (count == 0) return KT_OK;
-    const uint64_t tiles = (count + 15) / 16;
-    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((tiles + 3) / 4, 2048));
-    hipLaunchKernelGGL(mlp::mlpInfer, dim3(blocks), dim3(256), 0, stream, mlp::Net{w, b}, feat, count,
+    // one wave per 32 samples, three blocks of four waves per CU
+    const uint64_t pairs = (count + 31) / 32;
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((pairs + 3) / 4, 768));
+    hipLaunchKernelGGL(mlp::mlpInferLds, dim3(blocks), dim3(256), 0, stream, mlp::Net{w, b}, feat, count,
                        reinterpret_cast<float2 *>(out));
     KT_HIP(hipGetLastError());
     return KT_OK;
