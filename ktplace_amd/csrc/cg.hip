@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include <hip/hip_ext.h>
 
@@ -250,7 +251,7 @@ __global__ void __launch_bounds__(kBlock) cgStartFinish(CgArgs a) {
     AxisCtl &c = a.ctl[axis];
     if (c.frozen) return;
     double tot[3];
-    sumPartials<3>(a.partS, a.sys[axis].blocks, a.stride, axis, tot);
+    sumPartials<3>(a.partS, a.flatResidual ? a.sys[axis].sblocks : a.sys[axis].blocks, a.stride, axis, tot);
     if (threadIdx.x == 0) {
         CgState &st = c.s[0];
         c.bNorm = sqrt(tot[0]);
@@ -478,6 +479,9 @@ uint32_t flatGrid(const CgArgs &a, int only = -1) {
 // take the FP64 residual b - A x64 of the same rows instead: 1 refreshes the
 // FP32 residual of the axes that asked for it (partials r64.r64, r.z), 2 is
 // the true residual check of both axes (partials r.r, b.b); parity is -1.
+// MODE 3 and 4 (T = double) start the solve as cgStart does: r = b - A x64,
+// rounded to the working precision (3: FP32, 4: FP64), p = z = r / d, the
+// FP32 correction cleared (3); partials b.b, r.z, r.r.
 // W4: phase 1 reads four consecutive entries per thread with one wide load
 // per array (16-byte values, 8 + 4 bytes of 3-byte columns), from the block's
 // first entry rounded down to a multiple of four; entries outside the block
@@ -488,7 +492,7 @@ uint32_t flatGrid(const CgArgs &a, int only = -1) {
 template <typename T, uint32_t E, int U, int G2, int NT, bool C24 = false, int MODE = 0, bool W4 = false>
 __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
     static_assert(MODE == 0 || sizeof(T) == 8, "the residual modes run in FP64");
-    constexpr int K = MODE == 0 ? 1 : 2;
+    constexpr int K = MODE == 0 ? 1 : (MODE <= 2 ? 2 : 3);
     constexpr uint32_t kFlatEntries = E, kFlatSlack = flatSlack(E);
     // W4 keeps product k of the quads at slot k (the block's entry j at
     // j + lead), so each quad is two aligned 16-byte LDS stores
@@ -500,6 +504,8 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
         if (!a.ctl[axis].s[parity].live) return;
     } else if constexpr (MODE == 1) {
         if (!a.ctl[axis].needUpdate) return;
+    } else if constexpr (MODE >= 3) {
+        if (a.ctl[axis].frozen) return;
     }
     const SysView s = a.sys[axis];
     const T *__restrict__ p = MODE == 0 ? static_cast<const T *>(a.p[axis]) : reinterpret_cast<const T *>(a.x64[axis]);
@@ -518,6 +524,18 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
         } else {
             const double rhs = s.rhs[r];
             const double ri = rhs - sum;
+            if constexpr (MODE >= 3) {
+                using W = std::conditional_t<MODE == 3, float, double>;
+                const W rt = static_cast<W>(ri);
+                const W zt = rt / Mat<W>::diag(s)[r];
+                static_cast<W *>(a.r[axis])[r] = rt;
+                static_cast<W *>(a.p[axis])[r] = zt;
+                if constexpr (MODE == 3) static_cast<float *>(a.e[axis])[r] = 0.0f;
+                acc[0] += rhs * rhs;
+                acc[1] += static_cast<double>(rt) * static_cast<double>(zt);
+                acc[2] += ri * ri;
+                return;
+            }
             acc[0] += ri * ri;
             if constexpr (MODE == 1) {
                 const float rf = static_cast<float>(ri);
@@ -691,7 +709,7 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
         }
     }
     blockSum<K, NT>(acc);
-    writePartial<K>(MODE == 0 ? a.partA : a.partB, a.stride, axis, b, acc);
+    writePartial<K>(MODE == 0 ? a.partA : (MODE <= 2 ? a.partB : a.partS), a.stride, axis, b, acc);
 }
 
 // Cut c (c = 1 .. blocks - 1) = first row whose entries start at or after
@@ -1192,9 +1210,19 @@ int launchIteration(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t
     return KT_OK;
 }
 
+template <int MODE>
+void launchFlatResidual(kt_ctx *ctx, const CgArgs &a);
+
 template <typename T>
 int launchStart(kt_ctx *ctx, const CgArgs &a, int G) {
     const dim3 grid(gridBlocks(a));
+    if (a.flatResidual) {
+        launchFlatResidual<sizeof(T) == 4 ? 3 : 4>(ctx, a);
+        KT_HIP(hipGetLastError());
+        cgStartFinish<<<dim3(2), dim3(kBlock), 0, ctx->stream>>>(a);
+        KT_HIP(hipGetLastError());
+        return KT_OK;
+    }
     switch (G) {
         case 1: cgStart<T, 1><<<grid, dim3(kBlock), 0, ctx->stream>>>(a); break;
         case 2: cgStart<T, 2><<<grid, dim3(kBlock), 0, ctx->stream>>>(a); break;
@@ -1401,8 +1429,8 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
               a.flatRowLanes == 4) ? 1u : 0u;
     a.col24 = a.col24 || (a.wide && ctx->tune.colBytes == 3 && rows < (1ull << 24)) ? 1u : 0u;
     // FP64 residuals by the same entry blocks (the shape instantiated for them)
-    a.flatResidual =
-        (a.layout == 1 && a.flatThreads == 256 && a.flatUnroll == 4 && a.flatEntries == 2048 && rows > 0) ? 1u : 0u;
+    // (the residual kernels have their own entries per round)
+    a.flatResidual = (a.layout == 1 && a.flatThreads == 256 && a.flatEntries == 2048 && rows > 0) ? 1u : 0u;
 
     KT_CHECK(!(a.layout && a.halo), KT_EINVAL, "solve: the entry-block SpMV has no LDS window");
     if (a.halo) {
