@@ -416,14 +416,22 @@ uint64_t scanScratch(uint64_t m) {
     return total + 1;
 }
 
-// Inclusive in-place scans of m values.
+// Inclusive in-place scans of m values: small ones by the tile scan above,
+// large ones by rocPRIM's single-pass device scan (scan.hip).  Integer sums,
+// so either gives the same values.
+int deviceScan32(kt_ctx *ctx, uint32_t *data, uint64_t m);
+int deviceScan64(kt_ctx *ctx, uint64_t *data, uint64_t m);
+constexpr uint64_t kDeviceScanMin = 1ull << 16;
+
 int scanInclusive(kt_ctx *ctx, uint32_t *data, uint64_t m) {
+    if (m >= kDeviceScanMin) return deviceScan32(ctx, data, m);
     const uint64_t cap = scanScratch(m);
     KT_TRY(ctx->scanTmp.reserve(cap));
     return scanLevel<uint32_t>(ctx, data, m, ctx->scanTmp.p, cap);
 }
 
 int scanInclusive64(kt_ctx *ctx, uint64_t *data, uint64_t m) {
+    if (m >= kDeviceScanMin) return deviceScan64(ctx, data, m);
     const uint64_t cap = scanScratch(m);
     KT_TRY(ctx->scanTmp64.reserve(cap));
     return scanLevel<uint64_t>(ctx, data, m, ctx->scanTmp64.p, cap);

@@ -247,6 +247,7 @@ struct kt_ctx {
     kt::DevBuf<uint32_t> b2bLong;                // B2B: first reference of each edge block, per axis
     kt::DevBuf<uint32_t> scanTmp;   // block sums of the u32 scans
     kt::DevBuf<uint64_t> scanTmp64; // block sums of the u64 scans
+    kt::DevBuf<uint8_t> scanStorage; // temporary storage of the single-pass device scans
 
     kt::Sys sys[2];
 
