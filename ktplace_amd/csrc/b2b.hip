@@ -29,13 +29,11 @@ constexpr uint64_t kScanTile = kBlock * kScanItems;
 __global__ void __launch_bounds__(kBlock)
 vertexPositions(const double *__restrict__ x, const double *__restrict__ y,
                 const uint32_t *__restrict__ varOfVertex, const double *__restrict__ fixedX,
-                const double *__restrict__ fixedY, double *__restrict__ vx, double *__restrict__ vy,
-                uint32_t V) {
+                const double *__restrict__ fixedY, double2 *__restrict__ vxy, uint32_t V) {
     const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
     if (v >= V) return;
     const uint32_t var = varOfVertex[v];
-    vx[v] = var != kNoVar ? x[var] : fixedX[v];
-    vy[v] = var != kNoVar ? y[var] : fixedY[v];
+    vxy[v] = var != kNoVar ? make_double2(x[var], y[var]) : make_double2(fixedX[v], fixedY[v]);
 }
 
 // One thread per pin, both axes.  The pin's position (vertex position +
@@ -53,18 +51,16 @@ __device__ __forceinline__ double pinPos(const PinRec &r, int axis) { return axi
 __device__ __forceinline__ double pinTerm(const PinRec &r, int axis) { return axis ? r.termY : r.termX; }
 
 __global__ void __launch_bounds__(kBlock)
-pinStage(const uint32_t *__restrict__ pinCell, const uint32_t *__restrict__ varOfVertex,
-         const double *__restrict__ offX, const double *__restrict__ offY, const double *__restrict__ vx,
-         const double *__restrict__ vy, const uint32_t *__restrict__ movPrefix, PinRec *__restrict__ pin,
-         uint2 *__restrict__ varMov, uint64_t P) {
+pinStage(const uint32_t *__restrict__ pinCell, const uint2 *__restrict__ varMov, const double *__restrict__ offX,
+         const double *__restrict__ offY, const double2 *__restrict__ vxy, PinRec *__restrict__ pin, uint64_t P) {
     const uint64_t k = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (k >= P) return;
     const uint32_t c = pinCell[k];
-    const uint32_t var = varOfVertex[c];
-    const double cx = vx[c], cy = vy[c];
+    const bool movable = varMov[k].x != kNoVar;
+    const double2 cv = vxy[c];
+    const double cx = cv.x, cy = cv.y;
     const double px = cx + offX[k], py = cy + offY[k];
-    pin[k] = PinRec{px, var != kNoVar ? px - cx : px, py, var != kNoVar ? py - cy : py};
-    varMov[k] = make_uint2(var, movPrefix[k]);
+    pin[k] = PinRec{px, movable ? px - cx : px, py, movable ? py - cy : py};
 }
 
 // Everything a row's reference to a net reads about the net, in one 64-byte
@@ -443,11 +439,10 @@ int scanRowPtr(kt_ctx *ctx, uint32_t *rowPtr, uint64_t rows) {
 
 // px/py: fixed positions with the movable cells' positions written over
 int vertexPositionsOf(kt_ctx *ctx, const double *x, const double *y) {
-    KT_TRY(ctx->vx.reserve(std::max<uint32_t>(ctx->V, 1)));
-    KT_TRY(ctx->vy.reserve(std::max<uint32_t>(ctx->V, 1)));
+    KT_TRY(ctx->vxy.reserve(std::max<uint32_t>(ctx->V, 1)));
     if (ctx->V == 0) return KT_OK;
     vertexPositions<<<dim3((ctx->V + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream>>>(
-        x, y, ctx->varOfVertex.p, ctx->fixedX.p, ctx->fixedY.p, ctx->vx.p, ctx->vy.p, ctx->V);
+        x, y, ctx->varOfVertex.p, ctx->fixedX.p, ctx->fixedY.p, ctx->vxy.p, ctx->V);
     KT_HIP(hipGetLastError());
     return KT_OK;
 }
@@ -464,13 +459,11 @@ int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY) {
     KT_TRY(vertexPositionsLaunch(ctx));
     const uint64_t pins = std::max<uint64_t>(ctx->P, 1);
     KT_TRY(ctx->pinAt[0].reserve(4 * pins));  // one PinRec (4 doubles) per pin
-    KT_TRY(ctx->pinVar.reserve(2 * pins));    // (variable, movable rank) per pin
     PinRec *pin = reinterpret_cast<PinRec *>(ctx->pinAt[0].p);
-    uint2 *varMov = reinterpret_cast<uint2 *>(ctx->pinVar.p);
+    const uint2 *varMov = reinterpret_cast<const uint2 *>(ctx->pinVar.p);  // uploaded with the netlist
     if (ctx->P > 0) {
         pinStage<<<dim3(static_cast<uint32_t>((ctx->P + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream>>>(
-            ctx->pinCell.p, ctx->varOfVertex.p, ctx->offX.p, ctx->offY.p, ctx->vx.p, ctx->vy.p, ctx->movPrefix.p, pin,
-            varMov, ctx->P);
+            ctx->pinCell.p, varMov, ctx->offX.p, ctx->offY.p, ctx->vxy.p, pin, ctx->P);
         KT_HIP(hipGetLastError());
     }
     NetRec *rec = reinterpret_cast<NetRec *>(ctx->ext.p);

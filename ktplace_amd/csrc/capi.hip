@@ -165,13 +165,16 @@ int uploadNetlist(kt_ctx *ctx, const kt_netlist &nl) {
                  "upload: mov_vertex / var_of_vertex mismatch");
     }
     std::vector<uint32_t> netMov(N, 0);
-    std::vector<uint32_t> movPrefix(P);  // per pin: movable pins before it in its net
+    // per pin: its vertex's variable and the movable pins before it in its net
+    std::vector<uint32_t> varMov(2 * P);
     uint64_t bound = 0;
     for (uint32_t j = 0; j < N; ++j) {
         const uint64_t p = nl.net_start[j + 1] - nl.net_start[j];
         for (uint64_t k = nl.net_start[j]; k < nl.net_start[j + 1]; ++k) {
-            movPrefix[k] = netMov[j];
-            netMov[j] += nl.var_of_vertex[nl.pin_cell[k]] != kNoVar ? 1u : 0u;
+            const uint32_t var = nl.var_of_vertex[nl.pin_cell[k]];
+            varMov[2 * k] = var;
+            varMov[2 * k + 1] = netMov[j];
+            netMov[j] += var != kNoVar ? 1u : 0u;
         }
         if (p >= 2) bound += 4 * p - 6;  // 2 bound pins x (p-1) + (p-2) inner pins x 2
     }
@@ -233,7 +236,7 @@ int uploadNetlist(kt_ctx *ctx, const kt_netlist &nl) {
     KT_TRY(upload(ctx, ctx->refNet, nl.cell_pin_net, Pm));
     KT_TRY(upload(ctx, ctx->refPin, nl.cell_pin_idx, Pm));
     KT_TRY(upload(ctx, ctx->netMov, netMov.data(), N));
-    KT_TRY(upload(ctx, ctx->movPrefix, movPrefix.data(), P));
+    KT_TRY(upload(ctx, ctx->pinVar, varMov.data(), 2 * P));
     KT_TRY(upload(ctx, ctx->floating, floating.data(), n));
     if (nl.width) KT_TRY(upload(ctx, ctx->width, nl.width, n));
     if (nl.height) KT_TRY(upload(ctx, ctx->height, nl.height, n));
@@ -408,13 +411,13 @@ int kt_ctx_destroy(kt_ctx *ctx) {
         for (hipEvent_t ev : t.pending) (void)hipEventDestroy(ev);
     ctx->netStart.release(); ctx->pinCell.release(); ctx->varOfVertex.release(); ctx->movVertex.release();
     ctx->pinStart.release(); ctx->refNet.release(); ctx->refPin.release(); ctx->netMov.release();
-    ctx->floating.release(); ctx->hpwlScratch.release(); ctx->loopDev.release(); ctx->movPrefix.release();
+    ctx->floating.release(); ctx->hpwlScratch.release(); ctx->loopDev.release();
     ctx->b2bEdgeOff.release(); ctx->b2bEntOff.release(); ctx->b2bLong.release();
     for (int a = 0; a < 2; ++a) ctx->b2bEdges[a].release();
     ctx->offX.release(); ctx->offY.release(); ctx->weight.release(); ctx->fixedX.release();
     ctx->fixedY.release(); ctx->width.release(); ctx->height.release(); ctx->area.release();
     ctx->posX.release(); ctx->posY.release(); ctx->saveX.release(); ctx->saveY.release();
-    ctx->anchorX.release(); ctx->anchorY.release(); ctx->vx.release(); ctx->vy.release();
+    ctx->anchorX.release(); ctx->anchorY.release(); ctx->vxy.release();
     ctx->ext.release(); ctx->scanTmp.release(); ctx->pinVar.release();
     for (int a = 0; a < 2; ++a) ctx->pinAt[a].release();
     for (auto &s : ctx->sys) {
@@ -453,13 +456,13 @@ int kt_ctx_device_bytes(kt_ctx *ctx, uint64_t *bytes) {
     uint64_t b = 0;
     b += ctx->netStart.bytes() + ctx->pinCell.bytes() + ctx->varOfVertex.bytes() + ctx->movVertex.bytes();
     b += ctx->pinStart.bytes() + ctx->refNet.bytes() + ctx->refPin.bytes() + ctx->netMov.bytes();
-    b += ctx->floating.bytes() + ctx->hpwlScratch.bytes() + ctx->loopDev.bytes() + ctx->movPrefix.bytes();
+    b += ctx->floating.bytes() + ctx->hpwlScratch.bytes() + ctx->loopDev.bytes();
     b += ctx->b2bEdgeOff.bytes() + ctx->b2bEntOff.bytes() + ctx->b2bLong.bytes() + ctx->b2bEdges[0].bytes() +
          ctx->b2bEdges[1].bytes();
     b += ctx->offX.bytes() + ctx->offY.bytes() + ctx->weight.bytes() + ctx->fixedX.bytes() + ctx->fixedY.bytes();
     b += ctx->width.bytes() + ctx->height.bytes() + ctx->area.bytes();
     b += ctx->posX.bytes() + ctx->posY.bytes() + ctx->saveX.bytes() + ctx->saveY.bytes();
-    b += ctx->anchorX.bytes() + ctx->anchorY.bytes() + ctx->vx.bytes() + ctx->vy.bytes() + ctx->ext.bytes();
+    b += ctx->anchorX.bytes() + ctx->anchorY.bytes() + ctx->vxy.bytes() + ctx->ext.bytes();
     b += ctx->pinAt[0].bytes() + ctx->pinAt[1].bytes();
     b += ctx->pinVar.bytes();
     b += ctx->scanTmp.bytes();

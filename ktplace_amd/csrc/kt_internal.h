@@ -237,11 +237,10 @@ struct kt_ctx {
     uint64_t posCount = 0;
     kt::DevBuf<double> posX, posY, saveX, saveY, anchorX, anchorY;
     bool haveAnchors = false;
-    kt::DevBuf<double> vx, vy;      // vertex-indexed positions (px, py)
+    kt::DevBuf<double2> vxy;        // vertex-indexed positions (px, py), one 16-byte record per vertex
     kt::DevBuf<uint32_t> ext;       // [N][16]: per-net record of the B2B build (pins, weight, lo/hi per axis)
     kt::DevBuf<double> pinAt[2];    // per pin and axis: (vertex position + offset, rhs term of an edge to it)
-    kt::DevBuf<uint32_t> pinVar;    // per pin: its vertex's variable (kNoVar: fixed)
-    kt::DevBuf<uint32_t> movPrefix; // per pin: movable pins before it in its net (uploaded)
+    kt::DevBuf<uint32_t> pinVar;    // per pin: (its vertex's variable or kNoVar, movable pins before it in its net), uploaded
     kt::DevBuf<uint64_t> b2bEdgeOff, b2bEntOff;  // B2B: per reference, first edge / CSR slot (x | y << 32)
     kt::DevBuf<double> b2bEdges[2];              // B2B: (w, rhs term) per edge in row-walk order
     kt::DevBuf<uint32_t> b2bLong;                // B2B: first reference of each edge block, per axis

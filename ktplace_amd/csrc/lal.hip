@@ -166,8 +166,7 @@ __global__ void sumInOrder(const double *__restrict__ part, uint32_t m, double *
 // with fewer than two pins.
 __global__ void __launch_bounds__(kB)
 netHpwl(const uint32_t *__restrict__ netStart, const uint32_t *__restrict__ pinCell, const double *__restrict__ offX,
-        const double *__restrict__ offY, const double *__restrict__ vx, const double *__restrict__ vy, uint32_t N,
-        double *__restrict__ out) {
+        const double *__restrict__ offY, const double2 *__restrict__ vxy, uint32_t N, double *__restrict__ out) {
     const uint32_t j = blockIdx.x * kB + threadIdx.x;
     if (j >= N) return;
     const uint32_t b = netStart[j], e = netStart[j + 1];
@@ -178,7 +177,8 @@ netHpwl(const uint32_t *__restrict__ netStart, const uint32_t *__restrict__ pinC
     double lx = INFINITY, ly = INFINITY, ux = -INFINITY, uy = -INFINITY;
     for (uint32_t q = b; q < e; ++q) {
         const uint32_t c = pinCell[q];
-        const double px = vx[c] + offX[q], py = vy[c] + offY[q];
+        const double2 v = vxy[c];
+        const double px = v.x + offX[q], py = v.y + offY[q];
         // BBox::extend: std::min / std::max (kt_geometry.cc:141-146)
         lx = px < lx ? px : lx;
         ly = py < ly ? py : ly;
@@ -980,7 +980,7 @@ int lalHpwlInto(kt_ctx *ctx, const double *x, const double *y, double *dst) {
     double *part = perNet + N;
     if (N) {
         netHpwl<<<gridFor(N), kB, 0, ctx->stream>>>(ctx->netStart.p, ctx->pinCell.p, ctx->offX.p, ctx->offY.p,
-                                                    ctx->vx.p, ctx->vy.p, N, perNet);
+                                                    ctx->vxy.p, N, perNet);
         KT_HIP(hipGetLastError());
         chunkSums<<<gridFor(chunks), kB, 0, ctx->stream>>>(perNet, N, part);
         KT_HIP(hipGetLastError());
