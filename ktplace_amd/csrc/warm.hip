@@ -94,12 +94,23 @@ __global__ void __launch_bounds__(kB) warmStats(const double *__restrict__ x, co
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ void warmStatsFinish(const double *__restrict__ part, uint32_t chunks, uint32_t n, double dieW,
-                                double dieH, WarmScalars *__restrict__ sc) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One wave: lane l sums chunks l, l + 64, ... in order, then lane 0 adds the
+// 64 lane sums in lane order (a fixed order, so the result is reproducible).
+__global__ void __launch_bounds__(64) warmStatsFinish(const double *__restrict__ part, uint32_t chunks, uint32_t n,
+                                                      double dieW, double dieH, WarmScalars *__restrict__ sc) {
+    __shared__ double lanes[6][64];
     double t[6] = {0, 0, 0, 0, 0, 0};
-    for (uint32_t k = 0; k < chunks; ++k)
+    for (uint32_t k = threadIdx.x; k < chunks; k += 64)
+#pragma unroll
         for (int j = 0; j < 6; ++j) t[j] += part[k * 6 + j];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) lanes[j][threadIdx.x] = t[j];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int j = 0; j < 6; ++j) {
+        t[j] = 0.0;
+        for (int l = 0; l < 64; ++l) t[j] += lanes[j][l];
+    }
     const double inv = 1.0 / static_cast<double>(n);
     sc->cx = t[0] * inv;
     sc->cy = t[2] * inv;
