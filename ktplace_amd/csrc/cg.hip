@@ -739,7 +739,9 @@ __global__ void __launch_bounds__(256) flatBounds(const uint32_t *__restrict__ r
     info[b] = make_uint4(r0, r1, rowPtr[r0], rowPtr[r1]);
 }
 
-// alpha = rho / p.Ap; x += alpha p; r -= alpha Ap; tile partials of r.r, r.z.
+// alpha = rho / p.Ap; r -= alpha Ap; tile partials of r.r, r.z.  The iterate
+// update x += alpha p is left to cgPUpdate, which reads p anyway: alpha goes
+// there through the axis' control block (the same double, so x is the same).
 // PER = rows per thread when the tile is PER * 1024 rows: the thread's vector
 // entries are loaded before the tile partials are summed (they do not depend
 // on alpha), so their latency hides behind that sum.  PER = 0: any tile, the
@@ -752,9 +754,7 @@ __global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a, int parity) {
     const CgState st = a.ctl[axis].s[parity];
     if (!st.live) return;
     const SysView s = a.sys[axis];
-    T *__restrict__ x = static_cast<T *>(a.e[axis] ? a.e[axis] : static_cast<void *>(a.x64[axis]));
     T *__restrict__ r = static_cast<T *>(a.r[axis]);
-    const T *__restrict__ p = static_cast<const T *>(a.p[axis]);
     const T *__restrict__ ap = static_cast<const T *>(a.ap[axis]);
     const T *__restrict__ d = Mat<T>::diag(s);
     const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
@@ -764,14 +764,12 @@ __global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a, int parity) {
     share.issue(a.partA, s.sblocks, a.stride, axis);
     __builtin_amdgcn_sched_barrier(0);
     constexpr int kPre = PER > 0 ? PER : 1;
-    T xv[kPre], pv[kPre], rv[kPre], av[kPre], dv[kPre];
+    T rv[kPre], av[kPre], dv[kPre];
     if constexpr (PER > 0) {
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const uint32_t i = begin + threadIdx.x + k * kBlock;
             const uint32_t j = i < end ? i : begin;  // unconditional loads; a tile has a row
-            xv[k] = x[j];
-            pv[k] = p[j];
             rv[k] = r[j];
             av[k] = ap[j];
             dv[k] = d[j];
@@ -780,16 +778,19 @@ __global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a, int parity) {
     __builtin_amdgcn_sched_barrier(0);
     double pap[1];
     share.finish(pap);
+    const bool lead = tile == 0 && threadIdx.x == 0;
     if (!(pap[0] > 0.0)) {
-        if (tile == 0 && threadIdx.x == 0) a.ctl[axis].brk[parity] = 1;
+        if (lead) a.ctl[axis].brk[parity] = 1;
         return;
     }
-    if (tile == 0 && threadIdx.x == 0) a.ctl[axis].brk[parity] = 0;
     const double alpha = st.rho / pap[0];
+    if (lead) {
+        a.ctl[axis].brk[parity] = 0;
+        a.ctl[axis].alpha[parity] = alpha;
+    }
     const T alphaT = static_cast<T>(alpha);
     double acc[2] = {0.0, 0.0};
-    const auto step = [&](uint32_t i, T xi, T pi, T ri, T ai, T di) {
-        x[i] = xi + alphaT * pi;
+    const auto step = [&](uint32_t i, T ri, T ai, T di) {
         const T rn = ri - alphaT * ai;
         r[i] = rn;
         const double rd = static_cast<double>(rn);
@@ -800,17 +801,18 @@ __global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a, int parity) {
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const uint32_t i = begin + threadIdx.x + k * kBlock;
-            if (i < end) step(i, xv[k], pv[k], rv[k], av[k], dv[k]);
+            if (i < end) step(i, rv[k], av[k], dv[k]);
         }
     } else {
-        for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) step(i, x[i], p[i], r[i], ap[i], d[i]);
+        for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) step(i, r[i], ap[i], d[i]);
     }
     blockSum<2>(acc);
     writePartial<2>(a.partB, a.stride, axis, tile, acc);
 }
 
-// beta = r.z' / r.z; p = z + beta p; block 0 publishes state[parity ^ 1].
-// PER as in cgUpdate: r, d and p are loaded before the tile partials are summed.
+// x += alpha p (alpha from cgUpdate); beta = r.z' / r.z; p = z + beta p;
+// block 0 publishes state[parity ^ 1].  PER as in cgUpdate: x, r, d and p are
+// loaded before the tile partials are summed.
 template <typename T, int PER>
 __global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
     int axis;
@@ -840,20 +842,23 @@ __global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
         }
         return;
     }
+    T *__restrict__ x = static_cast<T *>(a.e[axis] ? a.e[axis] : static_cast<void *>(a.x64[axis]));
     T *__restrict__ p = static_cast<T *>(a.p[axis]);
     const T *__restrict__ r = static_cast<const T *>(a.r[axis]);
     const T *__restrict__ d = Mat<T>::diag(s);
     const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
     PartialShare<2> share;
     share.issue(a.partB, s.blocks, a.stride, axis);
+    const T alphaT = static_cast<T>(c.alpha[parity]);
     __builtin_amdgcn_sched_barrier(0);
     constexpr int kPre = PER > 0 ? PER : 1;
-    T rv[kPre], dv[kPre], pv[kPre];
+    T xv[kPre], rv[kPre], dv[kPre], pv[kPre];
     if constexpr (PER > 0) {
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const uint32_t i = begin + threadIdx.x + k * kBlock;
             const uint32_t j = i < end ? i : begin;
+            xv[k] = x[j];
             rv[k] = r[j];
             dv[k] = d[j];
             pv[k] = p[j];
@@ -865,16 +870,23 @@ __global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
     const double resid = sqrt(tot[0]) / c.bNorm;
     const double rhoNext = tot[1];
     const bool ok = rhoNext > 0.0;
-    if (ok) {
-        const T beta = static_cast<T>(rhoNext / st.rho);
-        if constexpr (PER > 0) {
+    // the iterate steps whatever beta turns out to be (the reference updates x
+    // before it looks at r.z, simpl_lss.cc:247,253); p only when the axis goes on
+    const T beta = static_cast<T>(ok ? rhoNext / st.rho : 0.0);
+    if constexpr (PER > 0) {
 #pragma unroll
-            for (int k = 0; k < PER; ++k) {
-                const uint32_t i = begin + threadIdx.x + k * kBlock;
-                if (i < end) p[i] = rv[k] / dv[k] + beta * pv[k];
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t i = begin + threadIdx.x + k * kBlock;
+            if (i < end) {
+                x[i] = xv[k] + alphaT * pv[k];
+                if (ok) p[i] = rv[k] / dv[k] + beta * pv[k];
             }
-        } else {
-            for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) p[i] = r[i] / d[i] + beta * p[i];
+        }
+    } else {
+        for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
+            const T pi = p[i];
+            x[i] = x[i] + alphaT * pi;
+            if (ok) p[i] = r[i] / d[i] + beta * pi;
         }
     }
     if (lead) {

@@ -95,6 +95,7 @@ struct AxisCtl {
     double refreshFactor; // refresh once the FP32 residual falls by this factor
     uint32_t maxIter;
     int32_t brk[2];       // p.Ap <= 0 in the iteration of that parity
+    double alpha[2];      // rho / p.Ap of the iteration of that parity (cgUpdate -> cgPUpdate)
     int32_t frozen;       // excluded from a restart (converged or broken down)
     int32_t needUpdate;   // mixed precision: FP64 refresh requested
     uint32_t refreshes;
