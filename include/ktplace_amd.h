@@ -284,6 +284,14 @@ int kt_set_spmv_columns(kt_ctx *ctx, int bytes);
  * per row) reads four consecutive entries per thread with one wide load per
  * array; 0 (default): one entry per load.  Results are the same either way. */
 int kt_set_spmv_wide_loads(kt_ctx *ctx, int on);
+/* Mixed precision (KT_PREC_MIXED) only, no counterpart in the reference: 1
+ * (default) runs the FP32 iterations on the symmetrically scaled system
+ * D^-1/2 A D^-1/2 (unit diagonal; FP32 values val_ij / (sqrt(d_i) sqrt(d_j)),
+ * converted once per system), which is Jacobi-PCG in other coordinates: the
+ * SpMV and the direction update then read no diagonal.  The FP64 refreshes,
+ * the stopping test and the result are those of the unscaled system.  0: the
+ * FP32 iterations divide by the diagonal as the FP64 solve does. */
+int kt_set_mixed_scaling(kt_ctx *ctx, int on);
 
 /* ---- per-kernel timings (HIP events on the context's stream) -------------- */
 #define KT_KERNEL_SPMV 0      /* FP64 CG SpMV + p.Ap */

@@ -23,7 +23,11 @@
 //     reads a value written by its own kernel.
 //   * z = r / d is never stored; it is recomputed where needed.
 //   * Vectors and matrix values are FP64 (plain solve) or FP32 (inner solve of
-//     the mixed-precision path); every dot product accumulates in FP64.
+//     the mixed-precision path); every dot product accumulates in FP64.  The
+//     FP32 inner solve runs by default on the scaled system D^-1/2 A D^-1/2
+//     (unit diagonal): Jacobi-PCG in the coordinates x^ = D^1/2 x, where
+//     z = r^, so the SpMV and the direction update read no diagonal; only
+//     r.r = sum d r^2 (the stopping test) still reads it.
 //   * Iterations are launched in pairs (parity 0, 1) captured in hipGraphs; the
 //     host polls the state every few iterations.
 #include <algorithm>
@@ -198,7 +202,8 @@ constexpr int kRowsPerGroup = 2;
 
 // ---------------------------------------------------------------- start
 // r = b - A x (FP64), p = z = r / d in the working precision T; partials
-// b.b, r.z, r.r.  For T = float the FP64 residual is rounded once.
+// b.b, r.z, r.r.  For T = float the FP64 residual is rounded once (on the
+// scaled system: r^ = D^-1/2 r rounded once, p = r^, partial r^.r^).
 template <typename T, int G>
 __global__ void __launch_bounds__(kBlock) cgStart(CgArgs a) {
     int axis;
@@ -230,8 +235,9 @@ __global__ void __launch_bounds__(kBlock) cgStart(CgArgs a) {
             const uint32_t i = row[q];
             if (valid[q] && lane == 0) {
                 const double ri = s.rhs[i] - (s.diag[i] * x[i] + off[q]);
-                const T rt = static_cast<T>(ri);
-                const T zt = rt / dT[i];
+                // scaled system: r^ = D^-1/2 r, and z = r^ (unit diagonal)
+                const T rt = static_cast<T>(a.unitDiag ? ri / sqrt(s.diag[i]) : ri);
+                const T zt = a.unitDiag ? rt : rt / dT[i];
                 r[i] = rt;
                 p[i] = zt;
                 if (e) e[i] = T(0);
@@ -305,6 +311,7 @@ __global__ void __launch_bounds__(kBlock) cgSpmv(CgArgs a, int parity) {
     T *__restrict__ ap = static_cast<T *>(a.ap[axis]);
     const T *__restrict__ val = Mat<T>::val(s);
     const T *__restrict__ diag = Mat<T>::diag(s);
+    const bool unit = sizeof(T) == 4 && a.unitDiag;  // the scaled FP32 system: d = 1, not read
     const uint32_t lane = threadIdx.x % G, grp = threadIdx.x / G;
     const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
     double acc[1] = {0.0};
@@ -321,7 +328,7 @@ __global__ void __launch_bounds__(kBlock) cgSpmv(CgArgs a, int parity) {
                 valid[q] = row[q] < end;
                 const uint32_t i = valid[q] ? row[q] : begin;
                 pi[q] = ownAt(src, i);
-                di[q] = diag[i];
+                di[q] = unit ? T(1) : diag[i];
             }
             rowsDot<T, G, R>(s.rowPtr, s.col, val, src, row, valid, lane, off);
 #pragma unroll
@@ -512,6 +519,8 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
     T *__restrict__ ap = static_cast<T *>(a.ap[axis]);
     const T *__restrict__ val = Mat<T>::val(s);
     const T *__restrict__ diag = Mat<T>::diag(s);
+    // the scaled FP32 system (mixed precision): unit diagonal, not read
+    const bool unit = MODE == 0 && sizeof(T) == 4 && a.unitDiag;
     const uint4 info = s.flatInfo[b];
     const uint32_t R0 = info.x, R1 = info.y, E0 = info.z, count = info.w - info.z;
     double acc[K] = {};
@@ -526,8 +535,9 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
             const double ri = rhs - sum;
             if constexpr (MODE >= 3) {
                 using W = std::conditional_t<MODE == 3, float, double>;
-                const W rt = static_cast<W>(ri);
-                const W zt = rt / Mat<W>::diag(s)[r];
+                const bool scaled = MODE == 3 && a.unitDiag;  // r^ = D^-1/2 r, z = r^
+                const W rt = static_cast<W>(scaled ? ri / sqrt(s.diag[r]) : ri);
+                const W zt = scaled ? rt : rt / Mat<W>::diag(s)[r];
                 static_cast<W *>(a.r[axis])[r] = rt;
                 static_cast<W *>(a.p[axis])[r] = zt;
                 if constexpr (MODE == 3) static_cast<float *>(a.e[axis])[r] = 0.0f;
@@ -538,9 +548,10 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
             }
             acc[0] += ri * ri;
             if constexpr (MODE == 1) {
-                const float rf = static_cast<float>(ri);
+                const float rf = static_cast<float>(a.unitDiag ? ri / sqrt(s.diag[r]) : ri);
                 static_cast<float *>(a.r[axis])[r] = rf;
-                acc[1] += static_cast<double>(rf) * static_cast<double>(rf / s.diag32[r]);
+                acc[1] += static_cast<double>(rf) *
+                          static_cast<double>(a.unitDiag ? rf : rf / s.diag32[r]);
             } else {
                 acc[1] += rhs * rhs;
             }
@@ -563,7 +574,7 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
             for (int q = 0; q < R; ++q) {
                 if (valid[q] && lane == 0) {
                     const double pd = static_cast<double>(p[row[q]]);
-                    finish(row[q], pd, static_cast<double>(diag[row[q]]) * pd + off[q]);
+                    finish(row[q], pd, (unit ? pd : static_cast<double>(diag[row[q]]) * pd) + off[q]);
                 }
             }
         }
@@ -589,7 +600,7 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
             pre0 = s.rowPtr[rq];
             pre1 = s.rowPtr[rq + 1];
             preP = p[rq];
-            preD = diag[rq];
+            preD = unit ? T(1) : diag[rq];
         };
         const auto chunkRow = [&](uint32_t chunk) {
             return R0 + chunk * (kGroups * G2) + kGroups * (lane64 / kWaveGroups) + kWaveGroups * wave +
@@ -757,6 +768,7 @@ __global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a, int parity) {
     T *__restrict__ r = static_cast<T *>(a.r[axis]);
     const T *__restrict__ ap = static_cast<const T *>(a.ap[axis]);
     const T *__restrict__ d = Mat<T>::diag(s);
+    const bool unit = sizeof(T) == 4 && a.unitDiag;
     const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
     // the partials are loaded first: the wait for them (an in-order count of
     // outstanding loads) then leaves the vector loads below in flight
@@ -794,8 +806,13 @@ __global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a, int parity) {
         const T rn = ri - alphaT * ai;
         r[i] = rn;
         const double rd = static_cast<double>(rn);
-        acc[0] += rd * rd;
-        acc[1] += rd * static_cast<double>(rn / di);
+        if (unit) {  // r^ = D^-1/2 r: r.r = sum d r^2, r.z = r^.r^
+            acc[0] += static_cast<double>(di) * (rd * rd);
+            acc[1] += rd * rd;
+        } else {
+            acc[0] += rd * rd;
+            acc[1] += rd * static_cast<double>(rn / di);
+        }
     };
     if constexpr (PER > 0) {
 #pragma unroll
@@ -846,6 +863,7 @@ __global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
     T *__restrict__ p = static_cast<T *>(a.p[axis]);
     const T *__restrict__ r = static_cast<const T *>(a.r[axis]);
     const T *__restrict__ d = Mat<T>::diag(s);
+    const bool unit = sizeof(T) == 4 && a.unitDiag;  // z = r^ on the scaled system: d not read
     const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
     PartialShare<2> share;
     share.issue(a.partB, s.blocks, a.stride, axis);
@@ -860,7 +878,7 @@ __global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
             const uint32_t j = i < end ? i : begin;
             xv[k] = x[j];
             rv[k] = r[j];
-            dv[k] = d[j];
+            dv[k] = unit ? T(1) : d[j];
             pv[k] = p[j];
         }
     }
@@ -879,14 +897,14 @@ __global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
             const uint32_t i = begin + threadIdx.x + k * kBlock;
             if (i < end) {
                 x[i] = xv[k] + alphaT * pv[k];
-                if (ok) p[i] = rv[k] / dv[k] + beta * pv[k];
+                if (ok) p[i] = (unit ? rv[k] : rv[k] / dv[k]) + beta * pv[k];
             }
         }
     } else {
         for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
             const T pi = p[i];
             x[i] = x[i] + alphaT * pi;
-            if (ok) p[i] = r[i] / d[i] + beta * pi;
+            if (ok) p[i] = (unit ? r[i] : r[i] / d[i]) + beta * pi;
         }
     }
     if (lead) {
@@ -912,7 +930,8 @@ __global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
 }
 
 // ---------------------------------------------------------------- FP64 refresh (mixed precision)
-// x64 += e (FP32 correction), e = 0, for axes that asked for a refresh.
+// x64 += e (FP32 correction; on the scaled system x64 += D^-1/2 e), e = 0, for
+// axes that asked for a refresh.
 __global__ void __launch_bounds__(kBlock) cgFlush(CgArgs a) {
     int axis;
     uint32_t tile;
@@ -923,12 +942,14 @@ __global__ void __launch_bounds__(kBlock) cgFlush(CgArgs a) {
     float *__restrict__ e = static_cast<float *>(a.e[axis]);
     const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
     for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
-        x[i] += static_cast<double>(e[i]);
+        const double ei = static_cast<double>(e[i]);
+        x[i] += a.unitDiag ? ei / sqrt(s.diag[i]) : ei;
         e[i] = 0.0f;
     }
 }
 
-// r64 = b - A x64 (FP64), r = (float) r64; partials r64.r64 and r.z.
+// r64 = b - A x64 (FP64), r = (float) r64 (scaled system: (float) D^-1/2 r64);
+// partials r64.r64 and r.z.
 template <int G>
 __global__ void __launch_bounds__(kBlock) cgRefresh(CgArgs a) {
     int axis;
@@ -957,10 +978,10 @@ __global__ void __launch_bounds__(kBlock) cgRefresh(CgArgs a) {
             const uint32_t i = row[q];
             if (valid[q] && lane == 0) {
                 const double ri = s.rhs[i] - (s.diag[i] * x[i] + off[q]);
-                const float rf = static_cast<float>(ri);
+                const float rf = static_cast<float>(a.unitDiag ? ri / sqrt(s.diag[i]) : ri);
                 r[i] = rf;
                 acc[0] += ri * ri;
-                acc[1] += static_cast<double>(rf) * static_cast<double>(rf / s.diag32[i]);
+                acc[1] += static_cast<double>(rf) * static_cast<double>(a.unitDiag ? rf : rf / s.diag32[i]);
             }
         }
     }
@@ -1069,6 +1090,23 @@ toFloat(const double *__restrict__ in, float *__restrict__ out, uint64_t n) {
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; i < n;
          i += static_cast<uint64_t>(gridDim.x) * kBlock)
         out[i] = static_cast<float>(in[i]);
+}
+
+// FP32 copy of the symmetrically scaled system D^-1/2 A D^-1/2 (mixed
+// precision): val_ij / (sqrt(d_i) sqrt(d_j)) per entry, four lanes per row;
+// its diagonal is 1.  d_i s_j and d_j s_i are the same product, so the
+// scaled matrix stays symmetric entry for entry.
+__global__ void __launch_bounds__(kBlock)
+scaledToFloat(const uint32_t *__restrict__ rowPtr, const uint32_t *__restrict__ col,
+              const double *__restrict__ val, const double *__restrict__ diag, float *__restrict__ out,
+              uint32_t rows) {
+    const uint64_t t = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
+    const uint32_t lane = static_cast<uint32_t>(t & 3u);
+    const uint64_t i = t >> 2;
+    if (i >= rows) return;
+    const double si = sqrt(diag[i]);
+    for (uint32_t k = rowPtr[i] + lane; k < rowPtr[i + 1]; k += 4)
+        out[k] = static_cast<float>(val[k] / (si * sqrt(diag[col[k]])));
 }
 
 // Plain product out = A in (SparseSystem::multiply order: diag first, then
@@ -1437,6 +1475,7 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
     a.col24 = (a.layout == 1 && ctx->tune.colBytes == 3 && a.flatThreads == 256 && a.flatUnroll == 4 &&
                a.flatEntries == 2048 && rows < (1ull << 24)) ? 1u : 0u;
     // wide loads: entry blocks of 256 threads and 2048 entries, 4 lanes per row
+    a.unitDiag = fp32 && ctx->tune.mixedScaling != 0 ? 1u : 0u;
     a.wide = (a.layout == 1 && ctx->tune.wideLoads && a.flatThreads == 256 && a.flatEntries == 2048 &&
               a.flatRowLanes == 4) ? 1u : 0u;
     a.col24 = a.col24 || (a.wide && ctx->tune.colBytes == 3 && rows < (1ull << 24)) ? 1u : 0u;
@@ -1454,13 +1493,21 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
     for (int ax = 0; ax < 2; ++ax) {
         if (fp32) {
             Sys &s = ctx->sys[ax];
-            if (!s.have32) {
+            const bool scaled = ctx->tune.mixedScaling != 0;
+            if (!s.have32 || s.scaled32 != scaled) {
                 KT_TRY(s.val32.reserve(nnz[ax] + kQuadPad));
                 KT_TRY(s.diag32.reserve(std::max<uint64_t>(rows, 1)));
-                if (nnz[ax]) {
+                if (nnz[ax] && scaled) {
+                    const uint64_t threads = 4ull * rows;
+                    scaledToFloat<<<dim3(static_cast<uint32_t>((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                                    ctx->stream>>>(s.rowPtr.p, s.col.p, s.val.p, s.diag.p, s.val32.p,
+                                                   static_cast<uint32_t>(rows));
+                    KT_HIP(hipGetLastError());
+                } else if (nnz[ax]) {
                     toFloat<<<dim3(1024), dim3(kBlock), 0, ctx->stream>>>(s.val.p, s.val32.p, nnz[ax]);
                     KT_HIP(hipGetLastError());
                 }
+                s.scaled32 = scaled;
                 if (rows) {
                     toFloat<<<dim3(256), dim3(kBlock), 0, ctx->stream>>>(s.diag.p, s.diag32.p, rows);
                     KT_HIP(hipGetLastError());
@@ -1625,7 +1672,9 @@ int solveFp64(kt_ctx *ctx, const kt_solve_opts &o, kt_solve_result &res) {
 // residual is recomputed as b - A x in FP64 and rounded once to FP32, and the
 // FP64 residual norm becomes the axis' residual.  An axis stops only on the
 // true FP64 residual.  Every dot product (alpha, beta, residuals) accumulates
-// in FP64.
+// in FP64.  With the scaling on (the default), the FP32 values are those of
+// D^-1/2 A D^-1/2, the FP32 residual and correction are D^-1/2 r and D^1/2 e,
+// and the refreshes convert between the two.
 int solveMixed(kt_ctx *ctx, const kt_solve_opts &o, kt_solve_result &res) {
     CgArgs a;
     int G;

@@ -132,6 +132,7 @@ struct Sys {
     DevBuf<double> val, diag, rhs;
     DevBuf<float> val32, diag32;
     bool have32 = false;
+    bool scaled32 = false;  // val32 holds val_ij / (sqrt(d_i) sqrt(d_j)) rather than val_ij
     DevBuf<uint16_t> colLo;  // 3-byte columns: low 16 bits
     DevBuf<uint8_t> colHi;   // 3-byte columns: high 8 bits
     bool have24 = false;
@@ -166,6 +167,8 @@ struct CgArgs {
     uint32_t col24;        // entry blocks: 1 = columns read as 16 + 8 bits
     uint32_t flatResidual; // 1: FP64 residuals (refresh, true residual) by entry blocks of 256 threads
     uint32_t wide;         // entry blocks: 1 = four consecutive entries per thread by wide loads
+    uint32_t unitDiag;     // mixed precision: the FP32 system is D^-1/2 A D^-1/2 (unit diagonal);
+                           // r, p, Ap and the correction e live in those coordinates
 };
 
 // Launch-shape knobs (set by the autotuner or the caller; 0 = default).
@@ -184,6 +187,7 @@ struct Tune {
     int flatThreads = 0;      // entry blocks: threads per block (0: 1024; 256, 512)
     int colBytes = 0;         // entry blocks: bytes per column index (0, 4; 3 with 256 threads)
     int wideLoads = 0;        // entry blocks: four consecutive entries per thread by wide loads
+    int mixedScaling = 1;     // mixed precision: FP32 solve on the symmetrically scaled system
 };
 
 // A captured hipGraph of kPairs iteration pairs for one argument set.

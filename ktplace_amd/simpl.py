@@ -106,6 +106,7 @@ _SIGS = {
     "kt_set_spmv_layout": (I32, [P, I32, U32, I32, I32, I32]),
     "kt_set_spmv_columns": (I32, [P, I32]),
     "kt_set_spmv_wide_loads": (I32, [P, I32]),
+    "kt_set_mixed_scaling": (I32, [P, I32]),
     "kt_kernel_times": (I32, [P, I32, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "kt_reset_kernel_times": (I32, [P]),
     "kt_probe_copy": (I32, [P, U64, I32, C.POINTER(C.c_double)]),
@@ -461,6 +462,10 @@ class Context:
                                       block_threads))
         _check(lib.kt_set_spmv_columns(self.h, col_bytes))
         _check(lib.kt_set_spmv_wide_loads(self.h, int(wide)))
+
+    def set_mixed_scaling(self, on: bool):
+        """Mixed precision: FP32 iterations on D^-1/2 A D^-1/2 (default) or on A."""
+        _check(lib.kt_set_mixed_scaling(self.h, int(on)))
 
     def kernel_time(self, kernel: int):
         n = C.c_uint64()

@@ -184,6 +184,41 @@ def test_mixed_precision_reaches_true_tolerance(b2b12k, gpu_ctx_factory):
     ctx.close()
 
 
+@pytest.mark.parametrize("layout", ["lane groups", "entry blocks", "entry blocks, wide"])
+def test_mixed_scaled_and_unscaled_agree(b2b12k, gpu_ctx_factory, layout):
+    """The FP32 iterations on D^-1/2 A D^-1/2 (the default) and on A: both
+    reach a true FP64 residual of 1e-9 within the same iteration count give
+    or take 3 %, agree with the oracle solved to 1e-12, and the scaled solve
+    repeats bit for bit.  A scaled solve that left out a D^-1/2 anywhere
+    (start, refresh or correction) would stall or miss the oracle."""
+    from ktplace_amd.simpl import KT_PREC_MIXED, KT_SPMV_ENTRY_BLOCKS, KT_STOP_TRUE
+    d, o, x0, y0 = b2b12k
+    ctx = gpu_ctx_factory(d)
+    ctx.build_b2b(x0, y0)
+    ref = o.solve(x0, y0, tol=1e-12, max_iter=100000)
+    if layout == "lane groups":
+        tuning = dict(lanes=4, tile=1024, rows=2)
+    else:
+        tuning = dict(lanes=4, tile=1024, rows=2, layout=KT_SPMV_ENTRY_BLOCKS, block_entries=2048, unroll=4,
+                      row_lanes=4, block_threads=256, col_bytes=3, wide=int(layout.endswith("wide")))
+    ctx.set_tuning(**tuning)
+    runs = {}
+    for scaled in (True, False, True):
+        ctx.set_mixed_scaling(scaled)
+        x, y = x0.copy(), y0.copy()
+        r = ctx.solve_both(x, y, 1e-9, 100000, precision=KT_PREC_MIXED, stop_rule=KT_STOP_TRUE)
+        print(layout, "scaled" if scaled else "unscaled", r["steps"], r["refinements"], r["true_resid"], flush=True)
+        assert max(r["true_resid"]) <= 1e-9
+        assert rel(x, ref["x"]) < 1e-6 and rel(y, ref["y"]) < 1e-6
+        if scaled in runs:
+            assert np.array_equal(runs[scaled][0], x) and np.array_equal(runs[scaled][1], y)
+        runs[scaled] = (x, y, r)
+    for ax in (0, 1):
+        a, b = runs[True][2]["steps"][ax], runs[False][2]["steps"][ax]
+        assert abs(a - b) <= max(2, 0.03 * b)
+    ctx.close()
+
+
 @pytest.mark.parametrize("lanes,tile,graphs,rows", [(1, 1024, False, 2), (2, 2048, True, 4), (8, 4096, True, 8),
                                                     (16, 8192, False, 2)])
 def test_launch_shapes_agree(b2b12k, gpu_ctx_factory, lanes, tile, graphs, rows):
