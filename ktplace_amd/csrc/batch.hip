@@ -424,23 +424,21 @@ __global__ void __launch_bounds__(kBB) bSpmvFlat(BArgs a, int parity) {
 // latency; rows beyond kPre * kBB of a tile are loaded after.
 constexpr int kPre = 4;
 
+// alpha = rho / p.Ap; r -= alpha Ap; partials r.r, r.z.  x += alpha p is left
+// to bPUpdate, which reads p anyway (alpha through the system's control block).
 __global__ void __launch_bounds__(kBB) bUpdate(BArgs a, int parity) {
     const Tile t = decode(a);
     AxisCtl &c = a.ctl[t.sys];
     const CgState st = c.s[parity];
     if (!st.live) return;
-    double *__restrict__ x = a.x;
     double *__restrict__ r = a.r;
-    const double *__restrict__ p = a.p;
     const double *__restrict__ ap = a.ap;
     const double *__restrict__ dg = a.diag;
-    double xv[kPre], pv[kPre], rv[kPre], av[kPre], dv[kPre];
+    double rv[kPre], av[kPre], dv[kPre];
 #pragma unroll
     for (int k = 0; k < kPre; ++k) {
         const uint32_t i = t.begin + threadIdx.x + k * kBB;
         const uint32_t j = i < t.end ? i : t.begin;  // unconditional loads; a tile has a row
-        xv[k] = x[j];
-        pv[k] = p[j];
         rv[k] = r[j];
         av[k] = ap[j];
         dv[k] = dg[j];
@@ -453,11 +451,13 @@ __global__ void __launch_bounds__(kBB) bUpdate(BArgs a, int parity) {
         if (t.first && threadIdx.x == 0) c.brk[parity] = 1;
         return;
     }
-    if (t.first && threadIdx.x == 0) c.brk[parity] = 0;
     const double alpha = st.rho / pap[0];
+    if (t.first && threadIdx.x == 0) {
+        c.brk[parity] = 0;
+        c.alpha[parity] = alpha;
+    }
     double acc[2] = {0.0, 0.0};
-    const auto step = [&](uint32_t i, double xi, double pi, double ri0, double ai, double di) {
-        x[i] = xi + alpha * pi;
+    const auto step = [&](uint32_t i, double ri0, double ai, double di) {
         const double ri = ri0 - alpha * ai;
         r[i] = ri;
         acc[0] += ri * ri;
@@ -466,9 +466,9 @@ __global__ void __launch_bounds__(kBB) bUpdate(BArgs a, int parity) {
 #pragma unroll
     for (int k = 0; k < kPre; ++k) {
         const uint32_t i = t.begin + threadIdx.x + k * kBB;
-        if (i < t.end) step(i, xv[k], pv[k], rv[k], av[k], dv[k]);
+        if (i < t.end) step(i, rv[k], av[k], dv[k]);
     }
-    for (uint32_t i = t.begin + threadIdx.x + kPre * kBB; i < t.end; i += kBB) step(i, x[i], p[i], r[i], ap[i], dg[i]);
+    for (uint32_t i = t.begin + threadIdx.x + kPre * kBB; i < t.end; i += kBB) step(i, r[i], ap[i], dg[i]);
     blockSum<2>(acc);
     writePartial<2>(a.partB, acc);
 }
@@ -492,14 +492,17 @@ __global__ void __launch_bounds__(kBB) bPUpdate(BArgs a, int parity) {
         }
         return;
     }
+    double *__restrict__ x = a.x;
     double *__restrict__ p = a.p;
     const double *__restrict__ r = a.r;
     const double *__restrict__ dg = a.diag;
-    double rv[kPre], dv[kPre], pv[kPre];
+    const double alpha = c.alpha[parity];
+    double xv[kPre], rv[kPre], dv[kPre], pv[kPre];
 #pragma unroll
     for (int k = 0; k < kPre; ++k) {
         const uint32_t i = t.begin + threadIdx.x + k * kBB;
         const uint32_t j = i < t.end ? i : t.begin;
+        xv[k] = x[j];
         rv[k] = r[j];
         dv[k] = dg[j];
         pv[k] = p[j];
@@ -509,14 +512,21 @@ __global__ void __launch_bounds__(kBB) bPUpdate(BArgs a, int parity) {
     const double resid = sqrt(tot[0]) / c.bNorm;
     const double rhoNext = tot[1];
     const bool ok = rhoNext > 0.0;
-    if (ok) {
-        const double beta = rhoNext / st.rho;
+    // x steps whatever r.z turns out to be (simpl_lss.cc:247,253); p only when
+    // the system goes on
+    const double beta = ok ? rhoNext / st.rho : 0.0;
 #pragma unroll
-        for (int k = 0; k < kPre; ++k) {
-            const uint32_t i = t.begin + threadIdx.x + k * kBB;
-            if (i < t.end) p[i] = rv[k] / dv[k] + beta * pv[k];
+    for (int k = 0; k < kPre; ++k) {
+        const uint32_t i = t.begin + threadIdx.x + k * kBB;
+        if (i < t.end) {
+            x[i] = xv[k] + alpha * pv[k];
+            if (ok) p[i] = rv[k] / dv[k] + beta * pv[k];
         }
-        for (uint32_t i = t.begin + threadIdx.x + kPre * kBB; i < t.end; i += kBB) p[i] = r[i] / dg[i] + beta * p[i];
+    }
+    for (uint32_t i = t.begin + threadIdx.x + kPre * kBB; i < t.end; i += kBB) {
+        const double pi = p[i];
+        x[i] = x[i] + alpha * pi;
+        if (ok) p[i] = r[i] / dg[i] + beta * pi;
     }
     if (lead) {
         CgState nx;
