@@ -16,7 +16,8 @@
 //   refCounts        per pinsOfCell reference: edges and CSR entries     (:47-82)
 //   scans            each reference's first edge and CSR slot; rowPtr     (:83)
 //   edgeFill         w and rhs term per edge, col / val = -w              (:89-104)
-//   rowSums          diag and rhs per row in the row walk's order, anchor (:105-113)
+//                    and diag / rhs of the rows inside its edge block     (:105-113)
+//   rowSumsRest      diag / rhs of the rows across edge blocks or with no edges
 #include "kt_internal.h"
 
 namespace kt {
@@ -63,19 +64,23 @@ pinStage(const uint32_t *__restrict__ pinCell, const uint2 *__restrict__ varMov,
     pin[k] = PinRec{px, movable ? px - cx : px, py, movable ? py - cy : py};
 }
 
-// Everything a row's reference to a net reads about the net, in one 64-byte
-// record (one memory sector): its pin range, movable pins and edge weight
-// base (weight / (pins - 1), simpl_lss.cc:55), and this build's first argmin /
-// argmax per axis with their pins' variables.  The row kernels visit nets in
-// random order, so one sector per visit instead of one per array.
-struct alignas(64) NetRec {
+// Everything a row's reference to a net reads about the net, in one 128-byte
+// record (one cache line): its pin range, movable pins and edge weight base
+// (weight / (pins - 1), simpl_lss.cc:55), this build's first argmin / argmax
+// per axis with their pins' variables, and those two pins' position and rhs
+// term per axis.  The references visit nets in random order, so one line per
+// visit instead of one per array; and an edge to lo or hi (every edge of an
+// inner pin, the one edge of a two-pin net) finds the far pin in the record
+// instead of a pin record of its own.
+struct alignas(128) NetRec {
     uint32_t start, size, mov, pad;
     uint32_t lox, hix, loy, hiy;
     uint32_t varLoX, varHiX, varLoY, varHiY;
     double base;
     double unused;
+    double ext[2][4];  // per axis: position and rhs term of lo, then of hi
 };
-static_assert(sizeof(NetRec) == 64, "one sector per net");
+static_assert(sizeof(NetRec) == 128, "one line per net");
 
 // One thread per net; the running best value is the reference's pinAt(lo),
 // recomputed there from the same operands, so the comparisons are identical.
@@ -88,16 +93,16 @@ netExtremes(const uint32_t *__restrict__ netStart, const uint32_t *__restrict__ 
     const uint32_t b = netStart[j];
     const uint32_t p = netStart[j + 1] - b;
     uint32_t lox = 0, hix = 0, loy = 0, hiy = 0;
+    PinRec pLoX{}, pHiX{}, pLoY{}, pHiY{};
     if (p > 0) {
-        double bestLoX = pin[b].posX, bestHiX = bestLoX;
-        double bestLoY = pin[b].posY, bestHiY = bestLoY;
+        const PinRec first = pin[b];
+        pLoX = pHiX = pLoY = pHiY = first;
         for (uint32_t q = 1; q < p; ++q) {
-            const double px = pin[b + q].posX;
-            const double py = pin[b + q].posY;
-            if (px < bestLoX) { lox = q; bestLoX = px; }
-            if (px > bestHiX) { hix = q; bestHiX = px; }
-            if (py < bestLoY) { loy = q; bestLoY = py; }
-            if (py > bestHiY) { hiy = q; bestHiY = py; }
+            const PinRec c = pin[b + q];
+            if (c.posX < pLoX.posX) { lox = q; pLoX = c; }
+            if (c.posX > pHiX.posX) { hix = q; pHiX = c; }
+            if (c.posY < pLoY.posY) { loy = q; pLoY = c; }
+            if (c.posY > pHiY.posY) { hiy = q; pHiY = c; }
         }
     }
     uint4 *dst = reinterpret_cast<uint4 *>(rec + j);
@@ -106,7 +111,12 @@ netExtremes(const uint32_t *__restrict__ netStart, const uint32_t *__restrict__ 
     dst[2] = p > 0 ? make_uint4(varMov[b + lox].x, varMov[b + hix].x, varMov[b + loy].x, varMov[b + hiy].x)
                    : make_uint4(kNoVar, kNoVar, kNoVar, kNoVar);
     // used only where lo != hi, i.e. p >= 2
-    rec[j].base = weight[j] / static_cast<double>(p - 1);
+    reinterpret_cast<double2 *>(dst)[3] = make_double2(weight[j] / static_cast<double>(p - 1), 0.0);
+    double2 *ext = reinterpret_cast<double2 *>(rec[j].ext[0]);
+    ext[0] = make_double2(pLoX.posX, pLoX.termX);
+    ext[1] = make_double2(pHiX.posX, pHiX.termX);
+    ext[2] = make_double2(pLoY.posY, pLoY.termY);
+    ext[3] = make_double2(pHiY.posY, pHiY.termY);
 }
 
 // The assembly is organised around the pinsOfCell references (e: row i's
@@ -119,11 +129,13 @@ netExtremes(const uint32_t *__restrict__ netStart, const uint32_t *__restrict__ 
 //   scans      exclusive scans of both counts over the references: each
 //              reference's first edge slot and first CSR slot, so rowPtr[i]
 //              is the CSR scan at the row's first reference
-//   edgeFill   each edge's weight w and rhs term w (there - own offset) into
-//              an edge list in row-walk order, and col / val = -w of the
-//              movable ones straight into their CSR slots, one lane per edge
-//   rowSums    diag and rhs of each row as sequential sums over its edge
-//              list, in the row walk's order, then the anchor term
+//   edgeFill   each edge's weight w and rhs term w (there - own offset), in
+//              row-walk order, and col / val = -w of the movable ones straight
+//              into their CSR slots, one lane per edge
+//   row sums   diag and rhs of each row as sequential sums over its edges in
+//              the row walk's order, then the anchor term: in edgeFill, from
+//              LDS, for the rows whose run lies in one edge block; in
+//              rowSumsRest, from the edge list, for the few others
 // Every double is the reference's expression and every sum keeps its order,
 // so rowPtr, col, val, diag and rhs are the reference's bit for bit.
 struct RefArgs {
@@ -138,6 +150,7 @@ struct RefArgs {
     unsigned long long *edgeOff;  // [Pm + 1] edges: counts at e + 1, then the scan (x | y << 32)
     unsigned long long *entOff;   // [Pm + 1] CSR entries, likewise
     double2 *edges[2];          // per axis: (w, rhs term) per edge in row-walk order
+    uint32_t *rowEdge[2];       // per axis: first edge of each row's run (rows [0, n])
     uint32_t *rowPtr[2];
     uint32_t *col[2];
     double *val[2];
@@ -195,13 +208,17 @@ __global__ void __launch_bounds__(kBlock) refCounts(RefArgs a) {
     a.entOff[e + 1] = static_cast<unsigned long long>(ents[0]) | (static_cast<unsigned long long>(ents[1]) << 32);
 }
 
-// rowPtr[i] = the CSR scan at row i's first reference (rows [0, n]).
+// rowPtr[i] = the CSR scan at row i's first reference, and the row's first
+// edge per axis (rows [0, n]).
 __global__ void __launch_bounds__(kBlock) refRowPtr(RefArgs a) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i > a.n) return;
-    const unsigned long long s = a.entOff[a.pinStart[i]];
+    const uint32_t e = a.pinStart[i];
+    const unsigned long long s = a.entOff[e], t = a.edgeOff[e];
     a.rowPtr[0][i] = half(s, 0);
     a.rowPtr[1][i] = half(s, 1);
+    a.rowEdge[0][i] = half(t, 0);
+    a.rowEdge[1][i] = half(t, 1);
 }
 
 // Edge-parallel fill: the edges of each axis (in row-walk order) are cut into
@@ -209,12 +226,18 @@ __global__ void __launch_bounds__(kBlock) refRowPtr(RefArgs a) {
 // reference (the last one whose first edge is at or before it) and writes
 // that edge.  Consecutive lanes write consecutive edge-list slots and, mostly,
 // consecutive CSR slots, and lanes on one reference share its net record and
-// read the net's pins side by side; no lane walks a long run alone.
-constexpr uint32_t kFillEdges = 2048;
-constexpr uint32_t kFillRefs = 4096;  // references a block keeps in LDS for the lookup
+// read the net's pins side by side; no lane walks a long run alone.  The block
+// keeps its edges in LDS and sums the rows whose run it holds whole; only the
+// edges of rows across block boundaries go to the edge list, for rowSumsRest.
+constexpr uint32_t kFillEdges = 512, kFillThreads = 256;
+constexpr uint32_t kFillRefs = 2048;  // references a block keeps in LDS for the lookup
+
+// The first edge of row i's run on one axis (rows [0, n]).
+__device__ __forceinline__ uint32_t rowEdge(const RefArgs &a, uint32_t i, int axis) { return a.rowEdge[axis][i]; }
 
 // first[axis][b] = the last reference whose first edge is at or before edge
-// b * kFillEdges of that axis.
+// b * kFillEdges of that axis; first[2 blocks + axis blocks + b] = the first
+// row whose run starts at or after that edge.
 __global__ void __launch_bounds__(kBlock) edgeBlockStart(RefArgs a, uint32_t blocks, uint32_t *__restrict__ first) {
     const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
     const int axis = blockIdx.y;
@@ -227,21 +250,70 @@ __global__ void __launch_bounds__(kBlock) edgeBlockStart(RefArgs a, uint32_t blo
         else hi = mid;
     }
     first[axis * blocks + b] = lo - 1;  // edgeOff[0] = 0 <= start, so lo >= 1
+    lo = 0;
+    hi = a.n;  // first i in [0, n] with rowEdge(i) >= start (rowEdge(n) = all edges)
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (rowEdge(a, mid, axis) < start) lo = mid + 1;
+        else hi = mid;
+    }
+    first[2 * blocks + axis * blocks + b] = lo;
 }
 
-__global__ void __launch_bounds__(kBlock) edgeFill(RefArgs a, uint32_t blocks, const uint32_t *__restrict__ first) {
+// diag and rhs of row i from its edges' (w, rhs term) in the row walk's order
+// (simpl_lss.cc:105-113): sequential sums, then the anchor term and the floor.
+template <typename Edge>
+__device__ __forceinline__ void rowTotals(const RefArgs &a, int axis, uint32_t i, Edge edge, uint32_t e0,
+                                          uint32_t e1) {
+    // the loads of eight edges are issued together (long runs: a bound pin
+    // of a 200-pin net), the sums still run one edge after the other
+    constexpr uint32_t kBatch = 8;
+    double diag = 0.0, rhs = 0.0;
+    for (uint32_t k0 = e0; k0 < e1; k0 += kBatch) {
+        double2 d[kBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) d[u] = edge(k0 + u < e1 ? k0 + u : e1 - 1);
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+            if (k0 + u < e1) {
+                diag += d[u].x;
+                rhs += d[u].y;
+            }
+        }
+    }
+    if (a.alpha > 0.0) {
+        const double anchor = a.anchor[axis][i];
+        const double w = a.alpha / refMax(fabs(a.pos[axis][i] - anchor), a.floor[axis]);
+        diag += w;
+        rhs += w * anchor;
+    }
+    a.diag[axis][i] = refMax(diag, 1e-12);
+    a.rhs[axis][i] = rhs;
+}
+
+__global__ void __launch_bounds__(kFillThreads) edgeFill(RefArgs a, uint32_t blocks, const uint32_t *__restrict__ first) {
     __shared__ uint32_t refOff[kFillRefs];
     __shared__ uint32_t refCount;
+    __shared__ double2 stage[kFillEdges];  // the block's edges, for the row sums below
+    __shared__ uint32_t headEnd, tailStart;  // edges of rows across the block's start / end
     const int axis = blockIdx.y;
     const uint32_t total = half(a.edgeOff[a.Pm], axis);
     const uint32_t start = blockIdx.x * kFillEdges;
     if (start >= total) return;  // block-uniform
     const uint32_t end = min(start + kFillEdges, total);
     const uint32_t e0 = first[axis * blocks + blockIdx.x];
+    // the first rows whose run starts in the block: their run bounds are loaded
+    // now, so they arrive while the edges are filled
+    const uint32_t r0 = first[2 * blocks + axis * blocks + blockIdx.x];
+    const uint32_t rFirst = r0 + threadIdx.x < a.n ? r0 + threadIdx.x : a.n - 1;  // n > 0 here
+    const uint32_t pa = rowEdge(a, rFirst, axis), pb = rowEdge(a, rFirst + 1, axis);
     // the block's references e0, e0 + 1, ... up to the one holding edge end - 1
-    if (threadIdx.x == 0) refCount = kFillRefs + 1;
+    if (threadIdx.x == 0) {
+        refCount = kFillRefs + 1;
+        tailStart = end;
+    }
     __syncthreads();
-    for (uint32_t base = 0; base < kFillRefs; base += kBlock) {
+    for (uint32_t base = 0; base < kFillRefs; base += kFillThreads) {
         const uint32_t r = base + threadIdx.x;
         const uint32_t e = e0 + r;
         if (r < kFillRefs && e <= a.Pm) {
@@ -250,10 +322,10 @@ __global__ void __launch_bounds__(kBlock) edgeFill(RefArgs a, uint32_t blocks, c
             if (o >= end) atomicMin(&refCount, r);  // references from here on start past the block
         }
         __syncthreads();
-        if (refCount <= base + kBlock) break;  // block-uniform
+        if (refCount <= base + kFillThreads) break;  // block-uniform
     }
     const uint32_t R = refCount;  // the block's references are e0 .. e0 + R - 1 when R <= kFillRefs
-    for (uint32_t k = start + threadIdx.x; k < end; k += kBlock) {
+    for (uint32_t k = start + threadIdx.x; k < end; k += kFillThreads) {
         uint32_t e, t;
         if (R <= kFillRefs) {  // largest r < R with refOff[r] <= k
             uint32_t lo = 0, hi = R;
@@ -276,65 +348,92 @@ __global__ void __launch_bounds__(kBlock) edgeFill(RefArgs a, uint32_t blocks, c
         }
         const uint32_t net = a.refNet[e];
         const uint32_t q = a.refPin[e];
-        const uint4 head = recHead(a.rec, net), x = recExt(a.rec, net);
+        const uint4 head = recHead(a.rec, net), x = recExt(a.rec, net), v = recVars(a.rec, net);
         const uint32_t lo = extLo(x, axis), hi = extHi(x, axis);
         const uint32_t b = head.x;
         const bool bound = q == lo || q == hi;
         const double base = a.rec[net].base;  // weight / (p - 1)
-        const PinRec ownPin = a.pin[b + q];
-        const double own = pinPos(ownPin, axis), ownOffset = pinTerm(ownPin, axis);
+        const double2 *ext = reinterpret_cast<const double2 *>(a.rec[net].ext[axis]);
+        const double2 eLo = ext[0], eHi = ext[1];  // (position, rhs term) of lo and of hi
+        double own, ownOffset;
+        if (bound) {
+            own = q == lo ? eLo.x : eHi.x;
+            ownOffset = q == lo ? eLo.y : eHi.y;
+        } else {
+            const PinRec ownPin = a.pin[b + q];
+            own = pinPos(ownPin, axis);
+            ownOffset = pinTerm(ownPin, axis);
+        }
         const uint32_t other = bound ? (t < q ? t : t + 1) : (t == 0 ? lo : hi);
-        const PinRec there = a.pin[b + other];
-        const double w = base / refMax(fabs(own - pinPos(there, axis)), a.floor[axis]);
+        double there, thereTerm;
+        uint32_t var, mi;
+        if (!bound || head.y == 2) {
+            // the far pin is lo or hi (a two-pin net's bound pins are its lo and
+            // hi, lo != hi here): all from the record.  Its CSR slot within the
+            // reference's run: inner pin, lo first (if movable); a two-pin net's
+            // bound pin has the one entry
+            const bool isLo = other == lo;
+            there = isLo ? eLo.x : eHi.x;
+            thereTerm = isLo ? eLo.y : eHi.y;
+            var = isLo ? extLo(v, axis) : extHi(v, axis);
+            mi = bound || t == 0 ? 0u : (extLo(v, axis) != kNoVar ? 1u : 0u);
+        } else {
+            // bound pin of a bigger net: the lanes of the reference read the
+            // net's pins side by side
+            const PinRec th = a.pin[b + other];
+            there = pinPos(th, axis);
+            thereTerm = pinTerm(th, axis);
+            const uint2 vm = a.varMov[b + other];
+            var = vm.x;
+            mi = vm.y - (other > q ? 1u : 0u);
+        }
+        const double w = base / refMax(fabs(own - there), a.floor[axis]);
         // movable: (there - cellPos) - ownOffset; fixed: there - ownOffset
-        a.edges[axis][k] = make_double2(w, w * (pinTerm(there, axis) - ownOffset));
-        const uint2 vm = a.varMov[b + other];
-        if (vm.x != kNoVar) {
-            const uint32_t loMovable = a.varMov[b + lo].x != kNoVar ? 1u : 0u;
-            const uint32_t mi = bound ? vm.y - (other > q ? 1u : 0u) : (t == 0 ? 0u : loMovable);
+        stage[k - start] = make_double2(w, w * (thereTerm - ownOffset));
+        if (var != kNoVar) {
             const uint32_t slot = half(a.entOff[e], axis) + mi;
-            a.col[axis][slot] = vm.x;
+            a.col[axis][slot] = var;
             a.val[axis][slot] = -w;
         }
     }
+    if (threadIdx.x == 0) headEnd = r0 < a.n ? min(pa, end) : end;
+    __syncthreads();
+    // the rows whose whole run lies in this block, one thread each, from LDS;
+    // rows with no edges or a run across a block boundary are rowSumsRest's
+    for (uint32_t i = r0 + threadIdx.x; i < a.n; i += kFillThreads) {
+        const bool pre = i == r0 + threadIdx.x;
+        const uint32_t ea = pre ? pa : rowEdge(a, i, axis);
+        if (ea >= end) break;  // runs start in order: the later rows are past the block too
+        const uint32_t eb = pre ? pb : rowEdge(a, i + 1, axis);
+        if (ea == eb) continue;
+        if (eb > end) {  // the one row that runs past the block's end
+            tailStart = ea;
+            continue;
+        }
+        rowTotals(a, axis, i, [&](uint32_t k) { return stage[k - start]; }, ea, eb);
+    }
+    __syncthreads();
+    // only the rows across the block's boundaries are summed from the edge
+    // list: the edges before the first row that starts here, and those of
+    // the row that runs past the end
+    double2 *__restrict__ edges = a.edges[axis];
+    const uint32_t h = headEnd, ts = tailStart;
+    for (uint32_t k = start + threadIdx.x; k < h; k += kFillThreads) edges[k] = stage[k - start];
+    for (uint32_t k = ts + threadIdx.x; k < end; k += kFillThreads) edges[k] = stage[k - start];
 }
 
-// A block sums kSumRows consecutive rows.  Their edges are one contiguous run
-// of the edge list; when it fits kSumEdges, the block first copies it to LDS
-// with coalesced loads, then each thread adds its row's edges from there in
-// order.  A longer run (rows with bound pins of big nets) is summed straight
-// from memory.
-constexpr uint32_t kSumRows = 64, kSumEdges = 1536, kSumThreads = 128;
-
-__global__ void __launch_bounds__(kSumThreads) rowSums(RefArgs a) {
-    __shared__ double2 stage[kSumEdges];
-    const uint32_t r0 = blockIdx.x * kSumRows;
-    const uint32_t i = r0 + threadIdx.x;  // threads past kSumRows only help stage
+// diag and rhs of the rows edgeFill left: rows with no edges, and rows whose
+// run crosses a boundary of the fill's edge blocks (at most one per boundary),
+// summed from the edge list in memory.  Every row is computed once, by
+// edgeFill or here.
+__global__ void __launch_bounds__(kBlock) rowSumsRest(RefArgs a) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const int axis = blockIdx.y;
-    const uint32_t r1 = min(r0 + kSumRows, a.n);
-    const uint32_t E0 = half(a.edgeOff[a.pinStart[r0]], axis), E1 = half(a.edgeOff[a.pinStart[r1]], axis);
+    if (i >= a.n) return;
+    const uint32_t ea = rowEdge(a, i, axis), eb = rowEdge(a, i + 1, axis);
+    if (ea < eb && ea / kFillEdges == (eb - 1) / kFillEdges) return;  // summed by edgeFill
     const double2 *__restrict__ edges = a.edges[axis];
-    const bool staged = E1 - E0 <= kSumEdges;  // block-uniform
-    if (staged) {
-        for (uint32_t k = threadIdx.x; k < E1 - E0; k += kSumThreads) stage[k] = edges[E0 + k];
-        __syncthreads();
-    }
-    if (threadIdx.x >= kSumRows || i >= a.n) return;
-    const uint32_t e0 = half(a.edgeOff[a.pinStart[i]], axis), e1 = half(a.edgeOff[a.pinStart[i + 1]], axis);
-    double diag = 0.0, rhs = 0.0;
-    for (uint32_t k = e0; k < e1; ++k) {
-        const double2 d = staged ? stage[k - E0] : edges[k];
-        diag += d.x;
-        rhs += d.y;
-    }
-    if (a.alpha > 0.0) {
-        const double anchor = a.anchor[axis][i];
-        const double w = a.alpha / refMax(fabs(a.pos[axis][i] - anchor), a.floor[axis]);
-        diag += w;
-        rhs += w * anchor;
-    }
-    a.diag[axis][i] = refMax(diag, 1e-12);
-    a.rhs[axis][i] = rhs;
+    rowTotals(a, axis, i, [&](uint32_t k) { return edges[k]; }, ea, eb);
 }
 
 // Inclusive in-place scan of one tile of kScanTile elements; tile totals go to sums.
@@ -455,7 +554,7 @@ int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY) {
     KT_CHECK(!(alpha > 0.0) || ctx->haveAnchors, KT_ESTATE, "buildB2B: alpha > 0 needs anchors");
     const uint32_t n = ctx->n;
     for (int axis = 0; axis < 2; ++axis) KT_TRY(ensureSystem(ctx, axis, n, ctx->nnzBound));
-    KT_TRY(ctx->ext.reserve(std::max<uint64_t>(16ull * ctx->N, 16)));  // one NetRec per net
+    KT_TRY(ctx->ext.reserve(std::max<uint64_t>(32ull * ctx->N, 32)));  // one NetRec (128 bytes) per net
     KT_TRY(vertexPositionsLaunch(ctx));
     const uint64_t pins = std::max<uint64_t>(ctx->P, 1);
     KT_TRY(ctx->pinAt[0].reserve(4 * pins));  // one PinRec (4 doubles) per pin
@@ -487,6 +586,9 @@ int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY) {
     KT_TRY(ctx->b2bEdgeOff.reserve(Pm + 1));
     KT_TRY(ctx->b2bEntOff.reserve(Pm + 1));
     for (int axis = 0; axis < 2; ++axis) KT_TRY(ctx->b2bEdges[axis].reserve(2 * std::max<uint64_t>(ctx->nnzBound, 1)));
+    KT_TRY(ctx->b2bRowEdge.reserve(2ull * (n + 1)));
+    a.rowEdge[0] = ctx->b2bRowEdge.p;
+    a.rowEdge[1] = ctx->b2bRowEdge.p + (n + 1);
     a.edgeOff = reinterpret_cast<unsigned long long *>(ctx->b2bEdgeOff.p);
     a.entOff = reinterpret_cast<unsigned long long *>(ctx->b2bEntOff.p);
     for (int axis = 0; axis < 2; ++axis) {
@@ -525,14 +627,14 @@ int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY) {
     if (Pm > 0) {
         // edges per axis are at most nnzBound (every edge of every net)
         const uint32_t blocks = static_cast<uint32_t>(std::max<uint64_t>((ctx->nnzBound + kFillEdges - 1) / kFillEdges, 1));
-        KT_TRY(ctx->b2bLong.reserve(2ull * blocks));
+        KT_TRY(ctx->b2bLong.reserve(4ull * blocks));
         edgeBlockStart<<<dim3((blocks + kBlock - 1) / kBlock, 2), dim3(kBlock), 0, ctx->stream>>>(a, blocks,
                                                                                                    ctx->b2bLong.p);
         KT_HIP(hipGetLastError());
-        edgeFill<<<dim3(blocks, 2), dim3(kBlock), 0, ctx->stream>>>(a, blocks, ctx->b2bLong.p);
+        edgeFill<<<dim3(blocks, 2), dim3(kFillThreads), 0, ctx->stream>>>(a, blocks, ctx->b2bLong.p);
         KT_HIP(hipGetLastError());
     }
-    rowSums<<<dim3((n + kSumRows - 1) / kSumRows, 2), dim3(kSumThreads), 0, ctx->stream>>>(a);
+    rowSumsRest<<<dim3((n + kBlock - 1) / kBlock, 2), dim3(kBlock), 0, ctx->stream>>>(a);
     KT_HIP(hipGetLastError());
     KT_TRY(timedEnd(ctx, KT_KERNEL_B2B_FILL, t0));
     return KT_OK;

@@ -412,7 +412,7 @@ int kt_ctx_destroy(kt_ctx *ctx) {
     ctx->netStart.release(); ctx->pinCell.release(); ctx->varOfVertex.release(); ctx->movVertex.release();
     ctx->pinStart.release(); ctx->refNet.release(); ctx->refPin.release(); ctx->netMov.release();
     ctx->floating.release(); ctx->hpwlScratch.release(); ctx->loopDev.release();
-    ctx->b2bEdgeOff.release(); ctx->b2bEntOff.release(); ctx->b2bLong.release();
+    ctx->b2bEdgeOff.release(); ctx->b2bEntOff.release(); ctx->b2bLong.release(); ctx->b2bRowEdge.release();
     for (int a = 0; a < 2; ++a) ctx->b2bEdges[a].release();
     ctx->offX.release(); ctx->offY.release(); ctx->weight.release(); ctx->fixedX.release();
     ctx->fixedY.release(); ctx->width.release(); ctx->height.release(); ctx->area.release();
@@ -422,7 +422,7 @@ int kt_ctx_destroy(kt_ctx *ctx) {
     for (int a = 0; a < 2; ++a) ctx->pinAt[a].release();
     for (auto &s : ctx->sys) {
         s.rowPtr.release(); s.col.release(); s.val.release(); s.diag.release(); s.rhs.release();
-        s.val32.release(); s.diag32.release();
+        s.val32.release(); s.diag32.release(); s.isd.release();
     }
     for (int a = 0; a < 2; ++a) {
         ctx->r[a].release(); ctx->p[a].release(); ctx->ap[a].release(); ctx->xs[a].release();
@@ -457,7 +457,7 @@ int kt_ctx_device_bytes(kt_ctx *ctx, uint64_t *bytes) {
     b += ctx->netStart.bytes() + ctx->pinCell.bytes() + ctx->varOfVertex.bytes() + ctx->movVertex.bytes();
     b += ctx->pinStart.bytes() + ctx->refNet.bytes() + ctx->refPin.bytes() + ctx->netMov.bytes();
     b += ctx->floating.bytes() + ctx->hpwlScratch.bytes() + ctx->loopDev.bytes();
-    b += ctx->b2bEdgeOff.bytes() + ctx->b2bEntOff.bytes() + ctx->b2bLong.bytes() + ctx->b2bEdges[0].bytes() +
+    b += ctx->b2bEdgeOff.bytes() + ctx->b2bEntOff.bytes() + ctx->b2bLong.bytes() + ctx->b2bRowEdge.bytes() + ctx->b2bEdges[0].bytes() +
          ctx->b2bEdges[1].bytes();
     b += ctx->offX.bytes() + ctx->offY.bytes() + ctx->weight.bytes() + ctx->fixedX.bytes() + ctx->fixedY.bytes();
     b += ctx->width.bytes() + ctx->height.bytes() + ctx->area.bytes();
@@ -468,7 +468,7 @@ int kt_ctx_device_bytes(kt_ctx *ctx, uint64_t *bytes) {
     b += ctx->scanTmp.bytes();
     for (auto &s : ctx->sys)
         b += s.rowPtr.bytes() + s.col.bytes() + s.val.bytes() + s.diag.bytes() + s.rhs.bytes() + s.val32.bytes() +
-             s.diag32.bytes();
+             s.diag32.bytes() + s.isd.bytes();
     for (int a = 0; a < 2; ++a)
         b += ctx->r[a].bytes() + ctx->p[a].bytes() + ctx->ap[a].bytes() + ctx->xs[a].bytes() + ctx->r32[a].bytes() +
              ctx->p32[a].bytes() + ctx->ap32[a].bytes() + ctx->x32[a].bytes();

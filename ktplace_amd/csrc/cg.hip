@@ -1092,21 +1092,47 @@ toFloat(const double *__restrict__ in, float *__restrict__ out, uint64_t n) {
         out[i] = static_cast<float>(in[i]);
 }
 
+// 1 / sqrt(d_i) per row, for the scaled FP32 system below.
+__global__ void __launch_bounds__(kBlock)
+invSqrtDiag(const double *__restrict__ diag, double *__restrict__ isd, uint32_t rows) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < rows) isd[i] = 1.0 / sqrt(diag[i]);
+}
+
 // FP32 copy of the symmetrically scaled system D^-1/2 A D^-1/2 (mixed
-// precision): val_ij / (sqrt(d_i) sqrt(d_j)) per entry, four lanes per row;
-// its diagonal is 1.  d_i s_j and d_j s_i are the same product, so the
-// scaled matrix stays symmetric entry for entry.
+// precision): val_ij (s_i s_j) per entry with s = 1 / sqrt(d), four lanes per
+// row, each with four entries' loads in flight per round; its diagonal is 1.
+// s_i s_j and s_j s_i are the same product, so the scaled matrix stays
+// symmetric entry for entry.
 __global__ void __launch_bounds__(kBlock)
 scaledToFloat(const uint32_t *__restrict__ rowPtr, const uint32_t *__restrict__ col,
-              const double *__restrict__ val, const double *__restrict__ diag, float *__restrict__ out,
+              const double *__restrict__ val, const double *__restrict__ isd, float *__restrict__ out,
               uint32_t rows) {
     const uint64_t t = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
     const uint32_t lane = static_cast<uint32_t>(t & 3u);
     const uint64_t i = t >> 2;
     if (i >= rows) return;
-    const double si = sqrt(diag[i]);
-    for (uint32_t k = rowPtr[i] + lane; k < rowPtr[i + 1]; k += 4)
-        out[k] = static_cast<float>(val[k] / (si * sqrt(diag[col[k]])));
+    const double si = isd[i];
+    const uint32_t k0 = rowPtr[i], k1 = rowPtr[i + 1];
+    constexpr int kPer = 4;
+    for (uint32_t base = k0 + lane; base < k1; base += 4 * kPer) {
+        uint32_t c[kPer];
+        double v[kPer], sj[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const uint32_t k = base + 4 * u;
+            const uint32_t kk = k < k1 ? k : k0;  // unconditional loads; the row has entry k0
+            c[u] = col[kk];
+            v[u] = val[kk];
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) sj[u] = isd[c[u]];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const uint32_t k = base + 4 * u;
+            if (k < k1) out[k] = static_cast<float>(v[u] * (si * sj[u]));
+        }
+    }
 }
 
 // Plain product out = A in (SparseSystem::multiply order: diag first, then
@@ -1498,9 +1524,13 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
                 KT_TRY(s.val32.reserve(nnz[ax] + kQuadPad));
                 KT_TRY(s.diag32.reserve(std::max<uint64_t>(rows, 1)));
                 if (nnz[ax] && scaled) {
+                    KT_TRY(s.isd.reserve(rows));
+                    invSqrtDiag<<<dim3(static_cast<uint32_t>((rows + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                                  ctx->stream>>>(s.diag.p, s.isd.p, static_cast<uint32_t>(rows));
+                    KT_HIP(hipGetLastError());
                     const uint64_t threads = 4ull * rows;
                     scaledToFloat<<<dim3(static_cast<uint32_t>((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                                    ctx->stream>>>(s.rowPtr.p, s.col.p, s.val.p, s.diag.p, s.val32.p,
+                                    ctx->stream>>>(s.rowPtr.p, s.col.p, s.val.p, s.isd.p, s.val32.p,
                                                    static_cast<uint32_t>(rows));
                     KT_HIP(hipGetLastError());
                 } else if (nnz[ax]) {

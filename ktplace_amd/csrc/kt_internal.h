@@ -132,7 +132,8 @@ struct Sys {
     DevBuf<double> val, diag, rhs;
     DevBuf<float> val32, diag32;
     bool have32 = false;
-    bool scaled32 = false;  // val32 holds val_ij / (sqrt(d_i) sqrt(d_j)) rather than val_ij
+    bool scaled32 = false;  // val32 holds val_ij (s_i s_j), s = 1 / sqrt(d), rather than val_ij
+    DevBuf<double> isd;     // s = 1 / sqrt(d) per row (the scaled FP32 system)
     DevBuf<uint16_t> colLo;  // 3-byte columns: low 16 bits
     DevBuf<uint8_t> colHi;   // 3-byte columns: high 8 bits
     bool have24 = false;
@@ -243,12 +244,13 @@ struct kt_ctx {
     kt::DevBuf<double> posX, posY, saveX, saveY, anchorX, anchorY;
     bool haveAnchors = false;
     kt::DevBuf<double2> vxy;        // vertex-indexed positions (px, py), one 16-byte record per vertex
-    kt::DevBuf<uint32_t> ext;       // [N][16]: per-net record of the B2B build (pins, weight, lo/hi per axis)
+    kt::DevBuf<uint32_t> ext;       // [N][32]: per-net record of the B2B build (pins, weight, lo/hi per axis)
     kt::DevBuf<double> pinAt[2];    // per pin and axis: (vertex position + offset, rhs term of an edge to it)
     kt::DevBuf<uint32_t> pinVar;    // per pin: (its vertex's variable or kNoVar, movable pins before it in its net), uploaded
     kt::DevBuf<uint64_t> b2bEdgeOff, b2bEntOff;  // B2B: per reference, first edge / CSR slot (x | y << 32)
     kt::DevBuf<double> b2bEdges[2];              // B2B: (w, rhs term) per edge in row-walk order
-    kt::DevBuf<uint32_t> b2bLong;                // B2B: first reference of each edge block, per axis
+    kt::DevBuf<uint32_t> b2bLong;                // B2B: first reference and first row of each edge block, per axis
+    kt::DevBuf<uint32_t> b2bRowEdge;             // B2B: first edge of each row's run, per axis
     kt::DevBuf<uint32_t> scanTmp;   // block sums of the u32 scans
     kt::DevBuf<uint64_t> scanTmp64; // block sums of the u64 scans
     kt::DevBuf<uint8_t> scanStorage; // temporary storage of the single-pass device scans
