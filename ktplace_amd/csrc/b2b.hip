@@ -97,12 +97,22 @@ netExtremes(const uint32_t *__restrict__ netStart, const uint32_t *__restrict__ 
     if (p > 0) {
         const PinRec first = pin[b];
         pLoX = pHiX = pLoY = pHiY = first;
-        for (uint32_t q = 1; q < p; ++q) {
-            const PinRec c = pin[b + q];
-            if (c.posX < pLoX.posX) { lox = q; pLoX = c; }
-            if (c.posX > pHiX.posX) { hix = q; pHiX = c; }
-            if (c.posY < pLoY.posY) { loy = q; pLoY = c; }
-            if (c.posY > pHiY.posY) { hiy = q; pHiY = c; }
+        // the pins' loads go out four at a time (a 200-pin net is one
+        // thread's loop), the comparisons run in pin order
+        constexpr uint32_t kBatch = 4;
+        for (uint32_t q0 = 1; q0 < p; q0 += kBatch) {
+            PinRec c[kBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u) c[u] = pin[b + (q0 + u < p ? q0 + u : 0u)];
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u) {
+                const uint32_t q = q0 + u;
+                if (q >= p) break;
+                if (c[u].posX < pLoX.posX) { lox = q; pLoX = c[u]; }
+                if (c[u].posX > pHiX.posX) { hix = q; pHiX = c[u]; }
+                if (c[u].posY < pLoY.posY) { loy = q; pLoY = c[u]; }
+                if (c[u].posY > pHiY.posY) { hiy = q; pHiY = c[u]; }
+            }
         }
     }
     uint4 *dst = reinterpret_cast<uint4 *>(rec + j);

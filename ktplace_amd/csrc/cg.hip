@@ -1135,6 +1135,45 @@ scaledToFloat(const uint32_t *__restrict__ rowPtr, const uint32_t *__restrict__ 
     }
 }
 
+// The same scaled copy over the entry blocks of the SpMV (one block of 256
+// threads per run of consecutive entries): column, value and output accesses
+// are coalesced, and each entry finds its row by a binary search of the
+// block's row pointers, kept in LDS with the rows' scale factors (in memory
+// when the block has more rows than the table holds).
+constexpr uint32_t kScaleRows = 2048;
+
+__global__ void __launch_bounds__(256)
+scaledToFloatFlat(const uint4 *__restrict__ info, const uint32_t *__restrict__ rowPtr,
+                  const uint32_t *__restrict__ col, const double *__restrict__ val, const double *__restrict__ isd,
+                  float *__restrict__ out) {
+    __shared__ uint32_t rp[kScaleRows + 1];
+    __shared__ double rs[kScaleRows];
+    const uint4 b = info[blockIdx.x];
+    const uint32_t R0 = b.x, R1 = b.y, E0 = b.z, E1 = b.w;
+    const uint32_t nr = R1 - R0;
+    const bool table = nr <= kScaleRows;  // block-uniform
+    if (table) {
+        for (uint32_t r = threadIdx.x; r <= nr; r += 256) {
+            rp[r] = rowPtr[R0 + r];
+            if (r < nr) rs[r] = isd[R0 + r];
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = E0 + threadIdx.x; k < E1; k += 256) {
+        const uint32_t c = col[k];
+        const double v = val[k], sj = isd[c];
+        // the row of entry k: the last r in [R0, R1) with rowPtr[r] <= k
+        uint32_t lo = 0, hi = nr;
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if ((table ? rp[mid] : rowPtr[R0 + mid]) <= k) lo = mid;
+            else hi = mid;
+        }
+        const double si = table ? rs[lo] : isd[R0 + lo];
+        out[k] = static_cast<float>(v * (si * sj));
+    }
+}
+
 // Plain product out = A in (SparseSystem::multiply order: diag first, then
 // the row's entries in order), one thread per row.
 __global__ void __launch_bounds__(256)
@@ -1516,6 +1555,7 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
                  "solve: the SpMV window (tile + 2 halo entries) exceeds the LDS budget");
     }
     uint32_t maxBlocks = 1;
+    bool scaleFlat[2] = {false, false};
     for (int ax = 0; ax < 2; ++ax) {
         if (fp32) {
             Sys &s = ctx->sys[ax];
@@ -1528,6 +1568,10 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
                     invSqrtDiag<<<dim3(static_cast<uint32_t>((rows + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                                   ctx->stream>>>(s.diag.p, s.isd.p, static_cast<uint32_t>(rows));
                     KT_HIP(hipGetLastError());
+                    // with entry blocks, over the blocks once they are cut (below)
+                    scaleFlat[ax] = a.layout == 1;
+                }
+                if (nnz[ax] && scaled && !scaleFlat[ax]) {
                     const uint64_t threads = 4ull * rows;
                     scaledToFloat<<<dim3(static_cast<uint32_t>((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                                     ctx->stream>>>(s.rowPtr.p, s.col.p, s.val.p, s.isd.p, s.val32.p,
@@ -1572,6 +1616,11 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
                                                                           sb, E, info);
             KT_HIP(hipGetLastError());
             a.sys[ax].flatInfo = info;
+            if (scaleFlat[ax]) {
+                scaledToFloatFlat<<<dim3(sb), dim3(256), 0, ctx->stream>>>(info, sy.rowPtr.p, sy.col.p, sy.val.p,
+                                                                        sy.isd.p, sy.val32.p);
+                KT_HIP(hipGetLastError());
+            }
             if (a.col24 && a.wide) {
                 if (!sy.havePk) {
                     // one spare quad: a block's loads may reach the quad after nnz
