@@ -121,7 +121,8 @@ __global__ void __launch_bounds__(64) warmStatsFinish(const double *__restrict__
     sc->mdy = t[5] * inv;
 }
 
-constexpr int kG = 8;               // lanes per CSR row in the row passes
+constexpr int kG = 4;               // lanes per CSR row in the row passes
+constexpr int kPer = 4;             // entries per lane loaded together
 constexpr int kRowsPerBlock = kB / kG;
 
 // One pass over the rows of one axis (blockIdx.y), kG lanes per row:
@@ -140,11 +141,32 @@ __global__ void __launch_bounds__(kB) warmRows(AxisSys sx, AxisSys sy, const dou
     const int lane = threadIdx.x % kG;
     const uint32_t row = blockIdx.x * kRowsPerBlock + threadIdx.x / kG;
     double t = 0.0, off = 0.0;
+    // the row's own data is loaded first, so it arrives with the products
+    const uint32_t rq = row < n ? row : 0u;  // n > 0
+    const double dRow = s.diag[rq], vRow = v[rq], bRow = kCurvature ? r[rq] : s.rhs[rq];
     if (row < n) {
-        for (uint32_t k = s.rowPtr[row] + lane; k < s.rowPtr[row + 1]; k += kG) {
-            const double a = s.val[k];
-            t += a * v[s.col[k]];
-            off += fabs(a);
+        // a lane's entries k, k + kG, ..., k + (kPer - 1) kG are loaded
+        // together (a row of up to kPer kG entries in one round), and summed in
+        // that order
+        const uint32_t k1 = s.rowPtr[row + 1];
+        for (uint32_t k = s.rowPtr[row] + lane; k < k1; k += kPer * kG) {
+            double a[kPer], g[kPer];
+            uint32_t c[kPer];
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                const uint32_t kk = k + u * kG < k1 ? k + u * kG : k;  // unconditional loads
+                a[u] = s.val[kk];
+                c[u] = s.col[kk];
+            }
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) g[u] = v[c[u]];
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                if (k + u * kG < k1) {
+                    t += a[u] * g[u];
+                    off += fabs(a[u]);
+                }
+            }
         }
     }
 #pragma unroll
@@ -154,8 +176,8 @@ __global__ void __launch_bounds__(kB) warmRows(AxisSys sx, AxisSys sy, const dou
     }
     if constexpr (!kCurvature) {
         if (row < n && lane == 0) {
-            const double d = s.diag[row];
-            r[row] = s.rhs[row] - (d * v[row] + t);
+            const double d = dRow;
+            r[row] = bRow - (d * vRow + t);
             const float share = d > 0.0 ? static_cast<float>(fmin(fmax((d - off) / d, 0.0), 1.0)) : 0.f;
             (axis ? fsy : fsx)[row] = share;
         }
@@ -163,9 +185,9 @@ __global__ void __launch_bounds__(kB) warmRows(AxisSys sx, AxisSys sy, const dou
         __shared__ double sh[2][kB];
         double num = 0.0, den = 0.0;
         if (row < n && lane == 0) {
-            const double di = v[row];
-            num = di * r[row];
-            den = di * (s.diag[row] * di + t);
+            const double di = vRow;
+            num = di * bRow;
+            den = di * (dRow * di + t);
         }
         sh[0][threadIdx.x] = num;
         sh[1][threadIdx.x] = den;
