@@ -1431,7 +1431,21 @@ SysView viewOf(kt_ctx *ctx, int axis, uint32_t tile) {
 // hipGraph of GraphCache::kPairs pairs.
 // only: the one live axis (0, 1) when the other has stopped, else -1; each
 // case has its own graph.
-int runPairs(kt_ctx *ctx, const CgArgs &a, int G, bool fp32, uint32_t pairs, bool timed, int only) {
+int runPairs(kt_ctx *ctx, const CgArgs &args, int G, bool fp32, uint32_t pairs, bool timed, int only) {
+    // With one axis live and entry-block SpMV (whose blocks do not depend on
+    // the tile), the vector kernels run half tiles: twice the blocks, so the
+    // one axis still spreads over every CU.  cgUpdate and cgPUpdate of one
+    // iteration use the same tile, and the partials buffer holds the blocks.
+    CgArgs a = args;
+    if (only >= 0 && a.layout == 1 && a.tile >= 2 * kBlock) {
+        const uint32_t t = a.tile / 2;
+        bool fits = true;
+        for (int ax = 0; ax < 2; ++ax) fits = fits && (a.sys[ax].rows + t - 1) / t <= a.stride;
+        if (fits) {
+            a.tile = t;
+            for (int ax = 0; ax < 2; ++ax) a.sys[ax].blocks = (a.sys[ax].rows + t - 1) / t;
+        }
+    }
     const auto direct = [&](uint32_t count) -> int {
         for (uint32_t k = 0; k < count; ++k)
             for (int parity = 0; parity < 2; ++parity) {
@@ -1771,6 +1785,26 @@ int solveMixed(kt_ctx *ctx, const kt_solve_opts &o, kt_solve_result &res) {
         const AxisCtl *c = ctx->ctlHost;
         if (c[0].needUpdate || c[1].needUpdate) {
             KT_TRY(launchRefresh(ctx, a, G));
+            // the first iterations after the refresh go out behind it, without
+            // a poll in between: the kernels read the refreshed state on the
+            // device, and an axis the refresh stopped steps as a no-op.  The
+            // axes that may step: live ones and those being refreshed.
+            bool may[2];
+            uint32_t cap = 2;
+            for (int ax = 0; ax < 2; ++ax) {
+                may[ax] = c[ax].s[0].live || c[ax].needUpdate;
+                if (may[ax] && c[ax].maxIter > c[ax].s[0].steps) cap = std::max(cap, c[ax].maxIter - c[ax].s[0].steps);
+            }
+            // not when every axis being refreshed reached tol on its recurrence
+            // residual and no other axis is live: the solve most likely ends
+            bool goesOn = false;
+            for (int ax = 0; ax < 2; ++ax)
+                goesOn = goesOn || c[ax].s[0].live || (c[ax].needUpdate && c[ax].s[0].resid > c[ax].tol);
+            if (goesOn) {
+                const int only = may[0] && may[1] ? -1 : (may[0] ? 0 : 1);
+                const uint32_t chunk = (std::min<uint32_t>(8, cap) + 1) & ~1u;  // as the first chunk of a solve
+                KT_TRY(runPairs(ctx, a, G, true, chunk / 2, timed, only));
+            }
             first = true;
             continue;
         }
