@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(kBlock) cgStart(CgArgs a) {
             if (valid[q] && lane == 0) {
                 const double ri = s.rhs[i] - (s.diag[i] * x[i] + off[q]);
                 // scaled system: r^ = D^-1/2 r, and z = r^ (unit diagonal)
-                const T rt = static_cast<T>(a.unitDiag ? ri / sqrt(s.diag[i]) : ri);
+                const T rt = static_cast<T>(a.unitDiag ? ri * s.isd[i] : ri);
                 const T zt = a.unitDiag ? rt : rt / dT[i];
                 r[i] = rt;
                 p[i] = zt;
@@ -536,7 +536,7 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
             if constexpr (MODE >= 3) {
                 using W = std::conditional_t<MODE == 3, float, double>;
                 const bool scaled = MODE == 3 && a.unitDiag;  // r^ = D^-1/2 r, z = r^
-                const W rt = static_cast<W>(scaled ? ri / sqrt(s.diag[r]) : ri);
+                const W rt = static_cast<W>(scaled ? ri * s.isd[r] : ri);
                 const W zt = scaled ? rt : rt / Mat<W>::diag(s)[r];
                 static_cast<W *>(a.r[axis])[r] = rt;
                 static_cast<W *>(a.p[axis])[r] = zt;
@@ -548,7 +548,7 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
             }
             acc[0] += ri * ri;
             if constexpr (MODE == 1) {
-                const float rf = static_cast<float>(a.unitDiag ? ri / sqrt(s.diag[r]) : ri);
+                const float rf = static_cast<float>(a.unitDiag ? ri * s.isd[r] : ri);
                 static_cast<float *>(a.r[axis])[r] = rf;
                 acc[1] += static_cast<double>(rf) *
                           static_cast<double>(a.unitDiag ? rf : rf / s.diag32[r]);
@@ -943,7 +943,7 @@ __global__ void __launch_bounds__(kBlock) cgFlush(CgArgs a) {
     const uint32_t begin = tile * a.tile, end = min(begin + a.tile, s.rows);
     for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
         const double ei = static_cast<double>(e[i]);
-        x[i] += a.unitDiag ? ei / sqrt(s.diag[i]) : ei;
+        x[i] += a.unitDiag ? ei * s.isd[i] : ei;
         e[i] = 0.0f;
     }
 }
@@ -978,7 +978,7 @@ __global__ void __launch_bounds__(kBlock) cgRefresh(CgArgs a) {
             const uint32_t i = row[q];
             if (valid[q] && lane == 0) {
                 const double ri = s.rhs[i] - (s.diag[i] * x[i] + off[q]);
-                const float rf = static_cast<float>(a.unitDiag ? ri / sqrt(s.diag[i]) : ri);
+                const float rf = static_cast<float>(a.unitDiag ? ri * s.isd[i] : ri);
                 r[i] = rf;
                 acc[0] += ri * ri;
                 acc[1] += static_cast<double>(rf) * static_cast<double>(a.unitDiag ? rf : rf / s.diag32[i]);
@@ -1423,6 +1423,7 @@ SysView viewOf(kt_ctx *ctx, int axis, uint32_t tile) {
     v.colLo = nullptr;
     v.colHi = nullptr;
     v.colPk = nullptr;
+    v.isd = nullptr;
     v.sblocks = v.blocks;
     return v;
 }
@@ -1577,20 +1578,24 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
             if (!s.have32 || s.scaled32 != scaled) {
                 KT_TRY(s.val32.reserve(nnz[ax] + kQuadPad));
                 KT_TRY(s.diag32.reserve(std::max<uint64_t>(rows, 1)));
-                if (nnz[ax] && scaled) {
+                if (scaled && rows) {
+                    // 1 / sqrt(d): the scaled copy below, and the start, refresh
+                    // and flush kernels' conversions between the coordinates
                     KT_TRY(s.isd.reserve(rows));
                     invSqrtDiag<<<dim3(static_cast<uint32_t>((rows + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                                   ctx->stream>>>(s.diag.p, s.isd.p, static_cast<uint32_t>(rows));
                     KT_HIP(hipGetLastError());
-                    // with entry blocks, over the blocks once they are cut (below)
-                    scaleFlat[ax] = a.layout == 1;
                 }
-                if (nnz[ax] && scaled && !scaleFlat[ax]) {
-                    const uint64_t threads = 4ull * rows;
-                    scaledToFloat<<<dim3(static_cast<uint32_t>((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                                    ctx->stream>>>(s.rowPtr.p, s.col.p, s.val.p, s.isd.p, s.val32.p,
-                                                   static_cast<uint32_t>(rows));
-                    KT_HIP(hipGetLastError());
+                if (nnz[ax] && scaled) {
+                    if (a.layout == 1) {
+                        scaleFlat[ax] = true;  // over the entry blocks once they are cut (below)
+                    } else {
+                        const uint64_t threads = 4ull * rows;
+                        scaledToFloat<<<dim3(static_cast<uint32_t>((threads + kBlock - 1) / kBlock)), dim3(kBlock),
+                                        0, ctx->stream>>>(s.rowPtr.p, s.col.p, s.val.p, s.isd.p, s.val32.p,
+                                                          static_cast<uint32_t>(rows));
+                        KT_HIP(hipGetLastError());
+                    }
                 } else if (nnz[ax]) {
                     toFloat<<<dim3(1024), dim3(kBlock), 0, ctx->stream>>>(s.val.p, s.val32.p, nnz[ax]);
                     KT_HIP(hipGetLastError());
@@ -1620,6 +1625,7 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
             a.e[ax] = nullptr;
         }
         a.sys[ax] = viewOf(ctx, ax, tile);
+        if (a.unitDiag) a.sys[ax].isd = ctx->sys[ax].isd.p;
         if (a.layout == 1 && rows > 0) {
             Sys &sy = ctx->sys[ax];
             const uint32_t E = a.flatEntries;

@@ -122,6 +122,7 @@ struct SysView {
     // entry-block SpMV with wide loads and 3-byte columns: the columns of
     // entries 4q .. 4q + 3 packed little-endian into words 3q .. 3q + 2
     const uint32_t *colPk;
+    const double *isd;  // mixed precision, scaled: 1 / sqrt(d) per row (null otherwise)
 };
 
 struct Sys {
