@@ -82,6 +82,24 @@ struct alignas(128) NetRec {
 };
 static_assert(sizeof(NetRec) == 128, "one line per net");
 
+__device__ __forceinline__ void writeNetRec(NetRec *r, uint32_t b, uint32_t p, uint32_t mov, double weight,
+                                            const uint2 *__restrict__ varMov, uint32_t lox, uint32_t hix,
+                                            uint32_t loy, uint32_t hiy, const PinRec &pLoX, const PinRec &pHiX,
+                                            const PinRec &pLoY, const PinRec &pHiY) {
+    uint4 *dst = reinterpret_cast<uint4 *>(r);
+    dst[0] = make_uint4(b, p, mov, 0u);
+    dst[1] = make_uint4(lox, hix, loy, hiy);
+    dst[2] = p > 0 ? make_uint4(varMov[b + lox].x, varMov[b + hix].x, varMov[b + loy].x, varMov[b + hiy].x)
+                   : make_uint4(kNoVar, kNoVar, kNoVar, kNoVar);
+    // used only where lo != hi, i.e. p >= 2
+    reinterpret_cast<double2 *>(dst)[3] = make_double2(weight / static_cast<double>(p - 1), 0.0);
+    double2 *ext = reinterpret_cast<double2 *>(r->ext[0]);
+    ext[0] = make_double2(pLoX.posX, pLoX.termX);
+    ext[1] = make_double2(pHiX.posX, pHiX.termX);
+    ext[2] = make_double2(pLoY.posY, pLoY.termY);
+    ext[3] = make_double2(pHiY.posY, pHiY.termY);
+}
+
 // One thread per net; the running best value is the reference's pinAt(lo),
 // recomputed there from the same operands, so the comparisons are identical.
 __global__ void __launch_bounds__(kBlock)
@@ -92,6 +110,7 @@ netExtremes(const uint32_t *__restrict__ netStart, const uint32_t *__restrict__ 
     if (j >= N) return;
     const uint32_t b = netStart[j];
     const uint32_t p = netStart[j + 1] - b;
+    if (p > kWideNetPins) return;  // netExtremesWide
     uint32_t lox = 0, hix = 0, loy = 0, hiy = 0;
     PinRec pLoX{}, pHiX{}, pLoY{}, pHiY{};
     if (p > 0) {
@@ -115,18 +134,53 @@ netExtremes(const uint32_t *__restrict__ netStart, const uint32_t *__restrict__ 
             }
         }
     }
-    uint4 *dst = reinterpret_cast<uint4 *>(rec + j);
-    dst[0] = make_uint4(b, p, netMov[j], 0u);
-    dst[1] = make_uint4(lox, hix, loy, hiy);
-    dst[2] = p > 0 ? make_uint4(varMov[b + lox].x, varMov[b + hix].x, varMov[b + loy].x, varMov[b + hiy].x)
-                   : make_uint4(kNoVar, kNoVar, kNoVar, kNoVar);
-    // used only where lo != hi, i.e. p >= 2
-    reinterpret_cast<double2 *>(dst)[3] = make_double2(weight[j] / static_cast<double>(p - 1), 0.0);
-    double2 *ext = reinterpret_cast<double2 *>(rec[j].ext[0]);
-    ext[0] = make_double2(pLoX.posX, pLoX.termX);
-    ext[1] = make_double2(pHiX.posX, pHiX.termX);
-    ext[2] = make_double2(pLoY.posY, pLoY.termY);
-    ext[3] = make_double2(pHiY.posY, pHiY.termY);
+    writeNetRec(rec + j, b, p, netMov[j], weight[j], varMov, lox, hix, loy, hiy, pLoX, pHiX, pLoY, pHiY);
+}
+
+// The nets of more than kWideNetPins pins, one wave each: every lane keeps the
+// first argmin / argmax of its pins (lane, lane + 64, ...), then the wave
+// reduces them; the smaller value wins, equal values (neither is less) go to
+// the smaller index, which is the sequential scan's first argmin / argmax.
+struct Ext {
+    double v;
+    uint32_t i;
+};
+__device__ __forceinline__ Ext extLess(Ext a, Ext b) { return b.v < a.v || (!(a.v < b.v) && b.i < a.i) ? b : a; }
+__device__ __forceinline__ Ext extMore(Ext a, Ext b) { return b.v > a.v || (!(a.v > b.v) && b.i < a.i) ? b : a; }
+__device__ __forceinline__ Ext waveExt(Ext e, bool less) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const Ext o{__shfl_xor(e.v, off, 64), static_cast<uint32_t>(__shfl_xor(static_cast<int>(e.i), off, 64))};
+        e = less ? extLess(e, o) : extMore(e, o);
+    }
+    return e;
+}
+
+__global__ void __launch_bounds__(256)
+netExtremesWide(const uint32_t *__restrict__ wide, uint32_t count, const uint32_t *__restrict__ netStart,
+                const uint32_t *__restrict__ netMov, const double *__restrict__ weight,
+                const PinRec *__restrict__ pin, const uint2 *__restrict__ varMov, NetRec *__restrict__ rec) {
+    const uint32_t lane = threadIdx.x & 63u, w = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (w >= count) return;  // the whole wave
+    const uint32_t j = wide[w];
+    const uint32_t b = netStart[j], p = netStart[j + 1] - b;  // p > kWideNetPins
+    constexpr double kInf = __builtin_huge_val();
+    Ext lx{kInf, ~0u}, hx{-kInf, ~0u}, ly{kInf, ~0u}, hy{-kInf, ~0u};
+    for (uint32_t q = lane; q < p; q += 64) {
+        const PinRec c = pin[b + q];
+        // strict comparisons in pin order within the lane: its first extreme
+        if (lx.i == ~0u || c.posX < lx.v) lx = Ext{c.posX, q};
+        if (hx.i == ~0u || c.posX > hx.v) hx = Ext{c.posX, q};
+        if (ly.i == ~0u || c.posY < ly.v) ly = Ext{c.posY, q};
+        if (hy.i == ~0u || c.posY > hy.v) hy = Ext{c.posY, q};
+    }
+    lx = waveExt(lx, true);
+    hx = waveExt(hx, false);
+    ly = waveExt(ly, true);
+    hy = waveExt(hy, false);
+    if (lane != 0) return;
+    writeNetRec(rec + j, b, p, netMov[j], weight[j], varMov, lx.i, hx.i, ly.i, hy.i, pin[b + lx.i], pin[b + hx.i],
+                pin[b + ly.i], pin[b + hy.i]);
 }
 
 // The assembly is organised around the pinsOfCell references (e: row i's
@@ -580,6 +634,11 @@ int buildB2B(kt_ctx *ctx, double alpha, double floorX, double floorY) {
         netExtremes<<<dim3((ctx->N + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream>>>(
             ctx->netStart.p, ctx->netMov.p, ctx->weight.p, pin, varMov, rec, ctx->N);
         KT_HIP(hipGetLastError());
+        if (ctx->numWideNets) {
+            netExtremesWide<<<dim3((ctx->numWideNets + 3) / 4), dim3(256), 0, ctx->stream>>>(
+                ctx->wideNets.p, ctx->numWideNets, ctx->netStart.p, ctx->netMov.p, ctx->weight.p, pin, varMov, rec);
+            KT_HIP(hipGetLastError());
+        }
     }
     RefArgs a{};
     a.pinStart = ctx->pinStart.p;

@@ -236,6 +236,12 @@ int uploadNetlist(kt_ctx *ctx, const kt_netlist &nl) {
     KT_TRY(upload(ctx, ctx->refNet, nl.cell_pin_net, Pm));
     KT_TRY(upload(ctx, ctx->refPin, nl.cell_pin_idx, Pm));
     KT_TRY(upload(ctx, ctx->netMov, netMov.data(), N));
+    // the nets whose extremes a wave finds (the B2B build's netExtremesWide)
+    std::vector<uint32_t> wideNets;
+    for (uint32_t j = 0; j < N; ++j)
+        if (nl.net_start[j + 1] - nl.net_start[j] > kWideNetPins) wideNets.push_back(j);
+    KT_TRY(upload(ctx, ctx->wideNets, wideNets.data(), wideNets.size()));
+    ctx->numWideNets = static_cast<uint32_t>(wideNets.size());
     KT_TRY(upload(ctx, ctx->pinVar, varMov.data(), 2 * P));
     KT_TRY(upload(ctx, ctx->floating, floating.data(), n));
     if (nl.width) KT_TRY(upload(ctx, ctx->width, nl.width, n));
@@ -411,6 +417,7 @@ int kt_ctx_destroy(kt_ctx *ctx) {
         for (hipEvent_t ev : t.pending) (void)hipEventDestroy(ev);
     ctx->netStart.release(); ctx->pinCell.release(); ctx->varOfVertex.release(); ctx->movVertex.release();
     ctx->pinStart.release(); ctx->refNet.release(); ctx->refPin.release(); ctx->netMov.release();
+    ctx->wideNets.release();
     ctx->floating.release(); ctx->hpwlScratch.release(); ctx->loopDev.release();
     ctx->b2bEdgeOff.release(); ctx->b2bEntOff.release(); ctx->b2bLong.release(); ctx->b2bRowEdge.release();
     for (int a = 0; a < 2; ++a) ctx->b2bEdges[a].release();
@@ -455,7 +462,7 @@ int kt_ctx_device_bytes(kt_ctx *ctx, uint64_t *bytes) {
     KT_CHECK(bytes, KT_EINVAL, "null out");
     uint64_t b = 0;
     b += ctx->netStart.bytes() + ctx->pinCell.bytes() + ctx->varOfVertex.bytes() + ctx->movVertex.bytes();
-    b += ctx->pinStart.bytes() + ctx->refNet.bytes() + ctx->refPin.bytes() + ctx->netMov.bytes();
+    b += ctx->pinStart.bytes() + ctx->refNet.bytes() + ctx->refPin.bytes() + ctx->netMov.bytes() + ctx->wideNets.bytes();
     b += ctx->floating.bytes() + ctx->hpwlScratch.bytes() + ctx->loopDev.bytes();
     b += ctx->b2bEdgeOff.bytes() + ctx->b2bEntOff.bytes() + ctx->b2bLong.bytes() + ctx->b2bRowEdge.bytes() + ctx->b2bEdges[0].bytes() +
          ctx->b2bEdges[1].bytes();

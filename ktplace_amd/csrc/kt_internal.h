@@ -70,6 +70,9 @@ struct DevBuf {
 };
 
 constexpr uint32_t kNoVar = KT_NO_VAR;
+// Nets with more pins than this get a wave of their own in the B2B build's
+// per-net extremes (the others one thread each).
+constexpr uint32_t kWideNetPins = 32;
 // Spare entries after a system's column and value arrays: the entry-block
 // SpMV's wide loads read whole aligned groups of four entries.
 constexpr uint64_t kQuadPad = 4;
@@ -233,6 +236,8 @@ struct kt_ctx {
     uint32_t n = 0, V = 0, N = 0;
     uint64_t P = 0, Pm = 0, nnzBound = 0;
     kt::DevBuf<uint32_t> netStart, pinCell, varOfVertex, movVertex, pinStart, refNet, refPin, netMov;
+    kt::DevBuf<uint32_t> wideNets;  // nets of more than kWideNetPins pins (one wave each in netExtremes)
+    uint32_t numWideNets = 0;
     kt::DevBuf<double> offX, offY, weight, fixedX, fixedY, width, height, area;
     // per movable cell: 1 if no net path joins it to a fixed vertex (its
     // component's translation is in the null space of the systems without
