@@ -272,6 +272,7 @@ __global__ void __launch_bounds__(kBlock) cgStartFinish(CgArgs a) {
         }
         c.trueResid = st.resid;
         c.refResid = st.resid;
+        c.refRz = tot[1];
         c.needUpdate = 0;
     }
 }
@@ -784,7 +785,7 @@ __global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a, int parity) {
             const uint32_t j = i < end ? i : begin;  // unconditional loads; a tile has a row
             rv[k] = r[j];
             av[k] = ap[j];
-            dv[k] = d[j];
+            dv[k] = unit ? T(1) : d[j];
         }
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -806,8 +807,7 @@ __global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a, int parity) {
         const T rn = ri - alphaT * ai;
         r[i] = rn;
         const double rd = static_cast<double>(rn);
-        if (unit) {  // r^ = D^-1/2 r: r.r = sum d r^2, r.z = r^.r^
-            acc[0] += static_cast<double>(di) * (rd * rd);
+        if (unit) {  // r^ = D^-1/2 r, r.z = r^.r^ (the residual is judged from it, cgPUpdate)
             acc[1] += rd * rd;
         } else {
             acc[0] += rd * rd;
@@ -821,7 +821,7 @@ __global__ void __launch_bounds__(kBlock) cgUpdate(CgArgs a, int parity) {
             if (i < end) step(i, rv[k], av[k], dv[k]);
         }
     } else {
-        for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) step(i, r[i], ap[i], d[i]);
+        for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) step(i, r[i], ap[i], unit ? T(1) : d[i]);
     }
     blockSum<2>(acc);
     writePartial<2>(a.partB, a.stride, axis, tile, acc);
@@ -885,7 +885,13 @@ __global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
     __builtin_amdgcn_sched_barrier(0);
     double tot[2];
     share.finish(tot);
-    const double resid = sqrt(tot[0]) / c.bNorm;
+    // on the scaled FP32 system the norm of r = D^1/2 r^ would need d in the
+    // residual update; the iterations judge the unscaled residual from r^.r^
+    // instead, scaled from the last refresh's FP64 residual: the refresh point
+    // is where r^.r^ fell by the square of the refresh factor.  The axis still
+    // stops only on an FP64 residual.
+    const double resid = unit ? (c.refRz > 0.0 ? c.refResid * sqrt(tot[1] / c.refRz) : 0.0)
+                              : sqrt(tot[0]) / c.bNorm;
     const double rhoNext = tot[1];
     const bool ok = rhoNext > 0.0;
     // the iterate steps whatever beta turns out to be (the reference updates x
@@ -1003,6 +1009,7 @@ __global__ void __launch_bounds__(kBlock) cgRefreshFinish(CgArgs a) {
         st.resid = resid;
         c.trueResid = resid;
         c.refResid = resid;
+        c.refRz = tot[1];
         ++c.refreshes;
         if (st.active && tot[1] > 0.0) st.rho = tot[1];
         st.live = (st.active && resid > c.tol && st.steps < c.maxIter) ? 1 : 0;

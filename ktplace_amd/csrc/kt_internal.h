@@ -95,6 +95,8 @@ struct AxisCtl {
     double tol;
     double trueResid;     // ||b - A x|| / ||b|| from the last FP64 pass
     double refResid;      // residual at the last mixed-precision refresh
+    double refRz;         // r.z just after that refresh (or the start): on the scaled FP32
+                          // system the iterations judge resid = refResid sqrt(r.z / refRz)
     double refreshFactor; // refresh once the FP32 residual falls by this factor
     uint32_t maxIter;
     int32_t brk[2];       // p.Ap <= 0 in the iteration of that parity
