@@ -292,6 +292,15 @@ int kt_set_spmv_wide_loads(kt_ctx *ctx, int on);
  * the stopping test and the result are those of the unscaled system.  0: the
  * FP32 iterations divide by the diagonal as the FP64 solve does. */
 int kt_set_mixed_scaling(kt_ctx *ctx, int on);
+/* Mixed precision (KT_PREC_MIXED) on the scaled system only, no counterpart in
+ * the reference: 1 (default) runs each FP32 iteration as two kernels where the
+ * solve runs the entry-block shape with wide loads and 3-byte columns: the
+ * SpMV also sums r.Ap and Ap.Ap, so the new residual's norm r'.r' = r.r -
+ * 2 alpha r.Ap + alpha^2 Ap.Ap is known without a second grid-wide reduction
+ * and one kernel updates r, the correction and p.  0: three kernels (the
+ * residual update and its r.r, then the direction update).  Same Jacobi-PCG
+ * iterates up to the rounding of r'.r'. */
+int kt_set_mixed_fused_update(kt_ctx *ctx, int on);
 
 /* ---- per-kernel timings (HIP events on the context's stream) -------------- */
 #define KT_KERNEL_SPMV 0      /* FP64 CG SpMV + p.Ap */

@@ -955,6 +955,13 @@ int kt_set_mixed_scaling(kt_ctx *ctx, int on) {
     return KT_OK;
 }
 
+int kt_set_mixed_fused_update(kt_ctx *ctx, int on) {
+    KT_GUARD(ctx);
+    KT_CHECK(on == 0 || on == 1, KT_EINVAL, "kt_set_mixed_fused_update: on must be 0 or 1");
+    ctx->tune.fusedUpdate = on;
+    return KT_OK;
+}
+
 int kt_set_spmv_layout(kt_ctx *ctx, int layout, uint32_t block_entries, int unroll, int row_lanes,
                        int block_threads) {
     KT_GUARD(ctx);

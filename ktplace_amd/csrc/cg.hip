@@ -499,8 +499,10 @@ uint32_t flatGrid(const CgArgs &a, int only = -1) {
 // entries.  Products and their order are unchanged.
 template <typename T, uint32_t E, int U, int G2, int NT, bool C24 = false, int MODE = 0, bool W4 = false>
 __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
-    static_assert(MODE == 0 || sizeof(T) == 8, "the residual modes run in FP64");
-    constexpr int K = MODE == 0 ? 1 : (MODE <= 2 ? 2 : 3);
+    static_assert(MODE == 0 || MODE == 6 || sizeof(T) == 8, "the residual modes run in FP64");
+    static_assert(MODE != 6 || sizeof(T) == 4, "the fused iteration runs in FP32");
+    // MODE 6: Ap like MODE 0, with the partials p.Ap, r.Ap, Ap.Ap and r.r (cgFUpdate)
+    constexpr int K = MODE == 0 ? 1 : (MODE <= 2 ? 2 : (MODE == 6 ? 4 : 3));
     constexpr uint32_t kFlatEntries = E, kFlatSlack = flatSlack(E);
     // W4 keeps product k of the quads at slot k (the block's entry j at
     // j + lead), so each quad is two aligned 16-byte LDS stores
@@ -508,7 +510,7 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
     int axis;
     uint32_t b;
     if (!decodeFlat(a, axis, b, parity)) return;
-    if constexpr (MODE == 0) {
+    if constexpr (MODE == 0 || MODE == 6) {
         if (!a.ctl[axis].s[parity].live) return;
     } else if constexpr (MODE == 1) {
         if (!a.ctl[axis].needUpdate) return;
@@ -516,21 +518,31 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
         if (a.ctl[axis].frozen) return;
     }
     const SysView s = a.sys[axis];
-    const T *__restrict__ p = MODE == 0 ? static_cast<const T *>(a.p[axis]) : reinterpret_cast<const T *>(a.x64[axis]);
+    const T *__restrict__ p = MODE == 0 || MODE == 6 ? static_cast<const T *>(a.p[axis])
+                                                     : reinterpret_cast<const T *>(a.x64[axis]);
+    const T *__restrict__ rres = static_cast<const T *>(a.r[axis]);  // MODE 6: the residual, for r.Ap
     T *__restrict__ ap = static_cast<T *>(a.ap[axis]);
     const T *__restrict__ val = Mat<T>::val(s);
     const T *__restrict__ diag = Mat<T>::diag(s);
     // the scaled FP32 system (mixed precision): unit diagonal, not read
-    const bool unit = MODE == 0 && sizeof(T) == 4 && a.unitDiag;
+    const bool unit = (MODE == 0 || MODE == 6) && sizeof(T) == 4 && a.unitDiag;
     const uint4 info = s.flatInfo[b];
     const uint32_t R0 = info.x, R1 = info.y, E0 = info.z, count = info.w - info.z;
     double acc[K] = {};
-    // row r's product (A v)_r = sum, with pd = v_r
-    const auto finish = [&](uint32_t r, double pd, double sum) {
+    // row r's product (A v)_r = sum, with pd = v_r (and rr = r_r for MODE 6)
+    const auto finish = [&](uint32_t r, double pd, double sum, double rr) {
         if constexpr (MODE == 0) {
             const T v = static_cast<T>(sum);
             ap[r] = v;
             acc[0] += pd * static_cast<double>(v);
+        } else if constexpr (MODE == 6) {
+            const T v = static_cast<T>(sum);
+            ap[r] = v;
+            const double vd = static_cast<double>(v);
+            acc[0] += pd * vd;
+            acc[1] += rr * vd;
+            acc[2] += vd * vd;
+            acc[3] += rr * rr;
         } else {
             const double rhs = s.rhs[r];
             const double ri = rhs - sum;
@@ -575,7 +587,8 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
             for (int q = 0; q < R; ++q) {
                 if (valid[q] && lane == 0) {
                     const double pd = static_cast<double>(p[row[q]]);
-                    finish(row[q], pd, (unit ? pd : static_cast<double>(diag[row[q]]) * pd) + off[q]);
+                    finish(row[q], pd, (unit ? pd : static_cast<double>(diag[row[q]]) * pd) + off[q],
+                           MODE == 6 ? static_cast<double>(rres[row[q]]) : 0.0);
                 }
             }
         }
@@ -595,13 +608,14 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
         constexpr uint32_t kWaveGroups = 64 / G2;
         const uint32_t lane64 = threadIdx.x & 63u, wave = threadIdx.x >> 6;
         uint32_t pre0, pre1;
-        T preP, preD;
+        T preP, preD, preR = T(0);
         const auto loadRows = [&](uint32_t r) {
             const uint32_t rq = r < R1 ? r : 0u;
             pre0 = s.rowPtr[rq];
             pre1 = s.rowPtr[rq + 1];
             preP = p[rq];
             preD = unit ? T(1) : diag[rq];
+            if constexpr (MODE == 6) preR = rres[rq];
         };
         const auto chunkRow = [&](uint32_t chunk) {
             return R0 + chunk * (kGroups * G2) + kGroups * (lane64 / kWaveGroups) + kWaveGroups * wave +
@@ -680,7 +694,7 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
         __syncthreads();
         // the rows r0 + grp of one pass, with their row data
         const uint32_t slot0 = W4 ? (E0 & ~3u) : E0;  // LDS slot of entry e: e - slot0
-        const auto pass = [&](uint32_t r0, uint32_t q0, uint32_t q1, T pr, T dr) {
+        const auto pass = [&](uint32_t r0, uint32_t q0, uint32_t q1, T pr, T dr, T rr) {
             const uint32_t r = r0 + grp;
             const bool valid = r < R1;
             const uint32_t e0 = q0 - slot0, e1 = valid ? q1 - slot0 : e0;
@@ -690,7 +704,7 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
                 // (SparseSystem::multiply, simpl_lss.h:49-57)
                 double sum = static_cast<double>(dr) * pd;
                 for (uint32_t k = e0; k < e1; ++k) sum += prod[k];
-                if (valid) finish(r, pd, sum);
+                if (valid) finish(r, pd, sum, static_cast<double>(rr));
             } else {
                 // lane l adds products l, l + G2, ... of the row, then a fixed
                 // butterfly; shorter waits on long rows, another order
@@ -698,7 +712,7 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
                 for (uint32_t k = e0 + lane; k < e1; k += G2) part += prod[k];
 #pragma unroll
                 for (int off = G2 / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, G2);
-                if (valid && lane == 0) finish(r, pd, static_cast<double>(dr) * pd + part);
+                if (valid && lane == 0) finish(r, pd, static_cast<double>(dr) * pd + part, static_cast<double>(rr));
             }
         };
         if constexpr (W4) {
@@ -709,19 +723,20 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
                 for (uint32_t k = 0; k < G2; ++k, r0 += kGroups) {
                     if (r0 >= R1) break;
                     const int src = static_cast<int>(k * kWaveGroups + lane64 / G2);
-                    pass(r0, __shfl(pre0, src), __shfl(pre1, src), __shfl(preP, src), __shfl(preD, src));
+                    pass(r0, __shfl(pre0, src), __shfl(pre1, src), __shfl(preP, src), __shfl(preD, src),
+                         MODE == 6 ? __shfl(preR, src) : T(0));
                 }
             }
         } else {
-            if (R0 < R1) pass(R0, pre0, pre1, preP, preD);
+            if (R0 < R1) pass(R0, pre0, pre1, preP, preD, preR);
             for (uint32_t r0 = R0 + kGroups; r0 < R1; r0 += kGroups) {
                 loadRows(r0 + grp);
-                pass(r0, pre0, pre1, preP, preD);
+                pass(r0, pre0, pre1, preP, preD, preR);
             }
         }
     }
     blockSum<K, NT>(acc);
-    writePartial<K>(MODE == 0 ? a.partA : (MODE <= 2 ? a.partB : a.partS), a.stride, axis, b, acc);
+    writePartial<K>(MODE == 0 || MODE == 6 ? a.partA : (MODE <= 2 ? a.partB : a.partS), a.stride, axis, b, acc);
 }
 
 // Cut c (c = 1 .. blocks - 1) = first row whose entries start at or after
@@ -926,6 +941,112 @@ __global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
         nx.steps = st.steps + 1;
         nx.resid = resid;
         nx.rho = ok ? rhoNext : st.rho;
+        nx.active = ok ? 1 : 0;
+        nx.live = (ok && resid > c.tol && nx.steps < c.maxIter && !refresh) ? 1 : 0;
+        nx.pad = 0;
+        c.s[parity ^ 1] = nx;
+        if (st.steps < a.histCap) a.history[hist] = resid;
+        if (refresh) c.needUpdate = 1;
+    }
+}
+
+// ---------------------------------------------------------------- fused vector update
+// The FP32 iteration on the scaled system in two kernels: cgSpmvFlat MODE 6
+// (Ap and the tile partials of p.Ap, r.Ap, Ap.Ap), then this kernel, which does
+// cgUpdate's and cgPUpdate's work in one pass: alpha = rho / p.Ap; r' = r -
+// alpha Ap; e += alpha p; rho' = r'.r' = rho - 2 alpha r.Ap + alpha^2 Ap.Ap
+// (the new residual's norm from the SpMV's partials, so no second grid-wide
+// reduction is needed before p is updated); beta = rho' / rho; p = r' + beta p.
+// The same Jacobi-PCG iterates as the three-kernel iteration up to the
+// rounding of rho'; the residual is judged from rho' as cgPUpdate does on
+// the scaled system, and the rules (p.Ap <= 0, rho' <= 0, refresh, stop) are
+// cgUpdate's and cgPUpdate's.  Block 0 of each axis publishes state[parity ^ 1].
+template <int PER>
+__global__ void __launch_bounds__(kBlock) cgFUpdate(CgArgs a, int parity) {
+    using T = float;
+    int axis;
+    uint32_t tile;
+    // grid block 0 carries every stopped axis' state forward (a stopped axis
+    // may have no block of its own in this launch)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+#pragma unroll
+        for (int ax = 0; ax < 2; ++ax)
+            if (!a.ctl[ax].s[parity].live) a.ctl[ax].s[parity ^ 1] = a.ctl[ax].s[parity];
+    }
+    if (!decode(a, axis, tile, parity)) return;
+    AxisCtl &c = a.ctl[axis];
+    const CgState st = c.s[parity];
+    if (!st.live) return;
+    const SysView sv = a.sys[axis];
+    const uint32_t begin = tile * a.tile, end = min(begin + a.tile, sv.rows);
+    T *__restrict__ r = static_cast<T *>(a.r[axis]);
+    const T *__restrict__ ap = static_cast<const T *>(a.ap[axis]);
+    T *__restrict__ p = static_cast<T *>(a.p[axis]);
+    T *__restrict__ e = static_cast<T *>(a.e[axis]);
+    PartialShare<4, 8> share;  // several thousand SpMV blocks per axis
+    share.issue(a.partA, sv.sblocks, a.stride, axis);
+    __builtin_amdgcn_sched_barrier(0);
+    constexpr int kPre = PER > 0 ? PER : 1;
+    T rv[kPre], av[kPre], pv[kPre], ev[kPre];
+    if constexpr (PER > 0) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t i = begin + threadIdx.x + k * kBlock;
+            const uint32_t j = i < end ? i : begin;  // unconditional loads; a tile has a row
+            rv[k] = r[j];
+            av[k] = ap[j];
+            pv[k] = p[j];
+            ev[k] = e[j];
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    double tot[4];
+    share.finish(tot);
+    // rho = r.r of the stored residual, summed by the SpMV: each step starts
+    // from the residual's own norm, and only beta's numerator and the judged
+    // residual come from the expansion
+    const double pap = tot[0], rap = tot[1], apap = tot[2], rho = tot[3];
+    const bool lead = tile == 0 && threadIdx.x == 0;
+    const size_t hist = static_cast<size_t>(axis) * a.histCap + st.steps;
+    if (!(pap > 0.0)) {  // p.Ap <= 0: the reference stops the axis (simpl_lss.cc:242-244)
+        if (lead) {
+            CgState nx = st;
+            nx.active = 0;
+            nx.live = 0;
+            nx.steps = st.steps + 1;
+            c.s[parity ^ 1] = nx;
+            if (st.steps < a.histCap) a.history[hist] = st.resid;
+        }
+        return;
+    }
+    const double alpha = rho / pap;
+    const double rhoNext = rho - 2.0 * alpha * rap + alpha * alpha * apap;
+    const bool ok = rhoNext > 0.0;
+    const double resid = c.refRz > 0.0 && ok ? c.refResid * sqrt(rhoNext / c.refRz) : 0.0;
+    const T alphaT = static_cast<T>(alpha);
+    const T beta = static_cast<T>(ok ? rhoNext / rho : 0.0);
+    const auto step = [&](uint32_t i, T ri, T ai, T pi, T ei) {
+        const T rn = ri - alphaT * ai;
+        r[i] = rn;
+        e[i] = ei + alphaT * pi;
+        if (ok) p[i] = rn + beta * pi;
+    };
+    if constexpr (PER > 0) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t i = begin + threadIdx.x + k * kBlock;
+            if (i < end) step(i, rv[k], av[k], pv[k], ev[k]);
+        }
+    } else {
+        for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) step(i, r[i], ap[i], p[i], e[i]);
+    }
+    if (lead) {
+        // the refresh rule and the pause as in cgPUpdate
+        const bool refresh = resid <= c.refreshFactor * c.refResid || resid <= c.tol;
+        CgState nx;
+        nx.steps = st.steps + 1;
+        nx.resid = resid;
+        nx.rho = ok ? rhoNext : rho;
         nx.active = ok ? 1 : 0;
         nx.live = (ok && resid > c.tol && nx.steps < c.maxIter && !refresh) ? 1 : 0;
         nx.pad = 0;
@@ -1295,8 +1416,40 @@ int launchSpmv(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, 
 #undef KT_SPMV_S
 }
 
+// One fused FP32 iteration: cgSpmvFlat MODE 6, then cgFUpdate (the shape
+// prepare() enables it for: 256-thread blocks of 2048 entries, wide loads,
+// 3-byte columns, four row lanes).
+int launchIterationFused(kt_ctx *ctx, const CgArgs &a, int parity, hipStream_t st, bool timed, int only) {
+    const dim3 fg(flatGrid(a, only));
+    const auto spmv = cgSpmvFlat<float, 2048, 4, 4, 256, true, 6, true>;
+    if (timed) {
+        hipEvent_t e0, e1;
+        KT_TRY(timedKernelEvents(ctx, KT_KERNEL_SPMV32, &e0, &e1));
+        hipExtLaunchKernelGGL(spmv, fg, dim3(256), 0, st, e0, e1, 0, a, parity);
+    } else {
+        spmv<<<fg, dim3(256), 0, st>>>(a, parity);
+    }
+    KT_HIP(hipGetLastError());
+    const dim3 grid(gridBlocks(a, only));
+    hipEvent_t t0 = nullptr;
+    if (timed) KT_TRY(timedBegin(ctx, KT_KERNEL_UPDATE, &t0));
+    switch (a.tile) {
+        case 1 * kBlock: cgFUpdate<1><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
+        case 2 * kBlock: cgFUpdate<2><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
+        case 4 * kBlock: cgFUpdate<4><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
+        case 8 * kBlock: cgFUpdate<8><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
+        default: cgFUpdate<0><<<grid, dim3(kBlock), 0, st>>>(a, parity); break;
+    }
+    KT_HIP(hipGetLastError());
+    if (timed) KT_TRY(timedEnd(ctx, KT_KERNEL_UPDATE, t0));
+    return KT_OK;
+}
+
 template <typename T>
 int launchIteration(kt_ctx *ctx, const CgArgs &a, int G, int parity, hipStream_t st, bool timed, int only) {
+    if constexpr (sizeof(T) == 4) {
+        if (a.fused) return launchIterationFused(ctx, a, parity, st, timed, only);
+    }
     const dim3 grid(gridBlocks(a, only));
     hipEvent_t t0 = nullptr;
     const int kSpmv = sizeof(T) == 8 ? KT_KERNEL_SPMV : KT_KERNEL_SPMV32;
@@ -1569,6 +1722,10 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
     // FP64 residuals by the same entry blocks (the shape instantiated for them)
     // (the residual kernels have their own entries per round)
     a.flatResidual = (a.layout == 1 && a.flatThreads == 256 && a.flatEntries == 2048 && rows > 0) ? 1u : 0u;
+    // the fused FP32 iteration: on the scaled system, in the entry-block shape
+    // the benchmark tunes to (the one instance of its SpMV)
+    a.fused = (fp32 && ctx->tune.fusedUpdate && a.unitDiag && a.wide && a.col24 && a.flatResidual &&
+               a.flatUnroll == 4) ? 1u : 0u;
 
     KT_CHECK(!(a.layout && a.halo), KT_EINVAL, "solve: the entry-block SpMV has no LDS window");
     if (a.halo) {
@@ -1680,10 +1837,10 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
     a.x64[0] = ctx->posX.p;
     a.x64[1] = ctx->posY.p;
     a.stride = maxBlocks;
-    KT_TRY(ctx->partials.reserve(2ull * maxBlocks * 6));
+    KT_TRY(ctx->partials.reserve(2ull * maxBlocks * 9));
     a.partA = ctx->partials.p;
-    a.partB = ctx->partials.p + 2ull * maxBlocks * 1;
-    a.partS = ctx->partials.p + 2ull * maxBlocks * 3;
+    a.partB = ctx->partials.p + 2ull * maxBlocks * (a.fused ? 4 : 1);  // fused: partA holds four per block
+    a.partS = ctx->partials.p + 2ull * maxBlocks * (a.fused ? 6 : 3);
     KT_TRY(ctx->ctl.reserve(2));
     a.ctl = ctx->ctl.p;
     const uint64_t histCap = std::min<uint64_t>(std::max<uint64_t>(o.max_iter, 1), 1u << 20);

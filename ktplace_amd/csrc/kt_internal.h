@@ -176,6 +176,8 @@ struct CgArgs {
     uint32_t wide;         // entry blocks: 1 = four consecutive entries per thread by wide loads
     uint32_t unitDiag;     // mixed precision: the FP32 system is D^-1/2 A D^-1/2 (unit diagonal);
                            // r, p, Ap and the correction e live in those coordinates
+    uint32_t fused;        // mixed precision, scaled: two kernels per iteration (the SpMV also
+                           // sums r.Ap and Ap.Ap; cgFUpdate does both vector updates)
 };
 
 // Launch-shape knobs (set by the autotuner or the caller; 0 = default).
@@ -195,6 +197,7 @@ struct Tune {
     int colBytes = 0;         // entry blocks: bytes per column index (0, 4; 3 with 256 threads)
     int wideLoads = 0;        // entry blocks: four consecutive entries per thread by wide loads
     int mixedScaling = 1;     // mixed precision: FP32 solve on the symmetrically scaled system
+    int fusedUpdate = 1;      // mixed precision, scaled: one vector kernel per iteration where the shape allows
 };
 
 // A captured hipGraph of kPairs iteration pairs for one argument set.

@@ -107,6 +107,7 @@ _SIGS = {
     "kt_set_spmv_columns": (I32, [P, I32]),
     "kt_set_spmv_wide_loads": (I32, [P, I32]),
     "kt_set_mixed_scaling": (I32, [P, I32]),
+    "kt_set_mixed_fused_update": (I32, [P, I32]),
     "kt_kernel_times": (I32, [P, I32, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "kt_reset_kernel_times": (I32, [P]),
     "kt_probe_copy": (I32, [P, U64, I32, C.POINTER(C.c_double)]),
@@ -466,6 +467,10 @@ class Context:
     def set_mixed_scaling(self, on: bool):
         """Mixed precision: FP32 iterations on D^-1/2 A D^-1/2 (default) or on A."""
         _check(lib.kt_set_mixed_scaling(self.h, int(on)))
+
+    def set_mixed_fused_update(self, on: bool):
+        """Mixed precision, scaled: two kernels per FP32 iteration (default, where the shape allows) or three."""
+        _check(lib.kt_set_mixed_fused_update(self.h, int(on)))
 
     def kernel_time(self, kernel: int):
         n = C.c_uint64()

@@ -184,6 +184,51 @@ def test_mixed_precision_reaches_true_tolerance(b2b12k, gpu_ctx_factory):
     ctx.close()
 
 
+@pytest.mark.parametrize("tol", [1e-6, 1e-9])
+def test_fused_update_matches_three_kernel_iteration(b2b12k, gpu_ctx_factory, tol):
+    """The two-kernel FP32 iteration (the SpMV also sums r.Ap and Ap.Ap; one
+    kernel updates r, the correction and p with r'.r' = r.r - 2 alpha r.Ap +
+    alpha^2 Ap.Ap) against the three-kernel iteration on the same scaled
+    system: both reach the true FP64 tolerance, agree with the oracle solved
+    to 1e-12 (at 1e-9), step the same number of times give or take 3 %, a
+    profiled solve (one launch per kernel) repeats the graph-replayed one bit
+    for bit, and the fused solve is the one that ran: its profile has no
+    direction update kernel.  A fused step with a wrong expansion of r'.r'
+    (a sign or a factor of alpha) gives a wrong beta and residual, and the
+    solve misses the tolerance or the count."""
+    from ktplace_amd.simpl import (KERNEL_PUPDATE, KERNEL_UPDATE, KT_PREC_MIXED, KT_SPMV_ENTRY_BLOCKS,
+                                   KT_STOP_TRUE)
+    d, o, x0, y0 = b2b12k
+    ctx = gpu_ctx_factory(d)
+    ctx.build_b2b(x0, y0)
+    ref = o.solve(x0, y0, tol=1e-12, max_iter=100000)
+    ctx.set_tuning(lanes=4, tile=1024, rows=2, layout=KT_SPMV_ENTRY_BLOCKS, block_entries=2048, unroll=4,
+                   row_lanes=4, block_threads=256, col_bytes=3, wide=1)
+    runs = {}
+    for on in (True, False):
+        ctx.set_mixed_fused_update(on)
+        for profile in (False, True):
+            ctx.reset_kernel_times()
+            x, y = x0.copy(), y0.copy()
+            r = ctx.solve_both(x, y, tol, 100000, precision=KT_PREC_MIXED, stop_rule=KT_STOP_TRUE,
+                               profile=profile)
+            print("fused" if on else "three kernels", "profiled" if profile else "graphs",
+                  r["steps"], r["refinements"], r["true_resid"], flush=True)
+            assert max(r["true_resid"]) <= tol
+            if tol <= 1e-9:
+                assert rel(x, ref["x"]) < 1e-6 and rel(y, ref["y"]) < 1e-6
+            if profile:
+                updates, directions = ctx.kernel_time(KERNEL_UPDATE)[0], ctx.kernel_time(KERNEL_PUPDATE)[0]
+                assert updates > 0 and (directions == 0 if on else directions > 0)
+                assert np.array_equal(x, runs[on][0]) and np.array_equal(y, runs[on][1])
+            else:
+                runs[on] = (x, y, r)
+    for ax in (0, 1):
+        a, b = runs[True][2]["steps"][ax], runs[False][2]["steps"][ax]
+        assert abs(a - b) <= max(2, 0.03 * b)
+    ctx.close()
+
+
 @pytest.mark.parametrize("layout", ["lane groups", "entry blocks", "entry blocks, wide"])
 def test_mixed_scaled_and_unscaled_agree(b2b12k, gpu_ctx_factory, layout):
     """The FP32 iterations on D^-1/2 A D^-1/2 (the default) and on A: both
@@ -481,11 +526,14 @@ def test_wide_loads_are_bit_identical(b2b12k, gpu_ctx_factory, precision):
     (from each block's first entry rounded down to a multiple of four, with 3-
     or 4-byte columns, 4 or 8 entries per round) give the same solution bit for
     bit as one entry per load, through graph replay and direct launches, after
-    the systems are rebuilt."""
+    the systems are rebuilt.  Mixed precision runs the three-kernel iteration
+    here (the fused one exists only in the wide 3-byte shape; it has its own
+    test)."""
     from ktplace_amd.simpl import KT_PREC_FP64, KT_PREC_MIXED, KT_SPMV_ENTRY_BLOCKS, KT_STOP_TRUE
     prec = KT_PREC_FP64 if precision == "fp64" else KT_PREC_MIXED
     d, o, x0, y0 = b2b12k
     ctx = gpu_ctx_factory(d)
+    ctx.set_mixed_fused_update(False)
     for positions in ((x0, y0), (x0[::-1].copy(), y0)):
         ctx.build_b2b(*positions)
         out = []
