@@ -31,19 +31,20 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per SpMV dispatch on this workload (averaged over the dispatches of a
 # solve), from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by the copy-probe
 # calibration: profiles/cgSpmv_f64_cells1m.md, profiles/cgSpmv_f32_cells1m.md
-MEASURED_TRAFFIC = {"fp64": int((552.0 + 13.3) * 1e6), "mixed": int((292.5 + 7.5) * 1e6)}
+MEASURED_TRAFFIC = {"fp64": int((552.0 + 13.3) * 1e6), "mixed": int((298.0 + 7.5) * 1e6)}
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def spmv_bytes(nnz: int, n: int, precision: str) -> int:
+def spmv_bytes(nnz: int, n: int, precision: str, fused: bool = False) -> int:
     """Algorithmic bytes of one SpMV over one axis: col (4) + val per entry;
-    rowPtr (4) + diag + x + y per row."""
+    rowPtr (4) + diag + x + y per row.  The fused FP32 iteration's SpMV also
+    reads the residual r (4 bytes per row) for its r.Ap and r.r partials."""
     if precision == "fp64":
         return 12 * nnz + 28 * n
-    return 8 * nnz + 16 * n
+    return 8 * nnz + 16 * n + (4 * n if fused else 0)
 
 
 def cpu_model() -> str:
@@ -253,7 +254,9 @@ def main() -> int:
         ms_prof = 1e3 * (time.perf_counter() - t0) / max(args.steps, 1)
         kid = simpl.KERNEL_SPMV if base_path == "fp64" else simpl.KERNEL_SPMV32
         launches, spmv_ms = ctx.kernel_time(kid)
-        total_bytes = sum(r["steps"][a] * spmv_bytes(nnz[a], d.n, base_path)
+        # the fused FP32 iteration launches no separate p update
+        fused = base_path != "fp64" and ctx.kernel_time(simpl.KERNEL_PUPDATE)[0] == 0
+        total_bytes = sum(r["steps"][a] * spmv_bytes(nnz[a], d.n, base_path, fused)
                           for r in presults for a in (0, 1))
         avg_ms = spmv_ms / max(launches, 1)
         warm_n, warm_ms = ctx.kernel_time(simpl.KERNEL_WARM)
@@ -300,7 +303,8 @@ def main() -> int:
                           traffic=(MEASURED_TRAFFIC.get(base_path) if args.workload == "cells1m" else None),
                           traffic_source="rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, cells1m, copy-probe scaled",
                           kernel=(f"{'cgSpmvFlat' if tuned.layout else 'cgSpmv'}"
-                                  f"<{'double' if base_path == 'fp64' else 'float'}> (x and y per launch)"),
+                                  f"<{'double' if base_path == 'fp64' else 'float'}> (x and y per launch"
+                                  f"{'; fused iteration, reads r' if fused else ''})"),
                           launches=launches, avg_launch_us=round(avg_ms * 1e3, 2),
                           bytes_per_launch=int(bytes_per_launch)))
 

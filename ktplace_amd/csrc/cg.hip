@@ -266,6 +266,7 @@ __global__ void __launch_bounds__(kBlock) cgStartFinish(CgArgs a) {
             st.live = 0;
         } else {
             st.rho = tot[1];
+            st.pad = 0;
             st.resid = sqrt(tot[2]) / c.bNorm;
             st.active = 1;
             st.live = (st.resid > c.tol && st.steps < c.maxIter) ? 1 : 0;
@@ -502,7 +503,7 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
     static_assert(MODE == 0 || MODE == 6 || sizeof(T) == 8, "the residual modes run in FP64");
     static_assert(MODE != 6 || sizeof(T) == 4, "the fused iteration runs in FP32");
     // MODE 6: Ap like MODE 0, with the partials p.Ap, r.Ap, Ap.Ap and r.r (cgFUpdate)
-    constexpr int K = MODE == 0 ? 1 : (MODE <= 2 ? 2 : (MODE == 6 ? 4 : 3));
+    constexpr int K = MODE == 0 ? 1 : (MODE <= 2 ? 2 : 3);
     constexpr uint32_t kFlatEntries = E, kFlatSlack = flatSlack(E);
     // W4 keeps product k of the quads at slot k (the block's entry j at
     // j + lead), so each quad is two aligned 16-byte LDS stores
@@ -542,7 +543,6 @@ __global__ void __launch_bounds__(NT) cgSpmvFlat(CgArgs a, int parity) {
             acc[0] += pd * vd;
             acc[1] += rr * vd;
             acc[2] += vd * vd;
-            acc[3] += rr * rr;
         } else {
             const double rhs = s.rhs[r];
             const double ri = rhs - sum;
@@ -957,8 +957,10 @@ __global__ void __launch_bounds__(kBlock) cgPUpdate(CgArgs a, int parity) {
 // alpha Ap; e += alpha p; rho' = r'.r' = rho - 2 alpha r.Ap + alpha^2 Ap.Ap
 // (the new residual's norm from the SpMV's partials, so no second grid-wide
 // reduction is needed before p is updated); beta = rho' / rho; p = r' + beta p.
-// The same Jacobi-PCG iterates as the three-kernel iteration up to the
-// rounding of rho'; the residual is judged from rho' as cgPUpdate does on
+// It also sums r'.r' per tile into partU: the next iteration's rho is that
+// direct sum, so the expansion's rounding never carries from one step to the
+// next.  The same Jacobi-PCG iterates as the three-kernel iteration up to the
+// rounding of beta's numerator; the residual is judged from rho' as cgPUpdate does on
 // the scaled system, and the rules (p.Ap <= 0, rho' <= 0, refresh, stop) are
 // cgUpdate's and cgPUpdate's.  Block 0 of each axis publishes state[parity ^ 1].
 template <int PER>
@@ -983,8 +985,13 @@ __global__ void __launch_bounds__(kBlock) cgFUpdate(CgArgs a, int parity) {
     const T *__restrict__ ap = static_cast<const T *>(a.ap[axis]);
     T *__restrict__ p = static_cast<T *>(a.p[axis]);
     T *__restrict__ e = static_cast<T *>(a.e[axis]);
-    PartialShare<4, 8> share;  // several thousand SpMV blocks per axis
+    PartialShare<3, 8> share;  // several thousand SpMV blocks per axis
     share.issue(a.partA, sv.sblocks, a.stride, axis);
+    // rho = r.r of the stored residual: st.pad > 0 says the last update summed
+    // it over st.pad tiles into partU[parity]; else it is st.rho (the start or
+    // a refresh computed it)
+    PartialShare<1, 1> shareR;
+    shareR.issue(a.partU + static_cast<size_t>(parity) * 2 * a.uStride, st.pad, a.uStride, axis);
     __builtin_amdgcn_sched_barrier(0);
     constexpr int kPre = PER > 0 ? PER : 1;
     T rv[kPre], av[kPre], pv[kPre], ev[kPre];
@@ -1000,12 +1007,12 @@ __global__ void __launch_bounds__(kBlock) cgFUpdate(CgArgs a, int parity) {
         }
     }
     __builtin_amdgcn_sched_barrier(0);
-    double tot[4];
+    double tot[3], rr[1];
     share.finish(tot);
-    // rho = r.r of the stored residual, summed by the SpMV: each step starts
-    // from the residual's own norm, and only beta's numerator and the judged
-    // residual come from the expansion
-    const double pap = tot[0], rap = tot[1], apap = tot[2], rho = tot[3];
+    shareR.finish(rr);
+    // each step starts from the residual's own norm; only beta's numerator and
+    // the judged residual come from the expansion
+    const double pap = tot[0], rap = tot[1], apap = tot[2], rho = st.pad > 0 ? rr[0] : st.rho;
     const bool lead = tile == 0 && threadIdx.x == 0;
     const size_t hist = static_cast<size_t>(axis) * a.histCap + st.steps;
     if (!(pap > 0.0)) {  // p.Ap <= 0: the reference stops the axis (simpl_lss.cc:242-244)
@@ -1025,11 +1032,13 @@ __global__ void __launch_bounds__(kBlock) cgFUpdate(CgArgs a, int parity) {
     const double resid = c.refRz > 0.0 && ok ? c.refResid * sqrt(rhoNext / c.refRz) : 0.0;
     const T alphaT = static_cast<T>(alpha);
     const T beta = static_cast<T>(ok ? rhoNext / rho : 0.0);
+    double rn2[1] = {0.0};
     const auto step = [&](uint32_t i, T ri, T ai, T pi, T ei) {
         const T rn = ri - alphaT * ai;
         r[i] = rn;
         e[i] = ei + alphaT * pi;
         if (ok) p[i] = rn + beta * pi;
+        rn2[0] += static_cast<double>(rn) * static_cast<double>(rn);
     };
     if constexpr (PER > 0) {
 #pragma unroll
@@ -1040,6 +1049,11 @@ __global__ void __launch_bounds__(kBlock) cgFUpdate(CgArgs a, int parity) {
     } else {
         for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) step(i, r[i], ap[i], p[i], e[i]);
     }
+    // this tile's r'.r', for the next iteration's rho
+    const uint32_t tiles = (sv.rows + a.tile - 1) / a.tile;
+    const bool share2 = tiles <= a.uStride;
+    blockSum<1>(rn2);
+    if (share2) writePartial<1>(a.partU + static_cast<size_t>(parity ^ 1) * 2 * a.uStride, a.uStride, axis, tile, rn2);
     if (lead) {
         // the refresh rule and the pause as in cgPUpdate
         const bool refresh = resid <= c.refreshFactor * c.refResid || resid <= c.tol;
@@ -1049,7 +1063,7 @@ __global__ void __launch_bounds__(kBlock) cgFUpdate(CgArgs a, int parity) {
         nx.rho = ok ? rhoNext : rho;
         nx.active = ok ? 1 : 0;
         nx.live = (ok && resid > c.tol && nx.steps < c.maxIter && !refresh) ? 1 : 0;
-        nx.pad = 0;
+        nx.pad = share2 ? static_cast<int32_t>(tiles) : 0;
         c.s[parity ^ 1] = nx;
         if (st.steps < a.histCap) a.history[hist] = resid;
         if (refresh) c.needUpdate = 1;
@@ -1133,6 +1147,7 @@ __global__ void __launch_bounds__(kBlock) cgRefreshFinish(CgArgs a) {
         c.refRz = tot[1];
         ++c.refreshes;
         if (st.active && tot[1] > 0.0) st.rho = tot[1];
+        st.pad = 0;  // rho is this kernel's, not the update partials'
         st.live = (st.active && resid > c.tol && st.steps < c.maxIter) ? 1 : 0;
         c.needUpdate = 0;
     }
@@ -1837,10 +1852,14 @@ int prepare(kt_ctx *ctx, const kt_solve_opts &o, bool fp32, CgArgs &a, int &G) {
     a.x64[0] = ctx->posX.p;
     a.x64[1] = ctx->posY.p;
     a.stride = maxBlocks;
-    KT_TRY(ctx->partials.reserve(2ull * maxBlocks * 9));
+    // fused: partU holds r'.r' per update tile, [parity][axis][2 maxBlocks]
+    // (one-axis iterations may run half tiles)
+    KT_TRY(ctx->partials.reserve(2ull * maxBlocks * 9 + (a.fused ? 8ull * maxBlocks : 0)));
     a.partA = ctx->partials.p;
-    a.partB = ctx->partials.p + 2ull * maxBlocks * (a.fused ? 4 : 1);  // fused: partA holds four per block
+    a.partB = ctx->partials.p + 2ull * maxBlocks * (a.fused ? 3 : 1);  // fused: partA holds three per block
     a.partS = ctx->partials.p + 2ull * maxBlocks * (a.fused ? 6 : 3);
+    a.partU = a.fused ? ctx->partials.p + 2ull * maxBlocks * 9 : nullptr;
+    a.uStride = 2 * maxBlocks;
     KT_TRY(ctx->ctl.reserve(2));
     a.ctl = ctx->ctl.p;
     const uint64_t histCap = std::min<uint64_t>(std::max<uint64_t>(o.max_iter, 1), 1u << 20);

@@ -85,7 +85,7 @@ struct CgState {
     uint32_t steps; // iterations this axis has stepped
     int32_t live;   // steps this iteration: active && resid > tol && steps < maxIter
     int32_t active; // the reference's Axis::active
-    int32_t pad;
+    int32_t pad;    // fused FP32 iteration: tiles of the r.r partials rho is summed from (0: rho)
 };
 
 // Per-axis control block in device memory.
@@ -159,6 +159,8 @@ struct CgArgs {
     double *partA;      // [axis][stride]    p.Ap per tile
     double *partB;      // [axis][stride][2] r.r, r.z per tile
     double *partS;      // [axis][stride][3] start partials
+    double *partU;      // fused iteration: [parity][axis][uStride] r'.r' per update tile
+    uint32_t uStride;
     AxisCtl *ctl;       // [2]
     double *history;    // [axis][histCap] residual after each step
     uint32_t histCap;
@@ -177,7 +179,8 @@ struct CgArgs {
     uint32_t unitDiag;     // mixed precision: the FP32 system is D^-1/2 A D^-1/2 (unit diagonal);
                            // r, p, Ap and the correction e live in those coordinates
     uint32_t fused;        // mixed precision, scaled: two kernels per iteration (the SpMV also
-                           // sums r.Ap and Ap.Ap; cgFUpdate does both vector updates)
+                           // sums r.Ap and Ap.Ap; cgFUpdate does both vector updates and sums
+                           // r'.r' per tile for the next iteration)
 };
 
 // Launch-shape knobs (set by the autotuner or the caller; 0 = default).
