@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per SpMV dispatch on this workload (averaged over the dispatches of a
 # solve), from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by the copy-probe
 # calibration: profiles/cgSpmv_f64_cells1m.md, profiles/cgSpmv_f32_cells1m.md
-MEASURED_TRAFFIC = {"fp64": int((552.0 + 13.3) * 1e6), "mixed": int((298.0 + 7.5) * 1e6)}
+MEASURED_TRAFFIC = {"fp64": int((552.0 + 13.3) * 1e6), "mixed": int((298.2 + 7.6) * 1e6)}
 
 
 def log(msg: str) -> None:

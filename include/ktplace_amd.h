@@ -297,9 +297,10 @@ int kt_set_mixed_scaling(kt_ctx *ctx, int on);
  * solve runs the entry-block shape with wide loads and 3-byte columns: the
  * SpMV also sums r.Ap and Ap.Ap, so the new residual's norm r'.r' = r.r -
  * 2 alpha r.Ap + alpha^2 Ap.Ap is known without a second grid-wide reduction
- * and one kernel updates r, the correction and p.  0: three kernels (the
+ * and one kernel updates r, the correction and p (and sums r'.r' per tile,
+ * from which the next iteration's rho comes).  0: three kernels (the
  * residual update and its r.r, then the direction update).  Same Jacobi-PCG
- * iterates up to the rounding of r'.r'. */
+ * iterates up to the rounding of beta's numerator. */
 int kt_set_mixed_fused_update(kt_ctx *ctx, int on);
 
 /* ---- per-kernel timings (HIP events on the context's stream) -------------- */
